@@ -1,0 +1,166 @@
+/*
+ * psm_abi.h — C ABI of the MI355X subgraph matcher (libpsm.so).
+ *
+ * Every entry point replaces a call surface of the reference matcher
+ * (scorpiontornado/PSM_Benchmark); the comment on each names it. The ABI is
+ * plain C: opaque handles, pointers and sizes, no C++ or torch types.
+ *
+ * Conventions
+ *   - Every function returns int: 0 = PSM_OK, > 0 a PSM status below,
+ *     < 0 a negated hipError_t. Nothing throws or aborts across the ABI.
+ *   - Host inputs are borrowed for the duration of the call. Device objects
+ *     are owned by their handle until the matching *_destroy.
+ *   - One psm_ctx owns one HIP stream; calls on one context are serialised
+ *     by the caller, calls on different contexts may run concurrently.
+ *   - The text of the last error of a context is in psm_last_error().
+ */
+#ifndef PSM_ABI_H
+#define PSM_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PSM_OK 0
+#define PSM_EINVAL 1      /* bad argument (size, method name, query too big) */
+#define PSM_EEMPTY 2      /* a candidate set is empty: the reference's filters return false */
+#define PSM_ENOMEM 3      /* device memory exhausted or an admission budget exceeded */
+#define PSM_ETIMEOUT 4    /* time limit hit (reference: Overtime: 1) */
+#define PSM_EOVERFLOW 5   /* a 64-bit counter would overflow */
+#define PSM_ENOTFOUND 6   /* generator could not produce the requested object */
+#define PSM_ESTATE 7      /* call out of order (e.g. enumerate before build tables) */
+
+/* filter methods: FilterVertices::{LDF,NLF,GQL,CFL}Filter
+ * (matching/FilterVertices.h:9-16) */
+#define PSM_FILTER_LDF 0
+#define PSM_FILTER_NLF 1
+#define PSM_FILTER_GQL 2
+#define PSM_FILTER_CFL 3
+
+/* order methods: GenerateQueryPlan::generate{GQL,CFL}QueryPlan
+ * (matching/GenerateQueryPlan.h:32,49) */
+#define PSM_ORDER_GQL 0
+#define PSM_ORDER_CFL 1
+
+/* the largest query the device path accepts (config.h:9 MAXIMUM_QUERY_GRAPH_SIZE) */
+#define PSM_MAX_QUERY_VERTICES 64
+
+typedef struct psm_ctx psm_ctx;
+typedef struct psm_graph psm_graph;
+typedef struct psm_query psm_query;
+
+/* Enumeration statistics. `count` is min(cap, total) when capped. `calls` is
+ * the reference's "Call Count" (one per assignment, EvaluateQuery.cpp:1009)
+ * and is exact only for uncapped runs. Times are device-side (HIP events). */
+typedef struct psm_enum_stats {
+    uint64_t count;
+    uint64_t calls;
+    uint32_t capped;          /* 1 when the cap was reached */
+    uint32_t tasks;           /* device tasks executed */
+    double kernel_ms;         /* enumeration kernels */
+    double expand_ms;         /* task pre-expansion kernels */
+    uint64_t probe_bytes;     /* algorithmic bytes read by the enumeration */
+} psm_enum_stats;
+
+/* ---- context -------------------------------------------------------- */
+int psm_ctx_create(int device, psm_ctx** out);
+int psm_ctx_destroy(psm_ctx* ctx);
+const char* psm_last_error(const psm_ctx* ctx);
+
+/* ---- data graph: class Graph (graph/graph.h:21-237) ------------------
+ * Uploads a CSR data graph (adjacency sorted ascending per vertex) and builds
+ * on the device side the label reverse index (graph.cpp:15-33) and the
+ * neighbour-label-frequency runs (graph.cpp:36-52). */
+int psm_graph_upload(psm_ctx* ctx, uint32_t num_vertices, const uint32_t* offsets,
+                     const uint32_t* neighbors, const uint32_t* labels, psm_graph** out);
+int psm_graph_destroy(psm_graph* g);
+/* |V|, |E|, labels count (max label + 1), max degree, max label frequency */
+int psm_graph_info(const psm_graph* g, uint64_t* info5);
+
+/* ---- one query run (plain path) ---------------------------------------
+ * A psm_query holds the device state the reference keeps in GLOBALARGS and
+ * UnitArgs (matching/Args.h:30-181): candidates, candidate-edge tables and
+ * the matching plan of one query graph against one data graph. */
+int psm_query_create(psm_ctx* ctx, const psm_graph* g, uint32_t n, const uint32_t* q_offsets,
+                     const uint32_t* q_neighbors, const uint32_t* q_labels, psm_query** out);
+int psm_query_destroy(psm_query* q);
+
+/* FilterVertices::{LDF,NLF,GQL,CFL}Filter + sortCandidates
+ * (FilterVertices.cpp:19-250,1012-1016). Candidate sets come out sorted.
+ * Returns PSM_EEMPTY where the reference's filter returns false. */
+int psm_filter(psm_query* q, int method);
+/* candidates_count (n entries) and a host copy of C(u) (candidate_count[u]
+ * entries): the ui** / ui* outputs of the reference's filters. */
+int psm_query_candidate_counts(const psm_query* q, uint32_t* counts);
+int psm_query_candidates(const psm_query* q, uint32_t u, uint32_t* out);
+
+/* BuildTable::buildTables (BuildTable.cpp:9-151): one table per ordered query
+ * edge (a,b); row j lists ascending positions k in C(b) of the neighbours of
+ * C(a)[j]. */
+int psm_build_tables(psm_query* q);
+/* Host copy of table (a,b): off has candidate_count[a]+1 entries, idx has
+ * *nnz entries (pass idx = NULL to query nnz). PSM_EINVAL if a-b is no edge. */
+int psm_query_table(const psm_query* q, uint32_t a, uint32_t b, uint32_t* off, uint32_t* idx,
+                    uint64_t* nnz);
+/* BuildTable::computeMemoryCostInBytes (BuildTable.cpp:300-326), in bytes. */
+int psm_memory_cost_bytes(const psm_query* q, uint64_t* bytes);
+
+/* GenerateQueryPlan::generateGQLQueryPlan / generateCFLQueryPlan
+ * (GenerateQueryPlan.cpp:9-83,270-471) followed by EvaluateQuery::generateBN
+ * / generateBNWithPivot (EvaluateQuery.cpp:1594-1666). order/pivot receive n
+ * entries each (either may be NULL). */
+int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t* pivot);
+
+/* EvaluateQuery::ParallelExecute with the GQL / EXPLORE engines
+ * (EvaluateQuery.cpp:26-112,980-1126): counts embeddings, at most `cap`
+ * (0 = no cap). */
+int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* stats);
+
+/* Convenience: filter + tables + plan + evaluate in one call, the sequence of
+ * split_C_test (StudyPerformance.cpp:1164-1716). */
+int psm_match(psm_query* q, int filter, int order, uint64_t cap, psm_enum_stats* stats);
+
+/* ---- batched form -------------------------------------------------------
+ * Many queries against one resident data graph, each stage launched once for
+ * the whole batch and enumeration in one persistent kernel. Queries are given
+ * as concatenated CSR arrays: query i has n[i] vertices, its offsets are
+ * q_offsets[off_base[i] .. off_base[i]+n[i]] (relative to its own neighbours,
+ * which start at nbr_base[i]), labels at lab_base[i]. counts/calls receive
+ * one entry per query (calls exact only for queries that did not hit cap).
+ * No counterpart in the reference, which matches one query per process. */
+typedef struct psm_batch_stats {
+    double filter_ms, table_ms, plan_ms, enum_ms, total_ms;
+    uint64_t tasks;
+    uint64_t probe_bytes;
+} psm_batch_stats;
+
+int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t num_queries, const uint32_t* n,
+                    const uint64_t* off_base, const uint64_t* nbr_base, const uint64_t* lab_base,
+                    const uint32_t* q_offsets, const uint32_t* q_neighbors, const uint32_t* q_labels,
+                    int filter, int order, uint64_t cap, uint64_t* counts, uint64_t* calls,
+                    uint32_t* status, psm_batch_stats* stats);
+
+/* ---- seeded generators (host) -------------------------------------------
+ * Data graphs for the benchmark configurations. offsets has V+1 entries,
+ * neighbors 2E. */
+int psm_gen_chung_lu(uint32_t V, uint64_t E, double gamma, uint64_t seed, uint32_t* offsets,
+                     uint32_t* neighbors);
+int psm_gen_rmat(uint32_t scale, uint64_t E, double a, double b, double c, uint64_t seed,
+                 uint32_t* offsets, uint32_t* neighbors);
+/* zipf_s <= 0 draws labels uniformly */
+int psm_gen_labels(uint32_t V, uint32_t L, double zipf_s, uint64_t seed, uint32_t* labels);
+/* A connected n-vertex query drawn from the data graph (random walk tree when
+ * dense = 0, greedy dense induced subgraph when dense = 1). Returns the number
+ * of query edges (pairs written to out_edges) or a negative status. */
+int64_t psm_gen_query(uint32_t V, const uint32_t* offsets, const uint32_t* neighbors, uint32_t n,
+                      int dense, uint64_t seed, uint32_t* out_vertices, uint32_t* out_edges,
+                      uint64_t max_edges);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSM_ABI_H */

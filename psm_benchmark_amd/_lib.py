@@ -1,0 +1,117 @@
+"""ctypes binding to libpsm.so, the C ABI declared in include/psm_abi.h.
+
+The library is built in-tree (psm_benchmark_amd/_build.py). Loading fails
+loudly when it is missing: there is no CPU fallback for the device path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsm.so")
+
+PSM_OK = 0
+PSM_EINVAL = 1
+PSM_EEMPTY = 2
+PSM_ENOMEM = 3
+PSM_ETIMEOUT = 4
+PSM_EOVERFLOW = 5
+PSM_ENOTFOUND = 6
+PSM_ESTATE = 7
+
+FILTERS = {"LDF": 0, "NLF": 1, "GQL": 2, "CFL": 3}
+ORDERS = {"GQL": 0, "CFL": 1}
+
+U32P = ctypes.POINTER(ctypes.c_uint32)
+U64P = ctypes.POINTER(ctypes.c_uint64)
+VP = ctypes.c_void_p
+
+
+class EnumStats(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_uint64),
+        ("calls", ctypes.c_uint64),
+        ("capped", ctypes.c_uint32),
+        ("tasks", ctypes.c_uint32),
+        ("kernel_ms", ctypes.c_double),
+        ("expand_ms", ctypes.c_double),
+        ("probe_bytes", ctypes.c_uint64),
+    ]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("filter_ms", ctypes.c_double),
+        ("table_ms", ctypes.c_double),
+        ("plan_ms", ctypes.c_double),
+        ("enum_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("tasks", ctypes.c_uint64),
+        ("probe_bytes", ctypes.c_uint64),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/psm_abi.h declares
+SIGNATURES = {
+    "psm_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
+    "psm_ctx_destroy": (ctypes.c_int, [VP]),
+    "psm_last_error": (ctypes.c_char_p, [VP]),
+    "psm_graph_upload": (ctypes.c_int, [VP, ctypes.c_uint32, U32P, U32P, U32P, ctypes.POINTER(VP)]),
+    "psm_graph_destroy": (ctypes.c_int, [VP]),
+    "psm_graph_info": (ctypes.c_int, [VP, U64P]),
+    "psm_query_create": (ctypes.c_int, [VP, VP, ctypes.c_uint32, U32P, U32P, U32P, ctypes.POINTER(VP)]),
+    "psm_query_destroy": (ctypes.c_int, [VP]),
+    "psm_filter": (ctypes.c_int, [VP, ctypes.c_int]),
+    "psm_query_candidate_counts": (ctypes.c_int, [VP, U32P]),
+    "psm_query_candidates": (ctypes.c_int, [VP, ctypes.c_uint32, U32P]),
+    "psm_build_tables": (ctypes.c_int, [VP]),
+    "psm_query_table": (ctypes.c_int, [VP, ctypes.c_uint32, ctypes.c_uint32, U32P, U32P, U64P]),
+    "psm_memory_cost_bytes": (ctypes.c_int, [VP, U64P]),
+    "psm_generate_plan": (ctypes.c_int, [VP, ctypes.c_int, U32P, U32P]),
+    "psm_evaluate": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
+    "psm_match": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
+    "psm_batch_match": (ctypes.c_int, [VP, VP, ctypes.c_uint32, U32P, U64P, U64P, U64P, U32P, U32P, U32P,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_uint64, U64P, U64P, U32P,
+                                       ctypes.POINTER(BatchStats)]),
+    "psm_gen_chung_lu": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64,
+                                        U32P, U32P]),
+    "psm_gen_rmat": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_uint64, U32P, U32P]),
+    "psm_gen_labels": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, U32P]),
+    "psm_gen_query": (ctypes.c_int64, [ctypes.c_uint32, U32P, U32P, ctypes.c_uint32, ctypes.c_int,
+                                       ctypes.c_uint64, U32P, U32P, ctypes.c_uint64]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libpsm.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m psm_benchmark_amd._build` "
+                "(the matcher has no CPU fallback)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+class PSMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"psm status {code}: {msg}")
+        self.code = code
+
+
+def u32p(a):
+    return a.ctypes.data_as(U32P)
+
+
+def u64p(a):
+    return a.ctypes.data_as(U64P)

@@ -1,0 +1,932 @@
+// device.hip — HIP kernels of the subgraph matcher, written for gfx950
+// (CDNA4, 64-wide wavefronts). All arithmetic is integer; nothing here is a
+// dense contraction, so the kernels are bound by memory traffic and latency.
+//
+//   compaction   order-preserving filter of label buckets / candidate lists
+//                (LDF, NLF, CFL generation, removal of pruned candidates)
+//   bitmaps      per (query, vertex) candidate bitmaps + per-word ranks
+//   gql_refine   GQL global refinement: one wavefront per candidate tests
+//                for a matching that saturates N_q(u) inside N(v)
+//   tables       BuildTable rows (count pass, scan, fill pass)
+//   cfl          CFL support counting and pruning
+//   expand/dfs   enumeration: task pre-expansion and the persistent
+//                lane-per-task DFS fed by a global atomic work queue
+
+#include "internal.h"
+
+namespace psm {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    return (1ull << lane_id()) - 1ull;
+}
+
+// Largest i in [0, n) with starts[i] <= key (starts ascending, starts[0] == 0).
+template <typename T, typename K>
+__device__ __forceinline__ uint32_t find_item(const T* items, uint32_t n, K key, K T::*field) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (items[mid].*field <= key) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t find_u64(const uint64_t* starts, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (starts[mid] <= key) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// FilterVertices.cpp:786-803: the data vertex's neighbour-label counts
+// dominate the query vertex's, both held as label-sorted runs.
+__device__ bool nlf_dominates(const DevGraph& g, uint32_t v, const uint2* qr, uint32_t qn) {
+    uint32_t i = g.nlf_off[v];
+    const uint32_t ie = g.nlf_off[v + 1];
+    for (uint32_t j = 0; j < qn; ++j) {
+        const uint2 r = qr[j];
+        bool found = false;
+        while (i < ie) {
+            const uint2 d = g.nlf[i];
+            if (d.x < r.x) { ++i; continue; }
+            if (d.x == r.x && d.y >= r.y) { found = true; ++i; }
+            break;
+        }
+        if (!found) return false;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Order-preserving compaction, pass 1: predicate per element, one 64-bit
+// mask per wavefront and a survivor count per block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_compact_flag(
+    DevGraph g, const CItem* __restrict__ items, uint32_t n_items, const QVert* __restrict__ qv,
+    const uint2* __restrict__ qnlf, const uint32_t* __restrict__ src_pool,
+    const uint32_t* __restrict__ cnt_pool, unsigned long long* __restrict__ masks,
+    uint32_t* __restrict__ block_counts) {
+    __shared__ uint32_t s_item;
+    __shared__ uint32_t s_cnt[kBlock / kWave];
+    const uint32_t b = blockIdx.x;
+    if (threadIdx.x == 0) s_item = find_item(items, n_items, b, &CItem::block_start);
+    __syncthreads();
+    const CItem it = items[s_item];
+    const uint32_t e = (b - it.block_start) * kBlock + threadIdx.x;
+    bool keep = false;
+    if (e < it.len) {
+        if (it.mode == kModeValid) {
+            keep = src_pool[it.src_begin + e] != kInvalid;
+        } else {
+            const uint32_t v = g.ri[it.src_begin + e];
+            const QVert q = qv[it.qv];
+            keep = g.off[v + 1] - g.off[v] >= q.degree;
+            if (keep && it.mode == kModeCflGen) keep = cnt_pool[it.cnt_base + v] == it.target;
+            if (keep && it.mode != kModeLDF) keep = nlf_dominates(g, v, qnlf + q.nlf_begin, q.nlf_count);
+        }
+    }
+    const unsigned long long m = __ballot(keep);
+    const uint32_t w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        masks[(size_t)b * (kBlock / kWave) + w] = m;
+        s_cnt[w] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0;
+        for (int i = 0; i < kBlock / kWave; ++i) c += s_cnt[i];
+        block_counts[b] = c;
+    }
+}
+
+// Pass 3: survivors written at their block's exclusive offset.
+__global__ __launch_bounds__(kBlock) void k_compact_scatter(
+    const CItem* __restrict__ items, uint32_t n_items, const uint32_t* __restrict__ src_ri,
+    const uint32_t* __restrict__ src_pool, const unsigned long long* __restrict__ masks,
+    const uint64_t* __restrict__ block_prefix, uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_item;
+    const uint32_t b = blockIdx.x;
+    if (threadIdx.x == 0) s_item = find_item(items, n_items, b, &CItem::block_start);
+    __syncthreads();
+    const CItem it = items[s_item];
+    const uint32_t w = threadIdx.x / kWave;
+    const unsigned long long m = masks[(size_t)b * (kBlock / kWave) + w];
+    if (!((m >> lane_id()) & 1ull)) return;
+    uint64_t pos = block_prefix[b];
+    for (uint32_t i = 0; i < w; ++i) pos += __popcll(masks[(size_t)b * (kBlock / kWave) + i]);
+    pos += __popcll(m & lanemask_lt());
+    const uint32_t e = (b - it.block_start) * kBlock + threadIdx.x;
+    out[pos] = it.mode == kModeValid ? src_pool[it.src_begin + e] : src_ri[it.src_begin + e];
+}
+
+// ---------------------------------------------------------------------------
+// Candidate bitmaps: one bit per data vertex per (query, query vertex) slot.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_set_bits(
+    const uint32_t* __restrict__ pool, const uint64_t* __restrict__ seg_begin,
+    const uint32_t* __restrict__ seg_len, const uint32_t* __restrict__ seg_slot, uint32_t n_seg,
+    uint32_t* __restrict__ bitmaps, uint64_t wps) {
+    // grid.y = segment, grid.x strides its elements
+    const uint32_t s = blockIdx.y;
+    if (s >= n_seg) return;
+    const uint32_t len = seg_len[s];
+    uint32_t* bm = bitmaps + (size_t)seg_slot[s] * wps;
+    const uint32_t* src = pool + seg_begin[s];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < len; i += gridDim.x * kBlock) {
+        const uint32_t v = src[i];
+        if (v != kInvalid) atomicOr(bm + (v >> 5), 1u << (v & 31));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_popc_words(const uint32_t* __restrict__ bm,
+                                                        uint32_t* __restrict__ out, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kBlock)
+        out[i] = __popc(bm[i]);
+}
+
+// ---------------------------------------------------------------------------
+// GQL global refinement (FilterVertices.cpp:104-124, verifyExactTwigIso
+// :704-741). One wavefront per candidate v of query vertex u. Left side:
+// the k query neighbours u'_i of u; right side: N(v); edge (i, w) iff w is a
+// live candidate of u'_i. v survives iff a matching saturates the left side.
+//
+// A left vertex with at least k neighbours can always be matched after the
+// others (at most k-1 right vertices are taken by then), so only the set T
+// of left vertices with fewer than k neighbours needs an explicit matching;
+// their neighbours number < k*|T| <= 31*31. Those are compacted into LDS and
+// matched by Kuhn's augmenting paths, each path search run by the whole
+// wavefront (64 right vertices per step).
+// ---------------------------------------------------------------------------
+constexpr int kRightCap = 1024;
+constexpr int kGqlWavesPerBlock = kBlock / kWave;
+
+struct GqlShared {
+    uint32_t rmask[kGqlWavesPerBlock][kRightCap];
+    uint8_t match_r[kGqlWavesPerBlock][kRightCap];
+    uint8_t seen[kGqlWavesPerBlock][kRightCap];
+    uint32_t st_left[kGqlWavesPerBlock][kMaxGqlDegree + 1];
+    uint32_t st_pos[kGqlWavesPerBlock][kMaxGqlDegree + 1];
+    uint32_t st_via[kGqlWavesPerBlock][kMaxGqlDegree + 1];
+};
+
+__device__ __forceinline__ uint32_t left_mask(const uint32_t* const* lb, uint32_t k, uint32_t w) {
+    uint32_t m = 0;
+    const uint32_t word = w >> 5, bit = w & 31;
+    for (uint32_t i = 0; i < k; ++i) m |= ((lb[i][word] >> bit) & 1u) << i;
+    return m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gql_refine(
+    DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_waves,
+    const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
+    uint32_t* __restrict__ bitmaps, uint64_t wps) {
+    __shared__ GqlShared sh;
+    const uint32_t wl = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t gw = blockIdx.x * kGqlWavesPerBlock + wl;
+    if (gw >= n_waves) return;
+    const uint32_t ii = find_item(items, n_items, gw, &GItem::wave_start);
+    const GItem it = items[ii];
+    const uint32_t j = gw - it.wave_start;
+    if (j >= it.len) return;
+    const uint32_t v = cand_pool[it.cand_begin + j];
+    if (v == kInvalid) return;
+    const uint32_t k = it.k;
+    const uint32_t* lb[kMaxGqlDegree];
+    for (uint32_t i = 0; i < k; ++i) lb[i] = bitmaps + (size_t)left_slots[it.left_begin + i] * wps;
+
+    const uint32_t vb = g.off[v], ve = g.off[v + 1];
+    // pass 1: number of right neighbours of every left vertex (saturating at k)
+    uint32_t cnt[kMaxGqlDegree];
+    for (uint32_t i = 0; i < k; ++i) cnt[i] = 0;
+    for (uint32_t base = vb; base < ve; base += kWave) {
+        const uint32_t p = base + lane;
+        const uint32_t m = p < ve ? left_mask(lb, k, g.adj[p]) : 0u;
+        for (uint32_t i = 0; i < k; ++i) {
+            const uint32_t c = __popcll(__ballot((m >> i) & 1u));
+            cnt[i] = min(cnt[i] + c, k);
+        }
+    }
+    bool ok = true;
+    uint32_t T = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+        if (cnt[i] == 0) ok = false;
+        if (cnt[i] < k) T |= 1u << i;
+    }
+    if (ok && T) {
+        // pass 2: right vertices adjacent to T, compacted into LDS
+        uint32_t nr = 0;
+        for (uint32_t base = vb; base < ve; base += kWave) {
+            const uint32_t p = base + lane;
+            const uint32_t m = p < ve ? (left_mask(lb, k, g.adj[p]) & T) : 0u;
+            const unsigned long long bal = __ballot(m != 0);
+            if (m) {
+                const uint32_t r = nr + __popcll(bal & lanemask_lt());
+                if (r < kRightCap) sh.rmask[wl][r] = m;
+            }
+            nr += __popcll(bal);
+        }
+        nr = min(nr, (uint32_t)kRightCap);  // bounded by 31*31 < kRightCap
+        for (uint32_t r = lane; r < nr; r += kWave) sh.match_r[wl][r] = 0xFF;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // Kuhn: augment from each left vertex of T in turn
+        for (uint32_t i = 0; i < k && ok; ++i) {
+            if (!((T >> i) & 1u)) continue;
+            for (uint32_t r = lane; r < nr; r += kWave) sh.seen[wl][r] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            int sp = 0;
+            if (lane == 0) { sh.st_left[wl][0] = i; sh.st_pos[wl][0] = 0; }
+            bool found = false;
+            while (sp >= 0) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                const uint32_t l = sh.st_left[wl][sp];
+                const uint32_t start = sh.st_pos[wl][sp];
+                uint32_t hit = kInvalid;
+                for (uint32_t c0 = start & ~(uint32_t)(kWave - 1); c0 < nr; c0 += kWave) {
+                    const uint32_t r = c0 + lane;
+                    const bool h = r >= start && r < nr && ((sh.rmask[wl][r] >> l) & 1u) &&
+                                   !sh.seen[wl][r];
+                    const unsigned long long bal = __ballot(h);
+                    if (bal) { hit = c0 + __ffsll((long long)bal) - 1; break; }
+                }
+                if (hit == kInvalid) { --sp; continue; }
+                const uint32_t owner = sh.match_r[wl][hit];
+                if (lane == 0) {
+                    sh.seen[wl][hit] = 1;
+                    sh.st_pos[wl][sp] = hit + 1;
+                    sh.st_via[wl][sp] = hit;
+                }
+                if (owner == 0xFF) { found = true; break; }
+                ++sp;
+                if (lane == 0) { sh.st_left[wl][sp] = owner; sh.st_pos[wl][sp] = 0; }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (!found) { ok = false; break; }
+            if (lane == 0)
+                for (int s = 0; s <= sp; ++s) sh.match_r[wl][sh.st_via[wl][s]] = (uint8_t)sh.st_left[wl][s];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        }
+    }
+    if (!ok && lane == 0) {
+        cand_pool[it.cand_begin + j] = kInvalid;
+        atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BuildTable (BuildTable.cpp:9-151) by definition: row j of table (a, b) lists
+// the ranks in C(b) of the neighbours of C(a)[j] that are in C(b). The rank
+// of w is the count of set bits before it in the (query, b) bitmap.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t bit_rank(const uint32_t* bm, const uint32_t* wp, uint32_t w) {
+    const uint32_t word = w >> 5;
+    return (wp[word] - wp[0]) + __popc(bm[word] & ((1u << (w & 31)) - 1u));
+}
+
+__global__ __launch_bounds__(kBlock) void k_table_count(
+    DevGraph g, const TItem* __restrict__ items, uint32_t n_items,
+    const uint64_t* __restrict__ item_row_start, uint64_t n_rows,
+    const uint32_t* __restrict__ cand_pool, const uint32_t* __restrict__ bitmaps, uint64_t wps,
+    uint64_t* __restrict__ row_deg) {
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n_rows;
+         r += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t ii = find_u64(item_row_start, n_items, r);
+        const TItem it = items[ii];
+        const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
+        const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
+        uint32_t c = 0;
+        for (uint32_t p = g.off[v]; p < g.off[v + 1]; ++p) {
+            const uint32_t w = g.adj[p];
+            c += (bm[w >> 5] >> (w & 31)) & 1u;
+        }
+        row_deg[r] = c;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_table_fill(
+    DevGraph g, const TItem* __restrict__ items, uint32_t n_items,
+    const uint64_t* __restrict__ item_row_start, uint64_t n_rows,
+    const uint32_t* __restrict__ cand_pool, const uint32_t* __restrict__ bitmaps,
+    const uint32_t* __restrict__ word_prefix, uint64_t wps, const uint64_t* __restrict__ row_off,
+    uint32_t* __restrict__ idx) {
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n_rows;
+         r += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t ii = find_u64(item_row_start, n_items, r);
+        const TItem it = items[ii];
+        const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
+        const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
+        const uint32_t* wp = word_prefix + (size_t)it.bm_b * wps;
+        uint64_t o = row_off[r];
+        for (uint32_t p = g.off[v]; p < g.off[v + 1]; ++p) {
+            const uint32_t w = g.adj[p];
+            if ((bm[w >> 5] >> (w & 31)) & 1u) idx[o++] = bit_rank(bm, wp, w);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CFL support counting (FilterVertices.cpp:841-966 restated with set
+// semantics): cnt[w] advances from `expect` to `next` once per pivot for a
+// data vertex w with the query vertex's label and degree that neighbours a
+// live candidate of that pivot.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_cfl_support(
+    DevGraph g, const uint32_t* __restrict__ cand_pool, uint64_t cand_begin, uint32_t len,
+    uint32_t label, uint32_t degree, uint32_t* __restrict__ cnt, uint32_t expect, uint32_t next) {
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < len; j += gridDim.x * kBlock) {
+        const uint32_t v = cand_pool[cand_begin + j];
+        if (v == kInvalid) continue;
+        for (uint32_t p = g.off[v]; p < g.off[v + 1]; ++p) {
+            const uint32_t w = g.adj[p];
+            if (g.lab[w] != label || g.off[w + 1] - g.off[w] < degree) continue;
+            if (expect == 0) {
+                // first pivot: any stale value below `next` becomes `next`
+                atomicMax(cnt + w, next);
+            } else {
+                atomicCAS(cnt + w, expect, next);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cfl_prune(const uint32_t* __restrict__ cnt,
+                                                       uint32_t* __restrict__ cand_pool,
+                                                       uint64_t cand_begin, uint32_t len,
+                                                       uint32_t target) {
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < len; j += gridDim.x * kBlock) {
+        const uint32_t v = cand_pool[cand_begin + j];
+        if (v != kInvalid && cnt[v] != target) cand_pool[cand_begin + j] = kInvalid;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Enumeration. A partial embedding at depth d assigns candidate index I[e]
+// (and data vertex M[e]) to query vertex order[e] for e < d. The valid
+// extensions at depth d are the candidate indices present in every operand
+// row E[b -> order[d]][I[src(b)]] whose data vertex is not already used.
+// They are produced lazily: the shortest operand row is walked, every
+// element is binary-searched in the other rows and checked against the
+// earlier same-label assignments. Call counting follows the GQL / EXPLORE
+// engines (one call per assignment, EvaluateQuery.cpp:1009,1081).
+// ---------------------------------------------------------------------------
+struct LevelBase {
+    uint64_t lo, hi;
+    uint32_t sel;
+};
+
+template <typename State>
+__device__ __forceinline__ LevelBase level_base(const DPlan& P, const DOp* __restrict__ ops,
+                                                const uint64_t* __restrict__ row_off, uint32_t d,
+                                                const State& st, unsigned long long& bytes) {
+    LevelBase b;
+    const uint32_t nops = P.op_count[d];
+    if (nops == 0) {
+        b.lo = 0;
+        b.hi = P.cand_len[d];
+        b.sel = kIdentity;
+        return b;
+    }
+    b.sel = 0;
+    b.lo = 0;
+    b.hi = ~0ull;
+    uint64_t best = ~0ull;
+    for (uint32_t j = 0; j < nops; ++j) {
+        const DOp op = ops[P.op_begin[d] + j];
+        const uint64_t r = op.row_base + st.I(op.src_depth);
+        const uint64_t lo = row_off[r], hi = row_off[r + 1];
+        if (hi - lo < best) { best = hi - lo; b.lo = lo; b.hi = hi; b.sel = j; }
+    }
+    bytes += 16ull * nops;
+    return b;
+}
+
+template <typename State>
+__device__ __forceinline__ bool test_element(const DPlan& P, const DOp* __restrict__ ops,
+                                             const uint64_t* __restrict__ row_off,
+                                             const uint32_t* __restrict__ idx,
+                                             const uint32_t* __restrict__ cand_pool, uint32_t d,
+                                             uint32_t x, uint32_t sel, const State& st,
+                                             uint32_t& v_out, unsigned long long& bytes) {
+    const uint32_t nops = P.op_count[d];
+    for (uint32_t j = 0; j < nops; ++j) {
+        if (j == sel) continue;
+        const DOp op = ops[P.op_begin[d] + j];
+        const uint64_t r = op.row_base + st.I(op.src_depth);
+        const uint64_t hi0 = row_off[r + 1];
+        uint64_t lo = row_off[r], hi = hi0;
+        bytes += 16;
+        // lower_bound of x in idx[lo, hi)
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            bytes += 4;
+            if (idx[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo == hi0 || idx[lo] != x) return false;
+    }
+    const uint32_t v = cand_pool[P.cand_base[d] + x];
+    bytes += 4;
+    unsigned long long mask = P.same_label[d];
+    while (mask) {
+        const uint32_t e = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+        if (st.M(e) == v) return false;
+    }
+    v_out = v;
+    return true;
+}
+
+// Per-lane state of the persistent DFS in global memory, laid out
+// [wave][level][lane] per field so that lanes at one level touch one line.
+struct LaneState {
+    uint32_t* I_;
+    uint32_t* M_;
+    uint32_t* sel_;
+    unsigned long long* cur_;
+    unsigned long long* end_;
+    size_t base;   // (wave * max_n) * 64 + lane
+    __device__ __forceinline__ size_t at(uint32_t lvl) const { return base + (size_t)lvl * kWave; }
+    __device__ __forceinline__ uint32_t I(uint32_t e) const { return I_[at(e)]; }
+    __device__ __forceinline__ uint32_t M(uint32_t e) const { return M_[at(e)]; }
+};
+
+// Task pre-expansion state: the prefix held in the task record.
+struct PrefixState {
+    const uint32_t* I_;
+    const uint32_t* M_;
+    __device__ __forceinline__ uint32_t I(uint32_t e) const { return I_[e]; }
+    __device__ __forceinline__ uint32_t M(uint32_t e) const { return M_[e]; }
+};
+
+// One thread per task: enumerate the valid extensions at the task's depth.
+// Count pass (write = 0): out_count[t] = number of children (or 1 when the
+// task's query is not being expanded); calls += children. Write pass: the
+// children (or the task itself) at out_pos[t].
+__global__ __launch_bounds__(kBlock) void k_expand(
+    const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
+    const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
+    const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ tasks, uint32_t n_tasks,
+    const uint8_t* __restrict__ expand_q, uint32_t* __restrict__ out_count,
+    const uint64_t* __restrict__ out_pos, DTask* __restrict__ out_tasks, QCounters* qc,
+    unsigned long long* __restrict__ qchild, int write) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_tasks) return;
+    const DTask task = tasks[t];
+    if (!expand_q[task.q]) {
+        if (write) {
+            out_tasks[out_pos[t]] = task;
+        } else {
+            out_count[t] = 1;
+            atomicAdd(qchild + task.q, 1ull);
+        }
+        return;
+    }
+    const DPlan& P = plans[task.q];
+    uint32_t I[kPrefixMax + 1], M[kPrefixMax + 1];
+    for (uint32_t e = 0; e < task.depth; ++e) {
+        I[e] = task.prefix[e];
+        M[e] = cand_pool[P.cand_base[e] + I[e]];
+    }
+    PrefixState st{I, M};
+    unsigned long long bytes = 0;
+    const uint32_t d = task.depth;
+    const LevelBase b = level_base(P, ops, row_off, d, st, bytes);
+    uint64_t cur = b.lo + task.lo;
+    const uint64_t end = task.hi == kIdentity ? b.hi : min(b.hi, b.lo + task.hi);
+    uint32_t c = 0;
+    uint64_t pos = write ? out_pos[t] : 0;
+    for (; cur < end; ++cur) {
+        const uint32_t x = b.sel == kIdentity ? (uint32_t)cur : idx[cur];
+        uint32_t v;
+        if (!test_element(P, ops, row_off, idx, cand_pool, d, x, b.sel, st, v, bytes)) continue;
+        if (write) {
+            DTask ch;
+            ch.q = task.q;
+            ch.depth = d + 1;
+            ch.lo = 0;
+            ch.hi = kIdentity;
+            for (uint32_t e = 0; e < kPrefixMax; ++e) ch.prefix[e] = e < d ? task.prefix[e] : 0;
+            ch.prefix[d] = x;
+            out_tasks[pos++] = ch;
+        }
+        ++c;
+    }
+    if (!write) {
+        out_count[t] = c;
+        atomicAdd(qchild + task.q, (unsigned long long)c);
+    } else {
+        // each child is one assignment at this depth; counted once the
+        // expansion is committed
+        atomicAdd(&qc[task.q].calls, (unsigned long long)c);
+        atomicAdd(&qc[task.q].probe_bytes, bytes);
+    }
+}
+
+// Persistent DFS: every lane runs its own depth-first search over one task at
+// a time; idle lanes of a wavefront take the next tasks from the global queue
+// with one atomic per wavefront. A query whose count reaches its cap is
+// marked done and its remaining tasks are dropped (count = min(cap, total)).
+__global__ __launch_bounds__(kBlock) void k_dfs(
+    const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
+    const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
+    const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ tasks, uint32_t n_tasks,
+    uint32_t* __restrict__ head, QCounters* qc, uint32_t max_n, uint32_t* lane_state) {
+    const uint32_t lane = lane_id();
+    const size_t wave = ((size_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const size_t per_field = (size_t)gridDim.x * (kBlock / kWave) * max_n * kWave;
+    LaneState st;
+    st.I_ = lane_state;
+    st.M_ = lane_state + per_field;
+    st.sel_ = lane_state + 2 * per_field;
+    st.cur_ = reinterpret_cast<unsigned long long*>(lane_state + 3 * per_field);
+    st.end_ = st.cur_ + per_field;
+    st.base = wave * max_n * kWave + lane;
+
+    bool active = false, drained = false;
+    uint32_t q = 0, tdepth = 0, d = 0, sel = 0, n = 0;
+    uint64_t cur = 0, end = 0;
+    unsigned long long lcount = 0, lcalls = 0, lbytes = 0;
+    uint32_t steps = 0;
+    const DPlan* P = plans;
+
+    while (true) {
+        if (!drained) {
+            const unsigned long long idle = __ballot(!active);
+            if (idle) {
+                const uint32_t nidle = __popcll(idle);
+                const uint32_t leader = __ffsll((long long)idle) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(head, nidle);
+                base = __shfl(base, leader);
+                if (base + nidle >= n_tasks) drained = true;
+                if (!active) {
+                    const uint32_t t = base + __popcll(idle & lanemask_lt());
+                    if (t < n_tasks) {
+                        const DTask task = tasks[t];
+                        q = task.q;
+                        P = plans + q;
+                        if (!__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
+                            n = P->n;
+                            tdepth = task.depth;
+                            d = tdepth;
+                            for (uint32_t e = 0; e < tdepth; ++e) {
+                                const uint32_t x = task.prefix[e];
+                                st.I_[st.at(e)] = x;
+                                st.M_[st.at(e)] = cand_pool[P->cand_base[e] + x];
+                            }
+                            const LevelBase b = level_base(*P, ops, row_off, d, st, lbytes);
+                            sel = b.sel;
+                            cur = b.lo + task.lo;
+                            end = task.hi == kIdentity ? b.hi : min(b.hi, b.lo + (uint64_t)task.hi);
+                            active = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            if (cur < end) {
+                const uint32_t x = sel == kIdentity ? (uint32_t)cur : idx[cur];
+                lbytes += sel == kIdentity ? 0 : 4;
+                ++cur;
+                uint32_t v;
+                if (test_element(*P, ops, row_off, idx, cand_pool, d, x, sel, st, v, lbytes)) {
+                    ++lcalls;
+                    if (d + 1 == n) {
+                        ++lcount;
+                    } else {
+                        const size_t a = st.at(d);
+                        st.cur_[a] = cur;
+                        st.end_[a] = end;
+                        st.sel_[a] = sel;
+                        st.I_[a] = x;
+                        st.M_[a] = v;
+                        ++d;
+                        const LevelBase b = level_base(*P, ops, row_off, d, st, lbytes);
+                        cur = b.lo;
+                        end = b.hi;
+                        sel = b.sel;
+                    }
+                }
+            } else if (d == tdepth) {
+                active = false;
+            } else {
+                --d;
+                const size_t a = st.at(d);
+                cur = st.cur_[a];
+                end = st.end_[a];
+                sel = st.sel_[a];
+            }
+            ++steps;
+            // a task of a query that reached its cap (here or in another lane) is dropped
+            if (active && (steps & 255) == 0 && __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED))
+                active = false;
+            const bool flush = !active || lcount >= 4096 || (steps & 1023) == 0;
+            if (flush && (lcount | lcalls | lbytes)) {
+                const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
+                atomicAdd(&qc[q].calls, lcalls);
+                atomicAdd(&qc[q].probe_bytes, lbytes);
+                if (lcount && prev + lcount >= P->cap) atomicExch(&qc[q].done, 1u);
+                lcount = lcalls = lbytes = 0;
+            }
+        }
+    }
+    if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);   // bytes of tasks dropped at fetch
+}
+
+
+// ---------------------------------------------------------------------------
+// Exclusive scans (64-bit totals) and per-slot bitmap word ranks.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long shfl_up64(unsigned long long x, uint32_t o) {
+    const uint32_t lo = __shfl_up((uint32_t)x, o), hi = __shfl_up((uint32_t)(x >> 32), o);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Block-wide exclusive scan of one value per thread; `total` gets the sum.
+__device__ __forceinline__ unsigned long long block_scan(unsigned long long x,
+                                                         unsigned long long& total) {
+    __shared__ unsigned long long wsum[kBlock / kWave];
+    const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
+    unsigned long long inc = x;
+    for (uint32_t o = 1; o < kWave; o <<= 1) {
+        const unsigned long long y = shfl_up64(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == kWave - 1) wsum[w] = inc;
+    __syncthreads();
+    unsigned long long base = 0;
+    total = 0;
+    for (uint32_t i = 0; i < kBlock / kWave; ++i) {
+        if (i < w) base += wsum[i];
+        total += wsum[i];
+    }
+    __syncthreads();
+    return base + inc - x;
+}
+
+constexpr uint32_t kScanPerThread = 8;
+constexpr uint32_t kScanChunk = kBlock * kScanPerThread;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const T* __restrict__ in, uint64_t n,
+                                                        unsigned long long* __restrict__ sums) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    unsigned long long x = 0;
+    for (uint32_t i = 0; i < kScanPerThread; ++i) {
+        const uint64_t p = base + (uint64_t)i * kBlock + threadIdx.x;
+        if (p < n) x += in[p];
+    }
+    unsigned long long total;
+    block_scan(x, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the chunk sums in place.
+__global__ __launch_bounds__(kBlock) void k_scan_sums(unsigned long long* __restrict__ sums,
+                                                      uint32_t n_chunks) {
+    unsigned long long run = 0;
+    for (uint32_t b = 0; b < n_chunks; b += kBlock) {
+        const uint32_t i = b + threadIdx.x;
+        const unsigned long long x = i < n_chunks ? sums[i] : 0;
+        unsigned long long total;
+        const unsigned long long ex = block_scan(x, total);
+        if (i < n_chunks) sums[i] = run + ex;
+        run += total;
+    }
+}
+
+// out[i] = sum of in[0..i), for i in [0, n]; out has n+1 entries.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_down(const T* __restrict__ in, uint64_t n,
+                                                      const unsigned long long* __restrict__ sums,
+                                                      uint64_t* __restrict__ out) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanPerThread;
+    unsigned long long v[kScanPerThread];
+    unsigned long long x = 0;
+    for (uint32_t i = 0; i < kScanPerThread; ++i) {
+        const uint64_t p = base + i;
+        v[i] = p < n ? (unsigned long long)in[p] : 0ull;
+        x += v[i];
+    }
+    unsigned long long total;
+    unsigned long long run = sums[blockIdx.x] + block_scan(x, total);
+    for (uint32_t i = 0; i < kScanPerThread; ++i) {
+        const uint64_t p = base + i;
+        if (p < n) out[p] = run;
+        if (p == n - 1) out[n] = run + v[i];
+        run += v[i];
+    }
+}
+
+// Per slot: prefix[slot][w] = set bits of words [0, w) of that slot's bitmap.
+__global__ __launch_bounds__(kBlock) void k_slot_prefix(const uint32_t* __restrict__ bitmaps,
+                                                        uint32_t* __restrict__ prefix, uint64_t wps) {
+    const uint32_t* bm = bitmaps + (size_t)blockIdx.x * wps;
+    uint32_t* out = prefix + (size_t)blockIdx.x * wps;
+    unsigned long long run = 0;
+    for (uint64_t b = 0; b < wps; b += kBlock) {
+        const uint64_t i = b + threadIdx.x;
+        const unsigned long long x = i < wps ? __popc(bm[i]) : 0u;
+        unsigned long long total;
+        const unsigned long long ex = block_scan(x, total);
+        if (i < wps) out[i] = (uint32_t)(run + ex);
+        run += total;
+    }
+}
+
+// out[i] = row_off[end_i] - row_off[start_i] for item i.
+__global__ __launch_bounds__(kBlock) void k_span(const uint64_t* __restrict__ row_off,
+                                                 const uint64_t* __restrict__ starts,
+                                                 const uint64_t* __restrict__ ends, uint32_t n,
+                                                 uint64_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = row_off[ends[i]] - row_off[starts[i]];
+}
+
+}  // namespace
+
+template <typename T>
+static hipError_t scan_impl(const T* in, uint64_t n, uint64_t* out, unsigned long long* tmp,
+                            hipStream_t s) {
+    if (n == 0) return hipMemsetAsync(out, 0, sizeof(uint64_t), s);
+    const uint64_t chunks = (n + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((uint32_t)chunks), dim3(kBlock), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, tmp, (uint32_t)chunks);
+    hipLaunchKernelGGL(k_scan_down<T>, dim3((uint32_t)chunks), dim3(kBlock), 0, s, in, n, tmp, out);
+    return hipGetLastError();
+}
+
+uint64_t scan_tmp_entries(uint64_t n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
+
+hipError_t launch_scan_u32(const uint32_t* in, uint64_t n, uint64_t* out,
+                           unsigned long long* tmp, hipStream_t s) {
+    return scan_impl(in, n, out, tmp, s);
+}
+
+hipError_t launch_scan_u64(const uint64_t* in, uint64_t n, uint64_t* out,
+                           unsigned long long* tmp, hipStream_t s) {
+    return scan_impl(in, n, out, tmp, s);
+}
+
+hipError_t launch_slot_prefix(const uint32_t* bitmaps, uint32_t* prefix, uint32_t n_slots,
+                              uint64_t words_per_slot, hipStream_t s) {
+    if (n_slots == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slot_prefix, dim3(n_slots), dim3(kBlock), 0, s, bitmaps, prefix,
+                       words_per_slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_span(const uint64_t* row_off, const uint64_t* starts, const uint64_t* ends,
+                       uint32_t n, uint64_t* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_span, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, row_off, starts,
+                       ends, n, out);
+    return hipGetLastError();
+}
+
+namespace {
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_compact_flag(const DevGraph& g, const CItem* items, uint32_t n_items,
+                               uint32_t n_blocks, const QVert* qv, const uint2* qnlf,
+                               const uint32_t* src_pool, const uint32_t* cnt_pool,
+                               unsigned long long* masks, uint32_t* block_counts, hipStream_t s) {
+    if (n_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact_flag, dim3(n_blocks), dim3(kBlock), 0, s, g, items, n_items, qv,
+                       qnlf, src_pool, cnt_pool, masks, block_counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_scatter(const CItem* items, uint32_t n_items, uint32_t n_blocks,
+                                  const uint32_t* src_ri, const uint32_t* src_pool,
+                                  const unsigned long long* masks, const uint64_t* block_prefix,
+                                  uint32_t* out, hipStream_t s) {
+    if (n_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact_scatter, dim3(n_blocks), dim3(kBlock), 0, s, items, n_items,
+                       src_ri, src_pool, masks, block_prefix, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_bits(const uint32_t* pool, const uint64_t* seg_begin, const uint32_t* seg_len,
+                           const uint32_t* seg_slot, uint32_t n_seg, uint32_t total_max,
+                           uint32_t* bitmaps, uint64_t words_per_slot, hipStream_t s) {
+    if (n_seg == 0) return hipSuccess;
+    uint32_t gx = (total_max + kBlock - 1) / kBlock;
+    gx = gx < 1 ? 1 : (gx > 64 ? 64 : gx);
+    hipLaunchKernelGGL(k_set_bits, dim3(gx, n_seg), dim3(kBlock), 0, s, pool, seg_begin, seg_len,
+                       seg_slot, n_seg, bitmaps, words_per_slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_words, hipStream_t s) {
+    if (n_words == 0) return hipSuccess;
+    uint64_t blocks = (n_words + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_popc_words, dim3((uint32_t)blocks), dim3(kBlock), 0, s, bitmaps, out, n_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
+                             uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
+                             uint32_t* bitmaps, uint64_t words_per_slot, hipStream_t s) {
+    if (n_waves == 0) return hipSuccess;
+    const uint32_t blocks = (n_waves + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock;
+    hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_waves,
+                       left_slots, cand_pool, bitmaps, words_per_slot);
+    return hipGetLastError();
+}
+
+static uint32_t grid_for(uint64_t n) {
+    uint64_t b = (n + kBlock - 1) / kBlock;
+    if (b > 131072) b = 131072;
+    return b ? (uint32_t)b : 1u;
+}
+
+hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,
+                              const uint64_t* item_row_start, uint64_t n_rows,
+                              const uint32_t* cand_pool, const uint32_t* bitmaps,
+                              uint64_t words_per_slot, uint64_t* row_deg, hipStream_t s) {
+    if (n_rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_table_count, dim3(grid_for(n_rows)), dim3(kBlock), 0, s, g, items, n_items,
+                       item_row_start, n_rows, cand_pool, bitmaps, words_per_slot, row_deg);
+    return hipGetLastError();
+}
+
+hipError_t launch_table_fill(const DevGraph& g, const TItem* items, uint32_t n_items,
+                             const uint64_t* item_row_start, uint64_t n_rows,
+                             const uint32_t* cand_pool, const uint32_t* bitmaps,
+                             const uint32_t* word_prefix, uint64_t words_per_slot,
+                             const uint64_t* row_off, uint32_t* idx, hipStream_t s) {
+    if (n_rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_table_fill, dim3(grid_for(n_rows)), dim3(kBlock), 0, s, g, items, n_items,
+                       item_row_start, n_rows, cand_pool, bitmaps, word_prefix, words_per_slot,
+                       row_off, idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_support(const DevGraph& g, const uint32_t* cand_pool, uint64_t cand_begin,
+                              uint32_t len, uint32_t label, uint32_t degree, uint32_t* cnt,
+                              uint32_t expect, uint32_t next, hipStream_t s) {
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cfl_support, dim3(grid_for(len)), dim3(kBlock), 0, s, g, cand_pool,
+                       cand_begin, len, label, degree, cnt, expect, next);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_prune(const uint32_t* cnt, uint32_t* cand_pool, uint64_t cand_begin,
+                            uint32_t len, uint32_t target, hipStream_t s) {
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cfl_prune, dim3(grid_for(len)), dim3(kBlock), 0, s, cnt, cand_pool,
+                       cand_begin, len, target);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
+                         const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
+                         uint32_t n_tasks, const uint8_t* expand_q, uint32_t* out_count,
+                         const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
+                         unsigned long long* qchild, int write, hipStream_t s) {
+    if (n_tasks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_expand, dim3((n_tasks + kBlock - 1) / kBlock), dim3(kBlock), 0, s, plans,
+                       ops, row_off, idx, cand_pool, tasks, n_tasks, expand_q, out_count, out_pos,
+                       out_tasks, qc, qchild, write);
+    return hipGetLastError();
+}
+
+uint32_t dfs_lane_state_words(uint32_t max_n) {
+    // I, M, sel (1 word each) + cur, end (2 words each) per level per lane
+    return 7u * max_n;
+}
+
+hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
+                      const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
+                      uint32_t n_tasks, uint32_t* head, QCounters* qc, uint32_t max_n,
+                      uint32_t grid_waves, uint32_t* lane_state, hipStream_t s) {
+    if (n_tasks == 0 || grid_waves == 0) return hipSuccess;
+    const uint32_t blocks = grid_waves / (kBlock / kWave);
+    hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(kBlock), 0, s, plans, ops, row_off, idx, cand_pool,
+                       tasks, n_tasks, head, qc, max_n, lane_state);
+    return hipGetLastError();
+}
+
+}  // namespace psm

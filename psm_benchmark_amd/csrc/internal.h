@@ -1,0 +1,167 @@
+// internal.h — structures shared by the device kernels (device.hip) and the
+// host orchestration (psm_abi.cpp). Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace psm {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;   // pruned candidate
+constexpr uint32_t kMaxQ = 64;               // query vertices (config.h:9)
+constexpr uint32_t kMaxGqlDegree = 32;       // query degree limit of the GQL refinement kernel
+constexpr uint32_t kPrefixMax = 8;           // prefix entries a task record carries
+constexpr uint32_t kIdentity = 0xFFFFFFFFu;  // "base list is C(u) itself"
+
+// Device-resident data graph (uint32 CSR + derived indexes).
+struct DevGraph {
+    uint32_t V;
+    uint32_t L;            // labels count
+    uint64_t nnz;          // 2E
+    const uint32_t* off;   // V+1
+    const uint32_t* adj;   // nnz
+    const uint32_t* lab;   // V
+    const uint32_t* ri_off;  // L+1: vertices grouped by label
+    const uint32_t* ri;      // V, ascending within a label
+    const uint32_t* nlf_off; // V+1: neighbour-label runs per vertex
+    const uint2* nlf;        // (label, count), sorted by label
+};
+
+// One query vertex as the candidate filters see it.
+struct QVert {
+    uint32_t label;
+    uint32_t degree;
+    uint32_t nlf_begin;    // into the batch's query NLF run array
+    uint32_t nlf_count;
+};
+
+// A segment of elements to filter and compact in order. Blocks of 256
+// elements are numbered consecutively over all items.
+enum CompactMode : uint32_t {
+    kModeLDF = 0,      // src = reverse index bucket, keep deg >= d(u)
+    kModeNLF = 1,      // + NLF dominance
+    kModeValid = 2,    // src = candidate pool, keep != kInvalid
+    kModeCflGen = 3,   // src = reverse index bucket, keep cnt == target && NLF
+};
+
+struct CItem {
+    uint64_t src_begin;    // first element in the source array
+    uint32_t len;
+    uint32_t block_start;  // first block of this item
+    uint32_t mode;
+    uint32_t qv;           // QVert index
+    uint64_t cnt_base;     // kModeCflGen: per-query support counter array offset
+    uint32_t target;
+    uint32_t pad;
+};
+
+// GQL refinement work item: the candidates of one query vertex u.
+struct GItem {
+    uint64_t cand_begin;
+    uint32_t len;
+    uint32_t wave_start;   // first wavefront of this item
+    uint32_t bm_self;      // bitmap slot of (query, u)
+    uint32_t k;            // query degree of u
+    uint32_t left_begin;   // into the left-slot array: bitmap slots of N_q(u)
+    uint32_t pad;
+};
+
+// BuildTable work item: ordered query edge (a, b) of one query.
+struct TItem {
+    uint64_t cand_a;       // candidate pool offset of C(a)
+    uint32_t len_a;
+    uint32_t bm_b;         // bitmap slot of (query, b)
+    uint64_t row_base;     // first row of this table in the global row arrays
+};
+
+// Enumeration plan of one query (depth d holds query vertex order[d]).
+struct DPlan {
+    uint32_t n;
+    uint32_t pad;
+    uint64_t cap;
+    uint64_t cand_base[kMaxQ];
+    uint32_t cand_len[kMaxQ];
+    uint32_t op_begin[kMaxQ];
+    uint32_t op_count[kMaxQ];
+    uint32_t pad2[kMaxQ];
+    uint64_t same_label[kMaxQ];   // earlier depths whose vertex has the same label
+};
+
+// One intersection operand of a depth: the table row E[b -> u_d] selected by
+// the candidate index assigned at depth src_depth.
+struct DOp {
+    uint32_t src_depth;
+    uint32_t pad;
+    uint64_t row_base;
+};
+
+// A unit of enumeration work: query q with the prefix I[0..depth-1] fixed;
+// the lane enumerates positions [lo, hi) of the depth's base list.
+struct DTask {
+    uint32_t q;
+    uint32_t depth;
+    uint32_t lo;
+    uint32_t hi;          // kIdentity: to the end of the base list
+    uint32_t prefix[kPrefixMax];
+};
+
+// Per-query enumeration counters.
+struct QCounters {
+    unsigned long long count;
+    unsigned long long calls;
+    unsigned long long probe_bytes;
+    uint32_t done;        // cap reached
+    uint32_t pad;
+};
+
+// ---- kernel launchers (device.hip) --------------------------------------
+hipError_t launch_compact_flag(const DevGraph& g, const CItem* items, uint32_t n_items,
+                               uint32_t n_blocks, const QVert* qv, const uint2* qnlf,
+                               const uint32_t* src_pool, const uint32_t* cnt_pool,
+                               unsigned long long* masks, uint32_t* block_counts, hipStream_t s);
+hipError_t launch_compact_scatter(const CItem* items, uint32_t n_items, uint32_t n_blocks,
+                                  const uint32_t* src_ri, const uint32_t* src_pool,
+                                  const unsigned long long* masks, const uint64_t* block_prefix,
+                                  uint32_t* out, hipStream_t s);
+hipError_t launch_set_bits(const uint32_t* pool, const uint64_t* seg_begin, const uint32_t* seg_len,
+                           const uint32_t* seg_slot, uint32_t n_seg, uint32_t total_max,
+                           uint32_t* bitmaps, uint64_t words_per_slot, hipStream_t s);
+hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_words, hipStream_t s);
+hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
+                             uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
+                             uint32_t* bitmaps, uint64_t words_per_slot, hipStream_t s);
+hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,
+                              const uint64_t* item_row_start, uint64_t n_rows,
+                              const uint32_t* cand_pool, const uint32_t* bitmaps,
+                              uint64_t words_per_slot, uint64_t* row_deg, hipStream_t s);
+hipError_t launch_table_fill(const DevGraph& g, const TItem* items, uint32_t n_items,
+                             const uint64_t* item_row_start, uint64_t n_rows,
+                             const uint32_t* cand_pool, const uint32_t* bitmaps,
+                             const uint32_t* word_prefix, uint64_t words_per_slot,
+                             const uint64_t* row_off, uint32_t* idx, hipStream_t s);
+hipError_t launch_cfl_support(const DevGraph& g, const uint32_t* cand_pool, uint64_t cand_begin,
+                              uint32_t len, uint32_t label, uint32_t degree, uint32_t* cnt,
+                              uint32_t expect, uint32_t next, hipStream_t s);
+hipError_t launch_cfl_prune(const uint32_t* cnt, uint32_t* cand_pool, uint64_t cand_begin,
+                            uint32_t len, uint32_t target, hipStream_t s);
+hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
+                         const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
+                         uint32_t n_tasks, const uint8_t* expand_q, uint32_t* out_count,
+                         const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
+                         unsigned long long* qchild, int write, hipStream_t s);
+hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
+                      const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
+                      uint32_t n_tasks, uint32_t* head, QCounters* qc, uint32_t max_n,
+                      uint32_t grid_waves, uint32_t* lane_state, hipStream_t s);
+uint32_t dfs_lane_state_words(uint32_t max_n);
+uint64_t scan_tmp_entries(uint64_t n);
+hipError_t launch_scan_u32(const uint32_t* in, uint64_t n, uint64_t* out, unsigned long long* tmp,
+                           hipStream_t s);
+hipError_t launch_scan_u64(const uint64_t* in, uint64_t n, uint64_t* out, unsigned long long* tmp,
+                           hipStream_t s);
+hipError_t launch_slot_prefix(const uint32_t* bitmaps, uint32_t* prefix, uint32_t n_slots,
+                              uint64_t words_per_slot, hipStream_t s);
+hipError_t launch_span(const uint64_t* row_off, const uint64_t* starts, const uint64_t* ends,
+                       uint32_t n, uint64_t* out, hipStream_t s);
+
+}  // namespace psm

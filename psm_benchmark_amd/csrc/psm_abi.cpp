@@ -1,0 +1,1208 @@
+// psm_abi.cpp — host side of the C ABI declared in include/psm_abi.h.
+//
+// A Batch runs the matching pipeline for a set of queries against one
+// resident data graph, each stage as a handful of launches over all queries:
+//   filter   LDF / NLF bucket scans, GQL refinement rounds, CFL passes
+//   tables   candidate bitmaps + ranks, BuildTable count / scan / fill
+//   plan     GQL or CFL matching order on the host (n <= 64 per query)
+//   evaluate task pre-expansion, then one persistent DFS kernel
+// The plain path (psm_query) is a batch of one query driven stage by stage.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "internal.h"
+#include "plan.h"
+#include "psm_abi.h"
+
+using namespace psm;
+
+// ---------------------------------------------------------------------------
+// context, errors, device buffers
+// ---------------------------------------------------------------------------
+struct psm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+    std::string err;
+};
+
+namespace {
+
+int set_err(psm_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(psm_ctx* ctx, hipError_t e, const char* what, int line) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s failed (%s) at psm_abi.cpp:%d", what, hipGetErrorString(e), line);
+    if (ctx) ctx->err = buf;
+    return e == hipErrorOutOfMemory ? PSM_ENOMEM : -(int)e;
+}
+
+#define HIP_TRY(ctx, expr)                                                      \
+    do {                                                                        \
+        hipError_t _e = (expr);                                                 \
+        if (_e != hipSuccess) return hip_fail((ctx), _e, #expr, __LINE__);      \
+    } while (0)
+
+#define PSM_TRY(expr)                     \
+    do {                                  \
+        int _r = (expr);                  \
+        if (_r != PSM_OK) return _r;      \
+    } while (0)
+
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t b) {
+        release();
+        bytes = b ? b : 16;
+        return hipMalloc(&p, bytes);
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Host vectors handed to upload() are often temporaries: the copy completes
+// before returning so the source may be freed at once.
+template <class T>
+hipError_t upload(DBuf& buf, const std::vector<T>& v, hipStream_t s) {
+    hipError_t e = buf.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess || v.empty()) return e;
+    e = hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+template <class T>
+hipError_t download(std::vector<T>& v, const void* src, size_t n, hipStream_t s) {
+    v.resize(n);
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// data graph
+// ---------------------------------------------------------------------------
+struct psm_graph {
+    psm_ctx* ctx = nullptr;
+    uint32_t V = 0, L = 0, max_deg = 0, max_freq = 0;
+    uint64_t nnz = 0;
+    std::vector<uint32_t> h_ri_off;   // host copy: bucket sizes drive work items
+    DBuf off, adj, lab, ri_off, ri, nlf_off, nlf;
+    DevGraph dg{};
+};
+
+// Reverse index (graph.cpp:15-33) and neighbour-label runs (graph.cpp:36-52)
+// built on the host with all cores, then uploaded.
+static void build_graph_indexes(uint32_t V, const uint32_t* off, const uint32_t* adj,
+                                const uint32_t* lab, uint32_t L, std::vector<uint32_t>& ri_off,
+                                std::vector<uint32_t>& ri, std::vector<uint32_t>& nlf_off,
+                                std::vector<uint2>& nlf) {
+    ri_off.assign(L + 1, 0);
+    for (uint32_t v = 0; v < V; ++v) ri_off[lab[v] + 1]++;
+    for (uint32_t l = 0; l < L; ++l) ri_off[l + 1] += ri_off[l];
+    ri.resize(V);
+    {
+        std::vector<uint32_t> fill(ri_off.begin(), ri_off.end() - 1);
+        for (uint32_t v = 0; v < V; ++v) ri[fill[lab[v]]++] = v;
+    }
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<uint32_t> runs(V, 0);
+    auto pass = [&](int write, unsigned t) {
+        std::vector<uint32_t> cnt(L, 0), touched;
+        for (uint32_t v = t; v < V; v += nt) {
+            touched.clear();
+            for (uint32_t p = off[v]; p < off[v + 1]; ++p) {
+                uint32_t l = lab[adj[p]];
+                if (cnt[l]++ == 0) touched.push_back(l);
+            }
+            if (write) {
+                std::sort(touched.begin(), touched.end());
+                uint32_t o = nlf_off[v];
+                for (uint32_t l : touched) nlf[o++] = make_uint2(l, cnt[l]);
+            } else {
+                runs[v] = (uint32_t)touched.size();
+            }
+            for (uint32_t l : touched) cnt[l] = 0;
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) th.emplace_back(pass, 0, t);
+    for (auto& x : th) x.join();
+    th.clear();
+    nlf_off.assign(V + 1, 0);
+    for (uint32_t v = 0; v < V; ++v) nlf_off[v + 1] = nlf_off[v] + runs[v];
+    nlf.resize(nlf_off[V]);
+    for (unsigned t = 0; t < nt; ++t) th.emplace_back(pass, 1, t);
+    for (auto& x : th) x.join();
+}
+
+// ---------------------------------------------------------------------------
+// query host state
+// ---------------------------------------------------------------------------
+namespace {
+
+struct QueryHost {
+    QGraph qg;
+    int status = PSM_OK;              // filter result (PSM_EEMPTY like the reference's false)
+    std::vector<uint32_t> cand_count;
+    std::vector<uint64_t> cand_begin;
+    uint32_t slot_base = 0;           // bitmap slot of (this query, u) = slot_base + u
+    std::vector<uint64_t> tab_row;    // n*n: first row of table (a,b), ~0 when no edge
+    std::vector<uint64_t> tab_nnz;    // n*n
+    Plan plan;
+    bool planned = false;
+    uint64_t count = 0, calls = 0, probe_bytes = 0;
+    bool capped = false;
+};
+
+constexpr uint64_t kNoTable = ~0ull;
+
+struct Batch {
+    psm_ctx* ctx = nullptr;
+    const psm_graph* g = nullptr;
+    std::vector<QueryHost*> qs;
+    // candidates
+    DBuf cand_pool;
+    uint64_t cand_total = 0;
+    bool bitmaps_current = false;     // bitmaps == final candidate sets
+    // bitmaps
+    DBuf bitmaps, wprefix;
+    uint64_t wps = 0;
+    uint32_t n_slots = 0;
+    // tables
+    DBuf row_off, idx;
+    uint64_t n_rows = 0, n_idx = 0;
+    bool tables_built = false;
+    // plan + enumeration
+    DBuf plans, ops, qcnt;
+    double t_filter = 0, t_table = 0, t_plan = 0, t_enum = 0, t_expand = 0;
+    uint64_t tasks_run = 0;
+
+    hipStream_t s() const { return ctx->stream; }
+
+    int filter(int method);
+    int bucket_filter(bool use_nlf, std::vector<QueryHost*>& active);
+    int ensure_bitmaps(bool from_candidates);
+    int compact_pool();
+    int gql_refine(std::vector<QueryHost*>& active);
+    int cfl_filter(std::vector<QueryHost*>& active);
+    int build_tables();
+    int make_plans(int order_method);
+    int upload_plans(uint64_t cap);
+    int evaluate(uint64_t cap);
+    int host_table(const QueryHost& q, uint32_t a, uint32_t b, std::vector<uint32_t>& off,
+                   std::vector<uint32_t>& idxv);
+    int nlf_count(const QGraph& q, uint32_t u, uint32_t label_override, uint32_t& out);
+};
+
+// ---- candidate filters ------------------------------------------------------
+
+// NLF / LDF over label buckets for every (query, u) with a live query: one
+// compaction over all buckets. Candidate pool = concatenation of all sets.
+int Batch::bucket_filter(bool use_nlf, std::vector<QueryHost*>& active) {
+    std::vector<QVert> qv;
+    std::vector<uint2> qnlf;
+    std::vector<CItem> items;
+    uint32_t blocks = 0;
+    for (QueryHost* q : qs) {
+        q->cand_count.assign(q->qg.n, 0);
+        q->cand_begin.assign(q->qg.n, 0);
+        for (uint32_t u = 0; u < q->qg.n; ++u) {
+            QVert v;
+            v.label = q->qg.lab[u];
+            v.degree = q->qg.degree(u);
+            v.nlf_begin = (uint32_t)qnlf.size();
+            v.nlf_count = (uint32_t)q->qg.nlf[u].size();
+            for (auto& r : q->qg.nlf[u]) qnlf.push_back(make_uint2(r.first, r.second));
+            CItem it{};
+            const uint32_t l = v.label;
+            it.src_begin = l < g->L ? g->h_ri_off[l] : 0;
+            it.len = l < g->L ? g->h_ri_off[l + 1] - g->h_ri_off[l] : 0;
+            it.block_start = blocks;
+            it.mode = use_nlf ? kModeNLF : kModeLDF;
+            it.qv = (uint32_t)qv.size();
+            qv.push_back(v);
+            items.push_back(it);
+            blocks += (it.len + 255) / 256;
+        }
+    }
+    DBuf d_items, d_qv, d_qnlf, d_masks, d_bc, d_bp, d_tmp, d_starts, d_ends, d_cnt, d_beg;
+    HIP_TRY(ctx, upload(d_items, items, s()));
+    HIP_TRY(ctx, upload(d_qv, qv, s()));
+    HIP_TRY(ctx, upload(d_qnlf, qnlf, s()));
+    HIP_TRY(ctx, d_masks.alloc((size_t)blocks * 4 * 8));
+    HIP_TRY(ctx, d_bc.alloc((size_t)blocks * 4));
+    HIP_TRY(ctx, d_bp.alloc((size_t)(blocks + 1) * 8));
+    HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(blocks) * 8));
+    HIP_TRY(ctx, launch_compact_flag(g->dg, d_items.as<CItem>(), (uint32_t)items.size(), blocks,
+                                     d_qv.as<QVert>(), d_qnlf.as<uint2>(), nullptr, nullptr,
+                                     d_masks.as<unsigned long long>(), d_bc.as<uint32_t>(), s()));
+    HIP_TRY(ctx, launch_scan_u32(d_bc.as<uint32_t>(), blocks, d_bp.as<uint64_t>(),
+                                 d_tmp.as<unsigned long long>(), s()));
+    // per item: begin = prefix[block_start], count = prefix[next start] - begin
+    const uint32_t ni = (uint32_t)items.size();
+    std::vector<uint64_t> starts(ni), ends(ni), zeros(ni, 0);
+    for (uint32_t i = 0; i < ni; ++i) {
+        starts[i] = items[i].block_start;
+        ends[i] = i + 1 < ni ? items[i + 1].block_start : blocks;
+    }
+    HIP_TRY(ctx, upload(d_starts, starts, s()));
+    HIP_TRY(ctx, upload(d_ends, ends, s()));
+    HIP_TRY(ctx, d_cnt.alloc((size_t)ni * 8));
+    HIP_TRY(ctx, d_beg.alloc((size_t)ni * 8));
+    HIP_TRY(ctx, launch_span(d_bp.as<uint64_t>(), d_starts.as<uint64_t>(), d_ends.as<uint64_t>(), ni,
+                             d_cnt.as<uint64_t>(), s()));
+    {
+        DBuf d_zero;
+        HIP_TRY(ctx, upload(d_zero, zeros, s()));
+        HIP_TRY(ctx, launch_span(d_bp.as<uint64_t>(), d_zero.as<uint64_t>(), d_starts.as<uint64_t>(),
+                                 ni, d_beg.as<uint64_t>(), s()));
+        std::vector<uint64_t> cnt, beg;
+        HIP_TRY(ctx, download(cnt, d_cnt.p, ni, s()));
+        HIP_TRY(ctx, download(beg, d_beg.p, ni, s()));
+        uint32_t i = 0;
+        cand_total = 0;
+        for (QueryHost* q : qs) {
+            for (uint32_t u = 0; u < q->qg.n; ++u, ++i) {
+                q->cand_count[u] = (uint32_t)cnt[i];
+                q->cand_begin[u] = beg[i];
+                cand_total = std::max(cand_total, beg[i] + cnt[i]);
+            }
+        }
+    }
+    HIP_TRY(ctx, cand_pool.alloc(cand_total * 4));
+    HIP_TRY(ctx, launch_compact_scatter(d_items.as<CItem>(), ni, blocks, g->ri.as<uint32_t>(), nullptr,
+                                        d_masks.as<unsigned long long>(), d_bp.as<uint64_t>(),
+                                        cand_pool.as<uint32_t>(), s()));
+    HIP_TRY(ctx, hipStreamSynchronize(s()));
+    // FilterVertices.cpp:19-65: the reference stops at the first empty set
+    active.clear();
+    for (QueryHost* q : qs) {
+        q->status = PSM_OK;
+        for (uint32_t u = 0; u < q->qg.n; ++u) {
+            if (q->status != PSM_OK) q->cand_count[u] = 0;
+            else if (q->cand_count[u] == 0) q->status = PSM_EEMPTY;
+        }
+        if (q->status == PSM_OK) active.push_back(q);
+    }
+    bitmaps_current = false;
+    return PSM_OK;
+}
+
+// Bitmap slots for every (query, u); set from the current candidates.
+int Batch::ensure_bitmaps(bool from_candidates) {
+    wps = ((uint64_t)g->V + 31) / 32;
+    uint32_t slots = 0;
+    for (QueryHost* q : qs) {
+        q->slot_base = slots;
+        slots += q->qg.n;
+    }
+    n_slots = slots;
+    if (bitmaps.bytes < (size_t)slots * wps * 4) HIP_TRY(ctx, bitmaps.alloc((size_t)slots * wps * 4));
+    HIP_TRY(ctx, hipMemsetAsync(bitmaps.p, 0, (size_t)slots * wps * 4, s()));
+    if (!from_candidates) return PSM_OK;
+    std::vector<uint64_t> sb;
+    std::vector<uint32_t> sl, ss;
+    uint32_t maxlen = 0;
+    for (QueryHost* q : qs)
+        for (uint32_t u = 0; u < q->qg.n; ++u) {
+            if (!q->cand_count[u]) continue;
+            sb.push_back(q->cand_begin[u]);
+            sl.push_back(q->cand_count[u]);
+            ss.push_back(q->slot_base + u);
+            maxlen = std::max(maxlen, q->cand_count[u]);
+        }
+    DBuf d_sb, d_sl, d_ss;
+    HIP_TRY(ctx, upload(d_sb, sb, s()));
+    HIP_TRY(ctx, upload(d_sl, sl, s()));
+    HIP_TRY(ctx, upload(d_ss, ss, s()));
+    for (size_t i = 0; i < sb.size(); i += 65535) {
+        uint32_t ns = (uint32_t)std::min<size_t>(65535, sb.size() - i);
+        HIP_TRY(ctx, launch_set_bits(cand_pool.as<uint32_t>(), d_sb.as<uint64_t>() + i,
+                                     d_sl.as<uint32_t>() + i, d_ss.as<uint32_t>() + i, ns, maxlen,
+                                     bitmaps.as<uint32_t>(), wps, s()));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s()));
+    bitmaps_current = true;
+    return PSM_OK;
+}
+
+// Removes kInvalid entries from every candidate list, preserving order.
+int Batch::compact_pool() {
+    std::vector<CItem> items;
+    uint32_t blocks = 0;
+    for (QueryHost* q : qs)
+        for (uint32_t u = 0; u < q->qg.n; ++u) {
+            CItem it{};
+            it.src_begin = q->cand_begin[u];
+            it.len = q->cand_count[u];
+            it.block_start = blocks;
+            it.mode = kModeValid;
+            items.push_back(it);
+            blocks += (it.len + 255) / 256;
+        }
+    const uint32_t ni = (uint32_t)items.size();
+    DBuf d_items, d_masks, d_bc, d_bp, d_tmp, d_starts, d_ends, d_zero, d_cnt, d_beg, out;
+    HIP_TRY(ctx, upload(d_items, items, s()));
+    HIP_TRY(ctx, d_masks.alloc((size_t)blocks * 4 * 8));
+    HIP_TRY(ctx, d_bc.alloc((size_t)blocks * 4));
+    HIP_TRY(ctx, d_bp.alloc((size_t)(blocks + 1) * 8));
+    HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(blocks) * 8));
+    HIP_TRY(ctx, launch_compact_flag(g->dg, d_items.as<CItem>(), ni, blocks, nullptr, nullptr,
+                                     cand_pool.as<uint32_t>(), nullptr,
+                                     d_masks.as<unsigned long long>(), d_bc.as<uint32_t>(), s()));
+    HIP_TRY(ctx, launch_scan_u32(d_bc.as<uint32_t>(), blocks, d_bp.as<uint64_t>(),
+                                 d_tmp.as<unsigned long long>(), s()));
+    std::vector<uint64_t> starts(ni), ends(ni), zeros(ni, 0), cnt, beg;
+    for (uint32_t i = 0; i < ni; ++i) {
+        starts[i] = items[i].block_start;
+        ends[i] = i + 1 < ni ? items[i + 1].block_start : blocks;
+    }
+    HIP_TRY(ctx, upload(d_starts, starts, s()));
+    HIP_TRY(ctx, upload(d_ends, ends, s()));
+    HIP_TRY(ctx, upload(d_zero, zeros, s()));
+    HIP_TRY(ctx, d_cnt.alloc((size_t)ni * 8));
+    HIP_TRY(ctx, d_beg.alloc((size_t)ni * 8));
+    HIP_TRY(ctx, launch_span(d_bp.as<uint64_t>(), d_starts.as<uint64_t>(), d_ends.as<uint64_t>(), ni,
+                             d_cnt.as<uint64_t>(), s()));
+    HIP_TRY(ctx, launch_span(d_bp.as<uint64_t>(), d_zero.as<uint64_t>(), d_starts.as<uint64_t>(), ni,
+                             d_beg.as<uint64_t>(), s()));
+    HIP_TRY(ctx, download(cnt, d_cnt.p, ni, s()));
+    HIP_TRY(ctx, download(beg, d_beg.p, ni, s()));
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < ni; ++i) total = std::max(total, beg[i] + cnt[i]);
+    HIP_TRY(ctx, out.alloc(total * 4));
+    HIP_TRY(ctx, launch_compact_scatter(d_items.as<CItem>(), ni, blocks, nullptr, cand_pool.as<uint32_t>(),
+                                        d_masks.as<unsigned long long>(), d_bp.as<uint64_t>(),
+                                        out.as<uint32_t>(), s()));
+    HIP_TRY(ctx, hipStreamSynchronize(s()));
+    std::swap(cand_pool.p, out.p);
+    std::swap(cand_pool.bytes, out.bytes);
+    cand_total = total;
+    uint32_t i = 0;
+    for (QueryHost* q : qs)
+        for (uint32_t u = 0; u < q->qg.n; ++u, ++i) {
+            q->cand_count[u] = (uint32_t)cnt[i];
+            q->cand_begin[u] = beg[i];
+        }
+    return PSM_OK;
+}
+
+// FilterVertices.cpp:67-143: two in-place rounds over u = 0..n-1; queries
+// advance in lock step, one launch per (round, u).
+int Batch::gql_refine(std::vector<QueryHost*>& active) {
+    PSM_TRY(ensure_bitmaps(true));
+    uint32_t max_n = 0;
+    for (QueryHost* q : active) max_n = std::max(max_n, q->qg.n);
+    DBuf d_items, d_left;
+    for (int round = 0; round < 2; ++round) {
+        for (uint32_t p = 0; p < max_n; ++p) {
+            std::vector<GItem> items;
+            std::vector<uint32_t> left;
+            uint32_t waves = 0;
+            for (QueryHost* q : active) {
+                if (p >= q->qg.n || q->cand_count[p] == 0) continue;
+                GItem it{};
+                it.cand_begin = q->cand_begin[p];
+                it.len = q->cand_count[p];
+                it.wave_start = waves;
+                it.bm_self = q->slot_base + p;
+                it.k = q->qg.degree(p);
+                it.left_begin = (uint32_t)left.size();
+                for (uint32_t i = q->qg.off[p]; i < q->qg.off[p + 1]; ++i)
+                    left.push_back(q->slot_base + q->qg.adj[i]);
+                items.push_back(it);
+                waves += it.len;
+            }
+            if (items.empty()) continue;
+            HIP_TRY(ctx, upload(d_items, items, s()));
+            HIP_TRY(ctx, upload(d_left, left, s()));
+            HIP_TRY(ctx, launch_gql_refine(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), waves,
+                                           d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
+                                           bitmaps.as<uint32_t>(), wps, s()));
+        }
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s()));
+    PSM_TRY(compact_pool());
+    bitmaps_current = true;   // refinement kept bitmaps == surviving candidates
+    for (QueryHost* q : active)
+        for (uint32_t u = 0; u < q->qg.n; ++u)
+            if (q->cand_count[u] == 0) q->status = PSM_EEMPTY;
+    return PSM_OK;
+}
+
+int Batch::filter(int method) {
+    const double t0 = now_ms();
+    std::vector<QueryHost*> active;
+    for (QueryHost* q : qs) {
+        if (q->qg.n == 0 || q->qg.n > kMaxQ)
+            return set_err(ctx, PSM_EINVAL, "query size must be in [1, 64]");
+        if (method == PSM_FILTER_GQL && q->qg.max_degree() > kMaxGqlDegree)
+            return set_err(ctx, PSM_EINVAL, "GQL filter: query vertex degree above 32");
+    }
+    tables_built = false;
+    for (QueryHost* q : qs) q->planned = false;
+    switch (method) {
+        case PSM_FILTER_LDF: PSM_TRY(bucket_filter(false, active)); break;
+        case PSM_FILTER_NLF: PSM_TRY(bucket_filter(true, active)); break;
+        case PSM_FILTER_GQL:
+            PSM_TRY(bucket_filter(true, active));
+            if (!active.empty()) PSM_TRY(gql_refine(active));
+            break;
+        case PSM_FILTER_CFL:
+            for (QueryHost* q : qs) active.push_back(q);
+            PSM_TRY(cfl_filter(active));
+            break;
+        default: return set_err(ctx, PSM_EINVAL, "unknown filter method");
+    }
+    t_filter += now_ms() - t0;
+    return PSM_OK;
+}
+
+// ---- BuildTable --------------------------------------------------------------
+int Batch::build_tables() {
+    const double t0 = now_ms();
+    if (!bitmaps_current) PSM_TRY(ensure_bitmaps(true));
+    if (wprefix.bytes < (size_t)n_slots * wps * 4) HIP_TRY(ctx, wprefix.alloc((size_t)n_slots * wps * 4));
+    HIP_TRY(ctx, launch_slot_prefix(bitmaps.as<uint32_t>(), wprefix.as<uint32_t>(), n_slots, wps, s()));
+    std::vector<TItem> items;
+    std::vector<uint64_t> row_start;
+    std::vector<std::pair<QueryHost*, uint32_t>> item_of;  // (query, a*n+b)
+    n_rows = 0;
+    for (QueryHost* q : qs) {
+        const uint32_t n = q->qg.n;
+        q->tab_row.assign((size_t)n * n, kNoTable);
+        q->tab_nnz.assign((size_t)n * n, 0);
+        for (uint32_t a = 0; a < n; ++a)
+            for (uint32_t i = q->qg.off[a]; i < q->qg.off[a + 1]; ++i) {
+                const uint32_t b = q->qg.adj[i];
+                TItem it{};
+                it.cand_a = q->cand_begin[a];
+                it.len_a = q->cand_count[a];
+                it.bm_b = q->slot_base + b;
+                it.row_base = n_rows;
+                q->tab_row[(size_t)a * n + b] = n_rows;
+                items.push_back(it);
+                row_start.push_back(n_rows);
+                item_of.push_back({q, a * n + b});
+                n_rows += it.len_a;
+            }
+    }
+    const uint32_t ni = (uint32_t)items.size();
+    DBuf d_items, d_rs, d_deg, d_tmp, d_ends, d_nnz;
+    HIP_TRY(ctx, upload(d_items, items, s()));
+    HIP_TRY(ctx, upload(d_rs, row_start, s()));
+    HIP_TRY(ctx, d_deg.alloc(n_rows * 8));
+    HIP_TRY(ctx, row_off.alloc((n_rows + 1) * 8));
+    HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(n_rows) * 8));
+    HIP_TRY(ctx, launch_table_count(g->dg, d_items.as<TItem>(), ni, d_rs.as<uint64_t>(), n_rows,
+                                    cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
+                                    d_deg.as<uint64_t>(), s()));
+    HIP_TRY(ctx, launch_scan_u64(d_deg.as<uint64_t>(), n_rows, row_off.as<uint64_t>(),
+                                 d_tmp.as<unsigned long long>(), s()));
+    HIP_TRY(ctx, hipMemcpyAsync(&n_idx, row_off.as<uint64_t>() + n_rows, 8, hipMemcpyDeviceToHost, s()));
+    HIP_TRY(ctx, hipStreamSynchronize(s()));
+    HIP_TRY(ctx, idx.alloc(n_idx * 4));
+    HIP_TRY(ctx, launch_table_fill(g->dg, d_items.as<TItem>(), ni, d_rs.as<uint64_t>(), n_rows,
+                                   cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(),
+                                   wprefix.as<uint32_t>(), wps, row_off.as<uint64_t>(),
+                                   idx.as<uint32_t>(), s()));
+    std::vector<uint64_t> ends(ni), nnz;
+    for (uint32_t i = 0; i < ni; ++i) ends[i] = row_start[i] + items[i].len_a;
+    HIP_TRY(ctx, upload(d_ends, ends, s()));
+    HIP_TRY(ctx, d_nnz.alloc((size_t)ni * 8));
+    HIP_TRY(ctx, launch_span(row_off.as<uint64_t>(), d_rs.as<uint64_t>(), d_ends.as<uint64_t>(), ni,
+                             d_nnz.as<uint64_t>(), s()));
+    HIP_TRY(ctx, download(nnz, d_nnz.p, ni, s()));
+    for (uint32_t i = 0; i < ni; ++i) item_of[i].first->tab_nnz[item_of[i].second] = nnz[i];
+    tables_built = true;
+    t_table += now_ms() - t0;
+    return PSM_OK;
+}
+
+// Host copy of one table (CFL path-estimate input; psm_query_table).
+int Batch::host_table(const QueryHost& q, uint32_t a, uint32_t b, std::vector<uint32_t>& off,
+                      std::vector<uint32_t>& idxv) {
+    const uint32_t n = q.qg.n;
+    const uint64_t r0 = q.tab_row[(size_t)a * n + b];
+    if (r0 == kNoTable) return PSM_EINVAL;
+    const uint32_t rows = q.cand_count[a];
+    std::vector<uint64_t> ro;
+    HIP_TRY(ctx, download(ro, row_off.as<uint64_t>() + r0, rows + 1, s()));
+    off.resize(rows + 1);
+    for (uint32_t i = 0; i <= rows; ++i) off[i] = (uint32_t)(ro[i] - ro[0]);
+    HIP_TRY(ctx, download(idxv, idx.as<uint32_t>() + ro[0], ro[rows] - ro[0], s()));
+    return PSM_OK;
+}
+
+// |C(u)| under the NLF filter for one query vertex whose label is replaced by
+// `label_override` (the CFL unit start-vertex rule runs on relabelled units).
+int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out) {
+    std::vector<QVert> qv(1);
+    std::vector<uint2> qn;
+    qv[0].label = label;
+    qv[0].degree = q.degree(u);
+    qv[0].nlf_begin = 0;
+    qv[0].nlf_count = (uint32_t)q.nlf[u].size();
+    for (auto& r : q.nlf[u]) qn.push_back(make_uint2(r.first, r.second));
+    out = 0;
+    if (label >= g->L) return PSM_OK;
+    std::vector<CItem> items(1);
+    items[0] = CItem{};
+    items[0].src_begin = g->h_ri_off[label];
+    items[0].len = g->h_ri_off[label + 1] - g->h_ri_off[label];
+    items[0].mode = kModeNLF;
+    const uint32_t blocks = (items[0].len + 255) / 256;
+    if (!blocks) return PSM_OK;
+    DBuf d_items, d_qv, d_qn, d_masks, d_bc;
+    HIP_TRY(ctx, upload(d_items, items, s()));
+    HIP_TRY(ctx, upload(d_qv, qv, s()));
+    if (qn.empty()) qn.push_back(make_uint2(0, 0));
+    HIP_TRY(ctx, upload(d_qn, qn, s()));
+    HIP_TRY(ctx, d_masks.alloc((size_t)blocks * 32));
+    HIP_TRY(ctx, d_bc.alloc((size_t)blocks * 4));
+    HIP_TRY(ctx, launch_compact_flag(g->dg, d_items.as<CItem>(), 1, blocks, d_qv.as<QVert>(),
+                                     d_qn.as<uint2>(), nullptr, nullptr,
+                                     d_masks.as<unsigned long long>(), d_bc.as<uint32_t>(), s()));
+    std::vector<uint32_t> bc;
+    HIP_TRY(ctx, download(bc, d_bc.p, blocks, s()));
+    for (uint32_t c : bc) out += c;
+    return PSM_OK;
+}
+
+// ---- CFL filter (FilterVertices.cpp:188-250) ------------------------------
+// Support counters use an epoch per pass so they never need clearing: a
+// pass with P pivots moves cnt[w] from base to base+P, base = epoch*64.
+int Batch::cfl_filter(std::vector<QueryHost*>& active) {
+    DBuf cnt;
+    HIP_TRY(ctx, cnt.alloc((size_t)g->V * 4));
+    HIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, (size_t)g->V * 4, s()));
+    uint32_t epoch = 0;
+    // The plain CFL filter runs query by query: its steps depend on the
+    // query's own BFS levels.
+    std::vector<std::vector<uint32_t>> sets;
+    std::vector<uint32_t> all;   // concatenated final candidates
+    for (QueryHost* q : active) {
+        const QGraph& qg = q->qg;
+        const uint32_t n = qg.n;
+        std::vector<uint32_t> nlf_counts(n, kInvalid);
+        auto nlf_of = [&](uint32_t u, uint32_t& c) -> int {
+            if (nlf_counts[u] == kInvalid) PSM_TRY(nlf_count(qg, u, qg.lab[u], nlf_counts[u]));
+            c = nlf_counts[u];
+            return PSM_OK;
+        };
+        CflTree t;
+        PSM_TRY(cfl_filter_plan(qg, g->L, g->h_ri_off, g->max_freq,
+                                [&](uint32_t u, uint32_t& c) { return nlf_of(u, c); }, t));
+        // per-vertex candidate lists live in their own device buffers while
+        // the passes run
+        std::vector<std::unique_ptr<DBuf>> cb(n);
+        std::vector<uint32_t> len(n, 0);
+        auto run_compact = [&](uint32_t u, uint32_t mode, uint32_t target) -> int {
+            std::vector<QVert> qv(1);
+            std::vector<uint2> qn;
+            qv[0].label = qg.lab[u];
+            qv[0].degree = qg.degree(u);
+            qv[0].nlf_begin = 0;
+            qv[0].nlf_count = (uint32_t)qg.nlf[u].size();
+            for (auto& r : qg.nlf[u]) qn.push_back(make_uint2(r.first, r.second));
+            if (qn.empty()) qn.push_back(make_uint2(0, 0));
+            std::vector<CItem> items(1);
+            items[0] = CItem{};
+            const uint32_t l = qg.lab[u];
+            items[0].src_begin = l < g->L ? g->h_ri_off[l] : 0;
+            items[0].len = l < g->L ? g->h_ri_off[l + 1] - g->h_ri_off[l] : 0;
+            items[0].mode = mode;
+            items[0].target = target;
+            const uint32_t blocks = (items[0].len + 255) / 256;
+            cb[u].reset(new DBuf());
+            len[u] = 0;
+            if (!blocks) { HIP_TRY(ctx, cb[u]->alloc(4)); return PSM_OK; }
+            DBuf d_items, d_qv, d_qn, d_masks, d_bc, d_bp, d_tmp;
+            HIP_TRY(ctx, upload(d_items, items, s()));
+            HIP_TRY(ctx, upload(d_qv, qv, s()));
+            HIP_TRY(ctx, upload(d_qn, qn, s()));
+            HIP_TRY(ctx, d_masks.alloc((size_t)blocks * 32));
+            HIP_TRY(ctx, d_bc.alloc((size_t)blocks * 4));
+            HIP_TRY(ctx, d_bp.alloc((size_t)(blocks + 1) * 8));
+            HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(blocks) * 8));
+            HIP_TRY(ctx, launch_compact_flag(g->dg, d_items.as<CItem>(), 1, blocks, d_qv.as<QVert>(),
+                                             d_qn.as<uint2>(), nullptr, cnt.as<uint32_t>(),
+                                             d_masks.as<unsigned long long>(), d_bc.as<uint32_t>(), s()));
+            HIP_TRY(ctx, launch_scan_u32(d_bc.as<uint32_t>(), blocks, d_bp.as<uint64_t>(),
+                                         d_tmp.as<unsigned long long>(), s()));
+            uint64_t total = 0;
+            HIP_TRY(ctx, hipMemcpyAsync(&total, d_bp.as<uint64_t>() + blocks, 8, hipMemcpyDeviceToHost, s()));
+            HIP_TRY(ctx, hipStreamSynchronize(s()));
+            HIP_TRY(ctx, cb[u]->alloc(total * 4));
+            HIP_TRY(ctx, launch_compact_scatter(d_items.as<CItem>(), 1, blocks, g->ri.as<uint32_t>(), nullptr,
+                                                d_masks.as<unsigned long long>(), d_bp.as<uint64_t>(),
+                                                cb[u]->as<uint32_t>(), s()));
+            len[u] = (uint32_t)total;
+            return PSM_OK;
+        };
+        auto support = [&](uint32_t u, const std::vector<uint32_t>& pivots, uint32_t& target) -> int {
+            ++epoch;
+            const uint32_t base = epoch * 64;
+            for (uint32_t i = 0; i < pivots.size(); ++i) {
+                const uint32_t p = pivots[i];
+                HIP_TRY(ctx, launch_cfl_support(g->dg, cb[p]->as<uint32_t>(), 0, len[p], qg.lab[u],
+                                                qg.degree(u), cnt.as<uint32_t>(),
+                                                i == 0 ? 0u : base + i, base + i + 1, s()));
+            }
+            target = base + (uint32_t)pivots.size();
+            return PSM_OK;
+        };
+        const uint32_t root = t.order[0];
+        PSM_TRY(run_compact(root, kModeNLF, 0));
+        for (uint32_t i = 0; i < n; ++i)
+            if (!cb[i]) { cb[i].reset(new DBuf()); HIP_TRY(ctx, cb[i]->alloc(4)); }
+        const uint32_t nlev = (uint32_t)t.level_off.size() - 1;
+        auto prune = [&](uint32_t u, const std::vector<uint32_t>& piv) -> int {
+            uint32_t target;
+            PSM_TRY(support(u, piv, target));
+            HIP_TRY(ctx, launch_cfl_prune(cnt.as<uint32_t>(), cb[u]->as<uint32_t>(), 0, len[u], target, s()));
+            return PSM_OK;
+        };
+        for (uint32_t lv = 1; lv < nlev; ++lv) {
+            for (uint32_t j = t.level_off[lv]; j < t.level_off[lv + 1]; ++j) {
+                const uint32_t u = t.order[j];
+                uint32_t target;
+                PSM_TRY(support(u, t.bn[u], target));
+                PSM_TRY(run_compact(u, kModeCflGen, target));
+            }
+            for (int j = (int)t.level_off[lv + 1] - 1; j >= (int)t.level_off[lv]; --j) {
+                const uint32_t u = t.order[j];
+                if (!t.fn[u].empty()) PSM_TRY(prune(u, t.fn[u]));
+            }
+        }
+        for (int lv = (int)nlev - 2; lv >= 0; --lv)
+            for (uint32_t j = t.level_off[lv]; j < t.level_off[lv + 1]; ++j) {
+                const uint32_t u = t.order[j];
+                if (!t.under[u].empty()) PSM_TRY(prune(u, t.under[u]));
+            }
+        // gather into the shared pool (compaction below drops pruned entries)
+        HIP_TRY(ctx, hipStreamSynchronize(s()));
+        q->cand_count.assign(n, 0);
+        q->cand_begin.assign(n, 0);
+        for (uint32_t u = 0; u < n; ++u) {
+            std::vector<uint32_t> h;
+            HIP_TRY(ctx, download(h, cb[u]->p, len[u], s()));
+            q->cand_begin[u] = all.size();
+            q->cand_count[u] = len[u];
+            all.insert(all.end(), h.begin(), h.end());
+        }
+    }
+    for (QueryHost* q : qs)
+        if (q->cand_count.size() != q->qg.n) {
+            q->cand_count.assign(q->qg.n, 0);
+            q->cand_begin.assign(q->qg.n, 0);
+        }
+    HIP_TRY(ctx, upload(cand_pool, all, s()));
+    cand_total = all.size();
+    PSM_TRY(compact_pool());
+    bitmaps_current = false;
+    for (QueryHost* q : qs) {
+        q->status = PSM_OK;
+        for (uint32_t u = 0; u < q->qg.n; ++u)
+            if (q->cand_count[u] == 0) q->status = PSM_EEMPTY;
+    }
+    return PSM_OK;
+}
+
+// ---- plans ----------------------------------------------------------------
+int Batch::make_plans(int order_method) {
+    const double t0 = now_ms();
+    if (!tables_built) return set_err(ctx, PSM_ESTATE, "build tables before generating a plan");
+    for (QueryHost* q : qs) {
+        if (order_method == PSM_ORDER_GQL) {
+            gql_order(q->qg, q->cand_count, g->V, q->plan);
+            backward_neighbours(q->qg, q->plan, false);
+        } else if (order_method == PSM_ORDER_CFL) {
+            const QGraph ug = unit_graph(q->qg);
+            std::vector<uint32_t> cache(ug.n, kInvalid);
+            auto nlf_of = [&](uint32_t u, uint32_t& c) -> int {
+                if (cache[u] == kInvalid) PSM_TRY(nlf_count(ug, u, ug.lab[u], cache[u]));
+                c = cache[u];
+                return PSM_OK;
+            };
+            CflTree t;
+            PSM_TRY(cfl_filter_plan(ug, g->L, g->h_ri_off, g->max_freq, nlf_of, t));
+            auto table = [&](uint32_t a, uint32_t b, std::vector<uint32_t>& off,
+                             std::vector<uint32_t>& ix) { return host_table(*q, a, b, off, ix); };
+            PSM_TRY(cfl_order(ug, t, q->cand_count, table, q->plan));
+            backward_neighbours(q->qg, q->plan, true);
+        } else {
+            return set_err(ctx, PSM_EINVAL, "unknown order method");
+        }
+        q->planned = true;
+    }
+    t_plan += now_ms() - t0;
+    return PSM_OK;
+}
+
+int Batch::upload_plans(uint64_t cap) {
+    std::vector<DPlan> hp(qs.size());
+    std::vector<DOp> hops;
+    for (size_t i = 0; i < qs.size(); ++i) {
+        QueryHost* q = qs[i];
+        DPlan& P = hp[i];
+        memset(&P, 0, sizeof(P));
+        const uint32_t n = q->qg.n;
+        P.n = n;
+        P.cap = cap ? cap : ~0ull;
+        std::vector<uint32_t> pos(n);
+        for (uint32_t d = 0; d < n; ++d) pos[q->plan.order[d]] = d;
+        for (uint32_t d = 0; d < n; ++d) {
+            const uint32_t u = q->plan.order[d];
+            P.cand_base[d] = q->cand_begin[u];
+            P.cand_len[d] = q->cand_count[u];
+            P.op_begin[d] = (uint32_t)hops.size();
+            for (uint32_t k = q->qg.off[u]; k < q->qg.off[u + 1]; ++k) {
+                const uint32_t b = q->qg.adj[k];
+                if (pos[b] >= d) continue;
+                DOp op{};
+                op.src_depth = pos[b];
+                op.row_base = q->tab_row[(size_t)b * n + u];
+                hops.push_back(op);
+            }
+            P.op_count[d] = (uint32_t)hops.size() - P.op_begin[d];
+            uint64_t m = 0;
+            for (uint32_t e = 0; e < d; ++e)
+                if (q->qg.lab[q->plan.order[e]] == q->qg.lab[u]) m |= 1ull << e;
+            P.same_label[d] = m;
+        }
+    }
+    if (hops.empty()) hops.push_back(DOp{});
+    HIP_TRY(ctx, upload(plans, hp, s()));
+    HIP_TRY(ctx, upload(ops, hops, s()));
+    return PSM_OK;
+}
+
+// ---- enumeration ------------------------------------------------------------
+int Batch::evaluate(uint64_t cap) {
+    for (QueryHost* q : qs)
+        if (!q->planned) return set_err(ctx, PSM_ESTATE, "generate a plan before evaluating");
+    const double t0 = now_ms();
+    PSM_TRY(upload_plans(cap));
+    const uint32_t nq = (uint32_t)qs.size();
+    HIP_TRY(ctx, qcnt.alloc(sizeof(QCounters) * nq));
+    HIP_TRY(ctx, hipMemsetAsync(qcnt.p, 0, sizeof(QCounters) * nq, s()));
+
+    std::vector<DTask> tasks;
+    std::vector<uint32_t> depth(nq, 0), ntask(nq, 0);
+    uint32_t max_n = 1;
+    for (uint32_t i = 0; i < nq; ++i) {
+        QueryHost* q = qs[i];
+        max_n = std::max(max_n, q->qg.n);
+        if (q->cand_count[q->plan.order[0]] == 0) continue;
+        DTask t{};
+        t.q = i;
+        t.depth = 0;
+        t.lo = 0;
+        t.hi = kIdentity;
+        tasks.push_back(t);
+        ntask[i] = 1;
+    }
+    // persistent grid: 8 wavefronts per CU
+    const uint32_t grid_waves_full = (uint32_t)ctx->num_cu * 8;
+    const uint64_t target_total = (uint64_t)grid_waves_full * 64 * 4;
+    const uint64_t budget_total = std::max<uint64_t>(target_total * 8, 1u << 22);
+    uint32_t active_q = 0;
+    for (uint32_t i = 0; i < nq; ++i) active_q += ntask[i] ? 1 : 0;
+    const uint64_t per_q_target = active_q ? std::max<uint64_t>(64, target_total / active_q) : 0;
+
+    DBuf d_tasks, d_tasks2, d_cnt, d_pos, d_tmp, d_expand, d_qchild;
+    HIP_TRY(ctx, upload(d_tasks, tasks, s()));
+    uint64_t n_tasks = tasks.size();
+    hipEvent_t e0, e1;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    HIP_TRY(ctx, hipEventCreate(&e1));
+    HIP_TRY(ctx, hipEventRecord(e0, s()));
+    for (uint32_t round = 0; round < kPrefixMax && n_tasks; ++round) {
+        std::vector<uint8_t> ex(nq, 0);
+        bool any = false;
+        for (uint32_t i = 0; i < nq; ++i) {
+            if (!ntask[i]) continue;
+            const uint32_t lim = std::min<uint32_t>(qs[i]->qg.n - 1, kPrefixMax);
+            if (ntask[i] < per_q_target && depth[i] < lim) { ex[i] = 1; any = true; }
+        }
+        if (!any) break;
+        HIP_TRY(ctx, upload(d_expand, ex, s()));
+        HIP_TRY(ctx, d_qchild.alloc((size_t)nq * 8));
+        HIP_TRY(ctx, hipMemsetAsync(d_qchild.p, 0, (size_t)nq * 8, s()));
+        HIP_TRY(ctx, d_cnt.alloc(n_tasks * 4));
+        HIP_TRY(ctx, d_pos.alloc((n_tasks + 1) * 8));
+        HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(n_tasks) * 8));
+        HIP_TRY(ctx, launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
+                                   idx.as<uint32_t>(), cand_pool.as<uint32_t>(), d_tasks.as<DTask>(),
+                                   (uint32_t)n_tasks, d_expand.as<uint8_t>(), d_cnt.as<uint32_t>(),
+                                   nullptr, nullptr, qcnt.as<QCounters>(),
+                                   d_qchild.as<unsigned long long>(), 0, s()));
+        HIP_TRY(ctx, launch_scan_u32(d_cnt.as<uint32_t>(), n_tasks, d_pos.as<uint64_t>(),
+                                     d_tmp.as<unsigned long long>(), s()));
+        uint64_t total = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(&total, d_pos.as<uint64_t>() + n_tasks, 8, hipMemcpyDeviceToHost, s()));
+        HIP_TRY(ctx, hipStreamSynchronize(s()));
+        if (total > budget_total || total > 0xFFFFFFF0ull) break;
+        std::vector<unsigned long long> qchild;
+        HIP_TRY(ctx, download(qchild, d_qchild.p, nq, s()));
+        HIP_TRY(ctx, d_tasks2.alloc(std::max<uint64_t>(total, 1) * sizeof(DTask)));
+        HIP_TRY(ctx, launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
+                                   idx.as<uint32_t>(), cand_pool.as<uint32_t>(), d_tasks.as<DTask>(),
+                                   (uint32_t)n_tasks, d_expand.as<uint8_t>(), nullptr,
+                                   d_pos.as<uint64_t>(), d_tasks2.as<DTask>(), qcnt.as<QCounters>(),
+                                   nullptr, 1, s()));
+        std::swap(d_tasks.p, d_tasks2.p);
+        std::swap(d_tasks.bytes, d_tasks2.bytes);
+        for (uint32_t i = 0; i < nq; ++i) {
+            if (ex[i]) depth[i]++;
+            ntask[i] = (uint32_t)std::min<uint64_t>(qchild[i], 0xFFFFFFFFu);
+        }
+        n_tasks = total;
+    }
+    HIP_TRY(ctx, hipEventRecord(e1, s()));
+    HIP_TRY(ctx, hipEventSynchronize(e1));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    t_expand += ms;
+
+    if (n_tasks) {
+        uint32_t gw = (uint32_t)std::min<uint64_t>(grid_waves_full, (n_tasks + 63) / 64);
+        gw = std::max<uint32_t>(4, (gw + 3) / 4 * 4);
+        DBuf d_head, d_state;
+        HIP_TRY(ctx, d_head.alloc(4));
+        HIP_TRY(ctx, hipMemsetAsync(d_head.p, 0, 4, s()));
+        HIP_TRY(ctx, d_state.alloc((size_t)gw * 64 * dfs_lane_state_words(max_n) * 4));
+        HIP_TRY(ctx, hipEventRecord(e0, s()));
+        HIP_TRY(ctx, launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
+                                idx.as<uint32_t>(), cand_pool.as<uint32_t>(), d_tasks.as<DTask>(),
+                                (uint32_t)n_tasks, d_head.as<uint32_t>(), qcnt.as<QCounters>(), max_n,
+                                gw, d_state.as<uint32_t>(), s()));
+        HIP_TRY(ctx, hipEventRecord(e1, s()));
+        HIP_TRY(ctx, hipEventSynchronize(e1));
+        hipEventElapsedTime(&ms, e0, e1);
+        t_enum += ms;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    tasks_run += n_tasks;
+    std::vector<QCounters> qc;
+    HIP_TRY(ctx, download(qc, qcnt.p, nq, s()));
+    for (uint32_t i = 0; i < nq; ++i) {
+        QueryHost* q = qs[i];
+        const uint64_t c = cap ? std::min<uint64_t>(cap, qc[i].count) : qc[i].count;
+        q->count = c;
+        q->capped = cap && qc[i].count >= cap;
+        q->calls = qc[i].calls;
+        q->probe_bytes = qc[i].probe_bytes;
+    }
+    (void)t0;
+    return PSM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// public ABI
+// ---------------------------------------------------------------------------
+struct psm_query {
+    psm_ctx* ctx = nullptr;
+    const psm_graph* g = nullptr;
+    QueryHost qh;
+    Batch batch;
+    bool filtered = false;
+};
+
+extern "C" {
+
+int psm_ctx_create(int device, psm_ctx** out) {
+    if (!out) return PSM_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return -(int)e;
+    if (device < 0 || device >= n) return PSM_EINVAL;
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return -(int)e;
+    psm_ctx* c = new psm_ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return -(int)e; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    *out = c;
+    return PSM_OK;
+}
+
+int psm_ctx_destroy(psm_ctx* ctx) {
+    if (!ctx) return PSM_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PSM_OK;
+}
+
+const char* psm_last_error(const psm_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32_t* adj,
+                     const uint32_t* lab, psm_graph** out) {
+    if (!ctx || !out || !off || !lab || (off[V] && !adj)) return PSM_EINVAL;
+    *out = nullptr;
+    hipSetDevice(ctx->device);
+    std::unique_ptr<psm_graph> g(new psm_graph());
+    g->ctx = ctx;
+    g->V = V;
+    g->nnz = off[V];
+    uint32_t maxl = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        maxl = std::max(maxl, lab[v]);
+        g->max_deg = std::max(g->max_deg, off[v + 1] - off[v]);
+        if (off[v + 1] < off[v]) return set_err(ctx, PSM_EINVAL, "offsets must be non-decreasing");
+    }
+    g->L = V ? maxl + 1 : 0;
+    std::vector<uint32_t> ri, nlf_off;
+    std::vector<uint2> nlf;
+    build_graph_indexes(V, off, adj, lab, g->L, g->h_ri_off, ri, nlf_off, nlf);
+    for (uint32_t l = 0; l < g->L; ++l)
+        g->max_freq = std::max(g->max_freq, g->h_ri_off[l + 1] - g->h_ri_off[l]);
+    hipStream_t s = ctx->stream;
+    auto up = [&](DBuf& b, const void* src, size_t bytes) -> hipError_t {
+        hipError_t e = b.alloc(bytes);
+        if (e != hipSuccess || !bytes) return e;
+        return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s);
+    };
+    HIP_TRY(ctx, up(g->off, off, (size_t)(V + 1) * 4));
+    HIP_TRY(ctx, up(g->adj, adj, (size_t)g->nnz * 4));
+    HIP_TRY(ctx, up(g->lab, lab, (size_t)V * 4));
+    HIP_TRY(ctx, up(g->ri_off, g->h_ri_off.data(), g->h_ri_off.size() * 4));
+    HIP_TRY(ctx, up(g->ri, ri.data(), ri.size() * 4));
+    HIP_TRY(ctx, up(g->nlf_off, nlf_off.data(), nlf_off.size() * 4));
+    HIP_TRY(ctx, up(g->nlf, nlf.data(), nlf.size() * sizeof(uint2)));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    DevGraph& d = g->dg;
+    d.V = V;
+    d.L = g->L;
+    d.nnz = g->nnz;
+    d.off = g->off.as<uint32_t>();
+    d.adj = g->adj.as<uint32_t>();
+    d.lab = g->lab.as<uint32_t>();
+    d.ri_off = g->ri_off.as<uint32_t>();
+    d.ri = g->ri.as<uint32_t>();
+    d.nlf_off = g->nlf_off.as<uint32_t>();
+    d.nlf = g->nlf.as<uint2>();
+    *out = g.release();
+    return PSM_OK;
+}
+
+int psm_graph_destroy(psm_graph* g) {
+    delete g;
+    return PSM_OK;
+}
+
+int psm_graph_info(const psm_graph* g, uint64_t* info) {
+    if (!g || !info) return PSM_EINVAL;
+    info[0] = g->V;
+    info[1] = g->nnz / 2;
+    info[2] = g->L;
+    info[3] = g->max_deg;
+    info[4] = g->max_freq;
+    return PSM_OK;
+}
+
+int psm_query_create(psm_ctx* ctx, const psm_graph* g, uint32_t n, const uint32_t* q_off,
+                     const uint32_t* q_adj, const uint32_t* q_lab, psm_query** out) {
+    if (!ctx || !g || !out || !q_off || !q_lab) return PSM_EINVAL;
+    if (n == 0 || n > kMaxQ) return set_err(ctx, PSM_EINVAL, "query size must be in [1, 64]");
+    std::unique_ptr<psm_query> q(new psm_query());
+    q->ctx = ctx;
+    q->g = g;
+    int r = make_qgraph(n, q_off, q_adj, q_lab, q->qh.qg);
+    if (r != PSM_OK) return set_err(ctx, r, "malformed query graph");
+    q->batch.ctx = ctx;
+    q->batch.g = g;
+    q->batch.qs.push_back(&q->qh);
+    *out = q.release();
+    return PSM_OK;
+}
+
+int psm_query_destroy(psm_query* q) {
+    delete q;
+    return PSM_OK;
+}
+
+int psm_filter(psm_query* q, int method) {
+    if (!q) return PSM_EINVAL;
+    hipSetDevice(q->ctx->device);
+    PSM_TRY(q->batch.filter(method));
+    q->filtered = true;
+    return q->qh.status;
+}
+
+int psm_query_candidate_counts(const psm_query* q, uint32_t* counts) {
+    if (!q || !counts) return PSM_EINVAL;
+    if (!q->filtered) return PSM_ESTATE;
+    std::copy(q->qh.cand_count.begin(), q->qh.cand_count.end(), counts);
+    return PSM_OK;
+}
+
+int psm_query_candidates(const psm_query* q, uint32_t u, uint32_t* out) {
+    if (!q || !out || u >= q->qh.qg.n) return PSM_EINVAL;
+    if (!q->filtered) return PSM_ESTATE;
+    const uint32_t c = q->qh.cand_count[u];
+    if (!c) return PSM_OK;
+    psm_ctx* ctx = q->ctx;
+    HIP_TRY(ctx, hipMemcpyAsync(out, q->batch.cand_pool.as<uint32_t>() + q->qh.cand_begin[u], (size_t)c * 4,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return PSM_OK;
+}
+
+int psm_build_tables(psm_query* q) {
+    if (!q) return PSM_EINVAL;
+    if (!q->filtered) return set_err(q->ctx, PSM_ESTATE, "filter before building tables");
+    hipSetDevice(q->ctx->device);
+    return q->batch.build_tables();
+}
+
+int psm_query_table(const psm_query* q, uint32_t a, uint32_t b, uint32_t* off, uint32_t* idx,
+                    uint64_t* nnz) {
+    if (!q || a >= q->qh.qg.n || b >= q->qh.qg.n) return PSM_EINVAL;
+    if (!q->batch.tables_built) return PSM_ESTATE;
+    const uint32_t n = q->qh.qg.n;
+    if (q->qh.tab_row[(size_t)a * n + b] == kNoTable) return PSM_EINVAL;
+    if (nnz) *nnz = q->qh.tab_nnz[(size_t)a * n + b];
+    if (!off && !idx) return PSM_OK;
+    std::vector<uint32_t> o, ix;
+    PSM_TRY(const_cast<Batch&>(q->batch).host_table(q->qh, a, b, o, ix));
+    if (off) std::copy(o.begin(), o.end(), off);
+    if (idx) std::copy(ix.begin(), ix.end(), idx);
+    return PSM_OK;
+}
+
+int psm_memory_cost_bytes(const psm_query* q, uint64_t* bytes) {
+    if (!q || !bytes) return PSM_EINVAL;
+    if (!q->batch.tables_built) return PSM_ESTATE;
+    const QGraph& qg = q->qh.qg;
+    const uint32_t n = qg.n;
+    uint64_t w = 0;
+    for (uint32_t u = 0; u < n; ++u) w += q->qh.cand_count[u];
+    for (uint32_t a = 0; a < n; ++a)
+        for (uint32_t k = qg.off[a]; k < qg.off[a + 1]; ++k) {
+            const uint32_t b = qg.adj[k];
+            if (a < b)
+                w += q->qh.tab_nnz[(size_t)a * n + b] + q->qh.cand_count[a] +
+                     q->qh.tab_nnz[(size_t)b * n + a] + q->qh.cand_count[b];
+        }
+    *bytes = w * 4;
+    return PSM_OK;
+}
+
+int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t* pivot) {
+    if (!q) return PSM_EINVAL;
+    hipSetDevice(q->ctx->device);
+    PSM_TRY(q->batch.make_plans(order_method));
+    if (order) std::copy(q->qh.plan.order.begin(), q->qh.plan.order.end(), order);
+    if (pivot) std::copy(q->qh.plan.pivot.begin(), q->qh.plan.pivot.end(), pivot);
+    return PSM_OK;
+}
+
+int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
+    if (!q) return PSM_EINVAL;
+    hipSetDevice(q->ctx->device);
+    q->batch.t_enum = q->batch.t_expand = 0;
+    PSM_TRY(q->batch.evaluate(cap));
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->count = q->qh.count;
+        st->calls = q->qh.calls;
+        st->capped = q->qh.capped;
+        st->tasks = (uint32_t)q->batch.tasks_run;
+        st->kernel_ms = q->batch.t_enum;
+        st->expand_ms = q->batch.t_expand;
+        st->probe_bytes = q->qh.probe_bytes;
+    }
+    return PSM_OK;
+}
+
+int psm_match(psm_query* q, int filter, int order, uint64_t cap, psm_enum_stats* st) {
+    int r = psm_filter(q, filter);
+    if (r != PSM_OK && r != PSM_EEMPTY) return r;
+    PSM_TRY(psm_build_tables(q));
+    PSM_TRY(psm_generate_plan(q, order, nullptr, nullptr));
+    return psm_evaluate(q, cap, st);
+}
+
+int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32_t* n,
+                    const uint64_t* off_base, const uint64_t* nbr_base, const uint64_t* lab_base,
+                    const uint32_t* q_off, const uint32_t* q_adj, const uint32_t* q_lab, int filter,
+                    int order, uint64_t cap, uint64_t* counts, uint64_t* calls, uint32_t* status,
+                    psm_batch_stats* st) {
+    if (!ctx || !g || !n || !off_base || !nbr_base || !lab_base || !q_off || !q_lab) return PSM_EINVAL;
+    hipSetDevice(ctx->device);
+    const double t0 = now_ms();
+    std::vector<std::unique_ptr<QueryHost>> hosts(nq);
+    Batch b;
+    b.ctx = ctx;
+    b.g = g;
+    for (uint32_t i = 0; i < nq; ++i) {
+        hosts[i].reset(new QueryHost());
+        int r = make_qgraph(n[i], q_off + off_base[i], q_adj + nbr_base[i], q_lab + lab_base[i],
+                            hosts[i]->qg);
+        if (r != PSM_OK) return set_err(ctx, r, "malformed query graph " + std::to_string(i));
+        b.qs.push_back(hosts[i].get());
+    }
+    PSM_TRY(b.filter(filter));
+    PSM_TRY(b.build_tables());
+    PSM_TRY(b.make_plans(order));
+    PSM_TRY(b.evaluate(cap));
+    for (uint32_t i = 0; i < nq; ++i) {
+        if (counts) counts[i] = hosts[i]->count;
+        if (calls) calls[i] = hosts[i]->calls;
+        if (status) status[i] = hosts[i]->status;
+    }
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->filter_ms = b.t_filter;
+        st->table_ms = b.t_table;
+        st->plan_ms = b.t_plan;
+        st->enum_ms = b.t_enum + b.t_expand;
+        st->total_ms = now_ms() - t0;
+        st->tasks = b.tasks_run;
+        for (auto& h : hosts) st->probe_bytes += h->probe_bytes;
+    }
+    return PSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+"""Python driver over the C ABI: device context, resident data graph, the
+plain per-query pipeline and the batched form.
+
+Method strings follow the reference's ``-BackMethods`` (``FILTER_ORDER_ENGINE``,
+e.g. ``GQL_GQL_GQL`` or ``CFL_CFL_EXPLORE``, matching/StudyPerformance.cpp:
+1018-1030). Engines GQL, EXPLORE and LFTJ count the same embeddings; the
+device path computes every depth's extensions as the intersection of the
+candidate-edge table rows of all earlier neighbours.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .graph_io import CSRGraph
+
+
+def _check(ctx_handle, code: int, what: str, ok=(L.PSM_OK,)):
+    if code not in ok:
+        msg = L.lib().psm_last_error(ctx_handle).decode() if ctx_handle else ""
+        raise L.PSMError(code, f"{what}: {msg}")
+    return code
+
+
+def parse_method(method: str):
+    parts = method.split("_")
+    if len(parts) != 3:
+        raise ValueError(f"method must be FILTER_ORDER_ENGINE, got {method!r}")
+    f, o, e = parts
+    if f not in L.FILTERS or o not in L.ORDERS or e not in ("GQL", "EXPLORE", "LFTJ"):
+        raise ValueError(f"unsupported method {method!r}")
+    return L.FILTERS[f], L.ORDERS[o], e
+
+
+class Device:
+    """One HIP stream on one GPU (psm_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(None, L.lib().psm_ctx_create(device, ctypes.byref(h)), "psm_ctx_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().psm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class DeviceGraph:
+    """A data graph resident in HBM (psm_graph)."""
+
+    def __init__(self, dev: Device, g: CSRGraph):
+        self.dev = dev
+        self.host = g
+        off = np.ascontiguousarray(g.offsets, np.uint32)
+        adj = np.ascontiguousarray(g.neighbors, np.uint32)
+        lab = np.ascontiguousarray(g.labels, np.uint32)
+        if adj.size == 0:
+            adj = np.zeros(1, np.uint32)
+        h = ctypes.c_void_p()
+        _check(dev.h, L.lib().psm_graph_upload(dev.h, lab.shape[0], L.u32p(off), L.u32p(adj),
+                                               L.u32p(lab), ctypes.byref(h)), "psm_graph_upload")
+        self.h = h
+
+    def info(self):
+        a = np.zeros(5, np.uint64)
+        L.lib().psm_graph_info(self.h, L.u64p(a))
+        return dict(V=int(a[0]), E=int(a[1]), labels=int(a[2]), max_degree=int(a[3]),
+                    max_label_frequency=int(a[4]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().psm_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Query:
+    """One query against one resident data graph, stage by stage (psm_query)."""
+
+    def __init__(self, dg: DeviceGraph, q: CSRGraph):
+        self.dg = dg
+        self.n = q.num_vertices
+        off = np.ascontiguousarray(q.offsets, np.uint32)
+        adj = np.ascontiguousarray(q.neighbors, np.uint32)
+        lab = np.ascontiguousarray(q.labels, np.uint32)
+        if adj.size == 0:
+            adj = np.zeros(1, np.uint32)
+        h = ctypes.c_void_p()
+        _check(dg.dev.h, L.lib().psm_query_create(dg.dev.h, dg.h, self.n, L.u32p(off), L.u32p(adj),
+                                                  L.u32p(lab), ctypes.byref(h)), "psm_query_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().psm_query_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def filter(self, method: str) -> bool:
+        """FilterVertices::<method>Filter; False where the reference returns false."""
+        r = _check(self.dg.dev.h, L.lib().psm_filter(self.h, L.FILTERS[method]), "psm_filter",
+                   ok=(L.PSM_OK, L.PSM_EEMPTY))
+        return r == L.PSM_OK
+
+    def candidate_counts(self) -> np.ndarray:
+        c = np.zeros(self.n, np.uint32)
+        _check(self.dg.dev.h, L.lib().psm_query_candidate_counts(self.h, L.u32p(c)), "counts")
+        return c
+
+    def candidates(self, u: int) -> np.ndarray:
+        c = self.candidate_counts()
+        out = np.zeros(max(1, int(c[u])), np.uint32)
+        _check(self.dg.dev.h, L.lib().psm_query_candidates(self.h, u, L.u32p(out)), "candidates")
+        return out[: int(c[u])]
+
+    def build_tables(self):
+        _check(self.dg.dev.h, L.lib().psm_build_tables(self.h), "psm_build_tables")
+
+    def table(self, a: int, b: int):
+        nnz = ctypes.c_uint64()
+        r = L.lib().psm_query_table(self.h, a, b, None, None, ctypes.byref(nnz))
+        if r == L.PSM_EINVAL:
+            return None
+        _check(self.dg.dev.h, r, "psm_query_table")
+        c = self.candidate_counts()
+        off = np.zeros(int(c[a]) + 1, np.uint32)
+        idx = np.zeros(max(1, nnz.value), np.uint32)
+        _check(self.dg.dev.h, L.lib().psm_query_table(self.h, a, b, L.u32p(off), L.u32p(idx),
+                                                      ctypes.byref(nnz)), "psm_query_table")
+        return off, idx[: nnz.value]
+
+    def memory_cost_bytes(self) -> int:
+        b = ctypes.c_uint64()
+        _check(self.dg.dev.h, L.lib().psm_memory_cost_bytes(self.h, ctypes.byref(b)), "memcost")
+        return b.value
+
+    def generate_plan(self, order: str):
+        o = np.zeros(self.n, np.uint32)
+        p = np.zeros(self.n, np.uint32)
+        _check(self.dg.dev.h, L.lib().psm_generate_plan(self.h, L.ORDERS[order], L.u32p(o), L.u32p(p)),
+               "psm_generate_plan")
+        return o, p
+
+    def evaluate(self, cap: int = 0) -> L.EnumStats:
+        st = L.EnumStats()
+        _check(self.dg.dev.h, L.lib().psm_evaluate(self.h, cap, ctypes.byref(st)), "psm_evaluate")
+        return st
+
+
+@dataclass
+class MatchResult:
+    count: int
+    calls: int
+    mem_words: int
+    filter_ok: bool
+    order: list
+    capped: bool
+    kernel_ms: float
+    probe_bytes: int
+    cand_counts: list = field(default_factory=list)
+
+
+def match(dg: DeviceGraph, q: CSRGraph, method: str = "GQL_GQL_GQL", cap: int = 0) -> MatchResult:
+    """The plain path of split_C_test at one query: filter, sort, BuildTable,
+    plan, enumerate (StudyPerformance.cpp:1164-1716)."""
+    f, o, _ = parse_method(method)
+    fname = method.split("_")[0]
+    oname = method.split("_")[1]
+    run = Query(dg, q)
+    try:
+        ok = run.filter(fname)
+        run.build_tables()
+        mem = run.memory_cost_bytes() // 4
+        order, _ = run.generate_plan(oname)
+        st = run.evaluate(cap)
+        return MatchResult(int(st.count), int(st.calls), int(mem), ok, [int(x) for x in order],
+                           bool(st.capped), float(st.kernel_ms + st.expand_ms), int(st.probe_bytes),
+                           [int(x) for x in run.candidate_counts()])
+    finally:
+        run.close()
+
+
+def pack_queries(queries):
+    """Concatenated CSR arrays for psm_batch_match."""
+    n = np.array([q.num_vertices for q in queries], np.uint32)
+    off_base = np.zeros(len(queries), np.uint64)
+    nbr_base = np.zeros(len(queries), np.uint64)
+    lab_base = np.zeros(len(queries), np.uint64)
+    offs, nbrs, labs = [], [], []
+    o = nb = lb = 0
+    for i, q in enumerate(queries):
+        off_base[i], nbr_base[i], lab_base[i] = o, nb, lb
+        offs.append(np.asarray(q.offsets, np.uint32))
+        nbrs.append(np.asarray(q.neighbors, np.uint32))
+        labs.append(np.asarray(q.labels, np.uint32))
+        o += q.num_vertices + 1
+        nb += q.neighbors.shape[0]
+        lb += q.num_vertices
+    cat = lambda xs: np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros(1, np.uint32), np.uint32)
+    adj = cat(nbrs)
+    if adj.size == 0:
+        adj = np.zeros(1, np.uint32)
+    return n, off_base, nbr_base, lab_base, cat(offs), adj, cat(labs)
+
+
+def batch_match(dg: DeviceGraph, queries, filter: str = "GQL", order: str = "GQL", cap: int = 0):
+    """All queries through the batched stages and one persistent enumeration
+    kernel. Returns (counts, calls, status, stats)."""
+    n, ob, nb, lb, off, adj, lab = pack_queries(queries)
+    m = len(queries)
+    counts = np.zeros(max(1, m), np.uint64)
+    calls = np.zeros(max(1, m), np.uint64)
+    status = np.zeros(max(1, m), np.uint32)
+    st = L.BatchStats()
+    _check(dg.dev.h, L.lib().psm_batch_match(dg.dev.h, dg.h, m, L.u32p(n), L.u64p(ob), L.u64p(nb),
+                                             L.u64p(lb), L.u32p(off), L.u32p(adj), L.u32p(lab),
+                                             L.FILTERS[filter], L.ORDERS[order], cap, L.u64p(counts),
+                                             L.u64p(calls), L.u32p(status), ctypes.byref(st)),
+           "psm_batch_match")
+    return counts[:m], calls[:m], status[:m], st

@@ -1,0 +1,62 @@
+"""Writes tests/golden/c1_ref.json: embedding counts, call counts and the
+BuildTable memory checksum of the reference binary (built from the
+reference's own sources by oracle/Makefile.ref) on the seeded C1-shaped
+workload: Chung-Lu 9,460 V / 34,998 E, gamma 2.5, 307 Zipf(1.1) labels
+(seed 1), random-walk queries of 4, 8 and 12 vertices, sparse and dense.
+
+Queries whose count exceeds 2e6 are skipped (they take the reference minutes).
+    python tests/golden/make_ref_golden.py
+"""
+import json
+import os
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from psm_benchmark_amd import gen  # noqa: E402
+from psm_benchmark_amd.graph_io import save_text  # noqa: E402
+from oracle_binding import OGraph, ORun  # noqa: E402
+import ref_runner as R  # noqa: E402
+
+METHODS = "GQL_GQL_GQL,CFL_CFL_EXPLORE,GQL_GQL_LFTJ"
+GRAPH = dict(V=9460, E=34998, labels=307, zipf=1.1, seed=1)
+
+
+def workload(n_seeds=10):
+    g = gen.chung_lu_graph(GRAPH["V"], GRAPH["E"], GRAPH["labels"], GRAPH["zipf"], seed=GRAPH["seed"])
+    qs = []
+    for seed in range(n_seeds):
+        for dense in (0, 1):
+            for n in (4, 8, 12):
+                qs.append((seed * 7 + n, dense, n))
+    return g, qs
+
+
+def main():
+    assert R.available(), "build oracle/_ref first: make -f oracle/Makefile.ref"
+    g, qs = workload()
+    d = OGraph(g)
+    td = R.deep_dir(tempfile.mkdtemp())
+    dp = os.path.join(td, "data.txt")
+    save_text(g, dp)
+    out = {"graph": GRAPH, "records": []}
+    for qseed, dense, n in qs:
+        q = gen.query(g, n, bool(dense), qseed)
+        if ORun(d, OGraph(q), "GQL_GQL_GQL", cap=2_000_000).count >= 2_000_000:
+            continue
+        qp = os.path.join(td, "q.txt")
+        save_text(q, qp)
+        for b in R.run(dp, qp, METHODS, timeout=120):
+            out["records"].append(dict(qseed=qseed, dense=dense, n=n, method=b["method"],
+                                       embeddings=b["embeddings"], calls=b["calls"],
+                                       mem_words=b["mem_words"]))
+    with open(os.path.join(HERE, "c1_ref.json"), "w") as f:
+        json.dump(out, f, indent=0)
+    print(len(out["records"]), "records")
+
+
+if __name__ == "__main__":
+    main()

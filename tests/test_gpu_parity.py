@@ -1,0 +1,138 @@
+"""Parity of the HIP path (libpsm.so, through its C ABI) with the reference's
+golden vectors and the CPU oracle. Bit-exact: counts, call counts, memory
+checksums, candidate sets and candidate-edge tables."""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from brute import count_embeddings, random_graph, random_connected_query
+from oracle_binding import OGraph, ORun
+from psm_benchmark_amd.graph_io import csr_from_edges, load_text
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HPRD = os.path.join(HERE, "golden", "hprd")
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from psm_benchmark_amd.matcher import Device
+    d = Device(0)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def hprd(dev):
+    from psm_benchmark_amd.matcher import DeviceGraph
+    g = load_text(os.path.join(HPRD, "HPRD-15.txt"))
+    return g, DeviceGraph(dev, g), OGraph(g)
+
+
+def golden(methods):
+    recs = json.load(open(os.path.join(HPRD, "golden.json")))
+    return [r for r in recs if r["threads"] == 1 and f'{r["filter"]}_{r["order"]}_{r["engine"]}' in methods]
+
+
+@pytest.mark.parametrize("rec", golden(("GQL_GQL_GQL", "CFL_CFL_EXPLORE")),
+                         ids=lambda r: f'{r["query"]}-{r["filter"]}')
+def test_hprd_golden_on_device(hprd, rec):
+    from psm_benchmark_amd.matcher import match
+    _, dg, _ = hprd
+    q = load_text(os.path.join(HPRD, rec["query"] + ".txt"))
+    r = match(dg, q, f'{rec["filter"]}_{rec["order"]}_{rec["engine"]}')
+    print(rec["query"], rec["filter"], r.count, r.calls, r.mem_words, flush=True)
+    assert (r.count, r.calls, r.mem_words) == (rec["embeddings"], rec["calls"], rec["mem_words"])
+
+
+@pytest.mark.parametrize("method", ["LDF", "NLF", "GQL", "CFL"])
+@pytest.mark.parametrize("qname", ["Q10-1", "Q10-3", "Q10-8"])
+def test_candidates_and_tables_match_oracle(hprd, method, qname):
+    from psm_benchmark_amd.matcher import Query
+    _, dg, og = hprd
+    q = load_text(os.path.join(HPRD, qname + ".txt"))
+    ref = ORun(og, OGraph(q), f"{method}_GQL_GQL")
+    run = Query(dg, q)
+    run.filter(method)
+    assert run.candidate_counts().tolist() == ref.cand_counts
+    for u in range(q.num_vertices):
+        assert np.array_equal(run.candidates(u), ref.candidates(u)), u
+    run.build_tables()
+    for a, b in itertools.product(range(q.num_vertices), repeat=2):
+        t, rt = run.table(a, b), ref.table(a, b)
+        assert (t is None) == (rt is None)
+        if t is not None:
+            assert np.array_equal(t[0], rt[0]) and np.array_equal(t[1], rt[1]), (a, b)
+    assert run.memory_cost_bytes() // 4 == ref.mem_words
+    o, _ = run.generate_plan("GQL")
+    assert o.tolist() == ref.order
+
+
+def test_c1_reference_vectors_on_device(dev):
+    from psm_benchmark_amd import gen
+    from psm_benchmark_amd.matcher import DeviceGraph, match
+    data = json.load(open(os.path.join(HERE, "golden", "c1_ref.json")))
+    gp = data["graph"]
+    g = gen.chung_lu_graph(gp["V"], gp["E"], gp["labels"], gp["zipf"], seed=gp["seed"])
+    dg = DeviceGraph(dev, g)
+    for rec in data["records"]:
+        q = gen.query(g, rec["n"], bool(rec["dense"]), rec["qseed"])
+        r = match(dg, q, rec["method"])
+        assert r.count == rec["embeddings"], rec
+        assert r.mem_words == rec["mem_words"], rec
+        if rec["method"] != "GQL_GQL_LFTJ":   # LFTJ also counts visited rejects
+            assert r.calls == rec["calls"], rec
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_bruteforce_tiny_on_device(dev, seed):
+    from psm_benchmark_amd.matcher import DeviceGraph, match
+    rng = np.random.default_rng(100 + seed)
+    data = random_graph(rng, 11, 0.45, 2)
+    q = random_connected_query(rng, data, int(rng.integers(2, 5)))
+    dg = DeviceGraph(dev, data)
+    expect = count_embeddings(data, q)
+    for m in ("LDF_GQL_GQL", "NLF_GQL_GQL", "GQL_GQL_GQL", "CFL_CFL_EXPLORE"):
+        assert match(dg, q, m).count == expect, m
+
+
+def test_clique_closed_form_on_device(dev):
+    from psm_benchmark_amd.matcher import DeviceGraph, match
+    n, k = 9, 5
+    kn = csr_from_edges(n, np.array(list(itertools.combinations(range(n), 2))), np.zeros(n, np.uint32))
+    kk = csr_from_edges(k, np.array(list(itertools.combinations(range(k), 2))), np.zeros(k, np.uint32))
+    dg = DeviceGraph(dev, kn)
+    r = match(dg, kk, "GQL_GQL_GQL")
+    assert r.count == math.factorial(n) // math.factorial(n - k)
+    assert r.calls == sum(math.factorial(n) // math.factorial(n - d - 1) for d in range(k))
+
+
+def test_cap_reports_min_cap_total(hprd):
+    from psm_benchmark_amd.matcher import match
+    _, dg, _ = hprd
+    q = load_text(os.path.join(HPRD, "Q10-8.txt"))
+    total = match(dg, q, "GQL_GQL_GQL").count
+    assert total == 994203
+    for cap in (1, 777, 100000, total, total + 1):
+        r = match(dg, q, "GQL_GQL_GQL", cap=cap)
+        assert r.count == min(cap, total)
+        assert r.capped == (cap <= total)
+
+
+def test_batch_equals_golden(hprd):
+    from psm_benchmark_amd.matcher import batch_match
+    _, dg, _ = hprd
+    recs = golden(("GQL_GQL_GQL",))
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
+    counts, calls, status, st = batch_match(dg, qs, "GQL", "GQL", 0)
+    assert counts.tolist() == [r["embeddings"] for r in recs]
+    assert calls.tolist() == [r["calls"] for r in recs]
+    recs = golden(("CFL_CFL_EXPLORE",))
+    counts, calls, status, st = batch_match(dg, qs, "CFL", "CFL", 0)
+    assert counts.tolist() == [r["embeddings"] for r in recs]
+    assert calls.tolist() == [r["calls"] for r in recs]
