@@ -1,0 +1,202 @@
+"""Benchmark of the batched matching path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload C2]
+
+Workload C2 (seeded, synthetic): Chung-Lu 1.1M vertices / 3.0M edges,
+100 Zipf(1.1) labels; 512 queries of 4/8/16/24/32 vertices, half dense and
+half sparse; GQL filter + GQL order; count per query capped at 1e5 and
+reported as min(cap, total). A step is one batched pass over the 512 queries
+(filter, candidate-edge tables, plans, enumeration) with the data graph
+resident in HBM. The JSON line carries queries/s, embeddings/s, the plain
+one-query-at-a-time GPU path, p50/p99 latencies, the roofline of the
+dominant kernel and the reference CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(g, qs, meta, cap: int, budget_s: float):
+    """The reference binary (oracle/_ref, built from the reference's sources)
+    on the first queries of the workload, one thread, until about budget_s of
+    its own reported query time; falls back to the oracle restatement."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import tempfile
+
+    from psm_benchmark_amd.graph_io import save_text
+    import ref_runner as R
+    if R.available():
+        td = R.deep_dir(tempfile.mkdtemp(prefix="psm_cpu_"))
+        dp = os.path.join(td, "data.txt")
+        save_text(g, dp)
+        spent, done, emb = 0.0, 0, 0
+        t_wall = time.time()
+        for i, q in enumerate(qs):
+            qp = os.path.join(td, "q.txt")
+            save_text(q, qp)
+            b = R.run(dp, qp, "GQL_GQL_GQL", threads="1", cap=str(cap), timeout=600)[0]
+            spent += b["total_s"]
+            emb += b["embeddings"]
+            done += 1
+            if spent >= budget_s or time.time() - t_wall > 3 * budget_s:
+                break
+        return dict(value=done / spent, unit="queries/s", cores=1, kind="reference",
+                    embeddings_per_s=emb / spent,
+                    sample=f"first {done} of {len(qs)} queries, reference SubgraphMatching.out "
+                           f"GQL_GQL_GQL -threadnums 1 -num {cap}, its own Total time "
+                           f"(filter+table+plan+enumerate, graph load excluded) = {spent:.2f}s")
+    from oracle_binding import OGraph, ORun
+    d = OGraph(g)
+    spent, done = 0.0, 0
+    for q in qs:
+        t = time.time()
+        ORun(d, OGraph(q), "GQL_GQL_GQL", cap=cap)
+        spent += time.time() - t
+        done += 1
+        if spent >= budget_s:
+            break
+    return dict(value=done / spent, unit="queries/s", cores=1, kind="port",
+                sample=f"first {done} of {len(qs)} queries through the oracle restatement, one thread")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="C2")
+    ap.add_argument("--cap", type=int, default=100_000)
+    ap.add_argument("--plain", type=int, default=512, help="queries timed through the plain path")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (queries shard "
+              "independently across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from psm_benchmark_amd import workloads as W
+    from psm_benchmark_amd.matcher import Device, DeviceGraph, Query, batch_match, match
+    from psm_benchmark_amd import _lib as L
+
+    t0 = time.time()
+    g = W.data_graph(args.workload)
+    qs, meta = W.c2_queries(g)
+    log(f"workload {args.workload}: V={g.num_vertices} E={g.num_edges} queries={len(qs)} "
+        f"({time.time() - t0:.1f}s to generate)")
+    dev = Device(0)
+    dg = DeviceGraph(dev, g)
+
+    for _ in range(args.warmup):
+        counts, calls, status, st = batch_match(dg, qs, "GQL", "GQL", args.cap)
+        log(f"warmup: total {st.total_ms:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
+            f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f})")
+    L.lib().psm_kernel_stats_reset(dev.h)
+    times, stats = [], []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        counts, calls, status, st = batch_match(dg, qs, "GQL", "GQL", args.cap)
+        times.append(time.perf_counter() - t)
+        stats.append(st)
+        log(f"step: {times[-1] * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
+            f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f})")
+    ks = kernel_stats(dev)
+    total = sum(times)
+    nq = len(qs)
+    emb = int(counts.sum())
+    step_ms = total / args.steps * 1e3
+
+    # plain path: one query at a time through the same library
+    lat = []
+    plain_counts = []
+    for q in qs[: args.plain]:
+        t = time.perf_counter()
+        r = match(dg, q, "GQL_GQL_GQL", cap=args.cap)
+        lat.append(time.perf_counter() - t)
+        plain_counts.append(r.count)
+    if plain_counts != counts[: len(plain_counts)].tolist():
+        raise SystemExit("batched and plain counts differ")
+    lat_ms = np.array(lat) * 1e3
+    plain_qps = len(lat) / sum(lat)
+
+    # dominant kernel roofline (bytes counted by the kernels themselves,
+    # durations from HIP events on the library's stream)
+    dom = max(ks.values(), key=lambda k: k["ms"])
+    achieved = dom["bytes"] / dom["launches"] / (dom["ms"] / dom["launches"] / 1e3) / 1e9
+    roofline = dict(kernel=dom["name"], bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_ms=round(dom["ms"] / dom["launches"], 4),
+                    bytes_per_launch=int(dom["bytes"] / dom["launches"]))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(g, qs, meta, args.cap, args.cpu_budget)
+
+    out = {
+        "metric": "queries/s (batched, 512 queries, cap 1e5) on 1 MI355X",
+        "value": round(nq * args.steps / total, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded Chung-Lu graph and random-walk queries)",
+        "config": {"workload": "C2 batched: Chung-Lu 1.1M V / 3.0M E, 100 Zipf(1.1) labels, "
+                               "512 queries of 4-32 vertices (half dense), GQL_GQL_GQL, cap 1e5",
+                   "queries": nq, "cap": args.cap},
+        "embeddings_per_s": round(emb * args.steps / total, 1),
+        "embeddings_per_step": emb,
+        "latency_ms": {"batched_p50": round(step_ms, 2), "batched_p99": round(step_ms, 2),
+                       "plain_p50": round(float(np.percentile(lat_ms, 50)), 3),
+                       "plain_p99": round(float(np.percentile(lat_ms, 99)), 3)},
+        "plain_gpu_queries_per_s": round(plain_qps, 2),
+        "batched_over_plain": round(nq * args.steps / total / plain_qps, 2),
+        "stage_ms": {k: round(float(np.mean([getattr(s, k) for s in stats])), 2)
+                     for k in ("filter_ms", "table_ms", "plan_ms", "enum_ms")},
+        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"] / args.steps, 3),
+                            bytes=int(v["bytes"] / args.steps)) for k, v in ks.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["batched_over_cpu"] = round(out["value"] / cpu["value"], 1)
+    print(json.dumps(out))
+
+
+KERNELS = ["gql_refine", "table_count", "table_fill", "dfs", "expand", "compact", "bitmap"]
+
+
+def kernel_stats(dev):
+    import ctypes
+
+    from psm_benchmark_amd import _lib as L
+    out = {}
+    for i, name in enumerate(KERNELS):
+        a = (ctypes.c_double * 3)()
+        L.lib().psm_kernel_stats(dev.h, i, a)
+        if a[0] > 0:
+            out[name] = dict(name=name, launches=int(a[0]), ms=float(a[1]), bytes=float(a[2]))
+    return out
+
+
+if __name__ == "__main__":
+    main()

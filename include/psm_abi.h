@@ -70,6 +70,15 @@ int psm_ctx_create(int device, psm_ctx** out);
 int psm_ctx_destroy(psm_ctx* ctx);
 const char* psm_last_error(const psm_ctx* ctx);
 
+/* Per-kernel-class statistics of a context, accumulated since the last
+ * reset: out3 = {launches, total milliseconds (HIP events on the context's
+ * stream), algorithmic bytes counted by the kernels}. Classes: 0 GQL
+ * refinement, 1 table count, 2 table fill, 3 enumeration DFS, 4 task
+ * pre-expansion, 5 compaction, 6 bitmaps. Instrumentation of this build; the
+ * reference logs phase times only (Args.cpp:517-535). */
+int psm_kernel_stats(psm_ctx* ctx, int kind, double* out3);
+int psm_kernel_stats_reset(psm_ctx* ctx);
+
 /* ---- data graph: class Graph (graph/graph.h:21-237) ------------------
  * Uploads a CSR data graph (adjacency sorted ascending per vertex) and builds
  * on the device side the label reverse index (graph.cpp:15-33) and the

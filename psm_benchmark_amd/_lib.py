@@ -57,6 +57,8 @@ SIGNATURES = {
     "psm_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
     "psm_ctx_destroy": (ctypes.c_int, [VP]),
     "psm_last_error": (ctypes.c_char_p, [VP]),
+    "psm_kernel_stats": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "psm_kernel_stats_reset": (ctypes.c_int, [VP]),
     "psm_graph_upload": (ctypes.c_int, [VP, ctypes.c_uint32, U32P, U32P, U32P, ctypes.POINTER(VP)]),
     "psm_graph_destroy": (ctypes.c_int, [VP]),
     "psm_graph_info": (ctypes.c_int, [VP, U64P]),

@@ -27,6 +27,15 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << lane_id()) - 1ull;
 }
 
+// Sum of v over the wavefront, added once to *p (all lanes must call).
+__device__ __forceinline__ void wave_add(unsigned long long* p, unsigned long long v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, o), hi = __shfl_xor((uint32_t)(v >> 32), o);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    if (lane_id() == 0 && v) atomicAdd(p, v);
+}
+
 // Largest i in [0, n) with starts[i] <= key (starts ascending, starts[0] == 0).
 template <typename T, typename K>
 __device__ __forceinline__ uint32_t find_item(const T* items, uint32_t n, K key, K T::*field) {
@@ -191,7 +200,7 @@ __device__ __forceinline__ uint32_t left_mask(const uint32_t* const* lb, uint32_
 __global__ __launch_bounds__(kBlock) void k_gql_refine(
     DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_waves,
     const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
-    uint32_t* __restrict__ bitmaps, uint64_t wps) {
+    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ bytes) {
     __shared__ GqlShared sh;
     const uint32_t wl = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
@@ -225,7 +234,8 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
         if (cnt[i] == 0) ok = false;
         if (cnt[i] < k) T |= 1u << i;
     }
-    if (ok && T) {
+    const bool pass2 = ok && T;
+    if (pass2) {
         // pass 2: right vertices adjacent to T, compacted into LDS
         uint32_t nr = 0;
         for (uint32_t base = vb; base < ve; base += kWave) {
@@ -283,6 +293,12 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
         cand_pool[it.cand_begin + j] = kInvalid;
         atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
     }
+    // algorithmic bytes: candidate, offsets, N(v), one bitmap word per
+    // (neighbour, left vertex) probe; pass 2 repeats the scan when it runs
+    if (lane == 0) {
+        const unsigned long long scan = 4ull * (ve - vb) * (1 + k);
+        atomicAdd(bytes, 12ull + scan * (pass2 ? 2 : 1));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -299,9 +315,12 @@ __global__ __launch_bounds__(kBlock) void k_table_count(
     DevGraph g, const TItem* __restrict__ items, uint32_t n_items,
     const uint64_t* __restrict__ item_row_start, uint64_t n_rows,
     const uint32_t* __restrict__ cand_pool, const uint32_t* __restrict__ bitmaps, uint64_t wps,
-    uint64_t* __restrict__ row_deg) {
-    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n_rows;
-         r += (uint64_t)gridDim.x * kBlock) {
+    uint64_t* __restrict__ row_deg, unsigned long long* __restrict__ bytes) {
+    unsigned long long by = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const uint64_t rows_rounded = (n_rows + stride - 1) / stride * stride;
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < rows_rounded; r += stride) {
+        if (r >= n_rows) continue;
         const uint32_t ii = find_u64(item_row_start, n_items, r);
         const TItem it = items[ii];
         const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
@@ -312,7 +331,10 @@ __global__ __launch_bounds__(kBlock) void k_table_count(
             c += (bm[w >> 5] >> (w & 31)) & 1u;
         }
         row_deg[r] = c;
+        // candidate + offsets + N(v) + one bitmap word per neighbour + the count
+        by += 4 + 8 + 8ull * (g.off[v + 1] - g.off[v]) + 8;
     }
+    wave_add(bytes, by);
 }
 
 __global__ __launch_bounds__(kBlock) void k_table_fill(
@@ -320,20 +342,28 @@ __global__ __launch_bounds__(kBlock) void k_table_fill(
     const uint64_t* __restrict__ item_row_start, uint64_t n_rows,
     const uint32_t* __restrict__ cand_pool, const uint32_t* __restrict__ bitmaps,
     const uint32_t* __restrict__ word_prefix, uint64_t wps, const uint64_t* __restrict__ row_off,
-    uint32_t* __restrict__ idx) {
-    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n_rows;
-         r += (uint64_t)gridDim.x * kBlock) {
+    uint32_t* __restrict__ idx, unsigned long long* __restrict__ bytes) {
+    unsigned long long by = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const uint64_t rows_rounded = (n_rows + stride - 1) / stride * stride;
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < rows_rounded; r += stride) {
+        if (r >= n_rows) continue;
         const uint32_t ii = find_u64(item_row_start, n_items, r);
         const TItem it = items[ii];
         const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
         const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
         const uint32_t* wp = word_prefix + (size_t)it.bm_b * wps;
-        uint64_t o = row_off[r];
+        const uint64_t o0 = row_off[r];
+        uint64_t o = o0;
         for (uint32_t p = g.off[v]; p < g.off[v + 1]; ++p) {
             const uint32_t w = g.adj[p];
             if ((bm[w >> 5] >> (w & 31)) & 1u) idx[o++] = bit_rank(bm, wp, w);
         }
+        // candidate + offsets + row offset + N(v) + bitmap word per neighbour
+        // + per entry: rank words (2) and the entry written
+        by += 4 + 8 + 8 + 8ull * (g.off[v + 1] - g.off[v]) + 12ull * (o - o0);
     }
+    wave_add(bytes, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,7 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         // each child is one assignment at this depth; counted once the
         // expansion is committed
         atomicAdd(&qc[task.q].calls, (unsigned long long)c);
-        atomicAdd(&qc[task.q].probe_bytes, bytes);
+        atomicAdd(&qc[task.q].expand_bytes, bytes);
     }
 }
 
@@ -848,11 +878,12 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
 
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
                              uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
-                             uint32_t* bitmaps, uint64_t words_per_slot, hipStream_t s) {
+                             uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* bytes,
+                             hipStream_t s) {
     if (n_waves == 0) return hipSuccess;
     const uint32_t blocks = (n_waves + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock;
     hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_waves,
-                       left_slots, cand_pool, bitmaps, words_per_slot);
+                       left_slots, cand_pool, bitmaps, words_per_slot, bytes);
     return hipGetLastError();
 }
 
@@ -865,10 +896,11 @@ static uint32_t grid_for(uint64_t n) {
 hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,
                               const uint64_t* item_row_start, uint64_t n_rows,
                               const uint32_t* cand_pool, const uint32_t* bitmaps,
-                              uint64_t words_per_slot, uint64_t* row_deg, hipStream_t s) {
+                              uint64_t words_per_slot, uint64_t* row_deg, unsigned long long* bytes,
+                              hipStream_t s) {
     if (n_rows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_table_count, dim3(grid_for(n_rows)), dim3(kBlock), 0, s, g, items, n_items,
-                       item_row_start, n_rows, cand_pool, bitmaps, words_per_slot, row_deg);
+                       item_row_start, n_rows, cand_pool, bitmaps, words_per_slot, row_deg, bytes);
     return hipGetLastError();
 }
 
@@ -876,11 +908,12 @@ hipError_t launch_table_fill(const DevGraph& g, const TItem* items, uint32_t n_i
                              const uint64_t* item_row_start, uint64_t n_rows,
                              const uint32_t* cand_pool, const uint32_t* bitmaps,
                              const uint32_t* word_prefix, uint64_t words_per_slot,
-                             const uint64_t* row_off, uint32_t* idx, hipStream_t s) {
+                             const uint64_t* row_off, uint32_t* idx, unsigned long long* bytes,
+                             hipStream_t s) {
     if (n_rows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_table_fill, dim3(grid_for(n_rows)), dim3(kBlock), 0, s, g, items, n_items,
                        item_row_start, n_rows, cand_pool, bitmaps, word_prefix, words_per_slot,
-                       row_off, idx);
+                       row_off, idx, bytes);
     return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -201,27 +202,31 @@ int64_t psm_gen_query(uint32_t V, const uint32_t* offsets, const uint32_t* nbrs,
                 if (r.below(8) == 0) cur = (uint32_t)r.below(verts.size());
             }
         } else {
-            while (verts.size() < n) {
-                // frontier scores: edges into the current set
-                std::vector<std::pair<uint32_t, uint32_t>> cand;  // (vertex, score)
-                for (uint32_t v : verts) {
-                    uint32_t d = offsets[v + 1] - offsets[v];
-                    // bound the scan on hubs: sample up to 256 neighbours
-                    uint32_t lim = std::min<uint32_t>(d, 256);
-                    for (uint32_t k = 0; k < lim; ++k) {
-                        uint32_t w = nbrs[offsets[v] + (d > 256 ? r.below(d) : k)];
-                        if (std::find(verts.begin(), verts.end(), w) != verts.end()) continue;
-                        bool seen = false;
-                        for (auto& c : cand) if (c.first == w) { c.second++; seen = true; break; }
-                        if (!seen) cand.push_back({w, 1});
-                    }
+            // start at a degree-weighted vertex (endpoint of a random edge):
+            // dense neighbourhoods sit around the hubs
+            const uint64_t nnz = offsets[V];
+            const uint64_t e = r.below(nnz);
+            verts[0] = (uint32_t)(std::upper_bound(offsets, offsets + V + 1, (uint32_t)e) - offsets - 1);
+            std::unordered_map<uint32_t, uint32_t> score;   // frontier vertex -> edges into the set
+            auto absorb = [&](uint32_t v) {
+                score.erase(v);
+                const uint32_t d = offsets[v + 1] - offsets[v];
+                const uint32_t lim = std::min<uint32_t>(d, 4096);
+                for (uint32_t k = 0; k < lim; ++k) {
+                    const uint32_t w = nbrs[offsets[v] + (d > 4096 ? r.below(d) : k)];
+                    if (std::find(verts.begin(), verts.end(), w) != verts.end()) continue;
+                    score[w]++;
                 }
-                if (cand.empty()) break;
-                uint32_t best = 0;
-                for (auto& c : cand) best = std::max(best, c.second);
-                std::vector<uint32_t> top;
-                for (auto& c : cand) if (c.second == best) top.push_back(c.first);
-                verts.push_back(top[r.below(top.size())]);
+            };
+            absorb(verts[0]);
+            while (verts.size() < n && !score.empty()) {
+                uint32_t best = 0, pick = 0, ties = 0;
+                for (auto& kv : score) {
+                    if (kv.second > best) { best = kv.second; pick = kv.first; ties = 1; }
+                    else if (kv.second == best && r.below(++ties) == 0) pick = kv.first;
+                }
+                verts.push_back(pick);
+                absorb(pick);
             }
         }
         if (verts.size() < n) continue;

@@ -109,9 +109,22 @@ struct DTask {
 struct QCounters {
     unsigned long long count;
     unsigned long long calls;
-    unsigned long long probe_bytes;
+    unsigned long long probe_bytes;    // algorithmic bytes of the DFS kernel
+    unsigned long long expand_bytes;   // algorithmic bytes of task pre-expansion
     uint32_t done;        // cap reached
     uint32_t pad;
+};
+
+// Kernel classes timed by the library (psm_kernel_stats).
+enum KernelKind : int {
+    kKGqlRefine = 0,
+    kKTableCount = 1,
+    kKTableFill = 2,
+    kKDfs = 3,
+    kKExpand = 4,
+    kKCompact = 5,
+    kKBitmap = 6,
+    kKNumKinds = 7,
 };
 
 // ---- kernel launchers (device.hip) --------------------------------------
@@ -129,16 +142,19 @@ hipError_t launch_set_bits(const uint32_t* pool, const uint64_t* seg_begin, cons
 hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_words, hipStream_t s);
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
                              uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
-                             uint32_t* bitmaps, uint64_t words_per_slot, hipStream_t s);
+                             uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* bytes,
+                             hipStream_t s);
 hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,
                               const uint64_t* item_row_start, uint64_t n_rows,
                               const uint32_t* cand_pool, const uint32_t* bitmaps,
-                              uint64_t words_per_slot, uint64_t* row_deg, hipStream_t s);
+                              uint64_t words_per_slot, uint64_t* row_deg, unsigned long long* bytes,
+                              hipStream_t s);
 hipError_t launch_table_fill(const DevGraph& g, const TItem* items, uint32_t n_items,
                              const uint64_t* item_row_start, uint64_t n_rows,
                              const uint32_t* cand_pool, const uint32_t* bitmaps,
                              const uint32_t* word_prefix, uint64_t words_per_slot,
-                             const uint64_t* row_off, uint32_t* idx, hipStream_t s);
+                             const uint64_t* row_off, uint32_t* idx, unsigned long long* bytes,
+                             hipStream_t s);
 hipError_t launch_cfl_support(const DevGraph& g, const uint32_t* cand_pool, uint64_t cand_begin,
                               uint32_t len, uint32_t label, uint32_t degree, uint32_t* cnt,
                               uint32_t expect, uint32_t next, hipStream_t s);

@@ -34,6 +34,13 @@ struct psm_ctx {
     hipStream_t stream = nullptr;
     int num_cu = 256;
     std::string err;
+    // per-kernel-class timing: HIP events around launches on `stream`
+    struct KStat { double launches = 0, ms = 0, bytes = 0; };
+    KStat kstat[kKNumKinds];
+    struct Pending { hipEvent_t a, b; int kind; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> ev_pool;
+    unsigned long long* d_kbytes = nullptr;   // kernel-counted algorithmic bytes per class
 };
 
 namespace {
@@ -102,6 +109,59 @@ hipError_t download(std::vector<T>& v, const void* src, size_t n, hipStream_t s)
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(s);
 }
+
+hipEvent_t take_event(psm_ctx* ctx) {
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Marks the start of a timed launch of class `kind`; returns its index.
+size_t timer_begin(psm_ctx* ctx, int kind) {
+    psm_ctx::Pending p{take_event(ctx), take_event(ctx), kind};
+    (void)hipEventRecord(p.a, ctx->stream);
+    ctx->pending.push_back(p);
+    return ctx->pending.size() - 1;
+}
+
+void timer_end(psm_ctx* ctx, size_t i) { (void)hipEventRecord(ctx->pending[i].b, ctx->stream); }
+
+// Waits for the stream and folds finished timers and byte counters into the
+// context's statistics.
+hipError_t timers_flush(psm_ctx* ctx) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return e;
+    for (auto& p : ctx->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            ctx->kstat[p.kind].ms += ms;
+            ctx->kstat[p.kind].launches += 1;
+        }
+        ctx->ev_pool.push_back(p.a);
+        ctx->ev_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+    if (ctx->d_kbytes) {
+        unsigned long long h[kKNumKinds];
+        e = hipMemcpy(h, ctx->d_kbytes, sizeof(h), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return e;
+        for (int k = 0; k < kKNumKinds; ++k) ctx->kstat[k].bytes += (double)h[k];
+        e = hipMemset(ctx->d_kbytes, 0, sizeof(h));
+    }
+    return e;
+}
+
+#define TIMED(ctx, kind, expr)                              \
+    do {                                                    \
+        const size_t _ti = timer_begin((ctx), (kind));      \
+        HIP_TRY((ctx), (expr));                             \
+        timer_end((ctx), _ti);                              \
+    } while (0)
 
 double now_ms() {
     return std::chrono::duration<double, std::milli>(
@@ -448,12 +508,13 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
             if (items.empty()) continue;
             HIP_TRY(ctx, upload(d_items, items, s()));
             HIP_TRY(ctx, upload(d_left, left, s()));
-            HIP_TRY(ctx, launch_gql_refine(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), waves,
-                                           d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
-                                           bitmaps.as<uint32_t>(), wps, s()));
+            TIMED(ctx, kKGqlRefine,
+                  launch_gql_refine(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), waves,
+                                    d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
+                                    bitmaps.as<uint32_t>(), wps, ctx->d_kbytes + kKGqlRefine, s()));
         }
     }
-    HIP_TRY(ctx, hipStreamSynchronize(s()));
+    HIP_TRY(ctx, timers_flush(ctx));
     PSM_TRY(compact_pool());
     bitmaps_current = true;   // refinement kept bitmaps == surviving candidates
     for (QueryHost* q : active)
@@ -526,18 +587,20 @@ int Batch::build_tables() {
     HIP_TRY(ctx, d_deg.alloc(n_rows * 8));
     HIP_TRY(ctx, row_off.alloc((n_rows + 1) * 8));
     HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(n_rows) * 8));
-    HIP_TRY(ctx, launch_table_count(g->dg, d_items.as<TItem>(), ni, d_rs.as<uint64_t>(), n_rows,
-                                    cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
-                                    d_deg.as<uint64_t>(), s()));
+    TIMED(ctx, kKTableCount,
+          launch_table_count(g->dg, d_items.as<TItem>(), ni, d_rs.as<uint64_t>(), n_rows,
+                             cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
+                             d_deg.as<uint64_t>(), ctx->d_kbytes + kKTableCount, s()));
     HIP_TRY(ctx, launch_scan_u64(d_deg.as<uint64_t>(), n_rows, row_off.as<uint64_t>(),
                                  d_tmp.as<unsigned long long>(), s()));
     HIP_TRY(ctx, hipMemcpyAsync(&n_idx, row_off.as<uint64_t>() + n_rows, 8, hipMemcpyDeviceToHost, s()));
     HIP_TRY(ctx, hipStreamSynchronize(s()));
     HIP_TRY(ctx, idx.alloc(n_idx * 4));
-    HIP_TRY(ctx, launch_table_fill(g->dg, d_items.as<TItem>(), ni, d_rs.as<uint64_t>(), n_rows,
-                                   cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(),
-                                   wprefix.as<uint32_t>(), wps, row_off.as<uint64_t>(),
-                                   idx.as<uint32_t>(), s()));
+    TIMED(ctx, kKTableFill,
+          launch_table_fill(g->dg, d_items.as<TItem>(), ni, d_rs.as<uint64_t>(), n_rows,
+                            cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(),
+                            wprefix.as<uint32_t>(), wps, row_off.as<uint64_t>(),
+                            idx.as<uint32_t>(), ctx->d_kbytes + kKTableFill, s()));
     std::vector<uint64_t> ends(ni), nnz;
     for (uint32_t i = 0; i < ni; ++i) ends[i] = row_start[i] + items[i].len_a;
     HIP_TRY(ctx, upload(d_ends, ends, s()));
@@ -546,6 +609,7 @@ int Batch::build_tables() {
                              d_nnz.as<uint64_t>(), s()));
     HIP_TRY(ctx, download(nnz, d_nnz.p, ni, s()));
     for (uint32_t i = 0; i < ni; ++i) item_of[i].first->tab_nnz[item_of[i].second] = nnz[i];
+    HIP_TRY(ctx, timers_flush(ctx));
     tables_built = true;
     t_table += now_ms() - t0;
     return PSM_OK;
@@ -879,7 +943,8 @@ int Batch::evaluate(uint64_t cap) {
         std::vector<unsigned long long> qchild;
         HIP_TRY(ctx, download(qchild, d_qchild.p, nq, s()));
         HIP_TRY(ctx, d_tasks2.alloc(std::max<uint64_t>(total, 1) * sizeof(DTask)));
-        HIP_TRY(ctx, launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
+        TIMED(ctx, kKExpand,
+              launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
                                    idx.as<uint32_t>(), cand_pool.as<uint32_t>(), d_tasks.as<DTask>(),
                                    (uint32_t)n_tasks, d_expand.as<uint8_t>(), nullptr,
                                    d_pos.as<uint64_t>(), d_tasks2.as<DTask>(), qcnt.as<QCounters>(),
@@ -906,20 +971,26 @@ int Batch::evaluate(uint64_t cap) {
         HIP_TRY(ctx, hipMemsetAsync(d_head.p, 0, 4, s()));
         HIP_TRY(ctx, d_state.alloc((size_t)gw * 64 * dfs_lane_state_words(max_n) * 4));
         HIP_TRY(ctx, hipEventRecord(e0, s()));
-        HIP_TRY(ctx, launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
-                                idx.as<uint32_t>(), cand_pool.as<uint32_t>(), d_tasks.as<DTask>(),
-                                (uint32_t)n_tasks, d_head.as<uint32_t>(), qcnt.as<QCounters>(), max_n,
-                                gw, d_state.as<uint32_t>(), s()));
+        TIMED(ctx, kKDfs,
+              launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
+                         idx.as<uint32_t>(), cand_pool.as<uint32_t>(), d_tasks.as<DTask>(),
+                         (uint32_t)n_tasks, d_head.as<uint32_t>(), qcnt.as<QCounters>(), max_n,
+                         gw, d_state.as<uint32_t>(), s()));
         HIP_TRY(ctx, hipEventRecord(e1, s()));
         HIP_TRY(ctx, hipEventSynchronize(e1));
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
         t_enum += ms;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     tasks_run += n_tasks;
+    HIP_TRY(ctx, timers_flush(ctx));
     std::vector<QCounters> qc;
     HIP_TRY(ctx, download(qc, qcnt.p, nq, s()));
+    for (uint32_t i = 0; i < nq; ++i) {
+        ctx->kstat[kKDfs].bytes += (double)qc[i].probe_bytes;
+        ctx->kstat[kKExpand].bytes += (double)qc[i].expand_bytes;
+    }
     for (uint32_t i = 0; i < nq; ++i) {
         QueryHost* q = qs[i];
         const uint64_t c = cap ? std::min<uint64_t>(cap, qc[i].count) : qc[i].count;
@@ -962,19 +1033,41 @@ int psm_ctx_create(int device, psm_ctx** out) {
     if (e != hipSuccess) { delete c; return -(int)e; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    e = hipMalloc(&c->d_kbytes, sizeof(unsigned long long) * kKNumKinds);
+    if (e == hipSuccess) e = hipMemset(c->d_kbytes, 0, sizeof(unsigned long long) * kKNumKinds);
+    if (e != hipSuccess) { (void)hipStreamDestroy(c->stream); delete c; return -(int)e; }
     *out = c;
     return PSM_OK;
 }
 
 int psm_ctx_destroy(psm_ctx* ctx) {
     if (!ctx) return PSM_OK;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)timers_flush(ctx);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->d_kbytes) (void)hipFree(ctx->d_kbytes);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PSM_OK;
 }
 
 const char* psm_last_error(const psm_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int psm_kernel_stats(psm_ctx* ctx, int kind, double* out3) {
+    if (!ctx || !out3 || kind < 0 || kind >= kKNumKinds) return PSM_EINVAL;
+    HIP_TRY(ctx, timers_flush(ctx));
+    out3[0] = ctx->kstat[kind].launches;
+    out3[1] = ctx->kstat[kind].ms;
+    out3[2] = ctx->kstat[kind].bytes;
+    return PSM_OK;
+}
+
+int psm_kernel_stats_reset(psm_ctx* ctx) {
+    if (!ctx) return PSM_EINVAL;
+    HIP_TRY(ctx, timers_flush(ctx));
+    for (auto& k : ctx->kstat) k = psm_ctx::KStat();
+    return PSM_OK;
+}
 
 int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32_t* adj,
                      const uint32_t* lab, psm_graph** out) {

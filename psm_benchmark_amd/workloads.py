@@ -1,0 +1,68 @@
+"""The benchmark workloads, seeded and regenerated on demand.
+
+C1  plain:    Chung-Lu 9,460 V / 34,998 E, 307 Zipf(1.1) labels; one 8-vertex
+              random-walk query; GQL filter + GQL order; uncapped.
+C2  batched:  Chung-Lu 1.1M V / 3.0M E, 100 Zipf(1.1) labels; 512 queries of
+              4/8/16/24/32 vertices, half dense (greedy induced subgraph) and
+              half sparse (walk tree); GQL filter + GQL order; cap 1e5.
+C3  join:     Chung-Lu 4M V / 35M E, 32 Zipf(1.0) labels; 64 cyclic queries of
+              12-16 vertices; CFL filter + CFL order; uncapped.
+C4  serving:  the C3 graph; Poisson arrivals; cap 1e5.
+C5  stress:   R-MAT (0.57, 0.19, 0.19) 2^24 V / 100M E, 8 uniform labels;
+              32 dense queries of 6-10 vertices; uncapped.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import gen
+from .graph_io import CSRGraph
+
+GRAPHS = {
+    "C1": dict(kind="chung_lu", V=9460, E=34998, labels=307, zipf=1.1, seed=1),
+    "C2": dict(kind="chung_lu", V=1_100_000, E=3_000_000, labels=100, zipf=1.1, seed=2),
+    "C3": dict(kind="chung_lu", V=4_000_000, E=35_000_000, labels=32, zipf=1.0, seed=3),
+    "C5": dict(kind="rmat", scale=24, E=100_000_000, labels=8, seed=5),
+}
+
+
+def data_graph(name: str) -> CSRGraph:
+    p = GRAPHS["C3" if name == "C4" else name]
+    if p["kind"] == "chung_lu":
+        return gen.chung_lu_graph(p["V"], p["E"], p["labels"], p["zipf"], seed=p["seed"])
+    return gen.rmat_graph(p["scale"], p["E"], p["labels"], seed=p["seed"])
+
+
+def _avg_degree(q: CSRGraph) -> float:
+    return 2.0 * q.num_edges / q.num_vertices
+
+
+def c2_queries(g: CSRGraph, count: int = 512, seed: int = 22):
+    """Sizes cycle through 4/8/16/24/32; even index dense, odd sparse.
+    Dense queries take the densest of a few greedy draws (a 4-vertex query
+    can reach average degree 3 at most: K4)."""
+    sizes = [4, 8, 16, 24, 32]
+    qs, meta = [], []
+    rng = np.random.default_rng(seed)
+    for i in range(count):
+        n = sizes[i % len(sizes)]
+        dense = i % 2 == 0
+        base = int(rng.integers(0, 2**62))
+        if dense:
+            best = None
+            for t in range(8):
+                q = gen.query(g, n, True, base + t)
+                if best is None or _avg_degree(q) > _avg_degree(best):
+                    best = q
+                if _avg_degree(best) > 3.0 or (n == 4 and best.num_edges == 6):
+                    break
+            q = best
+        else:
+            q = gen.query(g, n, False, base)
+        qs.append(q)
+        meta.append(dict(n=n, dense=dense, edges=q.num_edges))
+    return qs, meta
+
+
+def c1_query(g: CSRGraph, seed: int = 11) -> CSRGraph:
+    return gen.query(g, 8, True, seed)
