@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--plain", type=int, default=512, help="queries timed through the plain path")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--queries", type=int, default=512, help="use the first N workload queries")
+    ap.add_argument("--time-limit", type=float, default=120.0, help="per-batch enumeration limit, s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (queries shard "
@@ -97,10 +99,14 @@ def main():
     t0 = time.time()
     g = W.data_graph(args.workload)
     qs, meta = W.c2_queries(g)
+    qs, meta = qs[: args.queries], meta[: args.queries]
     log(f"workload {args.workload}: V={g.num_vertices} E={g.num_edges} queries={len(qs)} "
         f"({time.time() - t0:.1f}s to generate)")
     dev = Device(0)
+    L.lib().psm_set_time_limit(dev.h, args.time_limit)
+    t0 = time.time()
     dg = DeviceGraph(dev, g)
+    log(f"graph resident in HBM ({time.time() - t0:.1f}s)")
 
     for _ in range(args.warmup):
         counts, calls, status, st = batch_match(dg, qs, "GQL", "GQL", args.cap)
@@ -122,6 +128,7 @@ def main():
     step_ms = total / args.steps * 1e3
 
     # plain path: one query at a time through the same library
+    log(f"plain path over {min(args.plain, nq)} queries")
     lat = []
     plain_counts = []
     for q in qs[: args.plain]:
@@ -145,6 +152,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        log("cpu baseline")
         cpu = cpu_baseline(g, qs, meta, args.cap, args.cpu_budget)
 
     out = {

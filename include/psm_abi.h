@@ -79,6 +79,11 @@ const char* psm_last_error(const psm_ctx* ctx);
 int psm_kernel_stats(psm_ctx* ctx, int kind, double* out3);
 int psm_kernel_stats_reset(psm_ctx* ctx);
 
+/* The reference's -time_limit (StudyPerformance.cpp:1134-1137): enumeration
+ * stops at this many seconds and returns PSM_ETIMEOUT with partial counts
+ * (reference: "Overtime: 1"). Default 60 s; 0 disables the limit. */
+int psm_set_time_limit(psm_ctx* ctx, double seconds);
+
 /* ---- data graph: class Graph (graph/graph.h:21-237) ------------------
  * Uploads a CSR data graph (adjacency sorted ascending per vertex) and builds
  * on the device side the label reverse index (graph.cpp:15-33) and the

@@ -59,6 +59,7 @@ SIGNATURES = {
     "psm_last_error": (ctypes.c_char_p, [VP]),
     "psm_kernel_stats": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "psm_kernel_stats_reset": (ctypes.c_int, [VP]),
+    "psm_set_time_limit": (ctypes.c_int, [VP, ctypes.c_double]),
     "psm_graph_upload": (ctypes.c_int, [VP, ctypes.c_uint32, U32P, U32P, U32P, ctypes.POINTER(VP)]),
     "psm_graph_destroy": (ctypes.c_int, [VP]),
     "psm_graph_info": (ctypes.c_int, [VP, U64P]),

@@ -190,10 +190,14 @@ struct GqlShared {
     uint32_t st_via[kGqlWavesPerBlock][kMaxGqlDegree + 1];
 };
 
-__device__ __forceinline__ uint32_t left_mask(const uint32_t* const* lb, uint32_t k, uint32_t w) {
+// Bit i set when data vertex w is a live candidate of left vertex i.
+__device__ __forceinline__ uint32_t left_mask(const uint32_t* __restrict__ bitmaps, uint64_t wps,
+                                              const uint32_t* __restrict__ lslots, uint32_t k,
+                                              uint32_t w) {
     uint32_t m = 0;
     const uint32_t word = w >> 5, bit = w & 31;
-    for (uint32_t i = 0; i < k; ++i) m |= ((lb[i][word] >> bit) & 1u) << i;
+    for (uint32_t i = 0; i < k; ++i)
+        m |= ((bitmaps[(size_t)lslots[i] * wps + word] >> bit) & 1u) << i;
     return m;
 }
 
@@ -213,34 +217,28 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
     const uint32_t v = cand_pool[it.cand_begin + j];
     if (v == kInvalid) return;
     const uint32_t k = it.k;
-    const uint32_t* lb[kMaxGqlDegree];
-    for (uint32_t i = 0; i < k; ++i) lb[i] = bitmaps + (size_t)left_slots[it.left_begin + i] * wps;
-
+    const uint32_t* lslots = left_slots + it.left_begin;   // uniform: scalar loads
     const uint32_t vb = g.off[v], ve = g.off[v + 1];
-    // pass 1: number of right neighbours of every left vertex (saturating at k)
-    uint32_t cnt[kMaxGqlDegree];
-    for (uint32_t i = 0; i < k; ++i) cnt[i] = 0;
+    // pass 1: lane i (< k) counts the right neighbours of left vertex i,
+    // saturating at k
+    uint32_t my_cnt = 0;
     for (uint32_t base = vb; base < ve; base += kWave) {
         const uint32_t p = base + lane;
-        const uint32_t m = p < ve ? left_mask(lb, k, g.adj[p]) : 0u;
+        const uint32_t m = p < ve ? left_mask(bitmaps, wps, lslots, k, g.adj[p]) : 0u;
         for (uint32_t i = 0; i < k; ++i) {
             const uint32_t c = __popcll(__ballot((m >> i) & 1u));
-            cnt[i] = min(cnt[i] + c, k);
+            if (lane == i) my_cnt = min(my_cnt + c, k);
         }
     }
-    bool ok = true;
-    uint32_t T = 0;
-    for (uint32_t i = 0; i < k; ++i) {
-        if (cnt[i] == 0) ok = false;
-        if (cnt[i] < k) T |= 1u << i;
-    }
+    bool ok = __ballot(lane < k && my_cnt == 0) == 0;
+    const uint32_t T = (uint32_t)__ballot(lane < k && my_cnt < k);
     const bool pass2 = ok && T;
     if (pass2) {
         // pass 2: right vertices adjacent to T, compacted into LDS
         uint32_t nr = 0;
         for (uint32_t base = vb; base < ve; base += kWave) {
             const uint32_t p = base + lane;
-            const uint32_t m = p < ve ? (left_mask(lb, k, g.adj[p]) & T) : 0u;
+            const uint32_t m = p < ve ? (left_mask(bitmaps, wps, lslots, k, g.adj[p]) & T) : 0u;
             const unsigned long long bal = __ballot(m != 0);
             if (m) {
                 const uint32_t r = nr + __popcll(bal & lanemask_lt());
@@ -502,8 +500,8 @@ struct PrefixState {
 
 // One thread per task: enumerate the valid extensions at the task's depth.
 // Count pass (write = 0): out_count[t] = number of children (or 1 when the
-// task's query is not being expanded); calls += children. Write pass: the
-// children (or the task itself) at out_pos[t].
+// task's query is not being expanded). Write pass: the children (or the task
+// itself) at out_pos[t]; calls += children once the expansion is committed.
 __global__ __launch_bounds__(kBlock) void k_expand(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
@@ -524,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         return;
     }
     const DPlan& P = plans[task.q];
-    uint32_t I[kPrefixMax + 1], M[kPrefixMax + 1];
+    uint32_t I[kExpandMax + 1], M[kExpandMax + 1];
     for (uint32_t e = 0; e < task.depth; ++e) {
         I[e] = task.prefix[e];
         M[e] = cand_pool[P.cand_base[e] + I[e]];
@@ -532,12 +530,17 @@ __global__ __launch_bounds__(kBlock) void k_expand(
     PrefixState st{I, M};
     unsigned long long bytes = 0;
     const uint32_t d = task.depth;
-    const LevelBase b = level_base(P, ops, row_off, d, st, bytes);
-    uint64_t cur = b.lo + task.lo;
-    const uint64_t end = task.hi == kIdentity ? b.hi : min(b.hi, b.lo + task.hi);
+    LevelBase b;
+    if (task.hi == kAutoHi) {
+        b = level_base(P, ops, row_off, d, st, bytes);
+    } else {
+        b.lo = task.lo;
+        b.hi = task.hi;
+        b.sel = task.sel;
+    }
     uint32_t c = 0;
     uint64_t pos = write ? out_pos[t] : 0;
-    for (; cur < end; ++cur) {
+    for (uint64_t cur = b.lo; cur < b.hi; ++cur) {
         const uint32_t x = b.sel == kIdentity ? (uint32_t)cur : idx[cur];
         uint32_t v;
         if (!test_element(P, ops, row_off, idx, cand_pool, d, x, b.sel, st, v, bytes)) continue;
@@ -545,8 +548,10 @@ __global__ __launch_bounds__(kBlock) void k_expand(
             DTask ch;
             ch.q = task.q;
             ch.depth = d + 1;
+            ch.sel = 0;
+            ch.pad = 0;
             ch.lo = 0;
-            ch.hi = kIdentity;
+            ch.hi = kAutoHi;
             for (uint32_t e = 0; e < kPrefixMax; ++e) ch.prefix[e] = e < d ? task.prefix[e] : 0;
             ch.prefix[d] = x;
             out_tasks[pos++] = ch;
@@ -557,22 +562,34 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         out_count[t] = c;
         atomicAdd(qchild + task.q, (unsigned long long)c);
     } else {
-        // each child is one assignment at this depth; counted once the
-        // expansion is committed
+        // each child is one assignment at this depth
         atomicAdd(&qc[task.q].calls, (unsigned long long)c);
         atomicAdd(&qc[task.q].expand_bytes, bytes);
     }
 }
 
-// Persistent DFS: every lane runs its own depth-first search over one task at
-// a time; idle lanes of a wavefront take the next tasks from the global queue
-// with one atomic per wavefront. A query whose count reaches its cap is
-// marked done and its remaining tasks are dropped (count = min(cap, total)).
+__device__ __forceinline__ uint32_t load_acquire(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t load_relaxed(unsigned int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Persistent DFS. Every lane runs its own depth-first search over one task at
+// a time. Idle lanes claim the next queued tasks (one claim per wavefront).
+// A running lane splits off the second half of the remaining range at its
+// shallowest level with at least two entries left, as a new task, while the
+// queue is short (the reference's TaskPool donates half the siblings at the
+// shallowest splittable depth to idle threads, TaskPool.cpp:960-1362).
+// A query whose count reaches its cap is marked done and its tasks dropped
+// (count = min(cap, total)). Past the wall-clock deadline every lane stops
+// (the reference's Overtime).
 __global__ __launch_bounds__(kBlock) void k_dfs(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
-    const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ tasks, uint32_t n_tasks,
-    uint32_t* __restrict__ head, QCounters* qc, uint32_t max_n, uint32_t* lane_state) {
+    const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* ready, DQueue* Q,
+    QCounters* qc, uint32_t max_n, uint32_t* lane_state) {
     const uint32_t lane = lane_id();
     const size_t wave = ((size_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
     const size_t per_field = (size_t)gridDim.x * (kBlock / kWave) * max_n * kWave;
@@ -584,102 +601,175 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     st.end_ = st.cur_ + per_field;
     st.base = wave * max_n * kWave + lane;
 
-    bool active = false, drained = false;
+    if (threadIdx.x == 0 && Q->limit_ticks)
+        atomicCAS(&Q->deadline, 0ull, (unsigned long long)wall_clock64() + Q->limit_ticks);
+
+    bool active = false, waiting = false;
+    uint32_t wslot = 0, backoff = 0;
     uint32_t q = 0, tdepth = 0, d = 0, sel = 0, n = 0;
     uint64_t cur = 0, end = 0;
     unsigned long long lcount = 0, lcalls = 0, lbytes = 0;
     uint32_t steps = 0;
     const DPlan* P = plans;
+    const uint32_t hungry_below = gridDim.x * (kBlock / 2);   // queue shorter than half the lanes
 
     while (true) {
-        if (!drained) {
-            const unsigned long long idle = __ballot(!active);
-            if (idle) {
-                const uint32_t nidle = __popcll(idle);
-                const uint32_t leader = __ffsll((long long)idle) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(head, nidle);
-                base = __shfl(base, leader);
-                if (base + nidle >= n_tasks) drained = true;
-                if (!active) {
-                    const uint32_t t = base + __popcll(idle & lanemask_lt());
-                    if (t < n_tasks) {
-                        const DTask task = tasks[t];
-                        q = task.q;
-                        P = plans + q;
-                        if (!__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
-                            n = P->n;
-                            tdepth = task.depth;
-                            d = tdepth;
-                            for (uint32_t e = 0; e < tdepth; ++e) {
-                                const uint32_t x = task.prefix[e];
-                                st.I_[st.at(e)] = x;
-                                st.M_[st.at(e)] = cand_pool[P->cand_base[e] + x];
-                            }
-                            const LevelBase b = level_base(*P, ops, row_off, d, st, lbytes);
-                            sel = b.sel;
-                            cur = b.lo + task.lo;
-                            end = task.hi == kIdentity ? b.hi : min(b.hi, b.lo + (uint64_t)task.hi);
-                            active = true;
-                        }
+        // claim queued tasks for idle lanes, one atomic per wavefront
+        const unsigned long long want = __ballot(!active && !waiting);
+        if (want && backoff == 0) {
+            const uint32_t leader = __ffsll((long long)want) - 1;
+            uint32_t base = 0, got = 0;
+            if (lane == leader) {
+                const uint32_t h = load_relaxed(&Q->head), t = load_relaxed(&Q->tail);
+                if (h < t) {
+                    const uint32_t k = min((uint32_t)__popcll(want), t - h);
+                    if (atomicCAS(&Q->head, h, h + k) == h) {
+                        atomicAdd(&Q->busy, k);
+                        base = h;
+                        got = k;
                     }
                 }
             }
+            base = __shfl(base, leader);
+            got = __shfl(got, leader);
+            if (!active && !waiting) {
+                const uint32_t r = __popcll(want & lanemask_lt());
+                if (r < got) { waiting = true; wslot = base + r; }
+            }
+            if (!got) backoff = 16;
+        } else if (backoff) {
+            --backoff;
         }
-        if (__ballot(active) == 0) {
-            if (drained) break;
+        if (waiting && load_acquire(ready + wslot)) {
+            waiting = false;
+            const DTask task = queue[wslot];
+            q = task.q;
+            P = plans + q;
+            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout)) {
+                atomicSub(&Q->busy, 1u);
+            } else {
+                n = P->n;
+                tdepth = task.depth;
+                d = tdepth;
+                for (uint32_t e = 0; e < tdepth; ++e) {
+                    const uint32_t x = task.prefix[e];
+                    st.I_[st.at(e)] = x;
+                    st.M_[st.at(e)] = cand_pool[P->cand_base[e] + x];
+                }
+                if (task.hi == kAutoHi) {
+                    const LevelBase b = level_base(*P, ops, row_off, d, st, lbytes);
+                    sel = b.sel;
+                    cur = b.lo;
+                    end = b.hi;
+                } else {
+                    sel = task.sel;
+                    cur = task.lo;
+                    end = task.hi;
+                }
+                active = true;
+                steps = 0;
+            }
+        }
+        if (__ballot(active || waiting) == 0) {
+            // the whole wavefront is idle: leave once no work can appear
+            if (load_relaxed(&Q->timeout)) break;
+            if (load_relaxed(&Q->head) >= load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0) break;
+            __builtin_amdgcn_s_sleep(4);
             continue;
         }
-        if (active) {
-            if (cur < end) {
-                const uint32_t x = sel == kIdentity ? (uint32_t)cur : idx[cur];
-                lbytes += sel == kIdentity ? 0 : 4;
-                ++cur;
-                uint32_t v;
-                if (test_element(*P, ops, row_off, idx, cand_pool, d, x, sel, st, v, lbytes)) {
-                    ++lcalls;
-                    if (d + 1 == n) {
-                        ++lcount;
-                    } else {
-                        const size_t a = st.at(d);
-                        st.cur_[a] = cur;
-                        st.end_[a] = end;
-                        st.sel_[a] = sel;
-                        st.I_[a] = x;
-                        st.M_[a] = v;
-                        ++d;
-                        const LevelBase b = level_base(*P, ops, row_off, d, st, lbytes);
-                        cur = b.lo;
-                        end = b.hi;
-                        sel = b.sel;
-                    }
+        if (!active) continue;
+
+        // one step of this lane's search
+        if (cur < end) {
+            const uint32_t x = sel == kIdentity ? (uint32_t)cur : idx[cur];
+            lbytes += sel == kIdentity ? 0 : 4;
+            ++cur;
+            uint32_t v;
+            if (test_element(*P, ops, row_off, idx, cand_pool, d, x, sel, st, v, lbytes)) {
+                ++lcalls;
+                if (d + 1 == n) {
+                    ++lcount;
+                } else {
+                    const size_t a = st.at(d);
+                    st.cur_[a] = cur;
+                    st.end_[a] = end;
+                    st.sel_[a] = sel;
+                    st.I_[a] = x;
+                    st.M_[a] = v;
+                    ++d;
+                    const LevelBase b = level_base(*P, ops, row_off, d, st, lbytes);
+                    cur = b.lo;
+                    end = b.hi;
+                    sel = b.sel;
                 }
-            } else if (d == tdepth) {
-                active = false;
-            } else {
-                --d;
-                const size_t a = st.at(d);
-                cur = st.cur_[a];
-                end = st.end_[a];
-                sel = st.sel_[a];
             }
-            ++steps;
-            // a task of a query that reached its cap (here or in another lane) is dropped
-            if (active && (steps & 255) == 0 && __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED))
+        } else if (d == tdepth) {
+            active = false;
+        } else {
+            --d;
+            const size_t a = st.at(d);
+            cur = st.cur_[a];
+            end = st.end_[a];
+            sel = st.sel_[a];
+        }
+        ++steps;
+        if (active && (steps & 255) == 0) {
+            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout)) {
                 active = false;
-            const bool flush = !active || lcount >= 4096 || (steps & 1023) == 0;
-            if (flush && (lcount | lcalls | lbytes)) {
-                const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
-                atomicAdd(&qc[q].calls, lcalls);
-                atomicAdd(&qc[q].probe_bytes, lbytes);
-                if (lcount && prev + lcount >= P->cap) atomicExch(&qc[q].done, 1u);
-                lcount = lcalls = lbytes = 0;
+            } else if (Q->limit_ticks && (unsigned long long)wall_clock64() >
+                                             __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT)) {
+                atomicExch(&Q->timeout, 1u);
+                active = false;
             }
         }
+        // donation: split the shallowest level that still has >= 2 entries
+        if (active && (steps & 63) == 0 &&
+            load_relaxed(&Q->tail) - load_relaxed(&Q->head) < hungry_below) {
+            uint32_t e = tdepth;
+            uint64_t ce = 0, ee = 0;
+            for (; e <= d; ++e) {
+                ce = e == d ? cur : st.cur_[st.at(e)];
+                ee = e == d ? end : st.end_[st.at(e)];
+                if (ee - ce >= 2) break;
+            }
+            if (e <= d && e <= kPrefixMax) {
+                const uint32_t t = load_relaxed(&Q->tail);
+                if (t < Q->cap && atomicCAS(&Q->tail, t, t + 1) == t) {
+                    const uint64_t mid = ce + (ee - ce) / 2;
+                    DTask& nt = queue[t];
+                    nt.q = q;
+                    nt.depth = e;
+                    nt.sel = e == d ? sel : st.sel_[st.at(e)];
+                    nt.pad = 0;
+                    nt.lo = mid;
+                    nt.hi = ee;
+                    for (uint32_t k = 0; k < e; ++k) nt.prefix[k] = st.I_[st.at(k)];
+                    __threadfence();
+                    __hip_atomic_store(ready + t, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    if (e == d) end = mid;
+                    else st.end_[st.at(e)] = mid;
+                    atomicAdd(&Q->donated, 1u);
+                }
+            }
+        }
+        const bool flush = !active || lcount >= 4096 || (steps & 1023) == 0;
+        if (flush && (lcount | lcalls | lbytes)) {
+            const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
+            atomicAdd(&qc[q].calls, lcalls);
+            atomicAdd(&qc[q].probe_bytes, lbytes);
+            if (lcount && prev + lcount >= P->cap) atomicExch(&qc[q].done, 1u);
+            lcount = lcalls = lbytes = 0;
+        }
+        if (!active) atomicSub(&Q->busy, 1u);
     }
-    if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);   // bytes of tasks dropped at fetch
+    if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);   // bytes of tasks dropped at claim
 }
 
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        p[i] = v;
+}
 
 // ---------------------------------------------------------------------------
 // Exclusive scans (64-bit totals) and per-slot bitmap word ranks.
@@ -952,13 +1042,19 @@ uint32_t dfs_lane_state_words(uint32_t max_n) {
 }
 
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                      const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
-                      uint32_t n_tasks, uint32_t* head, QCounters* qc, uint32_t max_n,
-                      uint32_t grid_waves, uint32_t* lane_state, hipStream_t s) {
-    if (n_tasks == 0 || grid_waves == 0) return hipSuccess;
+                      const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
+                      DQueue* dq, QCounters* qc, uint32_t max_n, uint32_t grid_waves,
+                      uint32_t* lane_state, hipStream_t s) {
+    if (grid_waves == 0) return hipSuccess;
     const uint32_t blocks = grid_waves / (kBlock / kWave);
     hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(kBlock), 0, s, plans, ops, row_off, idx, cand_pool,
-                       tasks, n_tasks, head, qc, max_n, lane_state);
+                       queue, ready, dq, qc, max_n, lane_state);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(kBlock), 0, s, p, v, n);
     return hipGetLastError();
 }
 

@@ -10,7 +10,9 @@ namespace psm {
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;   // pruned candidate
 constexpr uint32_t kMaxQ = 64;               // query vertices (config.h:9)
 constexpr uint32_t kMaxGqlDegree = 32;       // query degree limit of the GQL refinement kernel
-constexpr uint32_t kPrefixMax = 8;           // prefix entries a task record carries
+constexpr uint32_t kPrefixMax = 32;          // prefix entries a task record carries
+constexpr uint32_t kExpandMax = 8;           // depth limit of task pre-expansion
+constexpr uint64_t kAutoHi = ~0ull;          // task range = the whole base list
 constexpr uint32_t kIdentity = 0xFFFFFFFFu;  // "base list is C(u) itself"
 
 // Device-resident data graph (uint32 CSR + derived indexes).
@@ -96,13 +98,30 @@ struct DOp {
 };
 
 // A unit of enumeration work: query q with the prefix I[0..depth-1] fixed;
-// the lane enumerates positions [lo, hi) of the depth's base list.
+// the lane enumerates positions [lo, hi) of the depth's base list (operand
+// `sel`, or C(u) itself when sel == kIdentity). hi == kAutoHi: the whole
+// list, located by the lane from the prefix.
 struct DTask {
     uint32_t q;
     uint32_t depth;
-    uint32_t lo;
-    uint32_t hi;          // kIdentity: to the end of the base list
+    uint32_t sel;
+    uint32_t pad;
+    uint64_t lo;
+    uint64_t hi;
     uint32_t prefix[kPrefixMax];
+};
+
+// Work queue of the persistent DFS: slots [0, tail) hold tasks, claimed in
+// order through head; a slot's ready flag is set once its record is written.
+struct DQueue {
+    unsigned int head;
+    unsigned int tail;
+    unsigned int busy;         // lanes holding a task
+    unsigned int timeout;      // the time limit passed
+    unsigned int cap;          // slots allocated
+    unsigned int donated;      // tasks split off by running lanes
+    unsigned long long limit_ticks;   // wall-clock ticks allowed (0: no limit)
+    unsigned long long deadline;      // set by the first lane to start
 };
 
 // Per-query enumeration counters.
@@ -166,9 +185,10 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
                          const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
                          unsigned long long* qchild, int write, hipStream_t s);
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                      const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
-                      uint32_t n_tasks, uint32_t* head, QCounters* qc, uint32_t max_n,
-                      uint32_t grid_waves, uint32_t* lane_state, hipStream_t s);
+                      const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
+                      DQueue* dq, QCounters* qc, uint32_t max_n, uint32_t grid_waves,
+                      uint32_t* lane_state, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 uint32_t dfs_lane_state_words(uint32_t max_n);
 uint64_t scan_tmp_entries(uint64_t n);
 hipError_t launch_scan_u32(const uint32_t* in, uint64_t n, uint64_t* out, unsigned long long* tmp,

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -41,6 +42,9 @@ struct psm_ctx {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
     unsigned long long* d_kbytes = nullptr;   // kernel-counted algorithmic bytes per class
+    double time_limit_s = 60.0;               // reference default -time_limit 60
+    double wall_khz = 100000.0;               // device wall clock rate
+    bool verbose = false;
 };
 
 namespace {
@@ -161,6 +165,17 @@ hipError_t timers_flush(psm_ctx* ctx) {
         const size_t _ti = timer_begin((ctx), (kind));      \
         HIP_TRY((ctx), (expr));                             \
         timer_end((ctx), _ti);                              \
+    } while (0)
+
+double now_ms();
+
+// Stage progress on stderr when PSM_VERBOSE is set.
+#define verbose(...)                                                  \
+    do {                                                              \
+        if (ctx->verbose) {                                           \
+            fprintf(stderr, "[psm] " __VA_ARGS__);                    \
+            fputc('\n', stderr);                                     \
+        }                                                             \
     } while (0)
 
 double now_ms() {
@@ -508,6 +523,7 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
             if (items.empty()) continue;
             HIP_TRY(ctx, upload(d_items, items, s()));
             HIP_TRY(ctx, upload(d_left, left, s()));
+            verbose("gql round %d u %u: %u candidates", round, p, waves);
             TIMED(ctx, kKGqlRefine,
                   launch_gql_refine(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), waves,
                                     d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
@@ -548,6 +564,8 @@ int Batch::filter(int method) {
         default: return set_err(ctx, PSM_EINVAL, "unknown filter method");
     }
     t_filter += now_ms() - t0;
+    verbose("filter %d: %zu queries, %llu candidates, %.1f ms", method, qs.size(),
+            (unsigned long long)cand_total, now_ms() - t0);
     return PSM_OK;
 }
 
@@ -612,6 +630,8 @@ int Batch::build_tables() {
     HIP_TRY(ctx, timers_flush(ctx));
     tables_built = true;
     t_table += now_ms() - t0;
+    verbose("tables: %u tables, %llu rows, %llu entries, %.1f ms", ni, (unsigned long long)n_rows,
+            (unsigned long long)n_idx, now_ms() - t0);
     return PSM_OK;
 }
 
@@ -875,17 +895,21 @@ int Batch::upload_plans(uint64_t cap) {
 }
 
 // ---- enumeration ------------------------------------------------------------
+// Root tasks (one per query) are pre-expanded level by level until each query
+// has enough tasks to spread over the chip, then one persistent DFS kernel
+// drains a shared work queue that running lanes refill by splitting their
+// ranges.
 int Batch::evaluate(uint64_t cap) {
     for (QueryHost* q : qs)
         if (!q->planned) return set_err(ctx, PSM_ESTATE, "generate a plan before evaluating");
-    const double t0 = now_ms();
     PSM_TRY(upload_plans(cap));
     const uint32_t nq = (uint32_t)qs.size();
     HIP_TRY(ctx, qcnt.alloc(sizeof(QCounters) * nq));
     HIP_TRY(ctx, hipMemsetAsync(qcnt.p, 0, sizeof(QCounters) * nq, s()));
 
     std::vector<DTask> tasks;
-    std::vector<uint32_t> depth(nq, 0), ntask(nq, 0);
+    std::vector<uint32_t> depth(nq, 0);
+    std::vector<uint64_t> ntask(nq, 0);
     uint32_t max_n = 1;
     for (uint32_t i = 0; i < nq; ++i) {
         QueryHost* q = qs[i];
@@ -893,15 +917,13 @@ int Batch::evaluate(uint64_t cap) {
         if (q->cand_count[q->plan.order[0]] == 0) continue;
         DTask t{};
         t.q = i;
-        t.depth = 0;
-        t.lo = 0;
-        t.hi = kIdentity;
+        t.hi = kAutoHi;
         tasks.push_back(t);
         ntask[i] = 1;
     }
     // persistent grid: 8 wavefronts per CU
     const uint32_t grid_waves_full = (uint32_t)ctx->num_cu * 8;
-    const uint64_t target_total = (uint64_t)grid_waves_full * 64 * 4;
+    const uint64_t target_total = (uint64_t)grid_waves_full * 64 * 2;
     const uint64_t budget_total = std::max<uint64_t>(target_total * 8, 1u << 22);
     uint32_t active_q = 0;
     for (uint32_t i = 0; i < nq; ++i) active_q += ntask[i] ? 1 : 0;
@@ -910,16 +932,13 @@ int Batch::evaluate(uint64_t cap) {
     DBuf d_tasks, d_tasks2, d_cnt, d_pos, d_tmp, d_expand, d_qchild;
     HIP_TRY(ctx, upload(d_tasks, tasks, s()));
     uint64_t n_tasks = tasks.size();
-    hipEvent_t e0, e1;
-    HIP_TRY(ctx, hipEventCreate(&e0));
-    HIP_TRY(ctx, hipEventCreate(&e1));
-    HIP_TRY(ctx, hipEventRecord(e0, s()));
-    for (uint32_t round = 0; round < kPrefixMax && n_tasks; ++round) {
+    const double te0 = now_ms();
+    for (uint32_t round = 0; round < kExpandMax && n_tasks; ++round) {
         std::vector<uint8_t> ex(nq, 0);
         bool any = false;
         for (uint32_t i = 0; i < nq; ++i) {
             if (!ntask[i]) continue;
-            const uint32_t lim = std::min<uint32_t>(qs[i]->qg.n - 1, kPrefixMax);
+            const uint32_t lim = std::min<uint32_t>(qs[i]->qg.n - 1, kExpandMax);
             if (ntask[i] < per_q_target && depth[i] < lim) { ex[i] = 1; any = true; }
         }
         if (!any) break;
@@ -929,77 +948,80 @@ int Batch::evaluate(uint64_t cap) {
         HIP_TRY(ctx, d_cnt.alloc(n_tasks * 4));
         HIP_TRY(ctx, d_pos.alloc((n_tasks + 1) * 8));
         HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(n_tasks) * 8));
-        HIP_TRY(ctx, launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
-                                   idx.as<uint32_t>(), cand_pool.as<uint32_t>(), d_tasks.as<DTask>(),
-                                   (uint32_t)n_tasks, d_expand.as<uint8_t>(), d_cnt.as<uint32_t>(),
-                                   nullptr, nullptr, qcnt.as<QCounters>(),
-                                   d_qchild.as<unsigned long long>(), 0, s()));
+        TIMED(ctx, kKExpand,
+              launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
+                            cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
+                            d_expand.as<uint8_t>(), d_cnt.as<uint32_t>(), nullptr, nullptr,
+                            qcnt.as<QCounters>(), d_qchild.as<unsigned long long>(), 0, s()));
         HIP_TRY(ctx, launch_scan_u32(d_cnt.as<uint32_t>(), n_tasks, d_pos.as<uint64_t>(),
                                      d_tmp.as<unsigned long long>(), s()));
         uint64_t total = 0;
         HIP_TRY(ctx, hipMemcpyAsync(&total, d_pos.as<uint64_t>() + n_tasks, 8, hipMemcpyDeviceToHost, s()));
         HIP_TRY(ctx, hipStreamSynchronize(s()));
-        if (total > budget_total || total > 0xFFFFFFF0ull) break;
+        if (total > budget_total || total > 0x7FFFFFFFull) break;
         std::vector<unsigned long long> qchild;
         HIP_TRY(ctx, download(qchild, d_qchild.p, nq, s()));
         HIP_TRY(ctx, d_tasks2.alloc(std::max<uint64_t>(total, 1) * sizeof(DTask)));
         TIMED(ctx, kKExpand,
-              launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
-                                   idx.as<uint32_t>(), cand_pool.as<uint32_t>(), d_tasks.as<DTask>(),
-                                   (uint32_t)n_tasks, d_expand.as<uint8_t>(), nullptr,
-                                   d_pos.as<uint64_t>(), d_tasks2.as<DTask>(), qcnt.as<QCounters>(),
-                                   nullptr, 1, s()));
+              launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
+                            cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
+                            d_expand.as<uint8_t>(), nullptr, d_pos.as<uint64_t>(), d_tasks2.as<DTask>(),
+                            qcnt.as<QCounters>(), nullptr, 1, s()));
         std::swap(d_tasks.p, d_tasks2.p);
         std::swap(d_tasks.bytes, d_tasks2.bytes);
         for (uint32_t i = 0; i < nq; ++i) {
             if (ex[i]) depth[i]++;
-            ntask[i] = (uint32_t)std::min<uint64_t>(qchild[i], 0xFFFFFFFFu);
+            ntask[i] = qchild[i];
         }
         n_tasks = total;
     }
-    HIP_TRY(ctx, hipEventRecord(e1, s()));
-    HIP_TRY(ctx, hipEventSynchronize(e1));
-    float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    t_expand += ms;
-
-    if (n_tasks) {
-        uint32_t gw = (uint32_t)std::min<uint64_t>(grid_waves_full, (n_tasks + 63) / 64);
-        gw = std::max<uint32_t>(4, (gw + 3) / 4 * 4);
-        DBuf d_head, d_state;
-        HIP_TRY(ctx, d_head.alloc(4));
-        HIP_TRY(ctx, hipMemsetAsync(d_head.p, 0, 4, s()));
-        HIP_TRY(ctx, d_state.alloc((size_t)gw * 64 * dfs_lane_state_words(max_n) * 4));
-        HIP_TRY(ctx, hipEventRecord(e0, s()));
-        TIMED(ctx, kKDfs,
-              launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(),
-                         idx.as<uint32_t>(), cand_pool.as<uint32_t>(), d_tasks.as<DTask>(),
-                         (uint32_t)n_tasks, d_head.as<uint32_t>(), qcnt.as<QCounters>(), max_n,
-                         gw, d_state.as<uint32_t>(), s()));
-        HIP_TRY(ctx, hipEventRecord(e1, s()));
-        HIP_TRY(ctx, hipEventSynchronize(e1));
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        t_enum += ms;
-    }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    tasks_run += n_tasks;
     HIP_TRY(ctx, timers_flush(ctx));
+    t_expand += now_ms() - te0;
+    verbose("expand: %llu tasks in %.1f ms", (unsigned long long)n_tasks, now_ms() - te0);
+
+    unsigned int timed_out = 0, donated = 0;
+    if (n_tasks) {
+        // queue: the expanded tasks followed by room for split-off work
+        const uint64_t cap_slots = std::min<uint64_t>(n_tasks + (4u << 20), 0xFFFFFFF0ull);
+        DBuf d_queue, d_ready, d_q, d_state;
+        HIP_TRY(ctx, d_queue.alloc(cap_slots * sizeof(DTask)));
+        HIP_TRY(ctx, hipMemcpyAsync(d_queue.p, d_tasks.p, n_tasks * sizeof(DTask), hipMemcpyDeviceToDevice, s()));
+        HIP_TRY(ctx, d_ready.alloc(cap_slots * 4));
+        HIP_TRY(ctx, hipMemsetAsync(d_ready.p, 0, cap_slots * 4, s()));
+        HIP_TRY(ctx, launch_fill_u32(d_ready.as<uint32_t>(), 1u, n_tasks, s()));
+        DQueue hq{};
+        hq.tail = (unsigned int)n_tasks;
+        hq.cap = (unsigned int)cap_slots;
+        hq.limit_ticks = ctx->time_limit_s > 0 ? (unsigned long long)(ctx->time_limit_s * ctx->wall_khz * 1000.0) : 0;
+        HIP_TRY(ctx, d_q.alloc(sizeof(DQueue)));
+        HIP_TRY(ctx, hipMemcpyAsync(d_q.p, &hq, sizeof(hq), hipMemcpyHostToDevice, s()));
+        const uint32_t gw = grid_waves_full / 4 * 4;
+        HIP_TRY(ctx, d_state.alloc((size_t)gw * 64 * dfs_lane_state_words(max_n) * 4));
+        const double td0 = now_ms();
+        TIMED(ctx, kKDfs,
+              launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
+                         cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_ready.as<uint32_t>(),
+                         d_q.as<DQueue>(), qcnt.as<QCounters>(), max_n, gw, d_state.as<uint32_t>(), s()));
+        HIP_TRY(ctx, hipMemcpyAsync(&hq, d_q.p, sizeof(hq), hipMemcpyDeviceToHost, s()));
+        HIP_TRY(ctx, timers_flush(ctx));
+        t_enum += now_ms() - td0;
+        timed_out = hq.timeout;
+        donated = hq.donated;
+        verbose("dfs: %.1f ms, %u tasks split off, timeout %u", now_ms() - td0, donated, timed_out);
+    }
+    tasks_run += n_tasks + donated;
     std::vector<QCounters> qc;
     HIP_TRY(ctx, download(qc, qcnt.p, nq, s()));
     for (uint32_t i = 0; i < nq; ++i) {
         ctx->kstat[kKDfs].bytes += (double)qc[i].probe_bytes;
         ctx->kstat[kKExpand].bytes += (double)qc[i].expand_bytes;
-    }
-    for (uint32_t i = 0; i < nq; ++i) {
         QueryHost* q = qs[i];
-        const uint64_t c = cap ? std::min<uint64_t>(cap, qc[i].count) : qc[i].count;
-        q->count = c;
+        q->count = cap ? std::min<uint64_t>(cap, qc[i].count) : qc[i].count;
         q->capped = cap && qc[i].count >= cap;
         q->calls = qc[i].calls;
         q->probe_bytes = qc[i].probe_bytes;
     }
-    (void)t0;
+    if (timed_out) return set_err(ctx, PSM_ETIMEOUT, "enumeration passed the time limit");
     return PSM_OK;
 }
 
@@ -1033,6 +1055,11 @@ int psm_ctx_create(int device, psm_ctx** out) {
     if (e != hipSuccess) { delete c; return -(int)e; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        c->wall_khz = khz;
+    const char* vb = getenv("PSM_VERBOSE");
+    c->verbose = vb && *vb && *vb != '0';
     e = hipMalloc(&c->d_kbytes, sizeof(unsigned long long) * kKNumKinds);
     if (e == hipSuccess) e = hipMemset(c->d_kbytes, 0, sizeof(unsigned long long) * kKNumKinds);
     if (e != hipSuccess) { (void)hipStreamDestroy(c->stream); delete c; return -(int)e; }
@@ -1059,6 +1086,12 @@ int psm_kernel_stats(psm_ctx* ctx, int kind, double* out3) {
     out3[0] = ctx->kstat[kind].launches;
     out3[1] = ctx->kstat[kind].ms;
     out3[2] = ctx->kstat[kind].bytes;
+    return PSM_OK;
+}
+
+int psm_set_time_limit(psm_ctx* ctx, double seconds) {
+    if (!ctx || seconds < 0) return PSM_EINVAL;
+    ctx->time_limit_s = seconds;
     return PSM_OK;
 }
 
