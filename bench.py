@@ -31,7 +31,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(g, qs, meta, cap: int, budget_s: float):
+def cpu_baseline(g, qs, meta, cap: int, budget_s: float, time_limit: float):
     """The reference binary (oracle/_ref, built from the reference's sources)
     on the first queries of the workload, one thread, until about budget_s of
     its own reported query time; falls back to the oracle restatement."""
@@ -44,21 +44,24 @@ def cpu_baseline(g, qs, meta, cap: int, budget_s: float):
         td = R.deep_dir(tempfile.mkdtemp(prefix="psm_cpu_"))
         dp = os.path.join(td, "data.txt")
         save_text(g, dp)
-        spent, done, emb = 0.0, 0, 0
+        spent, done, emb, timeouts = 0.0, 0, 0, 0
         t_wall = time.time()
         for i, q in enumerate(qs):
             qp = os.path.join(td, "q.txt")
             save_text(q, qp)
-            b = R.run(dp, qp, "GQL_GQL_GQL", threads="1", cap=str(cap), timeout=600)[0]
+            b = R.run(dp, qp, "GQL_GQL_GQL", threads="1", cap=str(cap),
+                      time_limit=max(1, int(round(time_limit))), timeout=600)[0]
             spent += b["total_s"]
             emb += b["embeddings"]
+            timeouts += b["overtime"]
             done += 1
             if spent >= budget_s or time.time() - t_wall > 3 * budget_s:
                 break
         return dict(value=done / spent, unit="queries/s", cores=1, kind="reference",
-                    embeddings_per_s=emb / spent,
+                    embeddings_per_s=emb / spent, timeouts=timeouts,
                     sample=f"first {done} of {len(qs)} queries, reference SubgraphMatching.out "
-                           f"GQL_GQL_GQL -threadnums 1 -num {cap}, its own Total time "
+                           f"GQL_GQL_GQL -threadnums 1 -num {cap} -time_limit "
+                           f"{max(1, int(round(time_limit)))}, its own Total time "
                            f"(filter+table+plan+enumerate, graph load excluded) = {spent:.2f}s")
     from oracle_binding import OGraph, ORun
     d = OGraph(g)
@@ -85,7 +88,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", type=int, default=512, help="use the first N workload queries")
-    ap.add_argument("--time-limit", type=float, default=120.0, help="per-batch enumeration limit, s")
+    ap.add_argument("--time-limit", type=float, default=2.0,
+                    help="enumeration time limit in seconds; 2 s is the -time_limit of the "
+                         "reference's recorded runs (test_anony/survey_bash/C_exp.py)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (queries shard "
@@ -122,6 +127,7 @@ def main():
         log(f"step: {times[-1] * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
             f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f})")
     ks = kernel_stats(dev)
+    timeouts = int((status == L.PSM_ETIMEOUT).sum())
     total = sum(times)
     nq = len(qs)
     emb = int(counts.sum())
@@ -130,14 +136,17 @@ def main():
     # plain path: one query at a time through the same library
     log(f"plain path over {min(args.plain, nq)} queries")
     lat = []
-    plain_counts = []
+    plain_counts, plain_timeouts = [], []
     for q in qs[: args.plain]:
         t = time.perf_counter()
         r = match(dg, q, "GQL_GQL_GQL", cap=args.cap)
         lat.append(time.perf_counter() - t)
         plain_counts.append(r.count)
-    if plain_counts != counts[: len(plain_counts)].tolist():
-        raise SystemExit("batched and plain counts differ")
+        plain_timeouts.append(r.timed_out)
+    mismatch = [i for i, c in enumerate(plain_counts)
+                if c != counts[i] and status[i] != L.PSM_ETIMEOUT and not plain_timeouts[i]]
+    if mismatch:
+        raise SystemExit(f"batched and plain counts differ for queries {mismatch[:10]}")
     lat_ms = np.array(lat) * 1e3
     plain_qps = len(lat) / sum(lat)
 
@@ -153,10 +162,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline(g, qs, meta, args.cap, args.cpu_budget)
+        cpu = cpu_baseline(g, qs, meta, args.cap, args.cpu_budget, args.time_limit)
 
     out = {
-        "metric": "queries/s (batched, 512 queries, cap 1e5) on 1 MI355X",
+        "metric": "queries/s + embeddings/s on 1 MI355X; p50/p99 latency batched vs plain",
         "value": round(nq * args.steps / total, 2),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -170,13 +179,16 @@ def main():
         "data": "synthetic (seeded Chung-Lu graph and random-walk queries)",
         "config": {"workload": "C2 batched: Chung-Lu 1.1M V / 3.0M E, 100 Zipf(1.1) labels, "
                                "512 queries of 4-32 vertices (half dense), GQL_GQL_GQL, cap 1e5",
-                   "queries": nq, "cap": args.cap},
+                   "queries": nq, "cap": args.cap, "time_limit_s": args.time_limit},
+        "timeouts": timeouts,
+        "completed_queries_per_s": round((nq - timeouts) * args.steps / total, 2),
         "embeddings_per_s": round(emb * args.steps / total, 1),
         "embeddings_per_step": emb,
         "latency_ms": {"batched_p50": round(step_ms, 2), "batched_p99": round(step_ms, 2),
                        "plain_p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "plain_p99": round(float(np.percentile(lat_ms, 99)), 3)},
         "plain_gpu_queries_per_s": round(plain_qps, 2),
+        "plain_timeouts": int(sum(plain_timeouts)),
         "batched_over_plain": round(nq * args.steps / total / plain_qps, 2),
         "stage_ms": {k: round(float(np.mean([getattr(s, k) for s in stats])), 2)
                      for k in ("filter_ms", "table_ms", "plan_ms", "enum_ms")},

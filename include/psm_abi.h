@@ -130,7 +130,8 @@ int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t*
 
 /* EvaluateQuery::ParallelExecute with the GQL / EXPLORE engines
  * (EvaluateQuery.cpp:26-112,980-1126): counts embeddings, at most `cap`
- * (0 = no cap). */
+ * (0 = no cap). Returns PSM_ETIMEOUT with the partial count when the time
+ * limit (psm_set_time_limit) stops it. */
 int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* stats);
 
 /* Convenience: filter + tables + plan + evaluate in one call, the sequence of
@@ -143,7 +144,9 @@ int psm_match(psm_query* q, int filter, int order, uint64_t cap, psm_enum_stats*
  * as concatenated CSR arrays: query i has n[i] vertices, its offsets are
  * q_offsets[off_base[i] .. off_base[i]+n[i]] (relative to its own neighbours,
  * which start at nbr_base[i]), labels at lab_base[i]. counts/calls receive
- * one entry per query (calls exact only for queries that did not hit cap).
+ * one entry per query (calls exact only for queries that did not hit cap);
+ * status[i] is PSM_OK, PSM_EEMPTY (a filter emptied a candidate set) or
+ * PSM_ETIMEOUT (partial count: the time limit stopped it).
  * No counterpart in the reference, which matches one query per process. */
 typedef struct psm_batch_stats {
     double filter_ms, table_ms, plan_ms, enum_ms, total_ms;

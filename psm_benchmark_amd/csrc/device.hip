@@ -646,6 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             q = task.q;
             P = plans + q;
             if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout)) {
+                if (!__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) atomicExch(&qc[q].timed_out, 1u);
                 atomicSub(&Q->busy, 1u);
             } else {
                 n = P->n;
@@ -714,12 +715,16 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         }
         ++steps;
         if (active && (steps & 255) == 0) {
-            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout)) {
+            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
+                active = false;
+            } else if (load_relaxed(&Q->timeout)) {
+                atomicExch(&qc[q].timed_out, 1u);
                 active = false;
             } else if (Q->limit_ticks && (unsigned long long)wall_clock64() >
                                              __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED,
                                                                __HIP_MEMORY_SCOPE_AGENT)) {
                 atomicExch(&Q->timeout, 1u);
+                atomicExch(&qc[q].timed_out, 1u);
                 active = false;
             }
         }

@@ -131,7 +131,7 @@ struct QCounters {
     unsigned long long probe_bytes;    // algorithmic bytes of the DFS kernel
     unsigned long long expand_bytes;   // algorithmic bytes of task pre-expansion
     uint32_t done;        // cap reached
-    uint32_t pad;
+    uint32_t timed_out;   // work was dropped at the time limit
 };
 
 // Kernel classes timed by the library (psm_kernel_stats).

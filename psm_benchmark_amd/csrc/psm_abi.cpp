@@ -259,6 +259,7 @@ struct QueryHost {
     bool planned = false;
     uint64_t count = 0, calls = 0, probe_bytes = 0;
     bool capped = false;
+    bool timed_out = false;           // partial count: the reference's "Overtime: 1"
 };
 
 constexpr uint64_t kNoTable = ~0ull;
@@ -1020,8 +1021,9 @@ int Batch::evaluate(uint64_t cap) {
         q->capped = cap && qc[i].count >= cap;
         q->calls = qc[i].calls;
         q->probe_bytes = qc[i].probe_bytes;
+        q->timed_out = qc[i].timed_out != 0;
     }
-    if (timed_out) return set_err(ctx, PSM_ETIMEOUT, "enumeration passed the time limit");
+    if (timed_out) set_err(ctx, PSM_ETIMEOUT, "enumeration passed the time limit");
     return PSM_OK;
 }
 
@@ -1269,6 +1271,7 @@ int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     hipSetDevice(q->ctx->device);
     q->batch.t_enum = q->batch.t_expand = 0;
     PSM_TRY(q->batch.evaluate(cap));
+    const int r = q->qh.timed_out ? PSM_ETIMEOUT : PSM_OK;
     if (st) {
         memset(st, 0, sizeof(*st));
         st->count = q->qh.count;
@@ -1279,7 +1282,7 @@ int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
         st->expand_ms = q->batch.t_expand;
         st->probe_bytes = q->qh.probe_bytes;
     }
-    return PSM_OK;
+    return r;
 }
 
 int psm_match(psm_query* q, int filter, int order, uint64_t cap, psm_enum_stats* st) {
@@ -1316,7 +1319,7 @@ int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32_
     for (uint32_t i = 0; i < nq; ++i) {
         if (counts) counts[i] = hosts[i]->count;
         if (calls) calls[i] = hosts[i]->calls;
-        if (status) status[i] = hosts[i]->status;
+        if (status) status[i] = hosts[i]->timed_out ? PSM_ETIMEOUT : hosts[i]->status;
     }
     if (st) {
         memset(st, 0, sizeof(*st));

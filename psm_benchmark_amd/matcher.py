@@ -153,8 +153,11 @@ class Query:
         return o, p
 
     def evaluate(self, cap: int = 0) -> L.EnumStats:
+        """Counts embeddings; st.timed_out is set when the time limit cut it short."""
         st = L.EnumStats()
-        _check(self.dg.dev.h, L.lib().psm_evaluate(self.h, cap, ctypes.byref(st)), "psm_evaluate")
+        r = _check(self.dg.dev.h, L.lib().psm_evaluate(self.h, cap, ctypes.byref(st)), "psm_evaluate",
+                   ok=(L.PSM_OK, L.PSM_ETIMEOUT))
+        st.timed_out = r == L.PSM_ETIMEOUT
         return st
 
 
@@ -169,6 +172,7 @@ class MatchResult:
     kernel_ms: float
     probe_bytes: int
     cand_counts: list = field(default_factory=list)
+    timed_out: bool = False
 
 
 def match(dg: DeviceGraph, q: CSRGraph, method: str = "GQL_GQL_GQL", cap: int = 0) -> MatchResult:
@@ -186,7 +190,7 @@ def match(dg: DeviceGraph, q: CSRGraph, method: str = "GQL_GQL_GQL", cap: int = 
         st = run.evaluate(cap)
         return MatchResult(int(st.count), int(st.calls), int(mem), ok, [int(x) for x in order],
                            bool(st.capped), float(st.kernel_ms + st.expand_ms), int(st.probe_bytes),
-                           [int(x) for x in run.candidate_counts()])
+                           [int(x) for x in run.candidate_counts()], bool(st.timed_out))
     finally:
         run.close()
 
