@@ -147,8 +147,8 @@ def main():
                 if c != counts[i] and status[i] != L.PSM_ETIMEOUT and not plain_timeouts[i]]
     if mismatch:
         raise SystemExit(f"batched and plain counts differ for queries {mismatch[:10]}")
-    lat_ms = np.array(lat) * 1e3
-    plain_qps = len(lat) / sum(lat)
+    lat_ms = np.array(lat if lat else [float("nan")]) * 1e3
+    plain_qps = len(lat) / sum(lat) if lat else float("nan")
 
     # dominant kernel roofline (bytes counted by the kernels themselves,
     # durations from HIP events on the library's stream)
