@@ -45,7 +45,26 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     subprocess.check_call([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
+    build_dropin()
     return LIB
+
+
+DROPIN = os.path.join(HERE, "dropin")
+DROPIN_LIB = os.path.join(HERE, "libpsm_dropin.so")
+DRIVER = os.path.join(HERE, "SubgraphMatching.out")
+
+
+def build_dropin() -> None:
+    """The reference's C++ call surfaces over the C ABI (host C++ only)."""
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + DROPIN]
+    common = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wno-unused-result", *inc]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    subprocess.check_call(["g++", *common, "-shared", "-o", DROPIN_LIB + ".tmp",
+                           os.path.join(DROPIN, "psm_dropin.cpp"), "-L" + HERE, "-lpsm", rpath])
+    os.replace(DROPIN_LIB + ".tmp", DROPIN_LIB)
+    subprocess.check_call(["g++", *common, "-o", DRIVER + ".tmp", os.path.join(DROPIN, "subgraph_matching.cpp"),
+                           "-L" + HERE, "-lpsm_dropin", "-lpsm", rpath])
+    os.replace(DRIVER + ".tmp", DRIVER)
 
 
 if __name__ == "__main__":
