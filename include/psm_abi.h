@@ -134,6 +134,22 @@ int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t*
  * limit (psm_set_time_limit) stops it. */
 int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* stats);
 
+/* Query decomposition + join: JoinCollect::sequence_collect over two units
+ * (JoinCollect.cpp:60-153,237-315; NoPartitionHashJoin hashjoin.cpp:337-781).
+ * Unit A is the first ceil(n/2) vertices of the generated plan's order, unit
+ * B every query edge leaving A; both are enumerated on the GPU into rows and
+ * joined by a device hash join on the shared vertices with the injectivity
+ * check across units. count equals the plain count (no cap). Needs
+ * psm_generate_plan. max_rows bounds each unit (0: 64M). */
+typedef struct psm_join_stats {
+    uint64_t count;
+    uint64_t rows_a, rows_b;
+    uint32_t keys;
+    double enum_ms, join_ms;
+} psm_join_stats;
+
+int psm_join_count(psm_query* q, uint64_t max_rows, psm_join_stats* stats);
+
 /* Convenience: filter + tables + plan + evaluate in one call, the sequence of
  * split_C_test (StudyPerformance.cpp:1164-1716). */
 int psm_match(psm_query* q, int filter, int order, uint64_t cap, psm_enum_stats* stats);

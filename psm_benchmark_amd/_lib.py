@@ -52,6 +52,17 @@ class BatchStats(ctypes.Structure):
     ]
 
 
+class JoinStats(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_uint64),
+        ("rows_a", ctypes.c_uint64),
+        ("rows_b", ctypes.c_uint64),
+        ("keys", ctypes.c_uint32),
+        ("enum_ms", ctypes.c_double),
+        ("join_ms", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/psm_abi.h declares
 SIGNATURES = {
     "psm_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
@@ -73,6 +84,7 @@ SIGNATURES = {
     "psm_memory_cost_bytes": (ctypes.c_int, [VP, U64P]),
     "psm_generate_plan": (ctypes.c_int, [VP, ctypes.c_int, U32P, U32P]),
     "psm_evaluate": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
+    "psm_join_count": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(JoinStats)]),
     "psm_match": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
     "psm_batch_match": (ctypes.c_int, [VP, VP, ctypes.c_uint32, U32P, U64P, U64P, U64P, U32P, U32P, U32P,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_uint64, U64P, U64P, U32P,

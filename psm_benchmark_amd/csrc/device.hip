@@ -589,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* ready, DQueue* Q,
-    QCounters* qc, uint32_t max_n, uint32_t* lane_state) {
+    QCounters* qc, uint32_t max_n, uint32_t* lane_state, EmitSink sink) {
     const uint32_t lane = lane_id();
     const size_t wave = ((size_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
     const size_t per_field = (size_t)gridDim.x * (kBlock / kWave) * max_n * kWave;
@@ -690,6 +690,17 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 ++lcalls;
                 if (d + 1 == n) {
                     ++lcount;
+                    if (sink.rows) {
+                        // materialise the embedding (one row of data vertex ids in depth order)
+                        const unsigned long long r = atomicAdd(sink.count, 1ull);
+                        if (r < sink.cap) {
+                            uint32_t* row = sink.rows + r * n;
+                            for (uint32_t e = 0; e < d; ++e) row[e] = st.M_[st.at(e)];
+                            row[d] = v;
+                        } else {
+                            atomicExch(sink.overflow, 1u);
+                        }
+                    }
                 } else {
                     const size_t a = st.at(d);
                     st.cur_[a] = cur;
@@ -769,6 +780,76 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         if (!active) atomicSub(&Q->busy, 1u);
     }
     if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);   // bytes of tasks dropped at claim
+}
+
+// ---------------------------------------------------------------------------
+// Two-unit hash join (JoinCollect.cpp:60-153 + hashjoin.cpp:337-781 restated):
+// rows of the build unit are chained under a 64-bit fingerprint of their key
+// columns in an open-addressing table; each probe row walks the chain of its
+// fingerprint, checks the key columns exactly and counts the build rows whose
+// non-key data vertices are disjoint from its own (injectivity across units).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long key_fingerprint(const uint32_t* row, const JoinKeys& k) {
+    unsigned long long h = 0x9E3779B97F4A7C15ull ^ k.nkeys;
+    for (uint32_t i = 0; i < k.nkeys; ++i) {
+        h ^= row[k.key[i]] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 31;
+    }
+    return h | 1ull;   // 0 marks an empty slot
+}
+
+__global__ __launch_bounds__(kBlock) void k_join_build(const uint32_t* __restrict__ rows, uint64_t n_rows,
+                                                        uint32_t width, JoinKeys keys,
+                                                        unsigned long long* fp, uint32_t* head,
+                                                        uint64_t mask, uint32_t* next) {
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n_rows;
+         r += (uint64_t)gridDim.x * kBlock) {
+        const unsigned long long h = key_fingerprint(rows + r * width, keys);
+        uint64_t slot = h & mask;
+        while (true) {
+            const unsigned long long prev = atomicCAS(fp + slot, 0ull, h);
+            if (prev == 0ull || prev == h) break;
+            slot = (slot + 1) & mask;
+        }
+        next[r] = atomicExch(head + slot, (uint32_t)r);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_join_probe(
+    const uint32_t* __restrict__ build_rows, uint32_t build_width, JoinKeys bk,
+    const uint32_t* __restrict__ probe_rows, uint64_t n_probe, uint32_t probe_width, JoinKeys pk,
+    const unsigned long long* __restrict__ fp, const uint32_t* __restrict__ head, uint64_t mask,
+    const uint32_t* __restrict__ next, unsigned long long* count, unsigned long long* bytes) {
+    unsigned long long c = 0, by = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const uint64_t rounded = (n_probe + stride - 1) / stride * stride;
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < rounded; r += stride) {
+        if (r >= n_probe) continue;
+        const uint32_t* pr = probe_rows + r * probe_width;
+        const unsigned long long h = key_fingerprint(pr, pk);
+        by += 4ull * probe_width;
+        uint64_t slot = h & mask;
+        uint32_t a = kInvalid;
+        while (true) {
+            const unsigned long long f = fp[slot];
+            by += 8;
+            if (f == 0ull) break;
+            if (f == h) { a = head[slot]; break; }
+            slot = (slot + 1) & mask;
+        }
+        for (; a != kInvalid; a = next[a]) {
+            const uint32_t* br = build_rows + (uint64_t)a * build_width;
+            by += 4ull * build_width + 4;
+            bool ok = true;
+            for (uint32_t i = 0; i < pk.nkeys && ok; ++i) ok = br[bk.key[i]] == pr[pk.key[i]];
+            for (uint32_t i = 0; i < bk.nrest && ok; ++i)
+                for (uint32_t j = 0; j < pk.nrest && ok; ++j) ok = br[bk.rest[i]] != pr[pk.rest[j]];
+            c += ok;
+        }
+    }
+    wave_add(count, c);
+    wave_add(bytes, by);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -1049,11 +1130,32 @@ uint32_t dfs_lane_state_words(uint32_t max_n) {
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
                       const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
                       DQueue* dq, QCounters* qc, uint32_t max_n, uint32_t grid_waves,
-                      uint32_t* lane_state, hipStream_t s) {
+                      uint32_t* lane_state, EmitSink sink, hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
     const uint32_t blocks = grid_waves / (kBlock / kWave);
     hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(kBlock), 0, s, plans, ops, row_off, idx, cand_pool,
-                       queue, ready, dq, qc, max_n, lane_state);
+                       queue, ready, dq, qc, max_n, lane_state, sink);
+    return hipGetLastError();
+}
+
+hipError_t launch_join_build(const uint32_t* rows, uint64_t n_rows, uint32_t width, JoinKeys keys,
+                             unsigned long long* fp, uint32_t* head, uint64_t mask, uint32_t* next,
+                             hipStream_t s) {
+    if (n_rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_join_build, dim3(grid_for(n_rows)), dim3(kBlock), 0, s, rows, n_rows, width, keys,
+                       fp, head, mask, next);
+    return hipGetLastError();
+}
+
+hipError_t launch_join_probe(const uint32_t* build_rows, uint32_t build_width, JoinKeys build_keys,
+                             const uint32_t* probe_rows, uint64_t n_probe, uint32_t probe_width,
+                             JoinKeys probe_keys, const unsigned long long* fp, const uint32_t* head,
+                             uint64_t mask, const uint32_t* next, unsigned long long* count,
+                             unsigned long long* bytes, hipStream_t s) {
+    if (n_probe == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_join_probe, dim3(grid_for(n_probe)), dim3(kBlock), 0, s, build_rows, build_width,
+                       build_keys, probe_rows, n_probe, probe_width, probe_keys, fp, head, mask, next, count,
+                       bytes);
     return hipGetLastError();
 }
 

@@ -124,6 +124,24 @@ struct DQueue {
     unsigned long long deadline;      // set by the first lane to start
 };
 
+// Where the DFS writes complete embeddings (rows of n data vertex ids in
+// depth order) when materialising a join unit; rows == nullptr: count only.
+struct EmitSink {
+    uint32_t* rows;
+    unsigned long long* count;
+    unsigned long long cap;
+    uint32_t* overflow;
+};
+
+// Column roles of one join side: key columns (shared query vertices, in the
+// same key order on both sides) and the remaining columns.
+struct JoinKeys {
+    uint32_t nkeys;
+    uint32_t nrest;
+    uint32_t key[kMaxQ];
+    uint32_t rest[kMaxQ];
+};
+
 // Per-query enumeration counters.
 struct QCounters {
     unsigned long long count;
@@ -143,7 +161,8 @@ enum KernelKind : int {
     kKExpand = 4,
     kKCompact = 5,
     kKBitmap = 6,
-    kKNumKinds = 7,
+    kKJoin = 7,
+    kKNumKinds = 8,
 };
 
 // ---- kernel launchers (device.hip) --------------------------------------
@@ -187,7 +206,15 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
                       const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
                       DQueue* dq, QCounters* qc, uint32_t max_n, uint32_t grid_waves,
-                      uint32_t* lane_state, hipStream_t s);
+                      uint32_t* lane_state, EmitSink sink, hipStream_t s);
+hipError_t launch_join_build(const uint32_t* rows, uint64_t n_rows, uint32_t width, JoinKeys keys,
+                             unsigned long long* fp, uint32_t* head, uint64_t mask, uint32_t* next,
+                             hipStream_t s);
+hipError_t launch_join_probe(const uint32_t* build_rows, uint32_t build_width, JoinKeys build_keys,
+                             const uint32_t* probe_rows, uint64_t n_probe, uint32_t probe_width,
+                             JoinKeys probe_keys, const unsigned long long* fp, const uint32_t* head,
+                             uint64_t mask, const uint32_t* next, unsigned long long* count,
+                             unsigned long long* bytes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 uint32_t dfs_lane_state_words(uint32_t max_n);
 uint64_t scan_tmp_entries(uint64_t n);

@@ -264,6 +264,12 @@ struct QueryHost {
 
 constexpr uint64_t kNoTable = ~0ull;
 
+struct JoinResult {
+    uint64_t rows_a = 0, rows_b = 0, count = 0;
+    uint32_t keys = 0;
+    double enum_ms = 0, join_ms = 0;
+};
+
 struct Batch {
     psm_ctx* ctx = nullptr;
     const psm_graph* g = nullptr;
@@ -295,8 +301,10 @@ struct Batch {
     int cfl_filter(std::vector<QueryHost*>& active);
     int build_tables();
     int make_plans(int order_method);
-    int upload_plans(uint64_t cap);
+    int run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSink sink,
+                  std::vector<QCounters>& qc, bool& timed_out);
     int evaluate(uint64_t cap);
+    int join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr);
     int host_table(const QueryHost& q, uint32_t a, uint32_t b, std::vector<uint32_t>& off,
                    std::vector<uint32_t>& idxv);
     int nlf_count(const QGraph& q, uint32_t u, uint32_t label_override, uint32_t& out);
@@ -857,54 +865,51 @@ int Batch::make_plans(int order_method) {
     return PSM_OK;
 }
 
-int Batch::upload_plans(uint64_t cap) {
-    std::vector<DPlan> hp(qs.size());
-    std::vector<DOp> hops;
-    for (size_t i = 0; i < qs.size(); ++i) {
-        QueryHost* q = qs[i];
-        DPlan& P = hp[i];
-        memset(&P, 0, sizeof(P));
-        const uint32_t n = q->qg.n;
-        P.n = n;
-        P.cap = cap ? cap : ~0ull;
-        std::vector<uint32_t> pos(n);
-        for (uint32_t d = 0; d < n; ++d) pos[q->plan.order[d]] = d;
-        for (uint32_t d = 0; d < n; ++d) {
-            const uint32_t u = q->plan.order[d];
-            P.cand_base[d] = q->cand_begin[u];
-            P.cand_len[d] = q->cand_count[u];
-            P.op_begin[d] = (uint32_t)hops.size();
-            for (uint32_t k = q->qg.off[u]; k < q->qg.off[u + 1]; ++k) {
-                const uint32_t b = q->qg.adj[k];
-                if (pos[b] >= d) continue;
-                DOp op{};
-                op.src_depth = pos[b];
-                op.row_base = q->tab_row[(size_t)b * n + u];
-                hops.push_back(op);
-            }
-            P.op_count[d] = (uint32_t)hops.size() - P.op_begin[d];
-            uint64_t m = 0;
-            for (uint32_t e = 0; e < d; ++e)
-                if (q->qg.lab[q->plan.order[e]] == q->qg.lab[u]) m |= 1ull << e;
-            P.same_label[d] = m;
+// Device plan of one enumeration unit: `order` lists query vertices by depth;
+// the operands of a depth are its earlier neighbours over the unit's edges
+// (all query edges for a whole query).
+void make_plan(const QueryHost& q, const std::vector<uint32_t>& order,
+               const std::vector<uint8_t>* unit_edge, uint64_t cap, DPlan& P, std::vector<DOp>& ops) {
+    memset(&P, 0, sizeof(P));
+    const uint32_t n = q.qg.n;
+    const uint32_t m = (uint32_t)order.size();
+    P.n = m;
+    P.cap = cap ? cap : ~0ull;
+    std::vector<uint32_t> pos(n, kInvalid);
+    for (uint32_t d = 0; d < m; ++d) pos[order[d]] = d;
+    for (uint32_t d = 0; d < m; ++d) {
+        const uint32_t u = order[d];
+        P.cand_base[d] = q.cand_begin[u];
+        P.cand_len[d] = q.cand_count[u];
+        P.op_begin[d] = (uint32_t)ops.size();
+        for (uint32_t k = q.qg.off[u]; k < q.qg.off[u + 1]; ++k) {
+            const uint32_t b = q.qg.adj[k];
+            if (pos[b] == kInvalid || pos[b] >= d) continue;
+            if (unit_edge && !(*unit_edge)[(size_t)b * n + u]) continue;
+            DOp op{};
+            op.src_depth = pos[b];
+            op.row_base = q.tab_row[(size_t)b * n + u];
+            ops.push_back(op);
         }
+        P.op_count[d] = (uint32_t)ops.size() - P.op_begin[d];
+        uint64_t msk = 0;
+        for (uint32_t e = 0; e < d; ++e)
+            if (q.qg.lab[order[e]] == q.qg.lab[u]) msk |= 1ull << e;
+        P.same_label[d] = msk;
     }
-    if (hops.empty()) hops.push_back(DOp{});
-    HIP_TRY(ctx, upload(plans, hp, s()));
-    HIP_TRY(ctx, upload(ops, hops, s()));
-    return PSM_OK;
 }
 
 // ---- enumeration ------------------------------------------------------------
-// Root tasks (one per query) are pre-expanded level by level until each query
+// Root tasks (one per plan) are pre-expanded level by level until each plan
 // has enough tasks to spread over the chip, then one persistent DFS kernel
 // drains a shared work queue that running lanes refill by splitting their
-// ranges.
-int Batch::evaluate(uint64_t cap) {
-    for (QueryHost* q : qs)
-        if (!q->planned) return set_err(ctx, PSM_ESTATE, "generate a plan before evaluating");
-    PSM_TRY(upload_plans(cap));
-    const uint32_t nq = (uint32_t)qs.size();
+// ranges. `sink` optionally materialises the embeddings (one plan only).
+int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSink sink,
+                     std::vector<QCounters>& qc, bool& timed_out) {
+    const uint32_t nq = (uint32_t)hp.size();
+    if (hops.empty()) hops.push_back(DOp{});
+    HIP_TRY(ctx, upload(plans, hp, s()));
+    HIP_TRY(ctx, upload(ops, hops, s()));
     HIP_TRY(ctx, qcnt.alloc(sizeof(QCounters) * nq));
     HIP_TRY(ctx, hipMemsetAsync(qcnt.p, 0, sizeof(QCounters) * nq, s()));
 
@@ -913,9 +918,8 @@ int Batch::evaluate(uint64_t cap) {
     std::vector<uint64_t> ntask(nq, 0);
     uint32_t max_n = 1;
     for (uint32_t i = 0; i < nq; ++i) {
-        QueryHost* q = qs[i];
-        max_n = std::max(max_n, q->qg.n);
-        if (q->cand_count[q->plan.order[0]] == 0) continue;
+        max_n = std::max(max_n, hp[i].n);
+        if (hp[i].n == 0 || hp[i].cand_len[0] == 0) continue;
         DTask t{};
         t.q = i;
         t.hi = kAutoHi;
@@ -939,7 +943,7 @@ int Batch::evaluate(uint64_t cap) {
         bool any = false;
         for (uint32_t i = 0; i < nq; ++i) {
             if (!ntask[i]) continue;
-            const uint32_t lim = std::min<uint32_t>(qs[i]->qg.n - 1, kExpandMax);
+            const uint32_t lim = std::min<uint32_t>(hp[i].n - 1, kExpandMax);
             if (ntask[i] < per_q_target && depth[i] < lim) { ex[i] = 1; any = true; }
         }
         if (!any) break;
@@ -980,7 +984,8 @@ int Batch::evaluate(uint64_t cap) {
     t_expand += now_ms() - te0;
     verbose("expand: %llu tasks in %.1f ms", (unsigned long long)n_tasks, now_ms() - te0);
 
-    unsigned int timed_out = 0, donated = 0;
+    unsigned int donated = 0;
+    timed_out = false;
     if (n_tasks) {
         // queue: the expanded tasks followed by room for split-off work
         const uint64_t cap_slots = std::min<uint64_t>(n_tasks + (4u << 20), 0xFFFFFFF0ull);
@@ -1002,20 +1007,33 @@ int Batch::evaluate(uint64_t cap) {
         TIMED(ctx, kKDfs,
               launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
                          cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_ready.as<uint32_t>(),
-                         d_q.as<DQueue>(), qcnt.as<QCounters>(), max_n, gw, d_state.as<uint32_t>(), s()));
+                         d_q.as<DQueue>(), qcnt.as<QCounters>(), max_n, gw, d_state.as<uint32_t>(), sink, s()));
         HIP_TRY(ctx, hipMemcpyAsync(&hq, d_q.p, sizeof(hq), hipMemcpyDeviceToHost, s()));
         HIP_TRY(ctx, timers_flush(ctx));
         t_enum += now_ms() - td0;
-        timed_out = hq.timeout;
+        timed_out = hq.timeout != 0;
         donated = hq.donated;
-        verbose("dfs: %.1f ms, %u tasks split off, timeout %u", now_ms() - td0, donated, timed_out);
+        verbose("dfs: %.1f ms, %u tasks split off, timeout %u", now_ms() - td0, donated, hq.timeout);
     }
     tasks_run += n_tasks + donated;
-    std::vector<QCounters> qc;
     HIP_TRY(ctx, download(qc, qcnt.p, nq, s()));
     for (uint32_t i = 0; i < nq; ++i) {
         ctx->kstat[kKDfs].bytes += (double)qc[i].probe_bytes;
         ctx->kstat[kKExpand].bytes += (double)qc[i].expand_bytes;
+    }
+    return PSM_OK;
+}
+
+int Batch::evaluate(uint64_t cap) {
+    for (QueryHost* q : qs)
+        if (!q->planned) return set_err(ctx, PSM_ESTATE, "generate a plan before evaluating");
+    std::vector<DPlan> hp(qs.size());
+    std::vector<DOp> hops;
+    for (size_t i = 0; i < qs.size(); ++i) make_plan(*qs[i], qs[i]->plan.order, nullptr, cap, hp[i], hops);
+    std::vector<QCounters> qc;
+    bool timed_out = false;
+    PSM_TRY(run_plans(hp, hops, EmitSink{nullptr, nullptr, 0, nullptr}, qc, timed_out));
+    for (size_t i = 0; i < qs.size(); ++i) {
         QueryHost* q = qs[i];
         q->count = cap ? std::min<uint64_t>(cap, qc[i].count) : qc[i].count;
         q->capped = cap && qc[i].count >= cap;
@@ -1024,6 +1042,129 @@ int Batch::evaluate(uint64_t cap) {
         q->timed_out = qc[i].timed_out != 0;
     }
     if (timed_out) set_err(ctx, PSM_ETIMEOUT, "enumeration passed the time limit");
+    return PSM_OK;
+}
+
+// ---- two-unit decomposition + hash join --------------------------------------
+// Unit A: the first ceil(n/2) vertices of the matching order (a connected
+// prefix) with the query edges among them. Unit B: the remaining vertices and
+// every query edge with an endpoint outside A, plus the A vertices those edges
+// reach (the keys). Every query edge is in exactly one unit, so joining the
+// units' embeddings on the keys, with data vertices distinct across the units,
+// gives exactly the query's embeddings.
+int Batch::join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr) {
+    const uint32_t n = q.qg.n;
+    jr = JoinResult();
+    if (!q.planned) return set_err(ctx, PSM_ESTATE, "generate a plan before joining");
+    const uint32_t k = (n + 1) / 2;
+    std::vector<uint8_t> inA(n, 0), inB(n, 0), eA((size_t)n * n, 0), eB((size_t)n * n, 0);
+    for (uint32_t d = 0; d < k; ++d) inA[q.plan.order[d]] = 1;
+    for (uint32_t u = 0; u < n; ++u)
+        for (uint32_t i = q.qg.off[u]; i < q.qg.off[u + 1]; ++i) {
+            const uint32_t w = q.qg.adj[i];
+            if (inA[u] && inA[w]) eA[(size_t)u * n + w] = 1;
+            else { eB[(size_t)u * n + w] = 1; inB[u] = inB[w] = 1; }
+        }
+    std::vector<uint32_t> orderA(q.plan.order.begin(), q.plan.order.begin() + k), orderB;
+    // unit B order: greedy by candidate count, preferring vertices adjacent
+    // (over unit-B edges) to those already placed
+    {
+        std::vector<uint8_t> placed(n, 0);
+        uint32_t nb = 0;
+        for (uint32_t u = 0; u < n; ++u) nb += inB[u];
+        while (orderB.size() < nb) {
+            uint32_t best = kInvalid;
+            bool best_adj = false;
+            for (uint32_t u = 0; u < n; ++u) {
+                if (!inB[u] || placed[u]) continue;
+                bool adj = false;
+                for (uint32_t w : orderB) adj |= eB[(size_t)w * n + u] != 0;
+                if (best == kInvalid || (adj && !best_adj) ||
+                    (adj == best_adj && q.cand_count[u] < q.cand_count[best])) {
+                    best = u;
+                    best_adj = adj;
+                }
+            }
+            placed[best] = 1;
+            orderB.push_back(best);
+        }
+    }
+    jr.rows_a = jr.rows_b = 0;
+    jr.keys = 0;
+    if (orderB.empty()) return set_err(ctx, PSM_EINVAL, "query too small to split");
+    // key columns (shared vertices, ascending query id) and the rest
+    std::vector<uint32_t> posA(n, kInvalid), posB(n, kInvalid);
+    for (uint32_t d = 0; d < orderA.size(); ++d) posA[orderA[d]] = d;
+    for (uint32_t d = 0; d < orderB.size(); ++d) posB[orderB[d]] = d;
+    JoinKeys ka{}, kb{};
+    for (uint32_t u = 0; u < n; ++u) {
+        if (inA[u] && inB[u]) {
+            ka.key[ka.nkeys++] = posA[u];
+            kb.key[kb.nkeys++] = posB[u];
+        } else if (inA[u]) {
+            ka.rest[ka.nrest++] = posA[u];
+        } else if (inB[u]) {
+            kb.rest[kb.nrest++] = posB[u];
+        }
+    }
+    jr.keys = ka.nkeys;
+    // materialise both units
+    DBuf rows[2], cnt, ovf;
+    HIP_TRY(ctx, cnt.alloc(8));
+    HIP_TRY(ctx, ovf.alloc(4));
+    const std::vector<uint32_t>* orders[2] = {&orderA, &orderB};
+    const std::vector<uint8_t>* edges[2] = {&eA, &eB};
+    uint64_t nrows[2] = {0, 0};
+    const double t0 = now_ms();
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t w = (uint32_t)orders[side]->size();
+        HIP_TRY(ctx, rows[side].alloc(std::max<uint64_t>(max_rows, 1) * w * 4));
+        HIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, 8, s()));
+        HIP_TRY(ctx, hipMemsetAsync(ovf.p, 0, 4, s()));
+        std::vector<DPlan> hp(1);
+        std::vector<DOp> hops;
+        make_plan(q, *orders[side], edges[side], 0, hp[0], hops);
+        std::vector<QCounters> qc;
+        bool to = false;
+        EmitSink sink{rows[side].as<uint32_t>(), cnt.as<unsigned long long>(), max_rows, ovf.as<uint32_t>()};
+        PSM_TRY(run_plans(hp, hops, sink, qc, to));
+        uint32_t overflow = 0;
+        HIP_TRY(ctx, hipMemcpy(&nrows[side], cnt.p, 8, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(&overflow, ovf.p, 4, hipMemcpyDeviceToHost));
+        if (to) return set_err(ctx, PSM_ETIMEOUT, "join unit enumeration passed the time limit");
+        if (overflow) return set_err(ctx, PSM_ENOMEM, "join unit has more rows than max_rows");
+    }
+    jr.rows_a = nrows[0];
+    jr.rows_b = nrows[1];
+    jr.enum_ms = now_ms() - t0;
+    // build on the smaller side (hashjoin.cpp:60-69), probe with the other
+    const double t1 = now_ms();
+    const int bs = nrows[0] <= nrows[1] ? 0 : 1, ps = 1 - bs;
+    const JoinKeys& bk = bs == 0 ? ka : kb;
+    const JoinKeys& pk = bs == 0 ? kb : ka;
+    uint64_t slots = 16;
+    while (slots < 2 * nrows[bs]) slots <<= 1;
+    DBuf fp, head, next, out;
+    HIP_TRY(ctx, fp.alloc(slots * 8));
+    HIP_TRY(ctx, hipMemsetAsync(fp.p, 0, slots * 8, s()));
+    HIP_TRY(ctx, head.alloc(slots * 4));
+    HIP_TRY(ctx, launch_fill_u32(head.as<uint32_t>(), kInvalid, slots, s()));
+    HIP_TRY(ctx, next.alloc(std::max<uint64_t>(nrows[bs], 1) * 4));
+    HIP_TRY(ctx, out.alloc(8));
+    HIP_TRY(ctx, hipMemsetAsync(out.p, 0, 8, s()));
+    TIMED(ctx, kKJoin,
+          launch_join_build(rows[bs].as<uint32_t>(), nrows[bs], (uint32_t)orders[bs]->size(), bk,
+                            fp.as<unsigned long long>(), head.as<uint32_t>(), slots - 1, next.as<uint32_t>(), s()));
+    TIMED(ctx, kKJoin,
+          launch_join_probe(rows[bs].as<uint32_t>(), (uint32_t)orders[bs]->size(), bk, rows[ps].as<uint32_t>(),
+                            nrows[ps], (uint32_t)orders[ps]->size(), pk, fp.as<unsigned long long>(),
+                            head.as<uint32_t>(), slots - 1, next.as<uint32_t>(), out.as<unsigned long long>(),
+                            ctx->d_kbytes + kKJoin, s()));
+    HIP_TRY(ctx, hipMemcpyAsync(&jr.count, out.p, 8, hipMemcpyDeviceToHost, s()));
+    HIP_TRY(ctx, timers_flush(ctx));
+    jr.join_ms = now_ms() - t1;
+    verbose("join: %llu x %llu rows on %u keys -> %llu in %.1f ms", (unsigned long long)nrows[0],
+            (unsigned long long)nrows[1], jr.keys, (unsigned long long)jr.count, jr.join_ms);
     return PSM_OK;
 }
 
@@ -1283,6 +1424,20 @@ int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
         st->probe_bytes = q->qh.probe_bytes;
     }
     return r;
+}
+
+int psm_join_count(psm_query* q, uint64_t max_rows, psm_join_stats* st) {
+    if (!q || !st) return PSM_EINVAL;
+    (void)hipSetDevice(q->ctx->device);
+    JoinResult jr;
+    PSM_TRY(q->batch.join_count(q->qh, max_rows ? max_rows : (64ull << 20), jr));
+    st->count = jr.count;
+    st->rows_a = jr.rows_a;
+    st->rows_b = jr.rows_b;
+    st->keys = jr.keys;
+    st->enum_ms = jr.enum_ms;
+    st->join_ms = jr.join_ms;
+    return PSM_OK;
 }
 
 int psm_match(psm_query* q, int filter, int order, uint64_t cap, psm_enum_stats* st) {

@@ -152,6 +152,12 @@ class Query:
                "psm_generate_plan")
         return o, p
 
+    def join_count(self, max_rows: int = 0) -> L.JoinStats:
+        """Two-unit decomposition of the planned query + device hash join."""
+        st = L.JoinStats()
+        _check(self.dg.dev.h, L.lib().psm_join_count(self.h, max_rows, ctypes.byref(st)), "psm_join_count")
+        return st
+
     def evaluate(self, cap: int = 0) -> L.EnumStats:
         """Counts embeddings; st.timed_out is set when the time limit cut it short."""
         st = L.EnumStats()
@@ -191,6 +197,20 @@ def match(dg: DeviceGraph, q: CSRGraph, method: str = "GQL_GQL_GQL", cap: int = 
         return MatchResult(int(st.count), int(st.calls), int(mem), ok, [int(x) for x in order],
                            bool(st.capped), float(st.kernel_ms + st.expand_ms), int(st.probe_bytes),
                            [int(x) for x in run.candidate_counts()], bool(st.timed_out))
+    finally:
+        run.close()
+
+
+def join_match(dg: DeviceGraph, q: CSRGraph, method: str = "CFL_CFL_EXPLORE", max_rows: int = 0):
+    """Filter, tables and plan as the plain path, then the decomposed count."""
+    parse_method(method)
+    f, o, _ = method.split("_")
+    run = Query(dg, q)
+    try:
+        run.filter(f)
+        run.build_tables()
+        run.generate_plan(o)
+        return run.join_count(max_rows)
     finally:
         run.close()
 

@@ -136,3 +136,42 @@ def test_batch_equals_golden(hprd):
     counts, calls, status, st = batch_match(dg, qs, "CFL", "CFL", 0)
     assert counts.tolist() == [r["embeddings"] for r in recs]
     assert calls.tolist() == [r["calls"] for r in recs]
+
+
+@pytest.mark.parametrize("method", ["GQL_GQL_GQL", "CFL_CFL_EXPLORE"])
+def test_join_equals_plain_on_hprd(hprd, method):
+    """Two-unit decomposition + device hash join counts every HPRD query's
+    embeddings exactly (the reference's Q-split result, JoinCollect.cpp)."""
+    from psm_benchmark_amd.matcher import join_match
+    _, dg, _ = hprd
+    for rec in golden((method,)):
+        q = load_text(os.path.join(HPRD, rec["query"] + ".txt"))
+        st = join_match(dg, q, method)
+        print(rec["query"], st.rows_a, st.rows_b, st.keys, st.count, flush=True)
+        assert st.count == rec["embeddings"], rec["query"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_join_bruteforce_tiny(dev, seed):
+    from psm_benchmark_amd.matcher import DeviceGraph, join_match
+    rng = np.random.default_rng(200 + seed)
+    data = random_graph(rng, 12, 0.45, 2)
+    q = random_connected_query(rng, data, int(rng.integers(3, 6)))
+    dg = DeviceGraph(dev, data)
+    assert join_match(dg, q, "GQL_GQL_GQL").count == count_embeddings(data, q)
+
+
+def test_time_limit_reports_partial_count(hprd):
+    """A zero-length time budget stops enumeration and reports PSM_ETIMEOUT
+    with a partial count, as the reference logs Overtime: 1."""
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.matcher import match
+    _, dg, _ = hprd
+    q = load_text(os.path.join(HPRD, "Q10-10.txt"))
+    L.lib().psm_set_time_limit(dg.dev.h, 1e-9)
+    try:
+        r = match(dg, q, "GQL_GQL_GQL")
+    finally:
+        L.lib().psm_set_time_limit(dg.dev.h, 60.0)
+    assert r.timed_out and r.count < 1015922
+    assert match(dg, q, "GQL_GQL_GQL").count == 1015922
