@@ -998,7 +998,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         DQueue hq{};
         hq.tail = (unsigned int)n_tasks;
         hq.cap = (unsigned int)cap_slots;
-        hq.limit_ticks = ctx->time_limit_s > 0 ? (unsigned long long)(ctx->time_limit_s * ctx->wall_khz * 1000.0) : 0;
+        hq.limit_ticks = ctx->time_limit_s > 0 ? std::max<unsigned long long>(1, (unsigned long long)(ctx->time_limit_s * ctx->wall_khz * 1000.0)) : 0;
         HIP_TRY(ctx, d_q.alloc(sizeof(DQueue)));
         HIP_TRY(ctx, hipMemcpyAsync(d_q.p, &hq, sizeof(hq), hipMemcpyHostToDevice, s()));
         const uint32_t gw = grid_waves_full / 4 * 4;
