@@ -32,6 +32,9 @@ void check(int r, const char* what) {
 }
 
 psm_query* session(const Graph* data, const Graph* query, bool fresh) {
+    // context() and device() take g_mu themselves: resolve them first
+    psm_ctx* ctx = psm_dropin::context();
+    psm_graph* dg = data->device();
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_sessions.find(query);
     if (it != g_sessions.end() && !fresh) return it->second;
@@ -44,9 +47,11 @@ psm_query* session(const Graph* data, const Graph* query, bool fresh) {
     const auto& a = query->neighbors();
     const auto& l = query->labels();
     ui dummy = 0;
-    check(psm_query_create(psm_dropin::context(), data->device(), query->getVerticesCount(), o.data(),
-                           a.empty() ? &dummy : a.data(), l.data(), &q),
-          "psm_query_create");
+    const int r = psm_query_create(ctx, dg, query->getVerticesCount(), o.data(),
+                                   a.empty() ? &dummy : a.data(), l.data(), &q);
+    if (r != PSM_OK)
+        throw std::runtime_error("psm_query_create failed with status " + std::to_string(r) + ": " +
+                                 psm_last_error(ctx));
     g_sessions[query] = q;
     return q;
 }
