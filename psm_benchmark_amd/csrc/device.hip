@@ -645,7 +645,14 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             const DTask task = queue[wslot];
             q = task.q;
             P = plans + q;
-            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout)) {
+            bool late = load_relaxed(&Q->timeout) != 0;
+            if (!late && Q->limit_ticks &&
+                (unsigned long long)wall_clock64() >
+                    __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                atomicExch(&Q->timeout, 1u);
+                late = true;
+            }
+            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || late) {
                 if (!__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) atomicExch(&qc[q].timed_out, 1u);
                 atomicSub(&Q->busy, 1u);
             } else {
