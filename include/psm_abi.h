@@ -13,6 +13,7 @@
  *   - One psm_ctx owns one HIP stream; calls on one context are serialised
  *     by the caller, calls on different contexts may run concurrently.
  *   - The text of the last error of a context is in psm_last_error().
+ *   - A context must outlive the graphs and queries created with it.
  */
 #ifndef PSM_ABI_H
 #define PSM_ABI_H
@@ -175,6 +176,39 @@ int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t num_queries, cons
                     const uint32_t* q_offsets, const uint32_t* q_neighbors, const uint32_t* q_labels,
                     int filter, int order, uint64_t cap, uint64_t* counts, uint64_t* calls,
                     uint32_t* status, psm_batch_stats* stats);
+
+/* ---- serving form ----------------------------------------------------------
+ * Continuous batching over a resident data graph: `workers` threads, each with
+ * its own HIP stream, admit the oldest pending requests as one batch while the
+ * batch holds fewer than max_batch requests and the estimated device
+ * footprint of running batches stays within mem_budget (0: half the free
+ * device memory); later arrivals join the next batch a worker forms, and
+ * batches of different workers overlap on the device. Every batch runs
+ * filter, tables, plan and the persistent enumeration with `cap` and the
+ * time limit. Times in psm_result are milliseconds since server creation.
+ * No counterpart in the reference, which runs one query per process. */
+typedef struct psm_server psm_server;
+typedef struct psm_result {
+    uint64_t request_id;
+    uint64_t count;
+    uint64_t calls;
+    int32_t status;           /* PSM_OK, PSM_EEMPTY, PSM_ETIMEOUT or an error */
+    uint32_t batch_size;
+    double submit_ms, start_ms, done_ms;
+} psm_result;
+
+int psm_server_create(psm_ctx* ctx, const psm_graph* g, uint32_t workers, uint32_t max_batch,
+                      uint64_t mem_budget, int filter, int order, uint64_t cap, double time_limit_s,
+                      psm_server** out);
+int psm_server_submit(psm_server* s, uint64_t request_id, uint32_t n, const uint32_t* q_offsets,
+                      const uint32_t* q_neighbors, const uint32_t* q_labels);
+/* Moves up to max_results finished results to out; waits up to wait_ms for
+ * the first one when none is ready. */
+int psm_server_poll(psm_server* s, psm_result* out, uint32_t max_results, uint32_t* n_out,
+                    double wait_ms);
+/* submitted, completed, batches formed, peak admitted footprint, pending */
+int psm_server_stats(psm_server* s, uint64_t* out5);
+int psm_server_destroy(psm_server* s);
 
 /* ---- seeded generators (host) -------------------------------------------
  * Data graphs for the benchmark configurations. offsets has V+1 entries,

@@ -63,6 +63,19 @@ class JoinStats(ctypes.Structure):
     ]
 
 
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("request_id", ctypes.c_uint64),
+        ("count", ctypes.c_uint64),
+        ("calls", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("batch_size", ctypes.c_uint32),
+        ("submit_ms", ctypes.c_double),
+        ("start_ms", ctypes.c_double),
+        ("done_ms", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/psm_abi.h declares
 SIGNATURES = {
     "psm_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
@@ -89,6 +102,13 @@ SIGNATURES = {
     "psm_batch_match": (ctypes.c_int, [VP, VP, ctypes.c_uint32, U32P, U64P, U64P, U64P, U32P, U32P, U32P,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_uint64, U64P, U64P, U32P,
                                        ctypes.POINTER(BatchStats)]),
+    "psm_server_create": (ctypes.c_int, [VP, VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(VP)]),
+    "psm_server_submit": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.c_uint32, U32P, U32P, U32P]),
+    "psm_server_poll": (ctypes.c_int, [VP, ctypes.POINTER(Result), ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint32), ctypes.c_double]),
+    "psm_server_stats": (ctypes.c_int, [VP, U64P]),
+    "psm_server_destroy": (ctypes.c_int, [VP]),
     "psm_gen_chung_lu": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64,
                                         U32P, U32P]),
     "psm_gen_rmat": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,

@@ -15,7 +15,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -73,22 +75,96 @@ int hip_fail(psm_ctx* ctx, hipError_t e, const char* what, int line) {
         if (_r != PSM_OK) return _r;      \
     } while (0)
 
+// Device memory comes from a process-wide cache of size-bucketed blocks:
+// hipFree synchronises the whole device, which would serialise concurrent
+// contexts (serving workers), so released blocks are kept for reuse. A block
+// is returned to the cache only after the stream that last used it has
+// drained, so a recycled block is never still read by an earlier launch.
+struct DevCache {
+    std::mutex mu;
+    std::multimap<size_t, void*> free_blocks;
+    size_t cached = 0;
+    static constexpr size_t kMaxCached = 48ull << 30;
+};
+DevCache& dev_cache() {
+    static DevCache c;
+    return c;
+}
+
+size_t bucket_size(size_t b) {
+    if (b <= 4096) return 4096;
+    size_t s = 4096;
+    while (s < b) s <<= 1;
+    // above 64 MiB, round to 1/8 steps of the power of two instead
+    if (s > (64ull << 20)) {
+        const size_t step = s / 16;
+        return (b + step - 1) / step * step;
+    }
+    return s;
+}
+
+thread_local hipStream_t tls_stream = nullptr;   // stream of the running batch stage
+
 struct DBuf {
     void* p = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;          // capacity of the block
+    hipStream_t st = nullptr;  // stream the block was allocated for
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) hipFree(p);
+        if (!p) return;
+        if (st) (void)hipStreamSynchronize(st);
+        else (void)hipDeviceSynchronize();
+        DevCache& c = dev_cache();
+        bool keep;
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            keep = c.cached + bytes <= DevCache::kMaxCached;
+            if (keep) {
+                c.free_blocks.emplace(bytes, p);
+                c.cached += bytes;
+            }
+        }
+        if (!keep) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
     }
     hipError_t alloc(size_t b) {
         release();
-        bytes = b ? b : 16;
-        return hipMalloc(&p, bytes);
+        const size_t want = bucket_size(b ? b : 16);
+        st = tls_stream;
+        DevCache& c = dev_cache();
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            auto it = c.free_blocks.find(want);
+            if (it != c.free_blocks.end()) {
+                p = it->second;
+                c.cached -= want;
+                c.free_blocks.erase(it);
+                bytes = want;
+                return hipSuccess;
+            }
+        }
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipErrorOutOfMemory) {
+            // give cached blocks back to the runtime and try once more
+            (void)hipGetLastError();
+            std::lock_guard<std::mutex> lk(c.mu);
+            for (auto& kv : c.free_blocks) (void)hipFree(kv.second);
+            c.free_blocks.clear();
+            c.cached = 0;
+            e = hipMalloc(&p, want);
+        }
+        if (e != hipSuccess) { p = nullptr; return e; }
+        bytes = want;
+        return hipSuccess;
+    }
+    void swap(DBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+        std::swap(st, o.st);
     }
     template <class T>
     T* as() const { return static_cast<T*>(p); }
@@ -491,8 +567,7 @@ int Batch::compact_pool() {
                                         d_masks.as<unsigned long long>(), d_bp.as<uint64_t>(),
                                         out.as<uint32_t>(), s()));
     HIP_TRY(ctx, hipStreamSynchronize(s()));
-    std::swap(cand_pool.p, out.p);
-    std::swap(cand_pool.bytes, out.bytes);
+    cand_pool.swap(out);
     cand_total = total;
     uint32_t i = 0;
     for (QueryHost* q : qs)
@@ -549,6 +624,7 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
 }
 
 int Batch::filter(int method) {
+    tls_stream = s();
     const double t0 = now_ms();
     std::vector<QueryHost*> active;
     for (QueryHost* q : qs) {
@@ -580,6 +656,7 @@ int Batch::filter(int method) {
 
 // ---- BuildTable --------------------------------------------------------------
 int Batch::build_tables() {
+    tls_stream = s();
     const double t0 = now_ms();
     if (!bitmaps_current) PSM_TRY(ensure_bitmaps(true));
     if (wprefix.bytes < (size_t)n_slots * wps * 4) HIP_TRY(ctx, wprefix.alloc((size_t)n_slots * wps * 4));
@@ -836,6 +913,7 @@ int Batch::cfl_filter(std::vector<QueryHost*>& active) {
 
 // ---- plans ----------------------------------------------------------------
 int Batch::make_plans(int order_method) {
+    tls_stream = s();
     const double t0 = now_ms();
     if (!tables_built) return set_err(ctx, PSM_ESTATE, "build tables before generating a plan");
     for (QueryHost* q : qs) {
@@ -972,8 +1050,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
                             cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
                             d_expand.as<uint8_t>(), nullptr, d_pos.as<uint64_t>(), d_tasks2.as<DTask>(),
                             qcnt.as<QCounters>(), nullptr, 1, s()));
-        std::swap(d_tasks.p, d_tasks2.p);
-        std::swap(d_tasks.bytes, d_tasks2.bytes);
+        d_tasks.swap(d_tasks2);
         for (uint32_t i = 0; i < nq; ++i) {
             if (ex[i]) depth[i]++;
             ntask[i] = qchild[i];
@@ -1025,6 +1102,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
 }
 
 int Batch::evaluate(uint64_t cap) {
+    tls_stream = s();
     for (QueryHost* q : qs)
         if (!q->planned) return set_err(ctx, PSM_ESTATE, "generate a plan before evaluating");
     std::vector<DPlan> hp(qs.size());
@@ -1053,6 +1131,7 @@ int Batch::evaluate(uint64_t cap) {
 // units' embeddings on the keys, with data vertices distinct across the units,
 // gives exactly the query's embeddings.
 int Batch::join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr) {
+    tls_stream = s();
     const uint32_t n = q.qg.n;
     jr = JoinResult();
     if (!q.planned) return set_err(ctx, PSM_ESTATE, "generate a plan before joining");
@@ -1249,7 +1328,8 @@ int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32
                      const uint32_t* lab, psm_graph** out) {
     if (!ctx || !out || !off || !lab || (off[V] && !adj)) return PSM_EINVAL;
     *out = nullptr;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
+    tls_stream = nullptr;   // graph buffers outlive batches: release with a device sync
     std::unique_ptr<psm_graph> g(new psm_graph());
     g->ctx = ctx;
     g->V = V;
@@ -1486,6 +1566,223 @@ int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32_
         st->tasks = b.tasks_run;
         for (auto& h : hosts) st->probe_bytes += h->probe_bytes;
     }
+    return PSM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching over a resident data graph. Worker threads,
+// each with its own context and HIP stream, take the oldest pending requests
+// as one batch while slots and the memory budget allow; requests arriving
+// meanwhile are admitted by the next free worker. Batches of different
+// workers run concurrently on the device. No counterpart in the reference,
+// which runs one query per process.
+// ---------------------------------------------------------------------------
+#include <condition_variable>
+#include <deque>
+
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    std::unique_ptr<QueryHost> q;
+    uint64_t est_bytes = 0;
+    double submit_ms = 0;
+};
+
+}  // namespace
+
+struct psm_server {
+    const psm_graph* g = nullptr;
+    int device = 0;
+    int filter = PSM_FILTER_GQL, order = PSM_ORDER_GQL;
+    uint64_t cap = 0;
+    double time_limit_s = 60.0;
+    uint32_t max_batch = 64;
+    uint64_t mem_budget = 0;
+    std::vector<psm_ctx*> ctxs;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request> pending;
+    std::deque<psm_result> done;
+    uint64_t mem_in_use = 0, peak_mem = 0, submitted = 0, completed = 0, batches = 0;
+    bool stop = false;
+    double t0 = 0;
+};
+
+namespace {
+
+// Upper bound of a request's device footprint before it runs: candidate
+// sets (label buckets), table rows, bitmaps + ranks of its query vertices.
+uint64_t footprint(const psm_graph* g, const QGraph& q) {
+    const uint64_t wps = ((uint64_t)g->V + 31) / 32;
+    uint64_t b = (uint64_t)q.n * wps * 8;
+    for (uint32_t u = 0; u < q.n; ++u) {
+        const uint32_t l = q.lab[u];
+        const uint64_t bucket = l < g->L ? g->h_ri_off[l + 1] - g->h_ri_off[l] : 0;
+        b += bucket * 4 * (2 + q.degree(u)) + bucket * 8 * q.degree(u);
+    }
+    return b + (64ull << 20);
+}
+
+void serve_worker(psm_server* S, psm_ctx* ctx) {
+    (void)hipSetDevice(S->device);
+    while (true) {
+        std::vector<Request> batch;
+        uint64_t est = 0;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
+            if (S->pending.empty()) return;   // stopping
+            // admission: oldest first, while slots and memory allow (a request
+            // that fits nowhere else runs alone once the device is idle)
+            while (!S->pending.empty() && batch.size() < S->max_batch) {
+                Request& r = S->pending.front();
+                const bool fits = S->mem_in_use + est + r.est_bytes <= S->mem_budget;
+                if (!fits && !(batch.empty() && S->mem_in_use == 0)) break;
+                est += r.est_bytes;
+                batch.push_back(std::move(r));
+                S->pending.pop_front();
+            }
+            if (batch.empty()) {
+                S->cv_done.wait(lk);   // memory frees up when another batch completes
+                continue;
+            }
+            S->mem_in_use += est;
+            S->peak_mem = std::max(S->peak_mem, S->mem_in_use);
+            ++S->batches;
+        }
+        const double start = now_ms() - S->t0;
+        Batch b;
+        b.ctx = ctx;
+        b.g = S->g;
+        for (Request& r : batch) b.qs.push_back(r.q.get());
+        int rc = b.filter(S->filter);
+        if (rc == PSM_OK) rc = b.build_tables();
+        if (rc == PSM_OK) rc = b.make_plans(S->order);
+        if (rc == PSM_OK) rc = b.evaluate(S->cap);
+        const double end = now_ms() - S->t0;
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (Request& r : batch) {
+            psm_result res{};
+            res.request_id = r.id;
+            res.count = r.q->count;
+            res.calls = r.q->calls;
+            res.status = rc != PSM_OK ? rc : (r.q->timed_out ? PSM_ETIMEOUT : r.q->status);
+            res.batch_size = (uint32_t)batch.size();
+            res.submit_ms = r.submit_ms;
+            res.start_ms = start;
+            res.done_ms = end;
+            S->done.push_back(res);
+            ++S->completed;
+        }
+        S->mem_in_use -= est;
+        S->cv_done.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int psm_server_create(psm_ctx* ctx, const psm_graph* g, uint32_t workers, uint32_t max_batch,
+                      uint64_t mem_budget, int filter, int order, uint64_t cap, double time_limit_s,
+                      psm_server** out) {
+    if (!ctx || !g || !out || workers == 0 || workers > 16 || max_batch == 0) return PSM_EINVAL;
+    if (filter < PSM_FILTER_LDF || filter > PSM_FILTER_CFL || (order != PSM_ORDER_GQL && order != PSM_ORDER_CFL))
+        return PSM_EINVAL;
+    std::unique_ptr<psm_server> S(new psm_server());
+    S->g = g;
+    S->device = ctx->device;
+    S->filter = filter;
+    S->order = order;
+    S->cap = cap;
+    S->time_limit_s = time_limit_s;
+    S->max_batch = max_batch;
+    if (!mem_budget) {
+        size_t free_b = 0, total_b = 0;
+        (void)hipSetDevice(ctx->device);
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) mem_budget = free_b / 2;
+        else mem_budget = 16ull << 30;
+    }
+    S->mem_budget = mem_budget;
+    S->t0 = now_ms();
+    for (uint32_t i = 0; i < workers; ++i) {
+        psm_ctx* c = nullptr;
+        const int r = psm_ctx_create(ctx->device, &c);
+        if (r != PSM_OK) {
+            for (psm_ctx* x : S->ctxs) psm_ctx_destroy(x);
+            return r;
+        }
+        c->time_limit_s = time_limit_s;
+        c->verbose = ctx->verbose;
+        S->ctxs.push_back(c);
+    }
+    for (psm_ctx* c : S->ctxs) S->workers.emplace_back(serve_worker, S.get(), c);
+    *out = S.release();
+    return PSM_OK;
+}
+
+int psm_server_submit(psm_server* S, uint64_t request_id, uint32_t n, const uint32_t* q_off,
+                      const uint32_t* q_adj, const uint32_t* q_lab) {
+    if (!S || !q_off || !q_lab) return PSM_EINVAL;
+    Request r;
+    r.id = request_id;
+    r.q.reset(new QueryHost());
+    const int rc = make_qgraph(n, q_off, q_adj, q_lab, r.q->qg);
+    if (rc != PSM_OK) return rc;
+    if (S->filter == PSM_FILTER_GQL && r.q->qg.max_degree() > kMaxGqlDegree) return PSM_EINVAL;
+    r.est_bytes = footprint(S->g, r.q->qg);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) return PSM_ESTATE;
+        r.submit_ms = now_ms() - S->t0;
+        S->pending.push_back(std::move(r));
+        ++S->submitted;
+    }
+    S->cv_work.notify_one();
+    return PSM_OK;
+}
+
+int psm_server_poll(psm_server* S, psm_result* out, uint32_t max_results, uint32_t* n_out, double wait_ms) {
+    if (!S || !n_out || (!out && max_results)) return PSM_EINVAL;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (S->done.empty() && wait_ms > 0)
+        S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(wait_ms),
+                            [&] { return !S->done.empty(); });
+    uint32_t k = 0;
+    while (k < max_results && !S->done.empty()) {
+        out[k++] = S->done.front();
+        S->done.pop_front();
+    }
+    *n_out = k;
+    return PSM_OK;
+}
+
+int psm_server_stats(psm_server* S, uint64_t* out5) {
+    if (!S || !out5) return PSM_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    out5[0] = S->submitted;
+    out5[1] = S->completed;
+    out5[2] = S->batches;
+    out5[3] = S->peak_mem;
+    out5[4] = S->pending.size();
+    return PSM_OK;
+}
+
+int psm_server_destroy(psm_server* S) {
+    if (!S) return PSM_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    S->cv_done.notify_all();
+    for (auto& t : S->workers) t.join();
+    for (psm_ctx* c : S->ctxs) psm_ctx_destroy(c);
+    delete S;
     return PSM_OK;
 }
 

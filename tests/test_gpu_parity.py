@@ -175,3 +175,23 @@ def test_time_limit_reports_partial_count(hprd):
         L.lib().psm_set_time_limit(dg.dev.h, 60.0)
     assert r.timed_out and r.count < 1015922
     assert match(dg, q, "GQL_GQL_GQL").count == 1015922
+
+
+def test_served_results_equal_alone_and_in_the_mix(hprd):
+    """Requests served through continuous batching (several workers, mixed
+    arrivals) give each query exactly its count when matched alone."""
+    from psm_benchmark_amd.serving import Server, run_poisson
+    g, dg, _ = hprd
+    recs = golden(("GQL_GQL_GQL",))
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
+    srv = Server(dg, workers=3, max_batch=4, filter="GQL", order="GQL")
+    try:
+        results, summary = run_poisson(srv, qs, rate_qps=500.0, n_requests=60, seed=7)
+        print(summary, srv.stats(), flush=True)
+    finally:
+        srv.close()
+    for k, r in results.items():
+        assert r.status == 0
+        assert r.count == recs[k % len(recs)]["embeddings"], k
+        assert r.calls == recs[k % len(recs)]["calls"], k
+    assert max(r.batch_size for r in results.values()) >= 1
