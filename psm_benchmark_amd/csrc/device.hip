@@ -204,15 +204,24 @@ __device__ __forceinline__ uint32_t left_mask(const uint32_t* __restrict__ bitma
 __global__ __launch_bounds__(kBlock) void k_gql_refine(
     DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_waves,
     const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
-    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ bytes) {
+    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ bytes,
+    const uint2* __restrict__ list) {
     __shared__ GqlShared sh;
     const uint32_t wl = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
     const uint32_t gw = blockIdx.x * kGqlWavesPerBlock + wl;
     if (gw >= n_waves) return;
-    const uint32_t ii = find_item(items, n_items, gw, &GItem::wave_start);
+    uint32_t ii, j;
+    if (list) {
+        // candidates deferred by the 8-lane kernel: (item, index) pairs
+        const uint2 e = list[gw];
+        ii = e.x;
+        j = e.y;
+    } else {
+        ii = find_item(items, n_items, gw, &GItem::wave_start);
+        j = gw - items[ii].wave_start;
+    }
     const GItem it = items[ii];
-    const uint32_t j = gw - it.wave_start;
     if (j >= it.len) return;
     const uint32_t v = cand_pool[it.cand_begin + j];
     if (v == kInvalid) return;
@@ -296,6 +305,79 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
     if (lane == 0) {
         const unsigned long long scan = 4ull * (ve - vb) * (1 + k);
         atomicAdd(bytes, 12ull + scan * (pass2 ? 2 : 1));
+    }
+}
+
+
+// GQL refinement for small candidates: 8 lanes per candidate v when
+// deg(v) <= 8 and k = deg_q(u) <= 8 (the common case on power-law graphs).
+// Lane j of the group holds neighbour w_j and its k-bit mask; the group's
+// right-neighbour set R_i of every left vertex i comes out of one ballot per
+// i. By Hall's theorem a matching saturating the left side exists iff every
+// subset S of left vertices has |union of R_i, i in S| >= |S|; the 2^k - 1
+// subsets are split over the group's 8 lanes. Candidates outside the limits
+// are appended to `large` for the wavefront kernel.
+constexpr uint32_t kGroup = 8;
+
+__global__ __launch_bounds__(kBlock) void k_gql_refine_small(
+    DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_cands,
+    const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
+    uint32_t* __restrict__ bitmaps, uint64_t wps, uint2* __restrict__ large,
+    uint32_t* __restrict__ n_large, unsigned long long* __restrict__ bytes) {
+    const uint32_t lane = lane_id();
+    const uint32_t lig = lane & (kGroup - 1);
+    const uint32_t gbase = lane & ~(kGroup - 1);
+    const uint32_t gid = (blockIdx.x * kBlock + threadIdx.x) / kGroup;
+    bool live = gid < n_cands;
+    uint32_t ii = 0, j = 0, v = kInvalid, k = 0, deg = 0, vb = 0;
+    GItem it{};
+    if (live) {
+        ii = find_item(items, n_items, gid, &GItem::wave_start);
+        it = items[ii];
+        j = gid - it.wave_start;
+        live = j < it.len;
+    }
+    if (live) {
+        v = cand_pool[it.cand_begin + j];
+        live = v != kInvalid;
+    }
+    if (live) {
+        vb = g.off[v];
+        deg = g.off[v + 1] - vb;
+        k = it.k;
+        if (deg > kGroup || k > kGroup) {
+            if (lig == 0) large[atomicAdd(n_large, 1u)] = make_uint2(ii, j);
+            live = false;
+        }
+    }
+    uint32_t m = 0;
+    if (live && lig < deg)
+        m = left_mask(bitmaps, wps, left_slots + it.left_begin, k, g.adj[vb + lig]);
+    // R[i]: group lanes (right vertices) adjacent to left vertex i
+    uint32_t R[kGroup];
+#pragma unroll
+    for (uint32_t i = 0; i < kGroup; ++i) {
+        const unsigned long long bal = __ballot((m >> i) & 1u);
+        R[i] = (uint32_t)(bal >> gbase) & 0xFFu;
+    }
+    bool fail = false;
+    if (live) {
+        const uint32_t full = (1u << k) - 1u;
+        for (uint32_t sub = lig + 1; sub <= full && !fail; sub += kGroup) {
+            uint32_t uni = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < kGroup; ++i)
+                if ((sub >> i) & 1u) uni |= R[i];
+            fail = __popc(uni) < __popc(sub);
+        }
+    }
+    const bool group_fail = ((__ballot(fail) >> gbase) & 0xFFull) != 0;
+    if (live && lig == 0) {
+        if (group_fail) {
+            cand_pool[it.cand_begin + j] = kInvalid;
+            atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
+        }
+        atomicAdd(bytes, 12ull + 4ull * deg * (1 + k));
     }
 }
 
@@ -1062,11 +1144,23 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
                              uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
                              uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* bytes,
-                             hipStream_t s) {
+                             const uint2* list, hipStream_t s) {
     if (n_waves == 0) return hipSuccess;
     const uint32_t blocks = (n_waves + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock;
     hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_waves,
-                       left_slots, cand_pool, bitmaps, words_per_slot, bytes);
+                       left_slots, cand_pool, bitmaps, words_per_slot, bytes, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32_t n_items,
+                                   uint32_t n_cands, const uint32_t* left_slots, uint32_t* cand_pool,
+                                   uint32_t* bitmaps, uint64_t words_per_slot, uint2* large,
+                                   uint32_t* n_large, unsigned long long* bytes, hipStream_t s) {
+    if (n_cands == 0) return hipSuccess;
+    const uint64_t threads = (uint64_t)n_cands * kGroup;
+    const uint32_t blocks = (uint32_t)((threads + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_gql_refine_small, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_cands,
+                       left_slots, cand_pool, bitmaps, words_per_slot, large, n_large, bytes);
     return hipGetLastError();
 }
 

@@ -181,7 +181,11 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
                              uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
                              uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* bytes,
-                             hipStream_t s);
+                             const uint2* list, hipStream_t s);
+hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32_t n_items,
+                                   uint32_t n_cands, const uint32_t* left_slots, uint32_t* cand_pool,
+                                   uint32_t* bitmaps, uint64_t words_per_slot, uint2* large,
+                                   uint32_t* n_large, unsigned long long* bytes, hipStream_t s);
 hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,
                               const uint64_t* item_row_start, uint64_t n_rows,
                               const uint32_t* cand_pool, const uint32_t* bitmaps,

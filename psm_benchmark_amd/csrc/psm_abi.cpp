@@ -584,7 +584,8 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
     PSM_TRY(ensure_bitmaps(true));
     uint32_t max_n = 0;
     for (QueryHost* q : active) max_n = std::max(max_n, q->qg.n);
-    DBuf d_items, d_left;
+    DBuf d_items, d_left, d_large, d_nlarge;
+    HIP_TRY(ctx, d_nlarge.alloc(4));
     for (int round = 0; round < 2; ++round) {
         for (uint32_t p = 0; p < max_n; ++p) {
             std::vector<GItem> items;
@@ -607,11 +608,25 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
             if (items.empty()) continue;
             HIP_TRY(ctx, upload(d_items, items, s()));
             HIP_TRY(ctx, upload(d_left, left, s()));
-            verbose("gql round %d u %u: %u candidates", round, p, waves);
+            if (d_large.bytes < (size_t)waves * sizeof(uint2)) HIP_TRY(ctx, d_large.alloc((size_t)waves * sizeof(uint2)));
+            HIP_TRY(ctx, hipMemsetAsync(d_nlarge.p, 0, 4, s()));
+            // small candidates: 8 lanes each; the rest are deferred to the
+            // wavefront-per-candidate kernel
             TIMED(ctx, kKGqlRefine,
-                  launch_gql_refine(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), waves,
-                                    d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
-                                    bitmaps.as<uint32_t>(), wps, ctx->d_kbytes + kKGqlRefine, s()));
+                  launch_gql_refine_small(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), waves,
+                                          d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
+                                          bitmaps.as<uint32_t>(), wps, d_large.as<uint2>(),
+                                          d_nlarge.as<uint32_t>(), ctx->d_kbytes + kKGqlRefine, s()));
+            uint32_t n_large = 0;
+            HIP_TRY(ctx, hipMemcpyAsync(&n_large, d_nlarge.p, 4, hipMemcpyDeviceToHost, s()));
+            HIP_TRY(ctx, hipStreamSynchronize(s()));
+            verbose("gql round %d u %u: %u candidates, %u to the wavefront kernel", round, p, waves, n_large);
+            if (n_large)
+                TIMED(ctx, kKGqlRefine,
+                      launch_gql_refine(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), n_large,
+                                        d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
+                                        bitmaps.as<uint32_t>(), wps, ctx->d_kbytes + kKGqlRefine,
+                                        d_large.as<uint2>(), s()));
         }
     }
     HIP_TRY(ctx, timers_flush(ctx));
