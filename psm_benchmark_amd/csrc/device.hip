@@ -650,6 +650,152 @@ __global__ __launch_bounds__(kBlock) void k_expand(
     }
 }
 
+// Current level of a lane's search: the base list [cur, end) (operand `sel`
+// or C(u) itself) and merge cursors over up to four other operand rows.
+// Elements of the base list come in increasing order, so every cursor only
+// moves forward; an exhausted operand ends the level. Operands beyond the
+// fourth are binary-searched.
+constexpr uint32_t kCursors = 4;
+
+struct Level {
+    uint64_t cur, end;
+    uint32_t sel, nb, nx;   // base operand, cached cursors, all other operands
+    uint64_t c0, c1, c2, c3, e0, e1, e2, e3;
+};
+
+__device__ __forceinline__ void set_cursor(Level& L, uint32_t i, uint64_t lo, uint64_t hi) {
+    switch (i) {
+        case 0: L.c0 = lo; L.e0 = hi; break;
+        case 1: L.c1 = lo; L.e1 = hi; break;
+        case 2: L.c2 = lo; L.e2 = hi; break;
+        default: L.c3 = lo; L.e3 = hi; break;
+    }
+}
+
+// Sets up level d. forced_sel != kAuto keeps the given base operand (a
+// restored or donated range); otherwise the shortest row is the base.
+constexpr uint32_t kAutoSel = 0xFFFFFFFEu;
+
+__device__ __forceinline__ void enter_level(const DPlan& P, const DOp* __restrict__ ops,
+                                            const uint64_t* __restrict__ row_off, uint32_t d,
+                                            const LaneState& st, uint32_t forced_sel, Level& L,
+                                            unsigned long long& bytes) {
+    const uint32_t nops = P.op_count[d];
+    L.nb = L.nx = 0;
+    if (nops == 0) {
+        L.sel = kIdentity;
+        L.cur = 0;
+        L.end = P.cand_len[d];
+        return;
+    }
+    uint32_t sel = forced_sel;
+    if (sel == kAutoSel) {
+        uint64_t best = ~0ull;
+        for (uint32_t j = 0; j < nops; ++j) {
+            const DOp op = ops[P.op_begin[d] + j];
+            const uint64_t r = op.row_base + st.I(op.src_depth);
+            const uint64_t len = row_off[r + 1] - row_off[r];
+            if (len < best) { best = len; sel = j; }
+        }
+        bytes += 16ull * nops;
+    }
+    L.sel = sel;
+    {
+        const DOp op = ops[P.op_begin[d] + sel];
+        const uint64_t r = op.row_base + st.I(op.src_depth);
+        L.cur = row_off[r];
+        L.end = row_off[r + 1];
+    }
+    for (uint32_t j = 0; j < nops; ++j) {
+        if (j == sel) continue;
+        if (L.nb < kCursors) {
+            const DOp op = ops[P.op_begin[d] + j];
+            const uint64_t r = op.row_base + st.I(op.src_depth);
+            set_cursor(L, L.nb, row_off[r], row_off[r + 1]);
+            ++L.nb;
+        }
+        ++L.nx;
+    }
+    bytes += 16ull * (1 + L.nb);
+}
+
+// First position p in [c, e) with idx[p] >= x, galloping from c.
+__device__ __forceinline__ uint64_t gallop(const uint32_t* __restrict__ idx, uint64_t c, uint64_t e,
+                                           uint32_t x, unsigned long long& bytes) {
+    if (c >= e) return e;
+    bytes += 4;
+    if (idx[c] >= x) return c;
+    uint64_t lo = c, step = 1;   // idx[lo] < x
+    while (lo + step < e) {
+        bytes += 4;
+        if (idx[lo + step] >= x) break;
+        lo += step;
+        step <<= 1;
+    }
+    uint64_t hi = min(lo + step, e);   // idx[hi] >= x or hi == e
+    ++lo;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        bytes += 4;
+        if (idx[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// 0: x fails; 1: x is a valid extension (v_out set); 2: an operand is
+// exhausted, no later element of this level can match.
+__device__ __forceinline__ int step_cursor(const uint32_t* __restrict__ idx, uint64_t& c, uint64_t e,
+                                           uint32_t x, unsigned long long& bytes) {
+    c = gallop(idx, c, e, x, bytes);
+    if (c >= e) return 2;
+    return idx[c] == x ? 1 : 0;
+}
+
+__device__ __forceinline__ int test_merge(const DPlan& P, const DOp* __restrict__ ops,
+                                          const uint64_t* __restrict__ row_off,
+                                          const uint32_t* __restrict__ idx,
+                                          const uint32_t* __restrict__ cand_pool, uint32_t d,
+                                          uint32_t x, Level& L, const LaneState& st, uint32_t& v_out,
+                                          unsigned long long& bytes) {
+    int r;
+    if (L.nb > 0 && (r = step_cursor(idx, L.c0, L.e0, x, bytes)) != 1) return r;
+    if (L.nb > 1 && (r = step_cursor(idx, L.c1, L.e1, x, bytes)) != 1) return r;
+    if (L.nb > 2 && (r = step_cursor(idx, L.c2, L.e2, x, bytes)) != 1) return r;
+    if (L.nb > 3 && (r = step_cursor(idx, L.c3, L.e3, x, bytes)) != 1) return r;
+    if (L.nx > L.nb) {
+        // operands past the cached ones: binary search
+        const uint32_t nops = P.op_count[d];
+        uint32_t pos = 0;
+        for (uint32_t j = 0; j < nops; ++j) {
+            if (j == L.sel) continue;
+            if (pos++ < kCursors) continue;
+            const DOp op = ops[P.op_begin[d] + j];
+            const uint64_t rr = op.row_base + st.I(op.src_depth);
+            const uint64_t hi0 = row_off[rr + 1];
+            uint64_t lo = row_off[rr], hi = hi0;
+            bytes += 16;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                bytes += 4;
+                if (idx[mid] < x) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo == hi0 || idx[lo] != x) return 0;
+        }
+    }
+    const uint32_t v = cand_pool[P.cand_base[d] + x];
+    bytes += 4;
+    unsigned long long mask = P.same_label[d];
+    while (mask) {
+        const uint32_t e = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+        if (st.M(e) == v) return 0;
+    }
+    v_out = v;
+    return 1;
+}
+
 __device__ __forceinline__ uint32_t load_acquire(uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -658,20 +804,28 @@ __device__ __forceinline__ uint32_t load_relaxed(unsigned int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ bool past_deadline(DQueue* Q) {
+    return Q->limit_ticks &&
+           (unsigned long long)wall_clock64() >
+               __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Persistent DFS. Every lane runs its own depth-first search over one task at
 // a time. Idle lanes claim the next queued tasks (one claim per wavefront).
 // A running lane splits off the second half of the remaining range at its
 // shallowest level with at least two entries left, as a new task, while the
 // queue is short (the reference's TaskPool donates half the siblings at the
 // shallowest splittable depth to idle threads, TaskPool.cpp:960-1362).
-// A query whose count reaches its cap is marked done and its tasks dropped
-// (count = min(cap, total)). Past the wall-clock deadline every lane stops
-// (the reference's Overtime).
+// pending[q] counts the queued and running tasks of query q; a task that
+// runs to its end decrements it, so a query left with pending work when the
+// deadline stops the search is known to be incomplete. A query whose count
+// reaches its cap is marked done and its tasks are dropped (count =
+// min(cap, total)).
 __global__ __launch_bounds__(kBlock) void k_dfs(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* ready, DQueue* Q,
-    QCounters* qc, uint32_t max_n, uint32_t* lane_state, EmitSink sink) {
+    QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t* lane_state, EmitSink sink) {
     const uint32_t lane = lane_id();
     const size_t wave = ((size_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
     const size_t per_field = (size_t)gridDim.x * (kBlock / kWave) * max_n * kWave;
@@ -688,8 +842,8 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
 
     bool active = false, waiting = false;
     uint32_t wslot = 0, backoff = 0;
-    uint32_t q = 0, tdepth = 0, d = 0, sel = 0, n = 0;
-    uint64_t cur = 0, end = 0;
+    uint32_t q = 0, tdepth = 0, d = 0, n = 0;
+    Level L{};
     unsigned long long lcount = 0, lcalls = 0, lbytes = 0;
     uint32_t steps = 0;
     const DPlan* P = plans;
@@ -728,14 +882,11 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             q = task.q;
             P = plans + q;
             bool late = load_relaxed(&Q->timeout) != 0;
-            if (!late && Q->limit_ticks &&
-                (unsigned long long)wall_clock64() >
-                    __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            if (!late && past_deadline(Q)) {
                 atomicExch(&Q->timeout, 1u);
                 late = true;
             }
-            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || late) {
-                if (!__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) atomicExch(&qc[q].timed_out, 1u);
+            if (late || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
                 atomicSub(&Q->busy, 1u);
             } else {
                 n = P->n;
@@ -747,14 +898,11 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     st.M_[st.at(e)] = cand_pool[P->cand_base[e] + x];
                 }
                 if (task.hi == kAutoHi) {
-                    const LevelBase b = level_base(*P, ops, row_off, d, st, lbytes);
-                    sel = b.sel;
-                    cur = b.lo;
-                    end = b.hi;
+                    enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes);
                 } else {
-                    sel = task.sel;
-                    cur = task.lo;
-                    end = task.hi;
+                    enter_level(*P, ops, row_off, d, st, task.sel, L, lbytes);
+                    L.cur = task.lo;
+                    L.end = task.hi;
                 }
                 active = true;
                 steps = 0;
@@ -770,61 +918,57 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         if (!active) continue;
 
         // one step of this lane's search
-        if (cur < end) {
-            const uint32_t x = sel == kIdentity ? (uint32_t)cur : idx[cur];
-            lbytes += sel == kIdentity ? 0 : 4;
-            ++cur;
+        bool finished = false;
+        if (L.cur < L.end) {
+            const uint32_t x = L.sel == kIdentity ? (uint32_t)L.cur : idx[L.cur];
+            lbytes += L.sel == kIdentity ? 0 : 4;
+            ++L.cur;
             uint32_t v;
-            if (test_element(*P, ops, row_off, idx, cand_pool, d, x, sel, st, v, lbytes)) {
+            const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes);
+            if (r == 2) {
+                L.cur = L.end;   // an operand ran out: nothing further at this level
+            } else if (r == 1) {
                 ++lcalls;
                 if (d + 1 == n) {
                     ++lcount;
                     if (sink.rows) {
-                        // materialise the embedding (one row of data vertex ids in depth order)
-                        const unsigned long long r = atomicAdd(sink.count, 1ull);
-                        if (r < sink.cap) {
-                            uint32_t* row = sink.rows + r * n;
-                            for (uint32_t e = 0; e < d; ++e) row[e] = st.M_[st.at(e)];
-                            row[d] = v;
+                        // materialise the embedding (data vertex ids in depth order)
+                        const unsigned long long row = atomicAdd(sink.count, 1ull);
+                        if (row < sink.cap) {
+                            uint32_t* out = sink.rows + row * n;
+                            for (uint32_t e = 0; e < d; ++e) out[e] = st.M_[st.at(e)];
+                            out[d] = v;
                         } else {
                             atomicExch(sink.overflow, 1u);
                         }
                     }
                 } else {
                     const size_t a = st.at(d);
-                    st.cur_[a] = cur;
-                    st.end_[a] = end;
-                    st.sel_[a] = sel;
+                    st.cur_[a] = L.cur;
+                    st.end_[a] = L.end;
+                    st.sel_[a] = L.sel;
                     st.I_[a] = x;
                     st.M_[a] = v;
                     ++d;
-                    const LevelBase b = level_base(*P, ops, row_off, d, st, lbytes);
-                    cur = b.lo;
-                    end = b.hi;
-                    sel = b.sel;
+                    enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes);
                 }
             }
         } else if (d == tdepth) {
+            finished = true;
             active = false;
         } else {
             --d;
             const size_t a = st.at(d);
-            cur = st.cur_[a];
-            end = st.end_[a];
-            sel = st.sel_[a];
+            enter_level(*P, ops, row_off, d, st, st.sel_[a], L, lbytes);
+            L.cur = st.cur_[a];
+            L.end = st.end_[a];
         }
         ++steps;
         if (active && (steps & 255) == 0) {
-            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
+            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout)) {
                 active = false;
-            } else if (load_relaxed(&Q->timeout)) {
-                atomicExch(&qc[q].timed_out, 1u);
-                active = false;
-            } else if (Q->limit_ticks && (unsigned long long)wall_clock64() >
-                                             __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT)) {
+            } else if (past_deadline(Q)) {
                 atomicExch(&Q->timeout, 1u);
-                atomicExch(&qc[q].timed_out, 1u);
                 active = false;
             }
         }
@@ -834,25 +978,26 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             uint32_t e = tdepth;
             uint64_t ce = 0, ee = 0;
             for (; e <= d; ++e) {
-                ce = e == d ? cur : st.cur_[st.at(e)];
-                ee = e == d ? end : st.end_[st.at(e)];
+                ce = e == d ? L.cur : st.cur_[st.at(e)];
+                ee = e == d ? L.end : st.end_[st.at(e)];
                 if (ee - ce >= 2) break;
             }
             if (e <= d && e <= kPrefixMax) {
                 const uint32_t t = load_relaxed(&Q->tail);
                 if (t < Q->cap && atomicCAS(&Q->tail, t, t + 1) == t) {
                     const uint64_t mid = ce + (ee - ce) / 2;
+                    atomicAdd(pending + q, 1u);
                     DTask& nt = queue[t];
                     nt.q = q;
                     nt.depth = e;
-                    nt.sel = e == d ? sel : st.sel_[st.at(e)];
+                    nt.sel = e == d ? L.sel : st.sel_[st.at(e)];
                     nt.pad = 0;
                     nt.lo = mid;
                     nt.hi = ee;
                     for (uint32_t k = 0; k < e; ++k) nt.prefix[k] = st.I_[st.at(k)];
                     __threadfence();
                     __hip_atomic_store(ready + t, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    if (e == d) end = mid;
+                    if (e == d) L.end = mid;
                     else st.end_[st.at(e)] = mid;
                     atomicAdd(&Q->donated, 1u);
                 }
@@ -866,6 +1011,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             if (lcount && prev + lcount >= P->cap) atomicExch(&qc[q].done, 1u);
             lcount = lcalls = lbytes = 0;
         }
+        if (finished) atomicSub(pending + q, 1u);
         if (!active) atomicSub(&Q->busy, 1u);
     }
     if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);   // bytes of tasks dropped at claim
@@ -1230,12 +1376,12 @@ uint32_t dfs_lane_state_words(uint32_t max_n) {
 
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
                       const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
-                      DQueue* dq, QCounters* qc, uint32_t max_n, uint32_t grid_waves,
+                      DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                       uint32_t* lane_state, EmitSink sink, hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
     const uint32_t blocks = grid_waves / (kBlock / kWave);
     hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(kBlock), 0, s, plans, ops, row_off, idx, cand_pool,
-                       queue, ready, dq, qc, max_n, lane_state, sink);
+                       queue, ready, dq, qc, pending, max_n, lane_state, sink);
     return hipGetLastError();
 }
 

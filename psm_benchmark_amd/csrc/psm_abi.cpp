@@ -1081,7 +1081,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
     if (n_tasks) {
         // queue: the expanded tasks followed by room for split-off work
         const uint64_t cap_slots = std::min<uint64_t>(n_tasks + (4u << 20), 0xFFFFFFF0ull);
-        DBuf d_queue, d_ready, d_q, d_state;
+        DBuf d_queue, d_ready, d_q, d_state, d_pending;
         HIP_TRY(ctx, d_queue.alloc(cap_slots * sizeof(DTask)));
         HIP_TRY(ctx, hipMemcpyAsync(d_queue.p, d_tasks.p, n_tasks * sizeof(DTask), hipMemcpyDeviceToDevice, s()));
         HIP_TRY(ctx, d_ready.alloc(cap_slots * 4));
@@ -1095,17 +1095,29 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         HIP_TRY(ctx, hipMemcpyAsync(d_q.p, &hq, sizeof(hq), hipMemcpyHostToDevice, s()));
         const uint32_t gw = grid_waves_full / 4 * 4;
         HIP_TRY(ctx, d_state.alloc((size_t)gw * 64 * dfs_lane_state_words(max_n) * 4));
+        std::vector<uint32_t> pend(nq);
+        for (uint32_t i = 0; i < nq; ++i) pend[i] = (uint32_t)ntask[i];
+        HIP_TRY(ctx, upload(d_pending, pend, s()));
         const double td0 = now_ms();
         TIMED(ctx, kKDfs,
               launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
                          cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_ready.as<uint32_t>(),
-                         d_q.as<DQueue>(), qcnt.as<QCounters>(), max_n, gw, d_state.as<uint32_t>(), sink, s()));
+                         d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
+                         d_state.as<uint32_t>(), sink, s()));
         HIP_TRY(ctx, hipMemcpyAsync(&hq, d_q.p, sizeof(hq), hipMemcpyDeviceToHost, s()));
         HIP_TRY(ctx, timers_flush(ctx));
         t_enum += now_ms() - td0;
         timed_out = hq.timeout != 0;
         donated = hq.donated;
         verbose("dfs: %.1f ms, %u tasks split off, timeout %u", now_ms() - td0, donated, hq.timeout);
+        // a query with queued or unfinished tasks and no cap reached is incomplete
+        std::vector<uint32_t> left;
+        HIP_TRY(ctx, download(left, d_pending.p, nq, s()));
+        HIP_TRY(ctx, download(qc, qcnt.p, nq, s()));
+        for (uint32_t i = 0; i < nq; ++i)
+            if (left[i] && !qc[i].done) qc[i].timed_out = 1;
+        HIP_TRY(ctx, hipMemcpyAsync(qcnt.p, qc.data(), sizeof(QCounters) * nq, hipMemcpyHostToDevice, s()));
+        HIP_TRY(ctx, hipStreamSynchronize(s()));
     }
     tasks_run += n_tasks + donated;
     HIP_TRY(ctx, download(qc, qcnt.p, nq, s()));
