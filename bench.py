@@ -77,6 +77,67 @@ def cpu_baseline(g, qs, meta, cap: int, budget_s: float, time_limit: float):
                 sample=f"first {done} of {len(qs)} queries through the oracle restatement, one thread")
 
 
+def copy_calibration(mib: int, reps: int) -> None:
+    """A plain device copy of known size (mib MiB read + mib MiB written per
+    launch), the reference point that scales the FETCH_SIZE / WRITE_SIZE
+    counters into HBM bytes."""
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").random_(0, 255)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "reps": reps}))
+
+
+def _pmc_rows(path: str, counter: str):
+    """(kernel name, value) per dispatch from a rocprofv3 counter_collection.csv
+    tree (values summed over the counter's instances)."""
+    import csv
+    import glob
+    per = {}
+    for f in glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                key = (f, row.get("Dispatch_Id") or row.get("Correlation_Id"))
+                name, val = row["Kernel_Name"], float(row["Counter_Value"])
+                old = per.get(key)
+                per[key] = (name, val + (old[1] if old else 0.0))
+    return list(per.values())
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, kernel, out_json):
+    """HBM traffic per launch of `kernel`: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) scaled by what the same counters read for the copy calibration."""
+    def avg(d, c, match):
+        v = [x for n, x in _pmc_rows(d, c) if match(n)]
+        return (sum(v) / len(v), len(v)) if v else (float("nan"), 0)
+    is_copy = lambda n: "elementwise" in n.lower() or "copy" in n.lower()
+    copy_bytes = None
+    for d in (copy_fetch_dir, copy_write_dir):
+        f = os.path.join(d, "copy.json")
+        if os.path.exists(f):
+            for line in open(f):
+                if '"copy_bytes"' in line:
+                    copy_bytes = json.loads(line)["copy_bytes"]
+    cf, ncf = avg(copy_fetch_dir, "FETCH_SIZE", is_copy)
+    cw, ncw = avg(copy_write_dir, "WRITE_SIZE", is_copy)
+    kf, nkf = avg(fetch_dir, "FETCH_SIZE", lambda n: kernel in n)
+    kw, nkw = avg(write_dir, "WRITE_SIZE", lambda n: kernel in n)
+    sf = copy_bytes / (cf * 1024) if copy_bytes else float("nan")
+    sw = copy_bytes / (cw * 1024) if copy_bytes else float("nan")
+    res = dict(kernel=kernel, fetch_kib=kf, write_kib=kw, dispatches=[nkf, nkw],
+               copy_bytes=copy_bytes, copy_fetch_kib=cf, copy_write_kib=cw, copy_dispatches=[ncf, ncw],
+               scale_fetch=sf, scale_write=sw,
+               traffic_bytes=kf * 1024 * sf + kw * 1024 * sw)
+    with open(out_json, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,7 +152,23 @@ def main():
     ap.add_argument("--time-limit", type=float, default=2.0,
                     help="enumeration time limit in seconds; 2 s is the -time_limit of the "
                          "reference's recorded runs (test_anony/survey_bash/C_exp.py)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--tune-sample", type=int, default=128, help="queries per autotuner trial")
+    ap.add_argument("--tune-limit", type=float, default=0.5, help="time limit of an autotuner trial (s)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain MIB-MiB device copy (PMC counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=6,
+                    metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR", "KERNEL", "OUT"),
+                    help="summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs into OUT (json)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_dfs_c2_batched_traffic.json"),
+                    help="HBM traffic summary of the dominant kernel (from --pmc-summary)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, 8)
+        return
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (queries shard "
               "independently across GPUs, see DESIGN.md)", file=sys.stderr)
@@ -113,17 +190,26 @@ def main():
     dg = DeviceGraph(dev, g)
     log(f"graph resident in HBM ({time.time() - t0:.1f}s)")
 
+    tuned, trials = None, []
+    if not args.no_tune:
+        from psm_benchmark_amd import autotune
+        t0 = time.time()
+        tuned, trials = autotune.tune(dg, qs, "GQL", "GQL", args.cap, sample=args.tune_sample,
+                                      trial_limit_s=args.tune_limit, restore_limit_s=args.time_limit,
+                                      log=log)
+        log(f"autotune picked {tuned} ({time.time() - t0:.1f}s)")
     for _ in range(args.warmup):
         counts, calls, status, st = batch_match(dg, qs, "GQL", "GQL", args.cap)
         log(f"warmup: total {st.total_ms:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
             f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f})")
     L.lib().psm_kernel_stats_reset(dev.h)
-    times, stats = [], []
+    times, stats, nodes = [], [], 0
     for _ in range(args.steps):
         t = time.perf_counter()
         counts, calls, status, st = batch_match(dg, qs, "GQL", "GQL", args.cap)
         times.append(time.perf_counter() - t)
         stats.append(st)
+        nodes += int(calls.sum())
         log(f"step: {times[-1] * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
             f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f})")
     ks = kernel_stats(dev)
@@ -154,8 +240,13 @@ def main():
     # durations from HIP events on the library's stream)
     dom = max(ks.values(), key=lambda k: k["ms"])
     achieved = dom["bytes"] / dom["launches"] / (dom["ms"] / dom["launches"] / 1e3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic):
+        tr = json.load(open(args.traffic))
+        if tr.get("kernel") == "k_" + dom["name"] and tr.get("traffic_bytes") == tr.get("traffic_bytes"):
+            traffic = int(tr["traffic_bytes"])
     roofline = dict(kernel=dom["name"], bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                     avg_launch_ms=round(dom["ms"] / dom["launches"], 4),
                     bytes_per_launch=int(dom["bytes"] / dom["launches"]))
 
@@ -184,6 +275,9 @@ def main():
         "completed_queries_per_s": round((nq - timeouts) * args.steps / total, 2),
         "embeddings_per_s": round(emb * args.steps / total, 1),
         "embeddings_per_step": emb,
+        "search_nodes_per_s": round(nodes / total, 1),
+        "dfs_lanes_busy": round(sum(s.dfs_lane_steps for s in stats) / max(1, sum(s.dfs_lane_iters for s in stats)), 4),
+        "launch_shape": tuned,
         "latency_ms": {"batched_p50": round(step_ms, 2), "batched_p99": round(step_ms, 2),
                        "plain_p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "plain_p99": round(float(np.percentile(lat_ms, 99)), 3)},

@@ -85,6 +85,25 @@ int psm_kernel_stats_reset(psm_ctx* ctx);
  * (reference: "Overtime: 1"). Default 60 s; 0 disables the limit. */
 int psm_set_time_limit(psm_ctx* ctx, double seconds);
 
+/* Enumeration launch shape, set by the autotuner (psm_benchmark_amd/autotune)
+ * or by hand; results never depend on it. Keys:
+ *   PSM_TUNE_DFS_WAVES_PER_CU       wavefronts per CU of the persistent DFS
+ *                                   grid, 1..32 (default 8)
+ *   PSM_TUNE_EXPAND_TASKS_PER_LANE  pre-expansion target: tasks per DFS
+ *                                   worker (lane or wavefront) before the
+ *                                   persistent kernel starts (2)
+ *   PSM_TUNE_DFS_STRATEGY           PSM_DFS_LANE: one search per lane;
+ *                                   PSM_DFS_WAVE: one search per wavefront
+ *                                   with chunked cooperative intersection
+ * No counterpart in the reference (its only knob is -threadnums). */
+#define PSM_TUNE_DFS_WAVES_PER_CU 0
+#define PSM_TUNE_EXPAND_TASKS_PER_LANE 1
+#define PSM_TUNE_DFS_STRATEGY 2
+#define PSM_DFS_LANE 0
+#define PSM_DFS_WAVE 1
+int psm_set_tuning(psm_ctx* ctx, int key, double value);
+int psm_get_tuning(psm_ctx* ctx, int key, double* value);
+
 /* ---- data graph: class Graph (graph/graph.h:21-237) ------------------
  * Uploads a CSR data graph (adjacency sorted ascending per vertex) and builds
  * on the device side the label reverse index (graph.cpp:15-33) and the
@@ -169,6 +188,9 @@ typedef struct psm_batch_stats {
     double filter_ms, table_ms, plan_ms, enum_ms, total_ms;
     uint64_t tasks;
     uint64_t probe_bytes;
+    /* DFS lane utilisation: loop iterations in which a lane held a task, and
+     * all loop iterations of all lanes */
+    uint64_t dfs_lane_steps, dfs_lane_iters;
 } psm_batch_stats;
 
 int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t num_queries, const uint32_t* n,

@@ -49,6 +49,8 @@ class BatchStats(ctypes.Structure):
         ("total_ms", ctypes.c_double),
         ("tasks", ctypes.c_uint64),
         ("probe_bytes", ctypes.c_uint64),
+        ("dfs_lane_steps", ctypes.c_uint64),
+        ("dfs_lane_iters", ctypes.c_uint64),
     ]
 
 
@@ -76,6 +78,12 @@ class Result(ctypes.Structure):
     ]
 
 
+PSM_TUNE_DFS_WAVES_PER_CU = 0
+PSM_TUNE_EXPAND_TASKS_PER_LANE = 1
+PSM_TUNE_DFS_STRATEGY = 2
+PSM_DFS_LANE = 0
+PSM_DFS_WAVE = 1
+
 # name -> (restype, argtypes); every symbol include/psm_abi.h declares
 SIGNATURES = {
     "psm_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
@@ -84,6 +92,8 @@ SIGNATURES = {
     "psm_kernel_stats": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "psm_kernel_stats_reset": (ctypes.c_int, [VP]),
     "psm_set_time_limit": (ctypes.c_int, [VP, ctypes.c_double]),
+    "psm_set_tuning": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_double]),
+    "psm_get_tuning": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "psm_graph_upload": (ctypes.c_int, [VP, ctypes.c_uint32, U32P, U32P, U32P, ctypes.POINTER(VP)]),
     "psm_graph_destroy": (ctypes.c_int, [VP]),
     "psm_graph_info": (ctypes.c_int, [VP, U64P]),

@@ -1,0 +1,104 @@
+"""Autotuner of the enumeration launch shape.
+
+The persistent DFS kernel's grid (wavefronts per CU) and the task
+pre-expansion target (tasks per DFS lane before the persistent kernel
+starts, i.e. the frontier chunk the BFS phase hands to the DFS phase) trade
+occupancy against per-lane state traffic and load balance, and the best
+point depends on the data graph and the query mix. The tuner runs the
+batched pipeline over a sample of the workload for each candidate setting
+under a short time limit and keeps the setting with the highest search
+throughput (assignments per millisecond of enumeration, the reference's
+"Call Count" rate), which is comparable across settings whether or not a
+trial runs into the limit. Results never depend on the setting: only the
+launch shape changes.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+import json
+import os
+
+from . import _lib as L
+from .matcher import batch_match
+
+GRID = {
+    "dfs_strategy": ("lane", "wave"),
+    "dfs_waves_per_cu": (4, 8, 16),
+    "expand_tasks_per_lane": (1.0, 4.0),
+}
+KEYS = {
+    "dfs_waves_per_cu": L.PSM_TUNE_DFS_WAVES_PER_CU,
+    "expand_tasks_per_lane": L.PSM_TUNE_EXPAND_TASKS_PER_LANE,
+    "dfs_strategy": L.PSM_TUNE_DFS_STRATEGY,
+}
+STRATEGIES = {"lane": L.PSM_DFS_LANE, "wave": L.PSM_DFS_WAVE}
+
+
+def get_config(dev) -> dict:
+    out = {}
+    for name, key in KEYS.items():
+        v = ctypes.c_double()
+        r = L.lib().psm_get_tuning(dev.h, key, ctypes.byref(v))
+        if r != L.PSM_OK:
+            raise L.PSMError(r, f"psm_get_tuning {name}")
+        out[name] = v.value
+    out["dfs_strategy"] = {c: n for n, c in STRATEGIES.items()}[int(out["dfs_strategy"])]
+    return out
+
+
+def set_config(dev, cfg: dict) -> None:
+    for name, value in cfg.items():
+        if name == "dfs_strategy":
+            value = STRATEGIES[value]
+        r = L.lib().psm_set_tuning(dev.h, KEYS[name], float(value))
+        if r != L.PSM_OK:
+            raise L.PSMError(r, f"psm_set_tuning {name}={value}: {L.lib().psm_last_error(dev.h).decode()}")
+
+
+def tune(dg, queries, filter: str = "GQL", order: str = "GQL", cap: int = 0, sample: int = 128,
+         trial_limit_s: float = 0.5, restore_limit_s: float = 60.0, grid: dict | None = None,
+         cache: str | None = None, log=None):
+    """Picks and sets the launch shape for `queries` on `dg`.
+
+    sample: queries used per trial (every k-th of the workload);
+    trial_limit_s: enumeration time limit of a trial; restore_limit_s: the
+    limit set again afterwards. Returns (best config, list of trials)."""
+    grid = grid or GRID
+    dev = dg.dev
+    step = max(1, len(queries) // max(1, sample))
+    sub = list(queries[::step])[:sample]
+    info = dg.info()
+    sig = dict(V=info["V"], E=info["E"], labels=info["labels"], queries=len(sub),
+               sizes=sorted({q.num_vertices for q in sub}), filter=filter, order=order, cap=cap)
+    if cache and os.path.exists(cache):
+        try:
+            saved = json.load(open(cache))
+            if saved.get("signature") == sig:
+                set_config(dev, saved["best"])
+                return saved["best"], saved["trials"]
+        except (ValueError, KeyError):
+            pass
+    names = list(grid)
+    trials = []
+    L.lib().psm_set_time_limit(dev.h, trial_limit_s)
+    try:
+        for values in itertools.product(*(grid[n] for n in names)):
+            cfg = dict(zip(names, values))
+            set_config(dev, cfg)
+            counts, calls, status, st = batch_match(dg, sub, filter, order, cap)
+            nodes = int(calls.sum())
+            rate = nodes / st.enum_ms if st.enum_ms > 0 else 0.0
+            trials.append(dict(config=cfg, nodes=nodes, enum_ms=round(st.enum_ms, 3),
+                               nodes_per_ms=round(rate, 1),
+                               timeouts=int((status == L.PSM_ETIMEOUT).sum())))
+            if log:
+                log(f"autotune {cfg}: {nodes} nodes in {st.enum_ms:.1f} ms ({rate:.0f}/ms)")
+    finally:
+        L.lib().psm_set_time_limit(dev.h, restore_limit_s)
+    best = max(trials, key=lambda t: t["nodes_per_ms"])["config"]
+    set_config(dev, best)
+    if cache:
+        with open(cache, "w") as fh:
+            json.dump(dict(signature=sig, best=best, trials=trials), fh, indent=1)
+    return best, trials

@@ -663,13 +663,15 @@ struct Level {
     uint64_t c0, c1, c2, c3, e0, e1, e2, e3;
 };
 
-__device__ __forceinline__ void set_cursor(Level& L, uint32_t i, uint64_t lo, uint64_t hi) {
-    switch (i) {
-        case 0: L.c0 = lo; L.e0 = hi; break;
-        case 1: L.c1 = lo; L.e1 = hi; break;
-        case 2: L.c2 = lo; L.e2 = hi; break;
-        default: L.c3 = lo; L.e3 = hi; break;
-    }
+// Adds a cursor in front, shifting the others back: only constant field
+// positions are used, so the level stays in registers (an indexed store into
+// the cursors would place the whole level in scratch memory). The order in
+// which cursors are tested does not matter: an element must be in all rows.
+__device__ __forceinline__ void push_cursor(Level& L, uint64_t lo, uint64_t hi) {
+    L.c3 = L.c2; L.e3 = L.e2;
+    L.c2 = L.c1; L.e2 = L.e1;
+    L.c1 = L.c0; L.e1 = L.e0;
+    L.c0 = lo;   L.e0 = hi;
 }
 
 // Sets up level d. forced_sel != kAuto keeps the given base operand (a
@@ -711,7 +713,7 @@ __device__ __forceinline__ void enter_level(const DPlan& P, const DOp* __restric
         if (L.nb < kCursors) {
             const DOp op = ops[P.op_begin[d] + j];
             const uint64_t r = op.row_base + st.I(op.src_depth);
-            set_cursor(L, L.nb, row_off[r], row_off[r + 1]);
+            push_cursor(L, row_off[r], row_off[r + 1]);
             ++L.nb;
         }
         ++L.nx;
@@ -845,6 +847,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     uint32_t q = 0, tdepth = 0, d = 0, n = 0;
     Level L{};
     unsigned long long lcount = 0, lcalls = 0, lbytes = 0;
+    unsigned long long busy_iters = 0, all_iters = 0;
     uint32_t steps = 0;
     const DPlan* P = plans;
     const uint32_t hungry_below = gridDim.x * (kBlock / 2);   // queue shorter than half the lanes
@@ -878,7 +881,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         }
         if (waiting && load_acquire(ready + wslot)) {
             waiting = false;
-            const DTask task = queue[wslot];
+            // the record is read field by field (a local copy of its prefix
+            // array would live in scratch memory)
+            const DTask& task = queue[wslot];
             q = task.q;
             P = plans + q;
             bool late = load_relaxed(&Q->timeout) != 0;
@@ -915,7 +920,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             __builtin_amdgcn_s_sleep(4);
             continue;
         }
+        ++all_iters;
         if (!active) continue;
+        ++busy_iters;
 
         // one step of this lane's search
         bool finished = false;
@@ -1015,6 +1022,473 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         if (!active) atomicSub(&Q->busy, 1u);
     }
     if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);   // bytes of tasks dropped at claim
+    wave_add(&Q->lane_steps, busy_iters);
+    wave_add(&Q->lane_iters, all_iters);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront-cooperative DFS. One wavefront (a one-wave workgroup) walks the
+// search tree of one task at a time, with its stack in LDS. The unit of work
+// is a generation: producing the candidates of depth g for a set of members,
+// the siblings of one run at depth g-1, which share every assignment above
+// g-1 (committed in the LDS path I/M). The members' base lists (each
+// member's shortest operand row) are concatenated and cut into chunks of 64
+// positions; every lane tests one position (coalesced load of the base
+// element, binary search in the member's other operand rows, injectivity
+// against the committed path and the member), and the survivors of a chunk
+// are compacted into the depth's LDS frame tagged with their member.
+// Survivors with the same tag form a run: the wavefront commits their parent
+// and generates depth g+1 from that run. The leaf depth only counts
+// survivors. Short base lists of many siblings thus share one chunk, a hub's
+// long list fills whole chunks, and every load of a chunk is independent
+// across lanes. Work is split off to other wavefronts as in the lane-per-task
+// kernel: half of the remaining base range of the current member at the
+// shallowest depth that has at least two positions left.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kWCache = 8;          // operand rows per member kept in LDS besides the base
+constexpr uint64_t kFresh = ~0ull;       // generation cursor: start at the member's base
+
+// One wavefront's LDS, for queries of up to N vertices.
+template <uint32_t N>
+struct WaveFrame {
+    uint32_t I[N];          // committed candidate indices
+    uint32_t M[N];          // committed data vertices
+    uint32_t run_lo[N];     // current run of the depth's frame
+    uint32_t run_hi[N];
+    uint32_t ccount[N];     // entries in the depth's frame
+    uint32_t gen_m[N];      // generation cursor: member,
+    uint32_t gen_cnt[N];    //   number of members,
+    uint32_t gen_sel[N];    //   base operand of the cursor member,
+    uint64_t gen_pos[N];    //   next base position (kFresh: not started),
+    uint64_t gen_end[N];    //   end of the cursor member's base range
+    uint32_t cx[N][kWave];  // frame: candidate index
+    uint8_t ctag[N][kWave]; // frame: member tag
+    // member cache of the active generation
+    uint64_t m_lo[kWave];   // first base position
+    uint64_t m_pre[kWave + 1];   // exclusive prefix of base lengths
+    uint32_t m_len[kWave];
+    uint32_t m_sel[kWave];
+    uint32_t m_x[kWave];    // member candidate index (depth g-1)
+    uint32_t m_v[kWave];    // member data vertex
+    uint64_t m_olo[kWCache][kWave];   // other operand rows: start
+    uint32_t m_olen[kWCache][kWave];  //                     length
+};
+
+// Is x in the sorted row idx[lo, lo + len)? (lower bound, then compare)
+__device__ __forceinline__ bool row_has(const uint32_t* __restrict__ idx, uint64_t lo, uint64_t len,
+                                        uint32_t x, unsigned long long& bytes) {
+    const uint64_t end = lo + len;
+    while (len > 0) {
+        const uint64_t half = len >> 1;
+        bytes += 4;
+        if (idx[lo + half] < x) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    if (lo >= end) return false;
+    bytes += 4;
+    return idx[lo] == x;
+}
+
+// Candidate index of the parent of depth-g members' operand rows: operand
+// rows of depth g whose source is depth g-1 take the member's own index,
+// the others the committed path (the root generation g == t has no members:
+// the whole prefix is committed).
+template <typename F>
+__device__ __forceinline__ uint32_t op_source(const F& S, const DOp& op, uint32_t g, uint32_t t,
+                                              uint32_t member_x) {
+    return (g > t && op.src_depth == g - 1) ? member_x : S.I[op.src_depth];
+}
+
+// Fills the member cache of generation g (all lanes; lane k = member k) and
+// returns the total base length of the remaining members.
+template <typename F>
+__device__ __forceinline__ uint64_t wave_build_cache(const DPlan& P, const DOp* __restrict__ ops,
+                                     const uint64_t* __restrict__ row_off,
+                                     const uint32_t* __restrict__ cand_pool, F& S, uint32_t g,
+                                     uint32_t t, uint32_t& cnt_out, unsigned long long& bytes) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t cnt = g == t ? 1u : S.run_hi[g - 1] - S.run_lo[g - 1];
+    const uint32_t gm = S.gen_m[g];
+    const uint64_t gpos = S.gen_pos[g];
+    uint64_t len = 0;
+    if (lane < cnt) {
+        uint32_t x = 0, v = 0;
+        if (g > t) {
+            x = S.cx[g - 1][S.run_lo[g - 1] + lane];
+            v = cand_pool[P.cand_base[g - 1] + x];
+            bytes += 4;
+        }
+        const uint32_t nops = P.op_count[g];
+        uint64_t lo = 0;
+        uint32_t sel = kIdentity;
+        if (lane < gm) {
+            len = 0;
+            sel = 0;
+        } else if (lane == gm && gpos != kFresh) {
+            lo = gpos;
+            len = S.gen_end[g] - gpos;
+            sel = S.gen_sel[g];
+        } else if (nops == 0) {
+            lo = 0;
+            len = P.cand_len[g];
+            sel = kIdentity;
+        } else {
+            uint64_t best = ~0ull;
+            for (uint32_t j = 0; j < nops; ++j) {
+                const DOp op = ops[P.op_begin[g] + j];
+                const uint64_t r = op.row_base + op_source(S, op, g, t, x);
+                const uint64_t a = row_off[r], b = row_off[r + 1];
+                if (b - a < best) { best = b - a; lo = a; sel = j; }
+            }
+            bytes += 16ull * nops;
+            len = best;
+        }
+        if (sel != kIdentity && len > 0) {
+            uint32_t slot = 0;
+            for (uint32_t j = 0; j < nops && slot < kWCache; ++j) {
+                if (j == sel) continue;
+                const DOp op = ops[P.op_begin[g] + j];
+                const uint64_t r = op.row_base + op_source(S, op, g, t, x);
+                const uint64_t a = row_off[r], b = row_off[r + 1];
+                S.m_olo[slot][lane] = a;
+                S.m_olen[slot][lane] = (uint32_t)(b - a);
+                ++slot;
+            }
+            bytes += 16ull * slot;
+        }
+        S.m_lo[lane] = lo;
+        S.m_len[lane] = (uint32_t)len;
+        S.m_sel[lane] = sel;
+        S.m_x[lane] = x;
+        S.m_v[lane] = v;
+    }
+    // exclusive prefix of the lengths over the members
+    uint64_t incl = len;
+    for (uint32_t o = 1; o < kWave; o <<= 1) {
+        const uint32_t lo32 = __shfl_up((uint32_t)incl, o), hi32 = __shfl_up((uint32_t)(incl >> 32), o);
+        if (lane >= o) incl += ((uint64_t)hi32 << 32) | lo32;
+    }
+    S.m_pre[lane + 1] = incl;
+    if (lane == 0) {
+        S.m_pre[0] = 0;
+        S.gen_cnt[g] = cnt;
+    }
+    const uint32_t tl = __shfl((uint32_t)incl, kWave - 1), th = __shfl((uint32_t)(incl >> 32), kWave - 1);
+    __syncthreads();
+    cnt_out = cnt;
+    return ((uint64_t)th << 32) | tl;
+}
+
+// Size of the contiguous run of equal tags starting at frame position lo.
+template <typename F>
+__device__ __forceinline__ uint32_t run_length(const F& S, uint32_t depth, uint32_t lo) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t n = S.ccount[depth];
+    const uint8_t tag = S.ctag[depth][lo];
+    const bool in = lane >= lo && lane < n && S.ctag[depth][lane] == tag;
+    return (uint32_t)__popcll(__ballot(in));
+}
+
+template <uint32_t N>
+__global__ __launch_bounds__(kWave) void k_dfs_wave(
+    const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
+    const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
+    const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* ready, DQueue* Q,
+    QCounters* qc, uint32_t* pending, uint32_t max_n) {
+    const uint32_t lane = threadIdx.x;
+    __shared__ WaveFrame<N> S;
+    if (lane == 0 && Q->limit_ticks)
+        atomicCAS(&Q->deadline, 0ull, (unsigned long long)wall_clock64() + Q->limit_ticks);
+    const uint32_t hungry_below = gridDim.x / 2;   // queue shorter than half the wavefronts
+    unsigned long long bytes = 0, busy_pos = 0, all_pos = 0;
+
+    while (true) {
+        // ---- claim one task
+        uint32_t slot = kInvalid;
+        if (lane == 0) {
+            const uint32_t h = load_relaxed(&Q->head), tl = load_relaxed(&Q->tail);
+            if (h < tl && atomicCAS(&Q->head, h, h + 1) == h) {
+                atomicAdd(&Q->busy, 1u);
+                slot = h;
+            }
+        }
+        slot = __shfl(slot, 0);
+        if (slot == kInvalid) {
+            all_pos += 1;
+            uint32_t stop = 0;
+            if (lane == 0)
+                stop = load_relaxed(&Q->timeout) ||
+                       (load_relaxed(&Q->head) >= load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0);
+            if (__shfl(stop, 0)) break;
+            __builtin_amdgcn_s_sleep(8);
+            continue;
+        }
+        while (!load_acquire(ready + slot)) __builtin_amdgcn_s_sleep(1);
+        const DTask& task = queue[slot];
+        const uint32_t q = task.q;
+        const DPlan& P = plans[q];
+        uint32_t drop = 0;
+        if (lane == 0) {
+            drop = load_relaxed(&Q->timeout) || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED);
+            if (!drop && past_deadline(Q)) {
+                atomicExch(&Q->timeout, 1u);
+                drop = 1;
+            }
+            if (drop) atomicSub(&Q->busy, 1u);
+        }
+        if (__shfl(drop, 0)) continue;
+        const uint32_t n = P.n, t = task.depth;
+        if (lane < t) {
+            const uint32_t x = task.prefix[lane];
+            S.I[lane] = x;
+            S.M[lane] = cand_pool[P.cand_base[lane] + x];
+        }
+        if (lane == 0) {
+            S.gen_m[t] = 0;
+            S.gen_cnt[t] = 1;
+            if (task.hi == kAutoHi) {
+                S.gen_pos[t] = kFresh;
+            } else {
+                S.gen_pos[t] = task.lo;
+                S.gen_end[t] = task.hi;
+                S.gen_sel[t] = task.sel;
+            }
+        }
+        __syncthreads();
+
+        uint32_t g = t, cnt = 0, it = 0;
+        int cache_g = -1;
+        uint64_t total = 0, flat = 0;
+        unsigned long long lcount = 0, lcalls = 0;
+        bool finished = false;
+        while (true) {
+            if (cache_g != (int)g) {
+                total = wave_build_cache(P, ops, row_off, cand_pool, S, g, t, cnt, bytes);
+                flat = 0;
+                cache_g = (int)g;
+            }
+            if (flat >= total) {
+                // generation g exhausted: its members' run in frame g-1 is done
+                if (g == t) {
+                    finished = true;
+                    break;
+                }
+                const uint32_t f = g - 1;
+                const uint32_t lo = S.run_hi[f];
+                if (lo < S.ccount[f]) {
+                    const uint32_t len = run_length(S, f, lo);
+                    const uint32_t tag = S.ctag[f][lo];
+                    __syncthreads();
+                    if (lane == 0) {
+                        S.run_lo[f] = lo;
+                        S.run_hi[f] = lo + len;
+                        if (f > t) {
+                            // parent of the new run: member `tag` of generation f
+                            const uint32_t x = S.cx[f - 1][S.run_lo[f - 1] + tag];
+                            S.I[f - 1] = x;
+                            S.M[f - 1] = cand_pool[P.cand_base[f - 1] + x];
+                        }
+                        S.gen_m[g] = 0;
+                        S.gen_pos[g] = kFresh;
+                    }
+                    __syncthreads();
+                } else {
+                    g = f;   // resume generation f at its cursor
+                }
+                cache_g = -1;
+                continue;
+            }
+
+            // ---- one chunk of 64 positions of generation g
+            const uint64_t fpos = flat + lane;
+            bool ok = false;
+            uint32_t xx = 0, k = 0;
+            all_pos += 1;
+            if (fpos < total) {
+                ++busy_pos;
+                uint32_t lo = 0, hi = cnt;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (S.m_pre[mid] <= fpos) lo = mid;
+                    else hi = mid;
+                }
+                k = lo;
+                const uint64_t pos = S.m_lo[k] + (fpos - S.m_pre[k]);
+                const uint32_t sel = S.m_sel[k];
+                if (sel == kIdentity) {
+                    xx = (uint32_t)pos;
+                } else {
+                    xx = idx[pos];
+                    bytes += 4;
+                }
+                ok = true;
+                const uint32_t nops = P.op_count[g];
+                const uint32_t nother = sel == kIdentity ? 0u : nops - 1;
+                const uint32_t ncache = min(nother, kWCache);
+                for (uint32_t j = 0; j < ncache && ok; ++j)
+                    ok = row_has(idx, S.m_olo[j][k], S.m_olen[j][k], xx, bytes);
+                if (ok && nother > kWCache) {
+                    uint32_t slot2 = 0;
+                    for (uint32_t j = 0; j < nops && ok; ++j) {
+                        if (j == sel) continue;
+                        if (slot2++ < kWCache) continue;
+                        const DOp op = ops[P.op_begin[g] + j];
+                        const uint64_t r = op.row_base + op_source(S, op, g, t, S.m_x[k]);
+                        const uint64_t a = row_off[r], b = row_off[r + 1];
+                        bytes += 16;
+                        ok = row_has(idx, a, b - a, xx, bytes);
+                    }
+                }
+                if (ok) {
+                    const uint32_t v = cand_pool[P.cand_base[g] + xx];
+                    bytes += 4;
+                    unsigned long long mask = P.same_label[g];
+                    while (mask) {
+                        const uint32_t e = __ffsll((long long)mask) - 1;
+                        mask &= mask - 1;
+                        const uint32_t me = (g > t && e == g - 1) ? S.m_v[k] : S.M[e];
+                        if (me == v) {
+                            ok = false;
+                            break;
+                        }
+                    }
+                }
+            }
+            const unsigned long long bal = __ballot(ok);
+            const uint32_t ns = (uint32_t)__popcll(bal);
+            const bool leaf = g + 1 == n;
+            if (!leaf && ok) {
+                const uint32_t p = (uint32_t)__popcll(bal & lanemask_lt());
+                S.cx[g][p] = xx;
+                S.ctag[g][p] = (uint8_t)k;
+            }
+            // advance the cursor past this chunk
+            flat += min<uint64_t>(kWave, total - flat);
+            {
+                const bool here = lane < cnt && S.m_pre[lane] <= flat && flat < S.m_pre[lane] + S.m_len[lane];
+                const unsigned long long hb = __ballot(here);
+                if (here) {
+                    S.gen_m[g] = lane;
+                    S.gen_pos[g] = S.m_lo[lane] + (flat - S.m_pre[lane]);
+                    S.gen_end[g] = S.m_lo[lane] + S.m_len[lane];
+                    S.gen_sel[g] = S.m_sel[lane];
+                }
+                if (!hb && lane == 0) S.gen_m[g] = cnt;
+            }
+            lcalls += ns;
+            if (leaf) {
+                lcount += ns;
+            } else if (ns) {
+                if (lane == 0) {
+                    S.ccount[g] = ns;
+                    S.run_lo[g] = 0;
+                }
+                __syncthreads();
+                const uint32_t len = run_length(S, g, 0);
+                const uint32_t tag = S.ctag[g][0];
+                if (lane == 0) {
+                    S.run_hi[g] = len;
+                    if (g > t) {
+                        S.I[g - 1] = S.m_x[tag];
+                        S.M[g - 1] = S.m_v[tag];
+                    }
+                    S.gen_m[g + 1] = 0;
+                    S.gen_pos[g + 1] = kFresh;
+                }
+                ++g;
+                cache_g = -1;
+            }
+            __syncthreads();
+
+            ++it;
+            if ((it & 15) == 0 || lcount >= 65536) {
+                uint32_t stop = 0;
+                if (lane == 0) {
+                    if (lcount | lcalls) {
+                        const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
+                        atomicAdd(&qc[q].calls, lcalls);
+                        if (lcount && prev + lcount >= P.cap) atomicExch(&qc[q].done, 1u);
+                    }
+                    stop = __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout);
+                    if (!stop && past_deadline(Q)) {
+                        atomicExch(&Q->timeout, 1u);
+                        stop = 1;
+                    }
+                }
+                lcount = lcalls = 0;
+                if (__shfl(stop, 0)) break;
+            }
+            // split off work while the queue is short
+            if ((it & 7) == 0) {
+                uint32_t hungry = 0;
+                if (lane == 0) hungry = load_relaxed(&Q->tail) - load_relaxed(&Q->head) < hungry_below;
+                if (__shfl(hungry, 0)) {
+                    const uint32_t e = t + lane;
+                    bool can = false;
+                    if (e <= g && e <= kPrefixMax && (e < g || cache_g == (int)g)) {
+                        can = S.gen_m[e] < S.gen_cnt[e] && S.gen_pos[e] != kFresh &&
+                              S.gen_end[e] - S.gen_pos[e] >= 2;
+                    }
+                    const unsigned long long cb = __ballot(can);
+                    if (cb) {
+                        const uint32_t ed = t + (uint32_t)(__ffsll((long long)cb) - 1);
+                        uint32_t ns2 = kInvalid;
+                        if (lane == 0) {
+                            const uint32_t tl = load_relaxed(&Q->tail);
+                            if (tl < Q->cap && atomicCAS(&Q->tail, tl, tl + 1) == tl) ns2 = tl;
+                        }
+                        ns2 = __shfl(ns2, 0);
+                        if (ns2 != kInvalid) {
+                            const uint64_t gp = S.gen_pos[ed], ge = S.gen_end[ed];
+                            const uint64_t mid = gp + (ge - gp) / 2;
+                            DTask& nt = queue[ns2];
+                            if (lane < ed) {
+                                const bool member = ed > t && lane == ed - 1;
+                                nt.prefix[lane] =
+                                    member ? S.cx[ed - 1][S.run_lo[ed - 1] + S.gen_m[ed]] : S.I[lane];
+                            }
+                            if (lane == 0) {
+                                nt.q = q;
+                                nt.depth = ed;
+                                nt.sel = S.gen_sel[ed];
+                                nt.pad = 0;
+                                nt.lo = mid;
+                                nt.hi = ge;
+                                atomicAdd(pending + q, 1u);
+                                atomicAdd(&Q->donated, 1u);
+                            }
+                            __threadfence();
+                            __syncthreads();
+                            if (lane == 0) {
+                                __hip_atomic_store(ready + ns2, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                                S.gen_end[ed] = mid;
+                            }
+                            __syncthreads();
+                            if ((int)ed == cache_g) cache_g = -1;
+                        }
+                    }
+                }
+            }
+        }
+        // task ended (finished, or stopped by the cap / time limit)
+        if (lane == 0) {
+            if (lcount | lcalls) {
+                const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
+                atomicAdd(&qc[q].calls, lcalls);
+                if (lcount && prev + lcount >= P.cap) atomicExch(&qc[q].done, 1u);
+            }
+            if (finished) atomicSub(pending + q, 1u);
+            atomicSub(&Q->busy, 1u);
+        }
+        wave_add(&qc[q].probe_bytes, bytes);
+        bytes = 0;
+    }
+    wave_add(&Q->lane_steps, busy_pos);
+    wave_add(&Q->lane_iters, all_pos);
 }
 
 // ---------------------------------------------------------------------------
@@ -1382,6 +1856,28 @@ hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_of
     const uint32_t blocks = grid_waves / (kBlock / kWave);
     hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(kBlock), 0, s, plans, ops, row_off, idx, cand_pool,
                        queue, ready, dq, qc, pending, max_n, lane_state, sink);
+    return hipGetLastError();
+}
+
+size_t dfs_wave_lds_bytes(uint32_t max_n) {
+    return max_n <= 16 ? sizeof(WaveFrame<16>) : max_n <= 32 ? sizeof(WaveFrame<32>) : sizeof(WaveFrame<64>);
+}
+
+hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
+                           const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
+                           DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
+                           hipStream_t s) {
+    if (grid_waves == 0) return hipSuccess;
+    if (max_n == 0 || max_n > kMaxQ) return hipErrorInvalidValue;
+    if (max_n <= 16)
+        hipLaunchKernelGGL(k_dfs_wave<16>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
+                           cand_pool, queue, ready, dq, qc, pending, max_n);
+    else if (max_n <= 32)
+        hipLaunchKernelGGL(k_dfs_wave<32>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
+                           cand_pool, queue, ready, dq, qc, pending, max_n);
+    else
+        hipLaunchKernelGGL(k_dfs_wave<64>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
+                           cand_pool, queue, ready, dq, qc, pending, max_n);
     return hipGetLastError();
 }
 

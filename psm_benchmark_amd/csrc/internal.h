@@ -122,6 +122,8 @@ struct DQueue {
     unsigned int donated;      // tasks split off by running lanes
     unsigned long long limit_ticks;   // wall-clock ticks allowed (0: no limit)
     unsigned long long deadline;      // set by the first lane to start
+    unsigned long long lane_steps;    // loop iterations in which a lane held a task
+    unsigned long long lane_iters;    // loop iterations of all lanes
 };
 
 // Where the DFS writes complete embeddings (rows of n data vertex ids in
@@ -221,6 +223,11 @@ hipError_t launch_join_probe(const uint32_t* build_rows, uint32_t build_width, J
                              unsigned long long* bytes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 uint32_t dfs_lane_state_words(uint32_t max_n);
+size_t dfs_wave_lds_bytes(uint32_t max_n);
+hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
+                           const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
+                           DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
+                           hipStream_t s);
 uint64_t scan_tmp_entries(uint64_t n);
 hipError_t launch_scan_u32(const uint32_t* in, uint64_t n, uint64_t* out, unsigned long long* tmp,
                            hipStream_t s);

@@ -45,6 +45,10 @@ struct psm_ctx {
     std::vector<hipEvent_t> ev_pool;
     unsigned long long* d_kbytes = nullptr;   // kernel-counted algorithmic bytes per class
     double time_limit_s = 60.0;               // reference default -time_limit 60
+    // enumeration launch shape (psm_set_tuning)
+    uint32_t dfs_waves_per_cu = 8;            // persistent DFS grid
+    double expand_tasks_per_lane = 2.0;       // pre-expansion target
+    uint32_t dfs_strategy = PSM_DFS_LANE;     // lane-per-task or wavefront-per-task DFS
     double wall_khz = 100000.0;               // device wall clock rate
     bool verbose = false;
 };
@@ -366,6 +370,7 @@ struct Batch {
     DBuf plans, ops, qcnt;
     double t_filter = 0, t_table = 0, t_plan = 0, t_enum = 0, t_expand = 0;
     uint64_t tasks_run = 0;
+    uint64_t lane_steps = 0, lane_iters = 0;   // DFS lane utilisation
 
     hipStream_t s() const { return ctx->stream; }
 
@@ -1019,9 +1024,12 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         tasks.push_back(t);
         ntask[i] = 1;
     }
-    // persistent grid: 8 wavefronts per CU
-    const uint32_t grid_waves_full = (uint32_t)ctx->num_cu * 8;
-    const uint64_t target_total = (uint64_t)grid_waves_full * 64 * 2;
+    // persistent grid: dfs_waves_per_cu wavefronts per CU; the wavefront
+    // kernel takes one task per wavefront, the lane kernel one per lane
+    const bool wave_dfs = ctx->dfs_strategy == PSM_DFS_WAVE && sink.rows == nullptr;
+    const uint32_t grid_waves_full = (uint32_t)ctx->num_cu * ctx->dfs_waves_per_cu;
+    const uint64_t target_total = std::max<uint64_t>(
+        64, (uint64_t)((double)grid_waves_full * (wave_dfs ? 1 : 64) * ctx->expand_tasks_per_lane));
     const uint64_t budget_total = std::max<uint64_t>(target_total * 8, 1u << 22);
     uint32_t active_q = 0;
     for (uint32_t i = 0; i < nq; ++i) active_q += ntask[i] ? 1 : 0;
@@ -1094,22 +1102,33 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         HIP_TRY(ctx, d_q.alloc(sizeof(DQueue)));
         HIP_TRY(ctx, hipMemcpyAsync(d_q.p, &hq, sizeof(hq), hipMemcpyHostToDevice, s()));
         const uint32_t gw = grid_waves_full / 4 * 4;
-        HIP_TRY(ctx, d_state.alloc((size_t)gw * 64 * dfs_lane_state_words(max_n) * 4));
+        if (!wave_dfs) HIP_TRY(ctx, d_state.alloc((size_t)gw * 64 * dfs_lane_state_words(max_n) * 4));
         std::vector<uint32_t> pend(nq);
         for (uint32_t i = 0; i < nq; ++i) pend[i] = (uint32_t)ntask[i];
         HIP_TRY(ctx, upload(d_pending, pend, s()));
         const double td0 = now_ms();
-        TIMED(ctx, kKDfs,
-              launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
-                         cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_ready.as<uint32_t>(),
-                         d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
-                         d_state.as<uint32_t>(), sink, s()));
+        if (wave_dfs) {
+            TIMED(ctx, kKDfs,
+                  launch_dfs_wave(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
+                                  cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_ready.as<uint32_t>(),
+                                  d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
+                                  s()));
+        } else {
+            TIMED(ctx, kKDfs,
+                  launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
+                             cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_ready.as<uint32_t>(),
+                             d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
+                             d_state.as<uint32_t>(), sink, s()));
+        }
         HIP_TRY(ctx, hipMemcpyAsync(&hq, d_q.p, sizeof(hq), hipMemcpyDeviceToHost, s()));
         HIP_TRY(ctx, timers_flush(ctx));
         t_enum += now_ms() - td0;
         timed_out = hq.timeout != 0;
         donated = hq.donated;
-        verbose("dfs: %.1f ms, %u tasks split off, timeout %u", now_ms() - td0, donated, hq.timeout);
+        lane_steps += hq.lane_steps;
+        lane_iters += hq.lane_iters;
+        verbose("dfs: %.1f ms, %u tasks split off, timeout %u, lanes busy %.3f", now_ms() - td0, donated,
+                hq.timeout, hq.lane_iters ? (double)hq.lane_steps / (double)hq.lane_iters : 0.0);
         // a query with queued or unfinished tasks and no cap reached is incomplete
         std::vector<uint32_t> left;
         HIP_TRY(ctx, download(left, d_pending.p, nq, s()));
@@ -1342,6 +1361,37 @@ int psm_set_time_limit(psm_ctx* ctx, double seconds) {
     if (!ctx || seconds < 0) return PSM_EINVAL;
     ctx->time_limit_s = seconds;
     return PSM_OK;
+}
+
+int psm_set_tuning(psm_ctx* ctx, int key, double value) {
+    if (!ctx) return PSM_EINVAL;
+    switch (key) {
+        case PSM_TUNE_DFS_WAVES_PER_CU:
+            if (value < 1 || value > 32) return set_err(ctx, PSM_EINVAL, "dfs waves per CU outside [1, 32]");
+            ctx->dfs_waves_per_cu = (uint32_t)value;
+            return PSM_OK;
+        case PSM_TUNE_EXPAND_TASKS_PER_LANE:
+            if (!(value > 0 && value <= 64)) return set_err(ctx, PSM_EINVAL, "expand tasks per lane outside (0, 64]");
+            ctx->expand_tasks_per_lane = value;
+            return PSM_OK;
+        case PSM_TUNE_DFS_STRATEGY:
+            if (value != PSM_DFS_LANE && value != PSM_DFS_WAVE)
+                return set_err(ctx, PSM_EINVAL, "dfs strategy must be PSM_DFS_LANE or PSM_DFS_WAVE");
+            ctx->dfs_strategy = (uint32_t)value;
+            return PSM_OK;
+        default:
+            return set_err(ctx, PSM_EINVAL, "unknown tuning key");
+    }
+}
+
+int psm_get_tuning(psm_ctx* ctx, int key, double* value) {
+    if (!ctx || !value) return PSM_EINVAL;
+    switch (key) {
+        case PSM_TUNE_DFS_WAVES_PER_CU: *value = ctx->dfs_waves_per_cu; return PSM_OK;
+        case PSM_TUNE_EXPAND_TASKS_PER_LANE: *value = ctx->expand_tasks_per_lane; return PSM_OK;
+        case PSM_TUNE_DFS_STRATEGY: *value = ctx->dfs_strategy; return PSM_OK;
+        default: return set_err(ctx, PSM_EINVAL, "unknown tuning key");
+    }
 }
 
 int psm_kernel_stats_reset(psm_ctx* ctx) {
@@ -1591,6 +1641,8 @@ int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32_
         st->enum_ms = b.t_enum + b.t_expand;
         st->total_ms = now_ms() - t0;
         st->tasks = b.tasks_run;
+        st->dfs_lane_steps = b.lane_steps;
+        st->dfs_lane_iters = b.lane_iters;
         for (auto& h : hosts) st->probe_bytes += h->probe_bytes;
     }
     return PSM_OK;

@@ -19,10 +19,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 HPRD = os.path.join(HERE, "golden", "hprd")
 
 
-@pytest.fixture(scope="module")
-def dev():
+@pytest.fixture(scope="module", params=["lane", "wave"])
+def dev(request):
+    """Every device test runs with both enumeration kernels: one search per
+    lane, and one search per wavefront with chunked cooperative tests."""
+    from psm_benchmark_amd import autotune
     from psm_benchmark_amd.matcher import Device
     d = Device(0)
+    autotune.set_config(d, {"dfs_strategy": request.param})
+    print("dfs strategy", request.param, flush=True)
     yield d
     d.close()
 
