@@ -149,12 +149,20 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", type=int, default=512, help="use the first N workload queries")
-    ap.add_argument("--time-limit", type=float, default=2.0,
-                    help="enumeration time limit in seconds; 2 s is the -time_limit of the "
-                         "reference's recorded runs (test_anony/survey_bash/C_exp.py)")
+    ap.add_argument("--time-limit", type=float, default=None,
+                    help="time limit in seconds (default 2 s for C2, the -time_limit of the "
+                         "reference's recorded runs, test_anony/survey_bash/C_exp.py; 20 s for "
+                         "the uncapped C3 / C5 workloads)")
+    ap.add_argument("--plain-limit", type=float, default=5.0,
+                    help="C3 / C5: time limit of one plain-path query (s)")
+    ap.add_argument("--plain-budget", type=float, default=60.0,
+                    help="C3 / C5: total time spent on plain-path and join queries (s)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--requests", type=int, default=1000, help="C4: requests per arrival rate")
+    ap.add_argument("--workers", type=int, default=4, help="C4: server workers (HIP streams)")
+    ap.add_argument("--max-batch", type=int, default=128, help="C4: requests per batch at most")
     ap.add_argument("--tune-sample", type=int, default=128, help="queries per autotuner trial")
-    ap.add_argument("--tune-limit", type=float, default=0.5, help="time limit of an autotuner trial (s)")
+    ap.add_argument("--tune-limit", type=float, default=0.8, help="time limit of an autotuner trial (s)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain MIB-MiB device copy (PMC counter calibration)")
     ap.add_argument("--pmc-summary", nargs=6,
@@ -173,6 +181,14 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (queries shard "
               "independently across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.time_limit is None:
+        args.time_limit = 2.0 if args.workload == "C2" else 20.0
+    if args.workload in ("C3", "C5"):
+        large_workload(args)
+        return
+    if args.workload == "C4":
+        serving_workload(args)
+        return
 
     from psm_benchmark_amd import workloads as W
     from psm_benchmark_amd.matcher import Device, DeviceGraph, Query, batch_match, match
@@ -293,6 +309,196 @@ def main():
     }
     if cpu:
         out["batched_over_cpu"] = round(out["value"] / cpu["value"], 1)
+    print(json.dumps(out))
+
+
+def roofline_of(ks, steps, traffic_file=None):
+    dom = max(ks.values(), key=lambda k: k["ms"])
+    achieved = dom["bytes"] / dom["launches"] / (dom["ms"] / dom["launches"] / 1e3) / 1e9
+    traffic = None
+    if traffic_file and os.path.exists(traffic_file):
+        tr = json.load(open(traffic_file))
+        if tr.get("kernel") == "k_" + dom["name"] and tr.get("traffic_bytes") == tr.get("traffic_bytes"):
+            traffic = int(tr["traffic_bytes"])
+    return dict(kernel=dom["name"], bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                avg_launch_ms=round(dom["ms"] / dom["launches"], 4),
+                bytes_per_launch=int(dom["bytes"] / dom["launches"]))
+
+
+def large_workload(args):
+    """C3 (Chung-Lu 4M V / 35M E, 32 Zipf(1.0) labels, 64 cyclic queries of
+    12-16 vertices, CFL_CFL_EXPLORE, plus the two-unit join) and C5 (R-MAT
+    2^24 V / 100M E, 8 uniform labels, 32 dense queries of 6-10 vertices with
+    triangles and 4-cliques, GQL_GQL_GQL): uncapped counts under a time limit.
+    The batched form runs all queries in one pipeline; the plain path and the
+    join run query by query within a time budget and must agree with it on
+    every query that none of them stopped early."""
+    from psm_benchmark_amd import autotune
+    from psm_benchmark_amd import workloads as W
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.matcher import Device, DeviceGraph, batch_match, join_match, match
+
+    name = args.workload
+    method = "CFL_CFL_EXPLORE" if name == "C3" else "GQL_GQL_GQL"
+    f, o, _ = method.split("_")
+    t0 = time.time()
+    g = W.data_graph(name)
+    qs, meta = (W.c3_queries if name == "C3" else W.c5_queries)(g)
+    qs, meta = qs[: args.queries], meta[: args.queries]
+    log(f"workload {name}: V={g.num_vertices} E={g.num_edges} queries={len(qs)} ({time.time() - t0:.1f}s)")
+    dev = Device(0)
+    dg = DeviceGraph(dev, g)
+    tuned = None
+    if not args.no_tune:
+        tuned, _ = autotune.tune(dg, qs, f, o, 0, sample=len(qs), trial_limit_s=max(1.0, args.tune_limit),
+                                 restore_limit_s=args.time_limit, log=log)
+        log(f"autotune picked {tuned}")
+    L.lib().psm_set_time_limit(dev.h, args.time_limit)
+    L.lib().psm_kernel_stats_reset(dev.h)
+    times, nodes = [], 0
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        counts, calls, status, st = batch_match(dg, qs, f, o, 0)
+        times.append(time.perf_counter() - t)
+        nodes += int(calls.sum())
+        log(f"batched step: {times[-1] * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
+            f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f}), timeouts {int((status == L.PSM_ETIMEOUT).sum())}")
+    ks = kernel_stats(dev)
+    total = sum(times)
+    done = status != L.PSM_ETIMEOUT
+    # plain path (and the join for C3), query by query within the budget
+    L.lib().psm_set_time_limit(dev.h, args.plain_limit)
+    plain, joins, lat = [], [], []
+    spent = 0.0
+    for i, q in enumerate(qs):
+        if spent > args.plain_budget:
+            break
+        t = time.perf_counter()
+        r = match(dg, q, method)
+        lat.append(time.perf_counter() - t)
+        spent += lat[-1]
+        plain.append((i, r.count, r.timed_out))
+        if name == "C3" and not r.timed_out:
+            t = time.perf_counter()
+            try:
+                js = join_match(dg, q, method)
+                joins.append((i, int(js.count), js.count == r.count))
+            except L.PSMError as e:
+                joins.append((i, None, str(e)))
+            spent += time.perf_counter() - t
+    mismatch = [i for i, c, to in plain if not to and done[i] and c != counts[i]]
+    if mismatch:
+        raise SystemExit(f"batched and plain counts differ for queries {mismatch[:10]}")
+    bad_join = [i for i, c, okj in joins if okj is False]
+    if bad_join:
+        raise SystemExit(f"join and plain counts differ for queries {bad_join[:10]}")
+    lat_ms = np.array(lat if lat else [float("nan")]) * 1e3
+    nq = len(qs)
+    out = {
+        "metric": "queries/s + embeddings/s on 1 MI355X; p50/p99 latency batched vs plain",
+        "value": round(nq * args.steps / total, 3),
+        "unit": "queries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": round(total / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded graph and queries)",
+        "config": {"workload": ("C3: Chung-Lu 4M V / 35M E, 32 Zipf(1.0) labels, 64 cyclic queries of "
+                                "12-16 vertices, CFL_CFL_EXPLORE, uncapped" if name == "C3" else
+                                "C5: R-MAT(0.57,0.19,0.19) 2^24 V / 100M E, 8 uniform labels, 32 dense "
+                                "queries of 6-10 vertices with triangles and 4-cliques, GQL_GQL_GQL, uncapped"),
+                   "queries": nq, "time_limit_s": args.time_limit},
+        "timeouts": int((~done).sum()),
+        "completed_queries_per_s": round(int(done.sum()) * args.steps / total, 3),
+        "embeddings_per_s": round(float(counts.sum()) * args.steps / total, 1),
+        "embeddings_completed": int(counts[done].sum()),
+        "search_nodes_per_s": round(nodes / total, 1),
+        "launch_shape": tuned,
+        "plain": {"queries": len(plain), "timeouts": int(sum(to for _, _, to in plain)),
+                  "time_limit_s": args.plain_limit,
+                  "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
+                  "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
+                  "queries_per_s": round(len(lat) / max(1e-9, sum(lat)), 3)},
+        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"] / args.steps, 3),
+                            bytes=int(v["bytes"] / args.steps)) for k, v in ks.items()},
+        "roofline": roofline_of(ks, args.steps),
+        "cpu_baseline": None,
+    }
+    if name == "C3":
+        out["join"] = {"queries": len(joins), "equal_to_plain": int(sum(1 for j in joins if j[2] is True)),
+                       "failed": [j[0] for j in joins if isinstance(j[2], str)]}
+    print(json.dumps(out))
+
+
+def serving_workload(args):
+    """C4: the C3 graph resident in HBM; random-walk queries of 4-32 vertices
+    (dense / sparse 50/50) arrive as seeded Poisson streams at 200, 1,000 and
+    5,000 QPS into the continuous-batching server (memory-based admission,
+    cap 1e5, GQL_GQL_GQL). Reports p50/p99 latency (completion - arrival) and
+    sustained queries/s per rate beside the plain one-query-at-a-time path,
+    and checks every served count against the plain count of the same query."""
+    from psm_benchmark_amd import autotune
+    from psm_benchmark_amd import workloads as W
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.matcher import Device, DeviceGraph, match
+    from psm_benchmark_amd.serving import Server, run_poisson
+
+    t0 = time.time()
+    g = W.data_graph("C4")
+    qs, meta = W.c2_queries(g, count=args.queries, seed=44)
+    log(f"workload C4: V={g.num_vertices} E={g.num_edges} queries={len(qs)} ({time.time() - t0:.1f}s)")
+    dev = Device(0)
+    dg = DeviceGraph(dev, g)
+    tuned = None
+    if not args.no_tune:
+        tuned, _ = autotune.tune(dg, qs, "GQL", "GQL", args.cap, sample=min(len(qs), args.tune_sample),
+                                 trial_limit_s=args.tune_limit, restore_limit_s=args.time_limit, log=log)
+        log(f"autotune picked {tuned}")
+    L.lib().psm_set_time_limit(dev.h, args.time_limit)
+    # plain path: one query at a time
+    lat, plain_counts, plain_to = [], [], []
+    for q in qs[: args.plain]:
+        t = time.perf_counter()
+        r = match(dg, q, "GQL_GQL_GQL", cap=args.cap)
+        lat.append(time.perf_counter() - t)
+        plain_counts.append(r.count)
+        plain_to.append(r.timed_out)
+    lat_ms = np.array(lat) * 1e3
+    plain = {"queries": len(lat), "queries_per_s": round(len(lat) / sum(lat), 2),
+             "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+             "p99_ms": round(float(np.percentile(lat_ms, 99)), 3), "timeouts": int(sum(plain_to))}
+    log(f"plain: {plain}")
+    rates = []
+    for rate in (200.0, 1000.0, 5000.0):
+        srv = Server(dg, workers=args.workers, max_batch=args.max_batch, filter="GQL", order="GQL",
+                     cap=args.cap, time_limit=args.time_limit)
+        try:
+            res, summary = run_poisson(srv, qs, rate_qps=rate, n_requests=args.requests, seed=int(rate))
+            summary["server"] = srv.stats()
+        finally:
+            srv.close()
+        bad = [k for k, r in res.items() if k % len(qs) < len(plain_counts) and r.status == 0
+               and not plain_to[k % len(qs)] and r.count != plain_counts[k % len(qs)]]
+        if bad:
+            raise SystemExit(f"served counts differ from the plain path for requests {bad[:10]}")
+        summary["checked_against_plain"] = sum(1 for k in res if k % len(qs) < len(plain_counts))
+        log(f"rate {rate}: {summary}")
+        rates.append(summary)
+    best = max(r["sustained_qps"] for r in rates)
+    out = {
+        "metric": "queries/s + embeddings/s on 1 MI355X; p50/p99 latency batched vs plain",
+        "value": best, "unit": "queries/s", "n_gpus": 1, "steps": len(rates), "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic (seeded graph, queries and Poisson arrivals)",
+        "config": {"workload": "C4 serving: Chung-Lu 4M V / 35M E resident, 32 Zipf(1.0) labels, random-walk "
+                               "queries of 4-32 vertices (dense/sparse 50/50), Poisson 200/1000/5000 QPS, "
+                               "GQL_GQL_GQL, cap 1e5",
+                   "requests_per_rate": args.requests, "workers": args.workers, "max_batch": args.max_batch,
+                   "cap": args.cap, "time_limit_s": args.time_limit},
+        "rates": rates, "plain": plain,
+        "served_over_plain": round(best / plain["queries_per_s"], 2),
+        "launch_shape": tuned, "cpu_baseline": None,
+    }
     print(json.dumps(out))
 
 

@@ -80,9 +80,11 @@ const char* psm_last_error(const psm_ctx* ctx);
 int psm_kernel_stats(psm_ctx* ctx, int kind, double* out3);
 int psm_kernel_stats_reset(psm_ctx* ctx);
 
-/* The reference's -time_limit (StudyPerformance.cpp:1134-1137): enumeration
- * stops at this many seconds and returns PSM_ETIMEOUT with partial counts
- * (reference: "Overtime: 1"). Default 60 s; 0 disables the limit. */
+/* The reference's -time_limit (StudyPerformance.cpp:1134-1148): counted from
+ * the start of filtering, as the reference sets its deadline before
+ * FilterVertices, so filter, tables, plan and enumeration share it;
+ * enumeration stops when it passes and returns PSM_ETIMEOUT with partial
+ * counts (reference: "Overtime: 1"). Default 60 s; 0 disables the limit. */
 int psm_set_time_limit(psm_ctx* ctx, double seconds);
 
 /* Enumeration launch shape, set by the autotuner (psm_benchmark_amd/autotune)
@@ -199,6 +201,24 @@ int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t num_queries, cons
                     int filter, int order, uint64_t cap, uint64_t* counts, uint64_t* calls,
                     uint32_t* status, psm_batch_stats* stats);
 
+/* The batched form stage by stage, so that the enumeration can run again
+ * (different cap, limit or launch shape) without filtering again:
+ * psm_batch_create takes the queries as psm_batch_match does;
+ * psm_batch_prepare runs filter, tables and plans; psm_batch_evaluate
+ * enumerates. enum_time_limit_s < 0 applies the context's time limit counted
+ * from the start of psm_batch_prepare (the reference's deadline), >= 0 a
+ * limit of that many seconds counted from this call (0: none). psm_batch_match
+ * is create + prepare + evaluate(-1) + destroy. */
+typedef struct psm_batch psm_batch;
+int psm_batch_create(psm_ctx* ctx, const psm_graph* g, uint32_t num_queries, const uint32_t* n,
+                     const uint64_t* off_base, const uint64_t* nbr_base, const uint64_t* lab_base,
+                     const uint32_t* q_offsets, const uint32_t* q_neighbors, const uint32_t* q_labels,
+                     psm_batch** out);
+int psm_batch_prepare(psm_batch* b, int filter, int order);
+int psm_batch_evaluate(psm_batch* b, uint64_t cap, double enum_time_limit_s, uint64_t* counts,
+                       uint64_t* calls, uint32_t* status, psm_batch_stats* stats);
+int psm_batch_destroy(psm_batch* b);
+
 /* ---- serving form ----------------------------------------------------------
  * Continuous batching over a resident data graph: `workers` threads, each with
  * its own HIP stream, admit the oldest pending requests as one batch while the
@@ -207,7 +227,8 @@ int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t num_queries, cons
  * device memory); later arrivals join the next batch a worker forms, and
  * batches of different workers overlap on the device. Every batch runs
  * filter, tables, plan and the persistent enumeration with `cap` and the
- * time limit. Times in psm_result are milliseconds since server creation.
+ * time limit, with the creating context's launch shape (psm_set_tuning).
+ * Times in psm_result are milliseconds since server creation.
  * No counterpart in the reference, which runs one query per process. */
 typedef struct psm_server psm_server;
 typedef struct psm_result {
@@ -241,8 +262,9 @@ int psm_gen_rmat(uint32_t scale, uint64_t E, double a, double b, double c, uint6
                  uint32_t* offsets, uint32_t* neighbors);
 /* zipf_s <= 0 draws labels uniformly */
 int psm_gen_labels(uint32_t V, uint32_t L, double zipf_s, uint64_t seed, uint32_t* labels);
-/* A connected n-vertex query drawn from the data graph (random walk tree when
- * dense = 0, greedy dense induced subgraph when dense = 1). Returns the number
+/* A connected n-vertex query drawn from the data graph: dense = 0 the tree of
+ * a random walk, 1 the induced subgraph of a greedy densest growth, 2 the
+ * induced subgraph of a random walk's vertices (cycles). Returns the number
  * of query edges (pairs written to out_edges) or a negative status. */
 int64_t psm_gen_query(uint32_t V, const uint32_t* offsets, const uint32_t* neighbors, uint32_t n,
                       int dense, uint64_t seed, uint32_t* out_vertices, uint32_t* out_edges,

@@ -112,6 +112,12 @@ SIGNATURES = {
     "psm_batch_match": (ctypes.c_int, [VP, VP, ctypes.c_uint32, U32P, U64P, U64P, U64P, U32P, U32P, U32P,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_uint64, U64P, U64P, U32P,
                                        ctypes.POINTER(BatchStats)]),
+    "psm_batch_create": (ctypes.c_int, [VP, VP, ctypes.c_uint32, U32P, U64P, U64P, U64P, U32P, U32P, U32P,
+                                        ctypes.POINTER(VP)]),
+    "psm_batch_prepare": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int]),
+    "psm_batch_evaluate": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.c_double, U64P, U64P, U32P,
+                                          ctypes.POINTER(BatchStats)]),
+    "psm_batch_destroy": (ctypes.c_int, [VP]),
     "psm_server_create": (ctypes.c_int, [VP, VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(VP)]),
     "psm_server_submit": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.c_uint32, U32P, U32P, U32P]),

@@ -1,32 +1,32 @@
 """Autotuner of the enumeration launch shape.
 
-The persistent DFS kernel's grid (wavefronts per CU) and the task
-pre-expansion target (tasks per DFS lane before the persistent kernel
-starts, i.e. the frontier chunk the BFS phase hands to the DFS phase) trade
-occupancy against per-lane state traffic and load balance, and the best
-point depends on the data graph and the query mix. The tuner runs the
-batched pipeline over a sample of the workload for each candidate setting
-under a short time limit and keeps the setting with the highest search
-throughput (assignments per millisecond of enumeration, the reference's
-"Call Count" rate), which is comparable across settings whether or not a
-trial runs into the limit. Results never depend on the setting: only the
-launch shape changes.
+Three knobs of the enumeration: the search strategy (one search per lane,
+or one per wavefront with chunked cooperative tests), the persistent DFS
+grid (wavefronts per CU) and the task pre-expansion target (tasks per DFS
+worker before the persistent kernel starts, i.e. the frontier chunk the BFS
+phase hands to the DFS phase). They trade occupancy against memory-system
+contention and load balance, and the best point depends on the data graph
+and the query mix. The tuner prepares (filter, tables, plans) a sample of
+the workload once, runs its enumeration for each candidate setting under a
+short time limit and keeps the setting with the highest search throughput
+(assignments per millisecond of enumeration, the reference's "Call Count"
+rate), which is comparable across settings whether or not a trial runs into
+the limit. Results never depend on the setting: only the launch shape
+changes.
 """
 from __future__ import annotations
 
 import ctypes
-import itertools
 import json
 import os
 
 from . import _lib as L
-from .matcher import batch_match
+from .matcher import Batch
 
-GRID = {
-    "dfs_strategy": ("lane", "wave"),
-    "dfs_waves_per_cu": (4, 8, 16),
-    "expand_tasks_per_lane": (1.0, 4.0),
-}
+# candidate launch shapes: (strategy, wavefronts per CU, tasks per worker)
+GRID = [("lane", w, e) for w in (2, 4, 8) for e in (2.0, 4.0, 8.0)] + \
+       [("wave", w, e) for w in (4, 8) for e in (1.0, 4.0)]
+NAMES = ("dfs_strategy", "dfs_waves_per_cu", "expand_tasks_per_lane")
 KEYS = {
     "dfs_waves_per_cu": L.PSM_TUNE_DFS_WAVES_PER_CU,
     "expand_tasks_per_lane": L.PSM_TUNE_EXPAND_TASKS_PER_LANE,
@@ -57,13 +57,14 @@ def set_config(dev, cfg: dict) -> None:
 
 
 def tune(dg, queries, filter: str = "GQL", order: str = "GQL", cap: int = 0, sample: int = 128,
-         trial_limit_s: float = 0.5, restore_limit_s: float = 60.0, grid: dict | None = None,
+         trial_limit_s: float = 0.8, restore_limit_s: float = 60.0, grid=None,
          cache: str | None = None, log=None):
     """Picks and sets the launch shape for `queries` on `dg`.
 
     sample: queries used per trial (every k-th of the workload);
     trial_limit_s: enumeration time limit of a trial; restore_limit_s: the
-    limit set again afterwards. Returns (best config, list of trials)."""
+    context's limit afterwards; grid: (strategy, waves per CU, tasks per worker)
+    tuples. Returns (best config, list of trials)."""
     grid = grid or GRID
     dev = dg.dev
     step = max(1, len(queries) // max(1, sample))
@@ -79,14 +80,16 @@ def tune(dg, queries, filter: str = "GQL", order: str = "GQL", cap: int = 0, sam
                 return saved["best"], saved["trials"]
         except (ValueError, KeyError):
             pass
-    names = list(grid)
     trials = []
-    L.lib().psm_set_time_limit(dev.h, trial_limit_s)
+    # filter, tables and plans once; each trial times the enumeration alone
+    # under its own limit
+    b = Batch(dg, sub)
     try:
-        for values in itertools.product(*(grid[n] for n in names)):
-            cfg = dict(zip(names, values))
+        b.prepare(filter, order)
+        for values in grid:
+            cfg = dict(zip(NAMES, values))
             set_config(dev, cfg)
-            counts, calls, status, st = batch_match(dg, sub, filter, order, cap)
+            counts, calls, status, st = b.evaluate(cap, trial_limit_s)
             nodes = int(calls.sum())
             rate = nodes / st.enum_ms if st.enum_ms > 0 else 0.0
             trials.append(dict(config=cfg, nodes=nodes, enum_ms=round(st.enum_ms, 3),
@@ -95,6 +98,7 @@ def tune(dg, queries, filter: str = "GQL", order: str = "GQL", cap: int = 0, sam
             if log:
                 log(f"autotune {cfg}: {nodes} nodes in {st.enum_ms:.1f} ms ({rate:.0f}/ms)")
     finally:
+        b.close()
         L.lib().psm_set_time_limit(dev.h, restore_limit_s)
     best = max(trials, key=lambda t: t["nodes_per_ms"])["config"]
     set_config(dev, best)

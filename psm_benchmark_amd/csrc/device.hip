@@ -12,6 +12,8 @@
 //   expand/dfs   enumeration: task pre-expansion and the persistent
 //                lane-per-task DFS fed by a global atomic work queue
 
+#include <algorithm>
+
 #include "internal.h"
 
 namespace psm {
@@ -158,6 +160,23 @@ __global__ __launch_bounds__(kBlock) void k_set_bits(
     }
 }
 
+// member[mem_base + v] |= 1 << u for every candidate v of segment (q, u).
+__global__ __launch_bounds__(kBlock) void k_set_member(
+    const uint32_t* __restrict__ pool, const uint64_t* __restrict__ seg_begin,
+    const uint32_t* __restrict__ seg_len, const uint64_t* __restrict__ seg_mem,
+    const uint32_t* __restrict__ seg_bit, uint32_t n_seg, unsigned long long* __restrict__ member) {
+    const uint32_t s = blockIdx.y;
+    if (s >= n_seg) return;
+    const uint32_t len = seg_len[s];
+    unsigned long long* mw = member + seg_mem[s];
+    const unsigned long long bit = 1ull << seg_bit[s];
+    const uint32_t* src = pool + seg_begin[s];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < len; i += gridDim.x * kBlock) {
+        const uint32_t v = src[i];
+        if (v != kInvalid) atomicOr(mw + v, bit);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_popc_words(const uint32_t* __restrict__ bm,
                                                         uint32_t* __restrict__ out, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
@@ -190,22 +209,24 @@ struct GqlShared {
     uint32_t st_via[kGqlWavesPerBlock][kMaxGqlDegree + 1];
 };
 
-// Bit i set when data vertex w is a live candidate of left vertex i.
-__device__ __forceinline__ uint32_t left_mask(const uint32_t* __restrict__ bitmaps, uint64_t wps,
-                                              const uint32_t* __restrict__ lslots, uint32_t k,
-                                              uint32_t w) {
+// Bit i set when data vertex w is a live candidate of left vertex i (the
+// i-th query neighbour of u, bitmap slot lslots[i] = first slot + its id):
+// one membership word per w instead of one bitmap word per left vertex.
+__device__ __forceinline__ uint32_t left_mask(const unsigned long long* __restrict__ member,
+                                              const GItem& it, const uint32_t* __restrict__ lslots,
+                                              uint32_t k, uint32_t w) {
+    const unsigned long long mw = member[it.mem_base + w];
+    const uint32_t first = it.bm_self - it.u;
     uint32_t m = 0;
-    const uint32_t word = w >> 5, bit = w & 31;
-    for (uint32_t i = 0; i < k; ++i)
-        m |= ((bitmaps[(size_t)lslots[i] * wps + word] >> bit) & 1u) << i;
+    for (uint32_t i = 0; i < k; ++i) m |= (uint32_t)((mw >> (lslots[i] - first)) & 1ull) << i;
     return m;
 }
 
 __global__ __launch_bounds__(kBlock) void k_gql_refine(
     DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_waves,
     const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
-    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ bytes,
-    const uint2* __restrict__ list) {
+    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
+    unsigned long long* __restrict__ bytes, const uint2* __restrict__ list) {
     __shared__ GqlShared sh;
     const uint32_t wl = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
@@ -229,15 +250,20 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
     const uint32_t* lslots = left_slots + it.left_begin;   // uniform: scalar loads
     const uint32_t vb = g.off[v], ve = g.off[v + 1];
     // pass 1: lane i (< k) counts the right neighbours of left vertex i,
-    // saturating at k
+    // saturating at k. Once every left vertex has k right neighbours a
+    // saturating matching exists (each left vertex in turn still finds an
+    // unused one), so the scan stops there.
     uint32_t my_cnt = 0;
+    uint32_t scanned = 0;
     for (uint32_t base = vb; base < ve; base += kWave) {
         const uint32_t p = base + lane;
-        const uint32_t m = p < ve ? left_mask(bitmaps, wps, lslots, k, g.adj[p]) : 0u;
+        const uint32_t m = p < ve ? left_mask(member, it, lslots, k, g.adj[p]) : 0u;
         for (uint32_t i = 0; i < k; ++i) {
             const uint32_t c = __popcll(__ballot((m >> i) & 1u));
             if (lane == i) my_cnt = min(my_cnt + c, k);
         }
+        scanned = min(base + kWave, ve) - vb;
+        if (__ballot(lane < k && my_cnt < k) == 0) break;
     }
     bool ok = __ballot(lane < k && my_cnt == 0) == 0;
     const uint32_t T = (uint32_t)__ballot(lane < k && my_cnt < k);
@@ -247,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
         uint32_t nr = 0;
         for (uint32_t base = vb; base < ve; base += kWave) {
             const uint32_t p = base + lane;
-            const uint32_t m = p < ve ? (left_mask(bitmaps, wps, lslots, k, g.adj[p]) & T) : 0u;
+            const uint32_t m = p < ve ? (left_mask(member, it, lslots, k, g.adj[p]) & T) : 0u;
             const unsigned long long bal = __ballot(m != 0);
             if (m) {
                 const uint32_t r = nr + __popcll(bal & lanemask_lt());
@@ -299,12 +325,14 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
     if (!ok && lane == 0) {
         cand_pool[it.cand_begin + j] = kInvalid;
         atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
+        atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
     }
-    // algorithmic bytes: candidate, offsets, N(v), one bitmap word per
-    // (neighbour, left vertex) probe; pass 2 repeats the scan when it runs
+    // algorithmic bytes: candidate, offsets, the scanned part of N(v), one
+    // bitmap word per (neighbour, left vertex) probe; pass 2 rescans all of
+    // N(v) when it runs
     if (lane == 0) {
-        const unsigned long long scan = 4ull * (ve - vb) * (1 + k);
-        atomicAdd(bytes, 12ull + scan * (pass2 ? 2 : 1));
+        const unsigned long long scan = 4ull * scanned * (1 + k);
+        atomicAdd(bytes, 12ull + scan + (pass2 ? 4ull * (ve - vb) * (1 + k) : 0ull));
     }
 }
 
@@ -316,14 +344,82 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
 // i. By Hall's theorem a matching saturating the left side exists iff every
 // subset S of left vertices has |union of R_i, i in S| >= |S|; the 2^k - 1
 // subsets are split over the group's 8 lanes. Candidates outside the limits
-// are appended to `large` for the wavefront kernel.
+// are deferred by degree class: k <= 8 and deg(v) <= 16 / 32 / 64 to the
+// same test with 16 / 32 / 64 lanes per candidate (k_gql_hall), the rest to
+// the wavefront kernel above. Deferred lists: class c at lists + c * cap.
 constexpr uint32_t kGroup = 8;
+constexpr uint32_t kDeferClasses = 4;   // 16, 32, 64 lanes, wavefront kernel
+
+__device__ __forceinline__ uint32_t defer_class(uint32_t deg, uint32_t k) {
+    if (k > kGroup || deg > 64) return 3;
+    return deg <= 16 ? 0 : deg <= 32 ? 1 : 2;
+}
+
+template <uint32_t G>
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long bal, uint32_t gbase) {
+    if constexpr (G == 64) return bal;
+    else return (bal >> gbase) & ((1ull << G) - 1ull);
+}
+
+// The Hall test of the 8-lane kernel with G lanes per candidate, over the
+// (item, index) pairs the 8-lane kernel deferred (k <= 8, deg(v) <= G).
+template <uint32_t G>
+__global__ __launch_bounds__(kBlock) void k_gql_hall(
+    DevGraph g, const GItem* __restrict__ items, const uint2* __restrict__ list, uint32_t n,
+    const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
+    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
+    unsigned long long* __restrict__ bytes) {
+    const uint32_t lane = lane_id();
+    const uint32_t lig = lane & (G - 1);
+    const uint32_t gbase = lane & ~(G - 1);
+    const uint32_t gid = (uint32_t)(((uint64_t)blockIdx.x * kBlock + threadIdx.x) / G);
+    bool live = gid < n;
+    uint32_t j = 0, v = kInvalid, k = 0, deg = 0, vb = 0;
+    GItem it{};
+    if (live) {
+        const uint2 e = list[gid];
+        it = items[e.x];
+        j = e.y;
+        v = cand_pool[it.cand_begin + j];
+        live = v != kInvalid;
+    }
+    if (live) {
+        vb = g.off[v];
+        deg = min(g.off[v + 1] - vb, G);   // deg <= G by construction of the list
+        k = min(it.k, kGroup);
+    }
+    uint32_t m = 0;
+    if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj[vb + lig]);
+    unsigned long long R[kGroup];
+#pragma unroll
+    for (uint32_t i = 0; i < kGroup; ++i) R[i] = group_bits<G>(__ballot((m >> i) & 1u), gbase);
+    bool fail = false;
+    if (live) {
+        const uint32_t full = (1u << k) - 1u;
+        for (uint32_t sub = lig + 1; sub <= full && !fail; sub += G) {
+            unsigned long long uni = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < kGroup; ++i)
+                if ((sub >> i) & 1u) uni |= R[i];
+            fail = __popcll(uni) < __popc(sub);
+        }
+    }
+    const bool group_fail = group_bits<G>(__ballot(fail), gbase) != 0;
+    if (live && lig == 0) {
+        if (group_fail) {
+            cand_pool[it.cand_begin + j] = kInvalid;
+            atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
+            atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
+        }
+        atomicAdd(bytes, 12ull + 4ull * deg * (1 + k));
+    }
+}
 
 __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
     DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_cands,
     const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
-    uint32_t* __restrict__ bitmaps, uint64_t wps, uint2* __restrict__ large,
-    uint32_t* __restrict__ n_large, unsigned long long* __restrict__ bytes) {
+    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
+    uint2* __restrict__ lists, uint32_t* __restrict__ n_lists, unsigned long long* __restrict__ bytes) {
     const uint32_t lane = lane_id();
     const uint32_t lig = lane & (kGroup - 1);
     const uint32_t gbase = lane & ~(kGroup - 1);
@@ -345,14 +441,24 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
         vb = g.off[v];
         deg = g.off[v + 1] - vb;
         k = it.k;
-        if (deg > kGroup || k > kGroup) {
-            if (lig == 0) large[atomicAdd(n_large, 1u)] = make_uint2(ii, j);
-            live = false;
-        }
     }
+    // defer the candidates outside the limits, one atomic per class per wavefront
+    const bool defer = live && lig == 0 && (deg > kGroup || k > kGroup);
+    const uint32_t dc = defer ? defer_class(deg, k) : kDeferClasses;
+#pragma unroll
+    for (uint32_t c = 0; c < kDeferClasses; ++c) {
+        const unsigned long long bal = __ballot(dc == c);
+        if (!bal) continue;
+        const uint32_t leader = __ffsll((long long)bal) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(n_lists + c, (uint32_t)__popcll(bal));
+        base = __shfl(base, leader);
+        if (dc == c) lists[(uint64_t)c * n_cands + base + __popcll(bal & lanemask_lt())] = make_uint2(ii, j);
+    }
+    if (live && (deg > kGroup || k > kGroup)) live = false;
     uint32_t m = 0;
     if (live && lig < deg)
-        m = left_mask(bitmaps, wps, left_slots + it.left_begin, k, g.adj[vb + lig]);
+        m = left_mask(member, it, left_slots + it.left_begin, k, g.adj[vb + lig]);
     // R[i]: group lanes (right vertices) adjacent to left vertex i
     uint32_t R[kGroup];
 #pragma unroll
@@ -376,6 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
         if (group_fail) {
             cand_pool[it.cand_begin + j] = kInvalid;
             atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
+            atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
         }
         atomicAdd(bytes, 12ull + 4ull * deg * (1 + k));
     }
@@ -558,16 +665,24 @@ __device__ __forceinline__ bool test_element(const DPlan& P, const DOp* __restri
     return true;
 }
 
-// Per-lane state of the persistent DFS in global memory, laid out
-// [wave][level][lane] per field so that lanes at one level touch one line.
+// Per-lane state of the persistent DFS, in LDS: per level the assigned
+// candidate index I and data vertex M, and the level's cursor and end as
+// offsets from the start of its base row (restored by re-reading the row
+// start on backtrack), plus the base operand. Laid out [field][level][lane
+// of the block], so the lanes of a wavefront hit distinct banks whatever
+// level each is at. 17 bytes per level per lane.
+constexpr uint32_t kLaneStateBytes = 17;
+constexpr uint8_t kSelIdentity = 0xFF;
+
 struct LaneState {
     uint32_t* I_;
     uint32_t* M_;
-    uint32_t* sel_;
-    unsigned long long* cur_;
-    unsigned long long* end_;
-    size_t base;   // (wave * max_n) * 64 + lane
-    __device__ __forceinline__ size_t at(uint32_t lvl) const { return base + (size_t)lvl * kWave; }
+    uint32_t* cur_;
+    uint32_t* end_;
+    uint8_t* sel_;
+    uint32_t stride;   // lanes per block
+    uint32_t lane;     // this lane's index in the block
+    __device__ __forceinline__ uint32_t at(uint32_t lvl) const { return lvl * stride + lane; }
     __device__ __forceinline__ uint32_t I(uint32_t e) const { return I_[at(e)]; }
     __device__ __forceinline__ uint32_t M(uint32_t e) const { return M_[at(e)]; }
 };
@@ -659,6 +774,7 @@ constexpr uint32_t kCursors = 4;
 
 struct Level {
     uint64_t cur, end;
+    uint64_t base0;         // start of the base row (0 for C(u) itself)
     uint32_t sel, nb, nx;   // base operand, cached cursors, all other operands
     uint64_t c0, c1, c2, c3, e0, e1, e2, e3;
 };
@@ -687,6 +803,7 @@ __device__ __forceinline__ void enter_level(const DPlan& P, const DOp* __restric
     if (nops == 0) {
         L.sel = kIdentity;
         L.cur = 0;
+        L.base0 = 0;
         L.end = P.cand_len[d];
         return;
     }
@@ -706,6 +823,7 @@ __device__ __forceinline__ void enter_level(const DPlan& P, const DOp* __restric
         const DOp op = ops[P.op_begin[d] + sel];
         const uint64_t r = op.row_base + st.I(op.src_depth);
         L.cur = row_off[r];
+        L.base0 = L.cur;
         L.end = row_off[r + 1];
     }
     for (uint32_t j = 0; j < nops; ++j) {
@@ -827,17 +945,18 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* ready, DQueue* Q,
-    QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t* lane_state, EmitSink sink) {
+    QCounters* qc, uint32_t* pending, uint32_t max_n, EmitSink sink) {
+    extern __shared__ __align__(16) unsigned char lane_smem[];
     const uint32_t lane = lane_id();
-    const size_t wave = ((size_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
-    const size_t per_field = (size_t)gridDim.x * (kBlock / kWave) * max_n * kWave;
+    const uint32_t per_field = blockDim.x * max_n;
     LaneState st;
-    st.I_ = lane_state;
-    st.M_ = lane_state + per_field;
-    st.sel_ = lane_state + 2 * per_field;
-    st.cur_ = reinterpret_cast<unsigned long long*>(lane_state + 3 * per_field);
+    st.I_ = reinterpret_cast<uint32_t*>(lane_smem);
+    st.M_ = st.I_ + per_field;
+    st.cur_ = st.M_ + per_field;
     st.end_ = st.cur_ + per_field;
-    st.base = wave * max_n * kWave + lane;
+    st.sel_ = reinterpret_cast<uint8_t*>(st.end_ + per_field);
+    st.stride = blockDim.x;
+    st.lane = threadIdx.x;
 
     if (threadIdx.x == 0 && Q->limit_ticks)
         atomicCAS(&Q->deadline, 0ull, (unsigned long long)wall_clock64() + Q->limit_ticks);
@@ -850,11 +969,12 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     unsigned long long busy_iters = 0, all_iters = 0;
     uint32_t steps = 0;
     const DPlan* P = plans;
-    const uint32_t hungry_below = gridDim.x * (kBlock / 2);   // queue shorter than half the lanes
+    const uint32_t hungry_below = gridDim.x * (blockDim.x / 2);   // queue shorter than half the lanes
 
     while (true) {
-        // claim queued tasks for idle lanes, one atomic per wavefront
-        const unsigned long long want = __ballot(!active && !waiting);
+        // claim queued tasks for idle lanes, one atomic per wavefront (none
+        // once the time limit has passed)
+        const unsigned long long want = __ballot(!active && !waiting && load_relaxed(&Q->timeout) == 0);
         if (want && backoff == 0) {
             const uint32_t leader = __ffsll((long long)want) - 1;
             uint32_t base = 0, got = 0;
@@ -950,10 +1070,10 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                         }
                     }
                 } else {
-                    const size_t a = st.at(d);
-                    st.cur_[a] = L.cur;
-                    st.end_[a] = L.end;
-                    st.sel_[a] = L.sel;
+                    const uint32_t a = st.at(d);
+                    st.cur_[a] = (uint32_t)(L.cur - L.base0);
+                    st.end_[a] = (uint32_t)(L.end - L.base0);
+                    st.sel_[a] = L.sel == kIdentity ? kSelIdentity : (uint8_t)L.sel;
                     st.I_[a] = x;
                     st.M_[a] = v;
                     ++d;
@@ -965,10 +1085,11 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             active = false;
         } else {
             --d;
-            const size_t a = st.at(d);
-            enter_level(*P, ops, row_off, d, st, st.sel_[a], L, lbytes);
-            L.cur = st.cur_[a];
-            L.end = st.end_[a];
+            const uint32_t a = st.at(d);
+            const uint32_t sel = st.sel_[a] == kSelIdentity ? kIdentity : st.sel_[a];
+            enter_level(*P, ops, row_off, d, st, sel, L, lbytes);
+            L.cur = L.base0 + st.cur_[a];
+            L.end = L.base0 + st.end_[a];
         }
         ++steps;
         if (active && (steps & 255) == 0) {
@@ -992,20 +1113,31 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             if (e <= d && e <= kPrefixMax) {
                 const uint32_t t = load_relaxed(&Q->tail);
                 if (t < Q->cap && atomicCAS(&Q->tail, t, t + 1) == t) {
+                    // shallower levels hold offsets from their base row: locate it
+                    uint64_t base = 0;
+                    uint32_t esel = L.sel;
+                    if (e < d) {
+                        const uint8_t sb = st.sel_[st.at(e)];
+                        esel = sb == kSelIdentity ? kIdentity : sb;
+                        if (esel != kIdentity) {
+                            const DOp op = ops[P->op_begin[e] + esel];
+                            base = row_off[op.row_base + st.I(op.src_depth)];
+                        }
+                    }
                     const uint64_t mid = ce + (ee - ce) / 2;
                     atomicAdd(pending + q, 1u);
                     DTask& nt = queue[t];
                     nt.q = q;
                     nt.depth = e;
-                    nt.sel = e == d ? L.sel : st.sel_[st.at(e)];
+                    nt.sel = esel;
                     nt.pad = 0;
-                    nt.lo = mid;
-                    nt.hi = ee;
+                    nt.lo = base + mid;
+                    nt.hi = base + ee;
                     for (uint32_t k = 0; k < e; ++k) nt.prefix[k] = st.I_[st.at(k)];
                     __threadfence();
                     __hip_atomic_store(ready + t, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                     if (e == d) L.end = mid;
-                    else st.end_[st.at(e)] = mid;
+                    else st.end_[st.at(e)] = (uint32_t)mid;
                     atomicAdd(&Q->donated, 1u);
                 }
             }
@@ -1207,8 +1339,11 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
     unsigned long long bytes = 0, busy_pos = 0, all_pos = 0;
 
     while (true) {
-        // ---- claim one task
-        uint32_t slot = kInvalid;
+        // ---- claim one task (none once the time limit has passed: queued
+        // tasks stay counted in pending[] and their queries report a timeout)
+        uint32_t slot = kInvalid, late = 0;
+        if (lane == 0) late = load_relaxed(&Q->timeout);
+        if (__shfl(late, 0)) break;
         if (lane == 0) {
             const uint32_t h = load_relaxed(&Q->head), tl = load_relaxed(&Q->tail);
             if (h < tl && atomicCAS(&Q->head, h, h + 1) == h) {
@@ -1753,6 +1888,16 @@ hipError_t launch_set_bits(const uint32_t* pool, const uint64_t* seg_begin, cons
     return hipGetLastError();
 }
 
+hipError_t launch_set_member(const uint32_t* pool, const uint64_t* seg_begin, const uint32_t* seg_len,
+                             const uint64_t* seg_mem, const uint32_t* seg_bit, uint32_t n_seg,
+                             uint32_t total_max, unsigned long long* member, hipStream_t s) {
+    if (n_seg == 0 || total_max == 0) return hipSuccess;
+    const uint32_t gx = min((total_max + kBlock - 1) / kBlock, 64u);
+    hipLaunchKernelGGL(k_set_member, dim3(gx, n_seg), dim3(kBlock), 0, s, pool, seg_begin, seg_len, seg_mem,
+                       seg_bit, n_seg, member);
+    return hipGetLastError();
+}
+
 hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_words, hipStream_t s) {
     if (n_words == 0) return hipSuccess;
     uint64_t blocks = (n_words + kBlock - 1) / kBlock;
@@ -1763,24 +1908,45 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
 
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
                              uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
-                             uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* bytes,
-                             const uint2* list, hipStream_t s) {
+                             uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
+                             unsigned long long* bytes, const uint2* list, hipStream_t s) {
     if (n_waves == 0) return hipSuccess;
     const uint32_t blocks = (n_waves + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock;
     hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_waves,
-                       left_slots, cand_pool, bitmaps, words_per_slot, bytes, list);
+                       left_slots, cand_pool, bitmaps, words_per_slot, member, bytes, list);
     return hipGetLastError();
 }
 
 hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32_t n_items,
                                    uint32_t n_cands, const uint32_t* left_slots, uint32_t* cand_pool,
-                                   uint32_t* bitmaps, uint64_t words_per_slot, uint2* large,
-                                   uint32_t* n_large, unsigned long long* bytes, hipStream_t s) {
+                                   uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
+                                   uint2* lists, uint32_t* n_lists, unsigned long long* bytes, hipStream_t s) {
     if (n_cands == 0) return hipSuccess;
     const uint64_t threads = (uint64_t)n_cands * kGroup;
     const uint32_t blocks = (uint32_t)((threads + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_gql_refine_small, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_cands,
-                       left_slots, cand_pool, bitmaps, words_per_slot, large, n_large, bytes);
+                       left_slots, cand_pool, bitmaps, words_per_slot, member, lists, n_lists, bytes);
+    return hipGetLastError();
+}
+
+hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items, const uint2* list,
+                           uint32_t n, const uint32_t* left_slots, uint32_t* cand_pool, uint32_t* bitmaps,
+                           uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
+                           hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t threads = (uint64_t)n * lanes;
+    const uint32_t blocks = (uint32_t)((threads + kBlock - 1) / kBlock);
+    if (lanes == 16)
+        hipLaunchKernelGGL(k_gql_hall<16>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n, left_slots,
+                           cand_pool, bitmaps, words_per_slot, member, bytes);
+    else if (lanes == 32)
+        hipLaunchKernelGGL(k_gql_hall<32>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n, left_slots,
+                           cand_pool, bitmaps, words_per_slot, member, bytes);
+    else if (lanes == 64)
+        hipLaunchKernelGGL(k_gql_hall<64>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n, left_slots,
+                           cand_pool, bitmaps, words_per_slot, member, bytes);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -1843,19 +2009,38 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
     return hipGetLastError();
 }
 
-uint32_t dfs_lane_state_words(uint32_t max_n) {
-    // I, M, sel (1 word each) + cur, end (2 words each) per level per lane
-    return 7u * max_n;
+size_t lane_lds_bytes(uint32_t lanes, uint32_t max_n) {
+    return ((size_t)lanes * max_n * kLaneStateBytes + 15) & ~(size_t)15;
 }
+
+// Lanes per block of the lane DFS: the largest of 256 / 128 / 64 whose LDS
+// state fits the device's per-block limit (gfx950: 160 KB).
+uint32_t dfs_lanes_per_block(uint32_t max_n) {
+    int dev = 0, lim = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0)
+        lim = 64 * 1024;
+    uint32_t lanes = kBlock;
+    while (lanes > kWave && lane_lds_bytes(lanes, max_n) > (size_t)lim) lanes /= 2;
+    return lanes;
+}
+
+size_t dfs_lane_lds_bytes(uint32_t max_n) { return lane_lds_bytes(dfs_lanes_per_block(max_n), max_n); }
 
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
                       const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
-                      uint32_t* lane_state, EmitSink sink, hipStream_t s) {
+                      EmitSink sink, hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
-    const uint32_t blocks = grid_waves / (kBlock / kWave);
-    hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(kBlock), 0, s, plans, ops, row_off, idx, cand_pool,
-                       queue, ready, dq, qc, pending, max_n, lane_state, sink);
+    if (max_n == 0 || max_n > kMaxQ) return hipErrorInvalidValue;
+    const uint32_t lanes = dfs_lanes_per_block(max_n);
+    const size_t lds = lane_lds_bytes(lanes, max_n);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dfs),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const uint32_t blocks = std::max<uint32_t>(1, grid_waves / (lanes / kWave));
+    hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(lanes), lds, s, plans, ops, row_off, idx, cand_pool, queue,
+                       ready, dq, qc, pending, max_n, sink);
     return hipGetLastError();
 }
 

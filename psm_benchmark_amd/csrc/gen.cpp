@@ -169,20 +169,23 @@ int psm_gen_labels(uint32_t V, uint32_t L, double zipf_s, uint64_t seed, uint32_
 // dense = 0: random walk, edges = first-visit (tree) edges.
 // dense = 1: greedy growth that adds the frontier vertex with most edges into
 //            the current set (random tie-break), edges = induced subgraph.
+// dense = 2: random walk, edges = induced subgraph of the walked vertices
+//            (the walk tree plus every data edge among them: cycles).
 // Output: vertex ids (data ids, query vertex i = out_vertices[i]) and the
 // query's undirected edges as pairs of query-local ids. Returns the number of
 // edges, or a negative PSM status.
 int64_t psm_gen_query(uint32_t V, const uint32_t* offsets, const uint32_t* nbrs,
                       uint32_t n, int dense, uint64_t seed,
                       uint32_t* out_vertices, uint32_t* out_edges, uint64_t max_edges) {
-    if (n == 0 || n > 64 || V == 0) return -PSM_EINVAL;
+    if (n == 0 || n > 64 || V == 0 || dense < 0 || dense > 2) return -PSM_EINVAL;
+    const bool walk = dense != 1, induced = dense != 0;
     Rng r(seed);
     for (int attempt = 0; attempt < 1000; ++attempt) {
         uint32_t start = (uint32_t)r.below(V);
         if (offsets[start + 1] == offsets[start]) continue;
         std::vector<uint32_t> verts{start};
         std::vector<std::pair<uint32_t, uint32_t>> tree;
-        if (!dense) {
+        if (walk) {
             uint32_t cur = 0;  // index into verts
             uint64_t steps = 0;
             while (verts.size() < n && steps++ < 100000) {
@@ -231,7 +234,7 @@ int64_t psm_gen_query(uint32_t V, const uint32_t* offsets, const uint32_t* nbrs,
         }
         if (verts.size() < n) continue;
         std::vector<std::pair<uint32_t, uint32_t>> edges;
-        if (!dense) {
+        if (!induced) {
             edges = tree;
         } else {
             for (uint32_t i = 0; i < n; ++i) {

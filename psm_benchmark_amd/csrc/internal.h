@@ -58,6 +58,8 @@ struct CItem {
 };
 
 // GQL refinement work item: the candidates of one query vertex u.
+// The candidate sets of one query are also held as membership words:
+// member[mem_base + w] has bit u set when data vertex w is in C(u).
 struct GItem {
     uint64_t cand_begin;
     uint32_t len;
@@ -65,7 +67,8 @@ struct GItem {
     uint32_t bm_self;      // bitmap slot of (query, u)
     uint32_t k;            // query degree of u
     uint32_t left_begin;   // into the left-slot array: bitmap slots of N_q(u)
-    uint32_t pad;
+    uint32_t u;            // the query vertex (bm_self - u = the query's first slot)
+    uint64_t mem_base;     // the query's membership words
 };
 
 // BuildTable work item: ordered query edge (a, b) of one query.
@@ -179,15 +182,24 @@ hipError_t launch_compact_scatter(const CItem* items, uint32_t n_items, uint32_t
 hipError_t launch_set_bits(const uint32_t* pool, const uint64_t* seg_begin, const uint32_t* seg_len,
                            const uint32_t* seg_slot, uint32_t n_seg, uint32_t total_max,
                            uint32_t* bitmaps, uint64_t words_per_slot, hipStream_t s);
+hipError_t launch_set_member(const uint32_t* pool, const uint64_t* seg_begin, const uint32_t* seg_len,
+                             const uint64_t* seg_mem, const uint32_t* seg_bit, uint32_t n_seg,
+                             uint32_t total_max, unsigned long long* member, hipStream_t s);
 hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_words, hipStream_t s);
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
                              uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
-                             uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* bytes,
-                             const uint2* list, hipStream_t s);
+                             uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
+                             unsigned long long* bytes, const uint2* list, hipStream_t s);
+// deferred candidates: class c (16, 32, 64 lanes, wavefront kernel) at
+// lists + c * n_cands, counts in n_lists[0..3]
 hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32_t n_items,
                                    uint32_t n_cands, const uint32_t* left_slots, uint32_t* cand_pool,
-                                   uint32_t* bitmaps, uint64_t words_per_slot, uint2* large,
-                                   uint32_t* n_large, unsigned long long* bytes, hipStream_t s);
+                                   uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
+                                   uint2* lists, uint32_t* n_lists, unsigned long long* bytes, hipStream_t s);
+hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items, const uint2* list,
+                           uint32_t n, const uint32_t* left_slots, uint32_t* cand_pool, uint32_t* bitmaps,
+                           uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
+                           hipStream_t s);
 hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,
                               const uint64_t* item_row_start, uint64_t n_rows,
                               const uint32_t* cand_pool, const uint32_t* bitmaps,
@@ -212,7 +224,7 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
                       const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
-                      uint32_t* lane_state, EmitSink sink, hipStream_t s);
+                      EmitSink sink, hipStream_t s);
 hipError_t launch_join_build(const uint32_t* rows, uint64_t n_rows, uint32_t width, JoinKeys keys,
                              unsigned long long* fp, uint32_t* head, uint64_t mask, uint32_t* next,
                              hipStream_t s);
@@ -222,7 +234,7 @@ hipError_t launch_join_probe(const uint32_t* build_rows, uint32_t build_width, J
                              uint64_t mask, const uint32_t* next, unsigned long long* count,
                              unsigned long long* bytes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
-uint32_t dfs_lane_state_words(uint32_t max_n);
+size_t dfs_lane_lds_bytes(uint32_t max_n);
 size_t dfs_wave_lds_bytes(uint32_t max_n);
 hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
                            const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,

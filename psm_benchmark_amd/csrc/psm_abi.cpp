@@ -17,6 +17,8 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <atomic>
+#include <functional>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -55,15 +57,28 @@ struct psm_ctx {
 
 namespace {
 
+// Stage helpers may run on several host threads of one context (the CFL
+// filter of a batch): the error text is written under a lock.
+std::mutex& err_mu() {
+    static std::mutex m;
+    return m;
+}
+
 int set_err(psm_ctx* ctx, int code, const std::string& msg) {
-    if (ctx) ctx->err = msg;
+    if (ctx) {
+        std::lock_guard<std::mutex> lk(err_mu());
+        ctx->err = msg;
+    }
     return code;
 }
 
 int hip_fail(psm_ctx* ctx, hipError_t e, const char* what, int line) {
     char buf[512];
     snprintf(buf, sizeof(buf), "%s failed (%s) at psm_abi.cpp:%d", what, hipGetErrorString(e), line);
-    if (ctx) ctx->err = buf;
+    if (ctx) {
+        std::lock_guard<std::mutex> lk(err_mu());
+        ctx->err = buf;
+    }
     return e == hipErrorOutOfMemory ? PSM_ENOMEM : -(int)e;
 }
 
@@ -371,6 +386,9 @@ struct Batch {
     double t_filter = 0, t_table = 0, t_plan = 0, t_enum = 0, t_expand = 0;
     uint64_t tasks_run = 0;
     uint64_t lane_steps = 0, lane_iters = 0;   // DFS lane utilisation
+    double t_begin = -1;   // start of filtering: the time limit counts from here
+    double enum_limit_s = -1;   // >= 0: an enumeration time limit counted from t_eval instead
+    double t_eval = 0;
 
     hipStream_t s() const { return ctx->stream; }
 
@@ -379,16 +397,20 @@ struct Batch {
     int ensure_bitmaps(bool from_candidates);
     int compact_pool();
     int gql_refine(std::vector<QueryHost*>& active);
+    int for_each_query(size_t count, const std::function<int(size_t, uint32_t, hipStream_t)>& fn,
+                       uint32_t& workers);
     int cfl_filter(std::vector<QueryHost*>& active);
+    int cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch,
+                         std::vector<std::vector<uint32_t>>& sets);
     int build_tables();
     int make_plans(int order_method);
     int run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSink sink,
                   std::vector<QCounters>& qc, bool& timed_out);
-    int evaluate(uint64_t cap);
+    int evaluate(uint64_t cap, double enum_limit = -1);
     int join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr);
     int host_table(const QueryHost& q, uint32_t a, uint32_t b, std::vector<uint32_t>& off,
-                   std::vector<uint32_t>& idxv);
-    int nlf_count(const QGraph& q, uint32_t u, uint32_t label_override, uint32_t& out);
+                   std::vector<uint32_t>& idxv, hipStream_t st);
+    int nlf_count(const QGraph& q, uint32_t u, uint32_t label_override, uint32_t& out, hipStream_t st);
 };
 
 // ---- candidate filters ------------------------------------------------------
@@ -589,20 +611,54 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
     PSM_TRY(ensure_bitmaps(true));
     uint32_t max_n = 0;
     for (QueryHost* q : active) max_n = std::max(max_n, q->qg.n);
+    // membership words: member[a * V + w] bit u <=> w in C(u) of active query a
+    DBuf member;
+    {
+        const size_t words = active.size() * (size_t)g->V;
+        HIP_TRY(ctx, member.alloc(words * 8));
+        HIP_TRY(ctx, hipMemsetAsync(member.p, 0, words * 8, s()));
+        std::vector<uint64_t> sb, sm;
+        std::vector<uint32_t> sl, sbit;
+        uint32_t mx = 0;
+        for (size_t a = 0; a < active.size(); ++a) {
+            QueryHost* q = active[a];
+            for (uint32_t u = 0; u < q->qg.n; ++u) {
+                if (!q->cand_count[u]) continue;
+                sb.push_back(q->cand_begin[u]);
+                sl.push_back(q->cand_count[u]);
+                sm.push_back(a * (uint64_t)g->V);
+                sbit.push_back(u);
+                mx = std::max(mx, q->cand_count[u]);
+            }
+        }
+        DBuf d_sb, d_sl, d_sm, d_sbit;
+        HIP_TRY(ctx, upload(d_sb, sb, s()));
+        HIP_TRY(ctx, upload(d_sl, sl, s()));
+        HIP_TRY(ctx, upload(d_sm, sm, s()));
+        HIP_TRY(ctx, upload(d_sbit, sbit, s()));
+        TIMED(ctx, kKBitmap,
+              launch_set_member(cand_pool.as<uint32_t>(), d_sb.as<uint64_t>(), d_sl.as<uint32_t>(),
+                                d_sm.as<uint64_t>(), d_sbit.as<uint32_t>(), (uint32_t)sb.size(), mx,
+                                member.as<unsigned long long>(), s()));
+        HIP_TRY(ctx, hipStreamSynchronize(s()));
+    }
     DBuf d_items, d_left, d_large, d_nlarge;
-    HIP_TRY(ctx, d_nlarge.alloc(4));
+    HIP_TRY(ctx, d_nlarge.alloc(16));
     for (int round = 0; round < 2; ++round) {
         for (uint32_t p = 0; p < max_n; ++p) {
             std::vector<GItem> items;
             std::vector<uint32_t> left;
             uint32_t waves = 0;
-            for (QueryHost* q : active) {
+            for (size_t a = 0; a < active.size(); ++a) {
+                QueryHost* q = active[a];
                 if (p >= q->qg.n || q->cand_count[p] == 0) continue;
                 GItem it{};
                 it.cand_begin = q->cand_begin[p];
                 it.len = q->cand_count[p];
                 it.wave_start = waves;
                 it.bm_self = q->slot_base + p;
+                it.u = p;
+                it.mem_base = a * (uint64_t)g->V;
                 it.k = q->qg.degree(p);
                 it.left_begin = (uint32_t)left.size();
                 for (uint32_t i = q->qg.off[p]; i < q->qg.off[p + 1]; ++i)
@@ -613,25 +669,36 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
             if (items.empty()) continue;
             HIP_TRY(ctx, upload(d_items, items, s()));
             HIP_TRY(ctx, upload(d_left, left, s()));
-            if (d_large.bytes < (size_t)waves * sizeof(uint2)) HIP_TRY(ctx, d_large.alloc((size_t)waves * sizeof(uint2)));
-            HIP_TRY(ctx, hipMemsetAsync(d_nlarge.p, 0, 4, s()));
-            // small candidates: 8 lanes each; the rest are deferred to the
-            // wavefront-per-candidate kernel
+            const size_t list_bytes = 4ull * waves * sizeof(uint2);
+            if (d_large.bytes < list_bytes) HIP_TRY(ctx, d_large.alloc(list_bytes));
+            HIP_TRY(ctx, hipMemsetAsync(d_nlarge.p, 0, 16, s()));
+            // small candidates: 8 lanes each; the rest are deferred by degree
+            // class to 16 / 32 / 64-lane groups or the wavefront kernel
             TIMED(ctx, kKGqlRefine,
                   launch_gql_refine_small(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), waves,
                                           d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
-                                          bitmaps.as<uint32_t>(), wps, d_large.as<uint2>(),
-                                          d_nlarge.as<uint32_t>(), ctx->d_kbytes + kKGqlRefine, s()));
-            uint32_t n_large = 0;
-            HIP_TRY(ctx, hipMemcpyAsync(&n_large, d_nlarge.p, 4, hipMemcpyDeviceToHost, s()));
+                                          bitmaps.as<uint32_t>(), wps, member.as<unsigned long long>(),
+                                          d_large.as<uint2>(), d_nlarge.as<uint32_t>(),
+                                          ctx->d_kbytes + kKGqlRefine, s()));
+            uint32_t n_def[4] = {0, 0, 0, 0};
+            HIP_TRY(ctx, hipMemcpyAsync(n_def, d_nlarge.p, 16, hipMemcpyDeviceToHost, s()));
             HIP_TRY(ctx, hipStreamSynchronize(s()));
-            verbose("gql round %d u %u: %u candidates, %u to the wavefront kernel", round, p, waves, n_large);
-            if (n_large)
+            verbose("gql round %d u %u: %u candidates, deferred %u / %u / %u / %u", round, p, waves, n_def[0],
+                    n_def[1], n_def[2], n_def[3]);
+            const uint32_t lanes[3] = {16, 32, 64};
+            for (int c = 0; c < 3; ++c)
+                if (n_def[c])
+                    TIMED(ctx, kKGqlRefine,
+                          launch_gql_hall(lanes[c], g->dg, d_items.as<GItem>(), d_large.as<uint2>() + (size_t)c * waves,
+                                          n_def[c], d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
+                                          bitmaps.as<uint32_t>(), wps, member.as<unsigned long long>(),
+                                          ctx->d_kbytes + kKGqlRefine, s()));
+            if (n_def[3])
                 TIMED(ctx, kKGqlRefine,
-                      launch_gql_refine(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), n_large,
+                      launch_gql_refine(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), n_def[3],
                                         d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
-                                        bitmaps.as<uint32_t>(), wps, ctx->d_kbytes + kKGqlRefine,
-                                        d_large.as<uint2>(), s()));
+                                        bitmaps.as<uint32_t>(), wps, member.as<unsigned long long>(),
+                                        ctx->d_kbytes + kKGqlRefine, d_large.as<uint2>() + 3ull * waves, s()));
         }
     }
     HIP_TRY(ctx, timers_flush(ctx));
@@ -646,6 +713,10 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
 int Batch::filter(int method) {
     tls_stream = s();
     const double t0 = now_ms();
+    // the reference sets its deadline before FilterVertices
+    // (StudyPerformance.cpp:1134-1148): filter, tables, plan and enumeration
+    // share one time limit
+    t_begin = t0;
     std::vector<QueryHost*> active;
     for (QueryHost* q : qs) {
         if (q->qg.n == 0 || q->qg.n > kMaxQ)
@@ -743,22 +814,22 @@ int Batch::build_tables() {
 
 // Host copy of one table (CFL path-estimate input; psm_query_table).
 int Batch::host_table(const QueryHost& q, uint32_t a, uint32_t b, std::vector<uint32_t>& off,
-                      std::vector<uint32_t>& idxv) {
+                      std::vector<uint32_t>& idxv, hipStream_t st) {
     const uint32_t n = q.qg.n;
     const uint64_t r0 = q.tab_row[(size_t)a * n + b];
     if (r0 == kNoTable) return PSM_EINVAL;
     const uint32_t rows = q.cand_count[a];
     std::vector<uint64_t> ro;
-    HIP_TRY(ctx, download(ro, row_off.as<uint64_t>() + r0, rows + 1, s()));
+    HIP_TRY(ctx, download(ro, row_off.as<uint64_t>() + r0, rows + 1, st));
     off.resize(rows + 1);
     for (uint32_t i = 0; i <= rows; ++i) off[i] = (uint32_t)(ro[i] - ro[0]);
-    HIP_TRY(ctx, download(idxv, idx.as<uint32_t>() + ro[0], ro[rows] - ro[0], s()));
+    HIP_TRY(ctx, download(idxv, idx.as<uint32_t>() + ro[0], ro[rows] - ro[0], st));
     return PSM_OK;
 }
 
 // |C(u)| under the NLF filter for one query vertex whose label is replaced by
 // `label_override` (the CFL unit start-vertex rule runs on relabelled units).
-int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out) {
+int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out, hipStream_t st) {
     std::vector<QVert> qv(1);
     std::vector<uint2> qn;
     qv[0].label = label;
@@ -776,17 +847,17 @@ int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out)
     const uint32_t blocks = (items[0].len + 255) / 256;
     if (!blocks) return PSM_OK;
     DBuf d_items, d_qv, d_qn, d_masks, d_bc;
-    HIP_TRY(ctx, upload(d_items, items, s()));
-    HIP_TRY(ctx, upload(d_qv, qv, s()));
+    HIP_TRY(ctx, upload(d_items, items, st));
+    HIP_TRY(ctx, upload(d_qv, qv, st));
     if (qn.empty()) qn.push_back(make_uint2(0, 0));
-    HIP_TRY(ctx, upload(d_qn, qn, s()));
+    HIP_TRY(ctx, upload(d_qn, qn, st));
     HIP_TRY(ctx, d_masks.alloc((size_t)blocks * 32));
     HIP_TRY(ctx, d_bc.alloc((size_t)blocks * 4));
     HIP_TRY(ctx, launch_compact_flag(g->dg, d_items.as<CItem>(), 1, blocks, d_qv.as<QVert>(),
                                      d_qn.as<uint2>(), nullptr, nullptr,
-                                     d_masks.as<unsigned long long>(), d_bc.as<uint32_t>(), s()));
+                                     d_masks.as<unsigned long long>(), d_bc.as<uint32_t>(), st));
     std::vector<uint32_t> bc;
-    HIP_TRY(ctx, download(bc, d_bc.p, blocks, s()));
+    HIP_TRY(ctx, download(bc, d_bc.p, blocks, st));
     for (uint32_t c : bc) out += c;
     return PSM_OK;
 }
@@ -794,124 +865,182 @@ int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out)
 // ---- CFL filter (FilterVertices.cpp:188-250) ------------------------------
 // Support counters use an epoch per pass so they never need clearing: a
 // pass with P pivots moves cnt[w] from base to base+P, base = epoch*64.
-int Batch::cfl_filter(std::vector<QueryHost*>& active) {
-    DBuf cnt;
-    HIP_TRY(ctx, cnt.alloc((size_t)g->V * 4));
-    HIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, (size_t)g->V * 4, s()));
-    uint32_t epoch = 0;
-    // The plain CFL filter runs query by query: its steps depend on the
-    // query's own BFS levels.
-    std::vector<std::vector<uint32_t>> sets;
-    std::vector<uint32_t> all;   // concatenated final candidates
-    for (QueryHost* q : active) {
-        const QGraph& qg = q->qg;
-        const uint32_t n = qg.n;
-        std::vector<uint32_t> nlf_counts(n, kInvalid);
-        auto nlf_of = [&](uint32_t u, uint32_t& c) -> int {
-            if (nlf_counts[u] == kInvalid) PSM_TRY(nlf_count(qg, u, qg.lab[u], nlf_counts[u]));
-            c = nlf_counts[u];
-            return PSM_OK;
-        };
-        CflTree t;
-        PSM_TRY(cfl_filter_plan(qg, g->L, g->h_ri_off, g->max_freq,
-                                [&](uint32_t u, uint32_t& c) { return nlf_of(u, c); }, t));
-        // per-vertex candidate lists live in their own device buffers while
-        // the passes run
-        std::vector<std::unique_ptr<DBuf>> cb(n);
-        std::vector<uint32_t> len(n, 0);
-        auto run_compact = [&](uint32_t u, uint32_t mode, uint32_t target) -> int {
-            std::vector<QVert> qv(1);
-            std::vector<uint2> qn;
-            qv[0].label = qg.lab[u];
-            qv[0].degree = qg.degree(u);
-            qv[0].nlf_begin = 0;
-            qv[0].nlf_count = (uint32_t)qg.nlf[u].size();
-            for (auto& r : qg.nlf[u]) qn.push_back(make_uint2(r.first, r.second));
-            if (qn.empty()) qn.push_back(make_uint2(0, 0));
-            std::vector<CItem> items(1);
-            items[0] = CItem{};
-            const uint32_t l = qg.lab[u];
-            items[0].src_begin = l < g->L ? g->h_ri_off[l] : 0;
-            items[0].len = l < g->L ? g->h_ri_off[l + 1] - g->h_ri_off[l] : 0;
-            items[0].mode = mode;
-            items[0].target = target;
-            const uint32_t blocks = (items[0].len + 255) / 256;
-            cb[u].reset(new DBuf());
-            len[u] = 0;
-            if (!blocks) { HIP_TRY(ctx, cb[u]->alloc(4)); return PSM_OK; }
-            DBuf d_items, d_qv, d_qn, d_masks, d_bc, d_bp, d_tmp;
-            HIP_TRY(ctx, upload(d_items, items, s()));
-            HIP_TRY(ctx, upload(d_qv, qv, s()));
-            HIP_TRY(ctx, upload(d_qn, qn, s()));
-            HIP_TRY(ctx, d_masks.alloc((size_t)blocks * 32));
-            HIP_TRY(ctx, d_bc.alloc((size_t)blocks * 4));
-            HIP_TRY(ctx, d_bp.alloc((size_t)(blocks + 1) * 8));
-            HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(blocks) * 8));
-            HIP_TRY(ctx, launch_compact_flag(g->dg, d_items.as<CItem>(), 1, blocks, d_qv.as<QVert>(),
-                                             d_qn.as<uint2>(), nullptr, cnt.as<uint32_t>(),
-                                             d_masks.as<unsigned long long>(), d_bc.as<uint32_t>(), s()));
-            HIP_TRY(ctx, launch_scan_u32(d_bc.as<uint32_t>(), blocks, d_bp.as<uint64_t>(),
-                                         d_tmp.as<unsigned long long>(), s()));
-            uint64_t total = 0;
-            HIP_TRY(ctx, hipMemcpyAsync(&total, d_bp.as<uint64_t>() + blocks, 8, hipMemcpyDeviceToHost, s()));
-            HIP_TRY(ctx, hipStreamSynchronize(s()));
-            HIP_TRY(ctx, cb[u]->alloc(total * 4));
-            HIP_TRY(ctx, launch_compact_scatter(d_items.as<CItem>(), 1, blocks, g->ri.as<uint32_t>(), nullptr,
-                                                d_masks.as<unsigned long long>(), d_bp.as<uint64_t>(),
-                                                cb[u]->as<uint32_t>(), s()));
-            len[u] = (uint32_t)total;
-            return PSM_OK;
-        };
-        auto support = [&](uint32_t u, const std::vector<uint32_t>& pivots, uint32_t& target) -> int {
-            ++epoch;
-            const uint32_t base = epoch * 64;
-            for (uint32_t i = 0; i < pivots.size(); ++i) {
-                const uint32_t p = pivots[i];
-                HIP_TRY(ctx, launch_cfl_support(g->dg, cb[p]->as<uint32_t>(), 0, len[p], qg.lab[u],
-                                                qg.degree(u), cnt.as<uint32_t>(),
-                                                i == 0 ? 0u : base + i, base + i + 1, s()));
-            }
-            target = base + (uint32_t)pivots.size();
-            return PSM_OK;
-        };
-        const uint32_t root = t.order[0];
-        PSM_TRY(run_compact(root, kModeNLF, 0));
-        for (uint32_t i = 0; i < n; ++i)
-            if (!cb[i]) { cb[i].reset(new DBuf()); HIP_TRY(ctx, cb[i]->alloc(4)); }
-        const uint32_t nlev = (uint32_t)t.level_off.size() - 1;
-        auto prune = [&](uint32_t u, const std::vector<uint32_t>& piv) -> int {
-            uint32_t target;
-            PSM_TRY(support(u, piv, target));
-            HIP_TRY(ctx, launch_cfl_prune(cnt.as<uint32_t>(), cb[u]->as<uint32_t>(), 0, len[u], target, s()));
-            return PSM_OK;
-        };
-        for (uint32_t lv = 1; lv < nlev; ++lv) {
-            for (uint32_t j = t.level_off[lv]; j < t.level_off[lv + 1]; ++j) {
-                const uint32_t u = t.order[j];
-                uint32_t target;
-                PSM_TRY(support(u, t.bn[u], target));
-                PSM_TRY(run_compact(u, kModeCflGen, target));
-            }
-            for (int j = (int)t.level_off[lv + 1] - 1; j >= (int)t.level_off[lv]; --j) {
-                const uint32_t u = t.order[j];
-                if (!t.fn[u].empty()) PSM_TRY(prune(u, t.fn[u]));
-            }
+// The CFL filter of one query on stream `st` with its own support counters;
+// its candidate sets (one per query vertex) go to `sets`. Its steps depend on
+// the query's own BFS levels.
+int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch,
+                            std::vector<std::vector<uint32_t>>& sets) {
+    tls_stream = st;
+    const QGraph& qg = q->qg;
+    const uint32_t n = qg.n;
+    std::vector<uint32_t> nlf_counts(n, kInvalid);
+    auto nlf_of = [&](uint32_t u, uint32_t& c) -> int {
+        if (nlf_counts[u] == kInvalid) PSM_TRY(nlf_count(qg, u, qg.lab[u], nlf_counts[u], st));
+        c = nlf_counts[u];
+        return PSM_OK;
+    };
+    CflTree t;
+    PSM_TRY(cfl_filter_plan(qg, g->L, g->h_ri_off, g->max_freq,
+                            [&](uint32_t u, uint32_t& c) { return nlf_of(u, c); }, t));
+    // per-vertex candidate lists live in their own device buffers while
+    // the passes run
+    std::vector<std::unique_ptr<DBuf>> cb(n);
+    std::vector<uint32_t> len(n, 0);
+    auto run_compact = [&](uint32_t u, uint32_t mode, uint32_t target) -> int {
+        std::vector<QVert> qv(1);
+        std::vector<uint2> qn;
+        qv[0].label = qg.lab[u];
+        qv[0].degree = qg.degree(u);
+        qv[0].nlf_begin = 0;
+        qv[0].nlf_count = (uint32_t)qg.nlf[u].size();
+        for (auto& r : qg.nlf[u]) qn.push_back(make_uint2(r.first, r.second));
+        if (qn.empty()) qn.push_back(make_uint2(0, 0));
+        std::vector<CItem> items(1);
+        items[0] = CItem{};
+        const uint32_t l = qg.lab[u];
+        items[0].src_begin = l < g->L ? g->h_ri_off[l] : 0;
+        items[0].len = l < g->L ? g->h_ri_off[l + 1] - g->h_ri_off[l] : 0;
+        items[0].mode = mode;
+        items[0].target = target;
+        const uint32_t blocks = (items[0].len + 255) / 256;
+        cb[u].reset(new DBuf());
+        len[u] = 0;
+        if (!blocks) { HIP_TRY(ctx, cb[u]->alloc(4)); return PSM_OK; }
+        DBuf d_items, d_qv, d_qn, d_masks, d_bc, d_bp, d_tmp;
+        HIP_TRY(ctx, upload(d_items, items, st));
+        HIP_TRY(ctx, upload(d_qv, qv, st));
+        HIP_TRY(ctx, upload(d_qn, qn, st));
+        HIP_TRY(ctx, d_masks.alloc((size_t)blocks * 32));
+        HIP_TRY(ctx, d_bc.alloc((size_t)blocks * 4));
+        HIP_TRY(ctx, d_bp.alloc((size_t)(blocks + 1) * 8));
+        HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(blocks) * 8));
+        HIP_TRY(ctx, launch_compact_flag(g->dg, d_items.as<CItem>(), 1, blocks, d_qv.as<QVert>(),
+                                         d_qn.as<uint2>(), nullptr, cnt.as<uint32_t>(),
+                                         d_masks.as<unsigned long long>(), d_bc.as<uint32_t>(), st));
+        HIP_TRY(ctx, launch_scan_u32(d_bc.as<uint32_t>(), blocks, d_bp.as<uint64_t>(),
+                                     d_tmp.as<unsigned long long>(), st));
+        uint64_t total = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(&total, d_bp.as<uint64_t>() + blocks, 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+        HIP_TRY(ctx, cb[u]->alloc(total * 4));
+        HIP_TRY(ctx, launch_compact_scatter(d_items.as<CItem>(), 1, blocks, g->ri.as<uint32_t>(), nullptr,
+                                            d_masks.as<unsigned long long>(), d_bp.as<uint64_t>(),
+                                            cb[u]->as<uint32_t>(), st));
+        len[u] = (uint32_t)total;
+        return PSM_OK;
+    };
+    auto support = [&](uint32_t u, const std::vector<uint32_t>& pivots, uint32_t& target) -> int {
+        ++epoch;
+        const uint32_t base = epoch * 64;
+        for (uint32_t i = 0; i < pivots.size(); ++i) {
+            const uint32_t p = pivots[i];
+            HIP_TRY(ctx, launch_cfl_support(g->dg, cb[p]->as<uint32_t>(), 0, len[p], qg.lab[u],
+                                            qg.degree(u), cnt.as<uint32_t>(),
+                                            i == 0 ? 0u : base + i, base + i + 1, st));
         }
-        for (int lv = (int)nlev - 2; lv >= 0; --lv)
-            for (uint32_t j = t.level_off[lv]; j < t.level_off[lv + 1]; ++j) {
-                const uint32_t u = t.order[j];
-                if (!t.under[u].empty()) PSM_TRY(prune(u, t.under[u]));
-            }
-        // gather into the shared pool (compaction below drops pruned entries)
-        HIP_TRY(ctx, hipStreamSynchronize(s()));
+        target = base + (uint32_t)pivots.size();
+        return PSM_OK;
+    };
+    const uint32_t root = t.order[0];
+    PSM_TRY(run_compact(root, kModeNLF, 0));
+    for (uint32_t i = 0; i < n; ++i)
+        if (!cb[i]) { cb[i].reset(new DBuf()); HIP_TRY(ctx, cb[i]->alloc(4)); }
+    const uint32_t nlev = (uint32_t)t.level_off.size() - 1;
+    auto prune = [&](uint32_t u, const std::vector<uint32_t>& piv) -> int {
+        uint32_t target;
+        PSM_TRY(support(u, piv, target));
+        HIP_TRY(ctx, launch_cfl_prune(cnt.as<uint32_t>(), cb[u]->as<uint32_t>(), 0, len[u], target, st));
+        return PSM_OK;
+    };
+    for (uint32_t lv = 1; lv < nlev; ++lv) {
+        for (uint32_t j = t.level_off[lv]; j < t.level_off[lv + 1]; ++j) {
+            const uint32_t u = t.order[j];
+            uint32_t target;
+            PSM_TRY(support(u, t.bn[u], target));
+            PSM_TRY(run_compact(u, kModeCflGen, target));
+        }
+        for (int j = (int)t.level_off[lv + 1] - 1; j >= (int)t.level_off[lv]; --j) {
+            const uint32_t u = t.order[j];
+            if (!t.fn[u].empty()) PSM_TRY(prune(u, t.fn[u]));
+        }
+    }
+    for (int lv = (int)nlev - 2; lv >= 0; --lv)
+        for (uint32_t j = t.level_off[lv]; j < t.level_off[lv + 1]; ++j) {
+            const uint32_t u = t.order[j];
+            if (!t.under[u].empty()) PSM_TRY(prune(u, t.under[u]));
+        }
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    sets.assign(n, {});
+    for (uint32_t u = 0; u < n; ++u) HIP_TRY(ctx, download(sets[u], cb[u]->p, len[u], st));
+    return PSM_OK;
+}
+
+// Runs fn(i, worker, stream) for i in [0, count) on up to 8 host threads of
+// this context, each with its own HIP stream (queries whose steps depend on
+// their own structure run concurrently this way). Returns the first error.
+int Batch::for_each_query(size_t count, const std::function<int(size_t, uint32_t, hipStream_t)>& fn,
+                          uint32_t& workers) {
+    const uint32_t nw = (uint32_t)std::min<size_t>(count, 8);
+    workers = nw;
+    std::vector<int> rc(nw, PSM_OK);
+    std::atomic<size_t> next{0};
+    auto worker = [&](uint32_t w) {
+        (void)hipSetDevice(ctx->device);
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            rc[w] = set_err(ctx, PSM_ENOMEM, "hipStreamCreate failed");
+            return;
+        }
+        tls_stream = st;
+        while (rc[w] == PSM_OK) {
+            const size_t i = next.fetch_add(1);
+            if (i >= count) break;
+            rc[w] = fn(i, w, st);
+        }
+        (void)hipStreamSynchronize(st);
+        tls_stream = nullptr;
+        (void)hipStreamDestroy(st);
+    };
+    if (nw <= 1) {
+        if (nw == 1) worker(0);
+    } else {
+        std::vector<std::thread> th;
+        for (uint32_t w = 0; w < nw; ++w) th.emplace_back(worker, w);
+        for (auto& t : th) t.join();
+    }
+    tls_stream = s();
+    for (int r : rc) PSM_TRY(r);
+    return PSM_OK;
+}
+
+// Queries are independent: they are filtered concurrently, each worker with
+// its own support counters.
+int Batch::cfl_filter(std::vector<QueryHost*>& active) {
+    std::vector<std::vector<std::vector<uint32_t>>> sets(active.size());
+    std::vector<std::unique_ptr<DBuf>> cnts(8);
+    std::vector<uint32_t> epochs(8, 0);
+    uint32_t nw = 0;
+    PSM_TRY(for_each_query(active.size(), [&](size_t i, uint32_t w, hipStream_t st) -> int {
+        if (!cnts[w]) {
+            cnts[w].reset(new DBuf());
+            HIP_TRY(ctx, cnts[w]->alloc((size_t)g->V * 4));
+            HIP_TRY(ctx, hipMemsetAsync(cnts[w]->p, 0, (size_t)g->V * 4, st));
+        }
+        const int r = cfl_filter_query(active[i], st, *cnts[w], epochs[w], sets[i]);
+        tls_stream = st;
+        return r;
+    }, nw));
+    for (auto& c : cnts) c.reset();
+    std::vector<uint32_t> all;   // concatenated final candidates
+    for (size_t i = 0; i < active.size(); ++i) {
+        QueryHost* q = active[i];
+        const uint32_t n = q->qg.n;
         q->cand_count.assign(n, 0);
         q->cand_begin.assign(n, 0);
         for (uint32_t u = 0; u < n; ++u) {
-            std::vector<uint32_t> h;
-            HIP_TRY(ctx, download(h, cb[u]->p, len[u], s()));
             q->cand_begin[u] = all.size();
-            q->cand_count[u] = len[u];
-            all.insert(all.end(), h.begin(), h.end());
+            q->cand_count[u] = (uint32_t)sets[i][u].size();
+            all.insert(all.end(), sets[i][u].begin(), sets[i][u].end());
         }
     }
     for (QueryHost* q : qs)
@@ -936,28 +1065,36 @@ int Batch::make_plans(int order_method) {
     tls_stream = s();
     const double t0 = now_ms();
     if (!tables_built) return set_err(ctx, PSM_ESTATE, "build tables before generating a plan");
-    for (QueryHost* q : qs) {
-        if (order_method == PSM_ORDER_GQL) {
+    if (order_method == PSM_ORDER_GQL) {
+        for (QueryHost* q : qs) {
             gql_order(q->qg, q->cand_count, g->V, q->plan);
             backward_neighbours(q->qg, q->plan, false);
-        } else if (order_method == PSM_ORDER_CFL) {
+            q->planned = true;
+        }
+    } else if (order_method == PSM_ORDER_CFL) {
+        // the CFL order reads unit NLF counts and tables from the device:
+        // queries are planned concurrently
+        uint32_t nw = 0;
+        PSM_TRY(for_each_query(qs.size(), [&](size_t i, uint32_t, hipStream_t st) -> int {
+            QueryHost* q = qs[i];
             const QGraph ug = unit_graph(q->qg);
             std::vector<uint32_t> cache(ug.n, kInvalid);
             auto nlf_of = [&](uint32_t u, uint32_t& c) -> int {
-                if (cache[u] == kInvalid) PSM_TRY(nlf_count(ug, u, ug.lab[u], cache[u]));
+                if (cache[u] == kInvalid) PSM_TRY(nlf_count(ug, u, ug.lab[u], cache[u], st));
                 c = cache[u];
                 return PSM_OK;
             };
             CflTree t;
             PSM_TRY(cfl_filter_plan(ug, g->L, g->h_ri_off, g->max_freq, nlf_of, t));
             auto table = [&](uint32_t a, uint32_t b, std::vector<uint32_t>& off,
-                             std::vector<uint32_t>& ix) { return host_table(*q, a, b, off, ix); };
+                             std::vector<uint32_t>& ix) { return host_table(*q, a, b, off, ix, st); };
             PSM_TRY(cfl_order(ug, t, q->cand_count, table, q->plan));
             backward_neighbours(q->qg, q->plan, true);
-        } else {
-            return set_err(ctx, PSM_EINVAL, "unknown order method");
-        }
-        q->planned = true;
+            q->planned = true;
+            return PSM_OK;
+        }, nw));
+    } else {
+        return set_err(ctx, PSM_EINVAL, "unknown order method");
     }
     t_plan += now_ms() - t0;
     return PSM_OK;
@@ -1098,11 +1235,18 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         DQueue hq{};
         hq.tail = (unsigned int)n_tasks;
         hq.cap = (unsigned int)cap_slots;
-        hq.limit_ticks = ctx->time_limit_s > 0 ? std::max<unsigned long long>(1, (unsigned long long)(ctx->time_limit_s * ctx->wall_khz * 1000.0)) : 0;
+        if (enum_limit_s > 0) {
+            const double left_s = enum_limit_s - (now_ms() - t_eval) / 1000.0;
+            hq.limit_ticks = (unsigned long long)std::max(1.0, left_s * ctx->wall_khz * 1000.0);
+        } else if (enum_limit_s < 0 && ctx->time_limit_s > 0) {
+            // what is left of the time limit, counted from the start of filtering
+            double left_s = ctx->time_limit_s;
+            if (t_begin >= 0) left_s -= (now_ms() - t_begin) / 1000.0;
+            hq.limit_ticks = (unsigned long long)std::max(1.0, left_s * ctx->wall_khz * 1000.0);
+        }
         HIP_TRY(ctx, d_q.alloc(sizeof(DQueue)));
         HIP_TRY(ctx, hipMemcpyAsync(d_q.p, &hq, sizeof(hq), hipMemcpyHostToDevice, s()));
         const uint32_t gw = grid_waves_full / 4 * 4;
-        if (!wave_dfs) HIP_TRY(ctx, d_state.alloc((size_t)gw * 64 * dfs_lane_state_words(max_n) * 4));
         std::vector<uint32_t> pend(nq);
         for (uint32_t i = 0; i < nq; ++i) pend[i] = (uint32_t)ntask[i];
         HIP_TRY(ctx, upload(d_pending, pend, s()));
@@ -1118,7 +1262,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
                   launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
                              cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_ready.as<uint32_t>(),
                              d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
-                             d_state.as<uint32_t>(), sink, s()));
+                             sink, s()));
         }
         HIP_TRY(ctx, hipMemcpyAsync(&hq, d_q.p, sizeof(hq), hipMemcpyDeviceToHost, s()));
         HIP_TRY(ctx, timers_flush(ctx));
@@ -1147,8 +1291,12 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
     return PSM_OK;
 }
 
-int Batch::evaluate(uint64_t cap) {
+// enum_limit >= 0 replaces the context's time limit (counted from the start
+// of filtering) by one counted from this call (0: none).
+int Batch::evaluate(uint64_t cap, double enum_limit) {
     tls_stream = s();
+    enum_limit_s = enum_limit;
+    t_eval = now_ms();
     for (QueryHost* q : qs)
         if (!q->planned) return set_err(ctx, PSM_ESTATE, "generate a plan before evaluating");
     std::vector<DPlan> hp(qs.size());
@@ -1531,7 +1679,7 @@ int psm_query_table(const psm_query* q, uint32_t a, uint32_t b, uint32_t* off, u
     if (nnz) *nnz = q->qh.tab_nnz[(size_t)a * n + b];
     if (!off && !idx) return PSM_OK;
     std::vector<uint32_t> o, ix;
-    PSM_TRY(const_cast<Batch&>(q->batch).host_table(q->qh, a, b, o, ix));
+    PSM_TRY(const_cast<Batch&>(q->batch).host_table(q->qh, a, b, o, ix, q->ctx->stream));
     if (off) std::copy(o.begin(), o.end(), off);
     if (idx) std::copy(ix.begin(), ix.end(), idx);
     return PSM_OK;
@@ -1605,47 +1753,97 @@ int psm_match(psm_query* q, int filter, int order, uint64_t cap, psm_enum_stats*
     return psm_evaluate(q, cap, st);
 }
 
-int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32_t* n,
-                    const uint64_t* off_base, const uint64_t* nbr_base, const uint64_t* lab_base,
-                    const uint32_t* q_off, const uint32_t* q_adj, const uint32_t* q_lab, int filter,
-                    int order, uint64_t cap, uint64_t* counts, uint64_t* calls, uint32_t* status,
-                    psm_batch_stats* st) {
-    if (!ctx || !g || !n || !off_base || !nbr_base || !lab_base || !q_off || !q_lab) return PSM_EINVAL;
-    hipSetDevice(ctx->device);
-    const double t0 = now_ms();
-    std::vector<std::unique_ptr<QueryHost>> hosts(nq);
+}  // extern "C"
+
+struct psm_batch {
+    psm_ctx* ctx = nullptr;
+    std::vector<std::unique_ptr<QueryHost>> hosts;
     Batch b;
-    b.ctx = ctx;
-    b.g = g;
+    bool prepared = false;
+    double t0 = 0;
+};
+
+extern "C" {
+
+int psm_batch_create(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32_t* n, const uint64_t* off_base,
+                     const uint64_t* nbr_base, const uint64_t* lab_base, const uint32_t* q_off,
+                     const uint32_t* q_adj, const uint32_t* q_lab, psm_batch** out) {
+    if (!ctx || !g || !n || !off_base || !nbr_base || !lab_base || !q_off || !q_lab || !out) return PSM_EINVAL;
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    std::unique_ptr<psm_batch> B(new psm_batch());
+    B->ctx = ctx;
+    B->t0 = now_ms();
+    B->hosts.resize(nq);
+    B->b.ctx = ctx;
+    B->b.g = g;
     for (uint32_t i = 0; i < nq; ++i) {
-        hosts[i].reset(new QueryHost());
-        int r = make_qgraph(n[i], q_off + off_base[i], q_adj + nbr_base[i], q_lab + lab_base[i],
-                            hosts[i]->qg);
+        B->hosts[i].reset(new QueryHost());
+        const int r = make_qgraph(n[i], q_off + off_base[i], q_adj + nbr_base[i], q_lab + lab_base[i],
+                                  B->hosts[i]->qg);
         if (r != PSM_OK) return set_err(ctx, r, "malformed query graph " + std::to_string(i));
-        b.qs.push_back(hosts[i].get());
+        B->b.qs.push_back(B->hosts[i].get());
     }
-    PSM_TRY(b.filter(filter));
-    PSM_TRY(b.build_tables());
-    PSM_TRY(b.make_plans(order));
-    PSM_TRY(b.evaluate(cap));
-    for (uint32_t i = 0; i < nq; ++i) {
-        if (counts) counts[i] = hosts[i]->count;
-        if (calls) calls[i] = hosts[i]->calls;
-        if (status) status[i] = hosts[i]->timed_out ? PSM_ETIMEOUT : hosts[i]->status;
+    *out = B.release();
+    return PSM_OK;
+}
+
+int psm_batch_prepare(psm_batch* B, int filter, int order) {
+    if (!B) return PSM_EINVAL;
+    (void)hipSetDevice(B->ctx->device);
+    B->prepared = false;
+    PSM_TRY(B->b.filter(filter));
+    PSM_TRY(B->b.build_tables());
+    PSM_TRY(B->b.make_plans(order));
+    B->prepared = true;
+    return PSM_OK;
+}
+
+int psm_batch_evaluate(psm_batch* B, uint64_t cap, double enum_time_limit_s, uint64_t* counts, uint64_t* calls,
+                       uint32_t* status, psm_batch_stats* st) {
+    if (!B) return PSM_EINVAL;
+    if (!B->prepared) return set_err(B->ctx, PSM_ESTATE, "prepare the batch before evaluating");
+    (void)hipSetDevice(B->ctx->device);
+    Batch& b = B->b;
+    const double e0 = b.t_enum + b.t_expand;
+    const uint64_t steps0 = b.lane_steps, iters0 = b.lane_iters, tasks0 = b.tasks_run;
+    PSM_TRY(b.evaluate(cap, enum_time_limit_s));
+    for (size_t i = 0; i < B->hosts.size(); ++i) {
+        const QueryHost* h = B->hosts[i].get();
+        if (counts) counts[i] = h->count;
+        if (calls) calls[i] = h->calls;
+        if (status) status[i] = h->timed_out ? PSM_ETIMEOUT : h->status;
     }
     if (st) {
         memset(st, 0, sizeof(*st));
         st->filter_ms = b.t_filter;
         st->table_ms = b.t_table;
         st->plan_ms = b.t_plan;
-        st->enum_ms = b.t_enum + b.t_expand;
-        st->total_ms = now_ms() - t0;
-        st->tasks = b.tasks_run;
-        st->dfs_lane_steps = b.lane_steps;
-        st->dfs_lane_iters = b.lane_iters;
-        for (auto& h : hosts) st->probe_bytes += h->probe_bytes;
+        st->enum_ms = b.t_enum + b.t_expand - e0;
+        st->total_ms = now_ms() - B->t0;
+        st->tasks = b.tasks_run - tasks0;
+        st->dfs_lane_steps = b.lane_steps - steps0;
+        st->dfs_lane_iters = b.lane_iters - iters0;
+        for (auto& h : B->hosts) st->probe_bytes += h->probe_bytes;
     }
     return PSM_OK;
+}
+
+int psm_batch_destroy(psm_batch* B) {
+    delete B;
+    return PSM_OK;
+}
+
+int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32_t* n,
+                    const uint64_t* off_base, const uint64_t* nbr_base, const uint64_t* lab_base,
+                    const uint32_t* q_off, const uint32_t* q_adj, const uint32_t* q_lab, int filter,
+                    int order, uint64_t cap, uint64_t* counts, uint64_t* calls, uint32_t* status,
+                    psm_batch_stats* st) {
+    psm_batch* B = nullptr;
+    PSM_TRY(psm_batch_create(ctx, g, nq, n, off_base, nbr_base, lab_base, q_off, q_adj, q_lab, &B));
+    std::unique_ptr<psm_batch> own(B);
+    PSM_TRY(psm_batch_prepare(B, filter, order));
+    return psm_batch_evaluate(B, cap, -1.0, counts, calls, status, st);
 }
 
 }  // extern "C"
@@ -1697,7 +1895,8 @@ namespace {
 // sets (label buckets), table rows, bitmaps + ranks of its query vertices.
 uint64_t footprint(const psm_graph* g, const QGraph& q) {
     const uint64_t wps = ((uint64_t)g->V + 31) / 32;
-    uint64_t b = (uint64_t)q.n * wps * 8;
+    // bitmaps + ranks of its query vertices, and its GQL membership words
+    uint64_t b = (uint64_t)q.n * wps * 8 + (uint64_t)g->V * 8;
     for (uint32_t u = 0; u < q.n; ++u) {
         const uint32_t l = q.lab[u];
         const uint64_t bucket = l < g->L ? g->h_ri_off[l + 1] - g->h_ri_off[l] : 0;
@@ -1797,6 +1996,10 @@ int psm_server_create(psm_ctx* ctx, const psm_graph* g, uint32_t workers, uint32
         }
         c->time_limit_s = time_limit_s;
         c->verbose = ctx->verbose;
+        // workers run with the creating context's launch shape
+        c->dfs_waves_per_cu = ctx->dfs_waves_per_cu;
+        c->expand_tasks_per_lane = ctx->expand_tasks_per_lane;
+        c->dfs_strategy = ctx->dfs_strategy;
         S->ctxs.push_back(c);
     }
     for (psm_ctx* c : S->ctxs) S->workers.emplace_back(serve_worker, S.get(), c);

@@ -49,8 +49,10 @@ def rmat_graph(scale, E, num_labels, seed, a=0.57, b=0.19, c=0.19) -> CSRGraph:
     return CSRGraph(off, adj, labels(1 << scale, num_labels, 0.0, seed ^ 0x5A5A5A5A))
 
 
-def query(data: CSRGraph, n: int, dense: bool, seed: int) -> CSRGraph:
-    """An n-vertex connected query drawn from `data` (labels kept)."""
+def query(data: CSRGraph, n: int, dense, seed: int) -> CSRGraph:
+    """An n-vertex connected query drawn from `data` (labels kept). dense:
+    False/0 random-walk tree, True/1 greedy densest induced subgraph, 2 the
+    induced subgraph of a random walk (with cycles)."""
     verts = np.zeros(n, np.uint32)
     max_e = n * (n - 1) // 2
     edges = np.zeros(2 * max(1, max_e), np.uint32)

@@ -253,3 +253,41 @@ def batch_match(dg: DeviceGraph, queries, filter: str = "GQL", order: str = "GQL
                                              L.u64p(calls), L.u32p(status), ctypes.byref(st)),
            "psm_batch_match")
     return counts[:m], calls[:m], status[:m], st
+
+
+class Batch:
+    """The batched form stage by stage (psm_batch): prepare once (filter,
+    tables, plans), evaluate as often as needed."""
+
+    def __init__(self, dg: DeviceGraph, queries):
+        self.dg = dg
+        self.m = len(queries)
+        self._arrays = pack_queries(queries)
+        n, ob, nb, lb, off, adj, lab = self._arrays
+        h = ctypes.c_void_p()
+        _check(dg.dev.h, L.lib().psm_batch_create(dg.dev.h, dg.h, self.m, L.u32p(n), L.u64p(ob), L.u64p(nb),
+                                                  L.u64p(lb), L.u32p(off), L.u32p(adj), L.u32p(lab),
+                                                  ctypes.byref(h)), "psm_batch_create")
+        self.h = h
+
+    def prepare(self, filter: str = "GQL", order: str = "GQL"):
+        _check(self.dg.dev.h, L.lib().psm_batch_prepare(self.h, L.FILTERS[filter], L.ORDERS[order]),
+               "psm_batch_prepare")
+
+    def evaluate(self, cap: int = 0, enum_time_limit: float = -1.0):
+        counts = np.zeros(max(1, self.m), np.uint64)
+        calls = np.zeros(max(1, self.m), np.uint64)
+        status = np.zeros(max(1, self.m), np.uint32)
+        st = L.BatchStats()
+        _check(self.dg.dev.h, L.lib().psm_batch_evaluate(self.h, cap, enum_time_limit, L.u64p(counts),
+                                                         L.u64p(calls), L.u32p(status), ctypes.byref(st)),
+               "psm_batch_evaluate")
+        return counts[: self.m], calls[: self.m], status[: self.m], st
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().psm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -66,3 +66,57 @@ def c2_queries(g: CSRGraph, count: int = 512, seed: int = 22):
 
 def c1_query(g: CSRGraph, seed: int = 11) -> CSRGraph:
     return gen.query(g, 8, True, seed)
+
+
+def _adjsets(q: CSRGraph):
+    off = q.offsets.astype(np.int64)
+    return [set(int(x) for x in q.neighbors[off[v]:off[v + 1]]) for v in range(q.num_vertices)]
+
+
+def has_cycle(q: CSRGraph) -> bool:
+    """A connected query has a cycle iff it has at least as many edges as vertices."""
+    return q.num_edges >= q.num_vertices
+
+
+def clique_size(q: CSRGraph, k: int) -> bool:
+    """True when the query contains a k-clique (k <= 4, small queries)."""
+    import itertools
+    adj = _adjsets(q)
+    for c in itertools.combinations(range(q.num_vertices), k):
+        if all(b in adj[a] for a, b in itertools.combinations(c, 2)):
+            return True
+    return False
+
+
+def c3_queries(g: CSRGraph, count: int = 64, seed: int = 33):
+    """64 queries of 12-16 vertices whose induced subgraph has a cycle: the
+    vertices of a random walk with every data edge among them."""
+    rng = np.random.default_rng(seed)
+    qs, meta = [], []
+    while len(qs) < count:
+        n = int(rng.integers(12, 17))
+        q = gen.query(g, n, 2, int(rng.integers(0, 2**62)))
+        if not has_cycle(q):
+            continue
+        qs.append(q)
+        meta.append(dict(n=n, edges=q.num_edges))
+    return qs, meta
+
+
+def c5_queries(g: CSRGraph, count: int = 32, seed: int = 55):
+    """32 dense queries of 6-10 vertices (greedy densest growth); every query
+    has a triangle and every other one a 4-clique."""
+    rng = np.random.default_rng(seed)
+    qs, meta = [], []
+    while len(qs) < count:
+        n = int(rng.integers(6, 11))
+        want4 = len(qs) % 2 == 0
+        for _ in range(64):
+            q = gen.query(g, n, 1, int(rng.integers(0, 2**62)))
+            if clique_size(q, 3) and (not want4 or clique_size(q, 4)):
+                break
+        else:
+            continue
+        qs.append(q)
+        meta.append(dict(n=n, edges=q.num_edges, k4=clique_size(q, 4)))
+    return qs, meta

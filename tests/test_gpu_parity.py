@@ -200,3 +200,55 @@ def test_served_results_equal_alone_and_in_the_mix(hprd):
         assert r.count == recs[k % len(recs)]["embeddings"], k
         assert r.calls == recs[k % len(recs)]["calls"], k
     assert max(r.batch_size for r in results.values()) >= 1
+
+
+@pytest.fixture(scope="module")
+def c2_graph():
+    from psm_benchmark_amd import workloads as W
+    g = W.data_graph("C2")
+    qs, _ = W.c2_queries(g)
+    return g, qs[::11][:32]
+
+
+def test_c2_scale_lane_wave_plain_agree(c2_graph):
+    """The C2 workload graph (1.1M vertices, 3.0M edges) at full size, 32 of
+    its queries (4 to 32 vertices): the
+    batched lane and wavefront kernels and the plain path agree on
+    min(cap, total) for every query none of them stopped early, and a
+    prepared batch evaluated again at a smaller cap reports min(cap, count)."""
+    from psm_benchmark_amd import autotune
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.matcher import Batch, Device, DeviceGraph, match
+    g, qs = c2_graph
+    dev = Device(0)
+    dg = DeviceGraph(dev, g)
+    L.lib().psm_set_time_limit(dev.h, 30.0)
+    cap = 10_000
+    got = {}
+    for strategy in ("lane", "wave"):
+        autotune.set_config(dev, {"dfs_strategy": strategy})
+        b = Batch(dg, qs)
+        b.prepare("GQL", "GQL")
+        counts, calls, status, st = b.evaluate(cap, 8.0)
+        small, _, status_s, _ = b.evaluate(1000, 8.0)
+        b.close()
+        for i in range(len(qs)):
+            if status[i] != L.PSM_ETIMEOUT and status_s[i] != L.PSM_ETIMEOUT:
+                assert small[i] == min(1000, counts[i]), (strategy, i)
+        got[strategy] = (counts, status)
+        print(strategy, int(counts.sum()), int((status == L.PSM_ETIMEOUT).sum()), flush=True)
+    autotune.set_config(dev, {"dfs_strategy": "lane"})
+    (cl, sl), (cw, sw) = got["lane"], got["wave"]
+    checked = 0
+    for i, q in enumerate(qs):
+        if sl[i] == L.PSM_ETIMEOUT or sw[i] == L.PSM_ETIMEOUT:
+            continue
+        assert cl[i] == cw[i], i
+        if i % 4 == 0:
+            r = match(dg, q, "GQL_GQL_GQL", cap=cap)
+            if not r.timed_out:
+                assert r.count == cl[i], i
+        checked += 1
+    assert checked >= len(qs) // 2
+    dg.close()
+    dev.close()
