@@ -493,6 +493,8 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
 // the ranks in C(b) of the neighbours of C(a)[j] that are in C(b). The rank
 // of w is the count of set bits before it in the (query, b) bitmap.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kRowCoop = 64;   // rows of candidates with more neighbours go to a wavefront
+
 __device__ __forceinline__ uint32_t bit_rank(const uint32_t* bm, const uint32_t* wp, uint32_t w) {
     const uint32_t word = w >> 5;
     return (wp[word] - wp[0]) + __popc(bm[word] & ((1u << (w & 31)) - 1u));
@@ -502,27 +504,79 @@ __global__ __launch_bounds__(kBlock) void k_table_count(
     DevGraph g, const TItem* __restrict__ items, uint32_t n_items,
     const uint64_t* __restrict__ item_row_start, uint64_t n_rows,
     const uint32_t* __restrict__ cand_pool, const uint32_t* __restrict__ bitmaps, uint64_t wps,
-    uint64_t* __restrict__ row_deg, unsigned long long* __restrict__ bytes) {
+    uint64_t* __restrict__ row_deg, uint64_t* __restrict__ long_rows, unsigned long long* __restrict__ n_long,
+    unsigned long long* __restrict__ bytes) {
     unsigned long long by = 0;
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     const uint64_t rows_rounded = (n_rows + stride - 1) / stride * stride;
     for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < rows_rounded; r += stride) {
-        if (r >= n_rows) continue;
-        const uint32_t ii = find_u64(item_row_start, n_items, r);
-        const TItem it = items[ii];
-        const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
-        const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
-        uint32_t c = 0;
-        for (uint32_t p = g.off[v]; p < g.off[v + 1]; ++p) {
-            const uint32_t w = g.adj[p];
-            c += (bm[w >> 5] >> (w & 31)) & 1u;
+        bool defer = false;
+        if (r < n_rows) {
+            const uint32_t ii = find_u64(item_row_start, n_items, r);
+            const TItem it = items[ii];
+            const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
+            const uint32_t vb = g.off[v], ve = g.off[v + 1];
+            if (ve - vb > kRowCoop) {
+                defer = true;   // a wavefront scans it (k_table_count_long)
+            } else {
+                const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
+                uint32_t c = 0;
+                for (uint32_t p = vb; p < ve; ++p) {
+                    const uint32_t w = g.adj[p];
+                    c += (bm[w >> 5] >> (w & 31)) & 1u;
+                }
+                row_deg[r] = c;
+                // candidate + offsets + N(v) + one bitmap word per neighbour + the count
+                by += 4 + 8 + 8ull * (ve - vb) + 8;
+            }
         }
-        row_deg[r] = c;
-        // candidate + offsets + N(v) + one bitmap word per neighbour + the count
-        by += 4 + 8 + 8ull * (g.off[v + 1] - g.off[v]) + 8;
+        const unsigned long long bal = __ballot(defer);
+        if (bal) {
+            const uint32_t leader = __ffsll((long long)bal) - 1;
+            unsigned long long base = 0;
+            if (lane_id() == leader) base = atomicAdd(n_long, (unsigned long long)__popcll(bal));
+            base = __shfl(base, leader);
+            if (defer) long_rows[base + __popcll(bal & lanemask_lt())] = r;
+        }
     }
     wave_add(bytes, by);
 }
+
+// Rows of candidates with more than kRowCoop neighbours, one wavefront each:
+// lanes read 64 consecutive neighbours at a time.
+__global__ __launch_bounds__(kBlock) void k_table_count_long(
+    DevGraph g, const TItem* __restrict__ items, uint32_t n_items,
+    const uint64_t* __restrict__ item_row_start, const uint64_t* __restrict__ long_rows, uint64_t n_long,
+    const uint32_t* __restrict__ cand_pool, const uint32_t* __restrict__ bitmaps, uint64_t wps,
+    uint64_t* __restrict__ row_deg, unsigned long long* __restrict__ bytes) {
+    const uint32_t lane = lane_id();
+    const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+    unsigned long long by = 0;
+    for (uint64_t i = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; i < n_long; i += waves) {
+        const uint64_t r = long_rows[i];
+        const uint32_t ii = find_u64(item_row_start, n_items, r);
+        const TItem it = items[ii];
+        const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
+        const uint32_t vb = g.off[v], ve = g.off[v + 1];
+        const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
+        uint64_t c = 0;
+        for (uint32_t base = vb; base < ve; base += kWave) {
+            const uint32_t p = base + lane;
+            bool hit = false;
+            if (p < ve) {
+                const uint32_t w = g.adj[p];
+                hit = (bm[w >> 5] >> (w & 31)) & 1u;
+            }
+            c += __popcll(__ballot(hit));
+        }
+        if (lane == 0) {
+            row_deg[r] = c;
+            by += 4 + 8 + 8ull * (ve - vb) + 8;
+        }
+    }
+    wave_add(bytes, by);
+}
+
 
 __global__ __launch_bounds__(kBlock) void k_table_fill(
     DevGraph g, const TItem* __restrict__ items, uint32_t n_items,
@@ -538,6 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_table_fill(
         const uint32_t ii = find_u64(item_row_start, n_items, r);
         const TItem it = items[ii];
         const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
+        if (g.off[v + 1] - g.off[v] > kRowCoop) continue;   // k_table_fill_long
         const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
         const uint32_t* wp = word_prefix + (size_t)it.bm_b * wps;
         const uint64_t o0 = row_off[r];
@@ -553,28 +608,103 @@ __global__ __launch_bounds__(kBlock) void k_table_fill(
     wave_add(bytes, by);
 }
 
+// Long rows of the fill pass, one wavefront each: every 64-neighbour chunk is
+// written in order (ballot prefix), so the row comes out ascending.
+__global__ __launch_bounds__(kBlock) void k_table_fill_long(
+    DevGraph g, const TItem* __restrict__ items, uint32_t n_items,
+    const uint64_t* __restrict__ item_row_start, const uint64_t* __restrict__ long_rows, uint64_t n_long,
+    const uint32_t* __restrict__ cand_pool, const uint32_t* __restrict__ bitmaps,
+    const uint32_t* __restrict__ word_prefix, uint64_t wps, const uint64_t* __restrict__ row_off,
+    uint32_t* __restrict__ idx, unsigned long long* __restrict__ bytes) {
+    const uint32_t lane = lane_id();
+    const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+    unsigned long long by = 0;
+    for (uint64_t i = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; i < n_long; i += waves) {
+        const uint64_t r = long_rows[i];
+        const uint32_t ii = find_u64(item_row_start, n_items, r);
+        const TItem it = items[ii];
+        const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
+        const uint32_t vb = g.off[v], ve = g.off[v + 1];
+        const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
+        const uint32_t* wp = word_prefix + (size_t)it.bm_b * wps;
+        const uint64_t o0 = row_off[r];
+        uint64_t o = o0;
+        for (uint32_t base = vb; base < ve; base += kWave) {
+            const uint32_t p = base + lane;
+            bool hit = false;
+            uint32_t w = 0;
+            if (p < ve) {
+                w = g.adj[p];
+                hit = (bm[w >> 5] >> (w & 31)) & 1u;
+            }
+            const unsigned long long bal = __ballot(hit);
+            if (hit) idx[o + __popcll(bal & lanemask_lt())] = bit_rank(bm, wp, w);
+            o += __popcll(bal);
+        }
+        if (lane == 0) by += 4 + 8 + 8 + 8ull * (ve - vb) + 12ull * (o - o0);
+    }
+    wave_add(bytes, by);
+}
+
 // ---------------------------------------------------------------------------
 // CFL support counting (FilterVertices.cpp:841-966 restated with set
 // semantics): cnt[w] advances from `expect` to `next` once per pivot for a
 // data vertex w with the query vertex's label and degree that neighbours a
 // live candidate of that pivot.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void cfl_mark(const DevGraph& g, uint32_t w, uint32_t label, uint32_t degree,
+                                         uint32_t* __restrict__ cnt, uint32_t expect, uint32_t next) {
+    if (g.lab[w] != label || g.off[w + 1] - g.off[w] < degree) return;
+    if (expect == 0) {
+        // first pivot: any stale value below `next` becomes `next`
+        atomicMax(cnt + w, next);
+    } else {
+        atomicCAS(cnt + w, expect, next);
+    }
+}
+
+// Thread per pivot candidate; candidates with more than kRowCoop neighbours
+// are listed for k_cfl_support_long (one atomic per wavefront).
 __global__ __launch_bounds__(kBlock) void k_cfl_support(
     DevGraph g, const uint32_t* __restrict__ cand_pool, uint64_t cand_begin, uint32_t len,
-    uint32_t label, uint32_t degree, uint32_t* __restrict__ cnt, uint32_t expect, uint32_t next) {
-    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < len; j += gridDim.x * kBlock) {
-        const uint32_t v = cand_pool[cand_begin + j];
-        if (v == kInvalid) continue;
-        for (uint32_t p = g.off[v]; p < g.off[v + 1]; ++p) {
-            const uint32_t w = g.adj[p];
-            if (g.lab[w] != label || g.off[w + 1] - g.off[w] < degree) continue;
-            if (expect == 0) {
-                // first pivot: any stale value below `next` becomes `next`
-                atomicMax(cnt + w, next);
-            } else {
-                atomicCAS(cnt + w, expect, next);
+    uint32_t label, uint32_t degree, uint32_t* __restrict__ cnt, uint32_t expect, uint32_t next,
+    uint32_t* __restrict__ defer, uint32_t* __restrict__ n_defer) {
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t rounded = (len + stride - 1) / stride * stride;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < rounded; j += stride) {
+        uint32_t v = kInvalid;
+        bool hub = false;
+        if (j < len) {
+            v = cand_pool[cand_begin + j];
+            if (v != kInvalid) {
+                const uint32_t vb = g.off[v], ve = g.off[v + 1];
+                hub = ve - vb > kRowCoop;
+                if (!hub)
+                    for (uint32_t p = vb; p < ve; ++p) cfl_mark(g, g.adj[p], label, degree, cnt, expect, next);
             }
         }
+        const unsigned long long bal = __ballot(hub);
+        if (bal) {
+            const uint32_t leader = __ffsll((long long)bal) - 1;
+            uint32_t base = 0;
+            if (lane_id() == leader) base = atomicAdd(n_defer, (uint32_t)__popcll(bal));
+            base = __shfl(base, leader);
+            if (hub) defer[base + __popcll(bal & lanemask_lt())] = v;
+        }
+    }
+}
+
+// The listed pivot candidates, one wavefront each over 64 neighbours at a time.
+__global__ __launch_bounds__(kBlock) void k_cfl_support_long(
+    DevGraph g, const uint32_t* __restrict__ defer, const uint32_t* __restrict__ n_defer, uint32_t label,
+    uint32_t degree, uint32_t* __restrict__ cnt, uint32_t expect, uint32_t next) {
+    const uint32_t lane = lane_id();
+    const uint32_t n = *n_defer;
+    const uint32_t waves = gridDim.x * (kBlock / kWave);
+    for (uint32_t i = (blockIdx.x * kBlock + threadIdx.x) / kWave; i < n; i += waves) {
+        const uint32_t v = defer[i];
+        const uint32_t vb = g.off[v], ve = g.off[v + 1];
+        for (uint32_t p = vb + lane; p < ve; p += kWave) cfl_mark(g, g.adj[p], label, degree, cnt, expect, next);
     }
 }
 
@@ -1959,11 +2089,29 @@ static uint32_t grid_for(uint64_t n) {
 hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,
                               const uint64_t* item_row_start, uint64_t n_rows,
                               const uint32_t* cand_pool, const uint32_t* bitmaps,
-                              uint64_t words_per_slot, uint64_t* row_deg, unsigned long long* bytes,
-                              hipStream_t s) {
+                              uint64_t words_per_slot, uint64_t* row_deg, uint64_t* long_rows,
+                              unsigned long long* n_long, unsigned long long* bytes, hipStream_t s) {
     if (n_rows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_table_count, dim3(grid_for(n_rows)), dim3(kBlock), 0, s, g, items, n_items,
-                       item_row_start, n_rows, cand_pool, bitmaps, words_per_slot, row_deg, bytes);
+                       item_row_start, n_rows, cand_pool, bitmaps, words_per_slot, row_deg, long_rows, n_long,
+                       bytes);
+    return hipGetLastError();
+}
+
+hipError_t launch_table_long(int fill, const DevGraph& g, const TItem* items, uint32_t n_items,
+                             const uint64_t* item_row_start, const uint64_t* long_rows, uint64_t n_long,
+                             const uint32_t* cand_pool, const uint32_t* bitmaps, const uint32_t* word_prefix,
+                             uint64_t words_per_slot, uint64_t* row_deg, const uint64_t* row_off, uint32_t* idx,
+                             unsigned long long* bytes, hipStream_t s) {
+    if (n_long == 0) return hipSuccess;
+    const uint64_t want = (n_long + (kBlock / kWave) - 1) / (kBlock / kWave);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, 65536);
+    if (fill)
+        hipLaunchKernelGGL(k_table_fill_long, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, item_row_start,
+                           long_rows, n_long, cand_pool, bitmaps, word_prefix, words_per_slot, row_off, idx, bytes);
+    else
+        hipLaunchKernelGGL(k_table_count_long, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, item_row_start,
+                           long_rows, n_long, cand_pool, bitmaps, words_per_slot, row_deg, bytes);
     return hipGetLastError();
 }
 
@@ -1982,10 +2130,18 @@ hipError_t launch_table_fill(const DevGraph& g, const TItem* items, uint32_t n_i
 
 hipError_t launch_cfl_support(const DevGraph& g, const uint32_t* cand_pool, uint64_t cand_begin,
                               uint32_t len, uint32_t label, uint32_t degree, uint32_t* cnt,
-                              uint32_t expect, uint32_t next, hipStream_t s) {
+                              uint32_t expect, uint32_t next, uint32_t* defer, uint32_t* n_defer,
+                              hipStream_t s) {
     if (len == 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(n_defer, 0, 4, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cfl_support, dim3(grid_for(len)), dim3(kBlock), 0, s, g, cand_pool,
-                       cand_begin, len, label, degree, cnt, expect, next);
+                       cand_begin, len, label, degree, cnt, expect, next, defer, n_defer);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint32_t blocks = std::min<uint32_t>((len + (kBlock / kWave) - 1) / (kBlock / kWave), 1024);
+    hipLaunchKernelGGL(k_cfl_support_long, dim3(blocks), dim3(kBlock), 0, s, g, defer, n_defer, label, degree,
+                       cnt, expect, next);
     return hipGetLastError();
 }
 

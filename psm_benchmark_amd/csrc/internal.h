@@ -200,20 +200,31 @@ hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items
                            uint32_t n, const uint32_t* left_slots, uint32_t* cand_pool, uint32_t* bitmaps,
                            uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
                            hipStream_t s);
+// thread per row; rows of candidates with more than 64 neighbours are listed
+// in long_rows (count in *n_long) for launch_table_long
 hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,
                               const uint64_t* item_row_start, uint64_t n_rows,
                               const uint32_t* cand_pool, const uint32_t* bitmaps,
-                              uint64_t words_per_slot, uint64_t* row_deg, unsigned long long* bytes,
-                              hipStream_t s);
+                              uint64_t words_per_slot, uint64_t* row_deg, uint64_t* long_rows,
+                              unsigned long long* n_long, unsigned long long* bytes, hipStream_t s);
+// wavefront per listed row: the count pass (fill = 0) or the fill pass
+hipError_t launch_table_long(int fill, const DevGraph& g, const TItem* items, uint32_t n_items,
+                             const uint64_t* item_row_start, const uint64_t* long_rows, uint64_t n_long,
+                             const uint32_t* cand_pool, const uint32_t* bitmaps, const uint32_t* word_prefix,
+                             uint64_t words_per_slot, uint64_t* row_deg, const uint64_t* row_off, uint32_t* idx,
+                             unsigned long long* bytes, hipStream_t s);
 hipError_t launch_table_fill(const DevGraph& g, const TItem* items, uint32_t n_items,
                              const uint64_t* item_row_start, uint64_t n_rows,
                              const uint32_t* cand_pool, const uint32_t* bitmaps,
                              const uint32_t* word_prefix, uint64_t words_per_slot,
                              const uint64_t* row_off, uint32_t* idx, unsigned long long* bytes,
                              hipStream_t s);
+// defer: room for len pivot candidates (those with more than 64 neighbours
+// are handled by wavefronts), n_defer: one counter
 hipError_t launch_cfl_support(const DevGraph& g, const uint32_t* cand_pool, uint64_t cand_begin,
                               uint32_t len, uint32_t label, uint32_t degree, uint32_t* cnt,
-                              uint32_t expect, uint32_t next, hipStream_t s);
+                              uint32_t expect, uint32_t next, uint32_t* defer, uint32_t* n_defer,
+                              hipStream_t s);
 hipError_t launch_cfl_prune(const uint32_t* cnt, uint32_t* cand_pool, uint64_t cand_begin,
                             uint32_t len, uint32_t target, hipStream_t s);
 hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row_off,

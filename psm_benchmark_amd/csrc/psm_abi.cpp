@@ -398,9 +398,9 @@ struct Batch {
     int compact_pool();
     int gql_refine(std::vector<QueryHost*>& active);
     int for_each_query(size_t count, const std::function<int(size_t, uint32_t, hipStream_t)>& fn,
-                       uint32_t& workers);
+                       uint32_t& workers, const std::function<void(uint32_t)>& worker_done = nullptr);
     int cfl_filter(std::vector<QueryHost*>& active);
-    int cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch,
+    int cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch, DBuf& defer,
                          std::vector<std::vector<uint32_t>>& sets);
     int build_tables();
     int make_plans(int order_method);
@@ -776,16 +776,27 @@ int Batch::build_tables() {
             }
     }
     const uint32_t ni = (uint32_t)items.size();
-    DBuf d_items, d_rs, d_deg, d_tmp, d_ends, d_nnz;
+    DBuf d_items, d_rs, d_deg, d_tmp, d_ends, d_nnz, d_long, d_nlong;
     HIP_TRY(ctx, upload(d_items, items, s()));
     HIP_TRY(ctx, upload(d_rs, row_start, s()));
     HIP_TRY(ctx, d_deg.alloc(n_rows * 8));
     HIP_TRY(ctx, row_off.alloc((n_rows + 1) * 8));
     HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(n_rows) * 8));
+    HIP_TRY(ctx, d_long.alloc(std::max<uint64_t>(n_rows, 1) * 8));
+    HIP_TRY(ctx, d_nlong.alloc(8));
+    HIP_TRY(ctx, hipMemsetAsync(d_nlong.p, 0, 8, s()));
     TIMED(ctx, kKTableCount,
           launch_table_count(g->dg, d_items.as<TItem>(), ni, d_rs.as<uint64_t>(), n_rows,
                              cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
-                             d_deg.as<uint64_t>(), ctx->d_kbytes + kKTableCount, s()));
+                             d_deg.as<uint64_t>(), d_long.as<uint64_t>(), d_nlong.as<unsigned long long>(),
+                             ctx->d_kbytes + kKTableCount, s()));
+    uint64_t n_long = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&n_long, d_nlong.p, 8, hipMemcpyDeviceToHost, s()));
+    HIP_TRY(ctx, hipStreamSynchronize(s()));
+    TIMED(ctx, kKTableCount,
+          launch_table_long(0, g->dg, d_items.as<TItem>(), ni, d_rs.as<uint64_t>(), d_long.as<uint64_t>(), n_long,
+                            cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), nullptr, wps, d_deg.as<uint64_t>(),
+                            nullptr, nullptr, ctx->d_kbytes + kKTableCount, s()));
     HIP_TRY(ctx, launch_scan_u64(d_deg.as<uint64_t>(), n_rows, row_off.as<uint64_t>(),
                                  d_tmp.as<unsigned long long>(), s()));
     HIP_TRY(ctx, hipMemcpyAsync(&n_idx, row_off.as<uint64_t>() + n_rows, 8, hipMemcpyDeviceToHost, s()));
@@ -796,6 +807,11 @@ int Batch::build_tables() {
                             cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(),
                             wprefix.as<uint32_t>(), wps, row_off.as<uint64_t>(),
                             idx.as<uint32_t>(), ctx->d_kbytes + kKTableFill, s()));
+    TIMED(ctx, kKTableFill,
+          launch_table_long(1, g->dg, d_items.as<TItem>(), ni, d_rs.as<uint64_t>(), d_long.as<uint64_t>(), n_long,
+                            cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wprefix.as<uint32_t>(), wps,
+                            nullptr, row_off.as<uint64_t>(), idx.as<uint32_t>(), ctx->d_kbytes + kKTableFill,
+                            s()));
     std::vector<uint64_t> ends(ni), nnz;
     for (uint32_t i = 0; i < ni; ++i) ends[i] = row_start[i] + items[i].len_a;
     HIP_TRY(ctx, upload(d_ends, ends, s()));
@@ -868,7 +884,7 @@ int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out,
 // The CFL filter of one query on stream `st` with its own support counters;
 // its candidate sets (one per query vertex) go to `sets`. Its steps depend on
 // the query's own BFS levels.
-int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch,
+int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch, DBuf& defer,
                             std::vector<std::vector<uint32_t>>& sets) {
     tls_stream = st;
     const QGraph& qg = q->qg;
@@ -936,7 +952,8 @@ int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& e
             const uint32_t p = pivots[i];
             HIP_TRY(ctx, launch_cfl_support(g->dg, cb[p]->as<uint32_t>(), 0, len[p], qg.lab[u],
                                             qg.degree(u), cnt.as<uint32_t>(),
-                                            i == 0 ? 0u : base + i, base + i + 1, st));
+                                            i == 0 ? 0u : base + i, base + i + 1, defer.as<uint32_t>(),
+                                            defer.as<uint32_t>() + g->V, st));
         }
         target = base + (uint32_t)pivots.size();
         return PSM_OK;
@@ -978,8 +995,10 @@ int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& e
 // Runs fn(i, worker, stream) for i in [0, count) on up to 8 host threads of
 // this context, each with its own HIP stream (queries whose steps depend on
 // their own structure run concurrently this way). Returns the first error.
+// worker_done(w) runs on the worker's thread before its stream is destroyed
+// (buffers tied to that stream are released there).
 int Batch::for_each_query(size_t count, const std::function<int(size_t, uint32_t, hipStream_t)>& fn,
-                          uint32_t& workers) {
+                          uint32_t& workers, const std::function<void(uint32_t)>& worker_done) {
     const uint32_t nw = (uint32_t)std::min<size_t>(count, 8);
     workers = nw;
     std::vector<int> rc(nw, PSM_OK);
@@ -998,6 +1017,7 @@ int Batch::for_each_query(size_t count, const std::function<int(size_t, uint32_t
             rc[w] = fn(i, w, st);
         }
         (void)hipStreamSynchronize(st);
+        if (worker_done) worker_done(w);
         tls_stream = nullptr;
         (void)hipStreamDestroy(st);
     };
@@ -1017,7 +1037,7 @@ int Batch::for_each_query(size_t count, const std::function<int(size_t, uint32_t
 // its own support counters.
 int Batch::cfl_filter(std::vector<QueryHost*>& active) {
     std::vector<std::vector<std::vector<uint32_t>>> sets(active.size());
-    std::vector<std::unique_ptr<DBuf>> cnts(8);
+    std::vector<std::unique_ptr<DBuf>> cnts(8), defers(8);
     std::vector<uint32_t> epochs(8, 0);
     uint32_t nw = 0;
     PSM_TRY(for_each_query(active.size(), [&](size_t i, uint32_t w, hipStream_t st) -> int {
@@ -1025,12 +1045,16 @@ int Batch::cfl_filter(std::vector<QueryHost*>& active) {
             cnts[w].reset(new DBuf());
             HIP_TRY(ctx, cnts[w]->alloc((size_t)g->V * 4));
             HIP_TRY(ctx, hipMemsetAsync(cnts[w]->p, 0, (size_t)g->V * 4, st));
+            defers[w].reset(new DBuf());   // hub pivot candidates + their count
+            HIP_TRY(ctx, defers[w]->alloc(((size_t)g->V + 1) * 4));
         }
-        const int r = cfl_filter_query(active[i], st, *cnts[w], epochs[w], sets[i]);
+        const int r = cfl_filter_query(active[i], st, *cnts[w], epochs[w], *defers[w], sets[i]);
         tls_stream = st;
         return r;
-    }, nw));
-    for (auto& c : cnts) c.reset();
+    }, nw, [&](uint32_t w) {
+        cnts[w].reset();
+        defers[w].reset();
+    }));
     std::vector<uint32_t> all;   // concatenated final candidates
     for (size_t i = 0; i < active.size(); ++i) {
         QueryHost* q = active[i];
