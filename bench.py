@@ -158,6 +158,8 @@ def main():
     ap.add_argument("--plain-budget", type=float, default=60.0,
                     help="C3 / C5: total time spent on plain-path and join queries (s)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--shape", default=None, metavar="STRATEGY,WAVES_PER_CU,TASKS_PER_WORKER",
+                    help="set this launch shape instead of autotuning (e.g. lane,4,4)")
     ap.add_argument("--requests", type=int, default=1000, help="C4: requests per arrival rate")
     ap.add_argument("--workers", type=int, default=4, help="C4: server workers (HIP streams)")
     ap.add_argument("--max-batch", type=int, default=128, help="C4: requests per batch at most")
@@ -207,7 +209,9 @@ def main():
     log(f"graph resident in HBM ({time.time() - t0:.1f}s)")
 
     tuned, trials = None, []
-    if not args.no_tune:
+    if args.shape:
+        tuned = apply_shape(dev, args.shape)
+    elif not args.no_tune:
         from psm_benchmark_amd import autotune
         t0 = time.time()
         tuned, trials = autotune.tune(dg, qs, "GQL", "GQL", args.cap, sample=args.tune_sample,
@@ -312,6 +316,14 @@ def main():
     print(json.dumps(out))
 
 
+def apply_shape(dev, text):
+    from psm_benchmark_amd import autotune
+    st, w, e = text.split(",")
+    cfg = {"dfs_strategy": st, "dfs_waves_per_cu": int(w), "expand_tasks_per_lane": float(e)}
+    autotune.set_config(dev, cfg)
+    return cfg
+
+
 def roofline_of(ks, steps, traffic_file=None):
     dom = max(ks.values(), key=lambda k: k["ms"])
     achieved = dom["bytes"] / dom["launches"] / (dom["ms"] / dom["launches"] / 1e3) / 1e9
@@ -350,7 +362,9 @@ def large_workload(args):
     dev = Device(0)
     dg = DeviceGraph(dev, g)
     tuned = None
-    if not args.no_tune:
+    if args.shape:
+        tuned = apply_shape(dev, args.shape)
+    elif not args.no_tune:
         tuned, _ = autotune.tune(dg, qs, f, o, 0, sample=len(qs), trial_limit_s=max(1.0, args.tune_limit),
                                  restore_limit_s=args.time_limit, log=log)
         log(f"autotune picked {tuned}")
@@ -450,7 +464,9 @@ def serving_workload(args):
     dev = Device(0)
     dg = DeviceGraph(dev, g)
     tuned = None
-    if not args.no_tune:
+    if args.shape:
+        tuned = apply_shape(dev, args.shape)
+    elif not args.no_tune:
         tuned, _ = autotune.tune(dg, qs, "GQL", "GQL", args.cap, sample=min(len(qs), args.tune_sample),
                                  trial_limit_s=args.tune_limit, restore_limit_s=args.time_limit, log=log)
         log(f"autotune picked {tuned}")

@@ -7,7 +7,7 @@
 // It restates, in code written for this project, the algorithms of
 // scorpiontornado/PSM_Benchmark that sit on the accelerated path. Each
 // function names the reference lines it follows. The restatement is pinned
-// (see tests/test_oracle_golden.py) by
+// (tests/test_oracle.py) by
 //   * the reference's recorded logs test_anony/C_output/HPRD/L15/Q10/*.txt
 //     (embedding counts, call counts and the BuildTable memory checksum for
 //     GQL_GQL_GQL and CFL_CFL_EXPLORE at one thread),
