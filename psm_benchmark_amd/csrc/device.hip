@@ -825,6 +825,8 @@ struct PrefixState {
     __device__ __forceinline__ uint32_t M(uint32_t e) const { return M_[e]; }
 };
 
+constexpr uint64_t kExpandChunk = 256;   // longest base range one expansion thread walks
+
 // One thread per task: enumerate the valid extensions at the task's depth.
 // Count pass (write = 0): out_count[t] = number of children (or 1 when the
 // task's query is not being expanded). Write pass: the children (or the task
@@ -865,8 +867,28 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         b.hi = task.hi;
         b.sel = task.sel;
     }
-    uint32_t c = 0;
     uint64_t pos = write ? out_pos[t] : 0;
+    if (b.hi - b.lo > kExpandChunk) {
+        // a long base range (a hub's row) is not walked by one thread: it is
+        // cut into range tasks of at most kExpandChunk positions at the same
+        // depth (no assignment is made, so no calls), expanded next round
+        const uint64_t pieces = (b.hi - b.lo + kExpandChunk - 1) / kExpandChunk;
+        if (write) {
+            for (uint64_t i = 0; i < pieces; ++i) {
+                DTask ch = task;
+                ch.sel = b.sel;
+                ch.lo = b.lo + i * kExpandChunk;
+                ch.hi = min(b.hi, ch.lo + kExpandChunk);
+                out_tasks[pos++] = ch;
+            }
+            atomicAdd(&qc[task.q].expand_bytes, bytes);
+        } else {
+            out_count[t] = (uint32_t)pieces;
+            atomicAdd(qchild + task.q, (unsigned long long)pieces);
+        }
+        return;
+    }
+    uint32_t c = 0;
     for (uint64_t cur = b.lo; cur < b.hi; ++cur) {
         const uint32_t x = b.sel == kIdentity ? (uint32_t)cur : idx[cur];
         uint32_t v;
