@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+PLAIN_SHAPE = "lane,8,2"   # the library's default launch shape (psm_ctx)
 
 
 def log(*a):
@@ -242,7 +243,10 @@ def main():
     emb = int(counts.sum())
     step_ms = total / args.steps * 1e3
 
-    # plain path: one query at a time through the same library
+    # plain path: one query at a time through the same library, with the
+    # library's default launch shape (what a caller gets without tuning; the
+    # batched shape expands every query to a chip-wide task count)
+    apply_shape(dev, PLAIN_SHAPE)
     log(f"plain path over {min(args.plain, nq)} queries")
     lat = []
     plain_counts, plain_timeouts = [], []
@@ -309,6 +313,7 @@ def main():
                        "plain_p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "plain_p99": round(float(np.percentile(lat_ms, 99)), 3)},
         "plain_gpu_queries_per_s": round(plain_qps, 2),
+        "plain_launch_shape": PLAIN_SHAPE,
         "plain_timeouts": int(sum(plain_timeouts)),
         "batched_over_plain": round(nq * args.steps / total / plain_qps, 2),
         "stage_ms": {k: round(float(np.mean([getattr(s, k) for s in stats])), 2)
@@ -353,9 +358,11 @@ def large_workload(args):
     12-16 vertices, CFL_CFL_EXPLORE, plus the two-unit join) and C5 (R-MAT
     2^24 V / 100M E, 8 uniform labels, 32 dense queries of 6-10 vertices with
     triangles and 4-cliques, GQL_GQL_GQL): uncapped counts under a time limit.
-    The batched form runs all queries in one pipeline; the plain path and the
-    join run query by query within a time budget and must agree with it on
-    every query that none of them stopped early."""
+    The batched form runs all queries in one pipeline; the plain path runs
+    query by query within a time budget and must agree with it on every
+    query that neither stopped early. C3 also compares the uncapped plain
+    count with the two-unit device join on 16 join-checkable queries of the
+    same graph (cyclic walks over the rarest labels, finite counts)."""
     from psm_benchmark_amd import autotune
     from psm_benchmark_amd import workloads as W
     from psm_benchmark_amd import _lib as L
@@ -391,7 +398,9 @@ def large_workload(args):
     ks = kernel_stats(dev)
     total = sum(times)
     done = status != L.PSM_ETIMEOUT
-    # plain path (and the join for C3), query by query within the budget
+    # plain path (and the join for C3), query by query within the budget,
+    # with the library's default launch shape
+    apply_shape(dev, PLAIN_SHAPE)
     L.lib().psm_set_time_limit(dev.h, args.plain_limit)
     plain, joins, lat = [], [], []
     spent = 0.0
@@ -411,6 +420,23 @@ def large_workload(args):
             except L.PSMError as e:
                 joins.append((i, None, str(e)))
             spent += time.perf_counter() - t
+    if name == "C3":
+        # join-checkable queries (rare labels): uncapped plain count vs the
+        # two-unit device join, on the same resident C3 graph
+        jq, _ = W.c3_join_queries(g)
+        for i, q in enumerate(jq):
+            r = match(dg, q, method)
+            if r.timed_out:
+                joins.append((f"j{i}", None, "plain timed out"))
+                continue
+            t = time.perf_counter()
+            try:
+                js = join_match(dg, q, method)
+                joins.append((f"j{i}", int(js.count), js.count == r.count))
+                log(f"join query j{i}: n={q.num_vertices} plain {r.count} join {js.count} "
+                    f"rows {js.rows_a}/{js.rows_b} ({(time.perf_counter() - t) * 1e3:.1f} ms)")
+            except L.PSMError as e:
+                joins.append((f"j{i}", None, str(e)))
     mismatch = [i for i, c, to in plain if not to and done[i] and c != counts[i]]
     if mismatch:
         raise SystemExit(f"batched and plain counts differ for queries {mismatch[:10]}")
@@ -450,7 +476,8 @@ def large_workload(args):
     }
     if name == "C3":
         out["join"] = {"queries": len(joins), "equal_to_plain": int(sum(1 for j in joins if j[2] is True)),
-                       "failed": [j[0] for j in joins if isinstance(j[2], str)]}
+                       "embeddings_checked": int(sum(j[1] for j in joins if j[2] is True)),
+                       "not_compared": {str(j[0]): j[2] for j in joins if isinstance(j[2], str)}}
     print(json.dumps(out))
 
 
@@ -481,7 +508,9 @@ def serving_workload(args):
                                  trial_limit_s=args.tune_limit, restore_limit_s=args.time_limit, log=log)
         log(f"autotune picked {tuned}")
     L.lib().psm_set_time_limit(dev.h, args.time_limit)
-    # plain path: one query at a time
+    # plain path: one query at a time, with the library's default launch shape
+    served_shape = autotune.get_config(dev)
+    apply_shape(dev, PLAIN_SHAPE)
     lat, plain_counts, plain_to = [], [], []
     for q in qs[: args.plain]:
         t = time.perf_counter()
@@ -490,10 +519,11 @@ def serving_workload(args):
         plain_counts.append(r.count)
         plain_to.append(r.timed_out)
     lat_ms = np.array(lat) * 1e3
-    plain = {"queries": len(lat), "queries_per_s": round(len(lat) / sum(lat), 2),
+    plain = {"queries": len(lat), "queries_per_s": round(len(lat) / sum(lat), 2), "launch_shape": PLAIN_SHAPE,
              "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
              "p99_ms": round(float(np.percentile(lat_ms, 99)), 3), "timeouts": int(sum(plain_to))}
     log(f"plain: {plain}")
+    autotune.set_config(dev, served_shape)   # server workers inherit the served shape
     rates = []
     # the three offered rates of the workload, and one below the server's
     # capacity so that unloaded latency shows beside them

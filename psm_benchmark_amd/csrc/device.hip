@@ -38,6 +38,21 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, unsigned long lo
     if (lane_id() == 0 && v) atomicAdd(p, v);
 }
 
+// Sum of v over the block, added once to *p. Every thread of the block must
+// call it (it synchronises the block).
+__device__ __forceinline__ void block_add(unsigned long long* p, unsigned long long v) {
+    __shared__ unsigned long long acc;
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, o), hi = __shfl_xor((uint32_t)(v >> 32), o);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    if (lane_id() == 0 && v) atomicAdd(&acc, v);
+    __syncthreads();
+    if (threadIdx.x == 0 && acc) atomicAdd(p, acc);
+}
+
 // Largest i in [0, n) with starts[i] <= key (starts ascending, starts[0] == 0).
 template <typename T, typename K>
 __device__ __forceinline__ uint32_t find_item(const T* items, uint32_t n, K key, K T::*field) {
@@ -222,16 +237,16 @@ __device__ __forceinline__ uint32_t left_mask(const unsigned long long* __restri
     return m;
 }
 
-__global__ __launch_bounds__(kBlock) void k_gql_refine(
-    DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_waves,
+// One candidate per wavefront; returns its algorithmic bytes (on every lane).
+__device__ __forceinline__ unsigned long long gql_refine_one(
+    const DevGraph& g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_waves,
     const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
     uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
-    unsigned long long* __restrict__ bytes, const uint2* __restrict__ list) {
-    __shared__ GqlShared sh;
+    const uint2* __restrict__ list, GqlShared& sh) {
     const uint32_t wl = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
     const uint32_t gw = blockIdx.x * kGqlWavesPerBlock + wl;
-    if (gw >= n_waves) return;
+    if (gw >= n_waves) return 0;
     uint32_t ii, j;
     if (list) {
         // candidates deferred by the 8-lane kernel: (item, index) pairs
@@ -243,9 +258,9 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
         j = gw - items[ii].wave_start;
     }
     const GItem it = items[ii];
-    if (j >= it.len) return;
+    if (j >= it.len) return 0;
     const uint32_t v = cand_pool[it.cand_begin + j];
-    if (v == kInvalid) return;
+    if (v == kInvalid) return 0;
     const uint32_t k = it.k;
     const uint32_t* lslots = left_slots + it.left_begin;   // uniform: scalar loads
     const uint32_t vb = g.off[v], ve = g.off[v + 1];
@@ -328,12 +343,20 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
         atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
     }
     // algorithmic bytes: candidate, offsets, the scanned part of N(v), one
-    // bitmap word per (neighbour, left vertex) probe; pass 2 rescans all of
-    // N(v) when it runs
-    if (lane == 0) {
-        const unsigned long long scan = 4ull * scanned * (1 + k);
-        atomicAdd(bytes, 12ull + scan + (pass2 ? 4ull * (ve - vb) * (1 + k) : 0ull));
-    }
+    // membership word per (neighbour, left vertex) probe; pass 2 rescans all
+    // of N(v) when it runs
+    return 12ull + 4ull * scanned * (1 + k) + (pass2 ? 4ull * (ve - vb) * (1 + k) : 0ull);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gql_refine(
+    DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_waves,
+    const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
+    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
+    unsigned long long* __restrict__ bytes, const uint2* __restrict__ list) {
+    __shared__ GqlShared sh;
+    const unsigned long long by =
+        gql_refine_one(g, items, n_items, n_waves, left_slots, cand_pool, bitmaps, wps, member, list, sh);
+    block_add(bytes, lane_id() == 0 ? by : 0ull);
 }
 
 
@@ -411,8 +434,10 @@ __global__ __launch_bounds__(kBlock) void k_gql_hall(
             atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
             atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
         }
-        atomicAdd(bytes, 12ull + 4ull * deg * (1 + k));
     }
+    // algorithmic bytes: candidate, offsets, N(v), one membership word per
+    // neighbour and left vertex; summed per block (one global atomic)
+    block_add(bytes, live && lig == 0 ? 12ull + 4ull * deg * (1 + k) : 0ull);
 }
 
 __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
@@ -484,8 +509,10 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
             atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
             atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
         }
-        atomicAdd(bytes, 12ull + 4ull * deg * (1 + k));
     }
+    // algorithmic bytes: candidate, offsets, N(v), one membership word per
+    // neighbour and left vertex; summed per block (one global atomic)
+    block_add(bytes, live && lig == 0 ? 12ull + 4ull * deg * (1 + k) : 0ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -1082,6 +1109,8 @@ __device__ __forceinline__ bool past_deadline(DQueue* Q) {
                __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+constexpr uint32_t kLeafBurst = 32;   // leaf-depth elements one step of the lane DFS tests
+
 // Persistent DFS. Every lane runs its own depth-first search over one task at
 // a time. Idle lanes claim the next queued tasks (one claim per wavefront).
 // A running lane splits off the second half of the remaining range at its
@@ -1117,6 +1146,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     uint32_t wslot = 0, backoff = 0;
     uint32_t q = 0, tdepth = 0, d = 0, n = 0;
     Level L{};
+    Level Lp{};   // the level above the leaf depth, kept while the lane is at the leaf
     unsigned long long lcount = 0, lcalls = 0, lbytes = 0;
     unsigned long long busy_iters = 0, all_iters = 0;
     uint32_t steps = 0;
@@ -1198,7 +1228,25 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
 
         // one step of this lane's search
         bool finished = false;
-        if (L.cur < L.end) {
+        if (L.cur < L.end && d + 1 == n && !sink.rows) {
+            // leaf depth: test up to kLeafBurst base elements in one step and
+            // count the valid ones (each is one assignment); the rest of the
+            // range stays splittable by the donation below
+            uint32_t burst = kLeafBurst;
+            while (L.cur < L.end && burst--) {
+                const uint32_t x = L.sel == kIdentity ? (uint32_t)L.cur : idx[L.cur];
+                lbytes += L.sel == kIdentity ? 0 : 4;
+                ++L.cur;
+                uint32_t v;
+                const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes);
+                if (r == 2) {
+                    L.cur = L.end;
+                } else if (r == 1) {
+                    ++lcalls;
+                    ++lcount;
+                }
+            }
+        } else if (L.cur < L.end) {
             const uint32_t x = L.sel == kIdentity ? (uint32_t)L.cur : idx[L.cur];
             lbytes += L.sel == kIdentity ? 0 : 4;
             ++L.cur;
@@ -1229,6 +1277,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     st.I_[a] = x;
                     st.M_[a] = v;
                     ++d;
+                    if (d + 1 == n) Lp = L;   // restored on the way back up
                     enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes);
                 }
             }
@@ -1238,10 +1287,17 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         } else {
             --d;
             const uint32_t a = st.at(d);
-            const uint32_t sel = st.sel_[a] == kSelIdentity ? kIdentity : st.sel_[a];
-            enter_level(*P, ops, row_off, d, st, sel, L, lbytes);
-            L.cur = L.base0 + st.cur_[a];
-            L.end = L.base0 + st.end_[a];
+            if (d + 2 == n) {
+                // back from the leaf depth: the level's cursors are still in
+                // registers; only its end may have moved (donation)
+                L = Lp;
+                L.end = L.base0 + st.end_[a];
+            } else {
+                const uint32_t sel = st.sel_[a] == kSelIdentity ? kIdentity : st.sel_[a];
+                enter_level(*P, ops, row_off, d, st, sel, L, lbytes);
+                L.cur = L.base0 + st.cur_[a];
+                L.end = L.base0 + st.end_[a];
+            }
         }
         ++steps;
         if (active && (steps & 255) == 0) {

@@ -120,3 +120,56 @@ def c5_queries(g: CSRGraph, count: int = 32, seed: int = 55):
         qs.append(q)
         meta.append(dict(n=n, edges=q.num_edges, k4=clique_size(q, 4)))
     return qs, meta
+
+
+def c3_join_queries(g: CSRGraph, count: int = 16, seed: int = 35, rare_labels: int = 16,
+                    sizes=(6, 9)):
+    """Join-checkable queries on the C3 graph: `sizes` (inclusive) vertices
+    with a cycle, the vertices of a random walk over data vertices of the
+    `rare_labels` least frequent labels, with every data edge among them.
+    Chosen so that their counts can be enumerated in full and the two-unit
+    join compared with the plain count at C3 scale (12-16-vertex queries of
+    this graph have more than 1e10 embeddings each, even on the rare
+    labels)."""
+    freq = np.bincount(g.labels, minlength=int(g.labels.max()) + 1)
+    allowed = np.zeros(freq.shape[0], bool)
+    allowed[np.argsort(freq, kind="stable")[:rare_labels]] = True
+    ok = allowed[g.labels]
+    off = g.offsets.astype(np.int64)
+    rng = np.random.default_rng(seed)
+    starts = np.flatnonzero(ok)
+    qs, meta = [], []
+    tries = 0
+    while len(qs) < count and tries < 100 * count:
+        tries += 1
+        n = int(rng.integers(sizes[0], sizes[1] + 1))
+        verts = [int(rng.choice(starts))]
+        cur = verts[0]
+        for _ in range(50 * n):
+            if len(verts) >= n:
+                break
+            nb = g.neighbors[off[cur]:off[cur + 1]]
+            nb = nb[ok[nb]]
+            if nb.size == 0:
+                cur = verts[int(rng.integers(0, len(verts)))]
+                continue
+            w = int(nb[int(rng.integers(0, nb.size))])
+            if w not in verts:
+                verts.append(w)
+            cur = w if rng.random() < 0.85 else verts[int(rng.integers(0, len(verts)))]
+        if len(verts) < n:
+            continue
+        pos = {v: i for i, v in enumerate(verts)}
+        edges = []
+        for i, v in enumerate(verts):
+            for w in g.neighbors[off[v]:off[v + 1]]:
+                j = pos.get(int(w))
+                if j is not None and i < j:
+                    edges.append((i, j))
+        from .graph_io import csr_from_edges
+        q = csr_from_edges(n, np.array(edges, np.int64).reshape(-1, 2), g.labels[np.array(verts)])
+        if not has_cycle(q):
+            continue
+        qs.append(q)
+        meta.append(dict(n=n, edges=q.num_edges))
+    return qs, meta
