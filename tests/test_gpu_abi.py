@@ -1,0 +1,62 @@
+"""Error behaviour and state of the C ABI entry points that steer the device
+path (launch shape, staged batches), through libpsm.so on the GPU."""
+import ctypes
+import os
+
+import pytest
+
+from psm_benchmark_amd.graph_io import load_text
+
+pytestmark = pytest.mark.gpu
+
+HPRD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hprd")
+
+
+@pytest.fixture(scope="module")
+def dg():
+    from psm_benchmark_amd.matcher import Device, DeviceGraph
+    dev = Device(0)
+    g = DeviceGraph(dev, load_text(os.path.join(HPRD, "HPRD-15.txt")))
+    yield g
+    g.close()
+    dev.close()
+
+
+def test_tuning_round_trip_and_rejection(dg):
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd import autotune
+    h = dg.dev.h
+    before = autotune.get_config(dg.dev)
+    assert before["dfs_strategy"] in ("lane", "wave")
+    cfg = {"dfs_strategy": "wave", "dfs_waves_per_cu": 6, "expand_tasks_per_lane": 3.0}
+    autotune.set_config(dg.dev, cfg)
+    assert autotune.get_config(dg.dev) == {**cfg, "dfs_waves_per_cu": 6.0}
+    for key, bad in ((L.PSM_TUNE_DFS_WAVES_PER_CU, 0), (L.PSM_TUNE_DFS_WAVES_PER_CU, 33),
+                     (L.PSM_TUNE_EXPAND_TASKS_PER_LANE, 0.0), (L.PSM_TUNE_DFS_STRATEGY, 7), (99, 1)):
+        assert L.lib().psm_set_tuning(h, key, float(bad)) == L.PSM_EINVAL
+    v = ctypes.c_double()
+    assert L.lib().psm_get_tuning(h, 99, ctypes.byref(v)) == L.PSM_EINVAL
+    autotune.set_config(dg.dev, before)
+
+
+def test_staged_batch_state_and_repeat(dg):
+    """evaluate before prepare is a state error; a prepared batch evaluates
+    twice with the same results; the counts equal the reference's logs."""
+    import json
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.matcher import Batch
+    recs = [r for r in json.load(open(os.path.join(HPRD, "golden.json")))
+            if r["threads"] == 1 and r["filter"] == "GQL" and r["engine"] == "GQL"][:4]
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
+    b = Batch(dg, qs)
+    st = L.BatchStats()
+    import numpy as np
+    c = np.zeros(len(qs), np.uint64)
+    r = L.lib().psm_batch_evaluate(b.h, 0, -1.0, L.u64p(c), None, None, ctypes.byref(st))
+    assert r == L.PSM_ESTATE
+    b.prepare("GQL", "GQL")
+    c1, k1, s1, _ = b.evaluate(0, 0.0)
+    c2, k2, s2, _ = b.evaluate(0, 0.0)
+    b.close()
+    assert c1.tolist() == c2.tolist() == [r["embeddings"] for r in recs]
+    assert k1.tolist() == k2.tolist() == [r["calls"] for r in recs]
