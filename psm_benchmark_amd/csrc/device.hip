@@ -237,27 +237,13 @@ __device__ __forceinline__ uint32_t left_mask(const unsigned long long* __restri
     return m;
 }
 
-// One candidate per wavefront; returns its algorithmic bytes (on every lane).
+// One candidate (item it, index j) per wavefront; returns its algorithmic
+// bytes (on every lane). wl: the wavefront's index in its block.
 __device__ __forceinline__ unsigned long long gql_refine_one(
-    const DevGraph& g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_waves,
-    const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
-    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
-    const uint2* __restrict__ list, GqlShared& sh) {
-    const uint32_t wl = threadIdx.x / kWave;
+    const DevGraph& g, const GItem& it, uint32_t j, const uint32_t* __restrict__ left_slots,
+    uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
+    unsigned long long* __restrict__ member, GqlShared& sh, uint32_t wl) {
     const uint32_t lane = lane_id();
-    const uint32_t gw = blockIdx.x * kGqlWavesPerBlock + wl;
-    if (gw >= n_waves) return 0;
-    uint32_t ii, j;
-    if (list) {
-        // candidates deferred by the 8-lane kernel: (item, index) pairs
-        const uint2 e = list[gw];
-        ii = e.x;
-        j = e.y;
-    } else {
-        ii = find_item(items, n_items, gw, &GItem::wave_start);
-        j = gw - items[ii].wave_start;
-    }
-    const GItem it = items[ii];
     if (j >= it.len) return 0;
     const uint32_t v = cand_pool[it.cand_begin + j];
     if (v == kInvalid) return 0;
@@ -348,14 +334,22 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
     return 12ull + 4ull * scanned * (1 + k) + (pass2 ? 4ull * (ve - vb) * (1 + k) : 0ull);
 }
 
+// The wavefront test over the deferred (item, index) pairs of class 3; the
+// pair count is read from device memory (written by the 8-lane kernel), and
+// a fixed grid strides over the list.
 __global__ __launch_bounds__(kBlock) void k_gql_refine(
-    DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_waves,
-    const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
-    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
-    unsigned long long* __restrict__ bytes, const uint2* __restrict__ list) {
+    DevGraph g, const GItem* __restrict__ items, const uint2* __restrict__ list,
+    const uint32_t* __restrict__ n_list, const uint32_t* __restrict__ left_slots,
+    uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
+    unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
     __shared__ GqlShared sh;
-    const unsigned long long by =
-        gql_refine_one(g, items, n_items, n_waves, left_slots, cand_pool, bitmaps, wps, member, list, sh);
+    const uint32_t n = *n_list;
+    const uint32_t wl = threadIdx.x / kWave;
+    unsigned long long by = 0;
+    for (uint32_t w = blockIdx.x * kGqlWavesPerBlock + wl; w < n; w += gridDim.x * kGqlWavesPerBlock) {
+        const uint2 e = list[w];
+        by += gql_refine_one(g, items[e.x], e.y, left_slots, cand_pool, bitmaps, wps, member, sh, wl);
+    }
     block_add(bytes, lane_id() == 0 ? by : 0ull);
 }
 
@@ -384,35 +378,13 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long bal,
     else return (bal >> gbase) & ((1ull << G) - 1ull);
 }
 
-// The Hall test of the 8-lane kernel with G lanes per candidate, over the
-// (item, index) pairs the 8-lane kernel deferred (k <= 8, deg(v) <= G).
+// Hall's condition for one candidate tested by a group of G lanes: lane lig
+// holds right vertex lig's k-bit left mask m (0 past deg); R[i] = the group's
+// lanes adjacent to left vertex i; the 2^k - 1 left subsets are split over
+// the group. True when some subset S has fewer than |S| right neighbours
+// (no matching saturates the left side). Every lane of the wavefront calls it.
 template <uint32_t G>
-__global__ __launch_bounds__(kBlock) void k_gql_hall(
-    DevGraph g, const GItem* __restrict__ items, const uint2* __restrict__ list, uint32_t n,
-    const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
-    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
-    unsigned long long* __restrict__ bytes) {
-    const uint32_t lane = lane_id();
-    const uint32_t lig = lane & (G - 1);
-    const uint32_t gbase = lane & ~(G - 1);
-    const uint32_t gid = (uint32_t)(((uint64_t)blockIdx.x * kBlock + threadIdx.x) / G);
-    bool live = gid < n;
-    uint32_t j = 0, v = kInvalid, k = 0, deg = 0, vb = 0;
-    GItem it{};
-    if (live) {
-        const uint2 e = list[gid];
-        it = items[e.x];
-        j = e.y;
-        v = cand_pool[it.cand_begin + j];
-        live = v != kInvalid;
-    }
-    if (live) {
-        vb = g.off[v];
-        deg = min(g.off[v + 1] - vb, G);   // deg <= G by construction of the list
-        k = min(it.k, kGroup);
-    }
-    uint32_t m = 0;
-    if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj[vb + lig]);
+__device__ __forceinline__ bool hall_fails(uint32_t m, bool live, uint32_t k, uint32_t lig, uint32_t gbase) {
     unsigned long long R[kGroup];
 #pragma unroll
     for (uint32_t i = 0; i < kGroup; ++i) R[i] = group_bits<G>(__ballot((m >> i) & 1u), gbase);
@@ -427,92 +399,159 @@ __global__ __launch_bounds__(kBlock) void k_gql_hall(
             fail = __popcll(uni) < __popc(sub);
         }
     }
-    const bool group_fail = group_bits<G>(__ballot(fail), gbase) != 0;
-    if (live && lig == 0) {
-        if (group_fail) {
-            cand_pool[it.cand_begin + j] = kInvalid;
-            atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
-            atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
+    return group_bits<G>(__ballot(fail), gbase) != 0;
+}
+
+// Removes candidate j of item it (value v): pool slot, bitmap bit, membership bit.
+__device__ __forceinline__ void gql_drop(const GItem& it, uint32_t j, uint32_t v, uint32_t* __restrict__ cand_pool,
+                                         uint32_t* __restrict__ bitmaps, uint64_t wps,
+                                         unsigned long long* __restrict__ member) {
+    cand_pool[it.cand_begin + j] = kInvalid;
+    atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
+    atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
+}
+
+// The Hall test with G lanes per candidate over the (item, index) pairs the
+// 8-lane kernel deferred to this class (k <= 8, deg(v) <= G). The pair count
+// is read from device memory; a fixed grid strides over the list, so the
+// launch needs no count from the host.
+template <uint32_t G>
+__global__ __launch_bounds__(kBlock) void k_gql_hall(
+    DevGraph g, const GItem* __restrict__ items, const uint2* __restrict__ list,
+    const uint32_t* __restrict__ n_list, const uint32_t* __restrict__ left_slots,
+    uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
+    unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
+    constexpr uint32_t kPerIter = kBlock / G;
+    const uint32_t n = *n_list;
+    const uint32_t lane = lane_id();
+    const uint32_t lig = lane & (G - 1);
+    const uint32_t gbase = lane & ~(G - 1);
+    unsigned long long by = 0;
+    // block-uniform trip count: every lane takes part in the ballots
+    for (uint64_t first = (uint64_t)blockIdx.x * kPerIter; first < n; first += (uint64_t)gridDim.x * kPerIter) {
+        const uint64_t gid = first + threadIdx.x / G;
+        bool live = gid < n;
+        uint32_t j = 0, v = kInvalid, k = 0, deg = 0, vb = 0;
+        GItem it{};
+        if (live) {
+            const uint2 e = list[gid];
+            it = items[e.x];
+            j = e.y;
+            v = cand_pool[it.cand_begin + j];
+            live = v != kInvalid;
+        }
+        if (live) {
+            vb = g.off[v];
+            deg = min(g.off[v + 1] - vb, G);   // deg <= G by construction of the list
+            k = min(it.k, kGroup);
+        }
+        uint32_t m = 0;
+        if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj[vb + lig]);
+        const bool fail = hall_fails<G>(m, live, k, lig, gbase);
+        if (live && lig == 0) {
+            if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member);
+            // algorithmic bytes: candidate, offsets, N(v), one membership
+            // word per neighbour and left vertex
+            by += 12ull + 4ull * deg * (1 + k);
         }
     }
-    // algorithmic bytes: candidate, offsets, N(v), one membership word per
-    // neighbour and left vertex; summed per block (one global atomic)
-    block_add(bytes, live && lig == 0 ? 12ull + 4ull * deg * (1 + k) : 0ull);
+    block_add(bytes, by);
+}
+
+// Deferred pairs are staged per block in LDS and appended to the global class
+// lists with one atomic per class per flush (a per-wavefront atomic on the
+// four list counters serialises the whole launch on four addresses).
+constexpr uint32_t kDeferBuf = 512;
+
+struct DeferStage {
+    uint2 buf[kDeferClasses][kDeferBuf];
+    uint32_t count[kDeferClasses];
+    uint32_t base[kDeferClasses];
+};
+
+// Appends the staged pairs of every class holding more than `threshold` to
+// lists + c * stride. Every thread of the block calls it after a barrier.
+__device__ __forceinline__ void defer_flush(DeferStage& st, uint2* __restrict__ lists, uint64_t stride,
+                                            uint32_t* __restrict__ n_lists, uint32_t threshold) {
+    if (threadIdx.x < kDeferClasses) {
+        const uint32_t c = threadIdx.x, m = st.count[c];
+        st.base[c] = (m > threshold && m > 0) ? atomicAdd(n_lists + c, m) : kInvalid;
+    }
+    __syncthreads();
+    for (uint32_t c = 0; c < kDeferClasses; ++c) {
+        const uint32_t b = st.base[c];
+        if (b == kInvalid) continue;
+        const uint32_t m = st.count[c];
+        for (uint32_t i = threadIdx.x; i < m; i += kBlock) lists[(uint64_t)c * stride + b + i] = st.buf[c][i];
+    }
+    __syncthreads();
+    if (threadIdx.x < kDeferClasses && st.base[threadIdx.x] != kInvalid) st.count[threadIdx.x] = 0;
+    __syncthreads();
 }
 
 __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
     DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_cands,
     const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
     uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
-    uint2* __restrict__ lists, uint32_t* __restrict__ n_lists, unsigned long long* __restrict__ bytes) {
+    uint2* __restrict__ lists, uint64_t list_stride, uint32_t* __restrict__ n_lists,
+    unsigned long long* __restrict__ bytes) {
+    constexpr uint32_t kPerIter = kBlock / kGroup;   // candidates per block iteration
+    __shared__ DeferStage st;
     const uint32_t lane = lane_id();
     const uint32_t lig = lane & (kGroup - 1);
     const uint32_t gbase = lane & ~(kGroup - 1);
-    const uint32_t gid = (blockIdx.x * kBlock + threadIdx.x) / kGroup;
-    bool live = gid < n_cands;
-    uint32_t ii = 0, j = 0, v = kInvalid, k = 0, deg = 0, vb = 0;
-    GItem it{};
-    if (live) {
-        ii = find_item(items, n_items, gid, &GItem::wave_start);
-        it = items[ii];
-        j = gid - it.wave_start;
-        live = j < it.len;
-    }
-    if (live) {
-        v = cand_pool[it.cand_begin + j];
-        live = v != kInvalid;
-    }
-    if (live) {
-        vb = g.off[v];
-        deg = g.off[v + 1] - vb;
-        k = it.k;
-    }
-    // defer the candidates outside the limits, one atomic per class per wavefront
-    const bool defer = live && lig == 0 && (deg > kGroup || k > kGroup);
-    const uint32_t dc = defer ? defer_class(deg, k) : kDeferClasses;
-#pragma unroll
-    for (uint32_t c = 0; c < kDeferClasses; ++c) {
-        const unsigned long long bal = __ballot(dc == c);
-        if (!bal) continue;
-        const uint32_t leader = __ffsll((long long)bal) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(n_lists + c, (uint32_t)__popcll(bal));
-        base = __shfl(base, leader);
-        if (dc == c) lists[(uint64_t)c * n_cands + base + __popcll(bal & lanemask_lt())] = make_uint2(ii, j);
-    }
-    if (live && (deg > kGroup || k > kGroup)) live = false;
-    uint32_t m = 0;
-    if (live && lig < deg)
-        m = left_mask(member, it, left_slots + it.left_begin, k, g.adj[vb + lig]);
-    // R[i]: group lanes (right vertices) adjacent to left vertex i
-    uint32_t R[kGroup];
-#pragma unroll
-    for (uint32_t i = 0; i < kGroup; ++i) {
-        const unsigned long long bal = __ballot((m >> i) & 1u);
-        R[i] = (uint32_t)(bal >> gbase) & 0xFFu;
-    }
-    bool fail = false;
-    if (live) {
-        const uint32_t full = (1u << k) - 1u;
-        for (uint32_t sub = lig + 1; sub <= full && !fail; sub += kGroup) {
-            uint32_t uni = 0;
-#pragma unroll
-            for (uint32_t i = 0; i < kGroup; ++i)
-                if ((sub >> i) & 1u) uni |= R[i];
-            fail = __popc(uni) < __popc(sub);
+    if (threadIdx.x < kDeferClasses) st.count[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long by = 0;
+    // block-uniform trip count: barriers and ballots see every lane
+    for (uint64_t first = (uint64_t)blockIdx.x * kPerIter; first < n_cands;
+         first += (uint64_t)gridDim.x * kPerIter) {
+        const uint64_t gid = first + threadIdx.x / kGroup;
+        bool live = gid < n_cands;
+        uint32_t ii = 0, j = 0, v = kInvalid, k = 0, deg = 0, vb = 0;
+        GItem it{};
+        if (live) {
+            ii = find_item(items, n_items, (uint32_t)gid, &GItem::wave_start);
+            it = items[ii];
+            j = (uint32_t)gid - it.wave_start;
+            live = j < it.len;
         }
-    }
-    const bool group_fail = ((__ballot(fail) >> gbase) & 0xFFull) != 0;
-    if (live && lig == 0) {
-        if (group_fail) {
-            cand_pool[it.cand_begin + j] = kInvalid;
-            atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
-            atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
+        if (live) {
+            v = cand_pool[it.cand_begin + j];
+            live = v != kInvalid;
         }
+        if (live) {
+            vb = g.off[v];
+            deg = g.off[v + 1] - vb;
+            k = it.k;
+        }
+        // defer the candidates outside the limits into the block's stage
+        const bool defer = live && lig == 0 && (deg > kGroup || k > kGroup);
+        const uint32_t dc = defer ? defer_class(deg, k) : kDeferClasses;
+#pragma unroll
+        for (uint32_t c = 0; c < kDeferClasses; ++c) {
+            const unsigned long long bal = __ballot(dc == c);
+            if (!bal) continue;
+            const uint32_t leader = __ffsll((long long)bal) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&st.count[c], (uint32_t)__popcll(bal));
+            base = __shfl(base, leader);
+            if (dc == c) st.buf[c][base + __popcll(bal & lanemask_lt())] = make_uint2(ii, j);
+        }
+        if (live && (deg > kGroup || k > kGroup)) live = false;
+        uint32_t m = 0;
+        if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj[vb + lig]);
+        const bool fail = hall_fails<kGroup>(m, live, k, lig, gbase);
+        if (live && lig == 0) {
+            if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member);
+            by += 12ull + 4ull * deg * (1 + k);
+        }
+        __syncthreads();
+        // at most kPerIter pairs per class join the stage per iteration
+        defer_flush(st, lists, list_stride, n_lists, kDeferBuf - kPerIter);
     }
-    // algorithmic bytes: candidate, offsets, N(v), one membership word per
-    // neighbour and left vertex; summed per block (one global atomic)
-    block_add(bytes, live && lig == 0 ? 12ull + 4ull * deg * (1 + k) : 0ull);
+    defer_flush(st, lists, list_stride, n_lists, 0);
+    block_add(bytes, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -2114,44 +2153,51 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
     return hipGetLastError();
 }
 
-hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
-                             uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
-                             uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
-                             unsigned long long* bytes, const uint2* list, hipStream_t s) {
-    if (n_waves == 0) return hipSuccess;
-    const uint32_t blocks = (n_waves + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock;
-    hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_waves,
-                       left_slots, cand_pool, bitmaps, words_per_slot, member, bytes, list);
+// Grids of the refinement kernels: fixed, strided over their work, so that
+// launches that read their counts from device memory need no host sync.
+constexpr uint32_t kGqlGrid = 2048;
+
+hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, const uint2* list, const uint32_t* n_list,
+                             uint32_t max_n, const uint32_t* left_slots, uint32_t* cand_pool, uint32_t* bitmaps,
+                             uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
+                             hipStream_t s) {
+    if (max_n == 0) return hipSuccess;
+    const uint32_t blocks =
+        std::min<uint32_t>(kGqlGrid / 2, (max_n + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock);
+    hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
+                       cand_pool, bitmaps, words_per_slot, member, bytes);
     return hipGetLastError();
 }
 
 hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32_t n_items,
                                    uint32_t n_cands, const uint32_t* left_slots, uint32_t* cand_pool,
                                    uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
-                                   uint2* lists, uint32_t* n_lists, unsigned long long* bytes, hipStream_t s) {
+                                   uint2* lists, uint64_t list_stride, uint32_t* n_lists,
+                                   unsigned long long* bytes, hipStream_t s) {
     if (n_cands == 0) return hipSuccess;
-    const uint64_t threads = (uint64_t)n_cands * kGroup;
-    const uint32_t blocks = (uint32_t)((threads + kBlock - 1) / kBlock);
+    if (list_stride < n_cands) return hipErrorInvalidValue;
+    const uint64_t want = ((uint64_t)n_cands * kGroup + kBlock - 1) / kBlock;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, kGqlGrid);
     hipLaunchKernelGGL(k_gql_refine_small, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_cands,
-                       left_slots, cand_pool, bitmaps, words_per_slot, member, lists, n_lists, bytes);
+                       left_slots, cand_pool, bitmaps, words_per_slot, member, lists, list_stride, n_lists, bytes);
     return hipGetLastError();
 }
 
 hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items, const uint2* list,
-                           uint32_t n, const uint32_t* left_slots, uint32_t* cand_pool, uint32_t* bitmaps,
-                           uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
-                           hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    const uint64_t threads = (uint64_t)n * lanes;
-    const uint32_t blocks = (uint32_t)((threads + kBlock - 1) / kBlock);
+                           const uint32_t* n_list, uint32_t max_n, const uint32_t* left_slots,
+                           uint32_t* cand_pool, uint32_t* bitmaps, uint64_t words_per_slot,
+                           unsigned long long* member, unsigned long long* bytes, hipStream_t s) {
+    if (max_n == 0) return hipSuccess;
+    const uint64_t want = ((uint64_t)max_n * lanes + kBlock - 1) / kBlock;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, kGqlGrid);
     if (lanes == 16)
-        hipLaunchKernelGGL(k_gql_hall<16>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n, left_slots,
+        hipLaunchKernelGGL(k_gql_hall<16>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
                            cand_pool, bitmaps, words_per_slot, member, bytes);
     else if (lanes == 32)
-        hipLaunchKernelGGL(k_gql_hall<32>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n, left_slots,
+        hipLaunchKernelGGL(k_gql_hall<32>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
                            cand_pool, bitmaps, words_per_slot, member, bytes);
     else if (lanes == 64)
-        hipLaunchKernelGGL(k_gql_hall<64>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n, left_slots,
+        hipLaunchKernelGGL(k_gql_hall<64>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
                            cand_pool, bitmaps, words_per_slot, member, bytes);
     else
         return hipErrorInvalidValue;

@@ -186,20 +186,24 @@ hipError_t launch_set_member(const uint32_t* pool, const uint64_t* seg_begin, co
                              const uint64_t* seg_mem, const uint32_t* seg_bit, uint32_t n_seg,
                              uint32_t total_max, unsigned long long* member, hipStream_t s);
 hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_words, hipStream_t s);
-hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, uint32_t n_items,
-                             uint32_t n_waves, const uint32_t* left_slots, uint32_t* cand_pool,
-                             uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
-                             unsigned long long* bytes, const uint2* list, hipStream_t s);
-// deferred candidates: class c (16, 32, 64 lanes, wavefront kernel) at
-// lists + c * n_cands, counts in n_lists[0..3]
+// The 8-lane kernel tests candidates with k <= 8 and deg(v) <= 8 and defers
+// the rest by class (16, 32, 64 lanes, wavefront kernel) to
+// lists + c * list_stride (list_stride >= n_cands), counts added to
+// n_lists[0..3]. The deferred-class kernels read their count from n_list;
+// max_n bounds it (sizes the grid).
 hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32_t n_items,
                                    uint32_t n_cands, const uint32_t* left_slots, uint32_t* cand_pool,
                                    uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
-                                   uint2* lists, uint32_t* n_lists, unsigned long long* bytes, hipStream_t s);
+                                   uint2* lists, uint64_t list_stride, uint32_t* n_lists,
+                                   unsigned long long* bytes, hipStream_t s);
 hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items, const uint2* list,
-                           uint32_t n, const uint32_t* left_slots, uint32_t* cand_pool, uint32_t* bitmaps,
-                           uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
-                           hipStream_t s);
+                           const uint32_t* n_list, uint32_t max_n, const uint32_t* left_slots,
+                           uint32_t* cand_pool, uint32_t* bitmaps, uint64_t words_per_slot,
+                           unsigned long long* member, unsigned long long* bytes, hipStream_t s);
+hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, const uint2* list, const uint32_t* n_list,
+                             uint32_t max_n, const uint32_t* left_slots, uint32_t* cand_pool, uint32_t* bitmaps,
+                             uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
+                             hipStream_t s);
 // thread per row; rows of candidates with more than 64 neighbours are listed
 // in long_rows (count in *n_long) for launch_table_long
 hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,

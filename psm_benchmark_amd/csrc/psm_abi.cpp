@@ -642,64 +642,78 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
                                 member.as<unsigned long long>(), s()));
         HIP_TRY(ctx, hipStreamSynchronize(s()));
     }
-    DBuf d_items, d_left, d_large, d_nlarge;
-    HIP_TRY(ctx, d_nlarge.alloc(16));
+    // work items of every query vertex position p (the candidates of vertex p
+    // of each query), uploaded once: both rounds use the same items
+    std::vector<GItem> items;
+    std::vector<uint32_t> left, item_begin(max_n + 1, 0), cands(max_n, 0);
+    uint32_t max_cands = 0;
+    for (uint32_t p = 0; p < max_n; ++p) {
+        item_begin[p] = (uint32_t)items.size();
+        uint32_t waves = 0;
+        for (size_t a = 0; a < active.size(); ++a) {
+            QueryHost* q = active[a];
+            if (p >= q->qg.n || q->cand_count[p] == 0) continue;
+            GItem it{};
+            it.cand_begin = q->cand_begin[p];
+            it.len = q->cand_count[p];
+            it.wave_start = waves;
+            it.bm_self = q->slot_base + p;
+            it.u = p;
+            it.mem_base = a * (uint64_t)g->V;
+            it.k = q->qg.degree(p);
+            it.left_begin = (uint32_t)left.size();
+            for (uint32_t i = q->qg.off[p]; i < q->qg.off[p + 1]; ++i) left.push_back(q->slot_base + q->qg.adj[i]);
+            items.push_back(it);
+            waves += it.len;
+        }
+        cands[p] = waves;
+        max_cands = std::max(max_cands, waves);
+    }
+    item_begin[max_n] = (uint32_t)items.size();
+    DBuf d_items, d_left, d_lists, d_nlists;
+    HIP_TRY(ctx, upload(d_items, items, s()));
+    HIP_TRY(ctx, upload(d_left, left, s()));
+    // deferred pairs of one (round, p): class c at c * max_cands; reused by
+    // every launch group (stream order)
+    HIP_TRY(ctx, d_lists.alloc(4ull * max_cands * sizeof(uint2)));
+    // deferral counters, four per (round, p), cleared by a fill kernel
+    const uint64_t n_counters = 2ull * max_n * 4;
+    HIP_TRY(ctx, d_nlists.alloc(n_counters * 4));
+    HIP_TRY(ctx, launch_fill_u32(d_nlists.as<uint32_t>(), 0u, n_counters, s()));
+    // Launch groups follow one another on the stream without host syncs: the
+    // deferred-class kernels read their counts from d_nlists.
     for (int round = 0; round < 2; ++round) {
         for (uint32_t p = 0; p < max_n; ++p) {
-            std::vector<GItem> items;
-            std::vector<uint32_t> left;
-            uint32_t waves = 0;
-            for (size_t a = 0; a < active.size(); ++a) {
-                QueryHost* q = active[a];
-                if (p >= q->qg.n || q->cand_count[p] == 0) continue;
-                GItem it{};
-                it.cand_begin = q->cand_begin[p];
-                it.len = q->cand_count[p];
-                it.wave_start = waves;
-                it.bm_self = q->slot_base + p;
-                it.u = p;
-                it.mem_base = a * (uint64_t)g->V;
-                it.k = q->qg.degree(p);
-                it.left_begin = (uint32_t)left.size();
-                for (uint32_t i = q->qg.off[p]; i < q->qg.off[p + 1]; ++i)
-                    left.push_back(q->slot_base + q->qg.adj[i]);
-                items.push_back(it);
-                waves += it.len;
-            }
-            if (items.empty()) continue;
-            HIP_TRY(ctx, upload(d_items, items, s()));
-            HIP_TRY(ctx, upload(d_left, left, s()));
-            const size_t list_bytes = 4ull * waves * sizeof(uint2);
-            if (d_large.bytes < list_bytes) HIP_TRY(ctx, d_large.alloc(list_bytes));
-            HIP_TRY(ctx, hipMemsetAsync(d_nlarge.p, 0, 16, s()));
-            // small candidates: 8 lanes each; the rest are deferred by degree
-            // class to 16 / 32 / 64-lane groups or the wavefront kernel
+            const uint32_t ni = item_begin[p + 1] - item_begin[p];
+            if (!ni) continue;
+            const GItem* its = d_items.as<GItem>() + item_begin[p];
+            uint32_t* cnt = d_nlists.as<uint32_t>() + ((uint64_t)round * max_n + p) * 4;
+            uint2* lists = d_lists.as<uint2>();
             TIMED(ctx, kKGqlRefine,
-                  launch_gql_refine_small(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), waves,
-                                          d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
-                                          bitmaps.as<uint32_t>(), wps, member.as<unsigned long long>(),
-                                          d_large.as<uint2>(), d_nlarge.as<uint32_t>(),
-                                          ctx->d_kbytes + kKGqlRefine, s()));
-            uint32_t n_def[4] = {0, 0, 0, 0};
-            HIP_TRY(ctx, hipMemcpyAsync(n_def, d_nlarge.p, 16, hipMemcpyDeviceToHost, s()));
-            HIP_TRY(ctx, hipStreamSynchronize(s()));
-            verbose("gql round %d u %u: %u candidates, deferred %u / %u / %u / %u", round, p, waves, n_def[0],
-                    n_def[1], n_def[2], n_def[3]);
+                  launch_gql_refine_small(g->dg, its, ni, cands[p], d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
+                                          bitmaps.as<uint32_t>(), wps, member.as<unsigned long long>(), lists,
+                                          max_cands, cnt, ctx->d_kbytes + kKGqlRefine, s()));
             const uint32_t lanes[3] = {16, 32, 64};
             for (int c = 0; c < 3; ++c)
-                if (n_def[c])
-                    TIMED(ctx, kKGqlRefine,
-                          launch_gql_hall(lanes[c], g->dg, d_items.as<GItem>(), d_large.as<uint2>() + (size_t)c * waves,
-                                          n_def[c], d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
-                                          bitmaps.as<uint32_t>(), wps, member.as<unsigned long long>(),
-                                          ctx->d_kbytes + kKGqlRefine, s()));
-            if (n_def[3])
                 TIMED(ctx, kKGqlRefine,
-                      launch_gql_refine(g->dg, d_items.as<GItem>(), (uint32_t)items.size(), n_def[3],
-                                        d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
-                                        bitmaps.as<uint32_t>(), wps, member.as<unsigned long long>(),
-                                        ctx->d_kbytes + kKGqlRefine, d_large.as<uint2>() + 3ull * waves, s()));
+                      launch_gql_hall(lanes[c], g->dg, its, lists + (uint64_t)c * max_cands, cnt + c, cands[p],
+                                      d_left.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
+                                      member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
+            TIMED(ctx, kKGqlRefine,
+                  launch_gql_refine(g->dg, its, lists + 3ull * max_cands, cnt + 3, cands[p], d_left.as<uint32_t>(),
+                                    cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
+                                    member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
         }
+    }
+    if (ctx->verbose) {
+        std::vector<uint32_t> nd;
+        HIP_TRY(ctx, download(nd, d_nlists.p, n_counters, s()));
+        for (int round = 0; round < 2; ++round)
+            for (uint32_t p = 0; p < max_n; ++p) {
+                const uint32_t* c = nd.data() + ((uint64_t)round * max_n + p) * 4;
+                verbose("gql round %d u %u: %u candidates, deferred %u / %u / %u / %u", round, p, cands[p], c[0],
+                        c[1], c[2], c[3]);
+            }
     }
     HIP_TRY(ctx, timers_flush(ctx));
     PSM_TRY(compact_pool());
