@@ -63,7 +63,13 @@ typedef struct psm_enum_stats {
     uint32_t tasks;           /* device tasks executed */
     double kernel_ms;         /* enumeration kernels */
     double expand_ms;         /* task pre-expansion kernels */
-    uint64_t probe_bytes;     /* algorithmic bytes read by the enumeration */
+    uint64_t probe_bytes;     /* bytes the enumeration kernels touched (base elements,
+                                 search probes, rows located) */
+    uint64_t algo_bytes;      /* algorithmic bytes of the search: for every search-tree
+                                 node below the root, 4 per entry of each operand row
+                                 (table row of an earlier query neighbour) plus 4 per
+                                 survivor written unless the survivors are leaves;
+                                 schedule-independent when the count is uncapped */
 } psm_enum_stats;
 
 /* ---- context -------------------------------------------------------- */
@@ -193,6 +199,7 @@ typedef struct psm_batch_stats {
     /* DFS lane utilisation: loop iterations in which a lane held a task, and
      * all loop iterations of all lanes */
     uint64_t dfs_lane_steps, dfs_lane_iters;
+    uint64_t algo_bytes;      /* as psm_enum_stats.algo_bytes, summed over the batch */
 } psm_batch_stats;
 
 int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t num_queries, const uint32_t* n,

@@ -719,7 +719,24 @@ enum Engine { kGQL = 0, kExplore = 1, kLFTJ = 2 };
 struct EnumResult {
     uint64_t count = 0, calls = 0;
     std::vector<uint64_t> depth_nodes;    // assignments per depth
+    uint64_t algo_bytes = 0;              // search bytes as defined below
 };
+
+// Algorithmic bytes of the search (the measure the device kernels count):
+// every search-tree node below the root reads, for each earlier query
+// neighbour b of the vertex it extends, the candidate-edge table row
+// T[b][u] at b's assigned candidate (4 bytes per entry), and writes its valid
+// extensions (4 bytes each) unless they are leaves. Independent of the engine:
+// the rows and the valid sets are fixed by the candidates, tables and order.
+uint64_t node_row_entries(const Tables& T, const std::vector<uint32_t>& earlier, uint32_t u,
+                          const std::vector<uint32_t>& eidx) {
+    uint64_t s = 0;
+    for (uint32_t b : earlier) {
+        const Table& t = T[b][u];
+        s += t.off[eidx[b] + 1] - t.off[eidx[b]];
+    }
+    return s;
+}
 
 void enumerate(const OGraph& d, const OGraph& q, const Cands& C, const Tables& T,
                const Plan& p, Engine eng, uint64_t cap, EnumResult& r) {
@@ -730,6 +747,17 @@ void enumerate(const OGraph& d, const OGraph& q, const Cands& C, const Tables& T
     std::vector<std::vector<uint32_t>> cand(n);   // vertex ids (GQL) or indices
     std::vector<uint32_t> pos(n, 0), emb(n, 0), eidx(n, 0), tmp;
     std::vector<uint8_t> visited(d.V, 0);
+    // every earlier neighbour of order[i] (the device's operand rows)
+    std::vector<std::vector<uint32_t>> earlier(n);
+    {
+        std::vector<uint8_t> seen(n, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t u = p.order[i];
+            for (uint32_t j = 0; j < q.deg(u); ++j)
+                if (seen[q.nbrs(u)[j]]) earlier[i].push_back(q.nbrs(u)[j]);
+            seen[u] = 1;
+        }
+    }
     uint32_t root = p.order[0];
     cand[0].resize(C[root].size());
     if (eng == kGQL) cand[0] = C[root];
@@ -784,16 +812,20 @@ void enumerate(const OGraph& d, const OGraph& q, const Cands& C, const Tables& T
             if (eng == kLFTJ && visited[v]) continue;
             r.depth_nodes[dep]++;
             emb[u] = v;
-            eidx[u] = x;
+            // candidate index of the assignment (the GQL engine walks vertex
+            // ids; C(u) is sorted)
+            eidx[u] = eng == kGQL ? (uint32_t)(std::lower_bound(C[u].begin(), C[u].end(), v) - C[u].begin()) : x;
             visited[v] = 1;
             if (dep == (int)n - 1) {
                 r.count++;
                 visited[v] = 0;
                 if (r.count >= cap) return;
             } else {
+                if (dep >= 1) r.algo_bytes += 4;   // a survivor written below the root level
                 ++dep;
                 pos[dep] = 0;
                 gen(dep);
+                r.algo_bytes += 4 * node_row_entries(T, earlier[dep], p.order[dep], eidx);
             }
         }
         --dep;
@@ -828,6 +860,7 @@ struct orc_result {
     uint32_t order[64];
     uint32_t pivot[64];
     uint64_t depth_nodes[64];
+    uint64_t algo_bytes;
 };
 
 void* orc_graph_new(uint32_t V, const uint32_t* off, const uint32_t* adj, const uint32_t* lab) {
@@ -882,6 +915,7 @@ void* orc_run(void* dg, void* qg, const char* filter, const char* order, const c
         out->mem_words = r->mem_words;
         out->n = q.V;
         out->filter_ok = r->filter_ok;
+        out->algo_bytes = r->er.algo_bytes;
         for (uint32_t u = 0; u < q.V; ++u) {
             out->cand_count[u] = (uint32_t)r->C[u].size();
             out->order[u] = r->plan.order[u];

@@ -37,6 +37,7 @@ class EnumStats(ctypes.Structure):
         ("kernel_ms", ctypes.c_double),
         ("expand_ms", ctypes.c_double),
         ("probe_bytes", ctypes.c_uint64),
+        ("algo_bytes", ctypes.c_uint64),
     ]
 
 
@@ -51,6 +52,7 @@ class BatchStats(ctypes.Structure):
         ("probe_bytes", ctypes.c_uint64),
         ("dfs_lane_steps", ctypes.c_uint64),
         ("dfs_lane_iters", ctypes.c_uint64),
+        ("algo_bytes", ctypes.c_uint64),
     ]
 
 

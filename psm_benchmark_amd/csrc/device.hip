@@ -94,6 +94,23 @@ __device__ bool nlf_dominates(const DevGraph& g, uint32_t v, const uint2* qr, ui
     return true;
 }
 
+// The run of label l among v's neighbours in the label-grouped adjacency:
+// (first position in adj_bl, length); length 0 when v has no such neighbour.
+__device__ __forceinline__ uint2 label_run(const DevGraph& g, uint32_t v, uint32_t l) {
+    const uint32_t end = g.nlf_off[v + 1];
+    uint32_t lo = g.nlf_off[v], hi = end;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (g.nlf[mid].x < l) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < end) {
+        const uint2 r = g.nlf[lo];
+        if (r.x == l) return make_uint2(g.nlf_beg[lo], r.y);
+    }
+    return make_uint2(0u, 0u);
+}
+
 // ---------------------------------------------------------------------------
 // Order-preserving compaction, pass 1: predicate per element, one 64-bit
 // mask per wavefront and a survivor count per block.
@@ -227,21 +244,27 @@ struct GqlShared {
 // Bit i set when data vertex w is a live candidate of left vertex i (the
 // i-th query neighbour of u, bitmap slot lslots[i] = first slot + its id):
 // one membership word per w instead of one bitmap word per left vertex.
-__device__ __forceinline__ uint32_t left_mask(const unsigned long long* __restrict__ member,
-                                              const GItem& it, const uint32_t* __restrict__ lslots,
-                                              uint32_t k, uint32_t w) {
-    const unsigned long long mw = member[it.mem_base + w];
+__device__ __forceinline__ uint32_t mask_of(unsigned long long mw, const GItem& it,
+                                            const uint32_t* __restrict__ lslots, uint32_t k) {
     const uint32_t first = it.bm_self - it.u;
     uint32_t m = 0;
     for (uint32_t i = 0; i < k; ++i) m |= (uint32_t)((mw >> (lslots[i] - first)) & 1ull) << i;
     return m;
 }
 
+__device__ __forceinline__ uint32_t left_mask(const unsigned long long* __restrict__ member,
+                                              const GItem& it, const uint32_t* __restrict__ lslots,
+                                              uint32_t k, uint32_t w) {
+    return mask_of(member[it.mem_base + w], it, lslots, k);
+}
+
+constexpr uint32_t kScanUnroll = 4;   // 64-neighbour chunks a wavefront loads before testing
+
 // One candidate (item it, index j) per wavefront; returns its algorithmic
 // bytes (on every lane). wl: the wavefront's index in its block.
 __device__ __forceinline__ unsigned long long gql_refine_one(
     const DevGraph& g, const GItem& it, uint32_t j, const uint32_t* __restrict__ left_slots,
-    uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
+    const uint32_t* __restrict__ left_labs, uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
     unsigned long long* __restrict__ member, GqlShared& sh, uint32_t wl) {
     const uint32_t lane = lane_id();
     if (j >= it.len) return 0;
@@ -249,22 +272,46 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
     if (v == kInvalid) return 0;
     const uint32_t k = it.k;
     const uint32_t* lslots = left_slots + it.left_begin;   // uniform: scalar loads
-    const uint32_t vb = g.off[v], ve = g.off[v + 1];
+    const uint32_t* labs = left_labs + it.lab_begin;
+    // Right vertices can only be neighbours carrying a left vertex's label:
+    // the scans below walk those label runs of the label-grouped adjacency.
     // pass 1: lane i (< k) counts the right neighbours of left vertex i,
     // saturating at k. Once every left vertex has k right neighbours a
     // saturating matching exists (each left vertex in turn still finds an
     // unused one), so the scan stops there.
     uint32_t my_cnt = 0;
-    uint32_t scanned = 0;
-    for (uint32_t base = vb; base < ve; base += kWave) {
-        const uint32_t p = base + lane;
-        const uint32_t m = p < ve ? left_mask(member, it, lslots, k, g.adj[p]) : 0u;
-        for (uint32_t i = 0; i < k; ++i) {
-            const uint32_t c = __popcll(__ballot((m >> i) & 1u));
-            if (lane == i) my_cnt = min(my_cnt + c, k);
+    uint32_t scanned = 0, relevant = 0;
+    bool saturated = false;
+    for (uint32_t li = 0; li < it.nlab && !saturated; ++li) {
+        const uint2 run = label_run(g, v, labs[li]);
+        relevant += run.y;
+        // kScanUnroll chunks per step: their neighbour and membership loads
+        // are independent and in flight together
+        for (uint32_t o = 0; o < run.y; o += kScanUnroll * kWave) {
+            uint32_t w[kScanUnroll];
+            unsigned long long mw[kScanUnroll];
+#pragma unroll
+            for (uint32_t c = 0; c < kScanUnroll; ++c) {
+                const uint32_t p = o + c * kWave + lane;
+                w[c] = p < run.y ? g.adj_bl[run.x + p] : kInvalid;
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < kScanUnroll; ++c) mw[c] = w[c] != kInvalid ? member[it.mem_base + w[c]] : 0ull;
+            uint32_t m[kScanUnroll];
+#pragma unroll
+            for (uint32_t c = 0; c < kScanUnroll; ++c) m[c] = mask_of(mw[c], it, lslots, k);
+            for (uint32_t i = 0; i < k; ++i) {
+                uint32_t cnt = 0;
+#pragma unroll
+                for (uint32_t c = 0; c < kScanUnroll; ++c) cnt += __popcll(__ballot((m[c] >> i) & 1u));
+                if (lane == i) my_cnt = min(my_cnt + cnt, k);
+            }
+            scanned += min(kScanUnroll * kWave, run.y - o);
+            if (__ballot(lane < k && my_cnt < k) == 0) {
+                saturated = true;
+                break;
+            }
         }
-        scanned = min(base + kWave, ve) - vb;
-        if (__ballot(lane < k && my_cnt < k) == 0) break;
     }
     bool ok = __ballot(lane < k && my_cnt == 0) == 0;
     const uint32_t T = (uint32_t)__ballot(lane < k && my_cnt < k);
@@ -272,15 +319,18 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
     if (pass2) {
         // pass 2: right vertices adjacent to T, compacted into LDS
         uint32_t nr = 0;
-        for (uint32_t base = vb; base < ve; base += kWave) {
-            const uint32_t p = base + lane;
-            const uint32_t m = p < ve ? (left_mask(member, it, lslots, k, g.adj[p]) & T) : 0u;
-            const unsigned long long bal = __ballot(m != 0);
-            if (m) {
-                const uint32_t r = nr + __popcll(bal & lanemask_lt());
-                if (r < kRightCap) sh.rmask[wl][r] = m;
+        for (uint32_t li = 0; li < it.nlab; ++li) {
+            const uint2 run = label_run(g, v, labs[li]);
+            for (uint32_t o = 0; o < run.y; o += kWave) {
+                const uint32_t p = o + lane;
+                const uint32_t m = p < run.y ? (left_mask(member, it, lslots, k, g.adj_bl[run.x + p]) & T) : 0u;
+                const unsigned long long bal = __ballot(m != 0);
+                if (m) {
+                    const uint32_t r = nr + __popcll(bal & lanemask_lt());
+                    if (r < kRightCap) sh.rmask[wl][r] = m;
+                }
+                nr += __popcll(bal);
             }
-            nr += __popcll(bal);
         }
         nr = min(nr, (uint32_t)kRightCap);  // bounded by 31*31 < kRightCap
         for (uint32_t r = lane; r < nr; r += kWave) sh.match_r[wl][r] = 0xFF;
@@ -328,10 +378,11 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
         atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
         atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
     }
-    // algorithmic bytes: candidate, offsets, the scanned part of N(v), one
-    // membership word per (neighbour, left vertex) probe; pass 2 rescans all
-    // of N(v) when it runs
-    return 12ull + 4ull * scanned * (1 + k) + (pass2 ? 4ull * (ve - vb) * (1 + k) : 0ull);
+    // algorithmic bytes: candidate, one run lookup per left label, the
+    // scanned part of the label runs, one membership word per (neighbour,
+    // left vertex) probe; pass 2 rescans the runs when it runs
+    return 4ull + 16ull * it.nlab * (pass2 ? 2 : 1) + 4ull * scanned * (1 + k) +
+           (pass2 ? 4ull * relevant * (1 + k) : 0ull);
 }
 
 // The wavefront test over the deferred (item, index) pairs of class 3; the
@@ -340,15 +391,15 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
 __global__ __launch_bounds__(kBlock) void k_gql_refine(
     DevGraph g, const GItem* __restrict__ items, const uint2* __restrict__ list,
     const uint32_t* __restrict__ n_list, const uint32_t* __restrict__ left_slots,
-    uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
-    unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
+    const uint32_t* __restrict__ left_labs, uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps,
+    uint64_t wps, unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
     __shared__ GqlShared sh;
     const uint32_t n = *n_list;
     const uint32_t wl = threadIdx.x / kWave;
     unsigned long long by = 0;
     for (uint32_t w = blockIdx.x * kGqlWavesPerBlock + wl; w < n; w += gridDim.x * kGqlWavesPerBlock) {
         const uint2 e = list[w];
-        by += gql_refine_one(g, items[e.x], e.y, left_slots, cand_pool, bitmaps, wps, member, sh, wl);
+        by += gql_refine_one(g, items[e.x], e.y, left_slots, left_labs, cand_pool, bitmaps, wps, member, sh, wl);
     }
     block_add(bytes, lane_id() == 0 ? by : 0ull);
 }
@@ -376,6 +427,34 @@ template <uint32_t G>
 __device__ __forceinline__ unsigned long long group_bits(unsigned long long bal, uint32_t gbase) {
     if constexpr (G == 64) return bal;
     else return (bal >> gbase) & ((1ull << G) - 1ull);
+}
+
+// The neighbours of v that can be right vertices of item it are the label
+// runs of its left labels in adj_bl, taken one after the other (at most
+// kGroup labels here: callers use the positions only for k <= kGroup, and
+// only the length otherwise). Every lane of a G-lane group gets their total
+// length; lane lig also gets the adj_bl position of element lig (when lig
+// is below the total). Every lane of the wavefront calls it.
+template <uint32_t G>
+__device__ __forceinline__ uint32_t group_relevant(const DevGraph& g, const GItem& it,
+                                                   const uint32_t* __restrict__ left_labs, uint32_t v,
+                                                   bool live, uint32_t lig, uint32_t& pos) {
+    uint2 run = make_uint2(0u, 0u);
+    if (live && lig < it.nlab && lig < kGroup) run = label_run(g, v, left_labs[it.lab_begin + lig]);
+    uint32_t incl = run.y;   // inclusive prefix of run lengths over the group's lanes
+#pragma unroll
+    for (uint32_t o = 1; o < kGroup; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, G);
+        if (lig >= o) incl += t;
+    }
+    const uint32_t total = __shfl(incl, kGroup - 1, G);
+    pos = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kGroup; ++r) {
+        const uint32_t e = __shfl(incl, r, G), b = __shfl(run.x, r, G), l = __shfl(run.y, r, G);
+        if (lig + l >= e && lig < e) pos = b + (lig + l - e);
+    }
+    return total;
 }
 
 // Hall's condition for one candidate tested by a group of G lanes: lane lig
@@ -419,8 +498,8 @@ template <uint32_t G>
 __global__ __launch_bounds__(kBlock) void k_gql_hall(
     DevGraph g, const GItem* __restrict__ items, const uint2* __restrict__ list,
     const uint32_t* __restrict__ n_list, const uint32_t* __restrict__ left_slots,
-    uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
-    unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
+    const uint32_t* __restrict__ left_labs, uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps,
+    uint64_t wps, unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
     constexpr uint32_t kPerIter = kBlock / G;
     const uint32_t n = *n_list;
     const uint32_t lane = lane_id();
@@ -431,7 +510,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_hall(
     for (uint64_t first = (uint64_t)blockIdx.x * kPerIter; first < n; first += (uint64_t)gridDim.x * kPerIter) {
         const uint64_t gid = first + threadIdx.x / G;
         bool live = gid < n;
-        uint32_t j = 0, v = kInvalid, k = 0, deg = 0, vb = 0;
+        uint32_t j = 0, v = kInvalid, k = 0, deg = 0;
         GItem it{};
         if (live) {
             const uint2 e = list[gid];
@@ -440,19 +519,18 @@ __global__ __launch_bounds__(kBlock) void k_gql_hall(
             v = cand_pool[it.cand_begin + j];
             live = v != kInvalid;
         }
-        if (live) {
-            vb = g.off[v];
-            deg = min(g.off[v + 1] - vb, G);   // deg <= G by construction of the list
-            k = min(it.k, kGroup);
-        }
+        uint32_t pos = 0;
+        // relevant degree <= G by construction of the list
+        deg = min(group_relevant<G>(g, it, left_labs, v, live, lig, pos), G);
+        if (live) k = min(it.k, kGroup);
         uint32_t m = 0;
-        if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj[vb + lig]);
+        if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj_bl[pos]);
         const bool fail = hall_fails<G>(m, live, k, lig, gbase);
         if (live && lig == 0) {
             if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member);
-            // algorithmic bytes: candidate, offsets, N(v), one membership
-            // word per neighbour and left vertex
-            by += 12ull + 4ull * deg * (1 + k);
+            // algorithmic bytes: candidate, one run lookup per left label,
+            // the runs, one membership word per neighbour and left vertex
+            by += 4ull + 16ull * it.nlab + 4ull * deg * (1 + k);
         }
     }
     block_add(bytes, by);
@@ -491,9 +569,10 @@ __device__ __forceinline__ void defer_flush(DeferStage& st, uint2* __restrict__ 
 
 __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
     DevGraph g, const GItem* __restrict__ items, uint32_t n_items, uint32_t n_cands,
-    const uint32_t* __restrict__ left_slots, uint32_t* __restrict__ cand_pool,
-    uint32_t* __restrict__ bitmaps, uint64_t wps, unsigned long long* __restrict__ member,
-    uint2* __restrict__ lists, uint64_t list_stride, uint32_t* __restrict__ n_lists,
+    const uint32_t* __restrict__ left_slots, const uint32_t* __restrict__ left_labs,
+    uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
+    unsigned long long* __restrict__ member, uint2* __restrict__ lists, uint64_t list_stride,
+    uint32_t* __restrict__ n_lists,
     unsigned long long* __restrict__ bytes) {
     constexpr uint32_t kPerIter = kBlock / kGroup;   // candidates per block iteration
     __shared__ DeferStage st;
@@ -508,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
          first += (uint64_t)gridDim.x * kPerIter) {
         const uint64_t gid = first + threadIdx.x / kGroup;
         bool live = gid < n_cands;
-        uint32_t ii = 0, j = 0, v = kInvalid, k = 0, deg = 0, vb = 0;
+        uint32_t ii = 0, j = 0, v = kInvalid, k = 0;
         GItem it{};
         if (live) {
             ii = find_item(items, n_items, (uint32_t)gid, &GItem::wave_start);
@@ -520,11 +599,9 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
             v = cand_pool[it.cand_begin + j];
             live = v != kInvalid;
         }
-        if (live) {
-            vb = g.off[v];
-            deg = g.off[v + 1] - vb;
-            k = it.k;
-        }
+        uint32_t pos = 0;
+        const uint32_t deg = group_relevant<kGroup>(g, it, left_labs, v, live, lig, pos);
+        if (live) k = it.k;
         // defer the candidates outside the limits into the block's stage
         const bool defer = live && lig == 0 && (deg > kGroup || k > kGroup);
         const uint32_t dc = defer ? defer_class(deg, k) : kDeferClasses;
@@ -540,11 +617,11 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
         }
         if (live && (deg > kGroup || k > kGroup)) live = false;
         uint32_t m = 0;
-        if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj[vb + lig]);
+        if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj_bl[pos]);
         const bool fail = hall_fails<kGroup>(m, live, k, lig, gbase);
         if (live && lig == 0) {
             if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member);
-            by += 12ull + 4ull * deg * (1 + k);
+            by += 4ull + 16ull * it.nlab + 4ull * deg * (1 + k);
         }
         __syncthreads();
         // at most kPerIter pairs per class join the stage per iteration
@@ -557,9 +634,12 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
 // ---------------------------------------------------------------------------
 // BuildTable (BuildTable.cpp:9-151) by definition: row j of table (a, b) lists
 // the ranks in C(b) of the neighbours of C(a)[j] that are in C(b). The rank
-// of w is the count of set bits before it in the (query, b) bitmap.
+// of w is the count of set bits before it in the (query, b) bitmap. Only
+// neighbours with b's label can be in C(b): a row scans that label's run of
+// the label-grouped adjacency (ids ascending inside it, so ranks come out
+// ascending), not all of N(v).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kRowCoop = 64;   // rows of candidates with more neighbours go to a wavefront
+constexpr uint32_t kRowCoop = 64;   // rows whose label run is longer go to a wavefront
 
 __device__ __forceinline__ uint32_t bit_rank(const uint32_t* bm, const uint32_t* wp, uint32_t w) {
     const uint32_t word = w >> 5;
@@ -581,19 +661,20 @@ __global__ __launch_bounds__(kBlock) void k_table_count(
             const uint32_t ii = find_u64(item_row_start, n_items, r);
             const TItem it = items[ii];
             const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
-            const uint32_t vb = g.off[v], ve = g.off[v + 1];
-            if (ve - vb > kRowCoop) {
+            const uint2 run = label_run(g, v, it.lab_b);
+            if (run.y > kRowCoop) {
                 defer = true;   // a wavefront scans it (k_table_count_long)
             } else {
                 const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
                 uint32_t c = 0;
-                for (uint32_t p = vb; p < ve; ++p) {
-                    const uint32_t w = g.adj[p];
+                for (uint32_t p = run.x; p < run.x + run.y; ++p) {
+                    const uint32_t w = g.adj_bl[p];
                     c += (bm[w >> 5] >> (w & 31)) & 1u;
                 }
                 row_deg[r] = c;
-                // candidate + offsets + N(v) + one bitmap word per neighbour + the count
-                by += 4 + 8 + 8ull * (ve - vb) + 8;
+                // candidate + run lookup + the run + one bitmap word per
+                // neighbour in it + the count
+                by += 4 + 16 + 8ull * run.y + 8;
             }
         }
         const unsigned long long bal = __ballot(defer);
@@ -623,21 +704,22 @@ __global__ __launch_bounds__(kBlock) void k_table_count_long(
         const uint32_t ii = find_u64(item_row_start, n_items, r);
         const TItem it = items[ii];
         const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
-        const uint32_t vb = g.off[v], ve = g.off[v + 1];
+        const uint2 run = label_run(g, v, it.lab_b);
+        const uint32_t vb = run.x, ve = run.x + run.y;
         const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
         uint64_t c = 0;
         for (uint32_t base = vb; base < ve; base += kWave) {
             const uint32_t p = base + lane;
             bool hit = false;
             if (p < ve) {
-                const uint32_t w = g.adj[p];
+                const uint32_t w = g.adj_bl[p];
                 hit = (bm[w >> 5] >> (w & 31)) & 1u;
             }
             c += __popcll(__ballot(hit));
         }
         if (lane == 0) {
             row_deg[r] = c;
-            by += 4 + 8 + 8ull * (ve - vb) + 8;
+            by += 4 + 16 + 8ull * run.y + 8;
         }
     }
     wave_add(bytes, by);
@@ -658,18 +740,19 @@ __global__ __launch_bounds__(kBlock) void k_table_fill(
         const uint32_t ii = find_u64(item_row_start, n_items, r);
         const TItem it = items[ii];
         const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
-        if (g.off[v + 1] - g.off[v] > kRowCoop) continue;   // k_table_fill_long
+        const uint2 run = label_run(g, v, it.lab_b);
+        if (run.y > kRowCoop) continue;   // k_table_fill_long
         const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
         const uint32_t* wp = word_prefix + (size_t)it.bm_b * wps;
         const uint64_t o0 = row_off[r];
         uint64_t o = o0;
-        for (uint32_t p = g.off[v]; p < g.off[v + 1]; ++p) {
-            const uint32_t w = g.adj[p];
+        for (uint32_t p = run.x; p < run.x + run.y; ++p) {
+            const uint32_t w = g.adj_bl[p];
             if ((bm[w >> 5] >> (w & 31)) & 1u) idx[o++] = bit_rank(bm, wp, w);
         }
-        // candidate + offsets + row offset + N(v) + bitmap word per neighbour
-        // + per entry: rank words (2) and the entry written
-        by += 4 + 8 + 8 + 8ull * (g.off[v + 1] - g.off[v]) + 12ull * (o - o0);
+        // candidate + run lookup + row offset + the run + bitmap word per
+        // neighbour in it + per entry: rank words (2) and the entry written
+        by += 4 + 16 + 8 + 8ull * run.y + 12ull * (o - o0);
     }
     wave_add(bytes, by);
 }
@@ -690,7 +773,8 @@ __global__ __launch_bounds__(kBlock) void k_table_fill_long(
         const uint32_t ii = find_u64(item_row_start, n_items, r);
         const TItem it = items[ii];
         const uint32_t v = cand_pool[it.cand_a + (r - item_row_start[ii])];
-        const uint32_t vb = g.off[v], ve = g.off[v + 1];
+        const uint2 run = label_run(g, v, it.lab_b);
+        const uint32_t vb = run.x, ve = run.x + run.y;
         const uint32_t* bm = bitmaps + (size_t)it.bm_b * wps;
         const uint32_t* wp = word_prefix + (size_t)it.bm_b * wps;
         const uint64_t o0 = row_off[r];
@@ -700,14 +784,14 @@ __global__ __launch_bounds__(kBlock) void k_table_fill_long(
             bool hit = false;
             uint32_t w = 0;
             if (p < ve) {
-                w = g.adj[p];
+                w = g.adj_bl[p];
                 hit = (bm[w >> 5] >> (w & 31)) & 1u;
             }
             const unsigned long long bal = __ballot(hit);
             if (hit) idx[o + __popcll(bal & lanemask_lt())] = bit_rank(bm, wp, w);
             o += __popcll(bal);
         }
-        if (lane == 0) by += 4 + 8 + 8 + 8ull * (ve - vb) + 12ull * (o - o0);
+        if (lane == 0) by += 4 + 16 + 8 + 8ull * run.y + 12ull * (o - o0);
     }
     wave_add(bytes, by);
 }
@@ -802,7 +886,8 @@ struct LevelBase {
 template <typename State>
 __device__ __forceinline__ LevelBase level_base(const DPlan& P, const DOp* __restrict__ ops,
                                                 const uint64_t* __restrict__ row_off, uint32_t d,
-                                                const State& st, unsigned long long& bytes) {
+                                                const State& st, unsigned long long& bytes,
+                                                unsigned long long& rows) {
     LevelBase b;
     const uint32_t nops = P.op_count[d];
     if (nops == 0) {
@@ -819,6 +904,7 @@ __device__ __forceinline__ LevelBase level_base(const DPlan& P, const DOp* __res
         const DOp op = ops[P.op_begin[d] + j];
         const uint64_t r = op.row_base + st.I(op.src_depth);
         const uint64_t lo = row_off[r], hi = row_off[r + 1];
+        rows += hi - lo;
         if (hi - lo < best) { best = hi - lo; b.lo = lo; b.hi = hi; b.sel = j; }
     }
     bytes += 16ull * nops;
@@ -923,11 +1009,11 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         M[e] = cand_pool[P.cand_base[e] + I[e]];
     }
     PrefixState st{I, M};
-    unsigned long long bytes = 0;
+    unsigned long long bytes = 0, rows = 0;
     const uint32_t d = task.depth;
     LevelBase b;
     if (task.hi == kAutoHi) {
-        b = level_base(P, ops, row_off, d, st, bytes);
+        b = level_base(P, ops, row_off, d, st, bytes, rows);   // a node entered here
     } else {
         b.lo = task.lo;
         b.hi = task.hi;
@@ -948,6 +1034,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(
                 out_tasks[pos++] = ch;
             }
             atomicAdd(&qc[task.q].expand_bytes, bytes);
+            if (rows) atomicAdd(&qc[task.q].expand_algo_bytes, 4ull * rows);
         } else {
             out_count[t] = (uint32_t)pieces;
             atomicAdd(qchild + task.q, (unsigned long long)pieces);
@@ -977,9 +1064,12 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         out_count[t] = c;
         atomicAdd(qchild + task.q, (unsigned long long)c);
     } else {
-        // each child is one assignment at this depth
+        // each child is one assignment at this depth; children below the
+        // root level are written survivors (expansion stops short of the leaf)
         atomicAdd(&qc[task.q].calls, (unsigned long long)c);
         atomicAdd(&qc[task.q].expand_bytes, bytes);
+        const unsigned long long algo = 4ull * rows + (d >= 1 ? 4ull * c : 0ull);
+        if (algo) atomicAdd(&qc[task.q].expand_algo_bytes, algo);
     }
 }
 
@@ -1015,7 +1105,7 @@ constexpr uint32_t kAutoSel = 0xFFFFFFFEu;
 __device__ __forceinline__ void enter_level(const DPlan& P, const DOp* __restrict__ ops,
                                             const uint64_t* __restrict__ row_off, uint32_t d,
                                             const LaneState& st, uint32_t forced_sel, Level& L,
-                                            unsigned long long& bytes) {
+                                            unsigned long long& bytes, unsigned long long& algo) {
     const uint32_t nops = P.op_count[d];
     L.nb = L.nx = 0;
     if (nops == 0) {
@@ -1032,6 +1122,7 @@ __device__ __forceinline__ void enter_level(const DPlan& P, const DOp* __restric
             const DOp op = ops[P.op_begin[d] + j];
             const uint64_t r = op.row_base + st.I(op.src_depth);
             const uint64_t len = row_off[r + 1] - row_off[r];
+            algo += 4ull * len;   // a fresh node: its operand rows
             if (len < best) { best = len; sel = j; }
         }
         bytes += 16ull * nops;
@@ -1186,7 +1277,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     uint32_t q = 0, tdepth = 0, d = 0, n = 0;
     Level L{};
     Level Lp{};   // the level above the leaf depth, kept while the lane is at the leaf
-    unsigned long long lcount = 0, lcalls = 0, lbytes = 0;
+    unsigned long long lcount = 0, lcalls = 0, lbytes = 0, lalgo = 0;
     unsigned long long busy_iters = 0, all_iters = 0;
     uint32_t steps = 0;
     const DPlan* P = plans;
@@ -1244,9 +1335,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     st.M_[st.at(e)] = cand_pool[P->cand_base[e] + x];
                 }
                 if (task.hi == kAutoHi) {
-                    enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes);
+                    enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo);
                 } else {
-                    enter_level(*P, ops, row_off, d, st, task.sel, L, lbytes);
+                    enter_level(*P, ops, row_off, d, st, task.sel, L, lbytes, lalgo);
                     L.cur = task.lo;
                     L.end = task.hi;
                 }
@@ -1315,9 +1406,10 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     st.sel_[a] = L.sel == kIdentity ? kSelIdentity : (uint8_t)L.sel;
                     st.I_[a] = x;
                     st.M_[a] = v;
+                    if (d >= 1) lalgo += 4;   // a survivor written below the root level
                     ++d;
                     if (d + 1 == n) Lp = L;   // restored on the way back up
-                    enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes);
+                    enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo);
                 }
             }
         } else if (d == tdepth) {
@@ -1333,7 +1425,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 L.end = L.base0 + st.end_[a];
             } else {
                 const uint32_t sel = st.sel_[a] == kSelIdentity ? kIdentity : st.sel_[a];
-                enter_level(*P, ops, row_off, d, st, sel, L, lbytes);
+                enter_level(*P, ops, row_off, d, st, sel, L, lbytes, lalgo);
                 L.cur = L.base0 + st.cur_[a];
                 L.end = L.base0 + st.end_[a];
             }
@@ -1394,13 +1486,15 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
             atomicAdd(&qc[q].calls, lcalls);
             atomicAdd(&qc[q].probe_bytes, lbytes);
+            if (lalgo) atomicAdd(&qc[q].algo_bytes, lalgo);
             if (lcount && prev + lcount >= P->cap) atomicExch(&qc[q].done, 1u);
-            lcount = lcalls = lbytes = 0;
+            lcount = lcalls = lbytes = lalgo = 0;
         }
         if (finished) atomicSub(pending + q, 1u);
         if (!active) atomicSub(&Q->busy, 1u);
     }
     if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);   // bytes of tasks dropped at claim
+    if (lalgo) atomicAdd(&qc[q].algo_bytes, lalgo);
     wave_add(&Q->lane_steps, busy_iters);
     wave_add(&Q->lane_iters, all_iters);
 }
@@ -1438,6 +1532,7 @@ struct WaveFrame {
     uint32_t gen_m[N];      // generation cursor: member,
     uint32_t gen_cnt[N];    //   number of members,
     uint32_t gen_sel[N];    //   base operand of the cursor member,
+    uint32_t counted[N];    //   members whose operand rows are counted
     uint64_t gen_pos[N];    //   next base position (kFresh: not started),
     uint64_t gen_end[N];    //   end of the cursor member's base range
     uint32_t cx[N][kWave];  // frame: candidate index
@@ -1488,7 +1583,8 @@ template <typename F>
 __device__ __forceinline__ uint64_t wave_build_cache(const DPlan& P, const DOp* __restrict__ ops,
                                      const uint64_t* __restrict__ row_off,
                                      const uint32_t* __restrict__ cand_pool, F& S, uint32_t g,
-                                     uint32_t t, uint32_t& cnt_out, unsigned long long& bytes) {
+                                     uint32_t t, uint32_t& cnt_out, unsigned long long& bytes,
+                                     unsigned long long& algo) {
     const uint32_t lane = threadIdx.x;
     const uint32_t cnt = g == t ? 1u : S.run_hi[g - 1] - S.run_lo[g - 1];
     const uint32_t gm = S.gen_m[g];
@@ -1516,14 +1612,18 @@ __device__ __forceinline__ uint64_t wave_build_cache(const DPlan& P, const DOp* 
             len = P.cand_len[g];
             sel = kIdentity;
         } else {
-            uint64_t best = ~0ull;
+            uint64_t best = ~0ull, rows = 0;
             for (uint32_t j = 0; j < nops; ++j) {
                 const DOp op = ops[P.op_begin[g] + j];
                 const uint64_t r = op.row_base + op_source(S, op, g, t, x);
                 const uint64_t a = row_off[r], b = row_off[r + 1];
+                rows += b - a;
                 if (b - a < best) { best = b - a; lo = a; sel = j; }
             }
             bytes += 16ull * nops;
+            // a fresh member is a node entered here; a cache rebuilt for the
+            // same run counts it only once
+            if (lane >= S.counted[g]) algo += 4ull * rows;
             len = best;
         }
         if (sel != kIdentity && len > 0) {
@@ -1552,10 +1652,13 @@ __device__ __forceinline__ uint64_t wave_build_cache(const DPlan& P, const DOp* 
         if (lane >= o) incl += ((uint64_t)hi32 << 32) | lo32;
     }
     S.m_pre[lane + 1] = incl;
+    const uint32_t counted = S.counted[g];
     if (lane == 0) {
         S.m_pre[0] = 0;
         S.gen_cnt[g] = cnt;
     }
+    __syncthreads();
+    if (lane == 0) S.counted[g] = max(counted, cnt);
     const uint32_t tl = __shfl((uint32_t)incl, kWave - 1), th = __shfl((uint32_t)(incl >> 32), kWave - 1);
     __syncthreads();
     cnt_out = cnt;
@@ -1583,7 +1686,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
     if (lane == 0 && Q->limit_ticks)
         atomicCAS(&Q->deadline, 0ull, (unsigned long long)wall_clock64() + Q->limit_ticks);
     const uint32_t hungry_below = gridDim.x / 2;   // queue shorter than half the wavefronts
-    unsigned long long bytes = 0, busy_pos = 0, all_pos = 0;
+    unsigned long long bytes = 0, algo = 0, busy_pos = 0, all_pos = 0;
 
     while (true) {
         // ---- claim one task (none once the time limit has passed: queued
@@ -1632,6 +1735,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
         if (lane == 0) {
             S.gen_m[t] = 0;
             S.gen_cnt[t] = 1;
+            S.counted[t] = 0;
             if (task.hi == kAutoHi) {
                 S.gen_pos[t] = kFresh;
             } else {
@@ -1649,7 +1753,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
         bool finished = false;
         while (true) {
             if (cache_g != (int)g) {
-                total = wave_build_cache(P, ops, row_off, cand_pool, S, g, t, cnt, bytes);
+                total = wave_build_cache(P, ops, row_off, cand_pool, S, g, t, cnt, bytes, algo);
                 flat = 0;
                 cache_g = (int)g;
             }
@@ -1676,6 +1780,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                         }
                         S.gen_m[g] = 0;
                         S.gen_pos[g] = kFresh;
+                        S.counted[g] = 0;
                     }
                     __syncthreads();
                 } else {
@@ -1762,6 +1867,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                 if (!hb && lane == 0) S.gen_m[g] = cnt;
             }
             lcalls += ns;
+            if (!leaf && g >= 1 && lane == 0) algo += 4ull * ns;   // survivors written below the root level
             if (leaf) {
                 lcount += ns;
             } else if (ns) {
@@ -1780,6 +1886,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                     }
                     S.gen_m[g + 1] = 0;
                     S.gen_pos[g + 1] = kFresh;
+                    S.counted[g + 1] = 0;
                 }
                 ++g;
                 cache_g = -1;
@@ -1867,7 +1974,8 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             atomicSub(&Q->busy, 1u);
         }
         wave_add(&qc[q].probe_bytes, bytes);
-        bytes = 0;
+        wave_add(&qc[q].algo_bytes, algo);
+        bytes = algo = 0;
     }
     wave_add(&Q->lane_steps, busy_pos);
     wave_add(&Q->lane_iters, all_pos);
@@ -2158,19 +2266,21 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
 constexpr uint32_t kGqlGrid = 2048;
 
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, const uint2* list, const uint32_t* n_list,
-                             uint32_t max_n, const uint32_t* left_slots, uint32_t* cand_pool, uint32_t* bitmaps,
+                             uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_labs,
+                             uint32_t* cand_pool, uint32_t* bitmaps,
                              uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
                              hipStream_t s) {
     if (max_n == 0) return hipSuccess;
-    const uint32_t blocks =
-        std::min<uint32_t>(kGqlGrid / 2, (max_n + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock);
+    // six blocks per CU fit the LDS of gfx950's 256 CUs (26 KB per block)
+    const uint32_t blocks = std::min<uint32_t>(1536, (max_n + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock);
     hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
-                       cand_pool, bitmaps, words_per_slot, member, bytes);
+                       left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
     return hipGetLastError();
 }
 
 hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32_t n_items,
-                                   uint32_t n_cands, const uint32_t* left_slots, uint32_t* cand_pool,
+                                   uint32_t n_cands, const uint32_t* left_slots, const uint32_t* left_labs,
+                                   uint32_t* cand_pool,
                                    uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
                                    uint2* lists, uint64_t list_stride, uint32_t* n_lists,
                                    unsigned long long* bytes, hipStream_t s) {
@@ -2179,26 +2289,27 @@ hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32
     const uint64_t want = ((uint64_t)n_cands * kGroup + kBlock - 1) / kBlock;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, kGqlGrid);
     hipLaunchKernelGGL(k_gql_refine_small, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_cands,
-                       left_slots, cand_pool, bitmaps, words_per_slot, member, lists, list_stride, n_lists, bytes);
+                       left_slots, left_labs, cand_pool, bitmaps, words_per_slot, member, lists, list_stride, n_lists,
+                       bytes);
     return hipGetLastError();
 }
 
 hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items, const uint2* list,
                            const uint32_t* n_list, uint32_t max_n, const uint32_t* left_slots,
-                           uint32_t* cand_pool, uint32_t* bitmaps, uint64_t words_per_slot,
+                           const uint32_t* left_labs, uint32_t* cand_pool, uint32_t* bitmaps, uint64_t words_per_slot,
                            unsigned long long* member, unsigned long long* bytes, hipStream_t s) {
     if (max_n == 0) return hipSuccess;
     const uint64_t want = ((uint64_t)max_n * lanes + kBlock - 1) / kBlock;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, kGqlGrid);
     if (lanes == 16)
         hipLaunchKernelGGL(k_gql_hall<16>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
-                           cand_pool, bitmaps, words_per_slot, member, bytes);
+                           left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
     else if (lanes == 32)
         hipLaunchKernelGGL(k_gql_hall<32>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
-                           cand_pool, bitmaps, words_per_slot, member, bytes);
+                           left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
     else if (lanes == 64)
         hipLaunchKernelGGL(k_gql_hall<64>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
-                           cand_pool, bitmaps, words_per_slot, member, bytes);
+                           left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -27,6 +27,8 @@ struct DevGraph {
     const uint32_t* ri;      // V, ascending within a label
     const uint32_t* nlf_off; // V+1: neighbour-label runs per vertex
     const uint2* nlf;        // (label, count), sorted by label
+    const uint32_t* adj_bl;  // nnz: neighbours of each vertex ordered by (label, id)
+    const uint32_t* nlf_beg; // per run: its first position in adj_bl
 };
 
 // One query vertex as the candidate filters see it.
@@ -69,6 +71,8 @@ struct GItem {
     uint32_t left_begin;   // into the left-slot array: bitmap slots of N_q(u)
     uint32_t u;            // the query vertex (bm_self - u = the query's first slot)
     uint64_t mem_base;     // the query's membership words
+    uint32_t lab_begin;    // into the label array: the distinct labels of N_q(u), ascending
+    uint32_t nlab;
 };
 
 // BuildTable work item: ordered query edge (a, b) of one query.
@@ -77,6 +81,8 @@ struct TItem {
     uint32_t len_a;
     uint32_t bm_b;         // bitmap slot of (query, b)
     uint64_t row_base;     // first row of this table in the global row arrays
+    uint32_t lab_b;        // label of b: only that run of N(v) can hold C(b) vertices
+    uint32_t pad;
 };
 
 // Enumeration plan of one query (depth d holds query vertex order[d]).
@@ -151,8 +157,14 @@ struct JoinKeys {
 struct QCounters {
     unsigned long long count;
     unsigned long long calls;
-    unsigned long long probe_bytes;    // algorithmic bytes of the DFS kernel
-    unsigned long long expand_bytes;   // algorithmic bytes of task pre-expansion
+    unsigned long long probe_bytes;    // bytes the DFS kernel touched (probes, base elements, rows located)
+    unsigned long long expand_bytes;   // the same for task pre-expansion
+    // Algorithmic enumeration bytes: for every search-tree node below the
+    // root, 4 per entry of each of its operand rows, plus 4 per survivor it
+    // writes unless its survivors are leaves; split by the kernel that
+    // entered the node.
+    unsigned long long algo_bytes;         // nodes of the DFS kernels
+    unsigned long long expand_algo_bytes;  // nodes of task pre-expansion
     uint32_t done;        // cap reached
     uint32_t timed_out;   // work was dropped at the time limit
 };
@@ -192,16 +204,18 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
 // n_lists[0..3]. The deferred-class kernels read their count from n_list;
 // max_n bounds it (sizes the grid).
 hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32_t n_items,
-                                   uint32_t n_cands, const uint32_t* left_slots, uint32_t* cand_pool,
+                                   uint32_t n_cands, const uint32_t* left_slots, const uint32_t* left_labs,
+                                   uint32_t* cand_pool,
                                    uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
                                    uint2* lists, uint64_t list_stride, uint32_t* n_lists,
                                    unsigned long long* bytes, hipStream_t s);
 hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items, const uint2* list,
                            const uint32_t* n_list, uint32_t max_n, const uint32_t* left_slots,
-                           uint32_t* cand_pool, uint32_t* bitmaps, uint64_t words_per_slot,
+                           const uint32_t* left_labs, uint32_t* cand_pool, uint32_t* bitmaps, uint64_t words_per_slot,
                            unsigned long long* member, unsigned long long* bytes, hipStream_t s);
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, const uint2* list, const uint32_t* n_list,
-                             uint32_t max_n, const uint32_t* left_slots, uint32_t* cand_pool, uint32_t* bitmaps,
+                             uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_labs,
+                             uint32_t* cand_pool, uint32_t* bitmaps,
                              uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
                              hipStream_t s);
 // thread per row; rows of candidates with more than 64 neighbours are listed

@@ -288,16 +288,20 @@ struct psm_graph {
     uint32_t V = 0, L = 0, max_deg = 0, max_freq = 0;
     uint64_t nnz = 0;
     std::vector<uint32_t> h_ri_off;   // host copy: bucket sizes drive work items
-    DBuf off, adj, lab, ri_off, ri, nlf_off, nlf;
+    DBuf off, adj, lab, ri_off, ri, nlf_off, nlf, adj_bl, nlf_beg;
     DevGraph dg{};
 };
 
 // Reverse index (graph.cpp:15-33) and neighbour-label runs (graph.cpp:36-52)
-// built on the host with all cores, then uploaded.
+// built on the host with all cores, then uploaded. Also the label-grouped
+// adjacency (the reference's OPTIMIZED_LABELED_GRAPH layout, graph.cpp:35-90):
+// adj_bl holds each vertex's neighbours ordered by (label, id), and
+// nlf_beg[i] is where run i (label nlf[i].x, nlf[i].y neighbours) starts in it.
 static void build_graph_indexes(uint32_t V, const uint32_t* off, const uint32_t* adj,
                                 const uint32_t* lab, uint32_t L, std::vector<uint32_t>& ri_off,
                                 std::vector<uint32_t>& ri, std::vector<uint32_t>& nlf_off,
-                                std::vector<uint2>& nlf) {
+                                std::vector<uint2>& nlf, std::vector<uint32_t>& adj_bl,
+                                std::vector<uint32_t>& nlf_beg) {
     ri_off.assign(L + 1, 0);
     for (uint32_t v = 0; v < V; ++v) ri_off[lab[v] + 1]++;
     for (uint32_t l = 0; l < L; ++l) ri_off[l + 1] += ri_off[l];
@@ -318,8 +322,16 @@ static void build_graph_indexes(uint32_t V, const uint32_t* off, const uint32_t*
             }
             if (write) {
                 std::sort(touched.begin(), touched.end());
-                uint32_t o = nlf_off[v];
-                for (uint32_t l : touched) nlf[o++] = make_uint2(l, cnt[l]);
+                uint32_t o = nlf_off[v], at = off[v];
+                for (uint32_t l : touched) {
+                    nlf[o] = make_uint2(l, cnt[l]);
+                    nlf_beg[o++] = at;
+                    const uint32_t c = cnt[l];
+                    cnt[l] = at;   // next free position of label l
+                    at += c;
+                }
+                // stable counting sort: ids stay ascending inside a run
+                for (uint32_t p = off[v]; p < off[v + 1]; ++p) adj_bl[cnt[lab[adj[p]]]++] = adj[p];
             } else {
                 runs[v] = (uint32_t)touched.size();
             }
@@ -333,6 +345,8 @@ static void build_graph_indexes(uint32_t V, const uint32_t* off, const uint32_t*
     nlf_off.assign(V + 1, 0);
     for (uint32_t v = 0; v < V; ++v) nlf_off[v + 1] = nlf_off[v] + runs[v];
     nlf.resize(nlf_off[V]);
+    nlf_beg.resize(nlf_off[V]);
+    adj_bl.resize(off[V]);
     for (unsigned t = 0; t < nt; ++t) th.emplace_back(pass, 1, t);
     for (auto& x : th) x.join();
 }
@@ -352,7 +366,7 @@ struct QueryHost {
     std::vector<uint64_t> tab_nnz;    // n*n
     Plan plan;
     bool planned = false;
-    uint64_t count = 0, calls = 0, probe_bytes = 0;
+    uint64_t count = 0, calls = 0, probe_bytes = 0, algo_bytes = 0;
     bool capped = false;
     bool timed_out = false;           // partial count: the reference's "Overtime: 1"
 };
@@ -645,7 +659,7 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
     // work items of every query vertex position p (the candidates of vertex p
     // of each query), uploaded once: both rounds use the same items
     std::vector<GItem> items;
-    std::vector<uint32_t> left, item_begin(max_n + 1, 0), cands(max_n, 0);
+    std::vector<uint32_t> left, labs, item_begin(max_n + 1, 0), cands(max_n, 0);
     uint32_t max_cands = 0;
     for (uint32_t p = 0; p < max_n; ++p) {
         item_begin[p] = (uint32_t)items.size();
@@ -662,7 +676,15 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
             it.mem_base = a * (uint64_t)g->V;
             it.k = q->qg.degree(p);
             it.left_begin = (uint32_t)left.size();
-            for (uint32_t i = q->qg.off[p]; i < q->qg.off[p + 1]; ++i) left.push_back(q->slot_base + q->qg.adj[i]);
+            it.lab_begin = (uint32_t)labs.size();
+            const size_t l0 = labs.size();
+            for (uint32_t i = q->qg.off[p]; i < q->qg.off[p + 1]; ++i) {
+                left.push_back(q->slot_base + q->qg.adj[i]);
+                labs.push_back(q->qg.lab[q->qg.adj[i]]);
+            }
+            std::sort(labs.begin() + l0, labs.end());
+            labs.erase(std::unique(labs.begin() + l0, labs.end()), labs.end());
+            it.nlab = (uint32_t)(labs.size() - l0);
             items.push_back(it);
             waves += it.len;
         }
@@ -670,9 +692,10 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
         max_cands = std::max(max_cands, waves);
     }
     item_begin[max_n] = (uint32_t)items.size();
-    DBuf d_items, d_left, d_lists, d_nlists;
+    DBuf d_items, d_left, d_labs, d_lists, d_nlists;
     HIP_TRY(ctx, upload(d_items, items, s()));
     HIP_TRY(ctx, upload(d_left, left, s()));
+    HIP_TRY(ctx, upload(d_labs, labs, s()));
     // deferred pairs of one (round, p): class c at c * max_cands; reused by
     // every launch group (stream order)
     HIP_TRY(ctx, d_lists.alloc(4ull * max_cands * sizeof(uint2)));
@@ -690,18 +713,19 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
             uint32_t* cnt = d_nlists.as<uint32_t>() + ((uint64_t)round * max_n + p) * 4;
             uint2* lists = d_lists.as<uint2>();
             TIMED(ctx, kKGqlRefine,
-                  launch_gql_refine_small(g->dg, its, ni, cands[p], d_left.as<uint32_t>(), cand_pool.as<uint32_t>(),
+                  launch_gql_refine_small(g->dg, its, ni, cands[p], d_left.as<uint32_t>(), d_labs.as<uint32_t>(),
+                                          cand_pool.as<uint32_t>(),
                                           bitmaps.as<uint32_t>(), wps, member.as<unsigned long long>(), lists,
                                           max_cands, cnt, ctx->d_kbytes + kKGqlRefine, s()));
             const uint32_t lanes[3] = {16, 32, 64};
             for (int c = 0; c < 3; ++c)
                 TIMED(ctx, kKGqlRefine,
                       launch_gql_hall(lanes[c], g->dg, its, lists + (uint64_t)c * max_cands, cnt + c, cands[p],
-                                      d_left.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
+                                      d_left.as<uint32_t>(), d_labs.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
                                       member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
             TIMED(ctx, kKGqlRefine,
                   launch_gql_refine(g->dg, its, lists + 3ull * max_cands, cnt + 3, cands[p], d_left.as<uint32_t>(),
-                                    cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
+                                    d_labs.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
                                     member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
         }
     }
@@ -781,6 +805,7 @@ int Batch::build_tables() {
                 it.cand_a = q->cand_begin[a];
                 it.len_a = q->cand_count[a];
                 it.bm_b = q->slot_base + b;
+                it.lab_b = q->qg.lab[b];
                 it.row_base = n_rows;
                 q->tab_row[(size_t)a * n + b] = n_rows;
                 items.push_back(it);
@@ -1322,9 +1347,11 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
     }
     tasks_run += n_tasks + donated;
     HIP_TRY(ctx, download(qc, qcnt.p, nq, s()));
+    // per-kernel statistics carry the algorithmic bytes of the nodes each
+    // kernel entered (what the roofline divides by launch time)
     for (uint32_t i = 0; i < nq; ++i) {
-        ctx->kstat[kKDfs].bytes += (double)qc[i].probe_bytes;
-        ctx->kstat[kKExpand].bytes += (double)qc[i].expand_bytes;
+        ctx->kstat[kKDfs].bytes += (double)qc[i].algo_bytes;
+        ctx->kstat[kKExpand].bytes += (double)qc[i].expand_algo_bytes;
     }
     return PSM_OK;
 }
@@ -1349,6 +1376,7 @@ int Batch::evaluate(uint64_t cap, double enum_limit) {
         q->capped = cap && qc[i].count >= cap;
         q->calls = qc[i].calls;
         q->probe_bytes = qc[i].probe_bytes;
+        q->algo_bytes = qc[i].algo_bytes + qc[i].expand_algo_bytes;
         q->timed_out = qc[i].timed_out != 0;
     }
     if (timed_out) set_err(ctx, PSM_ETIMEOUT, "enumeration passed the time limit");
@@ -1604,9 +1632,9 @@ int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32
         if (off[v + 1] < off[v]) return set_err(ctx, PSM_EINVAL, "offsets must be non-decreasing");
     }
     g->L = V ? maxl + 1 : 0;
-    std::vector<uint32_t> ri, nlf_off;
+    std::vector<uint32_t> ri, nlf_off, adj_bl, nlf_beg;
     std::vector<uint2> nlf;
-    build_graph_indexes(V, off, adj, lab, g->L, g->h_ri_off, ri, nlf_off, nlf);
+    build_graph_indexes(V, off, adj, lab, g->L, g->h_ri_off, ri, nlf_off, nlf, adj_bl, nlf_beg);
     for (uint32_t l = 0; l < g->L; ++l)
         g->max_freq = std::max(g->max_freq, g->h_ri_off[l + 1] - g->h_ri_off[l]);
     hipStream_t s = ctx->stream;
@@ -1622,6 +1650,8 @@ int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32
     HIP_TRY(ctx, up(g->ri, ri.data(), ri.size() * 4));
     HIP_TRY(ctx, up(g->nlf_off, nlf_off.data(), nlf_off.size() * 4));
     HIP_TRY(ctx, up(g->nlf, nlf.data(), nlf.size() * sizeof(uint2)));
+    HIP_TRY(ctx, up(g->adj_bl, adj_bl.data(), adj_bl.size() * 4));
+    HIP_TRY(ctx, up(g->nlf_beg, nlf_beg.data(), nlf_beg.size() * 4));
     HIP_TRY(ctx, hipStreamSynchronize(s));
     DevGraph& d = g->dg;
     d.V = V;
@@ -1634,6 +1664,8 @@ int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32
     d.ri = g->ri.as<uint32_t>();
     d.nlf_off = g->nlf_off.as<uint32_t>();
     d.nlf = g->nlf.as<uint2>();
+    d.adj_bl = g->adj_bl.as<uint32_t>();
+    d.nlf_beg = g->nlf_beg.as<uint32_t>();
     *out = g.release();
     return PSM_OK;
 }
@@ -1765,6 +1797,7 @@ int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
         st->kernel_ms = q->batch.t_enum;
         st->expand_ms = q->batch.t_expand;
         st->probe_bytes = q->qh.probe_bytes;
+        st->algo_bytes = q->qh.algo_bytes;
     }
     return r;
 }
@@ -1862,7 +1895,10 @@ int psm_batch_evaluate(psm_batch* B, uint64_t cap, double enum_time_limit_s, uin
         st->tasks = b.tasks_run - tasks0;
         st->dfs_lane_steps = b.lane_steps - steps0;
         st->dfs_lane_iters = b.lane_iters - iters0;
-        for (auto& h : B->hosts) st->probe_bytes += h->probe_bytes;
+        for (auto& h : B->hosts) {
+            st->probe_bytes += h->probe_bytes;
+            st->algo_bytes += h->algo_bytes;
+        }
     }
     return PSM_OK;
 }

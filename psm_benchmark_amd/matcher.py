@@ -179,6 +179,7 @@ class MatchResult:
     probe_bytes: int
     cand_counts: list = field(default_factory=list)
     timed_out: bool = False
+    algo_bytes: int = 0
 
 
 def match(dg: DeviceGraph, q: CSRGraph, method: str = "GQL_GQL_GQL", cap: int = 0) -> MatchResult:
@@ -196,7 +197,7 @@ def match(dg: DeviceGraph, q: CSRGraph, method: str = "GQL_GQL_GQL", cap: int = 
         st = run.evaluate(cap)
         return MatchResult(int(st.count), int(st.calls), int(mem), ok, [int(x) for x in order],
                            bool(st.capped), float(st.kernel_ms + st.expand_ms), int(st.probe_bytes),
-                           [int(x) for x in run.candidate_counts()], bool(st.timed_out))
+                           [int(x) for x in run.candidate_counts()], bool(st.timed_out), int(st.algo_bytes))
     finally:
         run.close()
 

@@ -27,6 +27,7 @@ class OrcResult(ctypes.Structure):
         ("order", ctypes.c_uint32 * 64),
         ("pivot", ctypes.c_uint32 * 64),
         ("depth_nodes", ctypes.c_uint64 * 64),
+        ("algo_bytes", ctypes.c_uint64),
     ]
 
 
@@ -105,6 +106,7 @@ class ORun:
     pivot = property(lambda s: [int(s.res.pivot[i]) for i in range(s.n)])
     cand_counts = property(lambda s: [int(s.res.cand_count[i]) for i in range(s.n)])
     depth_nodes = property(lambda s: [int(s.res.depth_nodes[i]) for i in range(s.n)])
+    algo_bytes = property(lambda s: int(s.res.algo_bytes))
 
     def candidates(self, u: int) -> np.ndarray:
         out = np.zeros(max(1, self.res.cand_count[u]), np.uint32)

@@ -117,6 +117,30 @@ def test_clique_closed_form_on_device(dev):
     assert r.calls == sum(math.factorial(n) // math.factorial(n - d - 1) for d in range(k))
 
 
+def test_search_bytes_clique_closed_form_on_device(dev):
+    from psm_benchmark_amd.matcher import DeviceGraph, match
+    from test_oracle import clique_search_bytes
+    n, k = 9, 5
+    kn = csr_from_edges(n, np.array(list(itertools.combinations(range(n), 2))), np.zeros(n, np.uint32))
+    kk = csr_from_edges(k, np.array(list(itertools.combinations(range(k), 2))), np.zeros(k, np.uint32))
+    dg = DeviceGraph(dev, kn)
+    assert match(dg, kk, "GQL_GQL_GQL").algo_bytes == clique_search_bytes(n, k)
+
+
+@pytest.mark.parametrize("method", ["GQL_GQL_GQL", "CFL_CFL_EXPLORE"])
+def test_search_bytes_match_oracle(hprd, method):
+    """The algorithmic bytes the enumeration kernels count (the roofline's
+    numerator) equal the oracle's sum over the same search tree, whatever
+    the schedule (expansion, lanes or wavefronts, split-off ranges)."""
+    from psm_benchmark_amd.matcher import match
+    _, dg, og = hprd
+    for qname in ("Q10-1", "Q10-2", "Q10-3", "Q10-4", "Q10-9"):
+        q = load_text(os.path.join(HPRD, qname + ".txt"))
+        r, ref = match(dg, q, method), ORun(og, OGraph(q), method)
+        print(qname, method, r.algo_bytes, ref.algo_bytes, flush=True)
+        assert r.count == ref.count and r.algo_bytes == ref.algo_bytes, qname
+
+
 def test_cap_reports_min_cap_total(hprd):
     from psm_benchmark_amd.matcher import match
     _, dg, _ = hprd

@@ -92,6 +92,35 @@ def test_clique_closed_form(n, k):
         assert r.calls == sum(math.factorial(n) // math.factorial(n - d - 1) for d in range(k))
 
 
+def clique_search_bytes(n: int, k: int) -> int:
+    """Algorithmic search bytes of a k-clique query on a one-label K_n: a
+    node at depth d (1 <= d < k) reads d table rows of n-1 entries and there
+    are n!/(n-d)! of them; the survivors at depths 1..k-2 are written."""
+    P = lambda d: math.factorial(n) // math.factorial(n - d)
+    rows = sum(P(d) * d * (n - 1) for d in range(1, k))
+    survivors = sum(P(d + 1) for d in range(1, k - 1))
+    return 4 * (rows + survivors)
+
+
+@pytest.mark.parametrize("n,k", [(6, 3), (7, 4), (8, 5)])
+def test_search_bytes_clique_closed_form(n, k):
+    kn = csr_from_edges(n, np.array(list(itertools.combinations(range(n), 2))), np.zeros(n, np.uint32))
+    kk = csr_from_edges(k, np.array(list(itertools.combinations(range(k), 2))), np.zeros(k, np.uint32))
+    for m in ("GQL_GQL_GQL", "CFL_CFL_EXPLORE", "GQL_GQL_LFTJ"):
+        assert ORun(OGraph(kn), OGraph(kk), m).algo_bytes == clique_search_bytes(n, k), m
+
+
+@pytest.mark.parametrize("qname", ["Q10-1", "Q10-2", "Q10-3", "Q10-9"])
+def test_search_bytes_independent_of_engine(hprd, qname):
+    # the measure is fixed by candidates, tables and order: the three engines
+    # walk different mechanics over the same search tree (the GQL order keeps
+    # the pivot among the backward neighbours, which all three engines need)
+    q = OGraph(load_text(os.path.join(HPRD, qname + ".txt")))
+    for f in ("GQL", "NLF"):
+        vals = {e: ORun(hprd, q, f"{f}_GQL_{e}").algo_bytes for e in ("GQL", "EXPLORE", "LFTJ")}
+        assert len(set(vals.values())) == 1 and vals["GQL"] > 0, vals
+
+
 def test_cap_is_min_cap_total(hprd):
     q = OGraph(load_text(os.path.join(HPRD, "Q10-1.txt")))
     total = ORun(hprd, q, "GQL_GQL_GQL").count
