@@ -276,6 +276,11 @@ def main():
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                     avg_launch_ms=round(dom["ms"] / dom["launches"], 4),
                     bytes_per_launch=int(dom["bytes"] / dom["launches"]))
+    if dom["name"] == "dfs":
+        roofline["bytes_definition"] = ("search-tree nodes entered by the kernel: 4 B per entry of each "
+                                        "operand row + 4 B per survivor written below the root unless "
+                                        "a leaf (kernel-counted; equals the oracle's sum when uncapped)")
+        roofline["touched_bytes_per_launch"] = int(sum(s.probe_bytes for s in stats) / args.steps)
 
     cpu = cpu_n = None
     if not args.no_cpu:

@@ -935,9 +935,13 @@ __device__ __forceinline__ bool test_element(const DPlan& P, const DOp* __restri
         }
         if (lo == hi0 || idx[lo] != x) return false;
     }
+    unsigned long long mask = P.same_label[d];
+    if (!mask) {
+        v_out = kInvalid;   // no earlier vertex shares the label: injective by construction
+        return true;
+    }
     const uint32_t v = cand_pool[P.cand_base[d] + x];
     bytes += 4;
-    unsigned long long mask = P.same_label[d];
     while (mask) {
         const uint32_t e = __ffsll((long long)mask) - 1;
         mask &= mask - 1;
@@ -1181,12 +1185,15 @@ __device__ __forceinline__ int step_cursor(const uint32_t* __restrict__ idx, uin
     return idx[c] == x ? 1 : 0;
 }
 
+// keep_v: the data vertex is needed beyond the injectivity test (a later
+// depth compares it, or embeddings are materialised); otherwise it is only
+// loaded when an earlier depth shares the label.
 __device__ __forceinline__ int test_merge(const DPlan& P, const DOp* __restrict__ ops,
                                           const uint64_t* __restrict__ row_off,
                                           const uint32_t* __restrict__ idx,
                                           const uint32_t* __restrict__ cand_pool, uint32_t d,
                                           uint32_t x, Level& L, const LaneState& st, uint32_t& v_out,
-                                          unsigned long long& bytes) {
+                                          unsigned long long& bytes, bool keep_v) {
     int r;
     if (L.nb > 0 && (r = step_cursor(idx, L.c0, L.e0, x, bytes)) != 1) return r;
     if (L.nb > 1 && (r = step_cursor(idx, L.c1, L.e1, x, bytes)) != 1) return r;
@@ -1213,9 +1220,13 @@ __device__ __forceinline__ int test_merge(const DPlan& P, const DOp* __restrict_
             if (lo == hi0 || idx[lo] != x) return 0;
         }
     }
+    unsigned long long mask = P.same_label[d];
+    if (!mask && !keep_v) {
+        v_out = kInvalid;
+        return 1;
+    }
     const uint32_t v = cand_pool[P.cand_base[d] + x];
     bytes += 4;
-    unsigned long long mask = P.same_label[d];
     while (mask) {
         const uint32_t e = __ffsll((long long)mask) - 1;
         mask &= mask - 1;
@@ -1368,7 +1379,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 lbytes += L.sel == kIdentity ? 0 : 4;
                 ++L.cur;
                 uint32_t v;
-                const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes);
+                const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes, false);
                 if (r == 2) {
                     L.cur = L.end;
                 } else if (r == 1) {
@@ -1381,7 +1392,8 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             lbytes += L.sel == kIdentity ? 0 : 4;
             ++L.cur;
             uint32_t v;
-            const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes);
+            const bool keep_v = sink.rows != nullptr || ((P->m_needed >> d) & 1ull);
+            const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes, keep_v);
             if (r == 2) {
                 L.cur = L.end;   // an operand ran out: nothing further at this level
             } else if (r == 1) {
@@ -1594,8 +1606,10 @@ __device__ __forceinline__ uint64_t wave_build_cache(const DPlan& P, const DOp* 
         uint32_t x = 0, v = 0;
         if (g > t) {
             x = S.cx[g - 1][S.run_lo[g - 1] + lane];
-            v = cand_pool[P.cand_base[g - 1] + x];
-            bytes += 4;
+            if ((P.m_needed >> (g - 1)) & 1ull) {
+                v = cand_pool[P.cand_base[g - 1] + x];
+                bytes += 4;
+            }
         }
         const uint32_t nops = P.op_count[g];
         uint64_t lo = 0;
@@ -1830,7 +1844,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                         ok = row_has(idx, a, b - a, xx, bytes);
                     }
                 }
-                if (ok) {
+                if (ok && P.same_label[g]) {
                     const uint32_t v = cand_pool[P.cand_base[g] + xx];
                     bytes += 4;
                     unsigned long long mask = P.same_label[g];

@@ -90,6 +90,7 @@ struct DPlan {
     uint32_t n;
     uint32_t pad;
     uint64_t cap;
+    uint64_t m_needed;            // depths whose data vertex a later depth compares
     uint64_t cand_base[kMaxQ];
     uint32_t cand_len[kMaxQ];
     uint32_t op_begin[kMaxQ];

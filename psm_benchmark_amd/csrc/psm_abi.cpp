@@ -1194,6 +1194,7 @@ void make_plan(const QueryHost& q, const std::vector<uint32_t>& order,
         for (uint32_t e = 0; e < d; ++e)
             if (q.qg.lab[order[e]] == q.qg.lab[u]) msk |= 1ull << e;
         P.same_label[d] = msk;
+        P.m_needed |= msk;
     }
 }
 
