@@ -800,7 +800,8 @@ __global__ __launch_bounds__(kBlock) void k_table_fill_long(
 // CFL support counting (FilterVertices.cpp:841-966 restated with set
 // semantics): cnt[w] advances from `expect` to `next` once per pivot for a
 // data vertex w with the query vertex's label and degree that neighbours a
-// live candidate of that pivot.
+// live candidate of that pivot. A pivot candidate's scan walks only the run
+// of that label in the label-grouped adjacency.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cfl_mark(const DevGraph& g, uint32_t w, uint32_t label, uint32_t degree,
                                          uint32_t* __restrict__ cnt, uint32_t expect, uint32_t next) {
@@ -814,7 +815,7 @@ __device__ __forceinline__ void cfl_mark(const DevGraph& g, uint32_t w, uint32_t
 }
 
 // Thread per pivot candidate; candidates with more than kRowCoop neighbours
-// are listed for k_cfl_support_long (one atomic per wavefront).
+// of the label are listed for k_cfl_support_long (one atomic per wavefront).
 __global__ __launch_bounds__(kBlock) void k_cfl_support(
     DevGraph g, const uint32_t* __restrict__ cand_pool, uint64_t cand_begin, uint32_t len,
     uint32_t label, uint32_t degree, uint32_t* __restrict__ cnt, uint32_t expect, uint32_t next,
@@ -827,10 +828,11 @@ __global__ __launch_bounds__(kBlock) void k_cfl_support(
         if (j < len) {
             v = cand_pool[cand_begin + j];
             if (v != kInvalid) {
-                const uint32_t vb = g.off[v], ve = g.off[v + 1];
-                hub = ve - vb > kRowCoop;
+                const uint2 run = label_run(g, v, label);
+                hub = run.y > kRowCoop;
                 if (!hub)
-                    for (uint32_t p = vb; p < ve; ++p) cfl_mark(g, g.adj[p], label, degree, cnt, expect, next);
+                    for (uint32_t p = run.x; p < run.x + run.y; ++p)
+                        cfl_mark(g, g.adj_bl[p], label, degree, cnt, expect, next);
             }
         }
         const unsigned long long bal = __ballot(hub);
@@ -853,8 +855,9 @@ __global__ __launch_bounds__(kBlock) void k_cfl_support_long(
     const uint32_t waves = gridDim.x * (kBlock / kWave);
     for (uint32_t i = (blockIdx.x * kBlock + threadIdx.x) / kWave; i < n; i += waves) {
         const uint32_t v = defer[i];
-        const uint32_t vb = g.off[v], ve = g.off[v + 1];
-        for (uint32_t p = vb + lane; p < ve; p += kWave) cfl_mark(g, g.adj[p], label, degree, cnt, expect, next);
+        const uint2 run = label_run(g, v, label);
+        for (uint32_t p = run.x + lane; p < run.x + run.y; p += kWave)
+            cfl_mark(g, g.adj_bl[p], label, degree, cnt, expect, next);
     }
 }
 

@@ -34,6 +34,10 @@ using namespace psm;
 // ---------------------------------------------------------------------------
 // context, errors, device buffers
 // ---------------------------------------------------------------------------
+// The library's default enumeration launch shape (what the plain path uses).
+constexpr uint32_t kDefaultWavesPerCu = 8;
+constexpr double kDefaultTasksPerLane = 2.0;
+
 struct psm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -48,8 +52,8 @@ struct psm_ctx {
     unsigned long long* d_kbytes = nullptr;   // kernel-counted algorithmic bytes per class
     double time_limit_s = 60.0;               // reference default -time_limit 60
     // enumeration launch shape (psm_set_tuning)
-    uint32_t dfs_waves_per_cu = 8;            // persistent DFS grid
-    double expand_tasks_per_lane = 2.0;       // pre-expansion target
+    uint32_t dfs_waves_per_cu = kDefaultWavesPerCu;        // persistent DFS grid
+    double expand_tasks_per_lane = kDefaultTasksPerLane;   // pre-expansion target
     uint32_t dfs_strategy = PSM_DFS_LANE;     // lane-per-task or wavefront-per-task DFS
     double wall_khz = 100000.0;               // device wall clock rate
     bool verbose = false;
@@ -1980,6 +1984,8 @@ uint64_t footprint(const psm_graph* g, const QGraph& q) {
     return b + (64ull << 20);
 }
 
+constexpr size_t kSmallBatch = 4;   // batches below this size use the default launch shape
+
 void serve_worker(psm_server* S, psm_ctx* ctx) {
     (void)hipSetDevice(S->device);
     while (true) {
@@ -2012,10 +2018,24 @@ void serve_worker(psm_server* S, psm_ctx* ctx) {
         b.ctx = ctx;
         b.g = S->g;
         for (Request& r : batch) b.qs.push_back(r.q.get());
+        // A batch of a few requests runs with the library's default launch
+        // shape (the plain path's): the served shape is tuned for full
+        // batches, and its chip-wide pre-expansion of one query would cost
+        // more than the query itself.
+        const uint32_t tuned_waves = ctx->dfs_waves_per_cu, tuned_strategy = ctx->dfs_strategy;
+        const double tuned_expand = ctx->expand_tasks_per_lane;
+        if (batch.size() < kSmallBatch) {
+            ctx->dfs_waves_per_cu = kDefaultWavesPerCu;
+            ctx->expand_tasks_per_lane = kDefaultTasksPerLane;
+            ctx->dfs_strategy = PSM_DFS_LANE;
+        }
         int rc = b.filter(S->filter);
         if (rc == PSM_OK) rc = b.build_tables();
         if (rc == PSM_OK) rc = b.make_plans(S->order);
         if (rc == PSM_OK) rc = b.evaluate(S->cap);
+        ctx->dfs_waves_per_cu = tuned_waves;
+        ctx->expand_tasks_per_lane = tuned_expand;
+        ctx->dfs_strategy = tuned_strategy;
         const double end = now_ms() - S->t0;
         std::lock_guard<std::mutex> lk(S->mu);
         for (Request& r : batch) {
