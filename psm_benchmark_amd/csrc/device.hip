@@ -1253,7 +1253,7 @@ __device__ __forceinline__ bool past_deadline(DQueue* Q) {
                __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-constexpr uint32_t kLeafBurst = 32;   // leaf-depth elements one step of the lane DFS tests
+constexpr uint32_t kLeafBurst = 32;   // base elements one step of the lane DFS tests at most
 
 // Persistent DFS. Every lane runs its own depth-first search over one task at
 // a time. Idle lanes claim the next queued tasks (one claim per wavefront).
@@ -1370,62 +1370,60 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         if (!active) continue;
         ++busy_iters;
 
-        // one step of this lane's search
+        // one step of this lane's search: a burst of up to kLeafBurst tests of
+        // base elements of the current level. A lane at the leaf depth counts
+        // the valid ones; a lane above it stops at its first valid one and
+        // descends after the burst. Lanes at any depth thus skip failing
+        // elements at the same pace, and descents happen together.
         bool finished = false;
-        if (L.cur < L.end && d + 1 == n && !sink.rows) {
-            // leaf depth: test up to kLeafBurst base elements in one step and
-            // count the valid ones (each is one assignment); the rest of the
-            // range stays splittable by the donation below
+        if (L.cur < L.end) {
+            const bool count_leaf = d + 1 == n && !sink.rows;
+            const bool keep_v = sink.rows != nullptr || ((P->m_needed >> d) & 1ull);
+            bool found = false;
+            uint32_t fx = 0, fv = 0;
             uint32_t burst = kLeafBurst;
             while (L.cur < L.end && burst--) {
                 const uint32_t x = L.sel == kIdentity ? (uint32_t)L.cur : idx[L.cur];
                 lbytes += L.sel == kIdentity ? 0 : 4;
                 ++L.cur;
                 uint32_t v;
-                const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes, false);
+                const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes, keep_v);
                 if (r == 2) {
-                    L.cur = L.end;
+                    L.cur = L.end;   // an operand ran out: nothing further at this level
                 } else if (r == 1) {
                     ++lcalls;
-                    ++lcount;
+                    if (count_leaf) {
+                        ++lcount;
+                    } else {
+                        found = true;
+                        fx = x;
+                        fv = v;
+                        break;
+                    }
                 }
             }
-        } else if (L.cur < L.end) {
-            const uint32_t x = L.sel == kIdentity ? (uint32_t)L.cur : idx[L.cur];
-            lbytes += L.sel == kIdentity ? 0 : 4;
-            ++L.cur;
-            uint32_t v;
-            const bool keep_v = sink.rows != nullptr || ((P->m_needed >> d) & 1ull);
-            const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes, keep_v);
-            if (r == 2) {
-                L.cur = L.end;   // an operand ran out: nothing further at this level
-            } else if (r == 1) {
-                ++lcalls;
-                if (d + 1 == n) {
-                    ++lcount;
-                    if (sink.rows) {
-                        // materialise the embedding (data vertex ids in depth order)
-                        const unsigned long long row = atomicAdd(sink.count, 1ull);
-                        if (row < sink.cap) {
-                            uint32_t* out = sink.rows + row * n;
-                            for (uint32_t e = 0; e < d; ++e) out[e] = st.M_[st.at(e)];
-                            out[d] = v;
-                        } else {
-                            atomicExch(sink.overflow, 1u);
-                        }
-                    }
+            if (found && d + 1 == n) {
+                ++lcount;
+                // materialise the embedding (data vertex ids in depth order)
+                const unsigned long long row = atomicAdd(sink.count, 1ull);
+                if (row < sink.cap) {
+                    uint32_t* out = sink.rows + row * n;
+                    for (uint32_t e = 0; e < d; ++e) out[e] = st.M_[st.at(e)];
+                    out[d] = fv;
                 } else {
-                    const uint32_t a = st.at(d);
-                    st.cur_[a] = (uint32_t)(L.cur - L.base0);
-                    st.end_[a] = (uint32_t)(L.end - L.base0);
-                    st.sel_[a] = L.sel == kIdentity ? kSelIdentity : (uint8_t)L.sel;
-                    st.I_[a] = x;
-                    st.M_[a] = v;
-                    if (d >= 1) lalgo += 4;   // a survivor written below the root level
-                    ++d;
-                    if (d + 1 == n) Lp = L;   // restored on the way back up
-                    enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo);
+                    atomicExch(sink.overflow, 1u);
                 }
+            } else if (found) {
+                const uint32_t a = st.at(d);
+                st.cur_[a] = (uint32_t)(L.cur - L.base0);
+                st.end_[a] = (uint32_t)(L.end - L.base0);
+                st.sel_[a] = L.sel == kIdentity ? kSelIdentity : (uint8_t)L.sel;
+                st.I_[a] = fx;
+                st.M_[a] = fv;
+                if (d >= 1) lalgo += 4;   // a survivor written below the root level
+                ++d;
+                if (d + 1 == n) Lp = L;   // restored on the way back up
+                enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo);
             }
         } else if (d == tdepth) {
             finished = true;
