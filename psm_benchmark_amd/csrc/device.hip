@@ -2444,7 +2444,16 @@ hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_of
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dfs),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    const uint32_t blocks = std::max<uint32_t>(1, grid_waves / (lanes / kWave));
+    uint32_t blocks = std::max<uint32_t>(1, grid_waves / (lanes / kWave));
+    // a persistent grid larger than what the CUs hold at once only adds
+    // blocks that start after the work has drained: keep it resident
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_dfs), (int)lanes,
+                                                     lds) == hipSuccess &&
+        hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 &&
+        cus > 0)
+        blocks = std::min<uint32_t>(blocks, (uint32_t)(per_cu * cus));
     hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(lanes), lds, s, plans, ops, row_off, idx, cand_pool, queue,
                        ready, dq, qc, pending, max_n, sink);
     return hipGetLastError();
