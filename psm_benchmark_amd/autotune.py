@@ -8,11 +8,13 @@ phase hands to the DFS phase). They trade occupancy against memory-system
 contention and load balance, and the best point depends on the data graph
 and the query mix. The tuner prepares (filter, tables, plans) a sample of
 the workload once, runs its enumeration for each candidate setting under a
-short time limit and keeps the setting with the highest search throughput
-(assignments per millisecond of enumeration, the reference's "Call Count"
-rate), which is comparable across settings whether or not a trial runs into
-the limit. Results never depend on the setting: only the launch shape
-changes.
+short time limit and keeps the best by `trial_key`: when a trial finishes
+every query of the sample, the shortest enumeration wins (with per-query
+caps, a wider search overshoots the caps and does more work for the same
+counts, so search throughput would pick a slower setting); trials that run
+into the limit rank below those and by search throughput among themselves
+(assignments per millisecond, the reference's "Call Count" rate). Results
+never depend on the setting: only the launch shape changes.
 """
 from __future__ import annotations
 
@@ -54,6 +56,13 @@ def set_config(dev, cfg: dict) -> None:
         r = L.lib().psm_set_tuning(dev.h, KEYS[name], float(value))
         if r != L.PSM_OK:
             raise L.PSMError(r, f"psm_set_tuning {name}={value}: {L.lib().psm_last_error(dev.h).decode()}")
+
+
+def trial_key(t: dict):
+    """Sort key of a trial (smaller is better): fewer timed-out queries
+    first; then, when none timed out, the shorter enumeration; otherwise the
+    higher search throughput."""
+    return (t["timeouts"], t["enum_ms"] if t["timeouts"] == 0 else -t["nodes_per_ms"])
 
 
 def tune(dg, queries, filter: str = "GQL", order: str = "GQL", cap: int = 0, sample: int = 128,
@@ -100,7 +109,7 @@ def tune(dg, queries, filter: str = "GQL", order: str = "GQL", cap: int = 0, sam
     finally:
         b.close()
         L.lib().psm_set_time_limit(dev.h, restore_limit_s)
-    best = max(trials, key=lambda t: t["nodes_per_ms"])["config"]
+    best = min(trials, key=trial_key)["config"]
     set_config(dev, best)
     if cache:
         with open(cache, "w") as fh:

@@ -1293,6 +1293,8 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     Level Lp{};   // the level above the leaf depth, kept while the lane is at the leaf
     unsigned long long lcount = 0, lcalls = 0, lbytes = 0, lalgo = 0;
     unsigned long long busy_iters = 0, all_iters = 0;
+    unsigned long long flush_at = 4096;   // leaf counts a lane holds before adding them to its query
+    bool capped_q = false;
     uint32_t steps = 0;
     const DPlan* P = plans;
     const uint32_t hungry_below = gridDim.x * (blockDim.x / 2);   // queue shorter than half the lanes
@@ -1357,6 +1359,11 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 }
                 active = true;
                 steps = 0;
+                // a capped query learns its running count sooner (smaller
+                // flushes, more frequent checks): every lane's unflushed
+                // embeddings and unchecked steps are searched past the cap
+                capped_q = P->cap != ~0ull;
+                flush_at = capped_q ? min(4096ull, max(16ull, P->cap >> 12)) : 4096ull;
             }
         }
         if (__ballot(active || waiting) == 0) {
@@ -1444,7 +1451,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
         }
         ++steps;
-        if (active && (steps & 255) == 0) {
+        if (active && (steps & (capped_q ? 31u : 255u)) == 0) {
             if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout)) {
                 active = false;
             } else if (past_deadline(Q)) {
@@ -1494,7 +1501,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 }
             }
         }
-        const bool flush = !active || lcount >= 4096 || (steps & 1023) == 0;
+        const bool flush = !active || lcount >= flush_at || (steps & 1023) == 0;
         if (flush && (lcount | lcalls | lbytes)) {
             const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
             atomicAdd(&qc[q].calls, lcalls);
@@ -1909,7 +1916,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             __syncthreads();
 
             ++it;
-            if ((it & 15) == 0 || lcount >= 65536) {
+            if ((it & 15) == 0 || lcount >= (P.cap != ~0ull ? min(65536ull, max(64ull, P.cap >> 10)) : 65536ull)) {
                 uint32_t stop = 0;
                 if (lane == 0) {
                     if (lcount | lcalls) {

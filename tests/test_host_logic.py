@@ -111,3 +111,14 @@ def test_pmc_summary_scales_counters_by_the_copy_calibration(tmp_path):
     assert r["fetch_kib"] == 500.0 and r["write_kib"] == 50.0
     assert r["scale_fetch"] == 2.0 and r["scale_write"] == 1.0
     assert r["traffic_bytes"] == 500 * 1024 * 2 + 50 * 1024
+
+
+def test_autotune_trial_order():
+    # completed trials rank by enumeration time (a wider search overshoots
+    # per-query caps), time-limited ones by search throughput, after them
+    from psm_benchmark_amd.autotune import trial_key
+    trials = [dict(name="fast_rate", timeouts=0, enum_ms=80.0, nodes_per_ms=9e6),
+              dict(name="short", timeouts=0, enum_ms=50.0, nodes_per_ms=5e6),
+              dict(name="limited_hi", timeouts=2, enum_ms=800.0, nodes_per_ms=9e6),
+              dict(name="limited_lo", timeouts=2, enum_ms=800.0, nodes_per_ms=1e6)]
+    assert [t["name"] for t in sorted(trials, key=trial_key)] == ["short", "fast_rate", "limited_hi", "limited_lo"]
