@@ -122,3 +122,14 @@ def test_autotune_trial_order():
               dict(name="limited_hi", timeouts=2, enum_ms=800.0, nodes_per_ms=9e6),
               dict(name="limited_lo", timeouts=2, enum_ms=800.0, nodes_per_ms=1e6)]
     assert [t["name"] for t in sorted(trials, key=trial_key)] == ["short", "fast_rate", "limited_hi", "limited_lo"]
+
+
+def test_autotune_pick_prefers_throughput_among_near_ties():
+    from psm_benchmark_amd.autotune import pick
+    trials = [dict(name="short", timeouts=0, enum_ms=100.0, nodes_per_ms=5e5),
+              dict(name="tie_fast", timeouts=0, enum_ms=108.0, nodes_per_ms=7e5),
+              dict(name="slow_fast", timeouts=0, enum_ms=150.0, nodes_per_ms=9e5)]
+    assert pick(trials)["name"] == "tie_fast"
+    limited = [dict(name="a", timeouts=3, enum_ms=800.0, nodes_per_ms=2e5),
+               dict(name="b", timeouts=3, enum_ms=800.0, nodes_per_ms=4e5)]
+    assert pick(limited)["name"] == "b"
