@@ -2091,8 +2091,11 @@ int psm_server_create(psm_ctx* ctx, const psm_graph* g, uint32_t workers, uint32
         }
         c->time_limit_s = time_limit_s;
         c->verbose = ctx->verbose;
-        // workers run with the creating context's launch shape
-        c->dfs_waves_per_cu = ctx->dfs_waves_per_cu;
+        // workers run with the creating context's launch shape, the
+        // persistent DFS grid split between them: batches of different
+        // workers overlap on the device, and grids sized for the whole chip
+        // would queue behind one another instead of sharing the CUs
+        c->dfs_waves_per_cu = std::max<uint32_t>(1, ctx->dfs_waves_per_cu / workers);
         c->expand_tasks_per_lane = ctx->expand_tasks_per_lane;
         c->dfs_strategy = ctx->dfs_strategy;
         S->ctxs.push_back(c);
