@@ -1702,7 +1702,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* ready, DQueue* Q,
-    QCounters* qc, uint32_t* pending, uint32_t max_n) {
+    QCounters* qc, uint32_t* pending) {
     const uint32_t lane = threadIdx.x;
     __shared__ WaveFrame<N> S;
     if (lane == 0 && Q->limit_ticks)
@@ -2478,13 +2478,13 @@ hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* r
     if (max_n == 0 || max_n > kMaxQ) return hipErrorInvalidValue;
     if (max_n <= 16)
         hipLaunchKernelGGL(k_dfs_wave<16>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, queue, ready, dq, qc, pending, max_n);
+                           cand_pool, queue, ready, dq, qc, pending);
     else if (max_n <= 32)
         hipLaunchKernelGGL(k_dfs_wave<32>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, queue, ready, dq, qc, pending, max_n);
+                           cand_pool, queue, ready, dq, qc, pending);
     else
         hipLaunchKernelGGL(k_dfs_wave<64>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, queue, ready, dq, qc, pending, max_n);
+                           cand_pool, queue, ready, dq, qc, pending);
     return hipGetLastError();
 }
 

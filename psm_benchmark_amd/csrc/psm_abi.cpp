@@ -1624,7 +1624,7 @@ int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32
                      const uint32_t* lab, psm_graph** out) {
     if (!ctx || !out || !off || !lab || (off[V] && !adj)) return PSM_EINVAL;
     *out = nullptr;
-    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
     tls_stream = nullptr;   // graph buffers outlive batches: release with a device sync
     std::unique_ptr<psm_graph> g(new psm_graph());
     g->ctx = ctx;
@@ -1713,7 +1713,7 @@ int psm_query_destroy(psm_query* q) {
 
 int psm_filter(psm_query* q, int method) {
     if (!q) return PSM_EINVAL;
-    hipSetDevice(q->ctx->device);
+    HIP_TRY(q->ctx, hipSetDevice(q->ctx->device));
     PSM_TRY(q->batch.filter(method));
     q->filtered = true;
     return q->qh.status;
@@ -1741,7 +1741,7 @@ int psm_query_candidates(const psm_query* q, uint32_t u, uint32_t* out) {
 int psm_build_tables(psm_query* q) {
     if (!q) return PSM_EINVAL;
     if (!q->filtered) return set_err(q->ctx, PSM_ESTATE, "filter before building tables");
-    hipSetDevice(q->ctx->device);
+    HIP_TRY(q->ctx, hipSetDevice(q->ctx->device));
     return q->batch.build_tables();
 }
 
@@ -1780,7 +1780,7 @@ int psm_memory_cost_bytes(const psm_query* q, uint64_t* bytes) {
 
 int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t* pivot) {
     if (!q) return PSM_EINVAL;
-    hipSetDevice(q->ctx->device);
+    HIP_TRY(q->ctx, hipSetDevice(q->ctx->device));
     PSM_TRY(q->batch.make_plans(order_method));
     if (order) std::copy(q->qh.plan.order.begin(), q->qh.plan.order.end(), order);
     if (pivot) std::copy(q->qh.plan.pivot.begin(), q->qh.plan.pivot.end(), pivot);
@@ -1789,7 +1789,7 @@ int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t*
 
 int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     if (!q) return PSM_EINVAL;
-    hipSetDevice(q->ctx->device);
+    HIP_TRY(q->ctx, hipSetDevice(q->ctx->device));
     q->batch.t_enum = q->batch.t_expand = 0;
     PSM_TRY(q->batch.evaluate(cap));
     const int r = q->qh.timed_out ? PSM_ETIMEOUT : PSM_OK;
@@ -1809,7 +1809,7 @@ int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
 
 int psm_join_count(psm_query* q, uint64_t max_rows, psm_join_stats* st) {
     if (!q || !st) return PSM_EINVAL;
-    (void)hipSetDevice(q->ctx->device);
+    HIP_TRY(q->ctx, hipSetDevice(q->ctx->device));
     JoinResult jr;
     PSM_TRY(q->batch.join_count(q->qh, max_rows ? max_rows : (64ull << 20), jr));
     st->count = jr.count;
@@ -1846,7 +1846,7 @@ int psm_batch_create(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32
                      const uint32_t* q_adj, const uint32_t* q_lab, psm_batch** out) {
     if (!ctx || !g || !n || !off_base || !nbr_base || !lab_base || !q_off || !q_lab || !out) return PSM_EINVAL;
     *out = nullptr;
-    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
     std::unique_ptr<psm_batch> B(new psm_batch());
     B->ctx = ctx;
     B->t0 = now_ms();
@@ -1866,7 +1866,7 @@ int psm_batch_create(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32
 
 int psm_batch_prepare(psm_batch* B, int filter, int order) {
     if (!B) return PSM_EINVAL;
-    (void)hipSetDevice(B->ctx->device);
+    HIP_TRY(B->ctx, hipSetDevice(B->ctx->device));
     B->prepared = false;
     PSM_TRY(B->b.filter(filter));
     PSM_TRY(B->b.build_tables());
@@ -1879,7 +1879,7 @@ int psm_batch_evaluate(psm_batch* B, uint64_t cap, double enum_time_limit_s, uin
                        uint32_t* status, psm_batch_stats* st) {
     if (!B) return PSM_EINVAL;
     if (!B->prepared) return set_err(B->ctx, PSM_ESTATE, "prepare the batch before evaluating");
-    (void)hipSetDevice(B->ctx->device);
+    HIP_TRY(B->ctx, hipSetDevice(B->ctx->device));
     Batch& b = B->b;
     const double e0 = b.t_enum + b.t_expand;
     const uint64_t steps0 = b.lane_steps, iters0 = b.lane_iters, tasks0 = b.tasks_run;
