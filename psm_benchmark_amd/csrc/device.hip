@@ -1460,7 +1460,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
         }
         // donation: split the shallowest level that still has >= 2 entries
-        if (active && (steps & 63) == 0 &&
+        if (active && (steps & 15) == 0 &&
             load_relaxed(&Q->tail) - load_relaxed(&Q->head) < hungry_below) {
             uint32_t e = tdepth;
             uint64_t ce = 0, ee = 0;
