@@ -386,18 +386,25 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
 }
 
 // The wavefront test over the deferred (item, index) pairs of class 3; the
-// pair count is read from device memory (written by the 8-lane kernel), and
-// a fixed grid strides over the list.
+// pair count is read from device memory (written by the 8-lane kernel).
+// Wavefronts take pairs one at a time from a shared cursor (zero before the
+// launch): a candidate's cost follows the length of its label runs, which is
+// heavy-tailed on power-law graphs, so a fixed striding would leave the
+// wavefronts that drew hubs finishing long after the rest.
 __global__ __launch_bounds__(kBlock) void k_gql_refine(
     DevGraph g, const GItem* __restrict__ items, const uint2* __restrict__ list,
-    const uint32_t* __restrict__ n_list, const uint32_t* __restrict__ left_slots,
+    const uint32_t* __restrict__ n_list, uint32_t* __restrict__ cursor, const uint32_t* __restrict__ left_slots,
     const uint32_t* __restrict__ left_labs, uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps,
     uint64_t wps, unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
     __shared__ GqlShared sh;
     const uint32_t n = *n_list;
     const uint32_t wl = threadIdx.x / kWave;
     unsigned long long by = 0;
-    for (uint32_t w = blockIdx.x * kGqlWavesPerBlock + wl; w < n; w += gridDim.x * kGqlWavesPerBlock) {
+    while (true) {
+        uint32_t w = 0;
+        if (lane_id() == 0) w = atomicAdd(cursor, 1u);
+        w = __shfl(w, 0);
+        if (w >= n) break;
         const uint2 e = list[w];
         by += gql_refine_one(g, items[e.x], e.y, left_slots, left_labs, cand_pool, bitmaps, wps, member, sh, wl);
     }
@@ -2288,14 +2295,14 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
 constexpr uint32_t kGqlGrid = 2048;
 
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, const uint2* list, const uint32_t* n_list,
-                             uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_labs,
+                             uint32_t* cursor, uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_labs,
                              uint32_t* cand_pool, uint32_t* bitmaps,
                              uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
                              hipStream_t s) {
     if (max_n == 0) return hipSuccess;
     // six blocks per CU fit the LDS of gfx950's 256 CUs (26 KB per block)
     const uint32_t blocks = std::min<uint32_t>(1536, (max_n + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock);
-    hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
+    hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, cursor, left_slots,
                        left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
     return hipGetLastError();
 }

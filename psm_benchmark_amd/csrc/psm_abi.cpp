@@ -703,8 +703,10 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
     // deferred pairs of one (round, p): class c at c * max_cands; reused by
     // every launch group (stream order)
     HIP_TRY(ctx, d_lists.alloc(4ull * max_cands * sizeof(uint2)));
-    // deferral counters, four per (round, p), cleared by a fill kernel
-    const uint64_t n_counters = 2ull * max_n * 4;
+    // per (round, p): the four deferral counters and the wavefront kernel's
+    // cursor (eight slots), cleared by a fill kernel
+    constexpr uint64_t kCounterSlots = 8;
+    const uint64_t n_counters = 2ull * max_n * kCounterSlots;
     HIP_TRY(ctx, d_nlists.alloc(n_counters * 4));
     HIP_TRY(ctx, launch_fill_u32(d_nlists.as<uint32_t>(), 0u, n_counters, s()));
     // Launch groups follow one another on the stream without host syncs: the
@@ -714,7 +716,7 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
             const uint32_t ni = item_begin[p + 1] - item_begin[p];
             if (!ni) continue;
             const GItem* its = d_items.as<GItem>() + item_begin[p];
-            uint32_t* cnt = d_nlists.as<uint32_t>() + ((uint64_t)round * max_n + p) * 4;
+            uint32_t* cnt = d_nlists.as<uint32_t>() + ((uint64_t)round * max_n + p) * kCounterSlots;
             uint2* lists = d_lists.as<uint2>();
             TIMED(ctx, kKGqlRefine,
                   launch_gql_refine_small(g->dg, its, ni, cands[p], d_left.as<uint32_t>(), d_labs.as<uint32_t>(),
@@ -728,7 +730,7 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
                                       d_left.as<uint32_t>(), d_labs.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
                                       member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
             TIMED(ctx, kKGqlRefine,
-                  launch_gql_refine(g->dg, its, lists + 3ull * max_cands, cnt + 3, cands[p], d_left.as<uint32_t>(),
+                  launch_gql_refine(g->dg, its, lists + 3ull * max_cands, cnt + 3, cnt + 4, cands[p], d_left.as<uint32_t>(),
                                     d_labs.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
                                     member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
         }
@@ -738,7 +740,7 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
         HIP_TRY(ctx, download(nd, d_nlists.p, n_counters, s()));
         for (int round = 0; round < 2; ++round)
             for (uint32_t p = 0; p < max_n; ++p) {
-                const uint32_t* c = nd.data() + ((uint64_t)round * max_n + p) * 4;
+                const uint32_t* c = nd.data() + ((uint64_t)round * max_n + p) * kCounterSlots;
                 verbose("gql round %d u %u: %u candidates, deferred %u / %u / %u / %u", round, p, cands[p], c[0],
                         c[1], c[2], c[3]);
             }
