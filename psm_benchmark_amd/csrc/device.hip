@@ -264,7 +264,7 @@ constexpr uint32_t kScanUnroll = 4;   // 64-neighbour chunks a wavefront loads b
 // bytes (on every lane). wl: the wavefront's index in its block.
 __device__ __forceinline__ unsigned long long gql_refine_one(
     const DevGraph& g, const GItem& it, uint32_t j, const uint32_t* __restrict__ left_slots,
-    const uint32_t* __restrict__ left_labs, uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
+    const uint32_t* __restrict__ left_lab, const uint32_t* __restrict__ left_labs, uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
     unsigned long long* __restrict__ member, GqlShared& sh, uint32_t wl) {
     const uint32_t lane = lane_id();
     if (j >= it.len) return 0;
@@ -273,18 +273,22 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
     const uint32_t k = it.k;
     const uint32_t* lslots = left_slots + it.left_begin;   // uniform: scalar loads
     const uint32_t* labs = left_labs + it.lab_begin;
-    // Right vertices can only be neighbours carrying a left vertex's label:
-    // the scans below walk those label runs of the label-grouped adjacency.
+    // Right vertices can only be neighbours carrying a left vertex's label,
+    // and the neighbours of one label can only serve the left vertices of
+    // that label (candidate sets are label-pure): the scans below walk the
+    // label runs of the label-grouped adjacency one label at a time.
     // pass 1: lane i (< k) counts the right neighbours of left vertex i,
-    // saturating at k. Once every left vertex has k right neighbours a
-    // saturating matching exists (each left vertex in turn still finds an
-    // unused one), so the scan stops there.
+    // saturating at k. A left vertex with k right neighbours can always be
+    // matched last (each in turn still finds an unused one), so a run is left
+    // as soon as every left vertex of its label has k; a left vertex whose
+    // run holds none fails the candidate at once.
+    const uint32_t my_lab = lane < k ? left_lab[it.left_begin + lane] : kInvalid;
     uint32_t my_cnt = 0;
-    uint32_t scanned = 0, relevant = 0;
-    bool saturated = false;
-    for (uint32_t li = 0; li < it.nlab && !saturated; ++li) {
+    uint32_t scanned = 0, rescanned = 0;
+    bool ok = true;
+    for (uint32_t li = 0; li < it.nlab && ok; ++li) {
+        const bool mine = my_lab == labs[li];
         const uint2 run = label_run(g, v, labs[li]);
-        relevant += run.y;
         // kScanUnroll chunks per step: their neighbour and membership loads
         // are independent and in flight together
         for (uint32_t o = 0; o < run.y; o += kScanUnroll * kWave) {
@@ -307,20 +311,20 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
                 if (lane == i) my_cnt = min(my_cnt + cnt, k);
             }
             scanned += min(kScanUnroll * kWave, run.y - o);
-            if (__ballot(lane < k && my_cnt < k) == 0) {
-                saturated = true;
-                break;
-            }
+            if (__ballot(mine && my_cnt < k) == 0) break;
         }
+        if (__ballot(mine && my_cnt == 0)) ok = false;
     }
-    bool ok = __ballot(lane < k && my_cnt == 0) == 0;
-    const uint32_t T = (uint32_t)__ballot(lane < k && my_cnt < k);
+    const uint32_t T = ok ? (uint32_t)__ballot(lane < k && my_cnt < k) : 0u;
     const bool pass2 = ok && T;
     if (pass2) {
-        // pass 2: right vertices adjacent to T, compacted into LDS
+        // pass 2: right vertices adjacent to T (only the runs of T's labels),
+        // compacted into LDS
         uint32_t nr = 0;
         for (uint32_t li = 0; li < it.nlab; ++li) {
+            if (__ballot(((T >> lane) & 1u) && my_lab == labs[li]) == 0) continue;
             const uint2 run = label_run(g, v, labs[li]);
+            rescanned += run.y;
             for (uint32_t o = 0; o < run.y; o += kWave) {
                 const uint32_t p = o + lane;
                 const uint32_t m = p < run.y ? (left_mask(member, it, lslots, k, g.adj_bl[run.x + p]) & T) : 0u;
@@ -381,8 +385,7 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
     // algorithmic bytes: candidate, one run lookup per left label, the
     // scanned part of the label runs, one membership word per (neighbour,
     // left vertex) probe; pass 2 rescans the runs when it runs
-    return 4ull + 16ull * it.nlab * (pass2 ? 2 : 1) + 4ull * scanned * (1 + k) +
-           (pass2 ? 4ull * relevant * (1 + k) : 0ull);
+    return 4ull + 16ull * it.nlab * (pass2 ? 2 : 1) + 4ull * (scanned + rescanned) * (1 + k);
 }
 
 // The wavefront test over the deferred (item, index) pairs of class 3; the
@@ -394,7 +397,8 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
 __global__ __launch_bounds__(kBlock) void k_gql_refine(
     DevGraph g, const GItem* __restrict__ items, const uint2* __restrict__ list,
     const uint32_t* __restrict__ n_list, uint32_t* __restrict__ cursor, const uint32_t* __restrict__ left_slots,
-    const uint32_t* __restrict__ left_labs, uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps,
+    const uint32_t* __restrict__ left_lab, const uint32_t* __restrict__ left_labs,
+    uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps,
     uint64_t wps, unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
     __shared__ GqlShared sh;
     const uint32_t n = *n_list;
@@ -406,7 +410,8 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
         w = __shfl(w, 0);
         if (w >= n) break;
         const uint2 e = list[w];
-        by += gql_refine_one(g, items[e.x], e.y, left_slots, left_labs, cand_pool, bitmaps, wps, member, sh, wl);
+        by += gql_refine_one(g, items[e.x], e.y, left_slots, left_lab, left_labs, cand_pool, bitmaps, wps, member, sh,
+                             wl);
     }
     block_add(bytes, lane_id() == 0 ? by : 0ull);
 }
@@ -2295,15 +2300,15 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
 constexpr uint32_t kGqlGrid = 2048;
 
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, const uint2* list, const uint32_t* n_list,
-                             uint32_t* cursor, uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_labs,
-                             uint32_t* cand_pool, uint32_t* bitmaps,
+                             uint32_t* cursor, uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_lab,
+                             const uint32_t* left_labs, uint32_t* cand_pool, uint32_t* bitmaps,
                              uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
                              hipStream_t s) {
     if (max_n == 0) return hipSuccess;
     // six blocks per CU fit the LDS of gfx950's 256 CUs (26 KB per block)
     const uint32_t blocks = std::min<uint32_t>(1536, (max_n + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock);
     hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, cursor, left_slots,
-                       left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
+                       left_lab, left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
     return hipGetLastError();
 }
 

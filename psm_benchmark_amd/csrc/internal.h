@@ -214,10 +214,11 @@ hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items
                            const uint32_t* n_list, uint32_t max_n, const uint32_t* left_slots,
                            const uint32_t* left_labs, uint32_t* cand_pool, uint32_t* bitmaps, uint64_t words_per_slot,
                            unsigned long long* member, unsigned long long* bytes, hipStream_t s);
-// cursor: zero before the launch (the wavefronts' shared position in list)
+// cursor: zero before the launch (the wavefronts' shared position in list);
+// left_lab: the label of each left vertex (parallel to left_slots)
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, const uint2* list, const uint32_t* n_list,
-                             uint32_t* cursor, uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_labs,
-                             uint32_t* cand_pool, uint32_t* bitmaps,
+                             uint32_t* cursor, uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_lab,
+                             const uint32_t* left_labs, uint32_t* cand_pool, uint32_t* bitmaps,
                              uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
                              hipStream_t s);
 // thread per row; rows of candidates with more than 64 neighbours are listed

@@ -663,7 +663,7 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
     // work items of every query vertex position p (the candidates of vertex p
     // of each query), uploaded once: both rounds use the same items
     std::vector<GItem> items;
-    std::vector<uint32_t> left, labs, item_begin(max_n + 1, 0), cands(max_n, 0);
+    std::vector<uint32_t> left, left_lab, labs, item_begin(max_n + 1, 0), cands(max_n, 0);
     uint32_t max_cands = 0;
     for (uint32_t p = 0; p < max_n; ++p) {
         item_begin[p] = (uint32_t)items.size();
@@ -684,6 +684,7 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
             const size_t l0 = labs.size();
             for (uint32_t i = q->qg.off[p]; i < q->qg.off[p + 1]; ++i) {
                 left.push_back(q->slot_base + q->qg.adj[i]);
+                left_lab.push_back(q->qg.lab[q->qg.adj[i]]);
                 labs.push_back(q->qg.lab[q->qg.adj[i]]);
             }
             std::sort(labs.begin() + l0, labs.end());
@@ -696,9 +697,10 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
         max_cands = std::max(max_cands, waves);
     }
     item_begin[max_n] = (uint32_t)items.size();
-    DBuf d_items, d_left, d_labs, d_lists, d_nlists;
+    DBuf d_items, d_left, d_left_lab, d_labs, d_lists, d_nlists;
     HIP_TRY(ctx, upload(d_items, items, s()));
     HIP_TRY(ctx, upload(d_left, left, s()));
+    HIP_TRY(ctx, upload(d_left_lab, left_lab, s()));
     HIP_TRY(ctx, upload(d_labs, labs, s()));
     // deferred pairs of one (round, p): class c at c * max_cands; reused by
     // every launch group (stream order)
@@ -731,7 +733,7 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
                                       member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
             TIMED(ctx, kKGqlRefine,
                   launch_gql_refine(g->dg, its, lists + 3ull * max_cands, cnt + 3, cnt + 4, cands[p], d_left.as<uint32_t>(),
-                                    d_labs.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
+                                    d_left_lab.as<uint32_t>(), d_labs.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
                                     member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
         }
     }
