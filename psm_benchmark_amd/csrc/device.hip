@@ -1513,14 +1513,20 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 }
             }
         }
-        const bool flush = !active || lcount >= flush_at || (steps & 1023) == 0;
-        if (flush && (lcount | lcalls | lbytes)) {
+        // the running count goes to the query often enough for its cap (one
+        // atomic); the statistics only at task ends and every 1024 steps, so
+        // that lanes sharing a query do not queue on its counters
+        const bool stat_flush = !active || (steps & 1023) == 0;
+        if (lcount && (stat_flush || lcount >= flush_at)) {
             const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
-            atomicAdd(&qc[q].calls, lcalls);
-            atomicAdd(&qc[q].probe_bytes, lbytes);
+            if (prev + lcount >= P->cap) atomicExch(&qc[q].done, 1u);
+            lcount = 0;
+        }
+        if (stat_flush && (lcalls | lbytes | lalgo)) {
+            if (lcalls) atomicAdd(&qc[q].calls, lcalls);
+            if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);
             if (lalgo) atomicAdd(&qc[q].algo_bytes, lalgo);
-            if (lcount && prev + lcount >= P->cap) atomicExch(&qc[q].done, 1u);
-            lcount = lcalls = lbytes = lalgo = 0;
+            lcalls = lbytes = lalgo = 0;
         }
         if (finished) atomicSub(pending + q, 1u);
         if (!active) atomicSub(&Q->busy, 1u);
