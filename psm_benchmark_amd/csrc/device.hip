@@ -1266,7 +1266,6 @@ __device__ __forceinline__ bool past_deadline(DQueue* Q) {
 }
 
 constexpr uint32_t kLeafBurst = 32;   // base elements one step of the lane DFS tests at most
-constexpr uint32_t kIdleWaitMax = 32;  // longest idle wait of a wavefront, in units of s_sleep(16)
 
 // Persistent DFS. Every lane runs its own depth-first search over one task at
 // a time. Idle lanes claim the next queued tasks (one claim per wavefront).
@@ -1300,7 +1299,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         atomicCAS(&Q->deadline, 0ull, (unsigned long long)wall_clock64() + Q->limit_ticks);
 
     bool active = false, waiting = false;
-    uint32_t wslot = 0, backoff = 0, idle_wait = 1;
+    uint32_t wslot = 0, backoff = 0;
     uint32_t q = 0, tdepth = 0, d = 0, n = 0;
     Level L{};
     Level Lp{};   // the level above the leaf depth, kept while the lane is at the leaf
@@ -1380,19 +1379,12 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
         }
         if (__ballot(active || waiting) == 0) {
-            // the whole wavefront is idle: leave once no work can appear;
-            // otherwise wait, twice as long each time nothing was found (up
-            // to kIdleWaitMax), and try again. Idle wavefronts polling the
-            // queue's counters back to back crowd the atomics that running
-            // lanes need to split work off.
+            // the whole wavefront is idle: leave once no work can appear
             if (load_relaxed(&Q->timeout)) break;
             if (load_relaxed(&Q->head) >= load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0) break;
-            for (uint32_t i = 0; i < idle_wait; ++i) __builtin_amdgcn_s_sleep(16);
-            idle_wait = min(2 * idle_wait, kIdleWaitMax);
-            backoff = 0;
+            __builtin_amdgcn_s_sleep(4);
             continue;
         }
-        idle_wait = 1;
         ++all_iters;
         if (!active) continue;
         ++busy_iters;
@@ -1735,7 +1727,6 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
         atomicCAS(&Q->deadline, 0ull, (unsigned long long)wall_clock64() + Q->limit_ticks);
     const uint32_t hungry_below = gridDim.x / 2;   // queue shorter than half the wavefronts
     unsigned long long bytes = 0, algo = 0, busy_pos = 0, all_pos = 0;
-    uint32_t idle_wait = 1;
 
     while (true) {
         // ---- claim one task (none once the time limit has passed: queued
@@ -1758,11 +1749,9 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                 stop = load_relaxed(&Q->timeout) ||
                        (load_relaxed(&Q->head) >= load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0);
             if (__shfl(stop, 0)) break;
-            for (uint32_t i = 0; i < idle_wait; ++i) __builtin_amdgcn_s_sleep(16);
-            idle_wait = min(2 * idle_wait, kIdleWaitMax);
+            __builtin_amdgcn_s_sleep(8);
             continue;
         }
-        idle_wait = 1;
         while (!load_acquire(ready + slot)) __builtin_amdgcn_s_sleep(1);
         const DTask& task = queue[slot];
         const uint32_t q = task.q;
