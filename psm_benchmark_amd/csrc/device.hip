@@ -1309,7 +1309,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     bool capped_q = false;
     uint32_t steps = 0;
     const DPlan* P = plans;
-    const uint32_t hungry_below = gridDim.x * (blockDim.x / 2);   // queue shorter than half the lanes
+    const uint32_t lanes_total = gridDim.x * blockDim.x;
 
     while (true) {
         // claim queued tasks for idle lanes, one atomic per wavefront (none
@@ -1471,9 +1471,12 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 active = false;
             }
         }
-        // donation: split the shallowest level that still has >= 2 entries
+        // donation: split the shallowest level that still has >= 2 entries,
+        // when the queue holds fewer tasks than there are lanes without one
+        // (work goes to lanes that wait for it; while every lane is busy,
+        // splitting would only add claims and records to the queue)
         if (active && (steps & 15) == 0 &&
-            load_relaxed(&Q->tail) - load_relaxed(&Q->head) < hungry_below) {
+            load_relaxed(&Q->tail) - load_relaxed(&Q->head) < lanes_total - min(load_relaxed(&Q->busy), lanes_total)) {
             uint32_t e = tdepth;
             uint64_t ce = 0, ee = 0;
             for (; e <= d; ++e) {
