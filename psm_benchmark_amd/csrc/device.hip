@@ -1306,12 +1306,23 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     unsigned long long lcount = 0, lcalls = 0, lbytes = 0, lalgo = 0;
     unsigned long long busy_iters = 0, all_iters = 0;
     unsigned long long flush_at = 4096;   // leaf counts a lane holds before adding them to its query
-    bool capped_q = false;
+    bool near_cap = false;                // the query's count was at half its cap or more at the last flush
+    bool hungry = true;                   // wavefront view: the queue holds fewer tasks than idle lanes
+    uint32_t witer = 0;
     uint32_t steps = 0;
     const DPlan* P = plans;
     const uint32_t lanes_total = gridDim.x * blockDim.x;
 
     while (true) {
+        // every 16 iterations one lane reads whether lanes wait for work
+        // (the donation rule below); the wavefront is converged here
+        if ((witer++ & 15) == 0) {
+            uint32_t h = 0;
+            if (lane == 0)
+                h = load_relaxed(&Q->tail) - load_relaxed(&Q->head) <
+                    lanes_total - min(load_relaxed(&Q->busy), lanes_total);
+            hungry = __shfl(h, 0) != 0;
+        }
         // claim queued tasks for idle lanes, one atomic per wavefront (none
         // once the time limit has passed)
         const unsigned long long want = __ballot(!active && !waiting && load_relaxed(&Q->timeout) == 0);
@@ -1374,8 +1385,8 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 // a capped query learns its running count sooner (smaller
                 // flushes, more frequent checks): every lane's unflushed
                 // embeddings and unchecked steps are searched past the cap
-                capped_q = P->cap != ~0ull;
-                flush_at = capped_q ? min(4096ull, max(16ull, P->cap >> 12)) : 4096ull;
+                near_cap = false;
+                flush_at = P->cap != ~0ull ? min(4096ull, max(16ull, P->cap >> 12)) : 4096ull;
             }
         }
         if (__ballot(active || waiting) == 0) {
@@ -1463,7 +1474,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
         }
         ++steps;
-        if (active && (steps & (capped_q ? 31u : 255u)) == 0) {
+        // the cap is checked every 32 steps once the query's count has
+        // reached half of it, otherwise with the time limit every 256
+        if (active && (steps & (near_cap ? 31u : 255u)) == 0) {
             if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout)) {
                 active = false;
             } else if (past_deadline(Q)) {
@@ -1475,8 +1488,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         // when the queue holds fewer tasks than there are lanes without one
         // (work goes to lanes that wait for it; while every lane is busy,
         // splitting would only add claims and records to the queue)
-        if (active && (steps & 15) == 0 &&
-            load_relaxed(&Q->tail) - load_relaxed(&Q->head) < lanes_total - min(load_relaxed(&Q->busy), lanes_total)) {
+        if (active && hungry && (steps & 15) == 0) {
             uint32_t e = tdepth;
             uint64_t ce = 0, ee = 0;
             for (; e <= d; ++e) {
@@ -1523,6 +1535,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         if (lcount && (stat_flush || lcount >= flush_at)) {
             const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
             if (prev + lcount >= P->cap) atomicExch(&qc[q].done, 1u);
+            near_cap = prev + lcount >= P->cap / 2;
             lcount = 0;
         }
         if (stat_flush && (lcalls | lbytes | lalgo)) {
