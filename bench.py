@@ -1,6 +1,6 @@
 """Benchmark of the batched matching path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload C2]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload C1|C2|C3|C4|C5]
 
 Workload C2 (seeded, synthetic): Chung-Lu 1.1M vertices / 3.0M edges,
 100 Zipf(1.1) labels; 512 queries of 4/8/16/24/32 vertices, half dense and
@@ -51,7 +51,7 @@ def cpu_baseline(g, qs, meta, cap: int, budget_s: float, time_limit: float, thre
         for i, q in enumerate(qs):
             qp = os.path.join(td, "q.txt")
             save_text(q, qp)
-            b = R.run(dp, qp, "GQL_GQL_GQL", threads=str(threads), cap=str(cap),
+            b = R.run(dp, qp, "GQL_GQL_GQL", threads=str(threads), cap=str(cap) if cap else "MAX",
                       time_limit=max(1, int(round(time_limit))), timeout=600)[0]
             spent += b["total_s"]
             emb += b["embeddings"]
@@ -60,9 +60,9 @@ def cpu_baseline(g, qs, meta, cap: int, budget_s: float, time_limit: float, thre
             if spent >= budget_s or time.time() - t_wall > 3 * budget_s:
                 break
         return dict(value=done / spent, unit="queries/s", cores=threads, kind="reference",
-                    embeddings_per_s=emb / spent, timeouts=timeouts,
+                    embeddings_per_s=emb / spent, timeouts=timeouts, embeddings=emb // max(1, done),
                     sample=f"first {done} of {len(qs)} queries, reference SubgraphMatching.out "
-                           f"GQL_GQL_GQL -threadnums {threads} -num {cap} -time_limit "
+                           f"GQL_GQL_GQL -threadnums {threads} -num {cap if cap else 'MAX'} -time_limit "
                            f"{max(1, int(round(time_limit)))}, its own Total time "
                            f"(filter+table+plan+enumerate, graph load excluded) = {spent:.2f}s")
     from oracle_binding import OGraph, ORun
@@ -189,6 +189,9 @@ def main():
         sys.exit(2)
     if args.time_limit is None:
         args.time_limit = 2.0 if args.workload == "C2" else 20.0
+    if args.workload == "C1":
+        plain_workload(args)
+        return
     if args.workload in ("C3", "C5"):
         large_workload(args)
         return
@@ -333,6 +336,63 @@ def main():
     if cpu_n:
         out["cpu_baseline_nproc"] = cpu_n
         out["batched_over_cpu_nproc"] = round(out["value"] / cpu_n["value"], 1)
+    print(json.dumps(out))
+
+
+def plain_workload(args):
+    """C1, the reference's CPU-runnable case: Chung-Lu 9,460 V / 34,998 E, 307
+    Zipf(1.1) labels, one 8-vertex random-walk query (seeded), GQL_GQL_GQL,
+    uncapped. A step is one plain-path match of the query on the GPU (filter,
+    tables, plan, enumeration; the graph resident in HBM). The reference
+    binary runs the same query on the host at one thread as the CPU baseline,
+    and its embedding count must equal the GPU's."""
+    from psm_benchmark_amd import workloads as W
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.matcher import Device, DeviceGraph, match
+
+    g = W.data_graph("C1")
+    q = W.c1_query(g)
+    log(f"workload C1: V={g.num_vertices} E={g.num_edges} query n={q.num_vertices} e={q.num_edges}")
+    dev = Device(0)
+    L.lib().psm_set_time_limit(dev.h, args.time_limit)
+    dg = DeviceGraph(dev, g)
+    if args.shape:
+        apply_shape(dev, args.shape)
+    for _ in range(args.warmup):
+        match(dg, q, "GQL_GQL_GQL")
+    L.lib().psm_kernel_stats_reset(dev.h)
+    lat, r = [], None
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        r = match(dg, q, "GQL_GQL_GQL")
+        lat.append(time.perf_counter() - t)
+    ks = kernel_stats(dev)
+    total = sum(lat)
+    lat_ms = np.array(lat) * 1e3
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline, 1 thread")
+        cpu = cpu_baseline(g, [q] * 1000, None, 0, args.cpu_budget, args.time_limit)
+        if cpu.get("kind") == "reference" and cpu.get("embeddings") != r.count:
+            raise SystemExit(f"reference count {cpu.get('embeddings')} != device count {r.count}")
+    out = {
+        "metric": "queries/s + embeddings/s on 1 MI355X; p50/p99 latency batched vs plain",
+        "value": round(args.steps / total, 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded graph and query)",
+        "config": {"workload": "C1 plain: Chung-Lu 9,460 V / 34,998 E, 307 Zipf(1.1) labels, one 8-vertex "
+                               "random-walk query, GQL_GQL_GQL, uncapped", "time_limit_s": args.time_limit},
+        "embeddings": r.count, "calls": r.calls, "timed_out": r.timed_out,
+        "embeddings_per_s": round(r.count * args.steps / total, 1),
+        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3),
+                       "p99": round(float(np.percentile(lat_ms, 99)), 3)},
+        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"] / args.steps, 4),
+                            bytes=int(v["bytes"] / args.steps)) for k, v in ks.items()},
+        "roofline": roofline_of(ks, args.steps),
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = round(out["value"] / cpu["value"], 2)
     print(json.dumps(out))
 
 
