@@ -276,3 +276,54 @@ def test_c2_scale_lane_wave_plain_agree(c2_graph):
     assert checked >= len(qs) // 2
     dg.close()
     dev.close()
+
+
+def hub_star_query(g, hub: int, leaves: int, seed: int):
+    """A query around a data hub: the hub, `leaves` of its neighbours, and the
+    data edges among those neighbours; labels are the data labels (so at
+    least one embedding exists). The hub's query degree exceeds 8, which
+    sends its candidates to the wavefront refinement test."""
+    rng = np.random.default_rng(seed)
+    off = g.offsets.astype(np.int64)
+    nb = g.neighbors[off[hub]:off[hub + 1]]
+    pick = [int(x) for x in rng.choice(nb, size=leaves, replace=False)]
+    verts = [hub] + pick
+    idx = {v: i for i, v in enumerate(verts)}
+    edges = [(0, i) for i in range(1, len(verts))]
+    for a in pick:
+        for b in g.neighbors[off[a]:off[a + 1]]:
+            b = int(b)
+            if b in idx and idx[a] < idx[b]:
+                edges.append((idx[a], idx[b]))
+    return csr_from_edges(len(verts), np.array(edges), g.labels[np.array(verts)])
+
+
+@pytest.mark.parametrize("labels,seed", [(3, 7), (12, 8)])
+def test_wavefront_refinement_on_hubs_matches_oracle(dev, labels, seed):
+    """Candidate sets and tables of GQL filtering on a power-law graph with
+    few labels (long label runs on the hubs) for queries whose centre has
+    query degree 10-12 (the wavefront test with its per-label early exits
+    and saturation), against the oracle element for element."""
+    from psm_benchmark_amd import gen
+    from psm_benchmark_amd.matcher import DeviceGraph, Query
+    g = gen.chung_lu_graph(20000, 200000, labels, 1.0, seed=seed)
+    og = OGraph(g)
+    dg = DeviceGraph(dev, g)
+    deg = np.diff(g.offsets.astype(np.int64))
+    hubs = np.argsort(-deg)[:40:8]
+    for i, hub in enumerate(hubs):
+        q = hub_star_query(g, int(hub), 10 + (i % 3), seed + i)
+        ref = ORun(og, OGraph(q), "GQL_GQL_GQL", cap=1000)
+        run = Query(dg, q)
+        run.filter("GQL")
+        print("hub", int(hub), "deg", int(deg[hub]), "query n", q.num_vertices, "cands", ref.cand_counts, flush=True)
+        assert run.candidate_counts().tolist() == ref.cand_counts
+        for u in range(q.num_vertices):
+            assert np.array_equal(run.candidates(u), ref.candidates(u)), u
+        run.build_tables()
+        for a, b in itertools.product(range(q.num_vertices), repeat=2):
+            t, rt = run.table(a, b), ref.table(a, b)
+            assert (t is None) == (rt is None)
+            if t is not None:
+                assert np.array_equal(t[0], rt[0]) and np.array_equal(t[1], rt[1]), (a, b)
+        run.close()
