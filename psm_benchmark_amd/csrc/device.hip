@@ -1251,8 +1251,17 @@ __device__ __forceinline__ int test_merge(const DPlan& P, const DOp* __restrict_
     return 1;
 }
 
-__device__ __forceinline__ uint32_t load_acquire(uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Device-coherent loads of a queued task record. A lane that splits a task
+// off writes its record back from its own caches (release fence) before it
+// sets the record's ready flag; the claiming lane reads the flag and then the
+// record with coherent loads, which fetch the written-back data. An acquire
+// on the flag instead would invalidate the claiming CU's and XCD's caches at
+// every claim, and claims come at millions per launch.
+__device__ __forceinline__ uint32_t coherent_u32(const uint32_t* p) {
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t coherent_u64(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint32_t load_relaxed(unsigned int* p) {
@@ -1350,12 +1359,12 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         } else if (backoff) {
             --backoff;
         }
-        if (waiting && load_acquire(ready + wslot)) {
+        if (waiting && load_relaxed(ready + wslot)) {
             waiting = false;
-            // the record is read field by field (a local copy of its prefix
-            // array would live in scratch memory)
-            const DTask& task = queue[wslot];
-            q = task.q;
+            // the record is read field by field with coherent loads (a local
+            // copy of its prefix array would live in scratch memory)
+            const DTask* task = queue + wslot;
+            q = coherent_u32(&task->q);
             P = plans + q;
             bool late = load_relaxed(&Q->timeout) != 0;
             if (!late && past_deadline(Q)) {
@@ -1366,19 +1375,20 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 atomicSub(&Q->busy, 1u);
             } else {
                 n = P->n;
-                tdepth = task.depth;
+                tdepth = coherent_u32(&task->depth);
                 d = tdepth;
                 for (uint32_t e = 0; e < tdepth; ++e) {
-                    const uint32_t x = task.prefix[e];
+                    const uint32_t x = coherent_u32(&task->prefix[e]);
                     st.I_[st.at(e)] = x;
                     st.M_[st.at(e)] = cand_pool[P->cand_base[e] + x];
                 }
-                if (task.hi == kAutoHi) {
+                const uint64_t thi = coherent_u64(&task->hi);
+                if (thi == kAutoHi) {
                     enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo);
                 } else {
-                    enter_level(*P, ops, row_off, d, st, task.sel, L, lbytes, lalgo);
-                    L.cur = task.lo;
-                    L.end = task.hi;
+                    enter_level(*P, ops, row_off, d, st, coherent_u32(&task->sel), L, lbytes, lalgo);
+                    L.cur = coherent_u64(&task->lo);
+                    L.end = thi;
                 }
                 active = true;
                 steps = 0;
@@ -1768,9 +1778,9 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             __builtin_amdgcn_s_sleep(8);
             continue;
         }
-        while (!load_acquire(ready + slot)) __builtin_amdgcn_s_sleep(1);
-        const DTask& task = queue[slot];
-        const uint32_t q = task.q;
+        while (!load_relaxed(ready + slot)) __builtin_amdgcn_s_sleep(1);
+        const DTask* task = queue + slot;   // read with coherent loads (see coherent_u32)
+        const uint32_t q = coherent_u32(&task->q);
         const DPlan& P = plans[q];
         uint32_t drop = 0;
         if (lane == 0) {
@@ -1782,9 +1792,9 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             if (drop) atomicSub(&Q->busy, 1u);
         }
         if (__shfl(drop, 0)) continue;
-        const uint32_t n = P.n, t = task.depth;
+        const uint32_t n = P.n, t = coherent_u32(&task->depth);
         if (lane < t) {
-            const uint32_t x = task.prefix[lane];
+            const uint32_t x = coherent_u32(&task->prefix[lane]);
             S.I[lane] = x;
             S.M[lane] = cand_pool[P.cand_base[lane] + x];
         }
@@ -1792,12 +1802,13 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             S.gen_m[t] = 0;
             S.gen_cnt[t] = 1;
             S.counted[t] = 0;
-            if (task.hi == kAutoHi) {
+            const uint64_t thi = coherent_u64(&task->hi);
+            if (thi == kAutoHi) {
                 S.gen_pos[t] = kFresh;
             } else {
-                S.gen_pos[t] = task.lo;
-                S.gen_end[t] = task.hi;
-                S.gen_sel[t] = task.sel;
+                S.gen_pos[t] = coherent_u64(&task->lo);
+                S.gen_end[t] = thi;
+                S.gen_sel[t] = coherent_u32(&task->sel);
             }
         }
         __syncthreads();
