@@ -1275,6 +1275,7 @@ __device__ __forceinline__ bool past_deadline(DQueue* Q) {
 }
 
 constexpr uint32_t kLeafBurst = 32;   // base elements one step of the lane DFS tests at most
+constexpr uint32_t kClaimMax = 8;     // queue slots a lane takes per claim at most
 
 // Persistent DFS. Every lane runs its own depth-first search over one task at
 // a time. Idle lanes claim the next queued tasks (one claim per wavefront).
@@ -1308,7 +1309,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         atomicCAS(&Q->deadline, 0ull, (unsigned long long)wall_clock64() + Q->limit_ticks);
 
     bool active = false, waiting = false;
-    uint32_t wslot = 0, backoff = 0;
+    uint32_t wslot = 0, wend = 0, backoff = 0;
     uint32_t q = 0, tdepth = 0, d = 0, n = 0;
     Level L{};
     Level Lp{};   // the level above the leaf depth, kept while the lane is at the leaf
@@ -1334,16 +1335,21 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         }
         // claim queued tasks for idle lanes, one atomic per wavefront (none
         // once the time limit has passed)
+        // While the queue is long, each lane gets several consecutive slots
+        // per claim (up to kClaimMax) and takes them one after the other:
+        // claims go through one compare-and-swap on the shared head, whose
+        // rate (not the search) bounded launches with millions of small tasks.
         const unsigned long long want = __ballot(!active && !waiting && load_relaxed(&Q->timeout) == 0);
         if (want && backoff == 0) {
             const uint32_t leader = __ffsll((long long)want) - 1;
-            uint32_t base = 0, got = 0;
+            uint32_t base = 0, got = 0, per = 1;
             if (lane == leader) {
                 const uint32_t h = load_relaxed(&Q->head), t = load_relaxed(&Q->tail);
                 if (h < t) {
-                    const uint32_t k = min((uint32_t)__popcll(want), t - h);
+                    per = min(kClaimMax, max(1u, (t - h) / (2u * lanes_total)));
+                    const uint32_t k = min((uint32_t)__popcll(want) * per, t - h);
                     if (atomicCAS(&Q->head, h, h + k) == h) {
-                        atomicAdd(&Q->busy, k);
+                        atomicAdd(&Q->busy, k);   // busy counts claimed, unfinished tasks
                         base = h;
                         got = k;
                     }
@@ -1351,9 +1357,14 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
             base = __shfl(base, leader);
             got = __shfl(got, leader);
+            per = __shfl(per, leader);
             if (!active && !waiting) {
                 const uint32_t r = __popcll(want & lanemask_lt());
-                if (r < got) { waiting = true; wslot = base + r; }
+                if (r * per < got) {
+                    waiting = true;
+                    wslot = base + r * per;
+                    wend = min(wslot + per, base + got);
+                }
             }
             if (!got) backoff = 16;
         } else if (backoff) {
@@ -1373,6 +1384,10 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
             if (late || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
                 atomicSub(&Q->busy, 1u);
+                if (wslot + 1 < wend) {   // the lane's next claimed slot
+                    ++wslot;
+                    waiting = true;
+                }
             } else {
                 n = P->n;
                 tdepth = coherent_u32(&task->depth);
@@ -1555,7 +1570,13 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             lcalls = lbytes = lalgo = 0;
         }
         if (finished) atomicSub(pending + q, 1u);
-        if (!active) atomicSub(&Q->busy, 1u);
+        if (!active) {
+            atomicSub(&Q->busy, 1u);
+            if (wslot + 1 < wend) {   // the lane's next claimed slot
+                ++wslot;
+                waiting = true;
+            }
+        }
     }
     if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);   // bytes of tasks dropped at claim
     if (lalgo) atomicAdd(&qc[q].algo_bytes, lalgo);
