@@ -1277,6 +1277,23 @@ __device__ __forceinline__ bool past_deadline(DQueue* Q) {
 constexpr uint32_t kLeafBurst = 32;   // base elements one step of the lane DFS tests at most
 constexpr uint32_t kClaimMax = 8;     // queue slots a lane takes per claim at most
 
+// Adds a lane's running count (setting the query's done flag at its cap)
+// and statistics to query q's counters and clears them.
+__device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t cap, unsigned long long& lcount,
+                                           unsigned long long& lcalls, unsigned long long& lbytes,
+                                           unsigned long long& lalgo, bool& near_cap) {
+    if (lcount) {
+        const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
+        if (prev + lcount >= cap) atomicExch(&qc[q].done, 1u);
+        near_cap = prev + lcount >= cap / 2;
+        lcount = 0;
+    }
+    if (lcalls) atomicAdd(&qc[q].calls, lcalls);
+    if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);
+    if (lalgo) atomicAdd(&qc[q].algo_bytes, lalgo);
+    lcalls = lbytes = lalgo = 0;
+}
+
 // Persistent DFS. Every lane runs its own depth-first search over one task at
 // a time. Idle lanes claim the next queued tasks (one claim per wavefront).
 // A running lane splits off the second half of the remaining range at its
@@ -1310,6 +1327,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
 
     bool active = false, waiting = false;
     uint32_t wslot = 0, wend = 0, backoff = 0;
+    uint32_t busy_done = 0, pend_done = 0;   // task ends not yet subtracted from busy / pending[q]
     uint32_t q = 0, tdepth = 0, d = 0, n = 0;
     Level L{};
     Level Lp{};   // the level above the leaf depth, kept while the lane is at the leaf
@@ -1375,7 +1393,14 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             // the record is read field by field with coherent loads (a local
             // copy of its prefix array would live in scratch memory)
             const DTask* task = queue + wslot;
-            q = coherent_u32(&task->q);
+            const uint32_t tq = coherent_u32(&task->q);
+            if (tq != q) {
+                // the lane's counters so far belong to the previous query
+                lane_flush(qc, q, P->cap, lcount, lcalls, lbytes, lalgo, near_cap);
+                if (pend_done) atomicSub(pending + q, pend_done);
+                pend_done = 0;
+            }
+            q = tq;
             P = plans + q;
             bool late = load_relaxed(&Q->timeout) != 0;
             if (!late && past_deadline(Q)) {
@@ -1383,10 +1408,15 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 late = true;
             }
             if (late || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
-                atomicSub(&Q->busy, 1u);
+                ++busy_done;
                 if (wslot + 1 < wend) {   // the lane's next claimed slot
                     ++wslot;
                     waiting = true;
+                } else {
+                    atomicSub(&Q->busy, busy_done);
+                    busy_done = 0;
+                    if (pend_done) atomicSub(pending + q, pend_done);
+                    pend_done = 0;
                 }
             } else {
                 n = P->n;
@@ -1553,33 +1583,31 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 }
             }
         }
-        // the running count goes to the query often enough for its cap (one
-        // atomic); the statistics only at task ends and every 1024 steps, so
-        // that lanes sharing a query do not queue on its counters
-        const bool stat_flush = !active || (steps & 1023) == 0;
-        if (lcount && (stat_flush || lcount >= flush_at)) {
-            const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
-            if (prev + lcount >= P->cap) atomicExch(&qc[q].done, 1u);
-            near_cap = prev + lcount >= P->cap / 2;
-            lcount = 0;
-        }
-        if (stat_flush && (lcalls | lbytes | lalgo)) {
-            if (lcalls) atomicAdd(&qc[q].calls, lcalls);
-            if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);
-            if (lalgo) atomicAdd(&qc[q].algo_bytes, lalgo);
-            lcalls = lbytes = lalgo = 0;
-        }
-        if (finished) atomicSub(pending + q, 1u);
+        // A lane's counters go to its query and to the queue when it has no
+        // claimed slot left (or moves to another query), not at every task
+        // end: with millions of small tasks, device-scope atomics on the few
+        // shared counters queue behind one another and every later load of
+        // the issuing wavefront waits for them. The running count goes to the
+        // query often enough for its cap.
+        const bool more = !active && wslot + 1 < wend;   // another claimed slot follows
+        const bool idle_next = !active && !more;
+        if (lcount >= flush_at || idle_next || (steps & 1023) == 0)
+            lane_flush(qc, q, P->cap, lcount, lcalls, lbytes, lalgo, near_cap);
+        if (finished) ++pend_done;
         if (!active) {
-            atomicSub(&Q->busy, 1u);
-            if (wslot + 1 < wend) {   // the lane's next claimed slot
+            ++busy_done;
+            if (more) {
                 ++wslot;
                 waiting = true;
+            } else {
+                atomicSub(&Q->busy, busy_done);
+                busy_done = 0;
+                if (pend_done) atomicSub(pending + q, pend_done);
+                pend_done = 0;
             }
         }
     }
-    if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);   // bytes of tasks dropped at claim
-    if (lalgo) atomicAdd(&qc[q].algo_bytes, lalgo);
+    lane_flush(qc, q, P->cap, lcount, lcalls, lbytes, lalgo, near_cap);
     wave_add(&Q->lane_steps, busy_iters);
     wave_add(&Q->lane_iters, all_iters);
 }
