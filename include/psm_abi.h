@@ -34,17 +34,21 @@ extern "C" {
 #define PSM_ENOTFOUND 6   /* generator could not produce the requested object */
 #define PSM_ESTATE 7      /* call out of order (e.g. enumerate before build tables) */
 
-/* filter methods: FilterVertices::{LDF,NLF,GQL,CFL}Filter
- * (matching/FilterVertices.h:9-16) */
+/* filter methods: FilterVertices::{LDF,NLF,GQL,CFL,DPiso}Filter
+ * (matching/FilterVertices.h:9-17) */
 #define PSM_FILTER_LDF 0
 #define PSM_FILTER_NLF 1
 #define PSM_FILTER_GQL 2
 #define PSM_FILTER_CFL 3
+#define PSM_FILTER_DPISO 4
 
-/* order methods: GenerateQueryPlan::generate{GQL,CFL}QueryPlan
- * (matching/GenerateQueryPlan.h:32,49) */
+/* order methods: GenerateQueryPlan::generate{GQL,CFL,DPiso}QueryPlan
+ * (matching/GenerateQueryPlan.h:32,49,55). The DPiso order keeps every
+ * earlier neighbour as an operand (EvaluateQuery::generateBN), the plan the
+ * reference's LFTJ and GQL engines run. */
 #define PSM_ORDER_GQL 0
 #define PSM_ORDER_CFL 1
+#define PSM_ORDER_DPISO 2
 
 /* the largest query the device path accepts (config.h:9 MAXIMUM_QUERY_GRAPH_SIZE) */
 #define PSM_MAX_QUERY_VERTICES 64

@@ -405,6 +405,73 @@ bool cfl_filter(const OGraph& d, const OGraph& q, Cands& C) {
 }
 
 // ---------------------------------------------------------------------------
+// DPiso filter (FilterVertices.cpp:252-294) and its plan
+// (GenerateFilteringPlan.cpp:183-214, start vertex :387-406).
+// ---------------------------------------------------------------------------
+// The query vertex of degree >= 2 with the fewest LDF candidates per unit of
+// degree: strict minimum starting from |V(data)|, vertex 0 when no vertex has
+// degree 2 or more.
+uint32_t dpiso_start_vertex(const OGraph& d, const OGraph& q) {
+    double best = d.V;
+    uint32_t start = 0;
+    std::vector<uint32_t> tmp;
+    for (uint32_t u = 0; u < q.V; ++u) {
+        if (q.deg(u) <= 1) continue;
+        nlf_candidates(d, q, u, false, tmp);
+        double score = tmp.size() / (double)q.deg(u);
+        if (score < best) { best = score; start = u; }
+    }
+    return start;
+}
+
+// BFS order from the start vertex; bn = neighbours earlier in that order,
+// fn = neighbours later, both in adjacency order.
+void dpiso_plan(const OGraph& d, const OGraph& q, Tree& t) {
+    bfs_tree(q, dpiso_start_vertex(d, q), t);
+    uint32_t n = q.V;
+    std::vector<uint32_t> pos(n);
+    for (uint32_t i = 0; i < n; ++i) pos[t.order[i]] = i;
+    t.bn.assign(n, {});
+    t.fn.assign(n, {});
+    for (uint32_t u = 0; u < n; ++u)
+        for (uint32_t j = 0; j < q.deg(u); ++j) {
+            uint32_t w = q.nbrs(u)[j];
+            (pos[w] < pos[u] ? t.bn[u] : t.fn[u]).push_back(w);
+        }
+}
+
+// LDF candidates, then three pruning sweeps over the plan order (k = 0, 1, 2):
+// even sweeps run forward from the second vertex against its backward
+// neighbours, the odd sweep runs backward from the second-to-last vertex
+// against its forward neighbours. A vertex without pivots keeps its set.
+bool dpiso_filter(const OGraph& d, const OGraph& q, Cands& C) {
+    if (!ldf_nlf_filter(d, q, false, C)) return false;
+    Tree t;
+    dpiso_plan(d, q, t);
+    uint32_t n = q.V;
+    std::vector<uint32_t> cnt(d.V, 0);
+    for (uint32_t k = 0; k < 3; ++k) {
+        if (k % 2 == 0) {
+            for (uint32_t i = 1; i < n; ++i) {
+                uint32_t u = t.order[i];
+                cfl_prune(d, q, u, t.bn[u], C, cnt);
+            }
+        } else {
+            for (int i = (int)n - 2; i >= 0; --i) {
+                uint32_t u = t.order[i];
+                cfl_prune(d, q, u, t.fn[u], C, cnt);
+            }
+        }
+    }
+    bool ok = true;
+    for (uint32_t u = 0; u < n; ++u) {
+        C[u].erase(std::remove(C[u].begin(), C[u].end(), kInvalid), C[u].end());
+        if (C[u].empty()) ok = false;
+    }
+    return ok;
+}
+
+// ---------------------------------------------------------------------------
 // Candidate-edge tables, BuildTable.cpp:9-151, by their definition:
 // T[a][b] row j = ascending positions k with C(b)[k] adjacent to C(a)[j].
 // ---------------------------------------------------------------------------
@@ -691,6 +758,20 @@ void cfl_order(const OGraph& d, const OGraph& q, const Cands& C, const Tables& T
 
 // EvaluateQuery.cpp:1594-1666: bn[i] = earlier neighbours of order[i], in
 // neighbour-id order, optionally without the pivot.
+// GenerateQueryPlan.cpp:1133-1160 after GenerateFilteringPlan.cpp:145-181,
+// both on the query unit (labels renumbered, so the start-vertex rule counts
+// LDF candidates of the renumbered labels): the matching order is the unit's
+// DPiso BFS order and each later vertex's pivot is its BFS parent.
+void dpiso_order(const OGraph& d, const OGraph& q, Plan& p) {
+    OGraph ug = unit_graph(q);
+    Tree t;
+    bfs_tree(ug, dpiso_start_vertex(d, ug), t);
+    uint32_t n = q.V;
+    p.order = t.order;
+    p.pivot.assign(n, 0);
+    for (uint32_t i = 1; i < n; ++i) p.pivot[i] = t.parent[p.order[i]];
+}
+
 void backward_neighbours(const OGraph& q, Plan& p, bool drop_pivot) {
     uint32_t n = q.V;
     p.bn.assign(n, {});
@@ -878,8 +959,9 @@ void orc_graph_free(void* g) { delete static_cast<OGraph*>(g); }
 
 uint32_t orc_graph_labels(void* g) { return static_cast<OGraph*>(g)->L; }
 
-// filter: "LDF" "NLF" "GQL" "CFL"; order: "GQL" "CFL"; engine: "GQL"
-// "EXPLORE" "LFTJ". Returns a run handle (NULL on a bad argument).
+// filter: "LDF" "NLF" "GQL" "CFL" "DPiso"; order: "GQL" "CFL" "DPiso";
+// engine: "GQL" "EXPLORE" "LFTJ" (not EXPLORE with the DPiso order).
+// Returns a run handle (NULL on a bad argument).
 void* orc_run(void* dg, void* qg, const char* filter, const char* order, const char* engine,
               uint64_t cap, orc_result* out) {
     const OGraph& d = *static_cast<OGraph*>(dg);
@@ -896,6 +978,7 @@ void* orc_run(void* dg, void* qg, const char* filter, const char* order, const c
     else if (f == "NLF") r->filter_ok = ldf_nlf_filter(d, q, true, r->C);
     else if (f == "GQL") r->filter_ok = gql_filter(d, q, r->C);
     else if (f == "CFL") r->filter_ok = cfl_filter(d, q, r->C);
+    else if (f == "DPiso") r->filter_ok = dpiso_filter(d, q, r->C);
     else { delete r; return nullptr; }
     for (auto& c : r->C) std::sort(c.begin(), c.end());   // sortCandidates
     build_tables(d, q, r->C, r->T);
@@ -906,6 +989,12 @@ void* orc_run(void* dg, void* qg, const char* filter, const char* order, const c
     } else if (o == "CFL") {
         cfl_order(d, q, r->C, r->T, r->plan);
         backward_neighbours(q, r->plan, true);
+    } else if (o == "DPiso" && eng != kExplore) {
+        // the reference plans DPiso orders with plain backward neighbours
+        // (StudyPerformance.cpp:1397-1414), which the pivot-based EXPLORE
+        // engine cannot use
+        dpiso_order(d, q, r->plan);
+        backward_neighbours(q, r->plan, false);
     } else { delete r; return nullptr; }
     enumerate(d, q, r->C, r->T, r->plan, eng, cap ? cap : UINT64_MAX, r->er);
     if (out) {

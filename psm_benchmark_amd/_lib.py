@@ -20,8 +20,8 @@ PSM_EOVERFLOW = 5
 PSM_ENOTFOUND = 6
 PSM_ESTATE = 7
 
-FILTERS = {"LDF": 0, "NLF": 1, "GQL": 2, "CFL": 3}
-ORDERS = {"GQL": 0, "CFL": 1}
+FILTERS = {"LDF": 0, "NLF": 1, "GQL": 2, "CFL": 3, "DPiso": 4}
+ORDERS = {"GQL": 0, "CFL": 1, "DPiso": 2}
 
 U32P = ctypes.POINTER(ctypes.c_uint32)
 U64P = ctypes.POINTER(ctypes.c_uint64)

@@ -221,6 +221,42 @@ int cfl_filter_plan(const QGraph& q, uint32_t data_L, const std::vector<uint32_t
     return PSM_OK;
 }
 
+int dpiso_filter_plan(const QGraph& q, uint32_t data_V, const NlfCountFn& ldf_count, CflTree& t) {
+    // start: strict minimum of |LDF candidates|/deg over vertices of degree
+    // 2 or more, from |V(data)|; vertex 0 when there is none
+    uint32_t root = 0;
+    double best = data_V;
+    for (uint32_t u = 0; u < q.n; ++u) {
+        if (q.degree(u) <= 1) continue;
+        uint32_t c = 0;
+        int r = ldf_count(u, c);
+        if (r != PSM_OK) return r;
+        const double score = c / (double)q.degree(u);
+        if (score < best) { best = score; root = u; }
+    }
+    bfs(q, root, t);
+    const uint32_t n = q.n;
+    std::vector<uint32_t> rank(n);
+    for (uint32_t i = 0; i < n; ++i) rank[t.order[i]] = i;
+    t.bn.assign(n, {});
+    t.fn.assign(n, {});
+    t.under.assign(n, {});
+    t.level_off.clear();
+    for (uint32_t u = 0; u < n; ++u)
+        for (uint32_t k = q.off[u]; k < q.off[u + 1]; ++k) {
+            const uint32_t w = q.adj[k];
+            (rank[w] < rank[u] ? t.bn[u] : t.fn[u]).push_back(w);
+        }
+    return PSM_OK;
+}
+
+void dpiso_order(const CflTree& t, Plan& p) {
+    const uint32_t n = (uint32_t)t.order.size();
+    p.order = t.order;
+    p.pivot.assign(n, 0);
+    for (uint32_t i = 1; i < n; ++i) p.pivot[i] = t.parent[p.order[i]];
+}
+
 namespace {
 
 // Estimated embeddings of every suffix of a tree path, from the candidate

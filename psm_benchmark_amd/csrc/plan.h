@@ -1,5 +1,6 @@
 // plan.h — host-side query preparation: the query graph with its derived
-// indexes, the CFL BFS filtering plan and the GQL / CFL matching orders.
+// indexes, the CFL and DPiso BFS filtering plans and the GQL / CFL / DPiso
+// matching orders.
 // These run on the host in the reference too (n <= 64, microseconds).
 #pragma once
 
@@ -57,6 +58,16 @@ using TableFn = std::function<int(uint32_t a, uint32_t b, std::vector<uint32_t>&
 // candidates| of u for the start-vertex score.
 int cfl_filter_plan(const QGraph& q, uint32_t data_L, const std::vector<uint32_t>& data_ri_off,
                     uint32_t data_max_freq, const NlfCountFn& nlf_count, CflTree& t);
+
+// GenerateFilteringPlan.cpp:183-214 (+ start vertex :387-406): BFS tree from
+// the vertex of degree >= 2 with the fewest LDF candidates per unit of degree
+// (ldf_count(u) gives |LDF candidates| of u); bn = neighbours earlier in the
+// BFS order, fn = neighbours later (under and level_off stay empty).
+int dpiso_filter_plan(const QGraph& q, uint32_t data_V, const NlfCountFn& ldf_count, CflTree& t);
+
+// GenerateQueryPlan.cpp:1133-1160: the matching order is the DPiso BFS order
+// of the unit graph and each later vertex's pivot is its BFS parent.
+void dpiso_order(const CflTree& t, Plan& p);
 
 // GenerateQueryPlan.cpp:270-471 on the unit graph and its BFS tree; table(a,b)
 // supplies candidate-edge tables for the path embedding estimates

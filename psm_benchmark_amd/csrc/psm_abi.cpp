@@ -417,8 +417,8 @@ struct Batch {
     int gql_refine(std::vector<QueryHost*>& active);
     int for_each_query(size_t count, const std::function<int(size_t, uint32_t, hipStream_t)>& fn,
                        uint32_t& workers, const std::function<void(uint32_t)>& worker_done = nullptr);
-    int cfl_filter(std::vector<QueryHost*>& active);
-    int cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch, DBuf& defer,
+    int cfl_filter(std::vector<QueryHost*>& active, bool dpiso);
+    int cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch, DBuf& defer, bool dpiso,
                          std::vector<std::vector<uint32_t>>& sets);
     int build_tables();
     int make_plans(int order_method);
@@ -428,7 +428,8 @@ struct Batch {
     int join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr);
     int host_table(const QueryHost& q, uint32_t a, uint32_t b, std::vector<uint32_t>& off,
                    std::vector<uint32_t>& idxv, hipStream_t st);
-    int nlf_count(const QGraph& q, uint32_t u, uint32_t label_override, uint32_t& out, hipStream_t st);
+    int nlf_count(const QGraph& q, uint32_t u, uint32_t label_override, uint32_t& out, hipStream_t st,
+                  uint32_t mode = kModeNLF);
 };
 
 // ---- candidate filters ------------------------------------------------------
@@ -780,8 +781,9 @@ int Batch::filter(int method) {
             if (!active.empty()) PSM_TRY(gql_refine(active));
             break;
         case PSM_FILTER_CFL:
+        case PSM_FILTER_DPISO:
             for (QueryHost* q : qs) active.push_back(q);
-            PSM_TRY(cfl_filter(active));
+            PSM_TRY(cfl_filter(active, method == PSM_FILTER_DPISO));
             break;
         default: return set_err(ctx, PSM_EINVAL, "unknown filter method");
     }
@@ -892,7 +894,8 @@ int Batch::host_table(const QueryHost& q, uint32_t a, uint32_t b, std::vector<ui
 
 // |C(u)| under the NLF filter for one query vertex whose label is replaced by
 // `label_override` (the CFL unit start-vertex rule runs on relabelled units).
-int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out, hipStream_t st) {
+int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out, hipStream_t st,
+                     uint32_t mode) {
     std::vector<QVert> qv(1);
     std::vector<uint2> qn;
     qv[0].label = label;
@@ -906,7 +909,7 @@ int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out,
     items[0] = CItem{};
     items[0].src_begin = g->h_ri_off[label];
     items[0].len = g->h_ri_off[label + 1] - g->h_ri_off[label];
-    items[0].mode = kModeNLF;
+    items[0].mode = mode;
     const uint32_t blocks = (items[0].len + 255) / 256;
     if (!blocks) return PSM_OK;
     DBuf d_items, d_qv, d_qn, d_masks, d_bc;
@@ -930,8 +933,10 @@ int Batch::nlf_count(const QGraph& q, uint32_t u, uint32_t label, uint32_t& out,
 // pass with P pivots moves cnt[w] from base to base+P, base = epoch*64.
 // The CFL filter of one query on stream `st` with its own support counters;
 // its candidate sets (one per query vertex) go to `sets`. Its steps depend on
-// the query's own BFS levels.
-int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch, DBuf& defer,
+// the query's own BFS levels. With `dpiso`, the DPiso filter
+// (FilterVertices.cpp:252-294) instead: LDF sets, then three pruning sweeps
+// along the DPiso BFS order with the same support and prune kernels.
+int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& epoch, DBuf& defer, bool dpiso,
                             std::vector<std::vector<uint32_t>>& sets) {
     tls_stream = st;
     const QGraph& qg = q->qg;
@@ -943,8 +948,14 @@ int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& e
         return PSM_OK;
     };
     CflTree t;
-    PSM_TRY(cfl_filter_plan(qg, g->L, g->h_ri_off, g->max_freq,
-                            [&](uint32_t u, uint32_t& c) { return nlf_of(u, c); }, t));
+    if (dpiso) {
+        PSM_TRY(dpiso_filter_plan(qg, g->V, [&](uint32_t u, uint32_t& c) {
+            return nlf_count(qg, u, qg.lab[u], c, st, kModeLDF);
+        }, t));
+    } else {
+        PSM_TRY(cfl_filter_plan(qg, g->L, g->h_ri_off, g->max_freq,
+                                [&](uint32_t u, uint32_t& c) { return nlf_of(u, c); }, t));
+    }
     // per-vertex candidate lists live in their own device buffers while
     // the passes run
     std::vector<std::unique_ptr<DBuf>> cb(n);
@@ -1005,17 +1016,48 @@ int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& e
         target = base + (uint32_t)pivots.size();
         return PSM_OK;
     };
-    const uint32_t root = t.order[0];
-    PSM_TRY(run_compact(root, kModeNLF, 0));
-    for (uint32_t i = 0; i < n; ++i)
-        if (!cb[i]) { cb[i].reset(new DBuf()); HIP_TRY(ctx, cb[i]->alloc(4)); }
-    const uint32_t nlev = (uint32_t)t.level_off.size() - 1;
     auto prune = [&](uint32_t u, const std::vector<uint32_t>& piv) -> int {
         uint32_t target;
         PSM_TRY(support(u, piv, target));
         HIP_TRY(ctx, launch_cfl_prune(cnt.as<uint32_t>(), cb[u]->as<uint32_t>(), 0, len[u], target, st));
         return PSM_OK;
     };
+    auto finish = [&]() -> int {
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+        sets.assign(n, {});
+        for (uint32_t u = 0; u < n; ++u) HIP_TRY(ctx, download(sets[u], cb[u]->p, len[u], st));
+        return PSM_OK;
+    };
+    if (dpiso) {
+        // LDFFilter stops at the first empty set (later sets stay empty and
+        // nothing is pruned); a vertex without pivots keeps its set
+        bool empty = false;
+        for (uint32_t u = 0; u < n; ++u) {
+            if (empty) { cb[u].reset(new DBuf()); HIP_TRY(ctx, cb[u]->alloc(4)); len[u] = 0; continue; }
+            PSM_TRY(run_compact(u, kModeLDF, 0));
+            empty = len[u] == 0;
+        }
+        if (!empty)
+            for (uint32_t k = 0; k < 3; ++k) {
+                if (k % 2 == 0) {
+                    for (uint32_t i = 1; i < n; ++i) {
+                        const uint32_t u = t.order[i];
+                        if (!t.bn[u].empty()) PSM_TRY(prune(u, t.bn[u]));
+                    }
+                } else {
+                    for (int i = (int)n - 2; i >= 0; --i) {
+                        const uint32_t u = t.order[i];
+                        if (!t.fn[u].empty()) PSM_TRY(prune(u, t.fn[u]));
+                    }
+                }
+            }
+        return finish();
+    }
+    const uint32_t root = t.order[0];
+    PSM_TRY(run_compact(root, kModeNLF, 0));
+    for (uint32_t i = 0; i < n; ++i)
+        if (!cb[i]) { cb[i].reset(new DBuf()); HIP_TRY(ctx, cb[i]->alloc(4)); }
+    const uint32_t nlev = (uint32_t)t.level_off.size() - 1;
     for (uint32_t lv = 1; lv < nlev; ++lv) {
         for (uint32_t j = t.level_off[lv]; j < t.level_off[lv + 1]; ++j) {
             const uint32_t u = t.order[j];
@@ -1033,10 +1075,7 @@ int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& e
             const uint32_t u = t.order[j];
             if (!t.under[u].empty()) PSM_TRY(prune(u, t.under[u]));
         }
-    HIP_TRY(ctx, hipStreamSynchronize(st));
-    sets.assign(n, {});
-    for (uint32_t u = 0; u < n; ++u) HIP_TRY(ctx, download(sets[u], cb[u]->p, len[u], st));
-    return PSM_OK;
+    return finish();
 }
 
 // Runs fn(i, worker, stream) for i in [0, count) on up to 8 host threads of
@@ -1082,7 +1121,7 @@ int Batch::for_each_query(size_t count, const std::function<int(size_t, uint32_t
 
 // Queries are independent: they are filtered concurrently, each worker with
 // its own support counters.
-int Batch::cfl_filter(std::vector<QueryHost*>& active) {
+int Batch::cfl_filter(std::vector<QueryHost*>& active, bool dpiso) {
     std::vector<std::vector<std::vector<uint32_t>>> sets(active.size());
     std::vector<std::unique_ptr<DBuf>> cnts(8), defers(8);
     std::vector<uint32_t> epochs(8, 0);
@@ -1095,7 +1134,7 @@ int Batch::cfl_filter(std::vector<QueryHost*>& active) {
             defers[w].reset(new DBuf());   // hub pivot candidates + their count
             HIP_TRY(ctx, defers[w]->alloc(((size_t)g->V + 1) * 4));
         }
-        const int r = cfl_filter_query(active[i], st, *cnts[w], epochs[w], *defers[w], sets[i]);
+        const int r = cfl_filter_query(active[i], st, *cnts[w], epochs[w], *defers[w], dpiso, sets[i]);
         tls_stream = st;
         return r;
     }, nw, [&](uint32_t w) {
@@ -1161,6 +1200,23 @@ int Batch::make_plans(int order_method) {
                              std::vector<uint32_t>& ix) { return host_table(*q, a, b, off, ix, st); };
             PSM_TRY(cfl_order(ug, t, q->cand_count, table, q->plan));
             backward_neighbours(q->qg, q->plan, true);
+            q->planned = true;
+            return PSM_OK;
+        }, nw));
+    } else if (order_method == PSM_ORDER_DPISO) {
+        // StudyPerformance.cpp:1397-1414: the unit's DPiso filtering plan
+        // (its start-vertex rule counts LDF candidates of the unit's
+        // renumbered labels), its BFS order, plain backward neighbours
+        uint32_t nw = 0;
+        PSM_TRY(for_each_query(qs.size(), [&](size_t i, uint32_t, hipStream_t st) -> int {
+            QueryHost* q = qs[i];
+            const QGraph ug = unit_graph(q->qg);
+            CflTree t;
+            PSM_TRY(dpiso_filter_plan(ug, g->V, [&](uint32_t u, uint32_t& c) {
+                return nlf_count(ug, u, ug.lab[u], c, st, kModeLDF);
+            }, t));
+            dpiso_order(t, q->plan);
+            backward_neighbours(q->qg, q->plan, false);
             q->planned = true;
             return PSM_OK;
         }, nw));
@@ -2068,7 +2124,7 @@ int psm_server_create(psm_ctx* ctx, const psm_graph* g, uint32_t workers, uint32
                       uint64_t mem_budget, int filter, int order, uint64_t cap, double time_limit_s,
                       psm_server** out) {
     if (!ctx || !g || !out || workers == 0 || workers > 16 || max_batch == 0) return PSM_EINVAL;
-    if (filter < PSM_FILTER_LDF || filter > PSM_FILTER_CFL || (order != PSM_ORDER_GQL && order != PSM_ORDER_CFL))
+    if (filter < PSM_FILTER_LDF || filter > PSM_FILTER_DPISO || order < PSM_ORDER_GQL || order > PSM_ORDER_DPISO)
         return PSM_EINVAL;
     std::unique_ptr<psm_server> S(new psm_server());
     S->g = g;

@@ -286,6 +286,10 @@ bool FilterVertices::CFLFilter(const Graph* d, const Graph* q, ui**& c, ui*& n) 
     return run_filter(d, q, PSM_FILTER_CFL, c, n);
 }
 
+bool FilterVertices::DPisoFilter(const Graph* d, const Graph* q, ui**& c, ui*& n) {
+    return run_filter(d, q, PSM_FILTER_DPISO, c, n);
+}
+
 void FilterVertices::sortCandidates(ui** candidates, ui* counts, ui num) {
     for (ui i = 0; i < num; ++i) std::sort(candidates[i], candidates[i] + counts[i]);
 }
@@ -339,6 +343,10 @@ void GenerateQueryPlan::generateGQLQueryPlan(const Graph*, const Graph* query, u
 
 void GenerateQueryPlan::generateCFLQueryPlan(const Graph*, const Graph* query, ui*, ui* order, ui* pivot) {
     check(psm_generate_plan(existing(query), PSM_ORDER_CFL, order, pivot), "psm_generate_plan");
+}
+
+void GenerateQueryPlan::generateDPisoQueryPlan(const Graph*, const Graph* query, ui*, ui* order, ui* pivot) {
+    check(psm_generate_plan(existing(query), PSM_ORDER_DPISO, order, pivot), "psm_generate_plan");
 }
 
 size_t EvaluateQuery::evaluate(const Graph*, const Graph* query, size_t limit, size_t& calls,

@@ -99,6 +99,11 @@ public:
                           ui*& candidates_count);
     static bool CFLFilter(const Graph* data_graph, const Graph* query_graph, ui**& candidates,
                           ui*& candidates_count);
+    // FilterVertices.h:17; the DPiso tree and order stay inside the library
+    // (generateDPisoQueryPlan rebuilds them on the query unit, as the
+    // reference's driver does).
+    static bool DPisoFilter(const Graph* data_graph, const Graph* query_graph, ui**& candidates,
+                            ui*& candidates_count);
     // Candidates come out sorted from the device; kept for call-site parity.
     static void sortCandidates(ui** candidates, ui* candidates_count, ui num);
 };
@@ -118,6 +123,10 @@ public:
                                      ui* candidates_count, ui* order, ui* pivot);
     static void generateCFLQueryPlan(const Graph* data_graph, const Graph* query_graph,
                                      ui* candidates_count, ui* order, ui* pivot);
+    // GenerateQueryPlan.h:55 (order = the unit's DPiso BFS order, pivot = BFS
+    // parent; operands are all earlier neighbours).
+    static void generateDPisoQueryPlan(const Graph* data_graph, const Graph* query_graph,
+                                       ui* candidates_count, ui* order, ui* pivot);
 };
 
 class EvaluateQuery {

@@ -101,6 +101,7 @@ int main(int argc, char** argv) {
             else if (f == "NLF") FilterVertices::NLFFilter(&data, &query, cand, cnt);
             else if (f == "GQL") FilterVertices::GQLFilter(&data, &query, cand, cnt);
             else if (f == "CFL") FilterVertices::CFLFilter(&data, &query, cand, cnt);
+            else if (f == "DPiso") FilterVertices::DPisoFilter(&data, &query, cand, cnt);
             else { Line() << "The specified filter type '" << f << "' is not supported." << std::endl; return 1; }
             FilterVertices::sortCandidates(cand, cnt, n);
             const int64_t filter_ns = ns(t, now());
@@ -118,6 +119,7 @@ int main(int argc, char** argv) {
             std::vector<ui> order(n), pivot(n);
             if (o == "GQL") GenerateQueryPlan::generateGQLQueryPlan(&data, &query, cnt, order.data(), pivot.data());
             else if (o == "CFL") GenerateQueryPlan::generateCFLQueryPlan(&data, &query, cnt, order.data(), pivot.data());
+            else if (o == "DPiso") GenerateQueryPlan::generateDPisoQueryPlan(&data, &query, cnt, order.data(), pivot.data());
             else { Line() << "The specified order type '" << o << "' is not supported." << std::endl; return 1; }
             const int64_t plan_ns = ns(t, now());
             Line() << "-----" << std::endl;

@@ -21,7 +21,7 @@ from psm_benchmark_amd.graph_io import save_text  # noqa: E402
 from oracle_binding import OGraph, ORun  # noqa: E402
 import ref_runner as R  # noqa: E402
 
-METHODS = "GQL_GQL_GQL,CFL_CFL_EXPLORE,GQL_GQL_LFTJ"
+METHODS = "GQL_GQL_GQL,CFL_CFL_EXPLORE,GQL_GQL_LFTJ,DPiso_DPiso_LFTJ"
 GRAPH = dict(V=9460, E=34998, labels=307, zipf=1.1, seed=1)
 
 

@@ -55,7 +55,37 @@ def test_hprd_golden_on_device(hprd, rec):
     assert (r.count, r.calls, r.mem_words) == (rec["embeddings"], rec["calls"], rec["mem_words"])
 
 
-@pytest.mark.parametrize("method", ["LDF", "NLF", "GQL", "CFL"])
+@pytest.mark.parametrize("rec", golden(("DPiso_DPiso_LFTJ",)), ids=lambda r: r["query"])
+def test_hprd_dpiso_golden_on_device(hprd, rec):
+    # counts and the BuildTable checksum against the reference's log; the
+    # device counts assignments (the GQL engine's Call Count), the LFTJ log
+    # also counts visited rejects, so calls are checked against the oracle's
+    # GQL engine on the same DPiso candidates, order and operands
+    from psm_benchmark_amd.matcher import match
+    _, dg, og = hprd
+    q = load_text(os.path.join(HPRD, rec["query"] + ".txt"))
+    r = match(dg, q, "DPiso_DPiso_LFTJ")
+    ref = ORun(og, OGraph(q), "DPiso_DPiso_GQL")
+    print(rec["query"], r.count, r.calls, r.mem_words, r.order, flush=True)
+    assert (r.count, r.mem_words) == (rec["embeddings"], rec["mem_words"])
+    assert r.order == ref.order and r.calls == ref.calls
+
+
+@pytest.mark.parametrize("qname", ["Q10-1", "Q10-4", "Q10-9"])
+def test_dpiso_plan_matches_oracle(hprd, qname):
+    from psm_benchmark_amd.matcher import Query
+    _, dg, og = hprd
+    q = load_text(os.path.join(HPRD, qname + ".txt"))
+    ref = ORun(og, OGraph(q), "DPiso_DPiso_LFTJ")
+    run = Query(dg, q)
+    run.filter("DPiso")
+    run.build_tables()
+    o, p = run.generate_plan("DPiso")
+    assert o.tolist() == ref.order
+    assert p.tolist()[1:] == ref.pivot[1:]
+
+
+@pytest.mark.parametrize("method", ["LDF", "NLF", "GQL", "CFL", "DPiso"])
 @pytest.mark.parametrize("qname", ["Q10-1", "Q10-3", "Q10-8"])
 def test_candidates_and_tables_match_oracle(hprd, method, qname):
     from psm_benchmark_amd.matcher import Query
@@ -85,13 +115,17 @@ def test_c1_reference_vectors_on_device(dev):
     gp = data["graph"]
     g = gen.chung_lu_graph(gp["V"], gp["E"], gp["labels"], gp["zipf"], seed=gp["seed"])
     dg = DeviceGraph(dev, g)
+    og = OGraph(g)
     for rec in data["records"]:
         q = gen.query(g, rec["n"], bool(rec["dense"]), rec["qseed"])
         r = match(dg, q, rec["method"])
         assert r.count == rec["embeddings"], rec
         assert r.mem_words == rec["mem_words"], rec
-        if rec["method"] != "GQL_GQL_LFTJ":   # LFTJ also counts visited rejects
+        if not rec["method"].endswith("_LFTJ"):
             assert r.calls == rec["calls"], rec
+        else:   # LFTJ also counts visited rejects: the GQL engine's count
+            f, o, _ = rec["method"].split("_")
+            assert r.calls == ORun(og, OGraph(q), f"{f}_{o}_GQL").calls, rec
 
 
 @pytest.mark.parametrize("seed", range(8))
@@ -102,7 +136,7 @@ def test_bruteforce_tiny_on_device(dev, seed):
     q = random_connected_query(rng, data, int(rng.integers(2, 5)))
     dg = DeviceGraph(dev, data)
     expect = count_embeddings(data, q)
-    for m in ("LDF_GQL_GQL", "NLF_GQL_GQL", "GQL_GQL_GQL", "CFL_CFL_EXPLORE"):
+    for m in ("LDF_GQL_GQL", "NLF_GQL_GQL", "GQL_GQL_GQL", "CFL_CFL_EXPLORE", "DPiso_DPiso_LFTJ"):
         assert match(dg, q, m).count == expect, m
 
 
@@ -165,6 +199,9 @@ def test_batch_equals_golden(hprd):
     counts, calls, status, st = batch_match(dg, qs, "CFL", "CFL", 0)
     assert counts.tolist() == [r["embeddings"] for r in recs]
     assert calls.tolist() == [r["calls"] for r in recs]
+    recs = golden(("DPiso_DPiso_LFTJ",))
+    counts, calls, status, st = batch_match(dg, qs, "DPiso", "DPiso", 0)
+    assert counts.tolist() == [r["embeddings"] for r in recs]
 
 
 @pytest.mark.parametrize("method", ["GQL_GQL_GQL", "CFL_CFL_EXPLORE"])

@@ -2,11 +2,12 @@
 checker of the device path.
 
 * HPRD-15 golden logs of the reference (test_anony/C_output/HPRD/L15/Q10):
-  embeddings, call counts and BuildTable memory checksum, GQL_GQL_GQL and
-  CFL_CFL_EXPLORE at one thread.
+  embeddings, call counts and BuildTable memory checksum, GQL_GQL_GQL,
+  CFL_CFL_EXPLORE and DPiso_DPiso_LFTJ at one thread.
 * Vectors from the reference binary built from its own sources on a seeded
   C1-shaped workload (tests/golden/c1_ref.json, made by make_ref_golden.py),
-  including GQL_GQL_LFTJ (call counts that include visited rejects).
+  including GQL_GQL_LFTJ (call counts that include visited rejects) and
+  DPiso_DPiso_LFTJ.
 * Brute force on tiny random graphs; k-cliques on a one-label complete graph
   (n!/(n-k)! embeddings); NLF/LDF candidate sets against their definition;
   the transpose symmetry of candidate-edge tables.
@@ -32,7 +33,7 @@ def hprd():
     return OGraph(load_text(os.path.join(HPRD, "HPRD-15.txt")))
 
 
-def golden_records(methods=("GQL_GQL_GQL", "CFL_CFL_EXPLORE")):
+def golden_records(methods=("GQL_GQL_GQL", "CFL_CFL_EXPLORE", "DPiso_DPiso_LFTJ")):
     recs = json.load(open(os.path.join(HPRD, "golden.json")))
     return [r for r in recs if r["threads"] == 1 and f'{r["filter"]}_{r["order"]}_{r["engine"]}' in methods]
 
@@ -76,7 +77,8 @@ def test_bruteforce_tiny(seed):
     assert q is not None
     expect = count_embeddings(data, q)
     d, qq = OGraph(data), OGraph(q)
-    for m in ("LDF_GQL_GQL", "NLF_GQL_EXPLORE", "GQL_GQL_GQL", "CFL_CFL_EXPLORE", "GQL_GQL_LFTJ"):
+    for m in ("LDF_GQL_GQL", "NLF_GQL_EXPLORE", "GQL_GQL_GQL", "CFL_CFL_EXPLORE", "GQL_GQL_LFTJ",
+              "DPiso_DPiso_LFTJ", "DPiso_DPiso_GQL"):
         assert ORun(d, qq, m).count == expect, m
 
 
@@ -85,7 +87,7 @@ def test_clique_closed_form(n, k):
     kn = csr_from_edges(n, np.array(list(itertools.combinations(range(n), 2))), np.zeros(n, np.uint32))
     kk = csr_from_edges(k, np.array(list(itertools.combinations(range(k), 2))), np.zeros(k, np.uint32))
     expect = math.factorial(n) // math.factorial(n - k)
-    for m in ("GQL_GQL_GQL", "CFL_CFL_EXPLORE", "GQL_GQL_LFTJ"):
+    for m in ("GQL_GQL_GQL", "CFL_CFL_EXPLORE", "GQL_GQL_LFTJ", "DPiso_DPiso_LFTJ"):
         r = ORun(OGraph(kn), OGraph(kk), m)
         assert r.count == expect
         # every partial injective map extends: calls = sum_d n!/(n-d-1)!
@@ -170,3 +172,37 @@ def test_tables_are_transposes(hprd):
             for j in range(len(off) - 1):
                 row = idx[off[j]:off[j + 1]]
                 assert np.all(np.diff(row.astype(np.int64)) > 0)
+
+
+@pytest.mark.parametrize("qname", ["Q10-1", "Q10-4", "Q10-9"])
+def test_dpiso_sets_are_pruned_ldf_sets(hprd, qname):
+    # DPiso prunes the LDF sets (it never adds a vertex) and keeps every
+    # vertex that takes part in an embedding: each vertex of an embedding
+    # found under the LDF sets is in the DPiso set of its query vertex
+    q = OGraph(load_text(os.path.join(HPRD, qname + ".txt")))
+    ldf = ORun(hprd, q, "LDF_GQL_GQL")
+    dp = ORun(hprd, q, "DPiso_DPiso_LFTJ")
+    assert dp.count == ldf.count
+    total_ldf = total_dp = 0
+    for u in range(dp.n):
+        a, b = set(ldf.candidates(u).tolist()), set(dp.candidates(u).tolist())
+        assert b <= a
+        total_ldf, total_dp = total_ldf + len(a), total_dp + len(b)
+    assert total_dp < total_ldf
+
+
+def test_dpiso_order_is_a_bfs_order_with_parent_pivots(hprd):
+    for k in range(1, 11):
+        qg = load_text(os.path.join(HPRD, f"Q10-{k}.txt"))
+        r = ORun(hprd, OGraph(qg), "DPiso_DPiso_LFTJ")
+        order, pivot = r.order, r.pivot
+        off = qg.offsets.astype(np.int64)
+        nbrs = lambda u: set(qg.neighbors[off[u]:off[u + 1]].tolist())
+        assert sorted(order) == list(range(qg.num_vertices))
+        pos = {u: i for i, u in enumerate(order)}
+        for i in range(1, len(order)):
+            p = pivot[i]
+            # the pivot is a neighbour placed earlier, and the first one in
+            # BFS order (the vertex that discovered it)
+            assert p in nbrs(order[i]) and pos[p] < i
+            assert p == min(nbrs(order[i]), key=lambda w: pos[w])
