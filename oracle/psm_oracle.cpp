@@ -10,7 +10,7 @@
 // (tests/test_oracle.py) by
 //   * the reference's recorded logs test_anony/C_output/HPRD/L15/Q10/*.txt
 //     (embedding counts, call counts and the BuildTable memory checksum for
-//     GQL_GQL_GQL and CFL_CFL_EXPLORE at one thread),
+//     GQL_GQL_GQL, CFL_CFL_EXPLORE and DPiso_DPiso_LFTJ at one thread),
 //   * brute-force enumeration on tiny graphs and closed forms (k-cliques),
 //   * the reference binary built from its own sources (oracle/_ref) on
 //     seeded synthetic graphs, when that binary is present.
