@@ -58,8 +58,8 @@ typedef struct psm_graph psm_graph;
 typedef struct psm_query psm_query;
 
 /* Enumeration statistics. `count` is min(cap, total) when capped. `calls` is
- * the reference's "Call Count" (one per assignment, EvaluateQuery.cpp:1009)
- * and is exact only for uncapped runs. Times are device-side (HIP events). */
+ * the reference's "Call Count" of the GQL and EXPLORE engines (one per
+ * assignment, EvaluateQuery.cpp:1009) and is exact only for uncapped runs. Times are device-side (HIP events). */
 typedef struct psm_enum_stats {
     uint64_t count;
     uint64_t calls;
@@ -74,6 +74,10 @@ typedef struct psm_enum_stats {
                                  (table row of an earlier query neighbour) plus 4 per
                                  survivor written unless the survivors are leaves;
                                  schedule-independent when the count is uncapped */
+    uint64_t visited_rejects; /* extensions present in every operand row whose data
+                                 vertex is already in the partial embedding: the LFTJ
+                                 engine's Call Count is calls + visited_rejects
+                                 (EvaluateQuery.cpp:278-295); exact when uncapped */
 } psm_enum_stats;
 
 /* ---- context -------------------------------------------------------- */

@@ -38,6 +38,7 @@ class EnumStats(ctypes.Structure):
         ("expand_ms", ctypes.c_double),
         ("probe_bytes", ctypes.c_uint64),
         ("algo_bytes", ctypes.c_uint64),
+        ("visited_rejects", ctypes.c_uint64),
     ]
 
 

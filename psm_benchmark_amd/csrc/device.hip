@@ -891,7 +891,10 @@ __global__ __launch_bounds__(kBlock) void k_cfl_prune(const uint32_t* __restrict
 // They are produced lazily: the shortest operand row is walked, every
 // element is binary-searched in the other rows and checked against the
 // earlier same-label assignments. Call counting follows the GQL / EXPLORE
-// engines (one call per assignment, EvaluateQuery.cpp:1009,1081).
+// engines (one call per assignment, EvaluateQuery.cpp:1009,1081); the
+// extensions present in every operand row whose data vertex an earlier depth
+// holds are counted apart (QCounters::rejects): the LFTJ engine also counts
+// those (EvaluateQuery.cpp:278-295 takes the visited test after the call).
 // ---------------------------------------------------------------------------
 struct LevelBase {
     uint64_t lo, hi;
@@ -926,8 +929,10 @@ __device__ __forceinline__ LevelBase level_base(const DPlan& P, const DOp* __res
     return b;
 }
 
+// 0: x is not in every operand row; 1: x is a valid extension (v_out set);
+// 3: x is in every operand row but its data vertex is already used.
 template <typename State>
-__device__ __forceinline__ bool test_element(const DPlan& P, const DOp* __restrict__ ops,
+__device__ __forceinline__ int test_element(const DPlan& P, const DOp* __restrict__ ops,
                                              const uint64_t* __restrict__ row_off,
                                              const uint32_t* __restrict__ idx,
                                              const uint32_t* __restrict__ cand_pool, uint32_t d,
@@ -948,22 +953,22 @@ __device__ __forceinline__ bool test_element(const DPlan& P, const DOp* __restri
             if (idx[mid] < x) lo = mid + 1;
             else hi = mid;
         }
-        if (lo == hi0 || idx[lo] != x) return false;
+        if (lo == hi0 || idx[lo] != x) return 0;
     }
     unsigned long long mask = P.same_label[d];
     if (!mask) {
         v_out = kInvalid;   // no earlier vertex shares the label: injective by construction
-        return true;
+        return 1;
     }
     const uint32_t v = cand_pool[P.cand_base[d] + x];
     bytes += 4;
     while (mask) {
         const uint32_t e = __ffsll((long long)mask) - 1;
         mask &= mask - 1;
-        if (st.M(e) == v) return false;
+        if (st.M(e) == v) return 3;
     }
     v_out = v;
-    return true;
+    return 1;
 }
 
 // Per-lane state of the persistent DFS, in LDS: per level the assigned
@@ -1060,11 +1065,15 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         }
         return;
     }
-    uint32_t c = 0;
+    uint32_t c = 0, rej = 0;
     for (uint64_t cur = b.lo; cur < b.hi; ++cur) {
         const uint32_t x = b.sel == kIdentity ? (uint32_t)cur : idx[cur];
         uint32_t v;
-        if (!test_element(P, ops, row_off, idx, cand_pool, d, x, b.sel, st, v, bytes)) continue;
+        const int r = test_element(P, ops, row_off, idx, cand_pool, d, x, b.sel, st, v, bytes);
+        if (r != 1) {
+            rej += r == 3;
+            continue;
+        }
         if (write) {
             DTask ch;
             ch.q = task.q;
@@ -1086,6 +1095,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         // each child is one assignment at this depth; children below the
         // root level are written survivors (expansion stops short of the leaf)
         atomicAdd(&qc[task.q].calls, (unsigned long long)c);
+        if (rej) atomicAdd(&qc[task.q].rejects, (unsigned long long)rej);
         atomicAdd(&qc[task.q].expand_bytes, bytes);
         const unsigned long long algo = 4ull * rows + (d >= 1 ? 4ull * c : 0ull);
         if (algo) atomicAdd(&qc[task.q].expand_algo_bytes, algo);
@@ -1192,7 +1202,8 @@ __device__ __forceinline__ uint64_t gallop(const uint32_t* __restrict__ idx, uin
 }
 
 // 0: x fails; 1: x is a valid extension (v_out set); 2: an operand is
-// exhausted, no later element of this level can match.
+// exhausted, no later element of this level can match; 3 (test_merge): x is
+// in every operand row but its data vertex is already used.
 __device__ __forceinline__ int step_cursor(const uint32_t* __restrict__ idx, uint64_t& c, uint64_t e,
                                            uint32_t x, unsigned long long& bytes) {
     c = gallop(idx, c, e, x, bytes);
@@ -1245,7 +1256,7 @@ __device__ __forceinline__ int test_merge(const DPlan& P, const DOp* __restrict_
     while (mask) {
         const uint32_t e = __ffsll((long long)mask) - 1;
         mask &= mask - 1;
-        if (st.M(e) == v) return 0;
+        if (st.M(e) == v) return 3;
     }
     v_out = v;
     return 1;
@@ -1280,8 +1291,9 @@ constexpr uint32_t kClaimMax = 8;     // queue slots a lane takes per claim at m
 // Adds a lane's running count (setting the query's done flag at its cap)
 // and statistics to query q's counters and clears them.
 __device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t cap, unsigned long long& lcount,
-                                           unsigned long long& lcalls, unsigned long long& lbytes,
-                                           unsigned long long& lalgo, bool& near_cap) {
+                                           unsigned long long& lcalls, unsigned long long& lrej,
+                                           unsigned long long& lbytes, unsigned long long& lalgo,
+                                           bool& near_cap) {
     if (lcount) {
         const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
         if (prev + lcount >= cap) atomicExch(&qc[q].done, 1u);
@@ -1289,9 +1301,10 @@ __device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t c
         lcount = 0;
     }
     if (lcalls) atomicAdd(&qc[q].calls, lcalls);
+    if (lrej) atomicAdd(&qc[q].rejects, lrej);
     if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);
     if (lalgo) atomicAdd(&qc[q].algo_bytes, lalgo);
-    lcalls = lbytes = lalgo = 0;
+    lcalls = lrej = lbytes = lalgo = 0;
 }
 
 // Persistent DFS. Every lane runs its own depth-first search over one task at
@@ -1331,7 +1344,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     uint32_t q = 0, tdepth = 0, d = 0, n = 0;
     Level L{};
     Level Lp{};   // the level above the leaf depth, kept while the lane is at the leaf
-    unsigned long long lcount = 0, lcalls = 0, lbytes = 0, lalgo = 0;
+    unsigned long long lcount = 0, lcalls = 0, lrej = 0, lbytes = 0, lalgo = 0;
     unsigned long long busy_iters = 0, all_iters = 0;
     unsigned long long flush_at = 4096;   // leaf counts a lane holds before adding them to its query
     bool near_cap = false;                // the query's count was at half its cap or more at the last flush
@@ -1396,7 +1409,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             const uint32_t tq = coherent_u32(&task->q);
             if (tq != q) {
                 // the lane's counters so far belong to the previous query
-                lane_flush(qc, q, P->cap, lcount, lcalls, lbytes, lalgo, near_cap);
+                lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap);
                 if (pend_done) atomicSub(pending + q, pend_done);
                 pend_done = 0;
             }
@@ -1475,6 +1488,8 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 const int r = test_merge(*P, ops, row_off, idx, cand_pool, d, x, L, st, v, lbytes, keep_v);
                 if (r == 2) {
                     L.cur = L.end;   // an operand ran out: nothing further at this level
+                } else if (r == 3) {
+                    ++lrej;
                 } else if (r == 1) {
                     ++lcalls;
                     if (count_leaf) {
@@ -1592,7 +1607,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         const bool more = !active && wslot + 1 < wend;   // another claimed slot follows
         const bool idle_next = !active && !more;
         if (lcount >= flush_at || idle_next || (steps & 1023) == 0)
-            lane_flush(qc, q, P->cap, lcount, lcalls, lbytes, lalgo, near_cap);
+            lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap);
         if (finished) ++pend_done;
         if (!active) {
             ++busy_done;
@@ -1607,7 +1622,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
         }
     }
-    lane_flush(qc, q, P->cap, lcount, lcalls, lbytes, lalgo, near_cap);
+    lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap);
     wave_add(&Q->lane_steps, busy_iters);
     wave_add(&Q->lane_iters, all_iters);
 }
@@ -1865,7 +1880,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
         uint32_t g = t, cnt = 0, it = 0;
         int cache_g = -1;
         uint64_t total = 0, flat = 0;
-        unsigned long long lcount = 0, lcalls = 0;
+        unsigned long long lcount = 0, lcalls = 0, lrej = 0;
         bool finished = false;
         while (true) {
             if (cache_g != (int)g) {
@@ -1908,7 +1923,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
 
             // ---- one chunk of 64 positions of generation g
             const uint64_t fpos = flat + lane;
-            bool ok = false;
+            bool ok = false, rej = false;
             uint32_t xx = 0, k = 0;
             all_pos += 1;
             if (fpos < total) {
@@ -1956,6 +1971,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                         const uint32_t me = (g > t && e == g - 1) ? S.m_v[k] : S.M[e];
                         if (me == v) {
                             ok = false;
+                            rej = true;
                             break;
                         }
                     }
@@ -1983,6 +1999,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                 if (!hb && lane == 0) S.gen_m[g] = cnt;
             }
             lcalls += ns;
+            lrej += (uint32_t)__popcll(__ballot(rej));
             if (!leaf && g >= 1 && lane == 0) algo += 4ull * ns;   // survivors written below the root level
             if (leaf) {
                 lcount += ns;
@@ -2018,13 +2035,14 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                         atomicAdd(&qc[q].calls, lcalls);
                         if (lcount && prev + lcount >= P.cap) atomicExch(&qc[q].done, 1u);
                     }
+                    if (lrej) atomicAdd(&qc[q].rejects, lrej);
                     stop = __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout);
                     if (!stop && past_deadline(Q)) {
                         atomicExch(&Q->timeout, 1u);
                         stop = 1;
                     }
                 }
-                lcount = lcalls = 0;
+                lcount = lcalls = lrej = 0;
                 if (__shfl(stop, 0)) break;
             }
             // split off work while the queue is short
@@ -2086,6 +2104,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                 atomicAdd(&qc[q].calls, lcalls);
                 if (lcount && prev + lcount >= P.cap) atomicExch(&qc[q].done, 1u);
             }
+            if (lrej) atomicAdd(&qc[q].rejects, lrej);
             if (finished) atomicSub(pending + q, 1u);
             atomicSub(&Q->busy, 1u);
         }

@@ -158,6 +158,9 @@ struct JoinKeys {
 struct QCounters {
     unsigned long long count;
     unsigned long long calls;
+    // extensions in every operand row whose data vertex an earlier depth
+    // holds (the LFTJ engine's Call Count adds them)
+    unsigned long long rejects;
     unsigned long long probe_bytes;    // bytes the DFS kernel touched (probes, base elements, rows located)
     unsigned long long expand_bytes;   // the same for task pre-expansion
     // Algorithmic enumeration bytes: for every search-tree node below the

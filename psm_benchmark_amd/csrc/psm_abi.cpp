@@ -370,7 +370,7 @@ struct QueryHost {
     std::vector<uint64_t> tab_nnz;    // n*n
     Plan plan;
     bool planned = false;
-    uint64_t count = 0, calls = 0, probe_bytes = 0, algo_bytes = 0;
+    uint64_t count = 0, calls = 0, rejects = 0, probe_bytes = 0, algo_bytes = 0;
     bool capped = false;
     bool timed_out = false;           // partial count: the reference's "Overtime: 1"
 };
@@ -1440,6 +1440,7 @@ int Batch::evaluate(uint64_t cap, double enum_limit) {
         q->count = cap ? std::min<uint64_t>(cap, qc[i].count) : qc[i].count;
         q->capped = cap && qc[i].count >= cap;
         q->calls = qc[i].calls;
+        q->rejects = qc[i].rejects;
         q->probe_bytes = qc[i].probe_bytes;
         q->algo_bytes = qc[i].algo_bytes + qc[i].expand_algo_bytes;
         q->timed_out = qc[i].timed_out != 0;
@@ -1863,6 +1864,7 @@ int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
         st->expand_ms = q->batch.t_expand;
         st->probe_bytes = q->qh.probe_bytes;
         st->algo_bytes = q->qh.algo_bytes;
+        st->visited_rejects = q->qh.rejects;
     }
     return r;
 }

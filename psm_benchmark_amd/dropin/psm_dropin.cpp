@@ -349,6 +349,8 @@ void GenerateQueryPlan::generateDPisoQueryPlan(const Graph*, const Graph* query,
     check(psm_generate_plan(existing(query), PSM_ORDER_DPISO, order, pivot), "psm_generate_plan");
 }
 
+std::string EvaluateQuery::engine_type = "GQL";
+
 size_t EvaluateQuery::evaluate(const Graph*, const Graph* query, size_t limit, size_t& calls,
                                bool& overtime, double time_limit_s) {
     psm_set_time_limit(psm_dropin::context(), time_limit_s);
@@ -357,7 +359,7 @@ size_t EvaluateQuery::evaluate(const Graph*, const Graph* query, size_t limit, s
     const int r = psm_evaluate(existing(query), cap, &st);
     check(r, "psm_evaluate");
     overtime = r == PSM_ETIMEOUT;
-    calls = st.calls;
+    calls = st.calls + (engine_type == "LFTJ" ? st.visited_rejects : 0);
     return st.count;
 }
 

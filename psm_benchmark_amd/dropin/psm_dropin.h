@@ -131,6 +131,11 @@ public:
 
 class EvaluateQuery {
 public:
+    // The engine whose Call Count evaluate() reports ("GQL", "EXPLORE" or
+    // "LFTJ"); the reference picks EvaluateQuery::enum_method by engine
+    // (EvaluateQuery.h:133, StudyPerformance.cpp:1692-1706). Counts do not
+    // depend on it.
+    static std::string engine_type;
     // Counts embeddings of the query prepared by the calls above (at most
     // output_limit_num); call_count receives the reference's Call Count.
     // Returns the embedding count. overtime is set when time_limit_s passes.

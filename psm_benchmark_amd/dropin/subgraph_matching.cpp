@@ -121,12 +121,17 @@ int main(int argc, char** argv) {
             else if (o == "CFL") GenerateQueryPlan::generateCFLQueryPlan(&data, &query, cnt, order.data(), pivot.data());
             else if (o == "DPiso") GenerateQueryPlan::generateDPisoQueryPlan(&data, &query, cnt, order.data(), pivot.data());
             else { Line() << "The specified order type '" << o << "' is not supported." << std::endl; return 1; }
+            if (e != "GQL" && e != "EXPLORE" && e != "LFTJ") {
+                Line() << "The specified engine type '" << e << "' is not supported." << std::endl;
+                return 1;
+            }
             const int64_t plan_ns = ns(t, now());
             Line() << "-----" << std::endl;
             Line() << "Enumerate..." << std::endl;
             t = now();
             size_t calls = 0;
             bool overtime = false;
+            EvaluateQuery::engine_type = e;
             const size_t emb = EvaluateQuery::evaluate(&data, &query, limit, calls, overtime, time_limit);
             const int64_t enum_ns = ns(t, now());
             const int64_t pre_ns = filter_ns + table_ns + plan_ns;

@@ -5,7 +5,9 @@ Method strings follow the reference's ``-BackMethods`` (``FILTER_ORDER_ENGINE``,
 e.g. ``GQL_GQL_GQL`` or ``CFL_CFL_EXPLORE``, matching/StudyPerformance.cpp:
 1018-1030). Engines GQL, EXPLORE and LFTJ count the same embeddings; the
 device path computes every depth's extensions as the intersection of the
-candidate-edge table rows of all earlier neighbours.
+candidate-edge table rows of all earlier neighbours. The Call Count follows
+the engine: one per assignment for GQL and EXPLORE, plus the extensions
+rejected because their data vertex is already used for LFTJ.
 """
 from __future__ import annotations
 
@@ -185,7 +187,7 @@ class MatchResult:
 def match(dg: DeviceGraph, q: CSRGraph, method: str = "GQL_GQL_GQL", cap: int = 0) -> MatchResult:
     """The plain path of split_C_test at one query: filter, sort, BuildTable,
     plan, enumerate (StudyPerformance.cpp:1164-1716)."""
-    f, o, _ = parse_method(method)
+    f, o, engine = parse_method(method)
     fname = method.split("_")[0]
     oname = method.split("_")[1]
     run = Query(dg, q)
@@ -195,7 +197,8 @@ def match(dg: DeviceGraph, q: CSRGraph, method: str = "GQL_GQL_GQL", cap: int = 
         mem = run.memory_cost_bytes() // 4
         order, _ = run.generate_plan(oname)
         st = run.evaluate(cap)
-        return MatchResult(int(st.count), int(st.calls), int(mem), ok, [int(x) for x in order],
+        calls = int(st.calls) + (int(st.visited_rejects) if engine == "LFTJ" else 0)
+        return MatchResult(int(st.count), calls, int(mem), ok, [int(x) for x in order],
                            bool(st.capped), float(st.kernel_ms + st.expand_ms), int(st.probe_bytes),
                            [int(x) for x in run.candidate_counts()], bool(st.timed_out), int(st.algo_bytes))
     finally:

@@ -35,7 +35,6 @@ def test_driver_log_matches_golden(tmp_path, qname):
     for b in blocks:
         g = gold[(b["Filter Type"], b["Order Type"], b["Engine Type"])]
         assert int(b["#Embeddings"]) == g["embeddings"]
-        if b["Engine Type"] != "LFTJ":   # the device counts assignments only
-            assert int(b["Call Count"]) == g["calls"]
+        assert int(b["Call Count"]) == g["calls"]
         assert round(float(b["Memory cost (MB)"]) * 2**20 / 4) == g["mem_words"]
         assert b["Overtime"] == "0"

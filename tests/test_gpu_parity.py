@@ -57,18 +57,31 @@ def test_hprd_golden_on_device(hprd, rec):
 
 @pytest.mark.parametrize("rec", golden(("DPiso_DPiso_LFTJ",)), ids=lambda r: r["query"])
 def test_hprd_dpiso_golden_on_device(hprd, rec):
-    # counts and the BuildTable checksum against the reference's log; the
-    # device counts assignments (the GQL engine's Call Count), the LFTJ log
-    # also counts visited rejects, so calls are checked against the oracle's
-    # GQL engine on the same DPiso candidates, order and operands
+    # the LFTJ Call Count (assignments + extensions whose data vertex is
+    # already used) against the reference's log, and the assignments alone
+    # against the oracle's GQL engine on the same DPiso plan
     from psm_benchmark_amd.matcher import match
     _, dg, og = hprd
     q = load_text(os.path.join(HPRD, rec["query"] + ".txt"))
     r = match(dg, q, "DPiso_DPiso_LFTJ")
+    g = match(dg, q, "DPiso_DPiso_GQL")
     ref = ORun(og, OGraph(q), "DPiso_DPiso_GQL")
     print(rec["query"], r.count, r.calls, r.mem_words, r.order, flush=True)
-    assert (r.count, r.mem_words) == (rec["embeddings"], rec["mem_words"])
-    assert r.order == ref.order and r.calls == ref.calls
+    assert (r.count, r.calls, r.mem_words) == (rec["embeddings"], rec["calls"], rec["mem_words"])
+    assert r.order == ref.order and g.calls == ref.calls
+
+
+@pytest.mark.parametrize("qname", ["Q10-2", "Q10-5", "Q10-10"])
+def test_lftj_calls_match_oracle(hprd, qname):
+    # GQL order + LFTJ engine (the oracle's LFTJ calls are pinned by the
+    # reference binary's GQL_GQL_LFTJ vectors)
+    from psm_benchmark_amd.matcher import match
+    _, dg, og = hprd
+    q = load_text(os.path.join(HPRD, qname + ".txt"))
+    r = match(dg, q, "GQL_GQL_LFTJ")
+    ref = ORun(og, OGraph(q), "GQL_GQL_LFTJ")
+    assert (r.count, r.calls) == (ref.count, ref.calls)
+    assert r.calls > match(dg, q, "GQL_GQL_GQL").calls
 
 
 @pytest.mark.parametrize("qname", ["Q10-1", "Q10-4", "Q10-9"])
@@ -115,17 +128,12 @@ def test_c1_reference_vectors_on_device(dev):
     gp = data["graph"]
     g = gen.chung_lu_graph(gp["V"], gp["E"], gp["labels"], gp["zipf"], seed=gp["seed"])
     dg = DeviceGraph(dev, g)
-    og = OGraph(g)
     for rec in data["records"]:
         q = gen.query(g, rec["n"], bool(rec["dense"]), rec["qseed"])
         r = match(dg, q, rec["method"])
         assert r.count == rec["embeddings"], rec
         assert r.mem_words == rec["mem_words"], rec
-        if not rec["method"].endswith("_LFTJ"):
-            assert r.calls == rec["calls"], rec
-        else:   # LFTJ also counts visited rejects: the GQL engine's count
-            f, o, _ = rec["method"].split("_")
-            assert r.calls == ORun(og, OGraph(q), f"{f}_{o}_GQL").calls, rec
+        assert r.calls == rec["calls"], rec
 
 
 @pytest.mark.parametrize("seed", range(8))
