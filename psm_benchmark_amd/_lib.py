@@ -68,6 +68,18 @@ class JoinStats(ctypes.Structure):
     ]
 
 
+class StarJoinStats(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_uint64),
+        ("units", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("unit_rows", ctypes.c_uint64),
+        ("max_inter_rows", ctypes.c_uint64),
+        ("enum_ms", ctypes.c_double),
+        ("join_ms", ctypes.c_double),
+    ]
+
+
 class Result(ctypes.Structure):
     _fields_ = [
         ("request_id", ctypes.c_uint64),
@@ -111,6 +123,7 @@ SIGNATURES = {
     "psm_generate_plan": (ctypes.c_int, [VP, ctypes.c_int, U32P, U32P]),
     "psm_evaluate": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
     "psm_join_count": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(JoinStats)]),
+    "psm_star_join_count": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(StarJoinStats)]),
     "psm_match": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
     "psm_batch_match": (ctypes.c_int, [VP, VP, ctypes.c_uint32, U32P, U64P, U64P, U64P, U32P, U32P, U32P,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_uint64, U64P, U64P, U32P,

@@ -2133,6 +2133,18 @@ __device__ __forceinline__ unsigned long long key_fingerprint(const uint32_t* ro
     return h | 1ull;   // 0 marks an empty slot
 }
 
+// A probe row matches build row `br` when the key columns agree and no
+// non-key data vertex of one equals a non-key data vertex of the other.
+__device__ __forceinline__ bool join_match(const uint32_t* br, const JoinKeys& bk, const uint32_t* pr,
+                                           const JoinKeys& pk) {
+    for (uint32_t i = 0; i < pk.nkeys; ++i)
+        if (br[bk.key[i]] != pr[pk.key[i]]) return false;
+    for (uint32_t i = 0; i < bk.nrest; ++i)
+        for (uint32_t j = 0; j < pk.nrest; ++j)
+            if (br[bk.rest[i]] == pr[pk.rest[j]]) return false;
+    return true;
+}
+
 __global__ __launch_bounds__(kBlock) void k_join_build(const uint32_t* __restrict__ rows, uint64_t n_rows,
                                                         uint32_t width, JoinKeys keys,
                                                         unsigned long long* fp, uint32_t* head,
@@ -2175,15 +2187,73 @@ __global__ __launch_bounds__(kBlock) void k_join_probe(
         for (; a != kInvalid; a = next[a]) {
             const uint32_t* br = build_rows + (uint64_t)a * build_width;
             by += 4ull * build_width + 4;
-            bool ok = true;
-            for (uint32_t i = 0; i < pk.nkeys && ok; ++i) ok = br[bk.key[i]] == pr[pk.key[i]];
-            for (uint32_t i = 0; i < bk.nrest && ok; ++i)
-                for (uint32_t j = 0; j < pk.nrest && ok; ++j) ok = br[bk.rest[i]] != pr[pk.rest[j]];
-            c += ok;
+            c += join_match(br, bk, pr, pk);
         }
     }
     wave_add(count, c);
     wave_add(bytes, by);
+}
+
+// Materialising probe of a multi-unit join (JoinCollect::concateTwoTabels,
+// JoinCollect.cpp:237-315, restated): each output row is the probe row
+// followed by the build row's non-key columns. A thread counts its probe
+// row's matches, the wavefront reserves room for all of its matches with one
+// atomic, and the thread walks the chain again to write them; rows past `cap`
+// are dropped and flag `overflow`.
+__global__ __launch_bounds__(kBlock) void k_join_probe_emit(
+    const uint32_t* __restrict__ build_rows, uint32_t build_width, JoinKeys bk,
+    const uint32_t* __restrict__ probe_rows, uint64_t n_probe, uint32_t probe_width, JoinKeys pk,
+    const unsigned long long* __restrict__ fp, const uint32_t* __restrict__ head, uint64_t mask,
+    const uint32_t* __restrict__ next, uint32_t* __restrict__ out, unsigned long long cap,
+    unsigned long long* count, uint32_t* overflow) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const uint64_t rounded = (n_probe + stride - 1) / stride * stride;
+    const uint32_t out_width = probe_width + bk.nrest;
+    const uint32_t lane = lane_id();
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < rounded; r += stride) {
+        const bool live = r < n_probe;
+        const uint32_t* pr = probe_rows + (live ? r : 0) * probe_width;
+        uint32_t a0 = kInvalid;
+        unsigned long long m = 0;
+        if (live) {
+            const unsigned long long h = key_fingerprint(pr, pk);
+            uint64_t slot = h & mask;
+            while (true) {
+                const unsigned long long f = fp[slot];
+                if (f == 0ull) break;
+                if (f == h) { a0 = head[slot]; break; }
+                slot = (slot + 1) & mask;
+            }
+            for (uint32_t a = a0; a != kInvalid; a = next[a])
+                m += join_match(build_rows + (uint64_t)a * build_width, bk, pr, pk);
+        }
+        // inclusive scan of m over the wavefront; lane 63 holds the total
+        unsigned long long inc = m;
+        for (uint32_t o = 1; o < kWave; o <<= 1) {
+            const uint32_t lo = __shfl_up((uint32_t)inc, o), hi = __shfl_up((uint32_t)(inc >> 32), o);
+            if (lane >= o) inc += ((unsigned long long)hi << 32) | lo;
+        }
+        const unsigned long long total =
+            ((unsigned long long)__shfl((uint32_t)(inc >> 32), kWave - 1) << 32) | __shfl((uint32_t)inc, kWave - 1);
+        unsigned long long base = 0;
+        if (lane == 0 && total) base = atomicAdd(count, total);
+        base = ((unsigned long long)__shfl((uint32_t)(base >> 32), 0) << 32) | __shfl((uint32_t)base, 0);
+        unsigned long long pos = base + inc - m;
+        if (m) {
+            for (uint32_t a = a0; a != kInvalid; a = next[a]) {
+                const uint32_t* br = build_rows + (uint64_t)a * build_width;
+                if (!join_match(br, bk, pr, pk)) continue;
+                if (pos < cap) {
+                    uint32_t* o = out + pos * out_width;
+                    for (uint32_t c = 0; c < probe_width; ++c) o[c] = pr[c];
+                    for (uint32_t c = 0; c < bk.nrest; ++c) o[probe_width + c] = br[bk.rest[c]];
+                } else {
+                    *overflow = 1u;
+                }
+                ++pos;
+            }
+        }
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -2619,6 +2689,18 @@ hipError_t launch_join_probe(const uint32_t* build_rows, uint32_t build_width, J
     hipLaunchKernelGGL(k_join_probe, dim3(grid_for(n_probe)), dim3(kBlock), 0, s, build_rows, build_width,
                        build_keys, probe_rows, n_probe, probe_width, probe_keys, fp, head, mask, next, count,
                        bytes);
+    return hipGetLastError();
+}
+
+hipError_t launch_join_probe_emit(const uint32_t* build_rows, uint32_t build_width, JoinKeys build_keys,
+                                  const uint32_t* probe_rows, uint64_t n_probe, uint32_t probe_width,
+                                  JoinKeys probe_keys, const unsigned long long* fp, const uint32_t* head,
+                                  uint64_t mask, const uint32_t* next, uint32_t* out, unsigned long long cap,
+                                  unsigned long long* count, uint32_t* overflow, hipStream_t s) {
+    if (n_probe == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_join_probe_emit, dim3(grid_for(n_probe)), dim3(kBlock), 0, s, build_rows, build_width,
+                       build_keys, probe_rows, n_probe, probe_width, probe_keys, fp, head, mask, next, out, cap,
+                       count, overflow);
     return hipGetLastError();
 }
 

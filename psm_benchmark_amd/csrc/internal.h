@@ -263,6 +263,11 @@ hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_of
 hipError_t launch_join_build(const uint32_t* rows, uint64_t n_rows, uint32_t width, JoinKeys keys,
                              unsigned long long* fp, uint32_t* head, uint64_t mask, uint32_t* next,
                              hipStream_t s);
+hipError_t launch_join_probe_emit(const uint32_t* build_rows, uint32_t build_width, JoinKeys build_keys,
+                                  const uint32_t* probe_rows, uint64_t n_probe, uint32_t probe_width,
+                                  JoinKeys probe_keys, const unsigned long long* fp, const uint32_t* head,
+                                  uint64_t mask, const uint32_t* next, uint32_t* out, unsigned long long cap,
+                                  unsigned long long* count, uint32_t* overflow, hipStream_t s);
 hipError_t launch_join_probe(const uint32_t* build_rows, uint32_t build_width, JoinKeys build_keys,
                              const uint32_t* probe_rows, uint64_t n_probe, uint32_t probe_width,
                              JoinKeys probe_keys, const unsigned long long* fp, const uint32_t* head,

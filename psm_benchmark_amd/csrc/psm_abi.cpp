@@ -383,6 +383,12 @@ struct JoinResult {
     double enum_ms = 0, join_ms = 0;
 };
 
+struct StarJoinResult {
+    uint64_t count = 0, unit_rows = 0, max_inter_rows = 0;
+    uint32_t units = 0;
+    double enum_ms = 0, join_ms = 0;
+};
+
 struct Batch {
     psm_ctx* ctx = nullptr;
     const psm_graph* g = nullptr;
@@ -426,6 +432,7 @@ struct Batch {
                   std::vector<QCounters>& qc, bool& timed_out);
     int evaluate(uint64_t cap, double enum_limit = -1);
     int join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr);
+    int star_join_count(QueryHost& q, uint64_t max_rows, StarJoinResult& jr);
     int host_table(const QueryHost& q, uint32_t a, uint32_t b, std::vector<uint32_t>& off,
                    std::vector<uint32_t>& idxv, hipStream_t st);
     int nlf_count(const QGraph& q, uint32_t u, uint32_t label_override, uint32_t& out, hipStream_t st,
@@ -1848,6 +1855,203 @@ int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t*
     return PSM_OK;
 }
 
+// ---- star decomposition + join sequence --------------------------------------
+// The STAR split (SplitGraph.cpp:895-910) restated as a greedy vertex cover:
+// vertices in id order take every query edge of theirs not yet taken, so
+// every query edge lies in exactly one star (centre + leaves). Each star is
+// enumerated on the device into rows (centre first, leaves by ascending
+// candidate count); the stars are then joined one at a time
+// (JoinCollect::sequence_collect, JoinCollect.cpp:60-153): start from the
+// star with the fewest rows, and join next the star sharing a vertex with the
+// joined ones that has the fewest rows (more shared vertices on ties), on the
+// shared vertices (ascending query id, keyTraits.h) with data vertices
+// distinct across the two sides. Each join counts its matches first; an
+// intermediate result is then materialised at exactly that size, the last
+// join only counts. The count equals the plain count.
+int Batch::star_join_count(QueryHost& q, uint64_t max_rows, StarJoinResult& jr) {
+    tls_stream = s();
+    const uint32_t n = q.qg.n;
+    jr = StarJoinResult();
+    if (!tables_built) return set_err(ctx, PSM_ESTATE, "build tables before joining");
+    struct Unit {
+        std::vector<uint32_t> cols;   // query vertex of each row column (= enumeration order)
+        std::vector<uint8_t> edge;    // n*n: the unit's query edges
+        std::unique_ptr<DBuf> rows;
+        uint64_t nrows = 0;
+    };
+    std::vector<Unit> units;
+    std::vector<uint8_t> taken((size_t)n * n, 0);
+    for (uint32_t u = 0; u < n; ++u) {
+        std::vector<uint32_t> leaves;
+        for (uint32_t i = q.qg.off[u]; i < q.qg.off[u + 1]; ++i) {
+            const uint32_t w = q.qg.adj[i];
+            if (!taken[(size_t)u * n + w]) leaves.push_back(w);
+        }
+        if (leaves.empty()) continue;
+        std::stable_sort(leaves.begin(), leaves.end(),
+                         [&](uint32_t a, uint32_t b) { return q.cand_count[a] < q.cand_count[b]; });
+        Unit un;
+        un.rows.reset(new DBuf());
+        un.cols.push_back(u);
+        un.edge.assign((size_t)n * n, 0);
+        for (uint32_t w : leaves) {
+            un.cols.push_back(w);
+            taken[(size_t)u * n + w] = taken[(size_t)w * n + u] = 1;
+            un.edge[(size_t)u * n + w] = un.edge[(size_t)w * n + u] = 1;
+        }
+        units.push_back(std::move(un));
+    }
+    if (units.empty()) return set_err(ctx, PSM_EINVAL, "query has no edge to split");
+    jr.units = (uint32_t)units.size();
+    DBuf cnt, ovf;
+    HIP_TRY(ctx, cnt.alloc(8));
+    HIP_TRY(ctx, ovf.alloc(4));
+    const double t0 = now_ms();
+    for (Unit& un : units) {
+        const uint32_t w = (uint32_t)un.cols.size();
+        HIP_TRY(ctx, un.rows->alloc(std::max<uint64_t>(max_rows, 1) * w * 4));
+        HIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, 8, s()));
+        HIP_TRY(ctx, hipMemsetAsync(ovf.p, 0, 4, s()));
+        std::vector<DPlan> hp(1);
+        std::vector<DOp> hops;
+        make_plan(q, un.cols, &un.edge, 0, hp[0], hops);
+        std::vector<QCounters> qc;
+        bool to = false;
+        EmitSink sink{un.rows->as<uint32_t>(), cnt.as<unsigned long long>(), max_rows, ovf.as<uint32_t>()};
+        PSM_TRY(run_plans(hp, hops, sink, qc, to));
+        uint32_t overflow = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(&un.nrows, cnt.p, 8, hipMemcpyDeviceToHost, s()));
+        HIP_TRY(ctx, hipMemcpyAsync(&overflow, ovf.p, 4, hipMemcpyDeviceToHost, s()));
+        HIP_TRY(ctx, hipStreamSynchronize(s()));
+        if (to) return set_err(ctx, PSM_ETIMEOUT, "star enumeration passed the time limit");
+        if (overflow) return set_err(ctx, PSM_ENOMEM, "a star has more rows than max_rows");
+        jr.unit_rows += un.nrows;
+    }
+    jr.enum_ms = now_ms() - t0;
+    const double t1 = now_ms();
+    // the joined result so far
+    std::vector<uint8_t> used(units.size(), 0);
+    size_t first = 0;
+    for (size_t i = 1; i < units.size(); ++i)
+        if (units[i].nrows < units[first].nrows) first = i;
+    used[first] = 1;
+    std::vector<uint32_t> cols = units[first].cols;
+    std::unique_ptr<DBuf> cur;      // owned intermediate rows (none while the first unit is the result)
+    const uint32_t* cur_rows = units[first].rows->as<uint32_t>();
+    uint64_t cur_n = units[first].nrows;
+    if (units.size() == 1) jr.count = cur_n;
+    for (size_t step = 1; step < units.size(); ++step) {
+        std::vector<uint8_t> in_cur(n, 0);
+        for (uint32_t c : cols) in_cur[c] = 1;
+        size_t nxt = units.size();
+        uint32_t nxt_shared = 0;
+        for (size_t i = 0; i < units.size(); ++i) {
+            if (used[i]) continue;
+            uint32_t sh = 0;
+            for (uint32_t c : units[i].cols) sh += in_cur[c];
+            if (!sh) continue;
+            if (nxt == units.size() || units[i].nrows < units[nxt].nrows ||
+                (units[i].nrows == units[nxt].nrows && sh > nxt_shared)) {
+                nxt = i;
+                nxt_shared = sh;
+            }
+        }
+        if (nxt == units.size()) return set_err(ctx, PSM_EINVAL, "query is not connected");
+        used[nxt] = 1;
+        Unit& un = units[nxt];
+        const bool last = step + 1 == units.size();
+        if (cur_n == 0 || un.nrows == 0) {   // nothing can match
+            jr.count = 0;
+            break;
+        }
+        // key columns on both sides, ascending query vertex id
+        JoinKeys kc{}, ku{};
+        std::vector<uint32_t> pos_c(n, kInvalid), pos_u(n, kInvalid);
+        for (uint32_t i = 0; i < cols.size(); ++i) pos_c[cols[i]] = i;
+        for (uint32_t i = 0; i < un.cols.size(); ++i) pos_u[un.cols[i]] = i;
+        for (uint32_t v = 0; v < n; ++v) {
+            if (pos_c[v] != kInvalid && pos_u[v] != kInvalid) {
+                kc.key[kc.nkeys++] = pos_c[v];
+                ku.key[ku.nkeys++] = pos_u[v];
+            } else if (pos_c[v] != kInvalid) {
+                kc.rest[kc.nrest++] = pos_c[v];
+            } else if (pos_u[v] != kInvalid) {
+                ku.rest[ku.nrest++] = pos_u[v];
+            }
+        }
+        // build on the smaller side (hashjoin.cpp:60-69)
+        const bool build_cur = cur_n <= un.nrows;
+        const uint32_t* b_rows = build_cur ? cur_rows : un.rows->as<uint32_t>();
+        const uint32_t* p_rows = build_cur ? un.rows->as<uint32_t>() : cur_rows;
+        const uint64_t b_n = build_cur ? cur_n : un.nrows, p_n = build_cur ? un.nrows : cur_n;
+        const uint32_t b_w = build_cur ? (uint32_t)cols.size() : (uint32_t)un.cols.size();
+        const uint32_t p_w = build_cur ? (uint32_t)un.cols.size() : (uint32_t)cols.size();
+        const JoinKeys& bk = build_cur ? kc : ku;
+        const JoinKeys& pk = build_cur ? ku : kc;
+        const std::vector<uint32_t>& b_cols = build_cur ? cols : un.cols;
+        const std::vector<uint32_t>& p_cols = build_cur ? un.cols : cols;
+        if (b_n >= kInvalid) return set_err(ctx, PSM_ENOMEM, "join side above 2^32 rows");
+        uint64_t slots = 16;
+        while (slots < 2 * b_n) slots <<= 1;
+        DBuf fp, head, next;
+        HIP_TRY(ctx, fp.alloc(slots * 8));
+        HIP_TRY(ctx, hipMemsetAsync(fp.p, 0, slots * 8, s()));
+        HIP_TRY(ctx, head.alloc(slots * 4));
+        HIP_TRY(ctx, launch_fill_u32(head.as<uint32_t>(), kInvalid, slots, s()));
+        HIP_TRY(ctx, next.alloc(b_n * 4));
+        TIMED(ctx, kKJoin,
+              launch_join_build(b_rows, b_n, b_w, bk, fp.as<unsigned long long>(), head.as<uint32_t>(), slots - 1,
+                                next.as<uint32_t>(), s()));
+        // count the matches (the last join stops here), then materialise
+        // exactly that many rows
+        HIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, 8, s()));
+        TIMED(ctx, kKJoin,
+              launch_join_probe(b_rows, b_w, bk, p_rows, p_n, p_w, pk, fp.as<unsigned long long>(),
+                                head.as<uint32_t>(), slots - 1, next.as<uint32_t>(), cnt.as<unsigned long long>(),
+                                ctx->d_kbytes + kKJoin, s()));
+        uint64_t matches = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(&matches, cnt.p, 8, hipMemcpyDeviceToHost, s()));
+        HIP_TRY(ctx, hipStreamSynchronize(s()));
+        if (last) {
+            jr.count = matches;
+            break;
+        }
+        if (matches > max_rows) return set_err(ctx, PSM_ENOMEM, "an intermediate join result has more rows than max_rows");
+        if (matches == 0) {
+            jr.count = 0;
+            break;
+        }
+        // output columns: the probe row, then the build row's non-key columns
+        std::vector<uint32_t> out_cols = p_cols;
+        for (uint32_t i = 0; i < bk.nrest; ++i) out_cols.push_back(b_cols[bk.rest[i]]);
+        std::unique_ptr<DBuf> out(new DBuf());
+        HIP_TRY(ctx, out->alloc(matches * out_cols.size() * 4));
+        HIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, 8, s()));
+        HIP_TRY(ctx, hipMemsetAsync(ovf.p, 0, 4, s()));
+        TIMED(ctx, kKJoin,
+              launch_join_probe_emit(b_rows, b_w, bk, p_rows, p_n, p_w, pk, fp.as<unsigned long long>(),
+                                     head.as<uint32_t>(), slots - 1, next.as<uint32_t>(), out->as<uint32_t>(),
+                                     matches, cnt.as<unsigned long long>(), ovf.as<uint32_t>(), s()));
+        uint64_t out_n = 0;
+        uint32_t overflow = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(&out_n, cnt.p, 8, hipMemcpyDeviceToHost, s()));
+        HIP_TRY(ctx, hipMemcpyAsync(&overflow, ovf.p, 4, hipMemcpyDeviceToHost, s()));
+        HIP_TRY(ctx, hipStreamSynchronize(s()));
+        if (overflow || out_n != matches) return set_err(ctx, PSM_EINVAL, "join probe passes disagree");
+        jr.max_inter_rows = std::max(jr.max_inter_rows, out_n);
+        cur = std::move(out);
+        cur_rows = cur->as<uint32_t>();
+        cur_n = out_n;
+        cols = out_cols;
+    }
+    HIP_TRY(ctx, timers_flush(ctx));
+    jr.join_ms = now_ms() - t1;
+    verbose("star join: %u stars, %llu unit rows, largest intermediate %llu -> %llu in %.1f ms", jr.units,
+            (unsigned long long)jr.unit_rows, (unsigned long long)jr.max_inter_rows,
+            (unsigned long long)jr.count, jr.join_ms);
+    return PSM_OK;
+}
+
 int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     if (!q) return PSM_EINVAL;
     HIP_TRY(q->ctx, hipSetDevice(q->ctx->device));
@@ -1867,6 +2071,21 @@ int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
         st->visited_rejects = q->qh.rejects;
     }
     return r;
+}
+
+int psm_star_join_count(psm_query* q, uint64_t max_rows, psm_star_join_stats* st) {
+    if (!q || !st) return PSM_EINVAL;
+    HIP_TRY(q->ctx, hipSetDevice(q->ctx->device));
+    StarJoinResult jr;
+    PSM_TRY(q->batch.star_join_count(q->qh, max_rows ? max_rows : (64ull << 20), jr));
+    memset(st, 0, sizeof(*st));
+    st->count = jr.count;
+    st->units = jr.units;
+    st->unit_rows = jr.unit_rows;
+    st->max_inter_rows = jr.max_inter_rows;
+    st->enum_ms = jr.enum_ms;
+    st->join_ms = jr.join_ms;
+    return PSM_OK;
 }
 
 int psm_join_count(psm_query* q, uint64_t max_rows, psm_join_stats* st) {

@@ -160,6 +160,13 @@ class Query:
         _check(self.dg.dev.h, L.lib().psm_join_count(self.h, max_rows, ctypes.byref(st)), "psm_join_count")
         return st
 
+    def star_join_count(self, max_rows: int = 0) -> L.StarJoinStats:
+        """Star decomposition of the query + a sequence of device hash joins."""
+        st = L.StarJoinStats()
+        _check(self.dg.dev.h, L.lib().psm_star_join_count(self.h, max_rows, ctypes.byref(st)),
+               "psm_star_join_count")
+        return st
+
     def evaluate(self, cap: int = 0) -> L.EnumStats:
         """Counts embeddings; st.timed_out is set when the time limit cut it short."""
         st = L.EnumStats()
@@ -215,6 +222,19 @@ def join_match(dg: DeviceGraph, q: CSRGraph, method: str = "CFL_CFL_EXPLORE", ma
         run.build_tables()
         run.generate_plan(o)
         return run.join_count(max_rows)
+    finally:
+        run.close()
+
+
+def star_join_match(dg: DeviceGraph, q: CSRGraph, filter: str = "GQL", max_rows: int = 0):
+    """Filter and tables as the plain path, then the star-decomposed count."""
+    if filter not in L.FILTERS:
+        raise ValueError(f"unsupported filter {filter!r}")
+    run = Query(dg, q)
+    try:
+        run.filter(filter)
+        run.build_tables()
+        return run.star_join_count(max_rows)
     finally:
         run.close()
 
