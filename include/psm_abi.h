@@ -288,6 +288,25 @@ int psm_server_poll(psm_server* s, psm_result* out, uint32_t max_results, uint32
 int psm_server_stats(psm_server* s, uint64_t* out5);
 int psm_server_destroy(psm_server* s);
 
+/* ---- graph files (host) -------------------------------------------------
+ * Graph::loadGraphFromFile (graph/graph.cpp:92-172): the "t V E" /
+ * "v id label degree" / "e u v" text format, memory-mapped and parsed by
+ * `threads` host threads (0: every hardware thread, at most 64) into a CSR
+ * with ascending adjacency. Adjacency comes from the e lines and every v
+ * line's degree field must agree with them (the reference places neighbours
+ * by the degree fields); PSM_EINVAL on a malformed line, an id out of range,
+ * a vertex listed twice or counts other than the header's, PSM_ENOTFOUND
+ * when the file cannot be opened; psm_text_graph_last_error() says which
+ * (per calling thread). The arrays belong to the handle. */
+typedef struct psm_text_graph psm_text_graph;
+int psm_text_graph_read(const char* path, uint32_t threads, psm_text_graph** out);
+int psm_text_graph_info(const psm_text_graph* g, uint32_t* num_vertices, uint64_t* num_edges);
+const uint32_t* psm_text_graph_offsets(const psm_text_graph* g);    /* V+1 entries */
+const uint32_t* psm_text_graph_neighbors(const psm_text_graph* g);  /* 2E entries */
+const uint32_t* psm_text_graph_labels(const psm_text_graph* g);     /* V entries */
+void psm_text_graph_free(psm_text_graph* g);
+const char* psm_text_graph_last_error(void);
+
 /* ---- seeded generators (host) -------------------------------------------
  * Data graphs for the benchmark configurations. offsets has V+1 entries,
  * neighbors 2E. */

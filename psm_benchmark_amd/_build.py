@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpsm.so")
-SOURCES = ["device.hip", "psm_abi.cpp", "plan.cpp", "gen.cpp"]
+SOURCES = ["device.hip", "psm_abi.cpp", "plan.cpp", "gen.cpp", "textio.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PSM_OFFLOAD_ARCH", "gfx950")
 

@@ -141,6 +141,13 @@ SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_uint32), ctypes.c_double]),
     "psm_server_stats": (ctypes.c_int, [VP, U64P]),
     "psm_server_destroy": (ctypes.c_int, [VP]),
+    "psm_text_graph_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(VP)]),
+    "psm_text_graph_info": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]),
+    "psm_text_graph_offsets": (ctypes.POINTER(ctypes.c_uint32), [VP]),
+    "psm_text_graph_neighbors": (ctypes.POINTER(ctypes.c_uint32), [VP]),
+    "psm_text_graph_labels": (ctypes.POINTER(ctypes.c_uint32), [VP]),
+    "psm_text_graph_free": (None, [VP]),
+    "psm_text_graph_last_error": (ctypes.c_char_p, []),
     "psm_gen_chung_lu": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64,
                                         U32P, U32P]),
     "psm_gen_rmat": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,

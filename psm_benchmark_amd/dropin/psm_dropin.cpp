@@ -114,37 +114,22 @@ Graph::~Graph() {
 
 // Text format of graph.cpp:92-172: "t V E", "v id label degree", "e u v".
 void Graph::loadGraphFromFile(const std::string& path) {
-    std::ifstream in(path);
-    if (!in.is_open()) {
+    // parsed by libpsm's multi-threaded reader (every hardware thread)
+    psm_text_graph* tg = nullptr;
+    const int r = psm_text_graph_read(path.c_str(), 0, &tg);
+    if (r == PSM_ENOTFOUND) {
         std::cout << "Can not open the graph file " << path << " ." << std::endl;
         exit(-1);
     }
-    char type;
-    ui V = 0, E = 0;
-    in >> type >> V >> E;
-    std::vector<ui> lab(V, 0), deg(V, 0);
-    std::vector<std::pair<ui, ui>> edges;
-    edges.reserve(E);
-    while (in >> type) {
-        if (type == 'v') {
-            ui id, l, d;
-            in >> id >> l >> d;
-            lab[id] = l;
-            deg[id] = d;
-        } else if (type == 'e') {
-            ui a, b;
-            in >> a >> b;
-            edges.push_back({a, b});
-        }
+    if (r != PSM_OK) {
+        std::cerr << "Bad graph file " << path << ": " << psm_text_graph_last_error() << std::endl;
+        exit(-1);
     }
-    std::vector<ui> off(V + 1, 0), fill(V, 0), adj(2 * edges.size());
-    for (auto& e : edges) { off[e.first + 1]++; off[e.second + 1]++; }
-    for (ui v = 0; v < V; ++v) off[v + 1] += off[v];
-    for (auto& e : edges) {
-        adj[off[e.first] + fill[e.first]++] = e.second;
-        adj[off[e.second] + fill[e.second]++] = e.first;
-    }
-    loadFromCSR(V, off.data(), adj.data(), lab.data());
+    ui V = 0;
+    uint64_t E = 0;
+    psm_text_graph_info(tg, &V, &E);
+    loadFromCSR(V, psm_text_graph_offsets(tg), psm_text_graph_neighbors(tg), psm_text_graph_labels(tg));
+    psm_text_graph_free(tg);
 }
 
 // Binary format of graph.cpp:221-403: [i32 4][u32 V][u32 E][u32 deg[V]],

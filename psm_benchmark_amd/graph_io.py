@@ -80,6 +80,33 @@ def load_text(path: str) -> CSRGraph:
     return csr_from_edges(labels.shape[0], np.array(edges, dtype=np.int64).reshape(-1, 2), labels)
 
 
+def load_text_parallel(path: str, threads: int = 0) -> CSRGraph:
+    """The graph load_text returns, read by libpsm's multi-threaded parser
+    (psm_text_graph_read; threads 0 = every hardware thread). Raises
+    PSMError on a malformed file, including v-line degree fields that
+    disagree with the e lines."""
+    import ctypes
+    from . import _lib as L
+    h = ctypes.c_void_p()
+    r = L.lib().psm_text_graph_read(path.encode(), threads, ctypes.byref(h))
+    if r != L.PSM_OK:
+        raise L.PSMError(r, L.lib().psm_text_graph_last_error().decode())
+    try:
+        v, e = ctypes.c_uint32(), ctypes.c_uint64()
+        L.lib().psm_text_graph_info(h, ctypes.byref(v), ctypes.byref(e))
+        n, m = v.value, 2 * e.value
+
+        def take(ptr, count):
+            if count == 0:
+                return np.zeros(0, np.uint32)
+            return np.ctypeslib.as_array(ptr, shape=(count,)).copy()
+        return CSRGraph(take(L.lib().psm_text_graph_offsets(h), n + 1),
+                        take(L.lib().psm_text_graph_neighbors(h), m),
+                        take(L.lib().psm_text_graph_labels(h), n))
+    finally:
+        L.lib().psm_text_graph_free(h)
+
+
 def save_text(g: CSRGraph, path: str) -> None:
     """Writes the ``t/v/e`` text format (degree field included: the
     reference's loader sizes adjacency from it)."""
