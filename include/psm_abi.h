@@ -186,25 +186,31 @@ typedef struct psm_join_stats {
 
 int psm_join_count(psm_query* q, uint64_t max_rows, psm_join_stats* stats);
 
-/* Query decomposition into stars + a join sequence: the STAR split
- * (SplitGraph.cpp:895-910) as a greedy vertex cover (each query edge in
- * exactly one star, centres in id order) and JoinCollect::sequence_collect
- * (JoinCollect.cpp:60-153) over the stars' device-enumerated rows, smallest
- * star first, then the connected star with the fewest rows, hash-joined on
- * the shared vertices with the injectivity check across sides. count equals
- * the plain count (no cap); the stars are this library's own greedy cover,
- * not the reference's star order. Needs psm_build_tables. max_rows bounds
- * every star and intermediate result (0: 64M); PSM_ENOMEM beyond it. */
-typedef struct psm_star_join_stats {
+/* Query split + a join sequence: the STAR / TWINTWIG / SEED splits
+ * (SplitGraph.cpp:874-980) as greedy covers (vertices in id order take their
+ * query edges not yet taken: one star each, or twin twigs of at most two
+ * edges, or twin twigs closed into triangles where the query has the edge)
+ * and JoinCollect::sequence_collect (JoinCollect.cpp:60-153) over the units'
+ * device-enumerated rows: smallest unit first, then the connected unit with
+ * the fewest rows, hash-joined on the shared vertices with the injectivity
+ * check across sides. count equals the plain count (no cap); the units are
+ * this library's own covers, not the reference's decomposition search.
+ * Needs psm_build_tables. max_rows bounds every unit and intermediate result
+ * (0: 64M); PSM_ENOMEM beyond it. */
+#define PSM_SPLIT_STAR 0
+#define PSM_SPLIT_TWINTWIG 1
+#define PSM_SPLIT_SEED 2
+
+typedef struct psm_split_join_stats {
     uint64_t count;
-    uint32_t units;           /* stars */
+    uint32_t units;
     uint32_t pad;
-    uint64_t unit_rows;       /* rows of all stars */
+    uint64_t unit_rows;       /* rows of all units */
     uint64_t max_inter_rows;  /* the largest intermediate result */
     double enum_ms, join_ms;
-} psm_star_join_stats;
+} psm_split_join_stats;
 
-int psm_star_join_count(psm_query* q, uint64_t max_rows, psm_star_join_stats* st);
+int psm_split_join_count(psm_query* q, int pattern, uint64_t max_rows, psm_split_join_stats* st);
 
 /* Convenience: filter + tables + plan + evaluate in one call, the sequence of
  * split_C_test (StudyPerformance.cpp:1164-1716). */

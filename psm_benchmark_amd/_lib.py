@@ -68,7 +68,7 @@ class JoinStats(ctypes.Structure):
     ]
 
 
-class StarJoinStats(ctypes.Structure):
+class SplitJoinStats(ctypes.Structure):
     _fields_ = [
         ("count", ctypes.c_uint64),
         ("units", ctypes.c_uint32),
@@ -98,6 +98,7 @@ PSM_TUNE_EXPAND_TASKS_PER_LANE = 1
 PSM_TUNE_DFS_STRATEGY = 2
 PSM_DFS_LANE = 0
 PSM_DFS_WAVE = 1
+SPLITS = {"STAR": 0, "TWINTWIG": 1, "SEED": 2}
 
 # name -> (restype, argtypes); every symbol include/psm_abi.h declares
 SIGNATURES = {
@@ -123,7 +124,7 @@ SIGNATURES = {
     "psm_generate_plan": (ctypes.c_int, [VP, ctypes.c_int, U32P, U32P]),
     "psm_evaluate": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
     "psm_join_count": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(JoinStats)]),
-    "psm_star_join_count": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(StarJoinStats)]),
+    "psm_split_join_count": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(SplitJoinStats)]),
     "psm_match": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
     "psm_batch_match": (ctypes.c_int, [VP, VP, ctypes.c_uint32, U32P, U64P, U64P, U64P, U32P, U32P, U32P,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_uint64, U64P, U64P, U32P,

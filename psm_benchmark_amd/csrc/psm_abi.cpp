@@ -383,7 +383,7 @@ struct JoinResult {
     double enum_ms = 0, join_ms = 0;
 };
 
-struct StarJoinResult {
+struct SplitJoinResult {
     uint64_t count = 0, unit_rows = 0, max_inter_rows = 0;
     uint32_t units = 0;
     double enum_ms = 0, join_ms = 0;
@@ -432,7 +432,7 @@ struct Batch {
                   std::vector<QCounters>& qc, bool& timed_out);
     int evaluate(uint64_t cap, double enum_limit = -1);
     int join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr);
-    int star_join_count(QueryHost& q, uint64_t max_rows, StarJoinResult& jr);
+    int split_join_count(QueryHost& q, int pattern, uint64_t max_rows, SplitJoinResult& jr);
     int host_table(const QueryHost& q, uint32_t a, uint32_t b, std::vector<uint32_t>& off,
                    std::vector<uint32_t>& idxv, hipStream_t st);
     int nlf_count(const QGraph& q, uint32_t u, uint32_t label_override, uint32_t& out, hipStream_t st,
@@ -1855,23 +1855,26 @@ int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t*
     return PSM_OK;
 }
 
-// ---- star decomposition + join sequence --------------------------------------
-// The STAR split (SplitGraph.cpp:895-910) restated as a greedy vertex cover:
-// vertices in id order take every query edge of theirs not yet taken, so
-// every query edge lies in exactly one star (centre + leaves). Each star is
-// enumerated on the device into rows (centre first, leaves by ascending
-// candidate count); the stars are then joined one at a time
-// (JoinCollect::sequence_collect, JoinCollect.cpp:60-153): start from the
-// star with the fewest rows, and join next the star sharing a vertex with the
-// joined ones that has the fewest rows (more shared vertices on ties), on the
-// shared vertices (ascending query id, keyTraits.h) with data vertices
-// distinct across the two sides. Each join counts its matches first; an
-// intermediate result is then materialised at exactly that size, the last
-// join only counts. The count equals the plain count.
-int Batch::star_join_count(QueryHost& q, uint64_t max_rows, StarJoinResult& jr) {
+// ---- query split + join sequence ---------------------------------------------
+// Splits (SplitGraph.cpp:874-980 restated as greedy covers): vertices in id
+// order take each of their query edges not yet taken, so every query edge
+// lies in exactly one unit. STAR: a vertex's taken edges form one star
+// (centre + leaves). TWINTWIG: its leaves, by ascending candidate count, are
+// cut into twin twigs of at most two edges. SEED: a twin twig whose two
+// leaves share a query edge not yet taken takes that edge too (a triangle).
+// Each unit is enumerated on the device into rows (centre first); the units
+// are then joined one at a time (JoinCollect::sequence_collect,
+// JoinCollect.cpp:60-153): start from the unit with the fewest rows, and
+// join next the unit sharing a vertex with the joined ones that has the
+// fewest rows (more shared vertices on ties), on the shared vertices
+// (ascending query id, keyTraits.h) with data vertices distinct across the
+// two sides. Each join counts its matches first; an intermediate result is
+// then materialised at exactly that size, the last join only counts. The
+// count equals the plain count.
+int Batch::split_join_count(QueryHost& q, int pattern, uint64_t max_rows, SplitJoinResult& jr) {
     tls_stream = s();
     const uint32_t n = q.qg.n;
-    jr = StarJoinResult();
+    jr = SplitJoinResult();
     if (!tables_built) return set_err(ctx, PSM_ESTATE, "build tables before joining");
     struct Unit {
         std::vector<uint32_t> cols;   // query vertex of each row column (= enumeration order)
@@ -1879,8 +1882,14 @@ int Batch::star_join_count(QueryHost& q, uint64_t max_rows, StarJoinResult& jr) 
         std::unique_ptr<DBuf> rows;
         uint64_t nrows = 0;
     };
+    if (pattern != PSM_SPLIT_STAR && pattern != PSM_SPLIT_TWINTWIG && pattern != PSM_SPLIT_SEED)
+        return set_err(ctx, PSM_EINVAL, "unknown split pattern");
     std::vector<Unit> units;
     std::vector<uint8_t> taken((size_t)n * n, 0);
+    auto take = [&](Unit& un, uint32_t a, uint32_t b) {
+        taken[(size_t)a * n + b] = taken[(size_t)b * n + a] = 1;
+        un.edge[(size_t)a * n + b] = un.edge[(size_t)b * n + a] = 1;
+    };
     for (uint32_t u = 0; u < n; ++u) {
         std::vector<uint32_t> leaves;
         for (uint32_t i = q.qg.off[u]; i < q.qg.off[u + 1]; ++i) {
@@ -1890,16 +1899,22 @@ int Batch::star_join_count(QueryHost& q, uint64_t max_rows, StarJoinResult& jr) 
         if (leaves.empty()) continue;
         std::stable_sort(leaves.begin(), leaves.end(),
                          [&](uint32_t a, uint32_t b) { return q.cand_count[a] < q.cand_count[b]; });
-        Unit un;
-        un.rows.reset(new DBuf());
-        un.cols.push_back(u);
-        un.edge.assign((size_t)n * n, 0);
-        for (uint32_t w : leaves) {
-            un.cols.push_back(w);
-            taken[(size_t)u * n + w] = taken[(size_t)w * n + u] = 1;
-            un.edge[(size_t)u * n + w] = un.edge[(size_t)w * n + u] = 1;
+        const size_t per = pattern == PSM_SPLIT_STAR ? leaves.size() : 2;
+        for (size_t i0 = 0; i0 < leaves.size(); i0 += per) {
+            Unit un;
+            un.rows.reset(new DBuf());
+            un.edge.assign((size_t)n * n, 0);
+            un.cols.push_back(u);
+            for (size_t i = i0; i < std::min(leaves.size(), i0 + per); ++i) {
+                un.cols.push_back(leaves[i]);
+                take(un, u, leaves[i]);
+            }
+            if (pattern == PSM_SPLIT_SEED && un.cols.size() == 3) {
+                const uint32_t a = un.cols[1], b = un.cols[2];
+                if (q.qg.has_edge(a, b) && !taken[(size_t)a * n + b]) take(un, a, b);
+            }
+            units.push_back(std::move(un));
         }
-        units.push_back(std::move(un));
     }
     if (units.empty()) return set_err(ctx, PSM_EINVAL, "query has no edge to split");
     jr.units = (uint32_t)units.size();
@@ -1923,8 +1938,8 @@ int Batch::star_join_count(QueryHost& q, uint64_t max_rows, StarJoinResult& jr) 
         HIP_TRY(ctx, hipMemcpyAsync(&un.nrows, cnt.p, 8, hipMemcpyDeviceToHost, s()));
         HIP_TRY(ctx, hipMemcpyAsync(&overflow, ovf.p, 4, hipMemcpyDeviceToHost, s()));
         HIP_TRY(ctx, hipStreamSynchronize(s()));
-        if (to) return set_err(ctx, PSM_ETIMEOUT, "star enumeration passed the time limit");
-        if (overflow) return set_err(ctx, PSM_ENOMEM, "a star has more rows than max_rows");
+        if (to) return set_err(ctx, PSM_ETIMEOUT, "unit enumeration passed the time limit");
+        if (overflow) return set_err(ctx, PSM_ENOMEM, "a unit has more rows than max_rows");
         jr.unit_rows += un.nrows;
     }
     jr.enum_ms = now_ms() - t0;
@@ -2046,7 +2061,7 @@ int Batch::star_join_count(QueryHost& q, uint64_t max_rows, StarJoinResult& jr) 
     }
     HIP_TRY(ctx, timers_flush(ctx));
     jr.join_ms = now_ms() - t1;
-    verbose("star join: %u stars, %llu unit rows, largest intermediate %llu -> %llu in %.1f ms", jr.units,
+    verbose("split join: %u units, %llu unit rows, largest intermediate %llu -> %llu in %.1f ms", jr.units,
             (unsigned long long)jr.unit_rows, (unsigned long long)jr.max_inter_rows,
             (unsigned long long)jr.count, jr.join_ms);
     return PSM_OK;
@@ -2073,11 +2088,11 @@ int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     return r;
 }
 
-int psm_star_join_count(psm_query* q, uint64_t max_rows, psm_star_join_stats* st) {
+int psm_split_join_count(psm_query* q, int pattern, uint64_t max_rows, psm_split_join_stats* st) {
     if (!q || !st) return PSM_EINVAL;
     HIP_TRY(q->ctx, hipSetDevice(q->ctx->device));
-    StarJoinResult jr;
-    PSM_TRY(q->batch.star_join_count(q->qh, max_rows ? max_rows : (64ull << 20), jr));
+    SplitJoinResult jr;
+    PSM_TRY(q->batch.split_join_count(q->qh, pattern, max_rows ? max_rows : (64ull << 20), jr));
     memset(st, 0, sizeof(*st));
     st->count = jr.count;
     st->units = jr.units;

@@ -160,11 +160,12 @@ class Query:
         _check(self.dg.dev.h, L.lib().psm_join_count(self.h, max_rows, ctypes.byref(st)), "psm_join_count")
         return st
 
-    def star_join_count(self, max_rows: int = 0) -> L.StarJoinStats:
-        """Star decomposition of the query + a sequence of device hash joins."""
-        st = L.StarJoinStats()
-        _check(self.dg.dev.h, L.lib().psm_star_join_count(self.h, max_rows, ctypes.byref(st)),
-               "psm_star_join_count")
+    def split_join_count(self, pattern: str = "STAR", max_rows: int = 0) -> L.SplitJoinStats:
+        """STAR / TWINTWIG / SEED split of the query + a sequence of device
+        hash joins."""
+        st = L.SplitJoinStats()
+        _check(self.dg.dev.h, L.lib().psm_split_join_count(self.h, L.SPLITS[pattern], max_rows, ctypes.byref(st)),
+               "psm_split_join_count")
         return st
 
     def evaluate(self, cap: int = 0) -> L.EnumStats:
@@ -226,15 +227,17 @@ def join_match(dg: DeviceGraph, q: CSRGraph, method: str = "CFL_CFL_EXPLORE", ma
         run.close()
 
 
-def star_join_match(dg: DeviceGraph, q: CSRGraph, filter: str = "GQL", max_rows: int = 0):
-    """Filter and tables as the plain path, then the star-decomposed count."""
-    if filter not in L.FILTERS:
-        raise ValueError(f"unsupported filter {filter!r}")
+def split_join_match(dg: DeviceGraph, q: CSRGraph, pattern: str = "STAR", filter: str = "GQL",
+                     max_rows: int = 0):
+    """Filter and tables as the plain path, then the count over a STAR,
+    TWINTWIG or SEED split joined unit by unit."""
+    if filter not in L.FILTERS or pattern not in L.SPLITS:
+        raise ValueError(f"unsupported filter {filter!r} or split {pattern!r}")
     run = Query(dg, q)
     try:
         run.filter(filter)
         run.build_tables()
-        return run.star_join_count(max_rows)
+        return run.split_join_count(pattern, max_rows)
     finally:
         run.close()
 

@@ -235,39 +235,41 @@ def test_join_bruteforce_tiny(dev, seed):
     assert join_match(dg, q, "GQL_GQL_GQL").count == count_embeddings(data, q)
 
 
-@pytest.mark.parametrize("filt", ["GQL", "LDF"])
-def test_star_join_equals_plain_on_hprd(hprd, filt):
-    """Star decomposition + a sequence of device hash joins counts every
-    HPRD query's embeddings exactly (the reference's STAR Q-split result)."""
-    from psm_benchmark_amd.matcher import star_join_match
+@pytest.mark.parametrize("pattern,filt", [("STAR", "GQL"), ("STAR", "LDF"), ("TWINTWIG", "GQL"),
+                                          ("SEED", "GQL"), ("SEED", "LDF")])
+def test_split_join_equals_plain_on_hprd(hprd, pattern, filt):
+    """A STAR / TWINTWIG / SEED split joined unit by unit on the device counts
+    every HPRD query's embeddings exactly (the reference's Q-split result)."""
+    from psm_benchmark_amd.matcher import split_join_match
     _, dg, _ = hprd
     for rec in golden(("GQL_GQL_GQL",)):
         q = load_text(os.path.join(HPRD, rec["query"] + ".txt"))
-        st = star_join_match(dg, q, filt, max_rows=1 << 24)
-        print(rec["query"], filt, st.units, st.unit_rows, st.max_inter_rows, st.count, flush=True)
+        st = split_join_match(dg, q, pattern, filt, max_rows=1 << 24)
+        print(rec["query"], pattern, filt, st.units, st.unit_rows, st.max_inter_rows, st.count, flush=True)
         assert st.count == rec["embeddings"], rec["query"]
         assert st.units >= 2
 
 
 @pytest.mark.parametrize("seed", range(8))
-def test_star_join_bruteforce_tiny(dev, seed):
-    from psm_benchmark_amd.matcher import DeviceGraph, star_join_match
+def test_split_join_bruteforce_tiny(dev, seed):
+    from psm_benchmark_amd.matcher import DeviceGraph, split_join_match
     rng = np.random.default_rng(300 + seed)
     data = random_graph(rng, 12, 0.5, 2)
     q = random_connected_query(rng, data, int(rng.integers(2, 7)))
     dg = DeviceGraph(dev, data)
     expect = count_embeddings(data, q)
-    for filt in ("LDF", "GQL"):
-        assert star_join_match(dg, q, filt).count == expect, filt
+    for pattern in ("STAR", "TWINTWIG", "SEED"):
+        for filt in ("LDF", "GQL"):
+            assert split_join_match(dg, q, pattern, filt).count == expect, (pattern, filt)
 
 
-def test_star_join_row_bound_is_enforced(hprd):
+def test_split_join_row_bound_is_enforced(hprd):
     from psm_benchmark_amd import _lib as L
-    from psm_benchmark_amd.matcher import star_join_match
+    from psm_benchmark_amd.matcher import split_join_match
     _, dg, _ = hprd
     q = load_text(os.path.join(HPRD, "Q10-10.txt"))
     with pytest.raises(L.PSMError) as e:
-        star_join_match(dg, q, "GQL", max_rows=8)
+        split_join_match(dg, q, "STAR", "GQL", max_rows=8)
     assert e.value.code == L.PSM_ENOMEM
 
 
