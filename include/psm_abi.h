@@ -208,6 +208,7 @@ typedef struct psm_split_join_stats {
     uint64_t unit_rows;       /* rows of all units */
     uint64_t max_inter_rows;  /* the largest intermediate result */
     double enum_ms, join_ms;
+    uint64_t calls;           /* assignments made enumerating the units */
 } psm_split_join_stats;
 
 int psm_split_join_count(psm_query* q, int pattern, uint64_t max_rows, psm_split_join_stats* st);

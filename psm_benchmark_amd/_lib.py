@@ -77,6 +77,7 @@ class SplitJoinStats(ctypes.Structure):
         ("max_inter_rows", ctypes.c_uint64),
         ("enum_ms", ctypes.c_double),
         ("join_ms", ctypes.c_double),
+        ("calls", ctypes.c_uint64),
     ]
 
 

@@ -384,7 +384,7 @@ struct JoinResult {
 };
 
 struct SplitJoinResult {
-    uint64_t count = 0, unit_rows = 0, max_inter_rows = 0;
+    uint64_t count = 0, unit_rows = 0, max_inter_rows = 0, calls = 0;
     uint32_t units = 0;
     double enum_ms = 0, join_ms = 0;
 };
@@ -1941,6 +1941,7 @@ int Batch::split_join_count(QueryHost& q, int pattern, uint64_t max_rows, SplitJ
         if (to) return set_err(ctx, PSM_ETIMEOUT, "unit enumeration passed the time limit");
         if (overflow) return set_err(ctx, PSM_ENOMEM, "a unit has more rows than max_rows");
         jr.unit_rows += un.nrows;
+        jr.calls += qc[0].calls;
     }
     jr.enum_ms = now_ms() - t0;
     const double t1 = now_ms();
@@ -2098,6 +2099,7 @@ int psm_split_join_count(psm_query* q, int pattern, uint64_t max_rows, psm_split
     st->units = jr.units;
     st->unit_rows = jr.unit_rows;
     st->max_inter_rows = jr.max_inter_rows;
+    st->calls = jr.calls;
     st->enum_ms = jr.enum_ms;
     st->join_ms = jr.join_ms;
     return PSM_OK;

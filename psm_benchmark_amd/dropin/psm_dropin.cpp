@@ -348,6 +348,29 @@ size_t EvaluateQuery::evaluate(const Graph*, const Graph* query, size_t limit, s
     return st.count;
 }
 
+size_t JoinCollect::sequence_collect(const Graph* query, const std::string& pattern, size_t& calls,
+                                     bool& overtime, double& enum_s, double& join_s, double time_limit_s) {
+    int p;
+    if (pattern == "STAR") p = PSM_SPLIT_STAR;
+    else if (pattern == "TWINTWIG") p = PSM_SPLIT_TWINTWIG;
+    else if (pattern == "SEED") p = PSM_SPLIT_SEED;
+    else throw std::runtime_error("split pattern " + pattern + " is not supported");
+    psm_set_time_limit(psm_dropin::context(), time_limit_s);
+    psm_split_join_stats st;
+    const int r = psm_split_join_count(existing(query), p, 0, &st);
+    overtime = r == PSM_ETIMEOUT;
+    if (overtime) {
+        calls = 0;
+        enum_s = join_s = 0;
+        return 0;
+    }
+    check(r, "psm_split_join_count");
+    calls = st.calls;
+    enum_s = st.enum_ms / 1e3;
+    join_s = st.join_ms / 1e3;
+    return st.count;
+}
+
 // ---------------------------------------------------------------------------
 // MatchingCommand: "-key value" pairs (utility/commandparser.cpp:3-19)
 // ---------------------------------------------------------------------------

@@ -144,6 +144,19 @@ public:
 };
 
 // Command line options and defaults of the reference (matchingcommand.h:70-159).
+// Q mode (-QorCandi Q, StudyPerformance.cpp:429-437): the query prepared by
+// the filter and buildTables calls above is split by -Qpattern (STAR,
+// TWINTWIG or SEED), its units are enumerated on the GPU and joined one by
+// one (JoinCollect.cpp:60-153). Returns the embedding count; call_count
+// receives the assignments made enumerating the units (the units are this
+// library's own cover, so it differs from the reference's Q-mode count).
+class JoinCollect {
+public:
+    static size_t sequence_collect(const Graph* query_graph, const std::string& pattern, size_t& call_count,
+                                   bool& overtime, double& enumerate_seconds, double& join_seconds,
+                                   double time_limit_s = 60.0);
+};
+
 class MatchingCommand {
 public:
     MatchingCommand(int argc, char** argv);

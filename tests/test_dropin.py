@@ -14,10 +14,13 @@ DRIVER = os.path.join(ROOT, "psm_benchmark_amd", "SubgraphMatching.out")
 HPRD = os.path.join(ROOT, "tests", "golden", "hprd")
 
 
-def test_driver_built_and_rejects_decomposed_mode():
+def test_driver_built_and_rejects_unsupported_modes():
     assert os.access(DRIVER, os.X_OK)
-    r = subprocess.run([DRIVER, "-d", "x", "-q", "y", "-QorCandi", "Q"], capture_output=True, text=True)
+    r = subprocess.run([DRIVER, "-d", "x", "-q", "y", "-QorCandi", "Z"], capture_output=True, text=True)
     assert r.returncode == 1 and "QorCandi" in r.stderr
+    r = subprocess.run([DRIVER, "-d", "x", "-q", "y", "-QorCandi", "Q", "-Qpattern", "SKETCHTREE"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "Qpattern" in r.stderr
 
 
 @pytest.mark.gpu
@@ -38,3 +41,19 @@ def test_driver_log_matches_golden(tmp_path, qname):
         assert int(b["Call Count"]) == g["calls"]
         assert round(float(b["Memory cost (MB)"]) * 2**20 / 4) == g["mem_words"]
         assert b["Overtime"] == "0"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pattern", ["STAR", "TWINTWIG", "SEED"])
+def test_driver_query_split_mode_counts_golden(tmp_path, pattern):
+    log = str(tmp_path / "log.txt")
+    gold = {r["query"]: r["embeddings"] for r in json.load(open(os.path.join(HPRD, "golden.json")))
+            if r["threads"] == 1 and r["filter"] == "GQL" and r["engine"] == "GQL"}
+    for qname in ("Q10-1", "Q10-4", "Q10-9"):
+        subprocess.run([DRIVER, "-d", os.path.join(HPRD, "HPRD-15.txt"), "-q", os.path.join(HPRD, qname + ".txt"),
+                        "-num", "MAX", "-time_limit", "60", "-threadnums", "1", "-QorCandi", "Q",
+                        "-Qpattern", pattern, "-BackMethods", "GQL_GQL_GQL", "-OutputFile", log],
+                       check=True, timeout=120)
+        b = R.parse_log(log)[-1]
+        assert int(b["#Embeddings"]) == gold[qname], (qname, pattern)
+        assert b["Overtime"] == "0" and int(b["Call Count"]) > 0
