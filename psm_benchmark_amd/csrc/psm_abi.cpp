@@ -1918,13 +1918,17 @@ int Batch::split_join_count(QueryHost& q, int pattern, uint64_t max_rows, SplitJ
     }
     if (units.empty()) return set_err(ctx, PSM_EINVAL, "query has no edge to split");
     jr.units = (uint32_t)units.size();
-    DBuf cnt, ovf;
+    DBuf cnt, ovf, stage;
     HIP_TRY(ctx, cnt.alloc(8));
     HIP_TRY(ctx, ovf.alloc(4));
+    // units are enumerated into one staging buffer of max_rows rows and kept
+    // at their exact size
+    size_t max_w = 0;
+    for (const Unit& un : units) max_w = std::max(max_w, un.cols.size());
+    HIP_TRY(ctx, stage.alloc(std::max<uint64_t>(max_rows, 1) * max_w * 4));
     const double t0 = now_ms();
     for (Unit& un : units) {
         const uint32_t w = (uint32_t)un.cols.size();
-        HIP_TRY(ctx, un.rows->alloc(std::max<uint64_t>(max_rows, 1) * w * 4));
         HIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, 8, s()));
         HIP_TRY(ctx, hipMemsetAsync(ovf.p, 0, 4, s()));
         std::vector<DPlan> hp(1);
@@ -1932,7 +1936,7 @@ int Batch::split_join_count(QueryHost& q, int pattern, uint64_t max_rows, SplitJ
         make_plan(q, un.cols, &un.edge, 0, hp[0], hops);
         std::vector<QCounters> qc;
         bool to = false;
-        EmitSink sink{un.rows->as<uint32_t>(), cnt.as<unsigned long long>(), max_rows, ovf.as<uint32_t>()};
+        EmitSink sink{stage.as<uint32_t>(), cnt.as<unsigned long long>(), max_rows, ovf.as<uint32_t>()};
         PSM_TRY(run_plans(hp, hops, sink, qc, to));
         uint32_t overflow = 0;
         HIP_TRY(ctx, hipMemcpyAsync(&un.nrows, cnt.p, 8, hipMemcpyDeviceToHost, s()));
@@ -1940,9 +1944,13 @@ int Batch::split_join_count(QueryHost& q, int pattern, uint64_t max_rows, SplitJ
         HIP_TRY(ctx, hipStreamSynchronize(s()));
         if (to) return set_err(ctx, PSM_ETIMEOUT, "unit enumeration passed the time limit");
         if (overflow) return set_err(ctx, PSM_ENOMEM, "a unit has more rows than max_rows");
+        HIP_TRY(ctx, un.rows->alloc(std::max<uint64_t>(un.nrows, 1) * w * 4));
+        if (un.nrows)
+            HIP_TRY(ctx, hipMemcpyAsync(un.rows->p, stage.p, un.nrows * w * 4, hipMemcpyDeviceToDevice, s()));
         jr.unit_rows += un.nrows;
         jr.calls += qc[0].calls;
     }
+    stage.release();
     jr.enum_ms = now_ms() - t0;
     const double t1 = now_ms();
     // the joined result so far

@@ -162,8 +162,9 @@ int psm_text_graph_read(const char* path, uint32_t threads, psm_text_graph** out
         if (q < le) {
             if (*q != 't') return fail("first line is not the 't V E' header");
             ++q;
-            if (!read_uint(q, le, UINT32_MAX, V) || !read_uint(q, le, UINT64_MAX / 4, E))
-                return fail("malformed header: " + std::string(p, le));
+            // (read_uint's bound keeps x * 10 within 64 bits)
+            if (!read_uint(q, le, UINT32_MAX, V) || !read_uint(q, le, UINT32_MAX, E))
+                return fail("malformed or out-of-range header: " + std::string(p, le));
             p = le + 1;
             break;
         }
