@@ -309,6 +309,26 @@ def test_served_results_equal_alone_and_in_the_mix(hprd):
     assert max(r.batch_size for r in results.values()) >= 1
 
 
+def test_served_dpiso_results_equal_golden(hprd):
+    """The serving loop with the DPiso filter and order: every request's count
+    equals the reference's DPiso_DPiso_LFTJ log, and its assignment count the
+    oracle's GQL engine on the same plan."""
+    from psm_benchmark_amd.serving import Server, run_poisson
+    g, dg, og = hprd
+    recs = golden(("DPiso_DPiso_LFTJ",))
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
+    calls = [ORun(og, OGraph(q), "DPiso_DPiso_GQL").calls for q in qs]
+    srv = Server(dg, workers=2, max_batch=4, filter="DPiso", order="DPiso")
+    try:
+        results, summary = run_poisson(srv, qs, rate_qps=500.0, n_requests=30, seed=11)
+    finally:
+        srv.close()
+    for k, r in results.items():
+        assert r.status == 0
+        assert r.count == recs[k % len(recs)]["embeddings"], k
+        assert r.calls == calls[k % len(recs)], k
+
+
 @pytest.fixture(scope="module")
 def c2_graph():
     from psm_benchmark_amd import workloads as W
