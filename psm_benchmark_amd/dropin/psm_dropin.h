@@ -176,6 +176,13 @@ public:
     std::string getFilterType() const { return get("-filter", "CFL"); }
     std::string getOrderType() const { return get("-order", "GQL"); }
     std::string getEngineType() const { return get("-engine", "LFTJ"); }
+    std::string getAlgorithm() const { return get("-a"); }
+    std::string getIndexType() const { return get("-i", "VertexCentric"); }
+    std::string getDepthThreshold() const { return get("-d0", "0"); }
+    std::string getWidthThreshold() const { return get("-w0", "1"); }
+    std::string getOrderNum() const { return get("-order_num", "100"); }
+    std::string getDistributionFilePath() const { return get("-dis_file", "temp.distribution"); }
+    std::string getCSRFilePath() const { return get("-csr"); }
 
 private:
     std::map<std::string, std::string> opts_;
