@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-PLAIN_SHAPE = "lane,8,2"   # the library's default launch shape (psm_ctx)
+PLAIN_SHAPE = "lane,2,2"   # the library's default launch shape (psm_ctx)
 
 
 def log(*a):
@@ -343,9 +343,12 @@ def plain_workload(args):
     """C1, the reference's CPU-runnable case: Chung-Lu 9,460 V / 34,998 E, 307
     Zipf(1.1) labels, one 8-vertex random-walk query (seeded), GQL_GQL_GQL,
     uncapped. A step is one plain-path match of the query on the GPU (filter,
-    tables, plan, enumeration; the graph resident in HBM). The reference
-    binary runs the same query on the host at one thread as the CPU baseline,
-    and its embedding count must equal the GPU's."""
+    tables, plan, enumeration; the graph resident in HBM), at the launch shape
+    the autotuner picks for it (--shape overrides; the library's default
+    shape is timed beside it). The reference binary runs the same query on the
+    host at one thread as the CPU baseline, and its embedding count must
+    equal the GPU's."""
+    from psm_benchmark_amd import autotune
     from psm_benchmark_amd import workloads as W
     from psm_benchmark_amd import _lib as L
     from psm_benchmark_amd.matcher import Device, DeviceGraph, match
@@ -356,8 +359,24 @@ def plain_workload(args):
     dev = Device(0)
     L.lib().psm_set_time_limit(dev.h, args.time_limit)
     dg = DeviceGraph(dev, g)
+    default_rate = None
     if args.shape:
-        apply_shape(dev, args.shape)
+        tuned = apply_shape(dev, args.shape)
+    elif not args.no_tune:
+        # the library default first, for reference
+        apply_shape(dev, PLAIN_SHAPE)
+        match(dg, q, "GQL_GQL_GQL")
+        t = time.perf_counter()
+        n_default = max(1, min(10, args.steps))
+        for _ in range(n_default):
+            match(dg, q, "GQL_GQL_GQL")
+        default_rate = round(n_default / (time.perf_counter() - t), 2)
+        tuned, _ = autotune.tune(dg, [q], "GQL", "GQL", 0, sample=1, trial_limit_s=max(1.0, args.tune_limit),
+                                 restore_limit_s=args.time_limit, log=log)
+        log(f"autotune picked {tuned}")
+    else:
+        tuned = apply_shape(dev, PLAIN_SHAPE)
+    L.lib().psm_set_time_limit(dev.h, args.time_limit)
     for _ in range(args.warmup):
         match(dg, q, "GQL_GQL_GQL")
     L.lib().psm_kernel_stats_reset(dev.h)
@@ -383,6 +402,7 @@ def plain_workload(args):
         "config": {"workload": "C1 plain: Chung-Lu 9,460 V / 34,998 E, 307 Zipf(1.1) labels, one 8-vertex "
                                "random-walk query, GQL_GQL_GQL, uncapped", "time_limit_s": args.time_limit},
         "embeddings": r.count, "calls": r.calls, "timed_out": r.timed_out,
+        "launch_shape": tuned, "default_shape_queries_per_s": default_rate, "default_shape": PLAIN_SHAPE,
         "embeddings_per_s": round(r.count * args.steps / total, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "p99": round(float(np.percentile(lat_ms, 99)), 3)},
