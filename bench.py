@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-PLAIN_SHAPE = "lane,2,2"   # the library's default launch shape (psm_ctx)
+PLAIN_SHAPE = "lane,1,2"   # the library's default launch shape (psm_ctx)
 
 
 def log(*a):

@@ -104,7 +104,7 @@ int psm_set_time_limit(psm_ctx* ctx, double seconds);
 /* Enumeration launch shape, set by the autotuner (psm_benchmark_amd/autotune)
  * or by hand; results never depend on it. Keys:
  *   PSM_TUNE_DFS_WAVES_PER_CU       wavefronts per CU of the persistent DFS
- *                                   grid, 1..32 (default 2)
+ *                                   grid, 1..32 (default 1)
  *   PSM_TUNE_EXPAND_TASKS_PER_LANE  pre-expansion target: tasks per DFS
  *                                   worker (lane or wavefront) before the
  *                                   persistent kernel starts (2)

@@ -35,11 +35,11 @@ using namespace psm;
 // context, errors, device buffers
 // ---------------------------------------------------------------------------
 // The library's default enumeration launch shape (what the plain path uses):
-// one lane-per-search wavefront pair per CU. Since lanes claim queue slots in
+// one lane-per-search wavefront per CU. Since lanes claim queue slots in
 // batches, a single search runs fastest on few waves (fewer wavefronts
 // contending for L2 and the queue): the 9,460-vertex C1 query enumerates in
-// 22 ms at 2 waves per CU against 48 ms at 8.
-constexpr uint32_t kDefaultWavesPerCu = 2;
+// 15 ms at 1 wave per CU, 19 ms at 2 and 48 ms at 8.
+constexpr uint32_t kDefaultWavesPerCu = 1;
 constexpr double kDefaultTasksPerLane = 2.0;
 
 struct psm_ctx {
