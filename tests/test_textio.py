@@ -67,3 +67,16 @@ def test_missing_file(tmp_path):
     with pytest.raises(L.PSMError) as e:
         load_text_parallel(str(tmp_path / "nope.txt"))
     assert e.value.code == L.PSM_ENOTFOUND
+
+
+def test_crlf_blank_lines_and_trailing_fields(tmp_path):
+    # Windows line ends, blank lines and a final line without a newline parse
+    # to the same graph as the plain file
+    plain = "t 3 2\nv 0 1 1\nv 1 2 2\nv 2 1 1\ne 0 1\ne 1 2\n"
+    crlf = "t 3 2\r\n\r\nv 0 1 1\r\nv 1 2 2\r\n  v 2 1 1\r\n\r\ne 0 1\r\ne 1 2"
+    a, b = tmp_path / "a.txt", tmp_path / "b.txt"
+    a.write_text(plain)
+    b.write_bytes(crlf.encode())
+    ga, gb = load_text_parallel(str(a)), load_text_parallel(str(b))
+    assert same(ga, gb) and same(ga, load_text(str(a)))
+    assert ga.offsets.tolist() == [0, 1, 3, 4] and ga.neighbors.tolist() == [1, 0, 2, 1]
