@@ -263,6 +263,29 @@ def test_split_join_bruteforce_tiny(dev, seed):
             assert split_join_match(dg, q, pattern, filt).count == expect, (pattern, filt)
 
 
+def test_split_join_equals_reference_vectors_on_c1(dev):
+    """The split joins on the seeded C1-shaped graph (307 labels) against the
+    reference binary's counts (tests/golden/c1_ref.json), and batched DPiso
+    against its DPiso_DPiso_LFTJ counts."""
+    from psm_benchmark_amd import gen
+    from psm_benchmark_amd.matcher import DeviceGraph, batch_match, split_join_match
+    data = json.load(open(os.path.join(HERE, "golden", "c1_ref.json")))
+    gp = data["graph"]
+    g = gen.chung_lu_graph(gp["V"], gp["E"], gp["labels"], gp["zipf"], seed=gp["seed"])
+    dg = DeviceGraph(dev, g)
+    recs = [r for r in data["records"] if r["method"] == "GQL_GQL_GQL" and r["n"] >= 4][:24]
+    qs = [gen.query(g, r["n"], bool(r["dense"]), r["qseed"]) for r in recs]
+    for pattern in ("STAR", "SEED"):
+        for q, r in zip(qs, recs):
+            st = split_join_match(dg, q, pattern, "GQL", max_rows=1 << 24)
+            assert st.count == r["embeddings"], (pattern, r)
+    drecs = {(r["qseed"], r["dense"], r["n"]): r for r in data["records"] if r["method"] == "DPiso_DPiso_LFTJ"}
+    keys = [(r["qseed"], r["dense"], r["n"]) for r in recs if (r["qseed"], r["dense"], r["n"]) in drecs]
+    dqs = [gen.query(g, n, bool(dense), qseed) for qseed, dense, n in keys]
+    counts, calls, status, st = batch_match(dg, dqs, "DPiso", "DPiso", 0)
+    assert counts.tolist() == [drecs[k]["embeddings"] for k in keys]
+
+
 def test_split_join_row_bound_is_enforced(hprd):
     from psm_benchmark_amd import _lib as L
     from psm_benchmark_amd.matcher import split_join_match
