@@ -1524,17 +1524,18 @@ int Batch::join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr) {
         }
     }
     jr.keys = ka.nkeys;
-    // materialise both units
-    DBuf rows[2], cnt, ovf;
+    // materialise both units, each through one staging buffer of max_rows
+    // rows and then kept at its exact size
+    DBuf rows[2], cnt, ovf, stage;
     HIP_TRY(ctx, cnt.alloc(8));
     HIP_TRY(ctx, ovf.alloc(4));
     const std::vector<uint32_t>* orders[2] = {&orderA, &orderB};
     const std::vector<uint8_t>* edges[2] = {&eA, &eB};
     uint64_t nrows[2] = {0, 0};
+    HIP_TRY(ctx, stage.alloc(std::max<uint64_t>(max_rows, 1) * std::max(orderA.size(), orderB.size()) * 4));
     const double t0 = now_ms();
     for (int side = 0; side < 2; ++side) {
         const uint32_t w = (uint32_t)orders[side]->size();
-        HIP_TRY(ctx, rows[side].alloc(std::max<uint64_t>(max_rows, 1) * w * 4));
         HIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, 8, s()));
         HIP_TRY(ctx, hipMemsetAsync(ovf.p, 0, 4, s()));
         std::vector<DPlan> hp(1);
@@ -1542,14 +1543,19 @@ int Batch::join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr) {
         make_plan(q, *orders[side], edges[side], 0, hp[0], hops);
         std::vector<QCounters> qc;
         bool to = false;
-        EmitSink sink{rows[side].as<uint32_t>(), cnt.as<unsigned long long>(), max_rows, ovf.as<uint32_t>()};
+        EmitSink sink{stage.as<uint32_t>(), cnt.as<unsigned long long>(), max_rows, ovf.as<uint32_t>()};
         PSM_TRY(run_plans(hp, hops, sink, qc, to));
         uint32_t overflow = 0;
-        HIP_TRY(ctx, hipMemcpy(&nrows[side], cnt.p, 8, hipMemcpyDeviceToHost));
-        HIP_TRY(ctx, hipMemcpy(&overflow, ovf.p, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpyAsync(&nrows[side], cnt.p, 8, hipMemcpyDeviceToHost, s()));
+        HIP_TRY(ctx, hipMemcpyAsync(&overflow, ovf.p, 4, hipMemcpyDeviceToHost, s()));
+        HIP_TRY(ctx, hipStreamSynchronize(s()));
         if (to) return set_err(ctx, PSM_ETIMEOUT, "join unit enumeration passed the time limit");
         if (overflow) return set_err(ctx, PSM_ENOMEM, "join unit has more rows than max_rows");
+        HIP_TRY(ctx, rows[side].alloc(std::max<uint64_t>(nrows[side], 1) * w * 4));
+        if (nrows[side])
+            HIP_TRY(ctx, hipMemcpyAsync(rows[side].p, stage.p, nrows[side] * w * 4, hipMemcpyDeviceToDevice, s()));
     }
+    stage.release();
     jr.rows_a = nrows[0];
     jr.rows_b = nrows[1];
     jr.enum_ms = now_ms() - t0;
