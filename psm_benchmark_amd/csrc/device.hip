@@ -1285,7 +1285,11 @@ __device__ __forceinline__ bool past_deadline(DQueue* Q) {
                __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-constexpr uint32_t kLeafBurst = 32;   // base elements one step of the lane DFS tests at most
+// base elements one step of the lane DFS tests at most: the cap and queue
+// checks count steps, so a shorter step lets capped queries stop sooner
+// (C4 plain path 37 -> 44 queries/s from 32 to 16) at about 1 % of an
+// uncapped search's rate
+constexpr uint32_t kLeafBurst = 16;
 constexpr uint32_t kClaimMax = 8;     // queue slots a lane takes per claim at most
 
 // Adds a lane's running count (setting the query's done flag at its cap)
