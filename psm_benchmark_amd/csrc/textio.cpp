@@ -217,8 +217,16 @@ int psm_text_graph_read(const char* path, uint32_t threads, psm_text_graph** out
                     " e lines");
 
     // degrees from the edges, offsets, scatter through per-vertex cursors
+    // (one thread needs no atomic read-modify-writes)
     std::vector<uint32_t> cnt(V, 0);
     run([&](uint32_t i) {
+        if (T == 1) {
+            for (const auto& e : parts[i].edges) {
+                ++cnt[e.first];
+                ++cnt[e.second];
+            }
+            return;
+        }
         for (const auto& e : parts[i].edges) {
             __atomic_fetch_add(&cnt[e.first], 1u, __ATOMIC_RELAXED);
             __atomic_fetch_add(&cnt[e.second], 1u, __ATOMIC_RELAXED);
@@ -234,9 +242,16 @@ int psm_text_graph_read(const char* path, uint32_t threads, psm_text_graph** out
     g->adj.assign(2 * E, 0);
     std::vector<uint32_t> cur(g->off.begin(), g->off.begin() + V);
     run([&](uint32_t i) {
-        for (const auto& e : parts[i].edges) {
-            g->adj[__atomic_fetch_add(&cur[e.first], 1u, __ATOMIC_RELAXED)] = e.second;
-            g->adj[__atomic_fetch_add(&cur[e.second], 1u, __ATOMIC_RELAXED)] = e.first;
+        if (T == 1) {
+            for (const auto& e : parts[i].edges) {
+                g->adj[cur[e.first]++] = e.second;
+                g->adj[cur[e.second]++] = e.first;
+            }
+        } else {
+            for (const auto& e : parts[i].edges) {
+                g->adj[__atomic_fetch_add(&cur[e.first], 1u, __ATOMIC_RELAXED)] = e.second;
+                g->adj[__atomic_fetch_add(&cur[e.second], 1u, __ATOMIC_RELAXED)] = e.first;
+            }
         }
         std::vector<std::pair<uint32_t, uint32_t>>().swap(parts[i].edges);
     });
