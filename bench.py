@@ -3,13 +3,16 @@
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload C1|C2|C3|C4|C5]
 
 Workload C2 (seeded, synthetic): Chung-Lu 1.1M vertices / 3.0M edges,
-100 Zipf(1.1) labels; 512 queries of 4/8/16/24/32 vertices, half dense and
-half sparse; GQL filter + GQL order; count per query capped at 1e5 and
-reported as min(cap, total). A step is one batched pass over the 512 queries
-(filter, candidate-edge tables, plans, enumeration) with the data graph
-resident in HBM. The JSON line carries queries/s, embeddings/s, the plain
-one-query-at-a-time GPU path, p50/p99 latencies, the roofline of the
-dominant kernel and the reference CPU baseline.
+100 Zipf(1.1) labels; 512 random-walk queries of 4/8/16/24/32 vertices, half
+dense and half sparse; GQL filter + GQL order; count per query capped at 1e5
+and reported as min(cap, total), no time limit (every query completes). A
+step is one batched pass over the 512 queries (filter, candidate-edge
+tables, plans, enumeration) with the data graph resident in HBM. The JSON
+line carries queries/s, embeddings/s, per-query p50/p99 latencies of the
+batched form and of the plain one-query-at-a-time GPU path, the roofline of
+the dominant kernel and the CPU baseline: the oracle restatement of the
+reference path on the host cores at 1 and nproc threads, on a fixed sample
+of the queries on which the GPU is timed as well.
 """
 from __future__ import annotations
 
@@ -32,51 +35,42 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(g, qs, meta, cap: int, budget_s: float, time_limit: float, threads: int = 1):
-    """The reference binary (oracle/_ref, built from the reference's sources)
-    on the first queries of the workload with `threads` worker threads (its
-    -threadnums), until about budget_s of its own reported query time; falls
-    back to the oracle restatement (one thread)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import tempfile
+def cpu_sample(qs, stride: int = 33):
+    """The CPU baseline's bounded sample: every stride-th query (stride odd and
+    prime to the five sizes, so sizes and the dense/sparse halves all occur)."""
+    return list(range(0, len(qs), stride))
 
-    from psm_benchmark_amd.graph_io import save_text
-    import ref_runner as R
-    if R.available():
-        td = R.deep_dir(tempfile.mkdtemp(prefix="psm_cpu_"))
-        dp = os.path.join(td, "data.txt")
-        save_text(g, dp)
-        spent, done, emb, timeouts = 0.0, 0, 0, 0
-        t_wall = time.time()
-        for i, q in enumerate(qs):
-            qp = os.path.join(td, "q.txt")
-            save_text(q, qp)
-            b = R.run(dp, qp, "GQL_GQL_GQL", threads=str(threads), cap=str(cap) if cap else "MAX",
-                      time_limit=max(1, int(round(time_limit))), timeout=600)[0]
-            spent += b["total_s"]
-            emb += b["embeddings"]
-            timeouts += b["overtime"]
-            done += 1
-            if spent >= budget_s or time.time() - t_wall > 3 * budget_s:
-                break
-        return dict(value=done / spent, unit="queries/s", cores=threads, kind="reference",
-                    embeddings_per_s=emb / spent, timeouts=timeouts, embeddings=emb // max(1, done),
-                    sample=f"first {done} of {len(qs)} queries, reference SubgraphMatching.out "
-                           f"GQL_GQL_GQL -threadnums {threads} -num {cap if cap else 'MAX'} -time_limit "
-                           f"{max(1, int(round(time_limit)))}, its own Total time "
-                           f"(filter+table+plan+enumerate, graph load excluded) = {spent:.2f}s")
+
+def cpu_baseline(g, qs, method: str, cap: int, threads: int, time_limit: float):
+    """The oracle restatement (oracle/psm_oracle.cpp, the project's own C++
+    port of the reference path) on the host cores, `threads` enumeration
+    workers (0: every hardware thread), each query under `time_limit` seconds
+    counted from the start of its filtering. Queries that hit it are timeouts:
+    their time is reported, they are not counted as completed."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_binding import OGraph, ORun
     d = OGraph(g)
-    spent, done = 0.0, 0
+    spent, done, emb, timeouts, used = 0.0, 0, 0, 0, 1
+    phases = dict(filter_s=0.0, build_s=0.0, plan_s=0.0, enum_s=0.0)
+    counts = []
     for q in qs:
-        t = time.time()
-        ORun(d, OGraph(q), "GQL_GQL_GQL", cap=cap)
-        spent += time.time() - t
-        done += 1
-        if spent >= budget_s:
-            break
-    return dict(value=done / spent, unit="queries/s", cores=1, kind="port",
-                sample=f"first {done} of {len(qs)} queries through the oracle restatement, one thread")
+        t = time.perf_counter()
+        r = ORun(d, OGraph(q), method, cap=cap, threads=threads, time_limit=time_limit)
+        spent += time.perf_counter() - t
+        used = int(r.times.threads)
+        for k in phases:
+            phases[k] += getattr(r.times, k)
+        if r.timed_out:
+            timeouts += 1
+            counts.append(None)
+        else:
+            done += 1
+            emb += r.count
+            counts.append(r.count)
+    return dict(value=done / spent if spent else float("nan"), unit="queries/s", cores=used, kind="port",
+                completed=done, timeouts=timeouts, seconds=round(spent, 3),
+                embeddings_per_s=emb / spent if spent else float("nan"),
+                phase_s={k: round(v, 3) for k, v in phases.items()}, counts=counts)
 
 
 def copy_calibration(mib: int, reps: int) -> None:
@@ -148,13 +142,15 @@ def main():
     ap.add_argument("--workload", default="C2")
     ap.add_argument("--cap", type=int, default=100_000)
     ap.add_argument("--plain", type=int, default=512, help="queries timed through the plain path")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-stride", type=int, default=33,
+                    help="C2: the CPU baseline runs every N-th query (and the GPU is timed on the same ones)")
+    ap.add_argument("--cpu-limit", type=float, default=60.0,
+                    help="per-query time limit of the CPU baseline (s); a query that hits it is a timeout")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", type=int, default=512, help="use the first N workload queries")
     ap.add_argument("--time-limit", type=float, default=None,
-                    help="time limit in seconds (default 2 s for C2, the -time_limit of the "
-                         "reference's recorded runs, test_anony/survey_bash/C_exp.py; 20 s for "
-                         "the uncapped C3 / C5 workloads)")
+                    help="time limit in seconds (default none for C1 and C2, whose queries all "
+                         "finish; 20 s for the uncapped C3 / C5 workloads)")
     ap.add_argument("--plain-limit", type=float, default=5.0,
                     help="C3 / C5: time limit of one plain-path query (s)")
     ap.add_argument("--plain-budget", type=float, default=60.0,
@@ -188,7 +184,7 @@ def main():
               "independently across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     if args.time_limit is None:
-        args.time_limit = 2.0 if args.workload == "C2" else 20.0
+        args.time_limit = 0.0 if args.workload in ("C1", "C2") else 20.0
     if args.workload == "C1":
         plain_workload(args)
         return
@@ -199,8 +195,9 @@ def main():
         serving_workload(args)
         return
 
+    from psm_benchmark_amd import autotune
     from psm_benchmark_amd import workloads as W
-    from psm_benchmark_amd.matcher import Device, DeviceGraph, Query, batch_match, match
+    from psm_benchmark_amd.matcher import Batch, Device, DeviceGraph, match
     from psm_benchmark_amd import _lib as L
 
     t0 = time.time()
@@ -219,32 +216,46 @@ def main():
     if args.shape:
         tuned = apply_shape(dev, args.shape)
     elif not args.no_tune:
-        from psm_benchmark_amd import autotune
         t0 = time.time()
         tuned, trials = autotune.tune(dg, qs, "GQL", "GQL", args.cap, sample=args.tune_sample,
                                       trial_limit_s=args.tune_limit, restore_limit_s=args.time_limit,
                                       log=log)
         log(f"autotune picked {tuned} ({time.time() - t0:.1f}s)")
+
+    def step(queries):
+        """One batched pass: create, filter + tables + plans, enumeration."""
+        t = time.perf_counter()
+        b = Batch(dg, queries)
+        try:
+            b.prepare("GQL", "GQL")
+            counts, calls, status, st = b.evaluate(args.cap)
+            dt = time.perf_counter() - t
+            lat = b.latencies()
+        finally:
+            b.close()
+        return dt, counts, calls, status, st, lat
+
     for _ in range(args.warmup):
-        counts, calls, status, st = batch_match(dg, qs, "GQL", "GQL", args.cap)
-        log(f"warmup: total {st.total_ms:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
+        dt, counts, calls, status, st, _ = step(qs)
+        log(f"warmup: {dt * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
             f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f})")
     L.lib().psm_kernel_stats_reset(dev.h)
-    times, stats, nodes = [], [], 0
+    times, stats, lats, nodes = [], [], [], 0
     for _ in range(args.steps):
-        t = time.perf_counter()
-        counts, calls, status, st = batch_match(dg, qs, "GQL", "GQL", args.cap)
-        times.append(time.perf_counter() - t)
+        dt, counts, calls, status, st, lat = step(qs)
+        times.append(dt)
         stats.append(st)
+        lats.append(lat)
         nodes += int(calls.sum())
-        log(f"step: {times[-1] * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
-            f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f})")
+        log(f"step: {dt * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
+            f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f}), timeouts {int((status == L.PSM_ETIMEOUT).sum())}")
     ks = kernel_stats(dev)
     timeouts = int((status == L.PSM_ETIMEOUT).sum())
     total = sum(times)
     nq = len(qs)
     emb = int(counts.sum())
     step_ms = total / args.steps * 1e3
+    lat_all = np.concatenate(lats)
 
     # plain path: one query at a time through the same library, with the
     # library's default launch shape (what a caller gets without tuning; the
@@ -265,6 +276,8 @@ def main():
         raise SystemExit(f"batched and plain counts differ for queries {mismatch[:10]}")
     lat_ms = np.array(lat if lat else [float("nan")]) * 1e3
     plain_qps = len(lat) / sum(lat) if lat else float("nan")
+    if tuned:
+        autotune.set_config(dev, tuned)
 
     # dominant kernel roofline (bytes counted by the kernels themselves,
     # durations from HIP events on the library's stream)
@@ -285,14 +298,31 @@ def main():
                                         "a leaf (kernel-counted; equals the oracle's sum when uncapped)")
         roofline["touched_bytes_per_launch"] = int(sum(s.probe_bytes for s in stats) / args.steps)
 
-    cpu = cpu_n = None
+    cpu = cpu_n = gpu_sample = None
     if not args.no_cpu:
-        log("cpu baseline, 1 thread")
-        cpu = cpu_baseline(g, qs, meta, args.cap, args.cpu_budget, args.time_limit)
+        idx = cpu_sample(qs, args.cpu_stride)
+        sample = [qs[i] for i in idx]
+        # the batched GPU form on exactly the CPU's queries (the like-for-like ratio)
+        reps = []
+        for _ in range(max(3, args.steps // 4)):
+            dt, scounts, _, sstatus, _, _ = step(sample)
+            reps.append(dt)
+        gpu_sample = dict(queries=len(sample), ms_per_pass=round(float(np.mean(reps)) * 1e3, 3),
+                          queries_per_s=round(len(sample) / float(np.mean(reps)), 2))
+        sample_desc = (f"queries {idx[0]}, {idx[1]}, ..., {idx[-1]} (every {args.cpu_stride}th of the {nq}: "
+                       f"{len(sample)} queries, all sizes, dense and sparse), GQL_GQL_GQL, cap {args.cap}, "
+                       f"per-query limit {args.cpu_limit:g} s; the GPU batched form is timed on the same queries")
+        log(f"cpu baseline, 1 thread, {len(sample)} queries")
+        cpu = cpu_baseline(g, sample, "GQL_GQL_GQL", args.cap, 1, args.cpu_limit)
         nproc = min(os.cpu_count() or 1, 16)   # the GPU host's CPU share
-        if nproc > 1 and cpu.get("kind") == "reference":
-            log(f"cpu baseline, {nproc} threads")
-            cpu_n = cpu_baseline(g, qs, meta, args.cap, args.cpu_budget, args.time_limit, threads=nproc)
+        log(f"cpu baseline, {nproc} threads")
+        cpu_n = cpu_baseline(g, sample, "GQL_GQL_GQL", args.cap, nproc, args.cpu_limit)
+        for c in (cpu, cpu_n):
+            bad = [i for i, x in zip(idx, c["counts"]) if x is not None and x != int(counts[i])]
+            if bad:
+                raise SystemExit(f"oracle and device counts differ for queries {bad[:10]}")
+            c["sample"] = sample_desc
+            del c["counts"]
 
     out = {
         "metric": "queries/s + embeddings/s on 1 MI355X; p50/p99 latency batched vs plain",
@@ -308,8 +338,8 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded Chung-Lu graph and random-walk queries)",
         "config": {"workload": "C2 batched: Chung-Lu 1.1M V / 3.0M E, 100 Zipf(1.1) labels, "
-                               "512 queries of 4-32 vertices (half dense), GQL_GQL_GQL, cap 1e5",
-                   "queries": nq, "cap": args.cap, "time_limit_s": args.time_limit},
+                               "512 random-walk queries of 4-32 vertices (half dense), GQL_GQL_GQL, cap 1e5",
+                   "queries": nq, "cap": args.cap, "time_limit_s": args.time_limit or None},
         "timeouts": timeouts,
         "completed_queries_per_s": round((nq - timeouts) * args.steps / total, 2),
         "embeddings_per_s": round(emb * args.steps / total, 1),
@@ -317,7 +347,10 @@ def main():
         "search_nodes_per_s": round(nodes / total, 1),
         "dfs_lanes_busy": round(sum(s.dfs_lane_steps for s in stats) / max(1, sum(s.dfs_lane_iters for s in stats)), 4),
         "launch_shape": tuned,
-        "latency_ms": {"batched_p50": round(step_ms, 2), "batched_p99": round(step_ms, 2),
+        "latency_ms": {"batched_p50": round(float(np.percentile(lat_all, 50)), 3),
+                       "batched_p99": round(float(np.percentile(lat_all, 99)), 3),
+                       "batched_definition": "per query: from the start of its batched step until its count "
+                                             "was final (cap reached or last task ended, device clock)",
                        "plain_p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "plain_p99": round(float(np.percentile(lat_ms, 99)), 3)},
         "plain_gpu_queries_per_s": round(plain_qps, 2),
@@ -332,10 +365,10 @@ def main():
         "cpu_baseline": cpu,
     }
     if cpu:
-        out["batched_over_cpu"] = round(out["value"] / cpu["value"], 1)
-    if cpu_n:
+        out["gpu_on_cpu_sample"] = gpu_sample
+        out["batched_over_cpu"] = round(gpu_sample["queries_per_s"] / cpu["value"], 1)
         out["cpu_baseline_nproc"] = cpu_n
-        out["batched_over_cpu_nproc"] = round(out["value"] / cpu_n["value"], 1)
+        out["batched_over_cpu_nproc"] = round(gpu_sample["queries_per_s"] / cpu_n["value"], 1)
     print(json.dumps(out))
 
 
@@ -388,12 +421,18 @@ def plain_workload(args):
     ks = kernel_stats(dev)
     total = sum(lat)
     lat_ms = np.array(lat) * 1e3
-    cpu = None
+    cpu = cpu_n = None
     if not args.no_cpu:
+        nproc = min(os.cpu_count() or 1, 16)   # the GPU host's CPU share
         log("cpu baseline, 1 thread")
-        cpu = cpu_baseline(g, [q] * 1000, None, 0, args.cpu_budget, args.time_limit)
-        if cpu.get("kind") == "reference" and cpu.get("embeddings") != r.count:
-            raise SystemExit(f"reference count {cpu.get('embeddings')} != device count {r.count}")
+        cpu = cpu_baseline(g, [q], "GQL_GQL_GQL", 0, 1, args.cpu_limit)
+        log(f"cpu baseline, {nproc} threads")
+        cpu_n = cpu_baseline(g, [q], "GQL_GQL_GQL", 0, nproc, args.cpu_limit)
+        for c in (cpu, cpu_n):
+            if c["counts"][0] is not None and c["counts"][0] != r.count:
+                raise SystemExit(f"oracle count {c['counts'][0]} != device count {r.count}")
+            c["sample"] = f"the same C1 query once, GQL_GQL_GQL uncapped, limit {args.cpu_limit:g} s"
+            del c["counts"]
     out = {
         "metric": "queries/s + embeddings/s on 1 MI355X; p50/p99 latency batched vs plain",
         "value": round(args.steps / total, 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
@@ -413,6 +452,8 @@ def plain_workload(args):
     }
     if cpu:
         out["gpu_over_cpu"] = round(out["value"] / cpu["value"], 2)
+        out["cpu_baseline_nproc"] = cpu_n
+        out["gpu_over_cpu_nproc"] = round(out["value"] / cpu_n["value"], 2)
     print(json.dumps(out))
 
 

@@ -111,19 +111,27 @@ int psm_set_time_limit(psm_ctx* ctx, double seconds);
  *   PSM_TUNE_DFS_STRATEGY           PSM_DFS_LANE: one search per lane;
  *                                   PSM_DFS_WAVE: one search per wavefront
  *                                   with chunked cooperative intersection
+ *   PSM_TUNE_QUEUE_SLOTS            free slots of the task ring beyond the
+ *                                   pre-expanded tasks (0: automatic, the
+ *                                   larger of 2^18 and 4 per DFS lane); the
+ *                                   ring reuses slots, so this bounds only
+ *                                   the tasks queued at one time
  * No counterpart in the reference (its only knob is -threadnums). */
 #define PSM_TUNE_DFS_WAVES_PER_CU 0
 #define PSM_TUNE_EXPAND_TASKS_PER_LANE 1
 #define PSM_TUNE_DFS_STRATEGY 2
+#define PSM_TUNE_QUEUE_SLOTS 3
 #define PSM_DFS_LANE 0
 #define PSM_DFS_WAVE 1
 int psm_set_tuning(psm_ctx* ctx, int key, double value);
 int psm_get_tuning(psm_ctx* ctx, int key, double* value);
 
 /* ---- data graph: class Graph (graph/graph.h:21-237) ------------------
- * Uploads a CSR data graph (adjacency sorted ascending per vertex) and builds
- * on the device side the label reverse index (graph.cpp:15-33) and the
- * neighbour-label-frequency runs (graph.cpp:36-52). */
+ * Builds on the host, from a CSR data graph (adjacency sorted ascending per
+ * vertex), the label reverse index (graph.cpp:15-33), the
+ * neighbour-label-frequency runs (graph.cpp:36-52) and the label-grouped
+ * adjacency, and uploads the graph with them to HBM, where it stays resident
+ * until psm_graph_destroy. */
 int psm_graph_upload(psm_ctx* ctx, uint32_t num_vertices, const uint32_t* offsets,
                      const uint32_t* neighbors, const uint32_t* labels, psm_graph** out);
 int psm_graph_destroy(psm_graph* g);
@@ -259,6 +267,12 @@ int psm_batch_create(psm_ctx* ctx, const psm_graph* g, uint32_t num_queries, con
 int psm_batch_prepare(psm_batch* b, int filter, int order);
 int psm_batch_evaluate(psm_batch* b, uint64_t cap, double enum_time_limit_s, uint64_t* counts,
                        uint64_t* calls, uint32_t* status, psm_batch_stats* stats);
+/* Per-query latency of the last psm_batch_evaluate: milliseconds from
+ * psm_batch_create until the query's count was final (its cap reached or its
+ * last enumeration task ended, read from the device clock), one entry per
+ * query; a query without enumeration tasks reports the end of the evaluate
+ * call. No counterpart in the reference. */
+int psm_batch_latencies(const psm_batch* b, double* ms);
 int psm_batch_destroy(psm_batch* b);
 
 /* ---- serving form ----------------------------------------------------------
@@ -325,7 +339,9 @@ int psm_gen_rmat(uint32_t scale, uint64_t E, double a, double b, double c, uint6
 int psm_gen_labels(uint32_t V, uint32_t L, double zipf_s, uint64_t seed, uint32_t* labels);
 /* A connected n-vertex query drawn from the data graph: dense = 0 the tree of
  * a random walk, 1 the induced subgraph of a greedy densest growth, 2 the
- * induced subgraph of a random walk's vertices (cycles). Returns the number
+ * induced subgraph of a random walk's vertices (cycles), 3 the induced
+ * subgraph of a degree-biased random walk's vertices (each step picks a
+ * neighbour with probability proportional to its degree). Returns the number
  * of query edges (pairs written to out_edges) or a negative status. */
 int64_t psm_gen_query(uint32_t V, const uint32_t* offsets, const uint32_t* neighbors, uint32_t n,
                       int dense, uint64_t seed, uint32_t* out_vertices, uint32_t* out_edges,

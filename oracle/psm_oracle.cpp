@@ -18,6 +18,8 @@
 // Exported C ABI (ctypes): orc_graph_new/free, orc_run/free and accessors.
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <iterator>
@@ -25,6 +27,7 @@
 #include <numeric>
 #include <queue>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -788,7 +791,8 @@ void backward_neighbours(const OGraph& q, Plan& p, bool drop_pivot) {
 }
 
 // ---------------------------------------------------------------------------
-// Enumeration engines at one thread (one task holding every root candidate):
+// Enumeration engines (at one thread, one task holding every root candidate;
+// enumerate() below also runs them with several threads):
 // GQL   EvaluateQuery.cpp:980-1050 + :2156-2194
 // EXPLORE :1052-1126 + :2369-2426
 // LFTJ  :231-372 + :2024-2082
@@ -819,8 +823,27 @@ uint64_t node_row_entries(const Tables& T, const std::vector<uint32_t>& earlier,
     return s;
 }
 
-void enumerate(const OGraph& d, const OGraph& q, const Cands& C, const Tables& T,
-               const Plan& p, Engine eng, uint64_t cap, EnumResult& r) {
+// Shared state of one enumeration run by `threads` workers. The root
+// candidates are claimed one at a time from `next` (the reference chunks them
+// into 16 * threads tasks and donates halves of running tasks to idle
+// threads, EvaluateQuery.cpp:58-99 / TaskPool.cpp; a root claimed per task is
+// the simplest schedule with the same counts). With one worker the claims
+// follow the root order, so calls and the cap stop equal the sequential
+// engines' exactly; with several the count is min(cap, total) and calls are
+// exact when the cap is not reached.
+struct SearchShared {
+    uint64_t cap = UINT64_MAX;
+    uint32_t threads = 1;
+    double time_limit_s = 0;   // 0: none
+    std::chrono::steady_clock::time_point t0;
+    std::atomic<uint64_t> next{0};
+    std::atomic<uint64_t> count{0};
+    std::atomic<bool> stop{false};
+    std::atomic<bool> timed_out{false};
+};
+
+void enumerate_worker(const OGraph& d, const OGraph& q, const Cands& C, const Tables& T,
+                      const Plan& p, Engine eng, SearchShared& sh, EnumResult& r) {
     uint32_t n = q.V;
     r = EnumResult();
     r.depth_nodes.assign(n, 0);
@@ -840,9 +863,7 @@ void enumerate(const OGraph& d, const OGraph& q, const Cands& C, const Tables& T
         }
     }
     uint32_t root = p.order[0];
-    cand[0].resize(C[root].size());
-    if (eng == kGQL) cand[0] = C[root];
-    else std::iota(cand[0].begin(), cand[0].end(), 0u);
+    const uint64_t n_roots = C[root].size();
 
     auto gen = [&](uint32_t dep) {
         uint32_t u = p.order[dep];
@@ -882,37 +903,95 @@ void enumerate(const OGraph& d, const OGraph& q, const Cands& C, const Tables& T
         }
     };
 
-    int dep = 0;
-    pos[0] = 0;
-    while (true) {
-        while (pos[dep] < cand[dep].size()) {
-            uint32_t u = p.order[dep];
-            uint32_t x = cand[dep][pos[dep]++];
-            uint32_t v = eng == kGQL ? x : C[u][x];
-            r.calls++;
-            if (eng == kLFTJ && visited[v]) continue;
-            r.depth_nodes[dep]++;
-            emb[u] = v;
-            // candidate index of the assignment (the GQL engine walks vertex
-            // ids; C(u) is sorted)
-            eidx[u] = eng == kGQL ? (uint32_t)(std::lower_bound(C[u].begin(), C[u].end(), v) - C[u].begin()) : x;
-            visited[v] = 1;
-            if (dep == (int)n - 1) {
-                r.count++;
-                visited[v] = 0;
-                if (r.count >= cap) return;
-            } else {
-                if (dep >= 1) r.algo_bytes += 4;   // a survivor written below the root level
-                ++dep;
-                pos[dep] = 0;
-                gen(dep);
-                r.algo_bytes += 4 * node_row_entries(T, earlier[dep], p.order[dep], eidx);
+    const bool single = sh.threads <= 1;
+    uint64_t unflushed = 0;   // leaves not yet added to sh.count (several workers)
+    uint64_t tick = 0;
+    while (!sh.stop.load(std::memory_order_relaxed)) {
+        // claim the next root candidate: this worker's task is its subtree
+        const uint64_t k = sh.next.fetch_add(1, std::memory_order_relaxed);
+        if (k >= n_roots) break;
+        cand[0].assign(1, eng == kGQL ? C[root][k] : (uint32_t)k);
+        int dep = 0;
+        pos[0] = 0;
+        while (true) {
+            while (pos[dep] < cand[dep].size()) {
+                uint32_t u = p.order[dep];
+                uint32_t x = cand[dep][pos[dep]++];
+                uint32_t v = eng == kGQL ? x : C[u][x];
+                r.calls++;
+                if ((++tick & 0xFFFF) == 0) {
+                    if (sh.stop.load(std::memory_order_relaxed)) goto out;
+                    if (sh.time_limit_s > 0 &&
+                        std::chrono::duration<double>(std::chrono::steady_clock::now() - sh.t0).count() >
+                            sh.time_limit_s) {
+                        sh.timed_out = true;
+                        sh.stop = true;
+                        goto out;
+                    }
+                }
+                if (eng == kLFTJ && visited[v]) continue;
+                r.depth_nodes[dep]++;
+                emb[u] = v;
+                // candidate index of the assignment (the GQL engine walks
+                // vertex ids; C(u) is sorted)
+                eidx[u] = eng == kGQL ? (uint32_t)(std::lower_bound(C[u].begin(), C[u].end(), v) - C[u].begin()) : x;
+                visited[v] = 1;
+                if (dep == (int)n - 1) {
+                    r.count++;
+                    visited[v] = 0;
+                    if (single) {
+                        if (r.count >= sh.cap) { sh.stop = true; goto out; }
+                    } else if (++unflushed == 256) {
+                        if (sh.count.fetch_add(unflushed) + unflushed >= sh.cap) sh.stop = true;
+                        unflushed = 0;
+                        if (sh.stop.load(std::memory_order_relaxed)) goto out;
+                    }
+                } else {
+                    if (dep >= 1) r.algo_bytes += 4;   // a survivor written below the root level
+                    ++dep;
+                    pos[dep] = 0;
+                    gen(dep);
+                    r.algo_bytes += 4 * node_row_entries(T, earlier[dep], p.order[dep], eidx);
+                }
             }
+            --dep;
+            if (dep < 0) break;
+            visited[emb[p.order[dep]]] = 0;
         }
-        --dep;
-        if (dep < 0) break;
-        visited[emb[p.order[dep]]] = 0;
     }
+out:
+    if (unflushed) sh.count.fetch_add(unflushed);
+}
+
+// Runs the engine with `threads` workers (0: every hardware thread) under an
+// optional time limit; r.count = min(cap, total).
+void enumerate(const OGraph& d, const OGraph& q, const Cands& C, const Tables& T,
+               const Plan& p, Engine eng, uint64_t cap, EnumResult& r, uint32_t threads = 1,
+               double time_limit_s = 0, bool* timed_out = nullptr) {
+    SearchShared sh;
+    sh.cap = cap;
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    sh.threads = threads;
+    sh.time_limit_s = time_limit_s;
+    sh.t0 = std::chrono::steady_clock::now();
+    if (threads == 1) {
+        enumerate_worker(d, q, C, T, p, eng, sh, r);
+    } else {
+        std::vector<EnumResult> part(threads);
+        std::vector<std::thread> pool;
+        for (uint32_t t = 0; t < threads; ++t)
+            pool.emplace_back([&, t] { enumerate_worker(d, q, C, T, p, eng, sh, part[t]); });
+        for (auto& th : pool) th.join();
+        r = EnumResult();
+        r.depth_nodes.assign(q.V, 0);
+        for (auto& x : part) {
+            r.calls += x.calls;
+            r.algo_bytes += x.algo_bytes;
+            for (uint32_t i = 0; i < x.depth_nodes.size(); ++i) r.depth_nodes[i] += x.depth_nodes[i];
+        }
+        r.count = std::min<uint64_t>(cap, sh.count.load());
+    }
+    if (timed_out) *timed_out = sh.timed_out.load();
 }
 
 // ---------------------------------------------------------------------------
@@ -959,11 +1038,25 @@ void orc_graph_free(void* g) { delete static_cast<OGraph*>(g); }
 
 uint32_t orc_graph_labels(void* g) { return static_cast<OGraph*>(g)->L; }
 
+// Phase times of one run (steady_clock, seconds), the phases the reference
+// logs (Args.cpp:517-535): filter (with sortCandidates), build table,
+// generate query plan, enumerate.
+struct orc_times {
+    double filter_s, build_s, plan_s, enum_s, total_s;
+    uint32_t threads;
+    uint32_t timed_out;
+};
+
 // filter: "LDF" "NLF" "GQL" "CFL" "DPiso"; order: "GQL" "CFL" "DPiso";
 // engine: "GQL" "EXPLORE" "LFTJ" (not EXPLORE with the DPiso order).
+// threads: enumeration workers (0: every hardware thread); time_limit_s > 0
+// stops the enumeration once that many seconds have passed since the start
+// of filtering (the reference's -time_limit), setting times->timed_out.
 // Returns a run handle (NULL on a bad argument).
-void* orc_run(void* dg, void* qg, const char* filter, const char* order, const char* engine,
-              uint64_t cap, orc_result* out) {
+void* orc_run_mt(void* dg, void* qg, const char* filter, const char* order, const char* engine,
+                 uint64_t cap, uint32_t threads, double time_limit_s, orc_result* out, orc_times* times) {
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
     const OGraph& d = *static_cast<OGraph*>(dg);
     const OGraph& q = *static_cast<OGraph*>(qg);
     if (q.V == 0 || q.V > 64) return nullptr;
@@ -973,7 +1066,9 @@ void* orc_run(void* dg, void* qg, const char* filter, const char* order, const c
     else if (e == "EXPLORE") eng = kExplore;
     else if (e == "LFTJ") eng = kLFTJ;
     else return nullptr;
+    if (!(o == "GQL" || o == "CFL" || (o == "DPiso" && eng != kExplore))) return nullptr;
     Run* r = new Run();
+    const clk::time_point t0 = clk::now();
     if (f == "LDF") r->filter_ok = ldf_nlf_filter(d, q, false, r->C);
     else if (f == "NLF") r->filter_ok = ldf_nlf_filter(d, q, true, r->C);
     else if (f == "GQL") r->filter_ok = gql_filter(d, q, r->C);
@@ -981,22 +1076,38 @@ void* orc_run(void* dg, void* qg, const char* filter, const char* order, const c
     else if (f == "DPiso") r->filter_ok = dpiso_filter(d, q, r->C);
     else { delete r; return nullptr; }
     for (auto& c : r->C) std::sort(c.begin(), c.end());   // sortCandidates
+    const clk::time_point t1 = clk::now();
     build_tables(d, q, r->C, r->T);
     r->mem_words = memory_words(q, r->C, r->T);
+    const clk::time_point t2 = clk::now();
     if (o == "GQL") {
         gql_order(q, r->C, d.V, r->plan);
         backward_neighbours(q, r->plan, false);
     } else if (o == "CFL") {
         cfl_order(d, q, r->C, r->T, r->plan);
         backward_neighbours(q, r->plan, true);
-    } else if (o == "DPiso" && eng != kExplore) {
+    } else {
         // the reference plans DPiso orders with plain backward neighbours
         // (StudyPerformance.cpp:1397-1414), which the pivot-based EXPLORE
         // engine cannot use
         dpiso_order(d, q, r->plan);
         backward_neighbours(q, r->plan, false);
-    } else { delete r; return nullptr; }
-    enumerate(d, q, r->C, r->T, r->plan, eng, cap ? cap : UINT64_MAX, r->er);
+    }
+    const clk::time_point t3 = clk::now();
+    double left = 0;
+    if (time_limit_s > 0) left = std::max(1e-6, time_limit_s - secs(t0, t3));
+    bool to = false;
+    enumerate(d, q, r->C, r->T, r->plan, eng, cap ? cap : UINT64_MAX, r->er, threads, left, &to);
+    const clk::time_point t4 = clk::now();
+    if (times) {
+        times->filter_s = secs(t0, t1);
+        times->build_s = secs(t1, t2);
+        times->plan_s = secs(t2, t3);
+        times->enum_s = secs(t3, t4);
+        times->total_s = secs(t0, t4);
+        times->threads = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+        times->timed_out = to;
+    }
     if (out) {
         memset(out, 0, sizeof(*out));
         out->count = r->er.count;
@@ -1013,6 +1124,12 @@ void* orc_run(void* dg, void* qg, const char* filter, const char* order, const c
         }
     }
     return r;
+}
+
+// One thread, no time limit.
+void* orc_run(void* dg, void* qg, const char* filter, const char* order, const char* engine,
+              uint64_t cap, orc_result* out) {
+    return orc_run_mt(dg, qg, filter, order, engine, cap, 1, 0.0, out, nullptr);
 }
 
 void orc_run_free(void* r) { delete static_cast<Run*>(r); }

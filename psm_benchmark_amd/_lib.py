@@ -97,6 +97,7 @@ class Result(ctypes.Structure):
 PSM_TUNE_DFS_WAVES_PER_CU = 0
 PSM_TUNE_EXPAND_TASKS_PER_LANE = 1
 PSM_TUNE_DFS_STRATEGY = 2
+PSM_TUNE_QUEUE_SLOTS = 3
 PSM_DFS_LANE = 0
 PSM_DFS_WAVE = 1
 SPLITS = {"STAR": 0, "TWINTWIG": 1, "SEED": 2}
@@ -135,6 +136,7 @@ SIGNATURES = {
     "psm_batch_prepare": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int]),
     "psm_batch_evaluate": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.c_double, U64P, U64P, U32P,
                                           ctypes.POINTER(BatchStats)]),
+    "psm_batch_latencies": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_double)]),
     "psm_batch_destroy": (ctypes.c_int, [VP]),
     "psm_server_create": (ctypes.c_int, [VP, VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(VP)]),

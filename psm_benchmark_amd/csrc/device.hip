@@ -1263,16 +1263,45 @@ __device__ __forceinline__ int test_merge(const DPlan& P, const DOp* __restrict_
 }
 
 // Device-coherent loads of a queued task record. A lane that splits a task
-// off writes its record back from its own caches (release fence) before it
-// sets the record's ready flag; the claiming lane reads the flag and then the
-// record with coherent loads, which fetch the written-back data. An acquire
-// on the flag instead would invalidate the claiming CU's and XCD's caches at
-// every claim, and claims come at millions per launch.
+// off writes the record, makes it visible at agent scope (release fence) and
+// then stores the slot's sequence word with release semantics. The claiming
+// lane loads the sequence word (relaxed), waits for that load to return
+// (record_wait: s_waitcnt on the vector memory counter, so no record load can
+// issue before the word was observed), and then reads the record with
+// agent-scope coherent loads, which are served by the L2 the release wrote
+// to. This is an acquire without its L1 invalidate: an acquire fence per
+// claim would invalidate the claiming CU's vector L1 millions of times per
+// launch, and the coherent loads do not use the L1 anyway.
 __device__ __forceinline__ uint32_t coherent_u32(const uint32_t* p) {
     return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t coherent_u64(const uint64_t* p) {
     return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void record_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Sequence word of ring position p (see DQueue).
+__device__ __forceinline__ uint32_t seq_load(uint32_t* seq, uint32_t mask, uint32_t p) {
+    return __hip_atomic_load(seq + (p & mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The record of position p is written: claimable.
+__device__ __forceinline__ void seq_publish(uint32_t* seq, uint32_t mask, uint32_t p) {
+    __hip_atomic_store(seq + (p & mask), p + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The record of position p has been read: its slot is free for p + mask + 1.
+// Release: the record loads complete before the slot can be rewritten.
+__device__ __forceinline__ void seq_release(uint32_t* seq, uint32_t mask, uint32_t p) {
+    __hip_atomic_store(seq + (p & mask), p + mask + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Marks query q's count final (first caller wins).
+__device__ __forceinline__ void mark_final(QCounters* qc, uint32_t q) {
+    if (__hip_atomic_load(&qc[q].done_tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        atomicCAS(&qc[q].done_tick, 0ull, (unsigned long long)wall_clock64());
+}
+// Subtracts k ended tasks from pending[q]; the last one marks q final.
+__device__ __forceinline__ void pending_sub(uint32_t* pending, QCounters* qc, uint32_t q, uint32_t k) {
+    if (atomicSub(pending + q, k) == k) mark_final(qc, q);
 }
 
 __device__ __forceinline__ uint32_t load_relaxed(unsigned int* p) {
@@ -1300,7 +1329,10 @@ __device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t c
                                            bool& near_cap) {
     if (lcount) {
         const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
-        if (prev + lcount >= cap) atomicExch(&qc[q].done, 1u);
+        if (prev + lcount >= cap) {
+            atomicExch(&qc[q].done, 1u);
+            mark_final(qc, q);
+        }
         near_cap = prev + lcount >= cap / 2;
         lcount = 0;
     }
@@ -1325,7 +1357,7 @@ __device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t c
 __global__ __launch_bounds__(kBlock) void k_dfs(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
-    const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* ready, DQueue* Q,
+    const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* seq, DQueue* Q,
     QCounters* qc, uint32_t* pending, uint32_t max_n, EmitSink sink) {
     extern __shared__ __align__(16) unsigned char lane_smem[];
     const uint32_t lane = lane_id();
@@ -1339,8 +1371,13 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     st.stride = blockDim.x;
     st.lane = threadIdx.x;
 
-    if (threadIdx.x == 0 && Q->limit_ticks)
-        atomicCAS(&Q->deadline, 0ull, (unsigned long long)wall_clock64() + Q->limit_ticks);
+    if (threadIdx.x == 0) {
+        const unsigned long long now = (unsigned long long)wall_clock64();
+        if (Q->limit_ticks) atomicCAS(&Q->deadline, 0ull, now + Q->limit_ticks);
+        if (__hip_atomic_load(&Q->start_tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            atomicCAS(&Q->start_tick, 0ull, now);
+    }
+    const uint32_t qmask = Q->mask;
 
     bool active = false, waiting = false;
     uint32_t wslot = 0, wend = 0, backoff = 0;
@@ -1370,23 +1407,31 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         }
         // claim queued tasks for idle lanes, one atomic per wavefront (none
         // once the time limit has passed)
-        // While the queue is long, each lane gets several consecutive slots
-        // per claim (up to kClaimMax) and takes them one after the other:
-        // claims go through one compare-and-swap on the shared head, whose
-        // rate (not the search) bounded launches with millions of small tasks.
+        // While the queue holds many tasks per lane (more than 16), each lane
+        // gets several consecutive positions per claim (up to kClaimMax) and
+        // takes them one after the other: claims go through one
+        // compare-and-swap on the shared head, whose rate (not the search)
+        // bounded launches with millions of small tasks. Below that a lane
+        // takes one task per claim: a lane holding several heavy subtrees
+        // would run them in turn while other lanes wait.
         const unsigned long long want = __ballot(!active && !waiting && load_relaxed(&Q->timeout) == 0);
         if (want && backoff == 0) {
             const uint32_t leader = __ffsll((long long)want) - 1;
             uint32_t base = 0, got = 0, per = 1;
             if (lane == leader) {
                 const uint32_t h = load_relaxed(&Q->head), t = load_relaxed(&Q->tail);
-                if (h < t) {
-                    per = min(kClaimMax, max(1u, (t - h) / (2u * lanes_total)));
+                if (t != h) {
+                    per = min(kClaimMax, max(1u, (t - h) / (16u * lanes_total)));
                     const uint32_t k = min((uint32_t)__popcll(want) * per, t - h);
+                    // busy counts claimed, unfinished tasks; it rises before
+                    // the claim so that head == tail with busy == 0 is only
+                    // seen once no lane holds or is taking a task
+                    atomicAdd(&Q->busy, k);
                     if (atomicCAS(&Q->head, h, h + k) == h) {
-                        atomicAdd(&Q->busy, k);   // busy counts claimed, unfinished tasks
                         base = h;
                         got = k;
+                    } else {
+                        atomicSub(&Q->busy, k);
                     }
                 }
             }
@@ -1405,16 +1450,17 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         } else if (backoff) {
             --backoff;
         }
-        if (waiting && load_relaxed(ready + wslot)) {
+        if (waiting && seq_load(seq, qmask, wslot) == wslot + 1u) {
             waiting = false;
+            record_wait();
             // the record is read field by field with coherent loads (a local
             // copy of its prefix array would live in scratch memory)
-            const DTask* task = queue + wslot;
+            const DTask* task = queue + (wslot & qmask);
             const uint32_t tq = coherent_u32(&task->q);
             if (tq != q) {
                 // the lane's counters so far belong to the previous query
                 lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap);
-                if (pend_done) atomicSub(pending + q, pend_done);
+                if (pend_done) pending_sub(pending, qc, q, pend_done);
                 pend_done = 0;
             }
             q = tq;
@@ -1425,6 +1471,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 late = true;
             }
             if (late || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
+                seq_release(seq, qmask, wslot);
                 ++busy_done;
                 if (wslot + 1 < wend) {   // the lane's next claimed slot
                     ++wslot;
@@ -1432,24 +1479,24 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 } else {
                     atomicSub(&Q->busy, busy_done);
                     busy_done = 0;
-                    if (pend_done) atomicSub(pending + q, pend_done);
+                    if (pend_done) pending_sub(pending, qc, q, pend_done);
                     pend_done = 0;
                 }
             } else {
                 n = P->n;
                 tdepth = coherent_u32(&task->depth);
                 d = tdepth;
-                for (uint32_t e = 0; e < tdepth; ++e) {
-                    const uint32_t x = coherent_u32(&task->prefix[e]);
-                    st.I_[st.at(e)] = x;
-                    st.M_[st.at(e)] = cand_pool[P->cand_base[e] + x];
-                }
+                for (uint32_t e = 0; e < tdepth; ++e) st.I_[st.at(e)] = coherent_u32(&task->prefix[e]);
                 const uint64_t thi = coherent_u64(&task->hi);
+                const uint64_t tlo = coherent_u64(&task->lo);
+                const uint32_t tsel = coherent_u32(&task->sel);
+                seq_release(seq, qmask, wslot);
+                for (uint32_t e = 0; e < tdepth; ++e) st.M_[st.at(e)] = cand_pool[P->cand_base[e] + st.I_[st.at(e)]];
                 if (thi == kAutoHi) {
                     enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo);
                 } else {
-                    enter_level(*P, ops, row_off, d, st, coherent_u32(&task->sel), L, lbytes, lalgo);
-                    L.cur = coherent_u64(&task->lo);
+                    enter_level(*P, ops, row_off, d, st, tsel, L, lbytes, lalgo);
+                    L.cur = tlo;
                     L.end = thi;
                 }
                 active = true;
@@ -1464,7 +1511,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         if (__ballot(active || waiting) == 0) {
             // the whole wavefront is idle: leave once no work can appear
             if (load_relaxed(&Q->timeout)) break;
-            if (load_relaxed(&Q->head) >= load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0) break;
+            if (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0) break;
             __builtin_amdgcn_s_sleep(4);
             continue;
         }
@@ -1515,7 +1562,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     for (uint32_t e = 0; e < d; ++e) out[e] = st.M_[st.at(e)];
                     out[d] = fv;
                 } else {
+                    // no room: the unit is rejected, so stop searching it
                     atomicExch(sink.overflow, 1u);
+                    atomicExch(&qc[q].done, 1u);
                 }
             } else if (found) {
                 const uint32_t a = st.at(d);
@@ -1572,7 +1621,10 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
             if (e <= d && e <= kPrefixMax) {
                 const uint32_t t = load_relaxed(&Q->tail);
-                if (t < Q->cap && atomicCAS(&Q->tail, t, t + 1) == t) {
+                const bool free_slot = seq_load(seq, qmask, t) == t;
+                if (!free_slot) {
+                    atomicAdd(&Q->full, 1u);
+                } else if (atomicCAS(&Q->tail, t, t + 1) == t) {
                     // shallower levels hold offsets from their base row: locate it
                     uint64_t base = 0;
                     uint32_t esel = L.sel;
@@ -1586,7 +1638,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     }
                     const uint64_t mid = ce + (ee - ce) / 2;
                     atomicAdd(pending + q, 1u);
-                    DTask& nt = queue[t];
+                    DTask& nt = queue[t & qmask];
                     nt.q = q;
                     nt.depth = e;
                     nt.sel = esel;
@@ -1595,7 +1647,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     nt.hi = base + ee;
                     for (uint32_t k = 0; k < e; ++k) nt.prefix[k] = st.I_[st.at(k)];
                     __threadfence();
-                    __hip_atomic_store(ready + t, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    seq_publish(seq, qmask, t);
                     if (e == d) L.end = mid;
                     else st.end_[st.at(e)] = (uint32_t)mid;
                     atomicAdd(&Q->donated, 1u);
@@ -1621,7 +1673,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             } else {
                 atomicSub(&Q->busy, busy_done);
                 busy_done = 0;
-                if (pend_done) atomicSub(pending + q, pend_done);
+                if (pend_done) pending_sub(pending, qc, q, pend_done);
                 pend_done = 0;
             }
         }
@@ -1813,12 +1865,17 @@ template <uint32_t N>
 __global__ __launch_bounds__(kWave) void k_dfs_wave(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
-    const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* ready, DQueue* Q,
+    const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* seq, DQueue* Q,
     QCounters* qc, uint32_t* pending) {
     const uint32_t lane = threadIdx.x;
     __shared__ WaveFrame<N> S;
-    if (lane == 0 && Q->limit_ticks)
-        atomicCAS(&Q->deadline, 0ull, (unsigned long long)wall_clock64() + Q->limit_ticks);
+    if (lane == 0) {
+        const unsigned long long now = (unsigned long long)wall_clock64();
+        if (Q->limit_ticks) atomicCAS(&Q->deadline, 0ull, now + Q->limit_ticks);
+        if (__hip_atomic_load(&Q->start_tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            atomicCAS(&Q->start_tick, 0ull, now);
+    }
+    const uint32_t qmask = Q->mask;
     const uint32_t hungry_below = gridDim.x / 2;   // queue shorter than half the wavefronts
     unsigned long long bytes = 0, algo = 0, busy_pos = 0, all_pos = 0;
 
@@ -1830,9 +1887,10 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
         if (__shfl(late, 0)) break;
         if (lane == 0) {
             const uint32_t h = load_relaxed(&Q->head), tl = load_relaxed(&Q->tail);
-            if (h < tl && atomicCAS(&Q->head, h, h + 1) == h) {
-                atomicAdd(&Q->busy, 1u);
-                slot = h;
+            if (tl != h) {
+                atomicAdd(&Q->busy, 1u);   // before the claim, as in k_dfs
+                if (atomicCAS(&Q->head, h, h + 1) == h) slot = h;
+                else atomicSub(&Q->busy, 1u);
             }
         }
         slot = __shfl(slot, 0);
@@ -1841,13 +1899,14 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             uint32_t stop = 0;
             if (lane == 0)
                 stop = load_relaxed(&Q->timeout) ||
-                       (load_relaxed(&Q->head) >= load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0);
+                       (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0);
             if (__shfl(stop, 0)) break;
             __builtin_amdgcn_s_sleep(8);
             continue;
         }
-        while (!load_relaxed(ready + slot)) __builtin_amdgcn_s_sleep(1);
-        const DTask* task = queue + slot;   // read with coherent loads (see coherent_u32)
+        while (seq_load(seq, qmask, slot) != slot + 1u) __builtin_amdgcn_s_sleep(1);
+        record_wait();
+        const DTask* task = queue + (slot & qmask);   // read with coherent loads (see coherent_u32)
         const uint32_t q = coherent_u32(&task->q);
         const DPlan& P = plans[q];
         uint32_t drop = 0;
@@ -1857,7 +1916,10 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                 atomicExch(&Q->timeout, 1u);
                 drop = 1;
             }
-            if (drop) atomicSub(&Q->busy, 1u);
+            if (drop) {
+                seq_release(seq, qmask, slot);
+                atomicSub(&Q->busy, 1u);
+            }
         }
         if (__shfl(drop, 0)) continue;
         const uint32_t n = P.n, t = coherent_u32(&task->depth);
@@ -1880,6 +1942,8 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             }
         }
         __syncthreads();
+        // one wavefront: lane 0's release waits for every lane's record loads
+        if (lane == 0) seq_release(seq, qmask, slot);
 
         uint32_t g = t, cnt = 0, it = 0;
         int cache_g = -1;
@@ -2037,7 +2101,10 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                     if (lcount | lcalls) {
                         const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
                         atomicAdd(&qc[q].calls, lcalls);
-                        if (lcount && prev + lcount >= P.cap) atomicExch(&qc[q].done, 1u);
+                        if (lcount && prev + lcount >= P.cap) {
+                            atomicExch(&qc[q].done, 1u);
+                            mark_final(qc, q);
+                        }
                     }
                     if (lrej) atomicAdd(&qc[q].rejects, lrej);
                     stop = __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout);
@@ -2066,13 +2133,14 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                         uint32_t ns2 = kInvalid;
                         if (lane == 0) {
                             const uint32_t tl = load_relaxed(&Q->tail);
-                            if (tl < Q->cap && atomicCAS(&Q->tail, tl, tl + 1) == tl) ns2 = tl;
+                            if (seq_load(seq, qmask, tl) != tl) atomicAdd(&Q->full, 1u);
+                            else if (atomicCAS(&Q->tail, tl, tl + 1) == tl) ns2 = tl;
                         }
                         ns2 = __shfl(ns2, 0);
                         if (ns2 != kInvalid) {
                             const uint64_t gp = S.gen_pos[ed], ge = S.gen_end[ed];
                             const uint64_t mid = gp + (ge - gp) / 2;
-                            DTask& nt = queue[ns2];
+                            DTask& nt = queue[ns2 & qmask];
                             if (lane < ed) {
                                 const bool member = ed > t && lane == ed - 1;
                                 nt.prefix[lane] =
@@ -2091,7 +2159,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                             __threadfence();
                             __syncthreads();
                             if (lane == 0) {
-                                __hip_atomic_store(ready + ns2, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                                seq_publish(seq, qmask, ns2);
                                 S.gen_end[ed] = mid;
                             }
                             __syncthreads();
@@ -2106,10 +2174,13 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             if (lcount | lcalls) {
                 const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
                 atomicAdd(&qc[q].calls, lcalls);
-                if (lcount && prev + lcount >= P.cap) atomicExch(&qc[q].done, 1u);
+                if (lcount && prev + lcount >= P.cap) {
+                    atomicExch(&qc[q].done, 1u);
+                    mark_final(qc, q);
+                }
             }
             if (lrej) atomicAdd(&qc[q].rejects, lrej);
-            if (finished) atomicSub(pending + q, 1u);
+            if (finished) pending_sub(pending, qc, q, 1u);
             atomicSub(&Q->busy, 1u);
         }
         wave_add(&qc[q].probe_bytes, bytes);
@@ -2258,6 +2329,11 @@ __global__ __launch_bounds__(kBlock) void k_join_probe_emit(
             }
         }
     }
+}
+
+__global__ void k_ring_init(uint32_t* seq, uint32_t slots, uint32_t n_tasks) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * blockDim.x)
+        seq[i] = (uint32_t)i + (i < n_tasks ? 1u : 0u);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -2628,7 +2704,7 @@ uint32_t dfs_lanes_per_block(uint32_t max_n) {
 size_t dfs_lane_lds_bytes(uint32_t max_n) { return lane_lds_bytes(dfs_lanes_per_block(max_n), max_n); }
 
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                      const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
+                      const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* seq,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                       EmitSink sink, hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
@@ -2649,7 +2725,7 @@ hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_of
         cus > 0)
         blocks = std::min<uint32_t>(blocks, (uint32_t)(per_cu * cus));
     hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(lanes), lds, s, plans, ops, row_off, idx, cand_pool, queue,
-                       ready, dq, qc, pending, max_n, sink);
+                       seq, dq, qc, pending, max_n, sink);
     return hipGetLastError();
 }
 
@@ -2658,20 +2734,20 @@ size_t dfs_wave_lds_bytes(uint32_t max_n) {
 }
 
 hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                           const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
+                           const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* seq,
                            DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                            hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
     if (max_n == 0 || max_n > kMaxQ) return hipErrorInvalidValue;
     if (max_n <= 16)
         hipLaunchKernelGGL(k_dfs_wave<16>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, queue, ready, dq, qc, pending);
+                           cand_pool, queue, seq, dq, qc, pending);
     else if (max_n <= 32)
         hipLaunchKernelGGL(k_dfs_wave<32>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, queue, ready, dq, qc, pending);
+                           cand_pool, queue, seq, dq, qc, pending);
     else
         hipLaunchKernelGGL(k_dfs_wave<64>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, queue, ready, dq, qc, pending);
+                           cand_pool, queue, seq, dq, qc, pending);
     return hipGetLastError();
 }
 
@@ -2705,6 +2781,12 @@ hipError_t launch_join_probe_emit(const uint32_t* build_rows, uint32_t build_wid
     hipLaunchKernelGGL(k_join_probe_emit, dim3(grid_for(n_probe)), dim3(kBlock), 0, s, build_rows, build_width,
                        build_keys, probe_rows, n_probe, probe_width, probe_keys, fp, head, mask, next, out, cap,
                        count, overflow);
+    return hipGetLastError();
+}
+
+hipError_t launch_ring_init(uint32_t* seq, uint32_t slots, uint32_t n_tasks, hipStream_t s) {
+    if (slots == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ring_init, dim3(grid_for(slots)), dim3(kBlock), 0, s, seq, slots, n_tasks);
     return hipGetLastError();
 }
 

@@ -2,8 +2,8 @@
 //
 // Data graphs: Chung-Lu power law and R-MAT, with Zipf or uniform labels.
 // Queries: connected vertex sets grown from a random start by random walk
-// ("sparse": the walk's tree edges) or by greedy densest growth ("dense": the
-// induced subgraph), keeping the data vertices' labels, so every query has at
+// ("sparse": the walk's tree edges), by a degree-biased random walk or greedy
+// densest growth ("dense": the induced subgraph), keeping the data vertices' labels, so every query has at
 // least one embedding. Everything is deterministic for a given seed
 // (xoshiro256** seeded through splitmix64).
 
@@ -171,21 +171,48 @@ int psm_gen_labels(uint32_t V, uint32_t L, double zipf_s, uint64_t seed, uint32_
 //            the current set (random tie-break), edges = induced subgraph.
 // dense = 2: random walk, edges = induced subgraph of the walked vertices
 //            (the walk tree plus every data edge among them: cycles).
+// dense = 3: degree-biased random walk (each step moves to a neighbour w with
+//            probability deg(w) / sum of the neighbours' degrees, from the
+//            endpoint of a random edge), edges = induced subgraph. The walk
+//            stays among high-degree vertices, whose induced subgraphs are
+//            dense on power-law graphs where a uniform walk's are trees.
 // Output: vertex ids (data ids, query vertex i = out_vertices[i]) and the
 // query's undirected edges as pairs of query-local ids. Returns the number of
 // edges, or a negative PSM status.
 int64_t psm_gen_query(uint32_t V, const uint32_t* offsets, const uint32_t* nbrs,
                       uint32_t n, int dense, uint64_t seed,
                       uint32_t* out_vertices, uint32_t* out_edges, uint64_t max_edges) {
-    if (n == 0 || n > 64 || V == 0 || dense < 0 || dense > 2) return -PSM_EINVAL;
-    const bool walk = dense != 1, induced = dense != 0;
+    if (n == 0 || n > 64 || V == 0 || dense < 0 || dense > 3) return -PSM_EINVAL;
+    const bool walk = dense != 1, induced = dense != 0, biased = dense == 3;
     Rng r(seed);
     for (int attempt = 0; attempt < 1000; ++attempt) {
         uint32_t start = (uint32_t)r.below(V);
         if (offsets[start + 1] == offsets[start]) continue;
         std::vector<uint32_t> verts{start};
         std::vector<std::pair<uint32_t, uint32_t>> tree;
-        if (walk) {
+        if (biased) {
+            const uint64_t nnz = offsets[V];
+            const uint64_t e = r.below(nnz);
+            uint32_t cur = (uint32_t)(std::upper_bound(offsets, offsets + V + 1, (uint32_t)e) - offsets - 1);
+            verts[0] = cur;
+            std::vector<uint64_t> cum;
+            uint64_t steps = 0;
+            while (verts.size() < n && steps++ < 100000) {
+                const uint32_t d = offsets[cur + 1] - offsets[cur];
+                cum.resize(d);
+                uint64_t acc = 0;
+                for (uint32_t k = 0; k < d; ++k) {
+                    const uint32_t w = nbrs[offsets[cur] + k];
+                    acc += offsets[w + 1] - offsets[w];
+                    cum[k] = acc;
+                }
+                const uint64_t x = r.below(acc);
+                const uint32_t k = (uint32_t)(std::upper_bound(cum.begin(), cum.end(), x) - cum.begin());
+                const uint32_t w = nbrs[offsets[cur] + k];
+                if (std::find(verts.begin(), verts.end(), w) == verts.end()) verts.push_back(w);
+                cur = w;
+            }
+        } else if (walk) {
             uint32_t cur = 0;  // index into verts
             uint64_t steps = 0;
             while (verts.size() < n && steps++ < 100000) {

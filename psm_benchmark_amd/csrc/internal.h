@@ -121,19 +121,29 @@ struct DTask {
     uint32_t prefix[kPrefixMax];
 };
 
-// Work queue of the persistent DFS: slots [0, tail) hold tasks, claimed in
-// order through head; a slot's ready flag is set once its record is written.
+// Work queue of the persistent DFS: a ring of `mask + 1` task records (a
+// power of two) addressed by 32-bit positions; position p lives in slot
+// p & mask. Positions [head, tail) are queued. seq[slot] is the slot's
+// sequence word: p + 1 once the record of position p is written (claimable),
+// p + mask + 1 once its claimant has read it (free for position
+// p + mask + 1). A producer takes position tail only when its slot is free,
+// so slots are reused for as long as the launch runs and donation never
+// stops at a fixed number of splits. Positions wrap modulo 2^32; only
+// differences of positions are compared.
 struct DQueue {
     unsigned int head;
     unsigned int tail;
     unsigned int busy;         // lanes holding a task
     unsigned int timeout;      // the time limit passed
-    unsigned int cap;          // slots allocated
+    unsigned int mask;         // ring slots - 1
     unsigned int donated;      // tasks split off by running lanes
     unsigned long long limit_ticks;   // wall-clock ticks allowed (0: no limit)
     unsigned long long deadline;      // set by the first lane to start
     unsigned long long lane_steps;    // loop iterations in which a lane held a task
     unsigned long long lane_iters;    // loop iterations of all lanes
+    unsigned long long start_tick;    // wall clock when the first block started
+    unsigned int full;         // donations skipped because the next slot was still held
+    unsigned int pad;
 };
 
 // Where the DFS writes complete embeddings (rows of n data vertex ids in
@@ -171,6 +181,9 @@ struct QCounters {
     unsigned long long expand_algo_bytes;  // nodes of task pre-expansion
     uint32_t done;        // cap reached
     uint32_t timed_out;   // work was dropped at the time limit
+    // wall clock (wall_clock64 ticks) when the query's count became final:
+    // its cap was reached or its last task ended; 0 while it runs
+    unsigned long long done_tick;
 };
 
 // Kernel classes timed by the library (psm_kernel_stats).
@@ -257,7 +270,7 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
                          const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
                          unsigned long long* qchild, int write, hipStream_t s);
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                      const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
+                      const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* seq,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                       EmitSink sink, hipStream_t s);
 hipError_t launch_join_build(const uint32_t* rows, uint64_t n_rows, uint32_t width, JoinKeys keys,
@@ -274,10 +287,13 @@ hipError_t launch_join_probe(const uint32_t* build_rows, uint32_t build_width, J
                              uint64_t mask, const uint32_t* next, unsigned long long* count,
                              unsigned long long* bytes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+// ring sequence words: seq[i] = i + 1 for the n_tasks queued positions, i for
+// the free slots
+hipError_t launch_ring_init(uint32_t* seq, uint32_t slots, uint32_t n_tasks, hipStream_t s);
 size_t dfs_lane_lds_bytes(uint32_t max_n);
 size_t dfs_wave_lds_bytes(uint32_t max_n);
 hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                           const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* ready,
+                           const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* seq,
                            DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                            hipStream_t s);
 uint64_t scan_tmp_entries(uint64_t n);

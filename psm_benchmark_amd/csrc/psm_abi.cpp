@@ -59,6 +59,7 @@ struct psm_ctx {
     uint32_t dfs_waves_per_cu = kDefaultWavesPerCu;        // persistent DFS grid
     double expand_tasks_per_lane = kDefaultTasksPerLane;   // pre-expansion target
     uint32_t dfs_strategy = PSM_DFS_LANE;     // lane-per-task or wavefront-per-task DFS
+    uint32_t queue_slots = 0;                 // free ring slots beyond the expanded tasks (0: automatic)
     double wall_khz = 100000.0;               // device wall clock rate
     bool verbose = false;
 };
@@ -377,6 +378,7 @@ struct QueryHost {
     uint64_t count = 0, calls = 0, rejects = 0, probe_bytes = 0, algo_bytes = 0;
     bool capped = false;
     bool timed_out = false;           // partial count: the reference's "Overtime: 1"
+    double done_ms = -1;              // host clock (now_ms) when the count became final
 };
 
 constexpr uint64_t kNoTable = ~0ull;
@@ -417,6 +419,10 @@ struct Batch {
     double t_begin = -1;   // start of filtering: the time limit counts from here
     double enum_limit_s = -1;   // >= 0: an enumeration time limit counted from t_eval instead
     double t_eval = 0;
+    // the last DFS launch: host clock just before it and the device wall
+    // clock when its first block started (maps done_tick to host time)
+    double dfs_launch_ms = 0;
+    unsigned long long dfs_start_tick = 0;
 
     hipStream_t s() const { return ctx->stream; }
 
@@ -1363,17 +1369,22 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
     unsigned int donated = 0;
     timed_out = false;
     if (n_tasks) {
-        // queue: the expanded tasks followed by room for split-off work
-        const uint64_t cap_slots = std::min<uint64_t>(n_tasks + (4u << 20), 0xFFFFFFF0ull);
-        DBuf d_queue, d_ready, d_q, d_state, d_pending;
-        HIP_TRY(ctx, d_queue.alloc(cap_slots * sizeof(DTask)));
+        // queue: a ring holding the expanded tasks and room for split-off
+        // work; slots are reused, so the room bounds only the tasks queued at
+        // once (donation runs while the queue is shorter than the idle lanes)
+        const uint64_t lanes_all = (uint64_t)grid_waves_full * 64;
+        const uint64_t room = ctx->queue_slots ? ctx->queue_slots : std::max<uint64_t>(1u << 18, 4 * lanes_all);
+        uint64_t ring = 64;
+        while (ring < n_tasks + room) ring <<= 1;
+        if (ring > (1ull << 31)) return set_err(ctx, PSM_ENOMEM, "task queue beyond 2^31 slots");
+        DBuf d_queue, d_seq, d_q, d_state, d_pending;
+        HIP_TRY(ctx, d_queue.alloc(ring * sizeof(DTask)));
         HIP_TRY(ctx, hipMemcpyAsync(d_queue.p, d_tasks.p, n_tasks * sizeof(DTask), hipMemcpyDeviceToDevice, s()));
-        HIP_TRY(ctx, d_ready.alloc(cap_slots * 4));
-        HIP_TRY(ctx, hipMemsetAsync(d_ready.p, 0, cap_slots * 4, s()));
-        HIP_TRY(ctx, launch_fill_u32(d_ready.as<uint32_t>(), 1u, n_tasks, s()));
+        HIP_TRY(ctx, d_seq.alloc(ring * 4));
+        HIP_TRY(ctx, launch_ring_init(d_seq.as<uint32_t>(), (uint32_t)ring, (uint32_t)n_tasks, s()));
         DQueue hq{};
         hq.tail = (unsigned int)n_tasks;
-        hq.cap = (unsigned int)cap_slots;
+        hq.mask = (unsigned int)(ring - 1);
         if (enum_limit_s > 0) {
             const double left_s = enum_limit_s - (now_ms() - t_eval) / 1000.0;
             hq.limit_ticks = (unsigned long long)std::max(1.0, left_s * ctx->wall_khz * 1000.0);
@@ -1390,16 +1401,17 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         for (uint32_t i = 0; i < nq; ++i) pend[i] = (uint32_t)ntask[i];
         HIP_TRY(ctx, upload(d_pending, pend, s()));
         const double td0 = now_ms();
+        dfs_launch_ms = td0;
         if (wave_dfs) {
             TIMED(ctx, kKDfs,
                   launch_dfs_wave(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
-                                  cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_ready.as<uint32_t>(),
+                                  cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_seq.as<uint32_t>(),
                                   d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
                                   s()));
         } else {
             TIMED(ctx, kKDfs,
                   launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
-                             cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_ready.as<uint32_t>(),
+                             cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_seq.as<uint32_t>(),
                              d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
                              sink, s()));
         }
@@ -1408,10 +1420,12 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         t_enum += now_ms() - td0;
         timed_out = hq.timeout != 0;
         donated = hq.donated;
+        dfs_start_tick = hq.start_tick;
         lane_steps += hq.lane_steps;
         lane_iters += hq.lane_iters;
-        verbose("dfs: %.1f ms, %u tasks split off, timeout %u, lanes busy %.3f", now_ms() - td0, donated,
-                hq.timeout, hq.lane_iters ? (double)hq.lane_steps / (double)hq.lane_iters : 0.0);
+        verbose("dfs: %.1f ms, %u tasks split off (ring %llu slots, %u skipped: slot held), timeout %u, "
+                "lanes busy %.3f", now_ms() - td0, donated, (unsigned long long)ring, hq.full, hq.timeout,
+                hq.lane_iters ? (double)hq.lane_steps / (double)hq.lane_iters : 0.0);
         // a query with queued or unfinished tasks and no cap reached is incomplete
         std::vector<uint32_t> left;
         HIP_TRY(ctx, download(left, d_pending.p, nq, s()));
@@ -1445,9 +1459,15 @@ int Batch::evaluate(uint64_t cap, double enum_limit) {
     for (size_t i = 0; i < qs.size(); ++i) make_plan(*qs[i], qs[i]->plan.order, nullptr, cap, hp[i], hops);
     std::vector<QCounters> qc;
     bool timed_out = false;
+    dfs_start_tick = 0;
     PSM_TRY(run_plans(hp, hops, EmitSink{nullptr, nullptr, 0, nullptr}, qc, timed_out));
+    const double t_end = now_ms();
     for (size_t i = 0; i < qs.size(); ++i) {
         QueryHost* q = qs[i];
+        // the device clock runs at wall_khz ticks per millisecond
+        q->done_ms = qc[i].done_tick && dfs_start_tick && qc[i].done_tick >= dfs_start_tick
+                         ? std::min(t_end, dfs_launch_ms + (double)(qc[i].done_tick - dfs_start_tick) / ctx->wall_khz)
+                         : t_end;
         q->count = cap ? std::min<uint64_t>(cap, qc[i].count) : qc[i].count;
         q->capped = cap && qc[i].count >= cap;
         q->calls = qc[i].calls;
@@ -1676,6 +1696,11 @@ int psm_set_tuning(psm_ctx* ctx, int key, double value) {
                 return set_err(ctx, PSM_EINVAL, "dfs strategy must be PSM_DFS_LANE or PSM_DFS_WAVE");
             ctx->dfs_strategy = (uint32_t)value;
             return PSM_OK;
+        case PSM_TUNE_QUEUE_SLOTS:
+            if (!(value >= 0 && value <= (double)(1u << 28)))
+                return set_err(ctx, PSM_EINVAL, "queue slots outside [0, 2^28]");
+            ctx->queue_slots = (uint32_t)value;
+            return PSM_OK;
         default:
             return set_err(ctx, PSM_EINVAL, "unknown tuning key");
     }
@@ -1687,6 +1712,7 @@ int psm_get_tuning(psm_ctx* ctx, int key, double* value) {
         case PSM_TUNE_DFS_WAVES_PER_CU: *value = ctx->dfs_waves_per_cu; return PSM_OK;
         case PSM_TUNE_EXPAND_TASKS_PER_LANE: *value = ctx->expand_tasks_per_lane; return PSM_OK;
         case PSM_TUNE_DFS_STRATEGY: *value = ctx->dfs_strategy; return PSM_OK;
+        case PSM_TUNE_QUEUE_SLOTS: *value = ctx->queue_slots; return PSM_OK;
         default: return set_err(ctx, PSM_EINVAL, "unknown tuning key");
     }
 }
@@ -2220,6 +2246,15 @@ int psm_batch_evaluate(psm_batch* B, uint64_t cap, double enum_time_limit_s, uin
             st->probe_bytes += h->probe_bytes;
             st->algo_bytes += h->algo_bytes;
         }
+    }
+    return PSM_OK;
+}
+
+int psm_batch_latencies(const psm_batch* B, double* ms) {
+    if (!B || !ms) return PSM_EINVAL;
+    for (size_t i = 0; i < B->hosts.size(); ++i) {
+        const QueryHost* h = B->hosts[i].get();
+        ms[i] = h->done_ms >= 0 ? h->done_ms - B->t0 : -1.0;
     }
     return PSM_OK;
 }

@@ -311,6 +311,14 @@ class Batch:
                "psm_batch_evaluate")
         return counts[: self.m], calls[: self.m], status[: self.m], st
 
+    def latencies(self) -> np.ndarray:
+        """Per query: ms from the batch's creation until its count was final
+        in the last evaluate."""
+        ms = np.zeros(max(1, self.m), np.float64)
+        _check(self.dg.dev.h, L.lib().psm_batch_latencies(self.h, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+               "psm_batch_latencies")
+        return ms[: self.m]
+
     def close(self):
         if getattr(self, "h", None):
             L.lib().psm_batch_destroy(self.h)

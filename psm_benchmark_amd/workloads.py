@@ -3,7 +3,8 @@
 C1  plain:    Chung-Lu 9,460 V / 34,998 E, 307 Zipf(1.1) labels; one 8-vertex
               random-walk query; GQL filter + GQL order; uncapped.
 C2  batched:  Chung-Lu 1.1M V / 3.0M E, 100 Zipf(1.1) labels; 512 queries of
-              4/8/16/24/32 vertices, half dense (greedy induced subgraph) and
+              4/8/16/24/32 vertices, half dense (induced subgraph of a
+              degree-biased walk, average degree > 3; K4 at 4 vertices) and
               half sparse (walk tree); GQL filter + GQL order; cap 1e5.
 C3  join:     Chung-Lu 4M V / 35M E, 32 Zipf(1.0) labels; 64 cyclic queries of
               12-16 vertices; CFL filter + CFL order; uncapped.
@@ -37,10 +38,20 @@ def _avg_degree(q: CSRGraph) -> float:
     return 2.0 * q.num_edges / q.num_vertices
 
 
+def is_dense(q: CSRGraph) -> bool:
+    """The dense half's rule: average degree above 3; a 4-vertex query can
+    reach 3 at most (K4), so there dense means K4."""
+    n = q.num_vertices
+    return q.num_edges == 6 if n == 4 else 2.0 * q.num_edges > 3.0 * n
+
+
 def c2_queries(g: CSRGraph, count: int = 512, seed: int = 22):
-    """Sizes cycle through 4/8/16/24/32; even index dense, odd sparse.
-    Dense queries take the densest of a few greedy draws (a 4-vertex query
-    can reach average degree 3 at most: K4)."""
+    """Random-walk queries; sizes cycle through 4/8/16/24/32, even index
+    dense, odd sparse. Sparse: the tree of a uniform random walk (average
+    degree 2(n-1)/n <= 3). Dense: the induced subgraph of a degree-biased
+    random walk (gen.query mode 3), redrawn until it meets is_dense. On a
+    power-law graph a uniform walk's induced subgraph is nearly a tree, so
+    the dense half walks towards high-degree vertices instead."""
     sizes = [4, 8, 16, 24, 32]
     qs, meta = [], []
     rng = np.random.default_rng(seed)
@@ -49,14 +60,12 @@ def c2_queries(g: CSRGraph, count: int = 512, seed: int = 22):
         dense = i % 2 == 0
         base = int(rng.integers(0, 2**62))
         if dense:
-            best = None
-            for t in range(8):
-                q = gen.query(g, n, True, base + t)
-                if best is None or _avg_degree(q) > _avg_degree(best):
-                    best = q
-                if _avg_degree(best) > 3.0 or (n == 4 and best.num_edges == 6):
+            for t in range(1 << 16):
+                q = gen.query(g, n, 3, base + t)
+                if is_dense(q):
                     break
-            q = best
+            else:
+                raise RuntimeError(f"no dense {n}-vertex walk in 65536 draws")
         else:
             q = gen.query(g, n, False, base)
         qs.append(q)

@@ -31,9 +31,22 @@ class OrcResult(ctypes.Structure):
     ]
 
 
+class OrcTimes(ctypes.Structure):
+    _fields_ = [
+        ("filter_s", ctypes.c_double),
+        ("build_s", ctypes.c_double),
+        ("plan_s", ctypes.c_double),
+        ("enum_s", ctypes.c_double),
+        ("total_s", ctypes.c_double),
+        ("threads", ctypes.c_uint32),
+        ("timed_out", ctypes.c_uint32),
+    ]
+
+
 def build(force: bool = False) -> str:
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC_PATH):
-        subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-o", LIB_PATH, SRC_PATH])
+        subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-o", LIB_PATH, SRC_PATH,
+                               "-lpthread"])
     return LIB_PATH
 
 
@@ -52,6 +65,10 @@ def lib():
         l.orc_run.restype = P
         l.orc_run.argtypes = [P, P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                               ctypes.c_uint64, ctypes.POINTER(OrcResult)]
+        l.orc_run_mt.restype = P
+        l.orc_run_mt.argtypes = [P, P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                 ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double,
+                                 ctypes.POINTER(OrcResult), ctypes.POINTER(OrcTimes)]
         l.orc_run_free.argtypes = [P]
         l.orc_run_cands.argtypes = [P, ctypes.c_uint32, U32P]
         l.orc_run_table.restype = ctypes.c_int64
@@ -85,11 +102,15 @@ class OGraph:
 class ORun:
     """One plain-path run: filter -> sort -> BuildTable -> order -> enumerate."""
 
-    def __init__(self, data: OGraph, query: OGraph, method: str = "GQL_GQL_GQL", cap: int = 0):
+    def __init__(self, data: OGraph, query: OGraph, method: str = "GQL_GQL_GQL", cap: int = 0,
+                 threads: int = 1, time_limit: float = 0.0):
+        """threads: enumeration workers (0: every hardware thread);
+        time_limit > 0: seconds from the start of filtering (timed_out)."""
         f, o, e = method.split("_")
         self.res = OrcResult()
-        self.h = lib().orc_run(data.h, query.h, f.encode(), o.encode(), e.encode(), cap,
-                               ctypes.byref(self.res))
+        self.times = OrcTimes()
+        self.h = lib().orc_run_mt(data.h, query.h, f.encode(), o.encode(), e.encode(), cap, threads,
+                                  time_limit, ctypes.byref(self.res), ctypes.byref(self.times))
         if not self.h:
             raise ValueError(f"oracle rejected method {method}")
         self.n = self.res.n
@@ -107,6 +128,7 @@ class ORun:
     cand_counts = property(lambda s: [int(s.res.cand_count[i]) for i in range(s.n)])
     depth_nodes = property(lambda s: [int(s.res.depth_nodes[i]) for i in range(s.n)])
     algo_bytes = property(lambda s: int(s.res.algo_bytes))
+    timed_out = property(lambda s: bool(s.times.timed_out))
 
     def candidates(self, u: int) -> np.ndarray:
         out = np.zeros(max(1, self.res.cand_count[u]), np.uint32)

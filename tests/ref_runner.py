@@ -2,7 +2,7 @@
 recipe oracle/Makefile.ref) on graph files and parses its log.
 
 Test infrastructure only: a checker for the oracle on synthetic inputs and the
-"reference" CPU baseline of bench.py. Absent binary -> available() is False.
+one-time calibration of the oracle restatement. Absent binary -> available() is False.
 """
 from __future__ import annotations
 

@@ -130,6 +130,40 @@ def test_cap_is_min_cap_total(hprd):
         assert ORun(hprd, q, "GQL_GQL_GQL", cap=cap).count == min(cap, total)
 
 
+@pytest.mark.parametrize("qname", ["Q10-1", "Q10-5", "Q10-10"])
+@pytest.mark.parametrize("method", ["GQL_GQL_GQL", "CFL_CFL_EXPLORE", "DPiso_DPiso_LFTJ"])
+def test_threaded_enumeration_matches_golden(hprd, qname, method):
+    # the threaded leg (root candidates claimed one at a time by workers)
+    # walks the same search tree: uncapped counts, call counts and search
+    # bytes equal the one-thread golden values at every thread count
+    f, o, e = method.split("_")
+    rec = next(r for r in golden_records((method,)) if r["query"] == qname)
+    q = OGraph(load_text(os.path.join(HPRD, qname + ".txt")))
+    one = ORun(hprd, q, method)
+    for t in (2, 5, 8):
+        r = ORun(hprd, q, method, threads=t)
+        assert (r.count, r.calls) == (rec["embeddings"], rec["calls"]), t
+        assert r.algo_bytes == one.algo_bytes and r.depth_nodes == one.depth_nodes, t
+        assert r.times.threads == t and not r.timed_out
+
+
+def test_threaded_cap_is_min_cap_total(hprd):
+    q = OGraph(load_text(os.path.join(HPRD, "Q10-8.txt")))
+    total = ORun(hprd, q, "GQL_GQL_GQL").count
+    for cap in (1, 300, 1000, 100000, total, total + 5):
+        for t in (1, 4, 8):
+            assert ORun(hprd, q, "GQL_GQL_GQL", cap=cap, threads=t).count == min(cap, total), (cap, t)
+
+
+def test_time_limit_reports_timeout(hprd):
+    q = OGraph(load_text(os.path.join(HPRD, "Q10-10.txt")))
+    for t in (1, 4):
+        r = ORun(hprd, q, "GQL_GQL_GQL", threads=t, time_limit=1e-4)
+        assert r.timed_out and r.count < 1015922, t
+        full = ORun(hprd, q, "GQL_GQL_GQL", threads=t, time_limit=600.0)
+        assert not full.timed_out and full.count == 1015922, t
+
+
 def _nlf(g, v):
     off = g.offsets.astype(np.int64)
     labs = g.labels[g.neighbors[off[v]:off[v + 1]]]
