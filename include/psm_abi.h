@@ -178,6 +178,20 @@ int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t*
  * limit (psm_set_time_limit) stops it. */
 int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* stats);
 
+/* psm_evaluate without device memory allocation: no counterpart in the
+ * reference. For callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * counts, call counts, visited rejects and search bytes as psm_evaluate
+ * (calls exact when uncapped); calls no hipMalloc / hipFree. Its plan,
+ * operands, counters and task ring live in a fixed device scratch of about
+ * 11 MB per GPU (allocated with the code object, held until the process
+ * ends): at most 2048 intersection operands (every query within
+ * PSM_MAX_QUERY_VERTICES fits; PSM_EINVAL beyond), 4096 root tasks and 65536
+ * ring slots. Every call on one GPU uses that GPU's one scratch: calls that
+ * overlap on one GPU (from any context) race and must be serialised by the
+ * caller. Needs psm_filter, psm_build_tables and psm_generate_plan first. */
+int psm_evaluate_noalloc(psm_query* q, uint64_t cap, psm_enum_stats* stats);
+
 /* Query decomposition + join: JoinCollect::sequence_collect over two units
  * (JoinCollect.cpp:60-153,237-315; NoPartitionHashJoin hashjoin.cpp:337-781).
  * Unit A is the first ceil(n/2) vertices of the generated plan's order, unit

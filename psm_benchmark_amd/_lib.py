@@ -125,6 +125,7 @@ SIGNATURES = {
     "psm_memory_cost_bytes": (ctypes.c_int, [VP, U64P]),
     "psm_generate_plan": (ctypes.c_int, [VP, ctypes.c_int, U32P, U32P]),
     "psm_evaluate": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
+    "psm_evaluate_noalloc": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
     "psm_join_count": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(JoinStats)]),
     "psm_split_join_count": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(SplitJoinStats)]),
     "psm_match": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),

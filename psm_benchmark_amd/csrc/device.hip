@@ -13,6 +13,7 @@
 //                lane-per-task DFS fed by a global atomic work queue
 
 #include <algorithm>
+#include <mutex>
 
 #include "internal.h"
 
@@ -2331,6 +2332,22 @@ __global__ __launch_bounds__(kBlock) void k_join_probe_emit(
     }
 }
 
+// Root range tasks of query q: task i enumerates positions
+// [i * len / n, (i + 1) * len / n) of C(order[0]) at depth 0.
+__global__ void k_root_tasks(DTask* ring, uint32_t n, uint32_t q, uint64_t len) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DTask t;
+    t.q = q;
+    t.depth = 0;
+    t.sel = kIdentity;
+    t.pad = 0;
+    t.lo = (uint64_t)i * len / n;
+    t.hi = (uint64_t)(i + 1) * len / n;
+    for (uint32_t e = 0; e < kPrefixMax; ++e) t.prefix[e] = 0;
+    ring[i] = t;
+}
+
 __global__ void k_ring_init(uint32_t* seq, uint32_t slots, uint32_t n_tasks) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * blockDim.x)
         seq[i] = (uint32_t)i + (i < n_tasks ? 1u : 0u);
@@ -2784,6 +2801,12 @@ hipError_t launch_join_probe_emit(const uint32_t* build_rows, uint32_t build_wid
     return hipGetLastError();
 }
 
+hipError_t launch_root_tasks(DTask* ring, uint32_t n, uint32_t q, uint64_t len, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_root_tasks, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ring, n, q, len);
+    return hipGetLastError();
+}
+
 hipError_t launch_ring_init(uint32_t* seq, uint32_t slots, uint32_t n_tasks, hipStream_t s) {
     if (slots == 0) return hipSuccess;
     hipLaunchKernelGGL(k_ring_init, dim3(grid_for(slots)), dim3(kBlock), 0, s, seq, slots, n_tasks);
@@ -2794,6 +2817,39 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(kBlock), 0, s, p, v, n);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fixed device scratch of the allocation-free entry point
+// (psm_evaluate_noalloc): one array per GPU the library runs on, held until
+// the process ends. Host code never passes `scratch` itself to a kernel (on
+// the host its name is a stand-in, not a device address): psm_ctx_create
+// resolves the array's address on each device once, with that device
+// current, and every later use goes through that address.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kScratchMaxDevices] = {};
+}  // namespace
+
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= (int)kScratchMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_addr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_addr[device] = ptr;
+    return hipSuccess;
+}
+
+void* scratch_address(int device) {
+    if (device < 0 || device >= (int)kScratchMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_addr[device];
 }
 
 }  // namespace psm

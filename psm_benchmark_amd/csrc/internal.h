@@ -287,6 +287,9 @@ hipError_t launch_join_probe(const uint32_t* build_rows, uint32_t build_width, J
                              uint64_t mask, const uint32_t* next, unsigned long long* count,
                              unsigned long long* bytes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+// n root range tasks of query q splitting C(order[0]) (len entries) evenly,
+// written to ring[0, n)
+hipError_t launch_root_tasks(DTask* ring, uint32_t n, uint32_t q, uint64_t len, hipStream_t s);
 // ring sequence words: seq[i] = i + 1 for the n_tasks queued positions, i for
 // the free slots
 hipError_t launch_ring_init(uint32_t* seq, uint32_t slots, uint32_t n_tasks, hipStream_t s);
@@ -297,6 +300,31 @@ hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* r
                            DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                            hipStream_t s);
 uint64_t scan_tmp_entries(uint64_t n);
+
+// ---- fixed device scratch of the allocation-free enumeration -------------
+// Layout (byte offsets, 256-aligned): queue header, query counters, pending
+// count, one plan, the operands, the ring's sequence words and its records.
+constexpr uint32_t kScratchOps = 2048;       // >= the edges of a 64-vertex query
+constexpr uint32_t kScratchSlots = 1u << 16;  // task ring slots
+constexpr uint32_t kScratchRootTasks = 4096;  // root range tasks at most
+constexpr size_t kScratchAlign(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kScrQueue = 0;
+constexpr size_t kScrCounters = kScrQueue + kScratchAlign(sizeof(DQueue));
+constexpr size_t kScrPending = kScrCounters + kScratchAlign(sizeof(QCounters));
+constexpr size_t kScrZeroEnd = kScrPending + 256;   // [0, kScrZeroEnd) is cleared per call
+constexpr size_t kScrPlan = kScrZeroEnd;
+constexpr size_t kScrOps = kScrPlan + kScratchAlign(sizeof(DPlan));
+constexpr size_t kScrSeq = kScrOps + kScratchAlign(sizeof(DOp) * kScratchOps);
+constexpr size_t kScrRing = kScrSeq + kScratchAlign(4ull * kScratchSlots);
+constexpr size_t kScratchBytes = kScrRing + kScratchAlign(sizeof(DTask) * kScratchSlots);
+constexpr uint32_t kScratchMaxDevices = 64;
+static_assert(kScratchRootTasks <= kScratchSlots / 2, "root tasks leave room for split-off work");
+static_assert(kScratchBytes <= (12u << 20), "the scratch stays modest");
+// resolves the scratch's address on `device` (which must be current); once
+// per device, thread-safe
+hipError_t scratch_setup(int device);
+// the address scratch_setup resolved on `device`, nullptr before
+void* scratch_address(int device);
 hipError_t launch_scan_u32(const uint32_t* in, uint64_t n, uint64_t* out, unsigned long long* tmp,
                            hipStream_t s);
 hipError_t launch_scan_u64(const uint64_t* in, uint64_t n, uint64_t* out, unsigned long long* tmp,

@@ -1634,6 +1634,9 @@ int psm_ctx_create(int device, psm_ctx** out) {
     if (device < 0 || device >= n) return PSM_EINVAL;
     e = hipSetDevice(device);
     if (e != hipSuccess) return -(int)e;
+    // the allocation-free entry point's scratch: its address on this device
+    e = scratch_setup(device);
+    if (e != hipSuccess) return -(int)e;
     psm_ctx* c = new psm_ctx();
     c->device = device;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -2131,6 +2134,109 @@ int psm_evaluate(psm_query* q, uint64_t cap, psm_enum_stats* st) {
         st->visited_rejects = q->qh.rejects;
     }
     return r;
+}
+
+// psm_evaluate through the fixed device scratch (device.hip): no device
+// memory is allocated or freed. The plan, operands, counters, task ring and
+// root tasks live in the scratch of the context's GPU; the query's
+// candidates and tables (its inputs, built by psm_filter / psm_build_tables)
+// are read where they are. Root range tasks replace the pre-expansion
+// kernels (which size their buffers per round); the persistent DFS splits
+// work off through the ring as psm_evaluate's does.
+int psm_evaluate_noalloc(psm_query* q, uint64_t cap, psm_enum_stats* st) {
+    if (!q) return PSM_EINVAL;
+    psm_ctx* ctx = q->ctx;
+    Batch& b = q->batch;
+    QueryHost& h = q->qh;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (!h.planned) return set_err(ctx, PSM_ESTATE, "generate a plan before evaluating");
+    unsigned char* scr = static_cast<unsigned char*>(scratch_address(ctx->device));
+    if (!scr) return set_err(ctx, PSM_ESTATE, "device scratch not set up on this device");
+    const double t0 = now_ms();
+    DPlan P;
+    std::vector<DOp> hops;
+    hops.reserve(kScratchOps);
+    make_plan(h, h.plan.order, nullptr, cap, P, hops);
+    if (hops.size() > kScratchOps)
+        return set_err(ctx, PSM_EINVAL, "query needs " + std::to_string(hops.size()) +
+                                            " intersection operands; the device scratch holds " +
+                                            std::to_string(kScratchOps));
+    if (P.n == 0 || P.n > kMaxQ) return set_err(ctx, PSM_EINVAL, "query size outside the device scratch");
+    if (hops.empty()) hops.push_back(DOp{});
+    const uint64_t roots = P.cand_len[0];
+    h.count = h.calls = h.rejects = h.probe_bytes = h.algo_bytes = 0;
+    h.capped = h.timed_out = false;
+    if (roots == 0) {
+        if (st) memset(st, 0, sizeof(*st));
+        return PSM_OK;
+    }
+    const uint32_t n_tasks = (uint32_t)std::min<uint64_t>(roots, kScratchRootTasks);
+    DQueue* dq = reinterpret_cast<DQueue*>(scr + kScrQueue);
+    QCounters* dqc = reinterpret_cast<QCounters*>(scr + kScrCounters);
+    uint32_t* dpend = reinterpret_cast<uint32_t*>(scr + kScrPending);
+    DPlan* dplan = reinterpret_cast<DPlan*>(scr + kScrPlan);
+    DOp* dops = reinterpret_cast<DOp*>(scr + kScrOps);
+    uint32_t* dseq = reinterpret_cast<uint32_t*>(scr + kScrSeq);
+    DTask* dring = reinterpret_cast<DTask*>(scr + kScrRing);
+    // the header, counters and pending count start at zero (a fill kernel,
+    // no memset node); the scratch keeps the last call's contents otherwise
+    HIP_TRY(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(scr), 0u, kScrZeroEnd / 4, ctx->stream));
+    DQueue hq{};
+    hq.tail = n_tasks;
+    hq.mask = kScratchSlots - 1;
+    if (ctx->time_limit_s > 0) {
+        double left_s = ctx->time_limit_s;
+        if (b.t_begin >= 0) left_s -= (now_ms() - b.t_begin) / 1000.0;
+        hq.limit_ticks = (unsigned long long)std::max(1.0, left_s * ctx->wall_khz * 1000.0);
+    }
+    const uint32_t pend = n_tasks;
+    HIP_TRY(ctx, hipMemcpyAsync(dq, &hq, sizeof(hq), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dpend, &pend, 4, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dplan, &P, sizeof(P), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dops, hops.data(), hops.size() * sizeof(DOp), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, launch_root_tasks(dring, n_tasks, 0, roots, ctx->stream));
+    HIP_TRY(ctx, launch_ring_init(dseq, kScratchSlots, n_tasks, ctx->stream));
+    const uint32_t gw = (uint32_t)ctx->num_cu * ctx->dfs_waves_per_cu / 4 * 4;
+    const double tk = now_ms();
+    if (ctx->dfs_strategy == PSM_DFS_WAVE) {
+        HIP_TRY(ctx, launch_dfs_wave(dplan, dops, b.row_off.as<uint64_t>(), b.idx.as<uint32_t>(),
+                                     b.cand_pool.as<uint32_t>(), dring, dseq, dq, dqc, dpend, P.n, gw, ctx->stream));
+    } else {
+        HIP_TRY(ctx, launch_dfs(dplan, dops, b.row_off.as<uint64_t>(), b.idx.as<uint32_t>(),
+                                b.cand_pool.as<uint32_t>(), dring, dseq, dq, dqc, dpend, P.n, gw,
+                                EmitSink{nullptr, nullptr, 0, nullptr}, ctx->stream));
+    }
+    QCounters qc{};
+    uint32_t left = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&hq, dq, sizeof(hq), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(&qc, dqc, sizeof(qc), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(&left, dpend, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const double t1 = now_ms();
+    h.count = cap ? std::min<uint64_t>(cap, qc.count) : qc.count;
+    h.capped = cap && qc.count >= cap;
+    h.calls = qc.calls;
+    h.rejects = qc.rejects;
+    h.probe_bytes = qc.probe_bytes;
+    h.algo_bytes = qc.algo_bytes;
+    h.timed_out = left && !qc.done;
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->count = h.count;
+        st->calls = h.calls;
+        st->capped = h.capped;
+        st->tasks = n_tasks + hq.donated;
+        st->kernel_ms = t1 - tk;
+        st->expand_ms = tk - t0;
+        st->probe_bytes = h.probe_bytes;
+        st->algo_bytes = h.algo_bytes;
+        st->visited_rejects = h.rejects;
+    }
+    if (h.timed_out) {
+        set_err(ctx, PSM_ETIMEOUT, "enumeration passed the time limit");
+        return PSM_ETIMEOUT;
+    }
+    return PSM_OK;
 }
 
 int psm_split_join_count(psm_query* q, int pattern, uint64_t max_rows, psm_split_join_stats* st) {
