@@ -116,11 +116,18 @@ int psm_set_time_limit(psm_ctx* ctx, double seconds);
  *                                   larger of 2^18 and 4 per DFS lane); the
  *                                   ring reuses slots, so this bounds only
  *                                   the tasks queued at one time
+ *   PSM_TUNE_CLAIM_DIVISOR          pre-expanded tasks per DFS lane above
+ *                                   which lanes whose previous tasks were
+ *                                   short take several (up to 8) per claim,
+ *                                   1..1024 (default 1): fewer claims for
+ *                                   millions of small tasks, one task per
+ *                                   claim for heavy ones
  * No counterpart in the reference (its only knob is -threadnums). */
 #define PSM_TUNE_DFS_WAVES_PER_CU 0
 #define PSM_TUNE_EXPAND_TASKS_PER_LANE 1
 #define PSM_TUNE_DFS_STRATEGY 2
 #define PSM_TUNE_QUEUE_SLOTS 3
+#define PSM_TUNE_CLAIM_DIVISOR 4
 #define PSM_DFS_LANE 0
 #define PSM_DFS_WAVE 1
 int psm_set_tuning(psm_ctx* ctx, int key, double value);

@@ -98,6 +98,7 @@ PSM_TUNE_DFS_WAVES_PER_CU = 0
 PSM_TUNE_EXPAND_TASKS_PER_LANE = 1
 PSM_TUNE_DFS_STRATEGY = 2
 PSM_TUNE_QUEUE_SLOTS = 3
+PSM_TUNE_CLAIM_DIVISOR = 4
 PSM_DFS_LANE = 0
 PSM_DFS_WAVE = 1
 SPLITS = {"STAR": 0, "TWINTWIG": 1, "SEED": 2}

@@ -37,6 +37,7 @@ KEYS = {
     "dfs_waves_per_cu": L.PSM_TUNE_DFS_WAVES_PER_CU,
     "expand_tasks_per_lane": L.PSM_TUNE_EXPAND_TASKS_PER_LANE,
     "dfs_strategy": L.PSM_TUNE_DFS_STRATEGY,
+    "claim_divisor": L.PSM_TUNE_CLAIM_DIVISOR,
 }
 STRATEGIES = {"lane": L.PSM_DFS_LANE, "wave": L.PSM_DFS_WAVE}
 

@@ -1265,7 +1265,7 @@ __device__ __forceinline__ int test_merge(const DPlan& P, const DOp* __restrict_
 
 // Device-coherent loads of a queued task record. A lane that splits a task
 // off writes the record, makes it visible at agent scope (release fence) and
-// then stores the slot's sequence word with release semantics. The claiming
+// then stores the slot's sequence word. The claiming
 // lane loads the sequence word (relaxed), waits for that load to return
 // (record_wait: s_waitcnt on the vector memory counter, so no record load can
 // issue before the word was observed), and then reads the record with
@@ -1280,29 +1280,40 @@ __device__ __forceinline__ uint64_t coherent_u64(const uint64_t* p) {
     return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void record_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Makes a split-off record visible at agent scope before its sequence word:
+// a release fence (L2 writeback and store wait, no L1 invalidate).
+__device__ __forceinline__ void record_publish_fence() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 
 // Sequence word of ring position p (see DQueue).
 __device__ __forceinline__ uint32_t seq_load(uint32_t* seq, uint32_t mask, uint32_t p) {
     return __hip_atomic_load(seq + (p & mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// The record of position p is written: claimable.
+// The record of position p is written: claimable. The caller has made the
+// record visible at agent scope first (record_publish_fence).
 __device__ __forceinline__ void seq_publish(uint32_t* seq, uint32_t mask, uint32_t p) {
-    __hip_atomic_store(seq + (p & mask), p + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(seq + (p & mask), p + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // The record of position p has been read: its slot is free for p + mask + 1.
-// Release: the record loads complete before the slot can be rewritten.
+// The record loads have returned (record_wait) before the word is stored, so
+// a producer that sees it can only overwrite a record nobody reads any more.
+// Nothing the claimant wrote needs to be visible, so no release (on gfx950 an
+// agent-scope release writes back the L2, at every claim).
 __device__ __forceinline__ void seq_release(uint32_t* seq, uint32_t mask, uint32_t p) {
-    __hip_atomic_store(seq + (p & mask), p + mask + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    record_wait();
+    __hip_atomic_store(seq + (p & mask), p + mask + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Marks query q's count final (first caller wins).
+// Query q's count reached its cap now (the first caller's time is kept).
 __device__ __forceinline__ void mark_final(QCounters* qc, uint32_t q) {
-    if (__hip_atomic_load(&qc[q].done_tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-        atomicCAS(&qc[q].done_tick, 0ull, (unsigned long long)wall_clock64());
+    if (__hip_atomic_load(&qc[q].cap_tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        atomicCAS(&qc[q].cap_tick, 0ull, (unsigned long long)wall_clock64());
 }
-// Subtracts k ended tasks from pending[q]; the last one marks q final.
+// Subtracts k ended tasks from pending[q] and records the time of the ends:
+// the query's last task end is the largest (no atomic here returns a value,
+// so the lane does not wait for these shared counters).
 __device__ __forceinline__ void pending_sub(uint32_t* pending, QCounters* qc, uint32_t q, uint32_t k) {
-    if (atomicSub(pending + q, k) == k) mark_final(qc, q);
+    atomicSub(pending + q, k);
+    atomicMax(&qc[q].end_tick, (unsigned long long)wall_clock64());
 }
 
 __device__ __forceinline__ uint32_t load_relaxed(unsigned int* p) {
@@ -1321,6 +1332,8 @@ __device__ __forceinline__ bool past_deadline(DQueue* Q) {
 // uncapped search's rate
 constexpr uint32_t kLeafBurst = 16;
 constexpr uint32_t kClaimMax = 8;     // queue slots a lane takes per claim at most
+constexpr uint32_t kFewTasks = 64;
+constexpr uint32_t kShortTask = 1024; // steps below which a finished task counts as short for claims    // a query with fewer queued and running tasks splits while the ring is short
 
 // Adds a lane's running count (setting the query's done flag at its cap)
 // and statistics to query q's counters and clears them.
@@ -1358,8 +1371,8 @@ __device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t c
 __global__ __launch_bounds__(kBlock) void k_dfs(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
-    const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* seq, DQueue* Q,
-    QCounters* qc, uint32_t* pending, uint32_t max_n, EmitSink sink) {
+    const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ init, DTask* queue, uint32_t* seq,
+    DQueue* Q, QCounters* qc, uint32_t* pending, uint32_t max_n, EmitSink sink) {
     extern __shared__ __align__(16) unsigned char lane_smem[];
     const uint32_t lane = lane_id();
     const uint32_t per_field = blockDim.x * max_n;
@@ -1379,8 +1392,10 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             atomicCAS(&Q->start_tick, 0ull, now);
     }
     const uint32_t qmask = Q->mask;
+    const uint32_t n0 = Q->n0;
+    const uint32_t claim_div = max(1u, Q->claim_div);
 
-    bool active = false, waiting = false;
+    bool active = false, waiting = false, wring = false;   // wring: the claimed positions are ring positions
     uint32_t wslot = 0, wend = 0, backoff = 0;
     uint32_t busy_done = 0, pend_done = 0;   // task ends not yet subtracted from busy / pending[q]
     uint32_t q = 0, tdepth = 0, d = 0, n = 0;
@@ -1391,58 +1406,92 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     unsigned long long flush_at = 4096;   // leaf counts a lane holds before adding them to its query
     bool near_cap = false;                // the query's count was at half its cap or more at the last flush
     bool hungry = true;                   // wavefront view: the queue holds fewer tasks than idle lanes
+    bool ring_short = true;               // wavefront view: the ring holds fewer than lanes / 8 tasks
     uint32_t witer = 0;
     uint32_t steps = 0;
+    uint32_t last_steps = kShortTask;   // steps of the lane's previous task (none yet: counts as long)
     const DPlan* P = plans;
     const uint32_t lanes_total = gridDim.x * blockDim.x;
 
     while (true) {
-        // every 16 iterations one lane reads whether lanes wait for work
-        // (the donation rule below); the wavefront is converged here
+        // every 16 iterations one lane reads the queue's state for the
+        // donation rule below; the wavefront is converged here. hungry: the
+        // ring and the array together hold fewer tasks than there are idle
+        // lanes; short: the ring holds fewer than an eighth of the lanes
         if ((witer++ & 15) == 0) {
             uint32_t h = 0;
-            if (lane == 0)
-                h = load_relaxed(&Q->tail) - load_relaxed(&Q->head) <
-                    lanes_total - min(load_relaxed(&Q->busy), lanes_total);
-            hungry = __shfl(h, 0) != 0;
+            if (lane == 0) {
+                const uint32_t len = load_relaxed(&Q->tail) - load_relaxed(&Q->head);
+                const uint32_t h0 = load_relaxed(&Q->head0);
+                const uint32_t left0 = h0 < n0 ? n0 - h0 : 0u;
+                h = (len + left0 < lanes_total - min(load_relaxed(&Q->busy), lanes_total) ? 1u : 0u) |
+                    (len < (lanes_total >> 3) ? 2u : 0u);
+            }
+            h = __shfl(h, 0);
+            hungry = (h & 1u) != 0;
+            ring_short = (h & 2u) != 0;
         }
         // claim queued tasks for idle lanes, one atomic per wavefront (none
-        // once the time limit has passed)
-        // While the queue holds many tasks per lane (more than 16), each lane
-        // gets several consecutive positions per claim (up to kClaimMax) and
-        // takes them one after the other: claims go through one
-        // compare-and-swap on the shared head, whose rate (not the search)
-        // bounded launches with millions of small tasks. Below that a lane
-        // takes one task per claim: a lane holding several heavy subtrees
-        // would run them in turn while other lanes wait.
+        // once the time limit has passed): split-off work from the ring
+        // first, one task per lane, then the pre-expanded array. While the
+        // array holds many tasks per lane (more than claim_div) and most of
+        // the claiming lanes' previous tasks were short, each lane gets
+        // several consecutive positions per claim (up to kClaimMax) and takes
+        // them one after the other: claims go through one compare-and-swap
+        // on the shared head, whose rate (not the search) bounded launches
+        // with millions of small tasks. Otherwise a lane takes one task per
+        // claim: a lane holding several heavy subtrees would run them in
+        // turn while other lanes wait.
         const unsigned long long want = __ballot(!active && !waiting && load_relaxed(&Q->timeout) == 0);
+        // lanes whose previous task was short (small tasks, many claims)
+        const unsigned long long quick = __ballot(!active && !waiting && last_steps < kShortTask);
         if (want && backoff == 0) {
             const uint32_t leader = __ffsll((long long)want) - 1;
-            uint32_t base = 0, got = 0, per = 1;
+            uint32_t base = 0, got = 0, per = 1, from_ring = 0;
             if (lane == leader) {
+                const uint32_t nw = (uint32_t)__popcll(want);
                 const uint32_t h = load_relaxed(&Q->head), t = load_relaxed(&Q->tail);
+                // busy counts claimed, unfinished tasks; it rises before a
+                // claim so that empty queues with busy == 0 are only seen
+                // once no lane holds or is taking a task
                 if (t != h) {
-                    per = min(kClaimMax, max(1u, (t - h) / (16u * lanes_total)));
-                    const uint32_t k = min((uint32_t)__popcll(want) * per, t - h);
-                    // busy counts claimed, unfinished tasks; it rises before
-                    // the claim so that head == tail with busy == 0 is only
-                    // seen once no lane holds or is taking a task
+                    const uint32_t k = min(nw, t - h);
                     atomicAdd(&Q->busy, k);
                     if (atomicCAS(&Q->head, h, h + k) == h) {
                         base = h;
                         got = k;
+                        from_ring = 1;
                     } else {
                         atomicSub(&Q->busy, k);
+                    }
+                } else {
+                    const uint32_t h0 = load_relaxed(&Q->head0);
+                    if (h0 < n0) {
+                        // several per claim while most idle lanes ran short
+                        // tasks (the queue then turns over fast); one per
+                        // claim after long ones
+                        if (2u * (uint32_t)__popcll(quick & want) >= nw)
+                            per = min(kClaimMax, max(1u, (n0 - h0) / (claim_div * lanes_total)));
+                        const uint32_t k = min(nw * per, n0 - h0);
+                        atomicAdd(&Q->busy, k);
+                        if (atomicCAS(&Q->head0, h0, h0 + k) == h0) {
+                            base = h0;
+                            got = k;
+                        } else {
+                            atomicSub(&Q->busy, k);
+                        }
                     }
                 }
             }
             base = __shfl(base, leader);
             got = __shfl(got, leader);
             per = __shfl(per, leader);
+            from_ring = __shfl(from_ring, leader);
             if (!active && !waiting) {
                 const uint32_t r = __popcll(want & lanemask_lt());
                 if (r * per < got) {
                     waiting = true;
+                    wring = from_ring != 0;
                     wslot = base + r * per;
                     wend = min(wslot + per, base + got);
                 }
@@ -1451,12 +1500,12 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         } else if (backoff) {
             --backoff;
         }
-        if (waiting && seq_load(seq, qmask, wslot) == wslot + 1u) {
+        if (waiting && (!wring || seq_load(seq, qmask, wslot) == wslot + 1u)) {
             waiting = false;
-            record_wait();
+            if (wring) record_wait();
             // the record is read field by field with coherent loads (a local
             // copy of its prefix array would live in scratch memory)
-            const DTask* task = queue + (wslot & qmask);
+            const DTask* task = wring ? queue + (wslot & qmask) : init + wslot;
             const uint32_t tq = coherent_u32(&task->q);
             if (tq != q) {
                 // the lane's counters so far belong to the previous query
@@ -1472,7 +1521,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 late = true;
             }
             if (late || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
-                seq_release(seq, qmask, wslot);
+                if (wring) seq_release(seq, qmask, wslot);
                 ++busy_done;
                 if (wslot + 1 < wend) {   // the lane's next claimed slot
                     ++wslot;
@@ -1491,7 +1540,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 const uint64_t thi = coherent_u64(&task->hi);
                 const uint64_t tlo = coherent_u64(&task->lo);
                 const uint32_t tsel = coherent_u32(&task->sel);
-                seq_release(seq, qmask, wslot);
+                if (wring) seq_release(seq, qmask, wslot);
                 for (uint32_t e = 0; e < tdepth; ++e) st.M_[st.at(e)] = cand_pool[P->cand_base[e] + st.I_[st.at(e)]];
                 if (thi == kAutoHi) {
                     enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo);
@@ -1512,7 +1561,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         if (__ballot(active || waiting) == 0) {
             // the whole wavefront is idle: leave once no work can appear
             if (load_relaxed(&Q->timeout)) break;
-            if (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0) break;
+            if (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->head0) >= n0 &&
+                load_relaxed(&Q->busy) == 0)
+                break;
             __builtin_amdgcn_s_sleep(4);
             continue;
         }
@@ -1608,11 +1659,16 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 active = false;
             }
         }
-        // donation: split the shallowest level that still has >= 2 entries,
-        // when the queue holds fewer tasks than there are lanes without one
-        // (work goes to lanes that wait for it; while every lane is busy,
-        // splitting would only add claims and records to the queue)
-        if (active && hungry && (steps & 15) == 0) {
+        // donation: split the shallowest level that still has >= 2 entries
+        // into the ring when it holds fewer tasks than there are lanes
+        // without one (work goes to lanes that wait for it), or, while it is
+        // short, when the lane's query has few tasks left (a search running
+        // on a handful of lanes is what the launch waits for; its split-off
+        // work is claimed before fresh array tasks). While every lane is busy
+        // with well-spread queries, splitting would only add claims and
+        // records.
+        if (active && (steps & 15) == 0 &&
+            (hungry || (ring_short && load_relaxed(pending + q) < kFewTasks))) {
             uint32_t e = tdepth;
             uint64_t ce = 0, ee = 0;
             for (; e <= d; ++e) {
@@ -1647,7 +1703,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     nt.lo = base + mid;
                     nt.hi = base + ee;
                     for (uint32_t k = 0; k < e; ++k) nt.prefix[k] = st.I_[st.at(k)];
-                    __threadfence();
+                    record_publish_fence();
                     seq_publish(seq, qmask, t);
                     if (e == d) L.end = mid;
                     else st.end_[st.at(e)] = (uint32_t)mid;
@@ -1667,6 +1723,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap);
         if (finished) ++pend_done;
         if (!active) {
+            last_steps = steps;
             ++busy_done;
             if (more) {
                 ++wslot;
@@ -1866,8 +1923,8 @@ template <uint32_t N>
 __global__ __launch_bounds__(kWave) void k_dfs_wave(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
-    const uint32_t* __restrict__ cand_pool, DTask* queue, uint32_t* seq, DQueue* Q,
-    QCounters* qc, uint32_t* pending) {
+    const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ init, DTask* queue, uint32_t* seq,
+    DQueue* Q, QCounters* qc, uint32_t* pending) {
     const uint32_t lane = threadIdx.x;
     __shared__ WaveFrame<N> S;
     if (lane == 0) {
@@ -1877,37 +1934,55 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             atomicCAS(&Q->start_tick, 0ull, now);
     }
     const uint32_t qmask = Q->mask;
+    const uint32_t n0 = Q->n0;
     const uint32_t hungry_below = gridDim.x / 2;   // queue shorter than half the wavefronts
     unsigned long long bytes = 0, algo = 0, busy_pos = 0, all_pos = 0;
 
     while (true) {
         // ---- claim one task (none once the time limit has passed: queued
         // tasks stay counted in pending[] and their queries report a timeout)
-        uint32_t slot = kInvalid, late = 0;
+        // split-off work from the ring first, then the pre-expanded array
+        uint32_t slot = kInvalid, late = 0, from_ring = 0;
         if (lane == 0) late = load_relaxed(&Q->timeout);
         if (__shfl(late, 0)) break;
         if (lane == 0) {
             const uint32_t h = load_relaxed(&Q->head), tl = load_relaxed(&Q->tail);
             if (tl != h) {
                 atomicAdd(&Q->busy, 1u);   // before the claim, as in k_dfs
-                if (atomicCAS(&Q->head, h, h + 1) == h) slot = h;
-                else atomicSub(&Q->busy, 1u);
+                if (atomicCAS(&Q->head, h, h + 1) == h) {
+                    slot = h;
+                    from_ring = 1;
+                } else {
+                    atomicSub(&Q->busy, 1u);
+                }
+            } else {
+                const uint32_t h0 = load_relaxed(&Q->head0);
+                if (h0 < n0) {
+                    atomicAdd(&Q->busy, 1u);
+                    if (atomicCAS(&Q->head0, h0, h0 + 1) == h0) slot = h0;
+                    else atomicSub(&Q->busy, 1u);
+                }
             }
         }
         slot = __shfl(slot, 0);
+        from_ring = __shfl(from_ring, 0);
         if (slot == kInvalid) {
             all_pos += 1;
             uint32_t stop = 0;
             if (lane == 0)
                 stop = load_relaxed(&Q->timeout) ||
-                       (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->busy) == 0);
+                       (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->head0) >= n0 &&
+                        load_relaxed(&Q->busy) == 0);
             if (__shfl(stop, 0)) break;
             __builtin_amdgcn_s_sleep(8);
             continue;
         }
-        while (seq_load(seq, qmask, slot) != slot + 1u) __builtin_amdgcn_s_sleep(1);
-        record_wait();
-        const DTask* task = queue + (slot & qmask);   // read with coherent loads (see coherent_u32)
+        if (from_ring) {
+            while (seq_load(seq, qmask, slot) != slot + 1u) __builtin_amdgcn_s_sleep(1);
+            record_wait();
+        }
+        // read with coherent loads (see coherent_u32)
+        const DTask* task = from_ring ? queue + (slot & qmask) : init + slot;
         const uint32_t q = coherent_u32(&task->q);
         const DPlan& P = plans[q];
         uint32_t drop = 0;
@@ -1918,7 +1993,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                 drop = 1;
             }
             if (drop) {
-                seq_release(seq, qmask, slot);
+                if (from_ring) seq_release(seq, qmask, slot);
                 atomicSub(&Q->busy, 1u);
             }
         }
@@ -1944,7 +2019,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
         }
         __syncthreads();
         // one wavefront: lane 0's release waits for every lane's record loads
-        if (lane == 0) seq_release(seq, qmask, slot);
+        if (lane == 0 && from_ring) seq_release(seq, qmask, slot);
 
         uint32_t g = t, cnt = 0, it = 0;
         int cache_g = -1;
@@ -2120,7 +2195,11 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             // split off work while the queue is short
             if ((it & 7) == 0) {
                 uint32_t hungry = 0;
-                if (lane == 0) hungry = load_relaxed(&Q->tail) - load_relaxed(&Q->head) < hungry_below;
+                if (lane == 0) {
+                    const uint32_t h0 = load_relaxed(&Q->head0);
+                    hungry = load_relaxed(&Q->tail) - load_relaxed(&Q->head) + (h0 < n0 ? n0 - h0 : 0u) <
+                             hungry_below;
+                }
                 if (__shfl(hungry, 0)) {
                     const uint32_t e = t + lane;
                     bool can = false;
@@ -2157,7 +2236,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                                 atomicAdd(pending + q, 1u);
                                 atomicAdd(&Q->donated, 1u);
                             }
-                            __threadfence();
+                            record_publish_fence();
                             __syncthreads();
                             if (lane == 0) {
                                 seq_publish(seq, qmask, ns2);
@@ -2334,7 +2413,7 @@ __global__ __launch_bounds__(kBlock) void k_join_probe_emit(
 
 // Root range tasks of query q: task i enumerates positions
 // [i * len / n, (i + 1) * len / n) of C(order[0]) at depth 0.
-__global__ void k_root_tasks(DTask* ring, uint32_t n, uint32_t q, uint64_t len) {
+__global__ void k_root_tasks(DTask* tasks, uint32_t n, uint32_t q, uint64_t len) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     DTask t;
@@ -2345,7 +2424,7 @@ __global__ void k_root_tasks(DTask* ring, uint32_t n, uint32_t q, uint64_t len) 
     t.lo = (uint64_t)i * len / n;
     t.hi = (uint64_t)(i + 1) * len / n;
     for (uint32_t e = 0; e < kPrefixMax; ++e) t.prefix[e] = 0;
-    ring[i] = t;
+    tasks[i] = t;
 }
 
 __global__ void k_ring_init(uint32_t* seq, uint32_t slots, uint32_t n_tasks) {
@@ -2721,7 +2800,7 @@ uint32_t dfs_lanes_per_block(uint32_t max_n) {
 size_t dfs_lane_lds_bytes(uint32_t max_n) { return lane_lds_bytes(dfs_lanes_per_block(max_n), max_n); }
 
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                      const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* seq,
+                      const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue, uint32_t* seq,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                       EmitSink sink, hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
@@ -2741,8 +2820,8 @@ hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_of
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 &&
         cus > 0)
         blocks = std::min<uint32_t>(blocks, (uint32_t)(per_cu * cus));
-    hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(lanes), lds, s, plans, ops, row_off, idx, cand_pool, queue,
-                       seq, dq, qc, pending, max_n, sink);
+    hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(lanes), lds, s, plans, ops, row_off, idx, cand_pool, init,
+                       queue, seq, dq, qc, pending, max_n, sink);
     return hipGetLastError();
 }
 
@@ -2751,20 +2830,21 @@ size_t dfs_wave_lds_bytes(uint32_t max_n) {
 }
 
 hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                           const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* seq,
+                           const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue,
+                           uint32_t* seq,
                            DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                            hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
     if (max_n == 0 || max_n > kMaxQ) return hipErrorInvalidValue;
     if (max_n <= 16)
         hipLaunchKernelGGL(k_dfs_wave<16>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, queue, seq, dq, qc, pending);
+                           cand_pool, init, queue, seq, dq, qc, pending);
     else if (max_n <= 32)
         hipLaunchKernelGGL(k_dfs_wave<32>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, queue, seq, dq, qc, pending);
+                           cand_pool, init, queue, seq, dq, qc, pending);
     else
         hipLaunchKernelGGL(k_dfs_wave<64>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, queue, seq, dq, qc, pending);
+                           cand_pool, init, queue, seq, dq, qc, pending);
     return hipGetLastError();
 }
 
@@ -2801,9 +2881,9 @@ hipError_t launch_join_probe_emit(const uint32_t* build_rows, uint32_t build_wid
     return hipGetLastError();
 }
 
-hipError_t launch_root_tasks(DTask* ring, uint32_t n, uint32_t q, uint64_t len, hipStream_t s) {
+hipError_t launch_root_tasks(DTask* tasks, uint32_t n, uint32_t q, uint64_t len, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_root_tasks, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ring, n, q, len);
+    hipLaunchKernelGGL(k_root_tasks, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, tasks, n, q, len);
     return hipGetLastError();
 }
 

@@ -121,15 +121,18 @@ struct DTask {
     uint32_t prefix[kPrefixMax];
 };
 
-// Work queue of the persistent DFS: a ring of `mask + 1` task records (a
-// power of two) addressed by 32-bit positions; position p lives in slot
-// p & mask. Positions [head, tail) are queued. seq[slot] is the slot's
-// sequence word: p + 1 once the record of position p is written (claimable),
-// p + mask + 1 once its claimant has read it (free for position
-// p + mask + 1). A producer takes position tail only when its slot is free,
-// so slots are reused for as long as the launch runs and donation never
-// stops at a fixed number of splits. Positions wrap modulo 2^32; only
-// differences of positions are compared.
+// Work queue of the persistent DFS, in two parts. The pre-expanded tasks are
+// a fixed array init[0, n0), claimed in order through head0. Work that
+// running lanes split off goes to a ring of `mask + 1` task records (a power
+// of two) addressed by 32-bit positions; position p lives in slot p & mask,
+// positions [head, tail) are queued, and idle lanes claim from the ring
+// before the array: split-off work belongs to searches already under way,
+// whose ends bound the launch. seq[slot] is the slot's sequence word: p + 1
+// once the record of position p is written (claimable), p + mask + 1 once
+// its claimant has read it (free for position p + mask + 1). A producer
+// takes position tail only when its slot is free, so slots are reused for as
+// long as the launch runs and splitting never stops at a fixed count.
+// Positions wrap modulo 2^32; only differences of positions are compared.
 struct DQueue {
     unsigned int head;
     unsigned int tail;
@@ -143,7 +146,9 @@ struct DQueue {
     unsigned long long lane_iters;    // loop iterations of all lanes
     unsigned long long start_tick;    // wall clock when the first block started
     unsigned int full;         // donations skipped because the next slot was still held
-    unsigned int pad;
+    unsigned int head0;        // next task of the pre-expanded array
+    unsigned int n0;           // pre-expanded tasks
+    unsigned int claim_div;    // array tasks per lane above which a lane takes several per claim
 };
 
 // Where the DFS writes complete embeddings (rows of n data vertex ids in
@@ -181,9 +186,11 @@ struct QCounters {
     unsigned long long expand_algo_bytes;  // nodes of task pre-expansion
     uint32_t done;        // cap reached
     uint32_t timed_out;   // work was dropped at the time limit
-    // wall clock (wall_clock64 ticks) when the query's count became final:
-    // its cap was reached or its last task ended; 0 while it runs
-    unsigned long long done_tick;
+    // wall clock (wall_clock64 ticks) when the query's count reached its cap
+    // (0: not reached), and the latest end of one of its tasks: the count is
+    // final at the first, or else at the second
+    unsigned long long cap_tick;
+    unsigned long long end_tick;
 };
 
 // Kernel classes timed by the library (psm_kernel_stats).
@@ -270,7 +277,7 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
                          const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
                          unsigned long long* qchild, int write, hipStream_t s);
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                      const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* seq,
+                      const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue, uint32_t* seq,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                       EmitSink sink, hipStream_t s);
 hipError_t launch_join_build(const uint32_t* rows, uint64_t n_rows, uint32_t width, JoinKeys keys,
@@ -288,22 +295,24 @@ hipError_t launch_join_probe(const uint32_t* build_rows, uint32_t build_width, J
                              unsigned long long* bytes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // n root range tasks of query q splitting C(order[0]) (len entries) evenly,
-// written to ring[0, n)
-hipError_t launch_root_tasks(DTask* ring, uint32_t n, uint32_t q, uint64_t len, hipStream_t s);
+// written to tasks[0, n)
+hipError_t launch_root_tasks(DTask* tasks, uint32_t n, uint32_t q, uint64_t len, hipStream_t s);
 // ring sequence words: seq[i] = i + 1 for the n_tasks queued positions, i for
 // the free slots
 hipError_t launch_ring_init(uint32_t* seq, uint32_t slots, uint32_t n_tasks, hipStream_t s);
 size_t dfs_lane_lds_bytes(uint32_t max_n);
 size_t dfs_wave_lds_bytes(uint32_t max_n);
 hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                           const uint32_t* idx, const uint32_t* cand_pool, DTask* queue, uint32_t* seq,
+                           const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue,
+                           uint32_t* seq,
                            DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                            hipStream_t s);
 uint64_t scan_tmp_entries(uint64_t n);
 
 // ---- fixed device scratch of the allocation-free enumeration -------------
 // Layout (byte offsets, 256-aligned): queue header, query counters, pending
-// count, one plan, the operands, the ring's sequence words and its records.
+// count, one plan, the operands, the ring's sequence words and records, the
+// root tasks.
 constexpr uint32_t kScratchOps = 2048;       // >= the edges of a 64-vertex query
 constexpr uint32_t kScratchSlots = 1u << 16;  // task ring slots
 constexpr uint32_t kScratchRootTasks = 4096;  // root range tasks at most
@@ -316,9 +325,9 @@ constexpr size_t kScrPlan = kScrZeroEnd;
 constexpr size_t kScrOps = kScrPlan + kScratchAlign(sizeof(DPlan));
 constexpr size_t kScrSeq = kScrOps + kScratchAlign(sizeof(DOp) * kScratchOps);
 constexpr size_t kScrRing = kScrSeq + kScratchAlign(4ull * kScratchSlots);
-constexpr size_t kScratchBytes = kScrRing + kScratchAlign(sizeof(DTask) * kScratchSlots);
+constexpr size_t kScrInit = kScrRing + kScratchAlign(sizeof(DTask) * kScratchSlots);
+constexpr size_t kScratchBytes = kScrInit + kScratchAlign(sizeof(DTask) * kScratchRootTasks);
 constexpr uint32_t kScratchMaxDevices = 64;
-static_assert(kScratchRootTasks <= kScratchSlots / 2, "root tasks leave room for split-off work");
 static_assert(kScratchBytes <= (12u << 20), "the scratch stays modest");
 // resolves the scratch's address on `device` (which must be current); once
 // per device, thread-safe

@@ -40,6 +40,7 @@ using namespace psm;
 // contending for L2 and the queue): the 9,460-vertex C1 query enumerates in
 // 15 ms at 1 wave per CU, 19 ms at 2 and 48 ms at 8.
 constexpr uint32_t kDefaultWavesPerCu = 1;
+constexpr uint32_t kDefaultClaimDiv = 1;     // array tasks per lane above which a claim takes several
 constexpr double kDefaultTasksPerLane = 2.0;
 
 struct psm_ctx {
@@ -59,7 +60,8 @@ struct psm_ctx {
     uint32_t dfs_waves_per_cu = kDefaultWavesPerCu;        // persistent DFS grid
     double expand_tasks_per_lane = kDefaultTasksPerLane;   // pre-expansion target
     uint32_t dfs_strategy = PSM_DFS_LANE;     // lane-per-task or wavefront-per-task DFS
-    uint32_t queue_slots = 0;                 // free ring slots beyond the expanded tasks (0: automatic)
+    uint32_t queue_slots = 0;                 // ring slots for split-off work (0: automatic)
+    uint32_t claim_div = kDefaultClaimDiv;    // array tasks per lane above which claims take several
     double wall_khz = 100000.0;               // device wall clock rate
     bool verbose = false;
 };
@@ -1369,22 +1371,24 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
     unsigned int donated = 0;
     timed_out = false;
     if (n_tasks) {
-        // queue: a ring holding the expanded tasks and room for split-off
-        // work; slots are reused, so the room bounds only the tasks queued at
-        // once (donation runs while the queue is shorter than the idle lanes)
+        // queue: the expanded tasks as a fixed array, and a ring for
+        // split-off work; ring slots are reused, so its size bounds only the
+        // split-off tasks queued at once (splitting runs while the ring is
+        // shorter than the idle lanes, or than an eighth of all lanes for a
+        // query with few tasks left)
         const uint64_t lanes_all = (uint64_t)grid_waves_full * 64;
         const uint64_t room = ctx->queue_slots ? ctx->queue_slots : std::max<uint64_t>(1u << 18, 4 * lanes_all);
         uint64_t ring = 64;
-        while (ring < n_tasks + room) ring <<= 1;
-        if (ring > (1ull << 31)) return set_err(ctx, PSM_ENOMEM, "task queue beyond 2^31 slots");
+        while (ring < room) ring <<= 1;
+        if (ring > (1ull << 31)) return set_err(ctx, PSM_ENOMEM, "task ring beyond 2^31 slots");
         DBuf d_queue, d_seq, d_q, d_state, d_pending;
         HIP_TRY(ctx, d_queue.alloc(ring * sizeof(DTask)));
-        HIP_TRY(ctx, hipMemcpyAsync(d_queue.p, d_tasks.p, n_tasks * sizeof(DTask), hipMemcpyDeviceToDevice, s()));
         HIP_TRY(ctx, d_seq.alloc(ring * 4));
-        HIP_TRY(ctx, launch_ring_init(d_seq.as<uint32_t>(), (uint32_t)ring, (uint32_t)n_tasks, s()));
+        HIP_TRY(ctx, launch_ring_init(d_seq.as<uint32_t>(), (uint32_t)ring, 0u, s()));
         DQueue hq{};
-        hq.tail = (unsigned int)n_tasks;
+        hq.n0 = (unsigned int)n_tasks;
         hq.mask = (unsigned int)(ring - 1);
+        hq.claim_div = ctx->claim_div;
         if (enum_limit_s > 0) {
             const double left_s = enum_limit_s - (now_ms() - t_eval) / 1000.0;
             hq.limit_ticks = (unsigned long long)std::max(1.0, left_s * ctx->wall_khz * 1000.0);
@@ -1405,13 +1409,15 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         if (wave_dfs) {
             TIMED(ctx, kKDfs,
                   launch_dfs_wave(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
-                                  cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_seq.as<uint32_t>(),
+                                  cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), d_queue.as<DTask>(),
+                                  d_seq.as<uint32_t>(),
                                   d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
                                   s()));
         } else {
             TIMED(ctx, kKDfs,
                   launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
-                             cand_pool.as<uint32_t>(), d_queue.as<DTask>(), d_seq.as<uint32_t>(),
+                             cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), d_queue.as<DTask>(),
+                             d_seq.as<uint32_t>(),
                              d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
                              sink, s()));
         }
@@ -1465,8 +1471,9 @@ int Batch::evaluate(uint64_t cap, double enum_limit) {
     for (size_t i = 0; i < qs.size(); ++i) {
         QueryHost* q = qs[i];
         // the device clock runs at wall_khz ticks per millisecond
-        q->done_ms = qc[i].done_tick && dfs_start_tick && qc[i].done_tick >= dfs_start_tick
-                         ? std::min(t_end, dfs_launch_ms + (double)(qc[i].done_tick - dfs_start_tick) / ctx->wall_khz)
+        const unsigned long long tick = qc[i].cap_tick ? qc[i].cap_tick : qc[i].end_tick;
+        q->done_ms = tick && dfs_start_tick && tick >= dfs_start_tick
+                         ? std::min(t_end, dfs_launch_ms + (double)(tick - dfs_start_tick) / ctx->wall_khz)
                          : t_end;
         q->count = cap ? std::min<uint64_t>(cap, qc[i].count) : qc[i].count;
         q->capped = cap && qc[i].count >= cap;
@@ -1699,6 +1706,10 @@ int psm_set_tuning(psm_ctx* ctx, int key, double value) {
                 return set_err(ctx, PSM_EINVAL, "dfs strategy must be PSM_DFS_LANE or PSM_DFS_WAVE");
             ctx->dfs_strategy = (uint32_t)value;
             return PSM_OK;
+        case PSM_TUNE_CLAIM_DIVISOR:
+            if (!(value >= 1 && value <= 1024)) return set_err(ctx, PSM_EINVAL, "claim divisor outside [1, 1024]");
+            ctx->claim_div = (uint32_t)value;
+            return PSM_OK;
         case PSM_TUNE_QUEUE_SLOTS:
             if (!(value >= 0 && value <= (double)(1u << 28)))
                 return set_err(ctx, PSM_EINVAL, "queue slots outside [0, 2^28]");
@@ -1716,6 +1727,7 @@ int psm_get_tuning(psm_ctx* ctx, int key, double* value) {
         case PSM_TUNE_EXPAND_TASKS_PER_LANE: *value = ctx->expand_tasks_per_lane; return PSM_OK;
         case PSM_TUNE_DFS_STRATEGY: *value = ctx->dfs_strategy; return PSM_OK;
         case PSM_TUNE_QUEUE_SLOTS: *value = ctx->queue_slots; return PSM_OK;
+        case PSM_TUNE_CLAIM_DIVISOR: *value = ctx->claim_div; return PSM_OK;
         default: return set_err(ctx, PSM_EINVAL, "unknown tuning key");
     }
 }
@@ -2178,12 +2190,14 @@ int psm_evaluate_noalloc(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     DOp* dops = reinterpret_cast<DOp*>(scr + kScrOps);
     uint32_t* dseq = reinterpret_cast<uint32_t*>(scr + kScrSeq);
     DTask* dring = reinterpret_cast<DTask*>(scr + kScrRing);
+    DTask* dinit = reinterpret_cast<DTask*>(scr + kScrInit);
     // the header, counters and pending count start at zero (a fill kernel,
     // no memset node); the scratch keeps the last call's contents otherwise
     HIP_TRY(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(scr), 0u, kScrZeroEnd / 4, ctx->stream));
     DQueue hq{};
-    hq.tail = n_tasks;
+    hq.n0 = n_tasks;
     hq.mask = kScratchSlots - 1;
+    hq.claim_div = ctx->claim_div;
     if (ctx->time_limit_s > 0) {
         double left_s = ctx->time_limit_s;
         if (b.t_begin >= 0) left_s -= (now_ms() - b.t_begin) / 1000.0;
@@ -2194,16 +2208,17 @@ int psm_evaluate_noalloc(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     HIP_TRY(ctx, hipMemcpyAsync(dpend, &pend, 4, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(dplan, &P, sizeof(P), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(dops, hops.data(), hops.size() * sizeof(DOp), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, launch_root_tasks(dring, n_tasks, 0, roots, ctx->stream));
-    HIP_TRY(ctx, launch_ring_init(dseq, kScratchSlots, n_tasks, ctx->stream));
+    HIP_TRY(ctx, launch_root_tasks(dinit, n_tasks, 0, roots, ctx->stream));
+    HIP_TRY(ctx, launch_ring_init(dseq, kScratchSlots, 0u, ctx->stream));
     const uint32_t gw = (uint32_t)ctx->num_cu * ctx->dfs_waves_per_cu / 4 * 4;
     const double tk = now_ms();
     if (ctx->dfs_strategy == PSM_DFS_WAVE) {
         HIP_TRY(ctx, launch_dfs_wave(dplan, dops, b.row_off.as<uint64_t>(), b.idx.as<uint32_t>(),
-                                     b.cand_pool.as<uint32_t>(), dring, dseq, dq, dqc, dpend, P.n, gw, ctx->stream));
+                                     b.cand_pool.as<uint32_t>(), dinit, dring, dseq, dq, dqc, dpend, P.n, gw,
+                                     ctx->stream));
     } else {
         HIP_TRY(ctx, launch_dfs(dplan, dops, b.row_off.as<uint64_t>(), b.idx.as<uint32_t>(),
-                                b.cand_pool.as<uint32_t>(), dring, dseq, dq, dqc, dpend, P.n, gw,
+                                b.cand_pool.as<uint32_t>(), dinit, dring, dseq, dq, dqc, dpend, P.n, gw,
                                 EmitSink{nullptr, nullptr, 0, nullptr}, ctx->stream));
     }
     QCounters qc{};

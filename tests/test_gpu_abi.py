@@ -30,9 +30,13 @@ def test_tuning_round_trip_and_rejection(dg):
     assert before["dfs_strategy"] in ("lane", "wave")
     cfg = {"dfs_strategy": "wave", "dfs_waves_per_cu": 6, "expand_tasks_per_lane": 3.0}
     autotune.set_config(dg.dev, cfg)
-    assert autotune.get_config(dg.dev) == {**cfg, "dfs_waves_per_cu": 6.0}
+    got = autotune.get_config(dg.dev)
+    assert {k: got[k] for k in cfg} == {**cfg, "dfs_waves_per_cu": 6.0}
+    autotune.set_config(dg.dev, {"claim_divisor": 3})
+    assert autotune.get_config(dg.dev)["claim_divisor"] == 3.0
     for key, bad in ((L.PSM_TUNE_DFS_WAVES_PER_CU, 0), (L.PSM_TUNE_DFS_WAVES_PER_CU, 33),
-                     (L.PSM_TUNE_EXPAND_TASKS_PER_LANE, 0.0), (L.PSM_TUNE_DFS_STRATEGY, 7), (99, 1)):
+                     (L.PSM_TUNE_EXPAND_TASKS_PER_LANE, 0.0), (L.PSM_TUNE_DFS_STRATEGY, 7),
+                     (L.PSM_TUNE_CLAIM_DIVISOR, 0), (L.PSM_TUNE_QUEUE_SLOTS, -1), (99, 1)):
         assert L.lib().psm_set_tuning(h, key, float(bad)) == L.PSM_EINVAL
     v = ctypes.c_double()
     assert L.lib().psm_get_tuning(h, 99, ctypes.byref(v)) == L.PSM_EINVAL

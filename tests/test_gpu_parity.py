@@ -171,10 +171,10 @@ def test_search_bytes_clique_closed_form_on_device(dev):
 
 
 def test_tiny_task_ring_keeps_golden_counts(dev, hprd):
-    """A task ring of 128 slots (64 pre-expanded tasks, 64 free slots): split-off
-    work wraps the ring many times over, slots are reused as they are
-    consumed, and donations wait when the next slot is still held; counts and
-    call counts still equal the golden records, batched and plain."""
+    """A ring of 64 slots for split-off work (and 64 pre-expanded tasks per
+    query): the ring wraps many times over, slots are reused as they are
+    consumed, and splitting waits when the next slot is still held; counts
+    and call counts still equal the golden records, batched and plain."""
     from psm_benchmark_amd import autotune
     from psm_benchmark_amd import _lib as L
     from psm_benchmark_amd.matcher import batch_match, match
