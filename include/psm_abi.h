@@ -179,6 +179,27 @@ int psm_memory_cost_bytes(const psm_query* q, uint64_t* bytes);
  * entries each (either may be NULL). */
 int psm_generate_plan(psm_query* q, int order_method, uint32_t* order, uint32_t* pivot);
 
+/* A caller-supplied matching order (and pivots: pivot[i] an earlier query
+ * neighbour of order[i], i >= 1; NULL for none), as the reference's engines
+ * take UnitArgs::order / pivots from whichever plan generator ran
+ * (EvaluateQuery.cpp:1594-1666 derives the backward neighbours from them).
+ * Extensions at each depth are the intersection of the table rows of every
+ * earlier neighbour, so the pivot never changes counts. Needs
+ * psm_build_tables; PSM_EINVAL unless order is a permutation of 0..n-1. */
+int psm_set_plan(psm_query* q, const uint32_t* order, const uint32_t* pivot);
+
+/* The BFS tree a CFL or DPiso step used (GraphOperations::bfsTraversal,
+ * graphoperations.cpp:156-195): PSM_TREE_FILTER the one the last CFL / DPiso
+ * filter built on the query graph (FilterVertices::CFLFilter's order and
+ * tree, FilterVertices.h:15-18), PSM_TREE_PLAN the one the last CFL / DPiso
+ * order built on the query unit (GenerateFilteringPlan::generateCFLFilterPlan
+ * (data, UnitArgs*), GenerateFilteringPlan.h:15). n entries each: BFS order,
+ * parent (the root's is 0, as in the reference), level. PSM_ESTATE when
+ * there is none. */
+#define PSM_TREE_FILTER 0
+#define PSM_TREE_PLAN 1
+int psm_query_bfs_tree(const psm_query* q, int which, uint32_t* order, uint32_t* parent, uint32_t* level);
+
 /* EvaluateQuery::ParallelExecute with the GQL / EXPLORE engines
  * (EvaluateQuery.cpp:26-112,980-1126): counts embeddings, at most `cap`
  * (0 = no cap). Returns PSM_ETIMEOUT with the partial count when the time

@@ -99,6 +99,8 @@ PSM_TUNE_EXPAND_TASKS_PER_LANE = 1
 PSM_TUNE_DFS_STRATEGY = 2
 PSM_TUNE_QUEUE_SLOTS = 3
 PSM_TUNE_CLAIM_DIVISOR = 4
+PSM_TREE_FILTER = 0
+PSM_TREE_PLAN = 1
 PSM_DFS_LANE = 0
 PSM_DFS_WAVE = 1
 SPLITS = {"STAR": 0, "TWINTWIG": 1, "SEED": 2}
@@ -125,6 +127,8 @@ SIGNATURES = {
     "psm_query_table": (ctypes.c_int, [VP, ctypes.c_uint32, ctypes.c_uint32, U32P, U32P, U64P]),
     "psm_memory_cost_bytes": (ctypes.c_int, [VP, U64P]),
     "psm_generate_plan": (ctypes.c_int, [VP, ctypes.c_int, U32P, U32P]),
+    "psm_set_plan": (ctypes.c_int, [VP, U32P, U32P]),
+    "psm_query_bfs_tree": (ctypes.c_int, [VP, ctypes.c_int, U32P, U32P, U32P]),
     "psm_evaluate": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
     "psm_evaluate_noalloc": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(EnumStats)]),
     "psm_join_count": (ctypes.c_int, [VP, ctypes.c_uint64, ctypes.POINTER(JoinStats)]),

@@ -381,6 +381,9 @@ struct QueryHost {
     bool capped = false;
     bool timed_out = false;           // partial count: the reference's "Overtime: 1"
     double done_ms = -1;              // host clock (now_ms) when the count became final
+    CflTree filter_tree;              // BFS tree of the last CFL / DPiso filter (query graph)
+    CflTree plan_tree;                // BFS tree of the last CFL / DPiso order (query unit)
+    bool has_filter_tree = false, has_plan_tree = false;
 };
 
 constexpr uint64_t kNoTable = ~0ull;
@@ -975,6 +978,8 @@ int Batch::cfl_filter_query(QueryHost* q, hipStream_t st, DBuf& cnt, uint32_t& e
         PSM_TRY(cfl_filter_plan(qg, g->L, g->h_ri_off, g->max_freq,
                                 [&](uint32_t u, uint32_t& c) { return nlf_of(u, c); }, t));
     }
+    q->filter_tree = t;
+    q->has_filter_tree = true;
     // per-vertex candidate lists live in their own device buffers while
     // the passes run
     std::vector<std::unique_ptr<DBuf>> cb(n);
@@ -1198,6 +1203,7 @@ int Batch::make_plans(int order_method) {
         for (QueryHost* q : qs) {
             gql_order(q->qg, q->cand_count, g->V, q->plan);
             backward_neighbours(q->qg, q->plan, false);
+            q->has_plan_tree = false;
             q->planned = true;
         }
     } else if (order_method == PSM_ORDER_CFL) {
@@ -1219,6 +1225,8 @@ int Batch::make_plans(int order_method) {
                              std::vector<uint32_t>& ix) { return host_table(*q, a, b, off, ix, st); };
             PSM_TRY(cfl_order(ug, t, q->cand_count, table, q->plan));
             backward_neighbours(q->qg, q->plan, true);
+            q->plan_tree = t;
+            q->has_plan_tree = true;
             q->planned = true;
             return PSM_OK;
         }, nw));
@@ -1236,6 +1244,8 @@ int Batch::make_plans(int order_method) {
             }, t));
             dpiso_order(t, q->plan);
             backward_neighbours(q->qg, q->plan, false);
+            q->plan_tree = t;
+            q->has_plan_tree = true;
             q->planned = true;
             return PSM_OK;
         }, nw));
@@ -1894,6 +1904,49 @@ int psm_memory_cost_bytes(const psm_query* q, uint64_t* bytes) {
                      q->qh.tab_nnz[(size_t)b * n + a] + q->qh.cand_count[b];
         }
     *bytes = w * 4;
+    return PSM_OK;
+}
+
+int psm_set_plan(psm_query* q, const uint32_t* order, const uint32_t* pivot) {
+    if (!q || !order) return PSM_EINVAL;
+    QueryHost& h = q->qh;
+    const uint32_t n = h.qg.n;
+    if (!q->batch.tables_built) return set_err(q->ctx, PSM_ESTATE, "build tables before setting a plan");
+    std::vector<uint8_t> seen(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (order[i] >= n || seen[order[i]]) return set_err(q->ctx, PSM_EINVAL, "order is not a permutation");
+        seen[order[i]] = 1;
+    }
+    h.plan.order.assign(order, order + n);
+    h.plan.pivot.assign(n, 0);
+    if (pivot) {
+        for (uint32_t i = 1; i < n; ++i) {
+            const uint32_t p = pivot[i];
+            bool earlier = false;
+            for (uint32_t e = 0; e < i; ++e) earlier |= order[e] == p;
+            if (p >= n || !earlier || !h.qg.has_edge(p, order[i]))
+                return set_err(q->ctx, PSM_EINVAL, "pivot " + std::to_string(i) + " is not an earlier neighbour");
+            h.plan.pivot[i] = p;
+        }
+    }
+    backward_neighbours(h.qg, h.plan, false);
+    h.has_plan_tree = false;
+    h.planned = true;
+    return PSM_OK;
+}
+
+int psm_query_bfs_tree(const psm_query* q, int which, uint32_t* order, uint32_t* parent, uint32_t* level) {
+    if (!q || (which != PSM_TREE_FILTER && which != PSM_TREE_PLAN)) return PSM_EINVAL;
+    const QueryHost& h = q->qh;
+    const bool has = which == PSM_TREE_FILTER ? h.has_filter_tree : h.has_plan_tree;
+    if (!has) return set_err(q->ctx, PSM_ESTATE, "no BFS tree: run a CFL or DPiso filter / order first");
+    const CflTree& t = which == PSM_TREE_FILTER ? h.filter_tree : h.plan_tree;
+    const uint32_t n = h.qg.n;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (order) order[i] = t.order[i];
+        if (parent) parent[i] = t.parent[i];
+        if (level) level[i] = t.level[i];
+    }
     return PSM_OK;
 }
 

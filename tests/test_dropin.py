@@ -1,10 +1,15 @@
-"""The C++ drop-in layer (psm_benchmark_amd/dropin): the reference's class
-names over the C ABI and a SubgraphMatching.out driver whose log has the
-reference's keys (parsed with the same parser as the reference's logs)."""
+"""The C++ drop-in layer (psm_benchmark_amd/dropin): the reference's classes
+and signatures over the C ABI, and a SubgraphMatching.out driver that makes
+split_C_test's calls (StudyPerformance.cpp:1164-1716) and writes the
+reference's log (parsed with the same parser as the reference's logs).
+Host-side pieces (binary graph files against the reference's own writer,
+set intersection, units and backward neighbours) are checked on the CPU by
+tests/model/dropin_host_test.cpp."""
 import json
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 import ref_runner as R
@@ -12,6 +17,82 @@ import ref_runner as R
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "psm_benchmark_amd", "SubgraphMatching.out")
 HPRD = os.path.join(ROOT, "tests", "golden", "hprd")
+LIBDIR = os.path.join(ROOT, "psm_benchmark_amd")
+REF_WRITER = os.path.join(ROOT, "oracle", "_ref", "EdgeListToCSR.out")
+
+
+@pytest.fixture(scope="module")
+def host_test(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("dropin") / "dropin_host_test")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
+                           "-I" + os.path.join(LIBDIR, "dropin"), os.path.join(ROOT, "tests", "model",
+                                                                                "dropin_host_test.cpp"),
+                           "-L" + LIBDIR, "-lpsm_dropin", "-lpsm", "-Wl,-rpath," + LIBDIR, "-o", out])
+    return out
+
+
+def _load_bin(host_test, prefix):
+    out = subprocess.run([host_test, "load_bin", prefix], capture_output=True, text=True, check=True).stdout
+    lines = out.strip().splitlines()
+    V, E = map(int, lines[0].split())
+    rows = [list(map(int, l.split())) for l in lines[1:]]
+    return V, E, {r[0]: (r[1], r[2:]) for r in rows}
+
+
+@pytest.mark.skipif(not os.access(REF_WRITER, os.X_OK), reason="the reference's writer is built here only")
+def test_binary_reader_reads_the_reference_writer(host_test, tmp_path):
+    """matching/EdgeListToCSR.cpp:77-121 writes the deg / adj / label triple
+    that graph.cpp:221-403 reads; the drop-in's reader loads it into the
+    same graph as the edge list (labels are random in the writer: range
+    checked)."""
+    rng = np.random.default_rng(3)
+    V = 300
+    edges = sorted({(int(a), int(b)) for a, b in rng.integers(0, V, (1500, 2)) if a < b})
+    V = max(b for _, b in edges) + 1
+    el = tmp_path / "g.txt"
+    el.write_text("".join(f"{a} {b}\n" for a, b in edges))
+    prefix = str(tmp_path / "g")
+    subprocess.run([REF_WRITER, str(el), prefix, "5"], check=True, capture_output=True)
+    nV, nE, rows = _load_bin(host_test, prefix)
+    assert (nV, nE) == (V, len(edges))
+    adj = {v: [] for v in range(V)}
+    for a, b in edges:
+        adj[a].append(b)
+        adj[b].append(a)
+    for v in range(V):
+        lab, nb = rows[v]
+        assert nb == sorted(adj[v]) and 0 <= lab < 5, v
+
+
+def test_binary_round_trip_and_rejections(host_test, tmp_path):
+    prefix = str(tmp_path / "h")
+    subprocess.run([host_test, "store_bin", os.path.join(HPRD, "HPRD-15.txt"), prefix], check=True)
+    from psm_benchmark_amd.graph_io import load_text
+    g = load_text(os.path.join(HPRD, "HPRD-15.txt"))
+    V, E, rows = _load_bin(host_test, prefix)
+    assert (V, E) == (g.num_vertices, g.num_edges)
+    off = g.offsets.astype(np.int64)
+    for v in (0, 1, 17, 4000, V - 1):
+        assert rows[v] == (int(g.labels[v]), [int(x) for x in g.neighbors[off[v]:off[v + 1]]])
+    # truncated adjacency, then degrees that do not sum to 2E
+    adj = open(prefix + "_adj.bin", "rb").read()
+    open(prefix + "_adj.bin", "wb").write(adj[:-8])
+    r = subprocess.run([host_test, "bad_bin", prefix], capture_output=True, text=True)
+    assert r.returncode == 0 and "ends early" in r.stdout
+    open(prefix + "_adj.bin", "wb").write(adj)
+    deg = bytearray(open(prefix + "_deg.bin", "rb").read())
+    deg[12] = (deg[12] + 1) % 256
+    open(prefix + "_deg.bin", "wb").write(bytes(deg))
+    r = subprocess.run([host_test, "bad_bin", prefix], capture_output=True, text=True)
+    assert r.returncode == 0 and "2E" in r.stdout
+
+
+def test_set_intersection_and_units(host_test):
+    for seed in (1, 2):
+        r = subprocess.run([host_test, "intersect", str(seed)], capture_output=True, text=True)
+        assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout
+    r = subprocess.run([host_test, "plan"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", (r.returncode, r.stderr)
 
 
 def test_driver_built_and_rejects_unsupported_modes():
@@ -41,6 +122,11 @@ def test_driver_log_matches_golden(tmp_path, qname):
         assert int(b["Call Count"]) == g["calls"]
         assert round(float(b["Memory cost (MB)"]) * 2**20 / 4) == g["mem_words"]
         assert b["Overtime"] == "0"
+        # log4Metrics (Args.cpp:456-515): one host thread drove the device
+        assert int(b["Thread Call Count"]) == g["calls"]
+        assert int(b["Thread Embedding Count"]) == g["embeddings"]
+        assert float(b["Thread Enum Time"]) >= float(b["Thread Execution Time"]) > 0
+        assert int(b["TaskNum"]) >= 1
 
 
 @pytest.mark.gpu

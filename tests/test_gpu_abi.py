@@ -64,3 +64,54 @@ def test_staged_batch_state_and_repeat(dg):
     b.close()
     assert c1.tolist() == c2.tolist() == [r["embeddings"] for r in recs]
     assert k1.tolist() == k2.tolist() == [r["calls"] for r in recs]
+
+
+def test_caller_supplied_plan_and_bfs_trees(dg):
+    """psm_set_plan: any order gives the golden count (extensions intersect
+    every earlier neighbour's row), bad orders and pivots are rejected;
+    psm_query_bfs_tree returns the CFL filter's and the CFL order's BFS
+    trees (a BFS order with parents earlier and adjacent, levels one more)."""
+    import json
+    import numpy as np
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.matcher import Query
+    rec = next(r for r in json.load(open(os.path.join(HPRD, "golden.json")))
+               if r["query"] == "Q10-3" and r["threads"] == 1 and r["filter"] == "CFL")
+    q = load_text(os.path.join(HPRD, "Q10-3.txt"))
+    run = Query(dg, q)
+    try:
+        run.filter("CFL")
+        run.build_tables()
+        n = q.num_vertices
+        lib = L.lib()
+        order, pivot = run.generate_plan("CFL")
+        off = q.offsets.astype(np.int64)
+        for which in (L.PSM_TREE_FILTER, L.PSM_TREE_PLAN):
+            bo, par, lev = (np.zeros(n, np.uint32) for _ in range(3))
+            assert lib.psm_query_bfs_tree(run.h, which, L.u32p(bo), L.u32p(par), L.u32p(lev)) == L.PSM_OK
+            assert sorted(bo.tolist()) == list(range(n))
+            pos = {int(v): i for i, v in enumerate(bo)}
+            assert lev[bo[0]] == 0
+            for v in bo[1:]:
+                p = int(par[v])
+                assert pos[p] < pos[int(v)] and int(v) in q.neighbors[off[p]:off[p + 1]]
+                assert lev[v] == lev[p] + 1
+        assert run.evaluate().calls == rec["calls"]
+        assert lib.psm_set_plan(run.h, L.u32p(np.arange(n, dtype=np.uint32)), None) == L.PSM_OK
+        assert run.evaluate().count == rec["embeddings"]
+        # a plan set by the caller has no BFS tree
+        z = np.zeros(n, np.uint32)
+        assert lib.psm_query_bfs_tree(run.h, L.PSM_TREE_PLAN, L.u32p(z), None, None) == L.PSM_ESTATE
+        rev = np.ascontiguousarray(order[::-1])
+        assert lib.psm_set_plan(run.h, L.u32p(rev), None) == L.PSM_OK
+        assert run.evaluate().count == rec["embeddings"]
+        o, pv = np.ascontiguousarray(order), np.ascontiguousarray(pivot)
+        assert lib.psm_set_plan(run.h, L.u32p(o), L.u32p(pv)) == L.PSM_OK
+        st = run.evaluate()
+        assert (st.count, st.calls) == (rec["embeddings"], rec["calls"])
+        assert lib.psm_set_plan(run.h, L.u32p(np.zeros(n, np.uint32)), None) == L.PSM_EINVAL
+        badp = pv.copy()
+        badp[1] = order[-1]
+        assert lib.psm_set_plan(run.h, L.u32p(o), L.u32p(np.ascontiguousarray(badp))) == L.PSM_EINVAL
+    finally:
+        run.close()
