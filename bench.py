@@ -159,6 +159,9 @@ def main():
     ap.add_argument("--shape", default=None, metavar="STRATEGY,WAVES_PER_CU,TASKS_PER_WORKER",
                     help="set this launch shape instead of autotuning (e.g. lane,4,4)")
     ap.add_argument("--requests", type=int, default=1000, help="C4: requests per arrival rate")
+    ap.add_argument("--no-serving", action="store_true", help="C2: leave out the C4 serving leg")
+    ap.add_argument("--serving-plain", type=int, default=128, help="C2 line's C4 leg: plain-path queries")
+    ap.add_argument("--serving-shape", default="lane,4,2", help="C2 line's C4 leg: launch shape of the server")
     ap.add_argument("--workers", type=int, default=4, help="C4: server workers (HIP streams)")
     ap.add_argument("--max-batch", type=int, default=128, help="C4: requests per batch at most")
     ap.add_argument("--low-rate", type=float, default=10.0, help="C4: extra rate below capacity (QPS)")
@@ -369,6 +372,11 @@ def main():
         out["batched_over_cpu"] = round(gpu_sample["queries_per_s"] / cpu["value"], 1)
         out["cpu_baseline_nproc"] = cpu_n
         out["batched_over_cpu_nproc"] = round(gpu_sample["queries_per_s"] / cpu_n["value"], 1)
+    if not args.no_serving:
+        # the serving form on its own workload (C4), reported beside the line
+        dg.close()
+        out["serving"] = run_serving(args, dev=dev, plain_queries=args.serving_plain, shape=args.serving_shape,
+                                     tune=False)
     print(json.dumps(out))
 
 
@@ -607,13 +615,15 @@ def large_workload(args):
     print(json.dumps(out))
 
 
-def serving_workload(args):
+def run_serving(args, g=None, dev=None, plain_queries=None, shape=None, tune=True):
     """C4: the C3 graph resident in HBM; random-walk queries of 4-32 vertices
     (dense / sparse 50/50) arrive as seeded Poisson streams at 200, 1,000 and
     5,000 QPS into the continuous-batching server (memory-based admission,
-    cap 1e5, GQL_GQL_GQL). Reports p50/p99 latency (completion - arrival) and
+    cap 1e5, GQL_GQL_GQL); each result is returned as soon as its count is
+    final on the device. Reports p50/p99 latency (completion - arrival) and
     sustained queries/s per rate beside the plain one-query-at-a-time path,
-    and checks every served count against the plain count of the same query."""
+    and checks every served count against the plain count of the same query.
+    Returns the summary dict."""
     from psm_benchmark_amd import autotune
     from psm_benchmark_amd import workloads as W
     from psm_benchmark_amd import _lib as L
@@ -621,15 +631,16 @@ def serving_workload(args):
     from psm_benchmark_amd.serving import Server, run_poisson
 
     t0 = time.time()
-    g = W.data_graph("C4")
+    if g is None:
+        g = W.data_graph("C4")
     qs, meta = W.c2_queries(g, count=args.queries, seed=44)
     log(f"workload C4: V={g.num_vertices} E={g.num_edges} queries={len(qs)} ({time.time() - t0:.1f}s)")
-    dev = Device(0)
+    dev = dev or Device(0)
     dg = DeviceGraph(dev, g)
     tuned = None
-    if args.shape:
-        tuned = apply_shape(dev, args.shape)
-    elif not args.no_tune:
+    if shape:
+        tuned = apply_shape(dev, shape)
+    elif tune:
         tuned, _ = autotune.tune(dg, qs, "GQL", "GQL", args.cap, sample=min(len(qs), args.tune_sample),
                                  trial_limit_s=args.tune_limit, restore_limit_s=args.time_limit, log=log)
         log(f"autotune picked {tuned}")
@@ -638,7 +649,7 @@ def serving_workload(args):
     served_shape = autotune.get_config(dev)
     apply_shape(dev, PLAIN_SHAPE)
     lat, plain_counts, plain_to = [], [], []
-    for q in qs[: args.plain]:
+    for q in qs[: plain_queries if plain_queries is not None else args.plain]:
         t = time.perf_counter()
         r = match(dg, q, "GQL_GQL_GQL", cap=args.cap)
         lat.append(time.perf_counter() - t)
@@ -671,20 +682,27 @@ def serving_workload(args):
         summary["checked_against_plain"] = sum(1 for k in res if k % len(qs) < len(plain_counts))
         log(f"rate {rate}: {summary}")
         rates.append(summary)
+    dg.close()
     best = max(r["sustained_qps"] for r in rates[:3])
+    return {"workload": "C4 serving: Chung-Lu 4M V / 35M E resident, 32 Zipf(1.0) labels, random-walk queries "
+                        "of 4-32 vertices (dense/sparse 50/50), Poisson 200/1000/5000 QPS, GQL_GQL_GQL, cap 1e5",
+            "requests_per_rate": args.requests, "workers": args.workers, "max_batch": args.max_batch,
+            "rates": rates, "plain": plain, "best_sustained_qps": best,
+            "served_over_plain": round(best / plain["queries_per_s"], 2), "launch_shape": tuned}
+
+
+def serving_workload(args):
+    """The C4 serving workload alone (--workload C4)."""
+    sv = run_serving(args, shape=args.shape, tune=not args.no_tune)
     out = {
         "metric": "queries/s + embeddings/s on 1 MI355X; p50/p99 latency batched vs plain",
-        "value": best, "unit": "queries/s", "n_gpus": 1, "steps": len(rates), "warmup": 0,
-        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": sv["best_sustained_qps"], "unit": "queries/s", "n_gpus": 1, "steps": len(sv["rates"]),
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic (seeded graph, queries and Poisson arrivals)",
-        "config": {"workload": "C4 serving: Chung-Lu 4M V / 35M E resident, 32 Zipf(1.0) labels, random-walk "
-                               "queries of 4-32 vertices (dense/sparse 50/50), Poisson 200/1000/5000 QPS, "
-                               "GQL_GQL_GQL, cap 1e5",
-                   "requests_per_rate": args.requests, "workers": args.workers, "max_batch": args.max_batch,
-                   "cap": args.cap, "time_limit_s": args.time_limit},
-        "rates": rates, "plain": plain,
-        "served_over_plain": round(best / plain["queries_per_s"], 2),
-        "launch_shape": tuned, "cpu_baseline": None,
+        "config": {"workload": sv["workload"], "requests_per_rate": args.requests, "workers": args.workers,
+                   "max_batch": args.max_batch, "cap": args.cap, "time_limit_s": args.time_limit},
+        "rates": sv["rates"], "plain": sv["plain"], "served_over_plain": sv["served_over_plain"],
+        "launch_shape": sv["launch_shape"], "cpu_baseline": None,
     }
     print(json.dumps(out))
 

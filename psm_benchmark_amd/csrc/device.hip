@@ -1303,16 +1303,46 @@ __device__ __forceinline__ void seq_release(uint32_t* seq, uint32_t mask, uint32
     __hip_atomic_store(seq + (p & mask), p + mask + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Posts query q's final count to the host-visible notify words (one writer
+// per query): the values first, then the flag, with system-scope stores.
+__device__ __forceinline__ void post_final(unsigned long long* notify, QCounters* qc, uint32_t q,
+                                           unsigned long long count, unsigned long long calls,
+                                           unsigned long long rejects) {
+    if (atomicCAS(&qc[q].notified, 0u, 1u) != 0u) return;
+    unsigned long long* w = notify + 4ull * q;
+    __hip_atomic_store(w + 1, count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(w + 2, calls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(w + 3, rejects, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(w, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Query q's count reached its cap now (the first caller's time is kept).
-__device__ __forceinline__ void mark_final(QCounters* qc, uint32_t q) {
+__device__ __forceinline__ void mark_final(QCounters* qc, uint32_t q, unsigned long long cap,
+                                           unsigned long long* notify) {
     if (__hip_atomic_load(&qc[q].cap_tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
         atomicCAS(&qc[q].cap_tick, 0ull, (unsigned long long)wall_clock64());
+    if (notify) post_final(notify, qc, q, cap, 0ull, 0ull);
 }
 // Subtracts k ended tasks from pending[q] and records the time of the ends:
-// the query's last task end is the largest (no atomic here returns a value,
-// so the lane does not wait for these shared counters).
-__device__ __forceinline__ void pending_sub(uint32_t* pending, QCounters* qc, uint32_t q, uint32_t k) {
-    atomicSub(pending + q, k);
+// the query's last task end is the largest. Without notify words no atomic
+// here returns a value, so the lane does not wait for these shared counters.
+// With them, the counts the lane flushed before are released ahead of the
+// subtraction, and the lane that ends the query's last task reads the final
+// counts (acquire) and posts them.
+__device__ __forceinline__ void pending_sub(uint32_t* pending, QCounters* qc, uint32_t q, uint32_t k,
+                                            unsigned long long* notify) {
+    if (notify) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (atomicSub(pending + q, k) == k && !__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            post_final(notify, qc, q,
+                       __hip_atomic_load(&qc[q].count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __hip_atomic_load(&qc[q].calls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __hip_atomic_load(&qc[q].rejects, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        }
+    } else {
+        atomicSub(pending + q, k);
+    }
     atomicMax(&qc[q].end_tick, (unsigned long long)wall_clock64());
 }
 
@@ -1340,12 +1370,12 @@ constexpr uint32_t kShortTask = 1024; // steps below which a finished task count
 __device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t cap, unsigned long long& lcount,
                                            unsigned long long& lcalls, unsigned long long& lrej,
                                            unsigned long long& lbytes, unsigned long long& lalgo,
-                                           bool& near_cap) {
+                                           bool& near_cap, unsigned long long* notify) {
     if (lcount) {
         const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
         if (prev + lcount >= cap) {
             atomicExch(&qc[q].done, 1u);
-            mark_final(qc, q);
+            mark_final(qc, q, cap, notify);
         }
         near_cap = prev + lcount >= cap / 2;
         lcount = 0;
@@ -1394,6 +1424,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     const uint32_t qmask = Q->mask;
     const uint32_t n0 = Q->n0;
     const uint32_t claim_div = max(1u, Q->claim_div);
+    unsigned long long* const notify = Q->notify;
 
     bool active = false, waiting = false, wring = false;   // wring: the claimed positions are ring positions
     uint32_t wslot = 0, wend = 0, backoff = 0;
@@ -1509,8 +1540,8 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             const uint32_t tq = coherent_u32(&task->q);
             if (tq != q) {
                 // the lane's counters so far belong to the previous query
-                lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap);
-                if (pend_done) pending_sub(pending, qc, q, pend_done);
+                lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap, notify);
+                if (pend_done) pending_sub(pending, qc, q, pend_done, notify);
                 pend_done = 0;
             }
             q = tq;
@@ -1529,7 +1560,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 } else {
                     atomicSub(&Q->busy, busy_done);
                     busy_done = 0;
-                    if (pend_done) pending_sub(pending, qc, q, pend_done);
+                    if (pend_done) pending_sub(pending, qc, q, pend_done, notify);
                     pend_done = 0;
                 }
             } else {
@@ -1720,7 +1751,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         const bool more = !active && wslot + 1 < wend;   // another claimed slot follows
         const bool idle_next = !active && !more;
         if (lcount >= flush_at || idle_next || (steps & 1023) == 0)
-            lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap);
+            lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap, notify);
         if (finished) ++pend_done;
         if (!active) {
             last_steps = steps;
@@ -1731,12 +1762,12 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             } else {
                 atomicSub(&Q->busy, busy_done);
                 busy_done = 0;
-                if (pend_done) pending_sub(pending, qc, q, pend_done);
+                if (pend_done) pending_sub(pending, qc, q, pend_done, notify);
                 pend_done = 0;
             }
         }
     }
-    lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap);
+    lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap, notify);
     wave_add(&Q->lane_steps, busy_iters);
     wave_add(&Q->lane_iters, all_iters);
 }
@@ -1761,6 +1792,8 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
 // shallowest depth that has at least two positions left.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kWCache = 8;          // operand rows per member kept in LDS besides the base
+constexpr uint32_t kStage = 1024;        // entries of an operand row window staged in LDS
+constexpr uint32_t kCoopRow = 1024;      // operand rows at least this long are tested cooperatively
 constexpr uint64_t kFresh = ~0ull;       // generation cursor: start at the member's base
 
 // One wavefront's LDS, for queries of up to N vertices.
@@ -1788,7 +1821,51 @@ struct WaveFrame {
     uint32_t m_v[kWave];    // member data vertex
     uint64_t m_olo[kWCache][kWave];   // other operand rows: start
     uint32_t m_olen[kWCache][kWave];  //                     length
+    uint32_t stage[kStage];           // window of one operand row (cooperative test)
 };
+
+// First position p in idx[lo, lo + len) with idx[p] >= v, found by the whole
+// wavefront (every lane calls it with the same lo, len, v): each round the
+// 64 lanes probe 64 evenly spaced positions and a ballot keeps the one
+// interval that holds the answer, so a row of m entries takes
+// log64(m) rounds of independent loads instead of log2(m) dependent ones.
+__device__ __forceinline__ uint64_t wave_lower_bound(const uint32_t* __restrict__ idx, uint64_t lo, uint64_t len,
+                                                     uint32_t v, unsigned long long& bytes) {
+    const uint32_t lane = threadIdx.x;
+    while (len > kWave) {
+        const uint64_t stride = (len + kWave - 1) / kWave;
+        const uint64_t p = lo + (uint64_t)lane * stride;
+        const bool valid = p < lo + len;
+        const bool less = valid && idx[p] < v;
+        if (valid) bytes += 4;
+        const uint32_t c = (uint32_t)__popcll(__ballot(less));
+        if (c == 0) return lo;
+        const uint64_t left = lo + (uint64_t)(c - 1) * stride + 1;
+        const uint64_t right = min(lo + (uint64_t)c * stride, lo + len);
+        lo = left;
+        len = right - left;
+    }
+    const bool valid = lane < len;
+    const bool less = valid && idx[lo + lane] < v;
+    if (valid) bytes += 4;
+    return lo + (uint32_t)__popcll(__ballot(less));
+}
+
+// Is x in the sorted stage[0, n)?
+__device__ __forceinline__ bool stage_has(const uint32_t* st, uint32_t n, uint32_t x) {
+    const uint32_t end = n;
+    uint32_t lo = 0;
+    while (n > 0) {
+        const uint32_t half = n >> 1;
+        if (st[lo + half] < x) {
+            lo += half + 1;
+            n -= half + 1;
+        } else {
+            n = half;
+        }
+    }
+    return lo < end && st[lo] == x;
+}
 
 // Is x in the sorted row idx[lo, lo + len)? (lower bound, then compare)
 __device__ __forceinline__ bool row_has(const uint32_t* __restrict__ idx, uint64_t lo, uint64_t len,
@@ -1935,6 +2012,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
     }
     const uint32_t qmask = Q->mask;
     const uint32_t n0 = Q->n0;
+    unsigned long long* const notify = Q->notify;
     const uint32_t hungry_below = gridDim.x / 2;   // queue shorter than half the wavefronts
     unsigned long long bytes = 0, algo = 0, busy_pos = 0, all_pos = 0;
 
@@ -2068,9 +2146,10 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             // ---- one chunk of 64 positions of generation g
             const uint64_t fpos = flat + lane;
             bool ok = false, rej = false;
-            uint32_t xx = 0, k = 0;
+            uint32_t xx = 0, k = 0, sel = kIdentity;
+            const bool act = fpos < total;
             all_pos += 1;
-            if (fpos < total) {
+            if (act) {
                 ++busy_pos;
                 uint32_t lo = 0, hi = cnt;
                 while (hi - lo > 1) {
@@ -2080,7 +2159,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                 }
                 k = lo;
                 const uint64_t pos = S.m_lo[k] + (fpos - S.m_pre[k]);
-                const uint32_t sel = S.m_sel[k];
+                sel = S.m_sel[k];
                 if (sel == kIdentity) {
                     xx = (uint32_t)pos;
                 } else {
@@ -2088,11 +2167,56 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                     bytes += 4;
                 }
                 ok = true;
-                const uint32_t nops = P.op_count[g];
-                const uint32_t nother = sel == kIdentity ? 0u : nops - 1;
-                const uint32_t ncache = min(nother, kWCache);
+            }
+            // When every active lane holds a position of one member (a long
+            // base list), its cached operand rows are tested cooperatively:
+            // the elements ascend with the lane, so only the window of each
+            // row between the chunk's smallest and largest element can hold
+            // them; the wavefront locates it (wave_lower_bound), stages it in
+            // LDS when it fits and every lane searches LDS. Chunks spread
+            // over several members test lane by lane.
+            const unsigned long long am = __ballot(act);
+            const uint32_t k0 = __shfl(k, 0);   // lane 0 is active whenever any lane is
+            const bool uni = am != 0 && __ballot(act && k != k0) == 0;
+            const uint32_t nops = P.op_count[g];
+            const uint32_t nother = sel == kIdentity ? 0u : nops - 1;
+            const uint32_t ncache = min(nother, kWCache);
+            const uint32_t sel0 = __shfl(sel, 0);
+            const uint32_t ncache0 = sel0 == kIdentity ? 0u : min(nops - 1, kWCache);   // wave-uniform
+            if (uni && ncache0 > 0) {
+                const uint32_t last = 63u - (uint32_t)__clzll(am);
+                const uint32_t xmin = __shfl(xx, 0), xmax = __shfl(xx, last);
+                for (uint32_t j = 0; j < ncache0; ++j) {
+                    if (__ballot(ok) == 0) break;
+                    const uint64_t rlo = S.m_olo[j][k0];
+                    const uint64_t rlen = S.m_olen[j][k0];
+                    if (rlen < kCoopRow) {
+                        // a short row: each lane's binary search is a few
+                        // cached probes, cheaper than locating a window
+                        if (ok) ok = row_has(idx, rlo, rlen, xx, bytes);
+                        continue;
+                    }
+                    const uint64_t wlo = wave_lower_bound(idx, rlo, rlen, xmin, bytes);
+                    const uint64_t whi = xmax == 0xFFFFFFFFu ? rlo + rlen
+                                                              : wave_lower_bound(idx, wlo, rlo + rlen - wlo, xmax + 1u, bytes);
+                    const uint64_t wl = whi - wlo;
+                    if (wl == 0) {
+                        ok = false;
+                    } else if (wl <= kStage) {
+                        for (uint32_t i = lane; i < (uint32_t)wl; i += kWave) S.stage[i] = idx[wlo + i];
+                        bytes += 4ull * (lane < wl ? (wl - lane + kWave - 1) / kWave : 0);
+                        __syncthreads();
+                        if (ok) ok = stage_has(S.stage, (uint32_t)wl, xx);
+                        __syncthreads();
+                    } else if (ok) {
+                        ok = row_has(idx, wlo, wl, xx, bytes);
+                    }
+                }
+            } else if (act) {
                 for (uint32_t j = 0; j < ncache && ok; ++j)
                     ok = row_has(idx, S.m_olo[j][k], S.m_olen[j][k], xx, bytes);
+            }
+            if (act) {
                 if (ok && nother > kWCache) {
                     uint32_t slot2 = 0;
                     for (uint32_t j = 0; j < nops && ok; ++j) {
@@ -2179,7 +2303,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                         atomicAdd(&qc[q].calls, lcalls);
                         if (lcount && prev + lcount >= P.cap) {
                             atomicExch(&qc[q].done, 1u);
-                            mark_final(qc, q);
+                            mark_final(qc, q, P.cap, notify);
                         }
                     }
                     if (lrej) atomicAdd(&qc[q].rejects, lrej);
@@ -2256,11 +2380,11 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                 atomicAdd(&qc[q].calls, lcalls);
                 if (lcount && prev + lcount >= P.cap) {
                     atomicExch(&qc[q].done, 1u);
-                    mark_final(qc, q);
+                    mark_final(qc, q, P.cap, notify);
                 }
             }
             if (lrej) atomicAdd(&qc[q].rejects, lrej);
-            if (finished) pending_sub(pending, qc, q, 1u);
+            if (finished) pending_sub(pending, qc, q, 1u, notify);
             atomicSub(&Q->busy, 1u);
         }
         wave_add(&qc[q].probe_bytes, bytes);

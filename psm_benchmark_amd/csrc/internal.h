@@ -149,6 +149,11 @@ struct DQueue {
     unsigned int head0;        // next task of the pre-expanded array
     unsigned int n0;           // pre-expanded tasks
     unsigned int claim_div;    // array tasks per lane above which a lane takes several per claim
+    // Host-visible (coherent pinned) words, 4 per query, or null: once a
+    // query's count is final the kernel writes its count, calls and visited
+    // rejects to words 1..3 and then 1 to word 0, so a caller can return it
+    // while the launch still runs.
+    unsigned long long* notify;
 };
 
 // Where the DFS writes complete embeddings (rows of n data vertex ids in
@@ -191,6 +196,8 @@ struct QCounters {
     // final at the first, or else at the second
     unsigned long long cap_tick;
     unsigned long long end_tick;
+    uint32_t notified;    // the final count has been posted to DQueue::notify
+    uint32_t pad;
 };
 
 // Kernel classes timed by the library (psm_kernel_stats).

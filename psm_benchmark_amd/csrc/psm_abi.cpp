@@ -428,6 +428,10 @@ struct Batch {
     // clock when its first block started (maps done_tick to host time)
     double dfs_launch_ms = 0;
     unsigned long long dfs_start_tick = 0;
+    // serving: host-visible completion words of the queries (4 per query,
+    // see DQueue::notify) and a callback run while the DFS launch is in flight
+    unsigned long long* notify = nullptr;
+    std::function<void()> poll;
 
     hipStream_t s() const { return ctx->stream; }
 
@@ -1399,6 +1403,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         hq.n0 = (unsigned int)n_tasks;
         hq.mask = (unsigned int)(ring - 1);
         hq.claim_div = ctx->claim_div;
+        hq.notify = sink.rows ? nullptr : notify;
         if (enum_limit_s > 0) {
             const double left_s = enum_limit_s - (now_ms() - t_eval) / 1000.0;
             hq.limit_ticks = (unsigned long long)std::max(1.0, left_s * ctx->wall_khz * 1000.0);
@@ -1430,6 +1435,19 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
                              d_seq.as<uint32_t>(),
                              d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
                              sink, s()));
+        }
+        if (poll && hq.notify) {
+            // results of queries that finish early are taken while the launch runs
+            hipEvent_t ev = nullptr;
+            HIP_TRY(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            HIP_TRY(ctx, hipEventRecord(ev, s()));
+            hipError_t qe;
+            while ((qe = hipEventQuery(ev)) == hipErrorNotReady) {
+                poll();
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+            }
+            (void)hipEventDestroy(ev);
+            if (qe != hipSuccess) return hip_fail(ctx, qe, "hipEventQuery", __LINE__);
         }
         HIP_TRY(ctx, hipMemcpyAsync(&hq, d_q.p, sizeof(hq), hipMemcpyDeviceToHost, s()));
         HIP_TRY(ctx, timers_flush(ctx));
@@ -2513,13 +2531,25 @@ constexpr size_t kSmallBatch = 4;   // batches below this size use the default l
 
 void serve_worker(psm_server* S, psm_ctx* ctx) {
     (void)hipSetDevice(S->device);
+    // coherent pinned words the DFS kernel posts finished queries to (4 per
+    // request of a batch), so each result leaves as soon as it is final
+    unsigned long long* words = nullptr;
+    unsigned long long* dwords = nullptr;
+    if (hipHostMalloc((void**)&words, (size_t)S->max_batch * 32, hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&dwords, words, 0) != hipSuccess) {
+        if (words) (void)hipHostFree(words);
+        words = dwords = nullptr;   // results then leave when their batch ends
+    }
     while (true) {
         std::vector<Request> batch;
         uint64_t est = 0;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
-            if (S->pending.empty()) return;   // stopping
+            if (S->pending.empty()) {   // stopping
+                if (words) (void)hipHostFree(words);
+                return;
+            }
             // admission: oldest first, while slots and memory allow (a request
             // that fits nowhere else runs alone once the device is idle)
             while (!S->pending.empty() && batch.size() < S->max_batch) {
@@ -2554,28 +2584,55 @@ void serve_worker(psm_server* S, psm_ctx* ctx) {
             ctx->expand_tasks_per_lane = kDefaultTasksPerLane;
             ctx->dfs_strategy = PSM_DFS_LANE;
         }
+        std::vector<uint8_t> delivered(batch.size(), 0);
+        auto deliver = [&](size_t i, uint64_t count, uint64_t calls, int status) {
+            psm_result res{};
+            res.request_id = batch[i].id;
+            res.count = count;
+            res.calls = calls;
+            res.status = status;
+            res.batch_size = (uint32_t)batch.size();
+            res.submit_ms = batch[i].submit_ms;
+            res.start_ms = start;
+            res.done_ms = now_ms() - S->t0;
+            S->done.push_back(res);
+            ++S->completed;
+            delivered[i] = 1;
+        };
+        if (words) {
+            memset(words, 0, batch.size() * 32);
+            b.notify = dwords;
+            b.poll = [&] {
+                bool any = false;
+                for (size_t i = 0; i < batch.size(); ++i) {
+                    if (delivered[i] || !__atomic_load_n(words + 4 * i, __ATOMIC_ACQUIRE)) continue;
+                    const uint64_t c = __atomic_load_n(words + 4 * i + 1, __ATOMIC_RELAXED);
+                    const uint64_t k = __atomic_load_n(words + 4 * i + 2, __ATOMIC_RELAXED);
+                    if (!any) S->mu.lock();
+                    any = true;
+                    deliver(i, S->cap ? std::min<uint64_t>(S->cap, c) : c, k, PSM_OK);
+                }
+                if (any) {
+                    S->mu.unlock();
+                    S->cv_done.notify_all();
+                }
+            };
+        }
         int rc = b.filter(S->filter);
         if (rc == PSM_OK) rc = b.build_tables();
         if (rc == PSM_OK) rc = b.make_plans(S->order);
         if (rc == PSM_OK) rc = b.evaluate(S->cap);
+        b.poll = nullptr;
         ctx->dfs_waves_per_cu = tuned_waves;
         ctx->expand_tasks_per_lane = tuned_expand;
         ctx->dfs_strategy = tuned_strategy;
-        const double end = now_ms() - S->t0;
         std::lock_guard<std::mutex> lk(S->mu);
-        for (Request& r : batch) {
-            psm_result res{};
-            res.request_id = r.id;
-            res.count = r.q->count;
-            res.calls = r.q->calls;
-            res.status = rc != PSM_OK ? rc : (r.q->timed_out ? PSM_ETIMEOUT : r.q->status);
-            res.batch_size = (uint32_t)batch.size();
-            res.submit_ms = r.submit_ms;
-            res.start_ms = start;
-            res.done_ms = end;
-            S->done.push_back(res);
-            ++S->completed;
+        for (size_t i = 0; i < batch.size(); ++i) {
+            if (delivered[i]) continue;
+            const QueryHost* q = batch[i].q.get();
+            deliver(i, q->count, q->calls, rc != PSM_OK ? rc : (q->timed_out ? PSM_ETIMEOUT : q->status));
         }
+        // the batch's device buffers (candidates, tables, queue) are freed now
         S->mem_in_use -= est;
         S->cv_done.notify_all();
     }
