@@ -137,17 +137,19 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, kernel, ou
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="C2")
     ap.add_argument("--cap", type=int, default=100_000)
     ap.add_argument("--plain", type=int, default=512, help="queries timed through the plain path")
-    ap.add_argument("--cpu-stride", type=int, default=33,
+    ap.add_argument("--cpu-stride", type=int, default=17,
                     help="C2: the CPU baseline runs every N-th query (and the GPU is timed on the same ones)")
     ap.add_argument("--cpu-limit", type=float, default=60.0,
                     help="per-query time limit of the CPU baseline (s); a query that hits it is a timeout")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", type=int, default=512, help="use the first N workload queries")
+    ap.add_argument("--pipeline", type=int, default=2,
+                    help="C2: batches in flight at once, each on its own context and HIP stream")
     ap.add_argument("--time-limit", type=float, default=None,
                     help="time limit in seconds (default none for C1 and C2, whose queries all "
                          "finish; 20 s for the uncapped C3 / C5 workloads)")
@@ -200,7 +202,7 @@ def main():
 
     from psm_benchmark_amd import autotune
     from psm_benchmark_amd import workloads as W
-    from psm_benchmark_amd.matcher import Batch, Device, DeviceGraph, match
+    from psm_benchmark_amd.matcher import Batch, Device, DeviceGraph, Pipeline, match
     from psm_benchmark_amd import _lib as L
 
     t0 = time.time()
@@ -226,7 +228,8 @@ def main():
         log(f"autotune picked {tuned} ({time.time() - t0:.1f}s)")
 
     def step(queries):
-        """One batched pass: create, filter + tables + plans, enumeration."""
+        """One batched pass on the graph's own context: create, filter +
+        tables + plans, enumeration."""
         t = time.perf_counter()
         b = Batch(dg, queries)
         try:
@@ -238,23 +241,34 @@ def main():
             b.close()
         return dt, counts, calls, status, st, lat
 
-    for _ in range(args.warmup):
-        dt, counts, calls, status, st, _ = step(qs)
-        log(f"warmup: {dt * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
+    # The timed steps: one batch of all the queries each, with --pipeline
+    # batches in flight on their own contexts (HIP streams), so one batch's
+    # filter and table launches run beside another's enumeration tail.
+    pipe = Pipeline(dg, args.pipeline, time_limit=args.time_limit)
+    for r in pipe.run([qs] * args.warmup, "GQL", "GQL", args.cap):
+        st = r.stats
+        log(f"warmup: {(r.end_s - r.start_s) * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
             f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f})")
-    L.lib().psm_kernel_stats_reset(dev.h)
-    times, stats, lats, nodes = [], [], [], 0
-    for _ in range(args.steps):
-        dt, counts, calls, status, st, lat = step(qs)
-        times.append(dt)
+    pipe.kernel_stats_reset()
+    t = time.perf_counter()
+    runs = pipe.run([qs] * args.steps, "GQL", "GQL", args.cap)
+    total = time.perf_counter() - t
+    stats, lats, nodes = [], [], 0
+    counts = runs[0].counts
+    for r in runs:
+        st = r.stats
         stats.append(st)
-        lats.append(lat)
-        nodes += int(calls.sum())
-        log(f"step: {dt * 1e3:.1f} ms (filter {st.filter_ms:.1f}, table {st.table_ms:.1f}, "
-            f"plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f}), timeouts {int((status == L.PSM_ETIMEOUT).sum())}")
-    ks = kernel_stats(dev)
-    timeouts = int((status == L.PSM_ETIMEOUT).sum())
-    total = sum(times)
+        lats.append(r.latencies_ms)
+        nodes += int(r.calls.sum())
+        if not np.array_equal(r.counts, counts):
+            raise SystemExit("two batches of the same queries gave different counts")
+        log(f"step: {(r.end_s - r.start_s) * 1e3:.1f} ms in flight (filter {st.filter_ms:.1f}, "
+            f"table {st.table_ms:.1f}, plan {st.plan_ms:.1f}, enum {st.enum_ms:.1f}), "
+            f"timeouts {int((r.status == L.PSM_ETIMEOUT).sum())}")
+    status = runs[-1].status
+    ks = kernel_stats(*pipe.devs)
+    pipe.close()
+    timeouts = int(max((r.status == L.PSM_ETIMEOUT).sum() for r in runs))
     nq = len(qs)
     emb = int(counts.sum())
     step_ms = total / args.steps * 1e3
@@ -342,7 +356,8 @@ def main():
         "data": "synthetic (seeded Chung-Lu graph and random-walk queries)",
         "config": {"workload": "C2 batched: Chung-Lu 1.1M V / 3.0M E, 100 Zipf(1.1) labels, "
                                "512 random-walk queries of 4-32 vertices (half dense), GQL_GQL_GQL, cap 1e5",
-                   "queries": nq, "cap": args.cap, "time_limit_s": args.time_limit or None},
+                   "queries": nq, "cap": args.cap, "time_limit_s": args.time_limit or None,
+                   "batches_in_flight": args.pipeline},
         "timeouts": timeouts,
         "completed_queries_per_s": round((nq - timeouts) * args.steps / total, 2),
         "embeddings_per_s": round(emb * args.steps / total, 1),
@@ -352,8 +367,9 @@ def main():
         "launch_shape": tuned,
         "latency_ms": {"batched_p50": round(float(np.percentile(lat_all, 50)), 3),
                        "batched_p99": round(float(np.percentile(lat_all, 99)), 3),
-                       "batched_definition": "per query: from the start of its batched step until its count "
-                                             "was final (cap reached or last task ended, device clock)",
+                       "batched_definition": "per query: from the start of its batched step (the batch's "
+                                             "creation on its context) until its count was final (cap "
+                                             "reached or last task ended, device clock)",
                        "plain_p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "plain_p99": round(float(np.percentile(lat_ms, 99)), 3)},
         "plain_gpu_queries_per_s": round(plain_qps, 2),
@@ -466,9 +482,13 @@ def plain_workload(args):
 
 
 def apply_shape(dev, text):
+    """STRATEGY,WAVES_PER_CU,TASKS_PER_WORKER[,BLOCK_LANES] (block lanes of the
+    lane DFS; 0 or absent: the library's choice)."""
     from psm_benchmark_amd import autotune
-    st, w, e = text.split(",")
-    cfg = {"dfs_strategy": st, "dfs_waves_per_cu": int(w), "expand_tasks_per_lane": float(e)}
+    f = text.split(",")
+    st, w, e = f[:3]
+    cfg = {"dfs_strategy": st, "dfs_waves_per_cu": int(w), "expand_tasks_per_lane": float(e),
+           "dfs_block_lanes": int(f[3]) if len(f) > 3 else 0}
     autotune.set_config(dev, cfg)
     return cfg
 
@@ -710,16 +730,21 @@ def serving_workload(args):
 KERNELS = ["gql_refine", "table_count", "table_fill", "dfs", "expand", "compact", "bitmap"]
 
 
-def kernel_stats(dev):
+def kernel_stats(*devs):
+    """Launches, HIP-event milliseconds and kernel-counted bytes per kernel
+    class, summed over the contexts."""
     import ctypes
 
     from psm_benchmark_amd import _lib as L
     out = {}
     for i, name in enumerate(KERNELS):
-        a = (ctypes.c_double * 3)()
-        L.lib().psm_kernel_stats(dev.h, i, a)
-        if a[0] > 0:
-            out[name] = dict(name=name, launches=int(a[0]), ms=float(a[1]), bytes=float(a[2]))
+        tot = [0.0, 0.0, 0.0]
+        for dev in devs:
+            a = (ctypes.c_double * 3)()
+            L.lib().psm_kernel_stats(dev.h, i, a)
+            tot = [x + y for x, y in zip(tot, a)]
+        if tot[0] > 0:
+            out[name] = dict(name=name, launches=int(tot[0]), ms=float(tot[1]), bytes=float(tot[2]))
     return out
 
 

@@ -122,12 +122,18 @@ int psm_set_time_limit(psm_ctx* ctx, double seconds);
  *                                   1..1024 (default 1): fewer claims for
  *                                   millions of small tasks, one task per
  *                                   claim for heavy ones
+ *   PSM_TUNE_DFS_BLOCK_LANES        lanes per workgroup of the lane DFS: 64,
+ *                                   128 or 256, or 0 for 64 (default 0);
+ *                                   halved while the block's LDS state
+ *                                   would not fit. Blocks of one wavefront
+ *                                   spread a small grid over every CU
  * No counterpart in the reference (its only knob is -threadnums). */
 #define PSM_TUNE_DFS_WAVES_PER_CU 0
 #define PSM_TUNE_EXPAND_TASKS_PER_LANE 1
 #define PSM_TUNE_DFS_STRATEGY 2
 #define PSM_TUNE_QUEUE_SLOTS 3
 #define PSM_TUNE_CLAIM_DIVISOR 4
+#define PSM_TUNE_DFS_BLOCK_LANES 5
 #define PSM_DFS_LANE 0
 #define PSM_DFS_WAVE 1
 int psm_set_tuning(psm_ctx* ctx, int key, double value);

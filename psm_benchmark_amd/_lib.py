@@ -99,6 +99,7 @@ PSM_TUNE_EXPAND_TASKS_PER_LANE = 1
 PSM_TUNE_DFS_STRATEGY = 2
 PSM_TUNE_QUEUE_SLOTS = 3
 PSM_TUNE_CLAIM_DIVISOR = 4
+PSM_TUNE_DFS_BLOCK_LANES = 5
 PSM_TREE_FILTER = 0
 PSM_TREE_PLAN = 1
 PSM_DFS_LANE = 0

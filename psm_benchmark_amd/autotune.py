@@ -38,6 +38,7 @@ KEYS = {
     "expand_tasks_per_lane": L.PSM_TUNE_EXPAND_TASKS_PER_LANE,
     "dfs_strategy": L.PSM_TUNE_DFS_STRATEGY,
     "claim_divisor": L.PSM_TUNE_CLAIM_DIVISOR,
+    "dfs_block_lanes": L.PSM_TUNE_DFS_BLOCK_LANES,
 }
 STRATEGIES = {"lane": L.PSM_DFS_LANE, "wave": L.PSM_DFS_WAVE}
 

@@ -1356,6 +1356,19 @@ __device__ __forceinline__ bool past_deadline(DQueue* Q) {
                __hip_atomic_load(&Q->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The launch's deadline in wall-clock ticks, 0 without a time limit: the
+// first wavefront to arrive sets it, every wavefront keeps the value in a
+// register (called by one lane).
+__device__ __forceinline__ unsigned long long launch_deadline(DQueue* Q, unsigned long long now) {
+    const unsigned long long lim = Q->limit_ticks;
+    if (!lim) return 0ull;
+    const unsigned long long old = atomicCAS(&Q->deadline, 0ull, now + lim);
+    return old ? old : now + lim;
+}
+__device__ __forceinline__ bool passed(unsigned long long deadline) {
+    return deadline && (unsigned long long)wall_clock64() > deadline;
+}
+
 // base elements one step of the lane DFS tests at most: the cap and queue
 // checks count steps, so a shorter step lets capped queries stop sooner
 // (C4 plain path 37 -> 44 queries/s from 32 to 16) at about 1 % of an
@@ -1415,12 +1428,15 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     st.stride = blockDim.x;
     st.lane = threadIdx.x;
 
-    if (threadIdx.x == 0) {
+    unsigned long long deadline = 0;
+    if (lane == 0) {
         const unsigned long long now = (unsigned long long)wall_clock64();
-        if (Q->limit_ticks) atomicCAS(&Q->deadline, 0ull, now + Q->limit_ticks);
-        if (__hip_atomic_load(&Q->start_tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        deadline = launch_deadline(Q, now);
+        if (threadIdx.x == 0 &&
+            __hip_atomic_load(&Q->start_tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
             atomicCAS(&Q->start_tick, 0ull, now);
     }
+    deadline = __shfl(deadline, 0);
     const uint32_t qmask = Q->mask;
     const uint32_t n0 = Q->n0;
     const uint32_t claim_div = max(1u, Q->claim_div);
@@ -1438,6 +1454,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     bool near_cap = false;                // the query's count was at half its cap or more at the last flush
     bool hungry = true;                   // wavefront view: the queue holds fewer tasks than idle lanes
     bool ring_short = true;               // wavefront view: the ring holds fewer than lanes / 8 tasks
+    bool few_left = false;                // the lane's query had fewer than kFewTasks tasks left
+    bool stop = false;                    // wavefront view: the time limit has passed
+    bool check = false;                   // this iteration checks the lanes' caps and the limit
     uint32_t witer = 0;
     uint32_t steps = 0;
     uint32_t last_steps = kShortTask;   // steps of the lane's previous task (none yet: counts as long)
@@ -1445,22 +1464,40 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     const uint32_t lanes_total = gridDim.x * blockDim.x;
 
     while (true) {
-        // every 16 iterations one lane reads the queue's state for the
-        // donation rule below; the wavefront is converged here. hungry: the
-        // ring and the array together hold fewer tasks than there are idle
-        // lanes; short: the ring holds fewer than an eighth of the lanes
+        // Shared state is read on the wavefront's iteration count, with the
+        // wavefront converged, never on a lane's own step count: lanes reach
+        // their step counts at different iterations, so per-lane checks issue
+        // one agent-scope load per lane, and with one query every one of them
+        // goes to the same L2 line. Every 16 iterations one lane reads the
+        // queue's state (hungry: the ring and the array together hold fewer
+        // tasks than there are idle lanes; ring_short: the ring holds fewer
+        // than an eighth of the lanes) and the time limit, and, while the
+        // ring is short, the running lanes read their queries' task counts
+        // for the donation rule (one load per distinct query). The lanes'
+        // caps are checked every 16 iterations while a lane's query is near
+        // its cap, otherwise every 256.
+        check = false;
         if ((witer++ & 15) == 0) {
             uint32_t h = 0;
             if (lane == 0) {
                 const uint32_t len = load_relaxed(&Q->tail) - load_relaxed(&Q->head);
                 const uint32_t h0 = load_relaxed(&Q->head0);
                 const uint32_t left0 = h0 < n0 ? n0 - h0 : 0u;
+                bool late = load_relaxed(&Q->timeout) != 0;
+                if (!late && passed(deadline)) {
+                    atomicExch(&Q->timeout, 1u);
+                    late = true;
+                }
                 h = (len + left0 < lanes_total - min(load_relaxed(&Q->busy), lanes_total) ? 1u : 0u) |
-                    (len < (lanes_total >> 3) ? 2u : 0u);
+                    (len < (lanes_total >> 3) ? 2u : 0u) | (late ? 4u : 0u);
             }
             h = __shfl(h, 0);
             hungry = (h & 1u) != 0;
             ring_short = (h & 2u) != 0;
+            stop = (h & 4u) != 0;
+            few_left = false;
+            if (!hungry && ring_short && active) few_left = load_relaxed(pending + q) < kFewTasks;
+            check = stop || (witer & 255u) == 1u || __ballot(active && near_cap) != 0;
         }
         // claim queued tasks for idle lanes, one atomic per wavefront (none
         // once the time limit has passed): split-off work from the ring
@@ -1473,7 +1510,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         // with millions of small tasks. Otherwise a lane takes one task per
         // claim: a lane holding several heavy subtrees would run them in
         // turn while other lanes wait.
-        const unsigned long long want = __ballot(!active && !waiting && load_relaxed(&Q->timeout) == 0);
+        const unsigned long long want = __ballot(!active && !waiting && !stop);
         // lanes whose previous task was short (small tasks, many claims)
         const unsigned long long quick = __ballot(!active && !waiting && last_steps < kShortTask);
         if (want && backoff == 0) {
@@ -1546,12 +1583,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
             q = tq;
             P = plans + q;
-            bool late = load_relaxed(&Q->timeout) != 0;
-            if (!late && past_deadline(Q)) {
-                atomicExch(&Q->timeout, 1u);
-                late = true;
-            }
-            if (late || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
+            if (stop || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
                 if (wring) seq_release(seq, qmask, wslot);
                 ++busy_done;
                 if (wslot + 1 < wend) {   // the lane's next claimed slot
@@ -1680,16 +1712,8 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
         }
         ++steps;
-        // the cap is checked every 32 steps once the query's count has
-        // reached half of it, otherwise with the time limit every 256
-        if (active && (steps & (near_cap ? 31u : 255u)) == 0) {
-            if (__atomic_load_n(&qc[q].done, __ATOMIC_RELAXED) || load_relaxed(&Q->timeout)) {
-                active = false;
-            } else if (past_deadline(Q)) {
-                atomicExch(&Q->timeout, 1u);
-                active = false;
-            }
-        }
+        // the lanes' caps and the time limit (see the top of the loop)
+        if (check && active && (stop || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED))) active = false;
         // donation: split the shallowest level that still has >= 2 entries
         // into the ring when it holds fewer tasks than there are lanes
         // without one (work goes to lanes that wait for it), or, while it is
@@ -1698,8 +1722,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         // work is claimed before fresh array tasks). While every lane is busy
         // with well-spread queries, splitting would only add claims and
         // records.
-        if (active && (steps & 15) == 0 &&
-            (hungry || (ring_short && load_relaxed(pending + q) < kFewTasks))) {
+        if (active && (steps & 15) == 0 && (hungry || few_left)) {
             uint32_t e = tdepth;
             uint64_t ce = 0, ee = 0;
             for (; e <= d; ++e) {
@@ -2909,27 +2932,30 @@ size_t lane_lds_bytes(uint32_t lanes, uint32_t max_n) {
     return ((size_t)lanes * max_n * kLaneStateBytes + 15) & ~(size_t)15;
 }
 
-// Lanes per block of the lane DFS: the largest of 256 / 128 / 64 whose LDS
-// state fits the device's per-block limit (gfx950: 160 KB).
-uint32_t dfs_lanes_per_block(uint32_t max_n) {
+// Lanes per block of the lane DFS: want (64 / 128 / 256; 0: 64) or less,
+// halved until the block's LDS state fits the device's per-block limit
+// (gfx950: 160 KB). One-wavefront blocks let the dispatcher spread a grid of
+// one wavefront per CU over every CU (four-wavefront blocks of the same grid
+// fill a quarter of them); a CU then holds as many of them as its LDS allows.
+uint32_t dfs_lanes_per_block(uint32_t max_n, uint32_t want) {
     int dev = 0, lim = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0)
         lim = 64 * 1024;
-    uint32_t lanes = kBlock;
+    uint32_t lanes = want ? std::min<uint32_t>(want, kBlock) : kWave;
     while (lanes > kWave && lane_lds_bytes(lanes, max_n) > (size_t)lim) lanes /= 2;
     return lanes;
 }
 
-size_t dfs_lane_lds_bytes(uint32_t max_n) { return lane_lds_bytes(dfs_lanes_per_block(max_n), max_n); }
+size_t dfs_lane_lds_bytes(uint32_t max_n) { return lane_lds_bytes(dfs_lanes_per_block(max_n, 0), max_n); }
 
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
                       const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue, uint32_t* seq,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
-                      EmitSink sink, hipStream_t s) {
+                      uint32_t block_lanes, EmitSink sink, hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
     if (max_n == 0 || max_n > kMaxQ) return hipErrorInvalidValue;
-    const uint32_t lanes = dfs_lanes_per_block(max_n);
+    const uint32_t lanes = dfs_lanes_per_block(max_n, block_lanes);
     const size_t lds = lane_lds_bytes(lanes, max_n);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dfs),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

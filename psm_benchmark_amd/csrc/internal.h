@@ -286,7 +286,7 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
                       const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue, uint32_t* seq,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
-                      EmitSink sink, hipStream_t s);
+                      uint32_t block_lanes, EmitSink sink, hipStream_t s);
 hipError_t launch_join_build(const uint32_t* rows, uint64_t n_rows, uint32_t width, JoinKeys keys,
                              unsigned long long* fp, uint32_t* head, uint64_t mask, uint32_t* next,
                              hipStream_t s);

@@ -40,6 +40,7 @@ using namespace psm;
 // contending for L2 and the queue): the 9,460-vertex C1 query enumerates in
 // 15 ms at 1 wave per CU, 19 ms at 2 and 48 ms at 8.
 constexpr uint32_t kDefaultWavesPerCu = 1;
+constexpr uint32_t kDefaultBlockLanes = 0;   // lanes per workgroup of the lane DFS (0: 64)
 constexpr uint32_t kDefaultClaimDiv = 1;     // array tasks per lane above which a claim takes several
 constexpr double kDefaultTasksPerLane = 2.0;
 
@@ -62,6 +63,7 @@ struct psm_ctx {
     uint32_t dfs_strategy = PSM_DFS_LANE;     // lane-per-task or wavefront-per-task DFS
     uint32_t queue_slots = 0;                 // ring slots for split-off work (0: automatic)
     uint32_t claim_div = kDefaultClaimDiv;    // array tasks per lane above which claims take several
+    uint32_t block_lanes = kDefaultBlockLanes;  // lanes per workgroup of the lane DFS (0: 64)
     double wall_khz = 100000.0;               // device wall clock rate
     bool verbose = false;
 };
@@ -1434,7 +1436,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
                              cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), d_queue.as<DTask>(),
                              d_seq.as<uint32_t>(),
                              d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
-                             sink, s()));
+                             ctx->block_lanes, sink, s()));
         }
         if (poll && hq.notify) {
             // results of queries that finish early are taken while the launch runs
@@ -1738,6 +1740,11 @@ int psm_set_tuning(psm_ctx* ctx, int key, double value) {
             if (!(value >= 1 && value <= 1024)) return set_err(ctx, PSM_EINVAL, "claim divisor outside [1, 1024]");
             ctx->claim_div = (uint32_t)value;
             return PSM_OK;
+        case PSM_TUNE_DFS_BLOCK_LANES:
+            if (value != 0 && value != 64 && value != 128 && value != 256)
+                return set_err(ctx, PSM_EINVAL, "dfs block lanes must be 0, 64, 128 or 256");
+            ctx->block_lanes = (uint32_t)value;
+            return PSM_OK;
         case PSM_TUNE_QUEUE_SLOTS:
             if (!(value >= 0 && value <= (double)(1u << 28)))
                 return set_err(ctx, PSM_EINVAL, "queue slots outside [0, 2^28]");
@@ -1756,6 +1763,7 @@ int psm_get_tuning(psm_ctx* ctx, int key, double* value) {
         case PSM_TUNE_DFS_STRATEGY: *value = ctx->dfs_strategy; return PSM_OK;
         case PSM_TUNE_QUEUE_SLOTS: *value = ctx->queue_slots; return PSM_OK;
         case PSM_TUNE_CLAIM_DIVISOR: *value = ctx->claim_div; return PSM_OK;
+        case PSM_TUNE_DFS_BLOCK_LANES: *value = ctx->block_lanes; return PSM_OK;
         default: return set_err(ctx, PSM_EINVAL, "unknown tuning key");
     }
 }
@@ -2290,7 +2298,7 @@ int psm_evaluate_noalloc(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     } else {
         HIP_TRY(ctx, launch_dfs(dplan, dops, b.row_off.as<uint64_t>(), b.idx.as<uint32_t>(),
                                 b.cand_pool.as<uint32_t>(), dinit, dring, dseq, dq, dqc, dpend, P.n, gw,
-                                EmitSink{nullptr, nullptr, 0, nullptr}, ctx->stream));
+                                ctx->block_lanes, EmitSink{nullptr, nullptr, 0, nullptr}, ctx->stream));
     }
     QCounters qc{};
     uint32_t left = 0;

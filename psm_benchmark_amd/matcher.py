@@ -44,6 +44,7 @@ class Device:
         h = ctypes.c_void_p()
         _check(None, L.lib().psm_ctx_create(device, ctypes.byref(h)), "psm_ctx_create")
         self.h = h
+        self.device = device
 
     def close(self):
         if getattr(self, "h", None):
@@ -284,21 +285,24 @@ def batch_match(dg: DeviceGraph, queries, filter: str = "GQL", order: str = "GQL
 
 class Batch:
     """The batched form stage by stage (psm_batch): prepare once (filter,
-    tables, plans), evaluate as often as needed."""
+    tables, plans), evaluate as often as needed. `dev` is the context (HIP
+    stream) the batch runs on, by default the graph's; any context on the
+    graph's GPU may run batches over it."""
 
-    def __init__(self, dg: DeviceGraph, queries):
+    def __init__(self, dg: DeviceGraph, queries, dev: Device | None = None):
         self.dg = dg
+        self.dev = dev or dg.dev
         self.m = len(queries)
         self._arrays = pack_queries(queries)
         n, ob, nb, lb, off, adj, lab = self._arrays
         h = ctypes.c_void_p()
-        _check(dg.dev.h, L.lib().psm_batch_create(dg.dev.h, dg.h, self.m, L.u32p(n), L.u64p(ob), L.u64p(nb),
-                                                  L.u64p(lb), L.u32p(off), L.u32p(adj), L.u32p(lab),
-                                                  ctypes.byref(h)), "psm_batch_create")
+        _check(self.dev.h, L.lib().psm_batch_create(self.dev.h, dg.h, self.m, L.u32p(n), L.u64p(ob),
+                                                    L.u64p(nb), L.u64p(lb), L.u32p(off), L.u32p(adj),
+                                                    L.u32p(lab), ctypes.byref(h)), "psm_batch_create")
         self.h = h
 
     def prepare(self, filter: str = "GQL", order: str = "GQL"):
-        _check(self.dg.dev.h, L.lib().psm_batch_prepare(self.h, L.FILTERS[filter], L.ORDERS[order]),
+        _check(self.dev.h, L.lib().psm_batch_prepare(self.h, L.FILTERS[filter], L.ORDERS[order]),
                "psm_batch_prepare")
 
     def evaluate(self, cap: int = 0, enum_time_limit: float = -1.0):
@@ -306,7 +310,7 @@ class Batch:
         calls = np.zeros(max(1, self.m), np.uint64)
         status = np.zeros(max(1, self.m), np.uint32)
         st = L.BatchStats()
-        _check(self.dg.dev.h, L.lib().psm_batch_evaluate(self.h, cap, enum_time_limit, L.u64p(counts),
+        _check(self.dev.h, L.lib().psm_batch_evaluate(self.h, cap, enum_time_limit, L.u64p(counts),
                                                          L.u64p(calls), L.u32p(status), ctypes.byref(st)),
                "psm_batch_evaluate")
         return counts[: self.m], calls[: self.m], status[: self.m], st
@@ -315,7 +319,7 @@ class Batch:
         """Per query: ms from the batch's creation until its count was final
         in the last evaluate."""
         ms = np.zeros(max(1, self.m), np.float64)
-        _check(self.dg.dev.h, L.lib().psm_batch_latencies(self.h, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+        _check(self.dev.h, L.lib().psm_batch_latencies(self.h, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
                "psm_batch_latencies")
         return ms[: self.m]
 
@@ -323,6 +327,86 @@ class Batch:
         if getattr(self, "h", None):
             L.lib().psm_batch_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+@dataclass
+class BatchRun:
+    counts: np.ndarray
+    calls: np.ndarray
+    status: np.ndarray
+    stats: object
+    latencies_ms: np.ndarray   # per query, from the batch's creation
+    start_s: float             # time.perf_counter() at the batch's creation
+    end_s: float               # ... and when its evaluate returned
+
+
+class Pipeline:
+    """A batched job of many batches over one resident graph: `depth`
+    contexts on the graph's GPU, each with its own HIP stream and host
+    thread, take the batches in turn (batch i on context i % depth), so the
+    filter, table and plan launches of one batch run while the enumeration of
+    another drains on the few lanes its longest queries still occupy. Every
+    batch's results are those it has alone; only the launches overlap. The
+    contexts take the graph context's launch shape and the given time limit.
+    """
+
+    def __init__(self, dg: DeviceGraph, depth: int = 2, time_limit: float = 60.0):
+        from . import autotune
+        if depth < 1:
+            raise ValueError("depth must be at least 1")
+        self.dg = dg
+        self.devs = [Device(dg.dev.device) for _ in range(depth)]
+        cfg = autotune.get_config(dg.dev)
+        for d in self.devs:
+            autotune.set_config(d, cfg)
+            _check(d.h, L.lib().psm_set_time_limit(d.h, time_limit), "psm_set_time_limit")
+
+    def run(self, batches, filter: str = "GQL", order: str = "GQL", cap: int = 0) -> list:
+        """Runs every batch (a list of query lists); returns a BatchRun per
+        batch, in order. The first error of any context is raised after all
+        of them have stopped."""
+        import threading
+        import time
+        out = [None] * len(batches)
+        errors = []
+
+        def work(w):
+            try:
+                for i in range(w, len(batches), len(self.devs)):
+                    if errors:
+                        return
+                    t = time.perf_counter()
+                    b = Batch(self.dg, batches[i], dev=self.devs[w])
+                    try:
+                        b.prepare(filter, order)
+                        counts, calls, status, st = b.evaluate(cap)
+                        end = time.perf_counter()
+                        out[i] = BatchRun(counts, calls, status, st, b.latencies(), t, end)
+                    finally:
+                        b.close()
+            except BaseException as e:   # re-raised on the caller's thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=work, args=(w,)) for w in range(len(self.devs))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        return out
+
+    def kernel_stats_reset(self):
+        for d in self.devs:
+            L.lib().psm_kernel_stats_reset(d.h)
+
+    def close(self):
+        for d in self.devs:
+            d.close()
+        self.devs = []
 
     def __del__(self):
         self.close()

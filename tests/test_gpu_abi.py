@@ -32,11 +32,13 @@ def test_tuning_round_trip_and_rejection(dg):
     autotune.set_config(dg.dev, cfg)
     got = autotune.get_config(dg.dev)
     assert {k: got[k] for k in cfg} == {**cfg, "dfs_waves_per_cu": 6.0}
-    autotune.set_config(dg.dev, {"claim_divisor": 3})
-    assert autotune.get_config(dg.dev)["claim_divisor"] == 3.0
+    autotune.set_config(dg.dev, {"claim_divisor": 3, "dfs_block_lanes": 256})
+    got = autotune.get_config(dg.dev)
+    assert got["claim_divisor"] == 3.0 and got["dfs_block_lanes"] == 256.0
     for key, bad in ((L.PSM_TUNE_DFS_WAVES_PER_CU, 0), (L.PSM_TUNE_DFS_WAVES_PER_CU, 33),
                      (L.PSM_TUNE_EXPAND_TASKS_PER_LANE, 0.0), (L.PSM_TUNE_DFS_STRATEGY, 7),
-                     (L.PSM_TUNE_CLAIM_DIVISOR, 0), (L.PSM_TUNE_QUEUE_SLOTS, -1), (99, 1)):
+                     (L.PSM_TUNE_CLAIM_DIVISOR, 0), (L.PSM_TUNE_QUEUE_SLOTS, -1),
+                     (L.PSM_TUNE_DFS_BLOCK_LANES, 32), (L.PSM_TUNE_DFS_BLOCK_LANES, 512), (99, 1)):
         assert L.lib().psm_set_tuning(h, key, float(bad)) == L.PSM_EINVAL
     v = ctypes.c_double()
     assert L.lib().psm_get_tuning(h, 99, ctypes.byref(v)) == L.PSM_EINVAL

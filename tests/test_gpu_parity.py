@@ -200,6 +200,49 @@ def test_tiny_task_ring_keeps_golden_counts(dev, hprd):
         autotune.set_config(dev, before)
 
 
+def test_block_widths_give_golden_counts(dev, hprd):
+    """The lane DFS in workgroups of 128 and 256 lanes (the default is one
+    wavefront) and at four wavefronts per CU: batched counts and call counts
+    still equal the golden records."""
+    from psm_benchmark_amd import autotune
+    from psm_benchmark_amd.matcher import batch_match
+    _, dg, _ = hprd
+    before = autotune.get_config(dev)
+    recs = golden(("GQL_GQL_GQL",))
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
+    try:
+        for lanes, waves in ((128, 1), (256, 1), (0, 4)):
+            autotune.set_config(dev, {"dfs_block_lanes": lanes, "dfs_waves_per_cu": waves})
+            counts, calls, status, _ = batch_match(dg, qs, "GQL", "GQL", 0)
+            assert counts.tolist() == [r["embeddings"] for r in recs], lanes
+            assert calls.tolist() == [r["calls"] for r in recs], lanes
+    finally:
+        autotune.set_config(dev, before)
+
+
+def test_pipeline_batches_equal_golden(dev, hprd):
+    """Batches through a two-context pipeline (two HIP streams, their
+    launches overlapping) each give the golden counts and call counts."""
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.matcher import Pipeline
+    _, dg, _ = hprd
+    recs = golden(("GQL_GQL_GQL",))
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
+    halves = [qs[::2], qs[1::2], qs, qs[::-1]]
+    want = [[r["embeddings"] for r in recs[::2]], [r["embeddings"] for r in recs[1::2]],
+            [r["embeddings"] for r in recs], [r["embeddings"] for r in recs[::-1]]]
+    p = Pipeline(dg, 2, time_limit=60.0)
+    try:
+        runs = p.run(halves, "GQL", "GQL", 0)
+    finally:
+        p.close()
+    for r, w in zip(runs, want):
+        assert r.counts.tolist() == w
+        assert not (r.status == L.PSM_ETIMEOUT).any()
+        assert r.end_s >= r.start_s and (r.latencies_ms >= 0).all()
+    assert runs[2].calls.tolist() == [r["calls"] for r in recs]
+
+
 def test_batch_latencies_are_within_the_step(hprd):
     """Per-query completion times of a batch lie between the start of the
     batch and the end of its evaluate call, and capped queries finish."""

@@ -1,10 +1,11 @@
 """Parity at the benchmark configurations' full sizes, through the C ABI.
 
 * C2: Chung-Lu 1.1M V / 3.0M E, all 512 random-walk queries, cap 1e5, no
-  time limit: the batched lane kernel, the batched wavefront kernel and the
-  plain path report the same min(cap, total) for every query and none stops
-  early; the oracle agrees on a sample; a prepared batch evaluated again at
-  a smaller cap reports min(cap, count).
+  time limit: the batched lane kernel, the batched wavefront kernel, three
+  batches through a two-context pipeline and the plain path report the same
+  min(cap, total) for every query and none stops early; the oracle agrees
+  on a sample; a prepared batch evaluated again at a smaller cap reports
+  min(cap, count).
 * C3: Chung-Lu 4M V / 35M E, 32 Zipf(1.0) labels, CFL_CFL_EXPLORE: capped
   counts of the 64 cyclic 12-16-vertex queries agree across the lane
   kernel, the wavefront kernel and the plain path; on the join-checkable
@@ -90,6 +91,15 @@ def test_c2_all_queries_lane_wave_plain_agree(dev, c2):
     for i, q in enumerate(qs):
         r = match(dg, q, "GQL_GQL_GQL", cap=cap)
         assert not r.timed_out and r.count == lane[i], i
+    # three batches through a two-context pipeline (overlapping launches)
+    from psm_benchmark_amd.matcher import Pipeline
+    p = Pipeline(dg, 2, time_limit=SAFETY_S)
+    try:
+        for r in p.run([qs] * 3, "GQL", "GQL", cap):
+            assert not (r.status == L.PSM_ETIMEOUT).any()
+            assert np.array_equal(r.counts, lane)
+    finally:
+        p.close()
     # a prepared batch evaluated again at a smaller cap
     b = Batch(dg, qs[:64])
     b.prepare("GQL", "GQL")
