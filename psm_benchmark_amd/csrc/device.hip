@@ -266,7 +266,7 @@ constexpr uint32_t kScanUnroll = 4;   // 64-neighbour chunks a wavefront loads b
 __device__ __forceinline__ unsigned long long gql_refine_one(
     const DevGraph& g, const GItem& it, uint32_t j, const uint32_t* __restrict__ left_slots,
     const uint32_t* __restrict__ left_lab, const uint32_t* __restrict__ left_labs, uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
-    unsigned long long* __restrict__ member, GqlShared& sh, uint32_t wl) {
+    unsigned long long* __restrict__ member, uint32_t* __restrict__ removed, GqlShared& sh, uint32_t wl) {
     const uint32_t lane = lane_id();
     if (j >= it.len) return 0;
     const uint32_t v = cand_pool[it.cand_begin + j];
@@ -284,12 +284,16 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
     // as soon as every left vertex of its label has k; a left vertex whose
     // run holds none fails the candidate at once.
     const uint32_t my_lab = lane < k ? left_lab[it.left_begin + lane] : kInvalid;
+    // lane li < nlab locates the run of left label li: the lookups (binary
+    // searches over v's label runs) proceed side by side, not one per label
+    // in turn; the loops below take each run from its lane
+    const uint2 my_run = lane < it.nlab ? label_run(g, v, labs[lane]) : make_uint2(0u, 0u);
     uint32_t my_cnt = 0;
     uint32_t scanned = 0, rescanned = 0;
     bool ok = true;
     for (uint32_t li = 0; li < it.nlab && ok; ++li) {
         const bool mine = my_lab == labs[li];
-        const uint2 run = label_run(g, v, labs[li]);
+        const uint2 run = make_uint2(__shfl(my_run.x, li), __shfl(my_run.y, li));
         // kScanUnroll chunks per step: their neighbour and membership loads
         // are independent and in flight together
         for (uint32_t o = 0; o < run.y; o += kScanUnroll * kWave) {
@@ -324,7 +328,7 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
         uint32_t nr = 0;
         for (uint32_t li = 0; li < it.nlab; ++li) {
             if (__ballot(((T >> lane) & 1u) && my_lab == labs[li]) == 0) continue;
-            const uint2 run = label_run(g, v, labs[li]);
+            const uint2 run = make_uint2(__shfl(my_run.x, li), __shfl(my_run.y, li));
             rescanned += run.y;
             for (uint32_t o = 0; o < run.y; o += kWave) {
                 const uint32_t p = o + lane;
@@ -382,6 +386,7 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
         cand_pool[it.cand_begin + j] = kInvalid;
         atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
         atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
+        removed[it.bm_self] = 1u;
     }
     // algorithmic bytes: candidate, one run lookup per left label, the
     // scanned part of the label runs, one membership word per (neighbour,
@@ -400,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
     const uint32_t* __restrict__ n_list, uint32_t* __restrict__ cursor, const uint32_t* __restrict__ left_slots,
     const uint32_t* __restrict__ left_lab, const uint32_t* __restrict__ left_labs,
     uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps,
-    uint64_t wps, unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
+    uint64_t wps, unsigned long long* __restrict__ member, GqlRound rd, unsigned long long* __restrict__ bytes) {
     __shared__ GqlShared sh;
     const uint32_t n = *n_list;
     const uint32_t wl = threadIdx.x / kWave;
@@ -411,8 +416,8 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
         w = __shfl(w, 0);
         if (w >= n) break;
         const uint2 e = list[w];
-        by += gql_refine_one(g, items[e.x], e.y, left_slots, left_lab, left_labs, cand_pool, bitmaps, wps, member, sh,
-                             wl);
+        by += gql_refine_one(g, items[e.x], e.y, left_slots, left_lab, left_labs, cand_pool, bitmaps, wps, member,
+                             rd.removed, sh, wl);
     }
     block_add(bytes, lane_id() == 0 ? by : 0ull);
 }
@@ -497,10 +502,33 @@ __device__ __forceinline__ bool hall_fails(uint32_t m, bool live, uint32_t k, ui
 // Removes candidate j of item it (value v): pool slot, bitmap bit, membership bit.
 __device__ __forceinline__ void gql_drop(const GItem& it, uint32_t j, uint32_t v, uint32_t* __restrict__ cand_pool,
                                          uint32_t* __restrict__ bitmaps, uint64_t wps,
-                                         unsigned long long* __restrict__ member) {
+                                         unsigned long long* __restrict__ member, uint32_t* __restrict__ removed) {
     cand_pool[it.cand_begin + j] = kInvalid;
     atomicAnd(bitmaps + (size_t)it.bm_self * wps + (v >> 5), ~(1u << (v & 31)));
     atomicAnd(member + it.mem_base + v, ~(1ull << it.u));
+    removed[it.bm_self] = 1u;
+}
+
+// Second-round skip flag of each item (GqlRound): the item's candidates of
+// u were tested in the first round against the sets of u's query neighbours
+// as they stood then; since, those of later neighbours (u' > u) changed only
+// in the first round, those of earlier ones (u' < u) only in the second. If
+// none changed, the second test would see the same bipartite graphs.
+__global__ __launch_bounds__(kBlock) void k_gql_skip(const GItem* __restrict__ items, uint32_t n_items,
+                                                     const uint32_t* __restrict__ left_slots,
+                                                     const uint32_t* __restrict__ removed1,
+                                                     const uint32_t* __restrict__ removed2,
+                                                     uint32_t* __restrict__ skip) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items) return;
+    const GItem it = items[i];
+    const uint32_t first = it.bm_self - it.u;
+    uint32_t changed = 0;
+    for (uint32_t e = 0; e < it.k; ++e) {
+        const uint32_t slot = left_slots[it.left_begin + e];
+        changed |= slot - first > it.u ? removed1[slot] : removed2[slot];
+    }
+    skip[i] = changed ? 0u : 1u;
 }
 
 // The Hall test with G lanes per candidate over the (item, index) pairs the
@@ -512,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_hall(
     DevGraph g, const GItem* __restrict__ items, const uint2* __restrict__ list,
     const uint32_t* __restrict__ n_list, const uint32_t* __restrict__ left_slots,
     const uint32_t* __restrict__ left_labs, uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps,
-    uint64_t wps, unsigned long long* __restrict__ member, unsigned long long* __restrict__ bytes) {
+    uint64_t wps, unsigned long long* __restrict__ member, GqlRound rd, unsigned long long* __restrict__ bytes) {
     constexpr uint32_t kPerIter = kBlock / G;
     const uint32_t n = *n_list;
     const uint32_t lane = lane_id();
@@ -540,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_hall(
         if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj_bl[pos]);
         const bool fail = hall_fails<G>(m, live, k, lig, gbase);
         if (live && lig == 0) {
-            if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member);
+            if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member, rd.removed);
             // algorithmic bytes: candidate, one run lookup per left label,
             // the runs, one membership word per neighbour and left vertex
             by += 4ull + 16ull * it.nlab + 4ull * deg * (1 + k);
@@ -585,7 +613,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
     const uint32_t* __restrict__ left_slots, const uint32_t* __restrict__ left_labs,
     uint32_t* __restrict__ cand_pool, uint32_t* __restrict__ bitmaps, uint64_t wps,
     unsigned long long* __restrict__ member, uint2* __restrict__ lists, uint64_t list_stride,
-    uint32_t* __restrict__ n_lists,
+    uint32_t* __restrict__ n_lists, GqlRound rd,
     unsigned long long* __restrict__ bytes) {
     constexpr uint32_t kPerIter = kBlock / kGroup;   // candidates per block iteration
     __shared__ DeferStage st;
@@ -606,7 +634,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
             ii = find_item(items, n_items, (uint32_t)gid, &GItem::wave_start);
             it = items[ii];
             j = (uint32_t)gid - it.wave_start;
-            live = j < it.len;
+            live = j < it.len && !(rd.skip && rd.skip[ii]);
         }
         if (live) {
             v = cand_pool[it.cand_begin + j];
@@ -633,7 +661,7 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
         if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj_bl[pos]);
         const bool fail = hall_fails<kGroup>(m, live, k, lig, gbase);
         if (live && lig == 0) {
-            if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member);
+            if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member, rd.removed);
             by += 4ull + 16ull * it.nlab + 4ull * deg * (1 + k);
         }
         __syncthreads();
@@ -2796,13 +2824,21 @@ constexpr uint32_t kGqlGrid = 2048;
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, const uint2* list, const uint32_t* n_list,
                              uint32_t* cursor, uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_lab,
                              const uint32_t* left_labs, uint32_t* cand_pool, uint32_t* bitmaps,
-                             uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
-                             hipStream_t s) {
+                             uint64_t words_per_slot, unsigned long long* member, GqlRound rd,
+                             unsigned long long* bytes, hipStream_t s) {
     if (max_n == 0) return hipSuccess;
     // six blocks per CU fit the LDS of gfx950's 256 CUs (26 KB per block)
     const uint32_t blocks = std::min<uint32_t>(1536, (max_n + kGqlWavesPerBlock - 1) / kGqlWavesPerBlock);
     hipLaunchKernelGGL(k_gql_refine, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, cursor, left_slots,
-                       left_lab, left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
+                       left_lab, left_labs, cand_pool, bitmaps, words_per_slot, member, rd, bytes);
+    return hipGetLastError();
+}
+
+hipError_t launch_gql_skip(const GItem* items, uint32_t n_items, const uint32_t* left_slots,
+                           const uint32_t* removed1, const uint32_t* removed2, uint32_t* skip, hipStream_t s) {
+    if (n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gql_skip, dim3((n_items + kBlock - 1) / kBlock), dim3(kBlock), 0, s, items, n_items,
+                       left_slots, removed1, removed2, skip);
     return hipGetLastError();
 }
 
@@ -2810,7 +2846,7 @@ hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32
                                    uint32_t n_cands, const uint32_t* left_slots, const uint32_t* left_labs,
                                    uint32_t* cand_pool,
                                    uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
-                                   uint2* lists, uint64_t list_stride, uint32_t* n_lists,
+                                   uint2* lists, uint64_t list_stride, uint32_t* n_lists, GqlRound rd,
                                    unsigned long long* bytes, hipStream_t s) {
     if (n_cands == 0) return hipSuccess;
     if (list_stride < n_cands) return hipErrorInvalidValue;
@@ -2818,26 +2854,26 @@ hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, kGqlGrid);
     hipLaunchKernelGGL(k_gql_refine_small, dim3(blocks), dim3(kBlock), 0, s, g, items, n_items, n_cands,
                        left_slots, left_labs, cand_pool, bitmaps, words_per_slot, member, lists, list_stride, n_lists,
-                       bytes);
+                       rd, bytes);
     return hipGetLastError();
 }
 
 hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items, const uint2* list,
                            const uint32_t* n_list, uint32_t max_n, const uint32_t* left_slots,
                            const uint32_t* left_labs, uint32_t* cand_pool, uint32_t* bitmaps, uint64_t words_per_slot,
-                           unsigned long long* member, unsigned long long* bytes, hipStream_t s) {
+                           unsigned long long* member, GqlRound rd, unsigned long long* bytes, hipStream_t s) {
     if (max_n == 0) return hipSuccess;
     const uint64_t want = ((uint64_t)max_n * lanes + kBlock - 1) / kBlock;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, kGqlGrid);
     if (lanes == 16)
         hipLaunchKernelGGL(k_gql_hall<16>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
-                           left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
+                           left_labs, cand_pool, bitmaps, words_per_slot, member, rd, bytes);
     else if (lanes == 32)
         hipLaunchKernelGGL(k_gql_hall<32>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
-                           left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
+                           left_labs, cand_pool, bitmaps, words_per_slot, member, rd, bytes);
     else if (lanes == 64)
         hipLaunchKernelGGL(k_gql_hall<64>, dim3(blocks), dim3(kBlock), 0, s, g, items, list, n_list, left_slots,
-                           left_labs, cand_pool, bitmaps, words_per_slot, member, bytes);
+                           left_labs, cand_pool, bitmaps, words_per_slot, member, rd, bytes);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -234,23 +234,36 @@ hipError_t launch_popc_words(const uint32_t* bitmaps, uint32_t* out, uint64_t n_
 // lists + c * list_stride (list_stride >= n_cands), counts added to
 // n_lists[0..3]. The deferred-class kernels read their count from n_list;
 // max_n bounds it (sizes the grid).
+// One (round, vertex position) step of the GQL refinement: removed[slot] is
+// set when a candidate of bitmap slot `slot` is dropped in this round; skip
+// (second round only, else null) holds one flag per item of the step: set
+// when none of the item's left vertices lost a candidate since the item's
+// first-round test, so every surviving candidate passes again untested.
+struct GqlRound {
+    uint32_t* removed;
+    const uint32_t* skip;
+};
 hipError_t launch_gql_refine_small(const DevGraph& g, const GItem* items, uint32_t n_items,
                                    uint32_t n_cands, const uint32_t* left_slots, const uint32_t* left_labs,
                                    uint32_t* cand_pool,
                                    uint32_t* bitmaps, uint64_t words_per_slot, unsigned long long* member,
-                                   uint2* lists, uint64_t list_stride, uint32_t* n_lists,
+                                   uint2* lists, uint64_t list_stride, uint32_t* n_lists, GqlRound rd,
                                    unsigned long long* bytes, hipStream_t s);
 hipError_t launch_gql_hall(uint32_t lanes, const DevGraph& g, const GItem* items, const uint2* list,
                            const uint32_t* n_list, uint32_t max_n, const uint32_t* left_slots,
                            const uint32_t* left_labs, uint32_t* cand_pool, uint32_t* bitmaps, uint64_t words_per_slot,
-                           unsigned long long* member, unsigned long long* bytes, hipStream_t s);
+                           unsigned long long* member, GqlRound rd, unsigned long long* bytes, hipStream_t s);
 // cursor: zero before the launch (the wavefronts' shared position in list);
 // left_lab: the label of each left vertex (parallel to left_slots)
 hipError_t launch_gql_refine(const DevGraph& g, const GItem* items, const uint2* list, const uint32_t* n_list,
                              uint32_t* cursor, uint32_t max_n, const uint32_t* left_slots, const uint32_t* left_lab,
                              const uint32_t* left_labs, uint32_t* cand_pool, uint32_t* bitmaps,
-                             uint64_t words_per_slot, unsigned long long* member, unsigned long long* bytes,
-                             hipStream_t s);
+                             uint64_t words_per_slot, unsigned long long* member, GqlRound rd,
+                             unsigned long long* bytes, hipStream_t s);
+// Second-round skip flags of the n_items items of one vertex position (see
+// GqlRound): removed1 / removed2 are the first and second rounds' flags.
+hipError_t launch_gql_skip(const GItem* items, uint32_t n_items, const uint32_t* left_slots,
+                           const uint32_t* removed1, const uint32_t* removed2, uint32_t* skip, hipStream_t s);
 // thread per row; rows of candidates with more than 64 neighbours are listed
 // in long_rows (count in *n_long) for launch_table_long
 hipError_t launch_table_count(const DevGraph& g, const TItem* items, uint32_t n_items,

@@ -740,40 +740,60 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
     const uint64_t n_counters = 2ull * max_n * kCounterSlots;
     HIP_TRY(ctx, d_nlists.alloc(n_counters * 4));
     HIP_TRY(ctx, launch_fill_u32(d_nlists.as<uint32_t>(), 0u, n_counters, s()));
+    // removal flags per round and bitmap slot, and the second round's skip
+    // flags per item (GqlRound)
+    uint32_t n_slots = 0;
+    for (QueryHost* q : active) n_slots = std::max(n_slots, q->slot_base + q->qg.n);
+    DBuf d_removed, d_skip;
+    HIP_TRY(ctx, d_removed.alloc(2ull * n_slots * 4));
+    HIP_TRY(ctx, launch_fill_u32(d_removed.as<uint32_t>(), 0u, 2ull * n_slots, s()));
+    HIP_TRY(ctx, d_skip.alloc(std::max<size_t>(1, items.size()) * 4));
     // Launch groups follow one another on the stream without host syncs: the
     // deferred-class kernels read their counts from d_nlists.
     for (int round = 0; round < 2; ++round) {
+        uint32_t* removed = d_removed.as<uint32_t>() + (uint64_t)round * n_slots;
         for (uint32_t p = 0; p < max_n; ++p) {
             const uint32_t ni = item_begin[p + 1] - item_begin[p];
             if (!ni) continue;
             const GItem* its = d_items.as<GItem>() + item_begin[p];
             uint32_t* cnt = d_nlists.as<uint32_t>() + ((uint64_t)round * max_n + p) * kCounterSlots;
             uint2* lists = d_lists.as<uint2>();
+            GqlRound rd{removed, nullptr};
+            if (round == 1) {
+                uint32_t* skip = d_skip.as<uint32_t>() + item_begin[p];
+                TIMED(ctx, kKGqlRefine,
+                      launch_gql_skip(its, ni, d_left.as<uint32_t>(), d_removed.as<uint32_t>(), removed, skip, s()));
+                rd.skip = skip;
+            }
             TIMED(ctx, kKGqlRefine,
                   launch_gql_refine_small(g->dg, its, ni, cands[p], d_left.as<uint32_t>(), d_labs.as<uint32_t>(),
                                           cand_pool.as<uint32_t>(),
                                           bitmaps.as<uint32_t>(), wps, member.as<unsigned long long>(), lists,
-                                          max_cands, cnt, ctx->d_kbytes + kKGqlRefine, s()));
+                                          max_cands, cnt, rd, ctx->d_kbytes + kKGqlRefine, s()));
             const uint32_t lanes[3] = {16, 32, 64};
             for (int c = 0; c < 3; ++c)
                 TIMED(ctx, kKGqlRefine,
                       launch_gql_hall(lanes[c], g->dg, its, lists + (uint64_t)c * max_cands, cnt + c, cands[p],
                                       d_left.as<uint32_t>(), d_labs.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
-                                      member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
+                                      member.as<unsigned long long>(), rd, ctx->d_kbytes + kKGqlRefine, s()));
             TIMED(ctx, kKGqlRefine,
                   launch_gql_refine(g->dg, its, lists + 3ull * max_cands, cnt + 3, cnt + 4, cands[p], d_left.as<uint32_t>(),
                                     d_left_lab.as<uint32_t>(), d_labs.as<uint32_t>(), cand_pool.as<uint32_t>(), bitmaps.as<uint32_t>(), wps,
-                                    member.as<unsigned long long>(), ctx->d_kbytes + kKGqlRefine, s()));
+                                    member.as<unsigned long long>(), rd, ctx->d_kbytes + kKGqlRefine, s()));
         }
     }
     if (ctx->verbose) {
-        std::vector<uint32_t> nd;
+        std::vector<uint32_t> nd, sk;
         HIP_TRY(ctx, download(nd, d_nlists.p, n_counters, s()));
+        HIP_TRY(ctx, download(sk, d_skip.p, items.size(), s()));
         for (int round = 0; round < 2; ++round)
             for (uint32_t p = 0; p < max_n; ++p) {
                 const uint32_t* c = nd.data() + ((uint64_t)round * max_n + p) * kCounterSlots;
-                verbose("gql round %d u %u: %u candidates, deferred %u / %u / %u / %u", round, p, cands[p], c[0],
-                        c[1], c[2], c[3]);
+                uint64_t skipped = 0;
+                if (round == 1)
+                    for (uint32_t i = item_begin[p]; i < item_begin[p + 1]; ++i) skipped += sk[i] ? items[i].len : 0;
+                verbose("gql round %d u %u: %u candidates, %llu skipped, deferred %u / %u / %u / %u", round, p,
+                        cands[p], (unsigned long long)skipped, c[0], c[1], c[2], c[3]);
             }
     }
     HIP_TRY(ctx, timers_flush(ctx));
