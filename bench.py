@@ -199,6 +199,9 @@ def main():
     if args.workload == "C4":
         serving_workload(args)
         return
+    if args.workload == "C4-batch":
+        c4_batch_workload(args)
+        return
 
     from psm_benchmark_amd import autotune
     from psm_benchmark_amd import workloads as W
@@ -725,6 +728,61 @@ def serving_workload(args):
         "launch_shape": sv["launch_shape"], "cpu_baseline": None,
     }
     print(json.dumps(out))
+
+
+def c4_batch_workload(args):
+    """The C4 requests as plain batches (--workload C4-batch): the C3 graph
+    resident, the first --queries (default 256 here) C4 random-walk queries
+    as one batch per step, GQL_GQL_GQL, cap 1e5, launch shape --shape or the
+    library default. A diagnostic line for the filter and table kernels that
+    bound the serving form (stage and per-kernel times), not a headline."""
+    from psm_benchmark_amd import workloads as W
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.matcher import Batch, Device, DeviceGraph
+
+    nq = args.queries if args.queries != 512 else 256
+    g = W.data_graph("C4")
+    qs, _ = W.c2_queries(g, count=max(nq, 1), seed=44)
+    qs = qs[:nq]
+    dev = Device(0)
+    L.lib().psm_set_time_limit(dev.h, args.time_limit)
+    dg = DeviceGraph(dev, g)
+    if args.shape:
+        apply_shape(dev, args.shape)
+    stats, times, timeouts = [], [], 0
+    for i in range(args.warmup + args.steps):
+        if i == args.warmup:
+            L.lib().psm_kernel_stats_reset(dev.h)
+        t = time.perf_counter()
+        b = Batch(dg, qs)
+        try:
+            b.prepare("GQL", "GQL")
+            counts, calls, status, st = b.evaluate(args.cap)
+        finally:
+            b.close()
+        if i >= args.warmup:
+            times.append(time.perf_counter() - t)
+            stats.append(st)
+            timeouts += int((status == L.PSM_ETIMEOUT).sum())
+        log(f"batch {i}: {(time.perf_counter() - t) * 1e3:.1f} ms (filter {st.filter_ms:.1f}, "
+            f"table {st.table_ms:.1f}, enum {st.enum_ms:.1f})")
+    ks = kernel_stats(dev)
+    dg.close()
+    total = sum(times)
+    print(json.dumps({
+        "metric": "queries/s of C4 requests as plain batches (diagnostic)", "value": round(nq * len(times) / total, 2),
+        "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total / len(times) * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded graph and random-walk queries)",
+        "config": {"workload": f"C4 requests as batches: Chung-Lu 4M V / 35M E, 32 Zipf(1.0) labels, {nq} "
+                               "random-walk queries of 4-32 vertices per batch, GQL_GQL_GQL, cap 1e5",
+                   "queries": nq, "cap": args.cap},
+        "timeouts": timeouts,
+        "stage_ms": {k: round(float(np.mean([getattr(s, k) for s in stats])), 2)
+                     for k in ("filter_ms", "table_ms", "plan_ms", "enum_ms")},
+        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"] / len(times), 3),
+                            bytes=int(v["bytes"] / len(times))) for k, v in ks.items()},
+        "roofline": None, "cpu_baseline": None}))
 
 
 KERNELS = ["gql_refine", "table_count", "table_fill", "dfs", "expand", "compact", "bitmap"]

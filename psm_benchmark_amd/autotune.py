@@ -30,7 +30,7 @@ from .matcher import Batch
 
 # candidate launch shapes: (strategy, wavefronts per CU, tasks per worker)
 GRID = [("lane", w, e) for w in (1, 2, 4, 8) for e in (2.0, 4.0, 8.0)] + \
-       [("wave", w, e) for w in (4, 8) for e in (1.0, 4.0)]
+       [("wave", w, e) for w in (1, 2, 4) for e in (1.0, 4.0)]
 NAMES = ("dfs_strategy", "dfs_waves_per_cu", "expand_tasks_per_lane")
 REPEATS = 2   # evaluations per trial, averaged
 KEYS = {

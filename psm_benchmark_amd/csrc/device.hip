@@ -435,6 +435,18 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
 // the wavefront kernel above. Deferred lists: class c at lists + c * cap.
 constexpr uint32_t kGroup = 8;
 constexpr uint32_t kDeferClasses = 4;   // 16, 32, 64 lanes, wavefront kernel
+constexpr uint32_t kItemsInLds = 512;   // item starts the 8-lane kernel keeps in LDS
+
+// The last index i with starts[i] <= key (starts ascending, starts[0] <= key).
+__device__ __forceinline__ uint32_t find_start(const uint32_t* starts, uint32_t n, uint32_t key) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (starts[mid] <= key) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 __device__ __forceinline__ uint32_t defer_class(uint32_t deg, uint32_t k) {
     if (k > kGroup || deg > 64) return 3;
@@ -458,7 +470,37 @@ __device__ __forceinline__ uint32_t group_relevant(const DevGraph& g, const GIte
                                                    const uint32_t* __restrict__ left_labs, uint32_t v,
                                                    bool live, uint32_t lig, uint32_t& pos) {
     uint2 run = make_uint2(0u, 0u);
-    if (live && lig < it.nlab && lig < kGroup) run = label_run(g, v, left_labs[it.lab_begin + lig]);
+    uint32_t r0 = 0, R = 0;
+    if (live) {
+        r0 = g.nlf_off[v];
+        R = g.nlf_off[v + 1] - r0;
+    }
+    const uint32_t want = live && lig < it.nlab && lig < kGroup ? left_labs[it.lab_begin + lig] : kInvalid;
+    if (R <= G) {
+        // v has at most G label runs: the group's lanes load one each in a
+        // single round, and every lane finds its label by a binary search
+        // over the lanes (R is the same on all lanes of a group)
+        const uint2 e = lig < R ? g.nlf[r0 + lig] : make_uint2(kInvalid, 0u);
+        const uint32_t eb = lig < R ? g.nlf_beg[r0 + lig] : 0u;
+        uint32_t lo = 0, len = R;
+        for (uint32_t round = G; round; round >>= 1) {   // len <= G halves every round
+            const uint32_t half = len >> 1;
+            const uint32_t lab = __shfl(e.x, min(lo + half, G - 1), G);
+            if (len) {
+                if (lab < want) {
+                    lo += half + 1;
+                    len -= half + 1;
+                } else {
+                    len = half;
+                }
+            }
+        }
+        const uint32_t at = min(lo, G - 1);
+        const uint32_t lab = __shfl(e.x, at, G), cnt = __shfl(e.y, at, G), beg = __shfl(eb, at, G);
+        if (want != kInvalid && lo < R && lab == want) run = make_uint2(beg, cnt);
+    } else if (want != kInvalid) {
+        run = label_run(g, v, want);
+    }
     uint32_t incl = run.y;   // inclusive prefix of run lengths over the group's lanes
 #pragma unroll
     for (uint32_t o = 1; o < kGroup; o <<= 1) {
@@ -617,10 +659,14 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
     unsigned long long* __restrict__ bytes) {
     constexpr uint32_t kPerIter = kBlock / kGroup;   // candidates per block iteration
     __shared__ DeferStage st;
+    __shared__ uint32_t s_start[kItemsInLds];   // the items' first positions, searched in LDS
     const uint32_t lane = lane_id();
     const uint32_t lig = lane & (kGroup - 1);
     const uint32_t gbase = lane & ~(kGroup - 1);
+    const bool starts_in_lds = n_items <= kItemsInLds;
     if (threadIdx.x < kDeferClasses) st.count[threadIdx.x] = 0;
+    if (starts_in_lds)
+        for (uint32_t i = threadIdx.x; i < n_items; i += kBlock) s_start[i] = items[i].wave_start;
     __syncthreads();
     unsigned long long by = 0;
     // block-uniform trip count: barriers and ballots see every lane
@@ -631,7 +677,8 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
         uint32_t ii = 0, j = 0, v = kInvalid, k = 0;
         GItem it{};
         if (live) {
-            ii = find_item(items, n_items, (uint32_t)gid, &GItem::wave_start);
+            ii = starts_in_lds ? find_start(s_start, n_items, (uint32_t)gid)
+                               : find_item(items, n_items, (uint32_t)gid, &GItem::wave_start);
             it = items[ii];
             j = (uint32_t)gid - it.wave_start;
             live = j < it.len && !(rd.skip && rd.skip[ii]);
