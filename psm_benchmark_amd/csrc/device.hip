@@ -39,6 +39,12 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, unsigned long lo
     if (lane_id() == 0 && v) atomicAdd(p, v);
 }
 
+// Sum of v over the wavefront, added once to *p (all lanes must call).
+__device__ __forceinline__ void wave_add_u32(unsigned int* p, uint32_t v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane_id() == 0 && v) atomicAdd(p, v);
+}
+
 // Sum of v over the block, added once to *p. Every thread of the block must
 // call it (it synchronises the block).
 __device__ __forceinline__ void block_add(unsigned long long* p, unsigned long long v) {
@@ -1079,102 +1085,176 @@ struct PrefixState {
 
 constexpr uint64_t kExpandChunk = 256;   // longest base range one expansion thread walks
 
-// One thread per task: enumerate the valid extensions at the task's depth.
-// Count pass (write = 0): out_count[t] = number of children (or 1 when the
-// task's query is not being expanded). Write pass: the children (or the task
-// itself) at out_pos[t]; calls += children once the expansion is committed.
+// Adds v to counter[q] for every lane of the wavefront (all lanes call; lanes
+// with q == kInvalid add nothing). Task arrays are sorted by query, so the
+// lanes of a wavefront usually share q: their sum then goes in one atomic.
+__device__ __forceinline__ void add_by_query(unsigned long long* counter, size_t stride, uint32_t q,
+                                             unsigned long long v) {
+    const unsigned long long valid = __ballot(q != kInvalid);
+    if (!valid) return;
+    const uint32_t lead = __ffsll((long long)valid) - 1;
+    const uint32_t ql = __shfl(q, lead);
+    if (__ballot(q != kInvalid && q != ql) == 0) {
+        unsigned long long sum = q != kInvalid ? v : 0ull;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const uint32_t lo = __shfl_xor((uint32_t)sum, o), hi = __shfl_xor((uint32_t)(sum >> 32), o);
+            sum += ((unsigned long long)hi << 32) | lo;
+        }
+        if (lane_id() == lead && sum) atomicAdd(counter + (size_t)ql * stride, sum);
+    } else if (q != kInvalid && v) {
+        atomicAdd(counter + (size_t)q * stride, v);
+    }
+}
+
+// One thread per task of the queries being expanded (the caller keeps the
+// tasks of the other queries apart): enumerate the valid extensions at the
+// task's depth. A level with exactly one valid extension is assigned at once
+// and the thread goes on to the next level (a chain of single candidates is
+// crossed in one round), until a level has none or several, or the depth
+// limit min(n - 1, kExpandMax) is reached (the leaf level is never assigned
+// here). Count pass (write = 0): out_count[t] = the tasks that replace task
+// t (its children, or t itself when it is at the limit), qchild[q] += them,
+// qchild[nq + q] += 1 if t was expanded. Write pass: those tasks at
+// out_pos[t]; calls += one per assignment made, once the expansion is
+// committed. A task of a query that is not expanded (expand_q[q] == 0) is
+// dropped.
 __global__ __launch_bounds__(kBlock) void k_expand(
     const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
     const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ tasks, uint32_t n_tasks,
     const uint8_t* __restrict__ expand_q, uint32_t* __restrict__ out_count,
     const uint64_t* __restrict__ out_pos, DTask* __restrict__ out_tasks, QCounters* qc,
-    unsigned long long* __restrict__ qchild, int write) {
+    unsigned long long* __restrict__ qchild, uint32_t nq, int write) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n_tasks) return;
-    const DTask task = tasks[t];
-    if (!expand_q[task.q]) {
-        if (write) {
-            out_tasks[out_pos[t]] = task;
-        } else {
-            out_count[t] = 1;
-            atomicAdd(qchild + task.q, 1ull);
+    // every lane reaches the per-query sums at the end
+    uint32_t q = kInvalid, out = 0, rej = 0, calls = 0, expanded = 0;
+    unsigned long long bytes = 0, algo = 0;
+    if (t < n_tasks && expand_q[tasks[t].q]) {
+        const DTask& task = tasks[t];
+        q = task.q;
+        const DPlan& P = plans[q];
+        const uint32_t lim = min(P.n - 1, kExpandMax);
+        uint32_t I[kExpandMax + 1], M[kExpandMax + 1];
+        uint32_t d = task.depth;
+        for (uint32_t e = 0; e < d; ++e) {
+            I[e] = task.prefix[e];
+            M[e] = cand_pool[P.cand_base[e] + I[e]];
         }
-        return;
-    }
-    const DPlan& P = plans[task.q];
-    uint32_t I[kExpandMax + 1], M[kExpandMax + 1];
-    for (uint32_t e = 0; e < task.depth; ++e) {
-        I[e] = task.prefix[e];
-        M[e] = cand_pool[P.cand_base[e] + I[e]];
-    }
-    PrefixState st{I, M};
-    unsigned long long bytes = 0, rows = 0;
-    const uint32_t d = task.depth;
-    LevelBase b;
-    if (task.hi == kAutoHi) {
-        b = level_base(P, ops, row_off, d, st, bytes, rows);   // a node entered here
-    } else {
-        b.lo = task.lo;
-        b.hi = task.hi;
-        b.sel = task.sel;
-    }
-    uint64_t pos = write ? out_pos[t] : 0;
-    if (b.hi - b.lo > kExpandChunk) {
-        // a long base range (a hub's row) is not walked by one thread: it is
-        // cut into range tasks of at most kExpandChunk positions at the same
-        // depth (no assignment is made, so no calls), expanded next round
-        const uint64_t pieces = (b.hi - b.lo + kExpandChunk - 1) / kExpandChunk;
-        if (write) {
-            for (uint64_t i = 0; i < pieces; ++i) {
-                DTask ch = task;
-                ch.sel = b.sel;
-                ch.lo = b.lo + i * kExpandChunk;
-                ch.hi = min(b.hi, ch.lo + kExpandChunk);
-                out_tasks[pos++] = ch;
+        PrefixState st{I, M};
+        uint64_t pos = write ? out_pos[t] : 0;
+        LevelBase b;
+        bool fresh = task.hi == kAutoHi;
+        if (!fresh) {
+            b.lo = task.lo;
+            b.hi = task.hi;
+            b.sel = task.sel;
+        }
+        if (d >= lim) {
+            // at the limit: the task stays as it is
+            out = 1;
+            if (write) out_tasks[pos] = task;
+        } else {
+            expanded = 1;
+            while (true) {
+                unsigned long long rows = 0;
+                if (fresh) b = level_base(P, ops, row_off, d, st, bytes, rows);   // a node entered here
+                if (b.hi - b.lo > kExpandChunk) {
+                    // a long base range (a hub's row) is not walked by one
+                    // thread: it is cut into range tasks of at most
+                    // kExpandChunk positions at the same depth (no assignment
+                    // is made), expanded next round
+                    const uint64_t pieces = (b.hi - b.lo + kExpandChunk - 1) / kExpandChunk;
+                    if (write) {
+                        for (uint64_t i = 0; i < pieces; ++i) {
+                            DTask ch = task;
+                            ch.depth = d;
+                            for (uint32_t e = 0; e < d; ++e) ch.prefix[e] = I[e];
+                            ch.sel = b.sel;
+                            ch.lo = b.lo + i * kExpandChunk;
+                            ch.hi = min(b.hi, ch.lo + kExpandChunk);
+                            out_tasks[pos++] = ch;
+                        }
+                    }
+                    out = (uint32_t)pieces;
+                    algo += 4ull * rows;
+                    break;
+                }
+                // the valid extensions of this level: counted, and written
+                // unless the level turns out to have exactly one
+                uint32_t c = 0, first = 0;
+                for (uint64_t cur = b.lo; cur < b.hi; ++cur) {
+                    const uint32_t x = b.sel == kIdentity ? (uint32_t)cur : idx[cur];
+                    uint32_t v;
+                    const int r = test_element(P, ops, row_off, idx, cand_pool, d, x, b.sel, st, v, bytes);
+                    if (r != 1) {
+                        rej += r == 3;
+                        continue;
+                    }
+                    if (c == 0) first = x;
+                    ++c;
+                }
+                // each valid extension is one assignment at this depth (a
+                // call); those below the root level are written survivors
+                calls += c;
+                algo += 4ull * rows + (d >= 1 ? 4ull * c : 0ull);
+                if (c == 1 && d + 1 < lim) {
+                    // a single extension: assign it and go on
+                    I[d] = first;
+                    M[d] = cand_pool[P.cand_base[d] + first];
+                    ++d;
+                    fresh = true;
+                    continue;
+                }
+                if (write && c) {
+                    for (uint64_t cur = b.lo; cur < b.hi; ++cur) {
+                        const uint32_t x = b.sel == kIdentity ? (uint32_t)cur : idx[cur];
+                        uint32_t v;
+                        unsigned long long scratch_bytes = 0;
+                        if (test_element(P, ops, row_off, idx, cand_pool, d, x, b.sel, st, v, scratch_bytes) != 1)
+                            continue;
+                        DTask ch;
+                        ch.q = q;
+                        ch.depth = d + 1;
+                        ch.sel = 0;
+                        ch.pad = 0;
+                        ch.lo = 0;
+                        ch.hi = kAutoHi;
+                        for (uint32_t e = 0; e < kPrefixMax; ++e) ch.prefix[e] = e < d ? I[e] : 0;
+                        ch.prefix[d] = x;
+                        out_tasks[pos++] = ch;
+                    }
+                }
+                out = c;
+                break;
             }
-            atomicAdd(&qc[task.q].expand_bytes, bytes);
-            if (rows) atomicAdd(&qc[task.q].expand_algo_bytes, 4ull * rows);
-        } else {
-            out_count[t] = (uint32_t)pieces;
-            atomicAdd(qchild + task.q, (unsigned long long)pieces);
         }
-        return;
-    }
-    uint32_t c = 0, rej = 0;
-    for (uint64_t cur = b.lo; cur < b.hi; ++cur) {
-        const uint32_t x = b.sel == kIdentity ? (uint32_t)cur : idx[cur];
-        uint32_t v;
-        const int r = test_element(P, ops, row_off, idx, cand_pool, d, x, b.sel, st, v, bytes);
-        if (r != 1) {
-            rej += r == 3;
-            continue;
-        }
-        if (write) {
-            DTask ch;
-            ch.q = task.q;
-            ch.depth = d + 1;
-            ch.sel = 0;
-            ch.pad = 0;
-            ch.lo = 0;
-            ch.hi = kAutoHi;
-            for (uint32_t e = 0; e < kPrefixMax; ++e) ch.prefix[e] = e < d ? task.prefix[e] : 0;
-            ch.prefix[d] = x;
-            out_tasks[pos++] = ch;
-        }
-        ++c;
     }
     if (!write) {
-        out_count[t] = c;
-        atomicAdd(qchild + task.q, (unsigned long long)c);
-    } else {
-        // each child is one assignment at this depth; children below the
-        // root level are written survivors (expansion stops short of the leaf)
-        atomicAdd(&qc[task.q].calls, (unsigned long long)c);
-        if (rej) atomicAdd(&qc[task.q].rejects, (unsigned long long)rej);
-        atomicAdd(&qc[task.q].expand_bytes, bytes);
-        const unsigned long long algo = 4ull * rows + (d >= 1 ? 4ull * c : 0ull);
-        if (algo) atomicAdd(&qc[task.q].expand_algo_bytes, algo);
+        if (t < n_tasks) out_count[t] = out;
+        add_by_query(qchild, 1, q, out);
+        add_by_query(qchild + nq, 1, q, expanded);
+        return;
+    }
+    static_assert(sizeof(QCounters) % sizeof(unsigned long long) == 0, "QCounters is an array of u64 strides");
+    const size_t stride = sizeof(QCounters) / sizeof(unsigned long long);
+    add_by_query(&qc->calls, stride, q, calls);
+    add_by_query(&qc->rejects, stride, q, rej);
+    add_by_query(&qc->expand_bytes, stride, q, bytes);
+    add_by_query(&qc->expand_algo_bytes, stride, q, algo);
+}
+
+// Copies segments of task records: segment s moves src[src_off[s] ..
+// + len) to dst[dst_off[s] ..], where dst_off is the exclusive prefix of
+// the lengths (n_seg + 1 entries, the last one the total).
+__global__ __launch_bounds__(kBlock) void k_gather_tasks(const DTask* __restrict__ src,
+                                                          const uint64_t* __restrict__ src_off,
+                                                          const uint64_t* __restrict__ dst_off, uint32_t n_seg,
+                                                          DTask* __restrict__ dst) {
+    const uint64_t total = dst_off[n_seg];
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t sg = find_u64(dst_off, n_seg, i);
+        dst[i] = src[src_off[sg] + (i - dst_off[sg])];
     }
 }
 
@@ -1453,12 +1533,11 @@ constexpr uint32_t kClaimMax = 8;     // queue slots a lane takes per claim at m
 constexpr uint32_t kFewTasks = 64;
 constexpr uint32_t kShortTask = 1024; // steps below which a finished task counts as short for claims    // a query with fewer queued and running tasks splits while the ring is short
 
-// Adds a lane's running count (setting the query's done flag at its cap)
-// and statistics to query q's counters and clears them.
-__device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t cap, unsigned long long& lcount,
-                                           unsigned long long& lcalls, unsigned long long& lrej,
-                                           unsigned long long& lbytes, unsigned long long& lalgo,
-                                           bool& near_cap, unsigned long long* notify) {
+// Adds a lane's running count to query q's count (setting the query's done
+// flag at its cap) and clears it.
+__device__ __forceinline__ void lane_flush_count(QCounters* qc, uint32_t q, uint64_t cap,
+                                                 unsigned long long& lcount, bool& near_cap,
+                                                 unsigned long long* notify) {
     if (lcount) {
         const unsigned long long prev = atomicAdd(&qc[q].count, lcount);
         if (prev + lcount >= cap) {
@@ -1468,6 +1547,15 @@ __device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t c
         near_cap = prev + lcount >= cap / 2;
         lcount = 0;
     }
+}
+
+// Adds a lane's running count and statistics to query q's counters and
+// clears them.
+__device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t cap, unsigned long long& lcount,
+                                           unsigned long long& lcalls, unsigned long long& lrej,
+                                           unsigned long long& lbytes, unsigned long long& lalgo,
+                                           bool& near_cap, unsigned long long* notify) {
+    lane_flush_count(qc, q, cap, lcount, near_cap, notify);
     if (lcalls) atomicAdd(&qc[q].calls, lcalls);
     if (lrej) atomicAdd(&qc[q].rejects, lrej);
     if (lbytes) atomicAdd(&qc[q].probe_bytes, lbytes);
@@ -1525,6 +1613,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     Level Lp{};   // the level above the leaf depth, kept while the lane is at the leaf
     unsigned long long lcount = 0, lcalls = 0, lrej = 0, lbytes = 0, lalgo = 0;
     unsigned long long busy_iters = 0, all_iters = 0;
+    uint32_t ldonated = 0, lfull = 0;   // this lane's donations, and those skipped for a held slot
     unsigned long long flush_at = 4096;   // leaf counts a lane holds before adding them to its query
     bool near_cap = false;                // the query's count was at half its cap or more at the last flush
     bool hungry = true;                   // wavefront view: the queue holds fewer tasks than idle lanes
@@ -1588,9 +1677,15 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         const unsigned long long want = __ballot(!active && !waiting && !stop);
         // lanes whose previous task was short (small tasks, many claims)
         const unsigned long long quick = __ballot(!active && !waiting && last_steps < kShortTask);
-        if (want && backoff == 0) {
+        // a wavefront with running lanes claims for its idle ones on every
+        // 16th iteration only: every claim is a round trip to the queue's
+        // words, which the running lanes of the wavefront wait for, and
+        // thousands of wavefronts racing for each donated task slow the
+        // donors down; a wavefront with no running lane claims at once
+        const bool claim_now = (witer & 15) == 1 || __ballot(active) == 0;
+        if (want && backoff == 0 && claim_now) {
             const uint32_t leader = __ffsll((long long)want) - 1;
-            uint32_t base = 0, got = 0, per = 1, from_ring = 0;
+            uint32_t base = 0, got = 0, per = 1, from_ring = 0, empty = 0;
             if (lane == leader) {
                 const uint32_t nw = (uint32_t)__popcll(want);
                 const uint32_t h = load_relaxed(&Q->head), t = load_relaxed(&Q->tail);
@@ -1598,6 +1693,10 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 // claim so that empty queues with busy == 0 are only seen
                 // once no lane holds or is taking a task
                 if (t != h) {
+                    // ring positions are claimed only up to the tail (a
+                    // position past it may never be published), so a claim
+                    // that loses the race to another wavefront takes nothing
+                    // and is tried again on the next iteration
                     const uint32_t k = min(nw, t - h);
                     atomicAdd(&Q->busy, k);
                     if (atomicCAS(&Q->head, h, h + k) == h) {
@@ -1616,13 +1715,18 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                         if (2u * (uint32_t)__popcll(quick & want) >= nw)
                             per = min(kClaimMax, max(1u, (n0 - h0) / (claim_div * lanes_total)));
                         const uint32_t k = min(nw * per, n0 - h0);
+                        // the array is complete before the launch, so a claim
+                        // is one fetch-add that cannot fail: concurrent
+                        // claims get consecutive ranges, and one that starts
+                        // past n0 gets nothing (head0 overshoots n0 by at
+                        // most one claim per wavefront)
                         atomicAdd(&Q->busy, k);
-                        if (atomicCAS(&Q->head0, h0, h0 + k) == h0) {
-                            base = h0;
-                            got = k;
-                        } else {
-                            atomicSub(&Q->busy, k);
-                        }
+                        const uint32_t old = atomicAdd(&Q->head0, k);
+                        got = old < n0 ? min(k, n0 - old) : 0u;
+                        base = old;
+                        if (got < k) atomicSub(&Q->busy, k - got);
+                    } else {
+                        empty = 1;
                     }
                 }
             }
@@ -1630,6 +1734,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             got = __shfl(got, leader);
             per = __shfl(per, leader);
             from_ring = __shfl(from_ring, leader);
+            empty = __shfl(empty, leader);
             if (!active && !waiting) {
                 const uint32_t r = __popcll(want & lanemask_lt());
                 if (r * per < got) {
@@ -1639,7 +1744,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     wend = min(wslot + per, base + got);
                 }
             }
-            if (!got) backoff = 16;
+            // back off only when both queues were empty: a lost race is
+            // retried at once (the idle lanes have nothing else to do)
+            if (!got && empty) backoff = 16;
         } else if (backoff) {
             --backoff;
         }
@@ -1698,10 +1805,14 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         }
         if (__ballot(active || waiting) == 0) {
             // the whole wavefront is idle: leave once no work can appear
-            if (load_relaxed(&Q->timeout)) break;
-            if (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->head0) >= n0 &&
-                load_relaxed(&Q->busy) == 0)
-                break;
+            uint32_t leave = 0;
+            if (lane == 0)
+                leave = load_relaxed(&Q->timeout) != 0 ||
+                        (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->head0) >= n0 &&
+                         load_relaxed(&Q->busy) == 0);
+            if (__shfl(leave, 0)) break;
+            // a short nap: idle wavefronts must take donated work at once
+            // (longer waits between polls made lone searches slower)
             __builtin_amdgcn_s_sleep(4);
             continue;
         }
@@ -1809,7 +1920,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 const uint32_t t = load_relaxed(&Q->tail);
                 const bool free_slot = seq_load(seq, qmask, t) == t;
                 if (!free_slot) {
-                    atomicAdd(&Q->full, 1u);
+                    ++lfull;
                 } else if (atomicCAS(&Q->tail, t, t + 1) == t) {
                     // shallower levels hold offsets from their base row: locate it
                     uint64_t base = 0;
@@ -1836,7 +1947,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     seq_publish(seq, qmask, t);
                     if (e == d) L.end = mid;
                     else st.end_[st.at(e)] = (uint32_t)mid;
-                    atomicAdd(&Q->donated, 1u);
+                    ++ldonated;
                 }
             }
         }
@@ -1844,12 +1955,15 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         // claimed slot left (or moves to another query), not at every task
         // end: with millions of small tasks, device-scope atomics on the few
         // shared counters queue behind one another and every later load of
-        // the issuing wavefront waits for them. The running count goes to the
-        // query often enough for its cap.
+        // the issuing wavefront waits for them. The running count alone goes
+        // to the query as often as its cap needs (every flush_at embeddings);
+        // the statistics wait for the task's end or its next 1024 steps.
         const bool more = !active && wslot + 1 < wend;   // another claimed slot follows
         const bool idle_next = !active && !more;
-        if (lcount >= flush_at || idle_next || (steps & 1023) == 0)
+        if (idle_next || (steps & 1023) == 0)
             lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap, notify);
+        else if (lcount >= flush_at)
+            lane_flush_count(qc, q, P->cap, lcount, near_cap, notify);
         if (finished) ++pend_done;
         if (!active) {
             last_steps = steps;
@@ -1868,6 +1982,10 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap, notify);
     wave_add(&Q->lane_steps, busy_iters);
     wave_add(&Q->lane_iters, all_iters);
+    // statistics of the queue, once per wavefront (on the queue's line every
+    // update would contend with claims and donations)
+    wave_add_u32(&Q->donated, ldonated);
+    wave_add_u32(&Q->full, lfull);
 }
 
 // ---------------------------------------------------------------------------
@@ -2134,8 +2252,10 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
             } else {
                 const uint32_t h0 = load_relaxed(&Q->head0);
                 if (h0 < n0) {
+                    // one fetch-add on the complete array (as in k_dfs)
                     atomicAdd(&Q->busy, 1u);
-                    if (atomicCAS(&Q->head0, h0, h0 + 1) == h0) slot = h0;
+                    const uint32_t old = atomicAdd(&Q->head0, 1u);
+                    if (old < n0) slot = old;
                     else atomicSub(&Q->busy, 1u);
                 }
             }
@@ -3003,11 +3123,20 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
                          const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
                          uint32_t n_tasks, const uint8_t* expand_q, uint32_t* out_count,
                          const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
-                         unsigned long long* qchild, int write, hipStream_t s) {
+                         unsigned long long* qchild, uint32_t nq, int write, hipStream_t s) {
     if (n_tasks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_expand, dim3((n_tasks + kBlock - 1) / kBlock), dim3(kBlock), 0, s, plans,
                        ops, row_off, idx, cand_pool, tasks, n_tasks, expand_q, out_count, out_pos,
-                       out_tasks, qc, qchild, write);
+                       out_tasks, qc, qchild, nq, write);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_tasks(const DTask* src, const uint64_t* src_off, const uint64_t* dst_off,
+                               uint32_t n_seg, uint64_t total, DTask* dst, hipStream_t s) {
+    if (n_seg == 0 || total == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((total + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_gather_tasks, dim3((uint32_t)blocks), dim3(kBlock), 0, s, src, src_off, dst_off,
+                       n_seg, dst);
     return hipGetLastError();
 }
 

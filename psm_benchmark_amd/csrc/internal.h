@@ -11,7 +11,7 @@ constexpr uint32_t kInvalid = 0xFFFFFFFFu;   // pruned candidate
 constexpr uint32_t kMaxQ = 64;               // query vertices (config.h:9)
 constexpr uint32_t kMaxGqlDegree = 32;       // query degree limit of the GQL refinement kernel
 constexpr uint32_t kPrefixMax = 32;          // prefix entries a task record carries
-constexpr uint32_t kExpandMax = 8;           // depth limit of task pre-expansion
+constexpr uint32_t kExpandMax = kPrefixMax - 1;   // depth limit of task pre-expansion (a task at depth d carries d prefix entries)
 constexpr uint64_t kAutoHi = ~0ull;          // task range = the whole base list
 constexpr uint32_t kIdentity = 0xFFFFFFFFu;  // "base list is C(u) itself"
 
@@ -295,7 +295,11 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
                          const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
                          uint32_t n_tasks, const uint8_t* expand_q, uint32_t* out_count,
                          const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
-                         unsigned long long* qchild, int write, hipStream_t s);
+                         unsigned long long* qchild, uint32_t nq, int write, hipStream_t s);
+// Copies segments of task records (dst_off: exclusive prefix of the segment
+// lengths, n_seg + 1 entries; total = dst_off[n_seg]).
+hipError_t launch_gather_tasks(const DTask* src, const uint64_t* src_off, const uint64_t* dst_off,
+                               uint32_t n_seg, uint64_t total, DTask* dst, hipStream_t s);
 hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
                       const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue, uint32_t* seq,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,

@@ -1332,7 +1332,6 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
     HIP_TRY(ctx, hipMemsetAsync(qcnt.p, 0, sizeof(QCounters) * nq, s()));
 
     std::vector<DTask> tasks;
-    std::vector<uint32_t> depth(nq, 0);
     std::vector<uint64_t> ntask(nq, 0);
     uint32_t max_n = 1;
     for (uint32_t i = 0; i < nq; ++i) {
@@ -1355,22 +1354,72 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
     for (uint32_t i = 0; i < nq; ++i) active_q += ntask[i] ? 1 : 0;
     const uint64_t per_q_target = active_q ? std::max<uint64_t>(64, target_total / active_q) : 0;
 
-    DBuf d_tasks, d_tasks2, d_cnt, d_pos, d_tmp, d_expand, d_qchild;
+    // The array holds the tasks of the queries still being expanded, sorted
+    // by query (children follow their parents). A query whose expansion
+    // ends (enough tasks, or the depth limit) has its segment moved once to
+    // the final array, so each round reads only the tasks it expands.
+    DBuf d_tasks, d_tasks2, d_final, d_cnt, d_pos, d_tmp, d_expand, d_qchild, d_seg;
     HIP_TRY(ctx, upload(d_tasks, tasks, s()));
-    uint64_t n_tasks = tasks.size();
+    uint64_t n_tasks = tasks.size();   // tasks in d_tasks
+    uint64_t n_final = 0;              // tasks in d_final
+    std::vector<uint64_t> final_cnt(nq, 0);   // tasks per query in d_final
+    HIP_TRY(ctx, d_final.alloc(std::max<uint64_t>(n_tasks, 1) * sizeof(DTask)));
+    // moves the segments of the queries with retire[i] from d_tasks to the
+    // end of d_final (segments in query order, as in d_tasks)
+    auto retire_segments = [&](const std::vector<uint8_t>& retire) -> int {
+        std::vector<uint64_t> src, dst;
+        uint64_t off = 0, add = 0;
+        for (uint32_t i = 0; i < nq; ++i) {
+            if (!ntask[i]) continue;
+            if (retire[i]) {
+                src.push_back(off);
+                dst.push_back(add);
+                add += ntask[i];
+                final_cnt[i] += ntask[i];
+            }
+            off += ntask[i];
+        }
+        if (!add) return PSM_OK;
+        dst.push_back(add);
+        const uint32_t nseg = (uint32_t)src.size();
+        std::vector<uint64_t> seg(src);
+        seg.insert(seg.end(), dst.begin(), dst.end());
+        HIP_TRY(ctx, upload(d_seg, seg, s()));
+        HIP_TRY(ctx, d_tasks2.alloc((n_final + add) * sizeof(DTask)));
+        if (n_final)
+            HIP_TRY(ctx, hipMemcpyAsync(d_tasks2.p, d_final.p, n_final * sizeof(DTask), hipMemcpyDeviceToDevice,
+                                        s()));
+        HIP_TRY(ctx, launch_gather_tasks(d_tasks.as<DTask>(), d_seg.as<uint64_t>(), d_seg.as<uint64_t>() + nseg,
+                                         nseg, add, d_tasks2.as<DTask>() + n_final, s()));
+        d_final.swap(d_tasks2);
+        n_final += add;
+        return PSM_OK;
+    };
     const double te0 = now_ms();
-    for (uint32_t round = 0; round < kExpandMax && n_tasks; ++round) {
-        std::vector<uint8_t> ex(nq, 0);
+    // a query is expanded while it has fewer tasks than its share and some
+    // of its tasks are above the depth limit (the kernel leaves the others
+    // as they are); each round takes every task at least one level deeper
+    std::vector<uint8_t> progress(nq, 1);
+    for (uint32_t round = 0; round <= kExpandMax && n_tasks; ++round) {
+        std::vector<uint8_t> ex(nq, 0), retire(nq, 0);
         bool any = false;
         for (uint32_t i = 0; i < nq; ++i) {
             if (!ntask[i]) continue;
-            const uint32_t lim = std::min<uint32_t>(hp[i].n - 1, kExpandMax);
-            if (ntask[i] < per_q_target && depth[i] < lim) { ex[i] = 1; any = true; }
+            if (round < kExpandMax && ntask[i] < per_q_target && progress[i]) {
+                ex[i] = 1;
+                any = true;
+            } else {
+                retire[i] = 1;
+            }
         }
-        if (!any) break;
+        PSM_TRY(retire_segments(retire));
+        if (!any) {
+            n_tasks = 0;
+            break;
+        }
         HIP_TRY(ctx, upload(d_expand, ex, s()));
-        HIP_TRY(ctx, d_qchild.alloc((size_t)nq * 8));
-        HIP_TRY(ctx, hipMemsetAsync(d_qchild.p, 0, (size_t)nq * 8, s()));
+        HIP_TRY(ctx, d_qchild.alloc((size_t)nq * 16));
+        HIP_TRY(ctx, hipMemsetAsync(d_qchild.p, 0, (size_t)nq * 16, s()));
         HIP_TRY(ctx, d_cnt.alloc(n_tasks * 4));
         HIP_TRY(ctx, d_pos.alloc((n_tasks + 1) * 8));
         HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(n_tasks) * 8));
@@ -1378,28 +1427,46 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
               launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
                             cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
                             d_expand.as<uint8_t>(), d_cnt.as<uint32_t>(), nullptr, nullptr,
-                            qcnt.as<QCounters>(), d_qchild.as<unsigned long long>(), 0, s()));
+                            qcnt.as<QCounters>(), d_qchild.as<unsigned long long>(), nq, 0, s()));
         HIP_TRY(ctx, launch_scan_u32(d_cnt.as<uint32_t>(), n_tasks, d_pos.as<uint64_t>(),
                                      d_tmp.as<unsigned long long>(), s()));
-        uint64_t total = 0;
-        HIP_TRY(ctx, hipMemcpyAsync(&total, d_pos.as<uint64_t>() + n_tasks, 8, hipMemcpyDeviceToHost, s()));
-        HIP_TRY(ctx, hipStreamSynchronize(s()));
-        if (total > budget_total || total > 0x7FFFFFFFull) break;
+        // the tasks per query that replace this round's (one read-back per
+        // round; their sum is the size of the next array) and how many of
+        // its tasks were expanded
         std::vector<unsigned long long> qchild;
-        HIP_TRY(ctx, download(qchild, d_qchild.p, nq, s()));
+        HIP_TRY(ctx, download(qchild, d_qchild.p, 2 * (size_t)nq, s()));
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < nq; ++i)
+            if (ex[i]) total += qchild[i];
+        if (total > budget_total || total > 0x7FFFFFFFull) {
+            // too many: the expanding queries keep the tasks they have
+            PSM_TRY(retire_segments(ex));
+            n_tasks = 0;
+            break;
+        }
         HIP_TRY(ctx, d_tasks2.alloc(std::max<uint64_t>(total, 1) * sizeof(DTask)));
         TIMED(ctx, kKExpand,
               launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
                             cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
                             d_expand.as<uint8_t>(), nullptr, d_pos.as<uint64_t>(), d_tasks2.as<DTask>(),
-                            qcnt.as<QCounters>(), nullptr, 1, s()));
+                            qcnt.as<QCounters>(), nullptr, nq, 1, s()));
         d_tasks.swap(d_tasks2);
+        verbose("expand round %u: %llu tasks -> %llu children, %.2f ms so far", round,
+                (unsigned long long)n_tasks, (unsigned long long)total, now_ms() - te0);
         for (uint32_t i = 0; i < nq; ++i) {
-            if (ex[i]) depth[i]++;
-            ntask[i] = qchild[i];
+            if (ex[i]) {
+                ntask[i] = qchild[i];
+                progress[i] = qchild[nq + i] != 0;
+            } else {
+                ntask[i] = 0;
+            }
         }
         n_tasks = total;
     }
+    // the final task array, in the order the queries' expansions ended
+    d_tasks.swap(d_final);
+    n_tasks = n_final;
+    ntask = final_cnt;
     HIP_TRY(ctx, timers_flush(ctx));
     t_expand += now_ms() - te0;
     verbose("expand: %llu tasks in %.1f ms", (unsigned long long)n_tasks, now_ms() - te0);
