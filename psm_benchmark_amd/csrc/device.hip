@@ -267,6 +267,38 @@ __device__ __forceinline__ uint32_t left_mask(const unsigned long long* __restri
 
 constexpr uint32_t kScanUnroll = 4;   // 64-neighbour chunks a wavefront loads before testing
 
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+    for (int o = kWave / 2; o > 0; o >>= 1) x = min(x, (uint32_t)__shfl_xor(x, o));
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t x) {
+    for (int o = kWave / 2; o > 0; o >>= 1) x |= (uint32_t)__shfl_xor(x, o);
+    return x;
+}
+
+// The run index of concatenated position p: the largest li < nlab with
+// start[li] <= p, where lane li holds start[li] (ascending). The loop count
+// depends on nlab only, so every lane of the wavefront runs it (it calls
+// __shfl with a per-lane source).
+__device__ __forceinline__ uint32_t run_of(uint32_t start, uint32_t nlab, uint32_t p) {
+    uint32_t lo = 0;
+    for (uint32_t s = 32; s > 0; s >>= 1)
+        if (s < nlab) {
+            const uint32_t st = __shfl(start, min(lo + s, nlab - 1));
+            if (lo + s < nlab && st <= p) lo += s;
+        }
+    return lo;
+}
+
+// The first concatenated position at or after `from` inside a run whose bit
+// is set in `labels` (lane li holds its run's start and length), or
+// kInvalid. Every lane calls it.
+__device__ __forceinline__ uint32_t next_pos(uint32_t start, uint32_t len, uint32_t labels, uint32_t from) {
+    const uint32_t lane = lane_id();
+    const bool in = lane < 32 && ((labels >> lane) & 1u) && start + len > from;
+    return wave_min_u32(in ? max(start, from) : kInvalid);
+}
+
 // One candidate (item it, index j) per wavefront; returns its algorithmic
 // bytes (on every lane). wl: the wavefront's index in its block.
 __device__ __forceinline__ unsigned long long gql_refine_one(
@@ -278,76 +310,146 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
     const uint32_t v = cand_pool[it.cand_begin + j];
     if (v == kInvalid) return 0;
     const uint32_t k = it.k;
-    const uint32_t* lslots = left_slots + it.left_begin;   // uniform: scalar loads
-    const uint32_t* labs = left_labs + it.lab_begin;
+    const uint32_t nlab = it.nlab;   // distinct left labels, at most k <= 32
+    // Lane i < k holds left vertex i's bit in a membership word and lane
+    // li < nlab the li-th distinct left label (one load each per candidate;
+    // the loops below read them with readlane instead of reloading them)
+    const uint32_t my_shift = lane < k ? left_slots[it.left_begin + lane] - (it.bm_self - it.u) : 0u;
+    const uint32_t my_want = lane < nlab ? left_labs[it.lab_begin + lane] : kInvalid;
     // Right vertices can only be neighbours carrying a left vertex's label,
     // and the neighbours of one label can only serve the left vertices of
     // that label (candidate sets are label-pure): the scans below walk the
-    // label runs of the label-grouped adjacency one label at a time.
-    // pass 1: lane i (< k) counts the right neighbours of left vertex i,
-    // saturating at k. A left vertex with k right neighbours can always be
-    // matched last (each in turn still finds an unused one), so a run is left
-    // as soon as every left vertex of its label has k; a left vertex whose
-    // run holds none fails the candidate at once.
+    // runs of the left labels in the label-grouped adjacency, taken one
+    // after the other as one sequence.
+    // Lane li < nlab gets the run (start in adj_bl, length) of its label my_want
+    // among v's neighbours: v's runs are read 64 at a time (coalesced) and
+    // matched against the wanted labels by ballots.
+    uint32_t run_x = 0, run_y = 0;
+    {
+        const uint32_t r0 = g.nlf_off[v], r1 = g.nlf_off[v + 1];
+        for (uint32_t base = r0; base < r1; base += kWave) {
+            const uint32_t jj = base + lane;
+            uint32_t lab_j = kInvalid, beg_j = 0, cnt_j = 0;
+            if (jj < r1) {
+                const uint2 r = g.nlf[jj];
+                lab_j = r.x;
+                cnt_j = r.y;
+                beg_j = g.nlf_beg[jj];
+            }
+            for (uint32_t li = 0; li < nlab; ++li) {
+                const unsigned long long bal =
+                    __ballot(lab_j == (uint32_t)__builtin_amdgcn_readlane((int)my_want, (int)li));
+                if (bal) {
+                    const uint32_t src = __ffsll((long long)bal) - 1;
+                    const uint32_t x = __shfl(beg_j, src), y = __shfl(cnt_j, src);
+                    if (lane == li) {
+                        run_x = x;
+                        run_y = y;
+                    }
+                }
+            }
+        }
+    }
+    // the runs' positions in the concatenation (exclusive prefix)
+    uint32_t incl = lane < nlab ? run_y : 0u;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if ((int)lane >= o) incl += t;
+    }
+    const uint32_t start = incl - (lane < nlab ? run_y : 0u);
+    const uint32_t total = __shfl(incl, kWave - 1);
+    // lane i < k: left vertex i, its label's run index and that run's end,
+    // and how many left vertices share its label
     const uint32_t my_lab = lane < k ? left_lab[it.left_begin + lane] : kInvalid;
-    // lane li < nlab locates the run of left label li: the lookups (binary
-    // searches over v's label runs) proceed side by side, not one per label
-    // in turn; the loops below take each run from its lane
-    const uint2 my_run = lane < it.nlab ? label_run(g, v, labs[lane]) : make_uint2(0u, 0u);
+    uint32_t my_li = 0, my_sat = 0;
+    for (uint32_t li = 0; li < nlab; ++li) {
+        const uint32_t want = (uint32_t)__builtin_amdgcn_readlane((int)my_want, (int)li);
+        const uint32_t same = (uint32_t)__popcll(__ballot(my_lab == want));
+        if (my_lab == want) {
+            my_li = li;
+            my_sat = same;
+        }
+    }
+    const uint32_t my_end = __shfl(incl, my_li);
+    const uint32_t my_len = __shfl(run_y, my_li);
+    // pass 1: lane i (< k) counts the right neighbours of left vertex i,
+    // saturating at my_sat, the number of left vertices with its label. A
+    // right vertex serves only left vertices of its own label, so the
+    // bipartite graph is a disjoint union of one part per label, and in its
+    // part a left vertex with my_sat right neighbours can always be matched
+    // last (each in turn still finds an unused one). The rest of a run is
+    // skipped once every left vertex of its label is saturated; a left
+    // vertex whose run holds no right neighbour fails the candidate. A step
+    // loads kScanUnroll chunks of 64 positions together (their neighbour and
+    // membership loads are independent and in flight at once).
+    bool ok = __ballot(lane < k && my_len == 0) == 0;
     uint32_t my_cnt = 0;
     uint32_t scanned = 0, rescanned = 0;
-    bool ok = true;
-    for (uint32_t li = 0; li < it.nlab && ok; ++li) {
-        const bool mine = my_lab == labs[li];
-        const uint2 run = make_uint2(__shfl(my_run.x, li), __shfl(my_run.y, li));
-        // kScanUnroll chunks per step: their neighbour and membership loads
-        // are independent and in flight together
-        for (uint32_t o = 0; o < run.y; o += kScanUnroll * kWave) {
-            uint32_t w[kScanUnroll];
-            unsigned long long mw[kScanUnroll];
+    const uint32_t all_labels = nlab >= 32 ? 0xFFFFFFFFu : (1u << nlab) - 1u;
+    uint32_t o = ok ? next_pos(start, run_y, all_labels, 0u) : kInvalid;
+    while (o < total) {
+        uint32_t w[kScanUnroll];
+        unsigned long long mw[kScanUnroll];
 #pragma unroll
-            for (uint32_t c = 0; c < kScanUnroll; ++c) {
-                const uint32_t p = o + c * kWave + lane;
-                w[c] = p < run.y ? g.adj_bl[run.x + p] : kInvalid;
-            }
-#pragma unroll
-            for (uint32_t c = 0; c < kScanUnroll; ++c) mw[c] = w[c] != kInvalid ? member[it.mem_base + w[c]] : 0ull;
-            uint32_t m[kScanUnroll];
-#pragma unroll
-            for (uint32_t c = 0; c < kScanUnroll; ++c) m[c] = mask_of(mw[c], it, lslots, k);
-            for (uint32_t i = 0; i < k; ++i) {
-                uint32_t cnt = 0;
-#pragma unroll
-                for (uint32_t c = 0; c < kScanUnroll; ++c) cnt += __popcll(__ballot((m[c] >> i) & 1u));
-                if (lane == i) my_cnt = min(my_cnt + cnt, k);
-            }
-            scanned += min(kScanUnroll * kWave, run.y - o);
-            if (__ballot(mine && my_cnt < k) == 0) break;
+        for (uint32_t c = 0; c < kScanUnroll; ++c) {
+            const uint32_t p = o + c * kWave + lane;
+            const uint32_t li = run_of(start, nlab, min(p, total - 1));
+            const uint32_t rx = __shfl(run_x, li), rs = __shfl(start, li);
+            w[c] = p < total ? g.adj_bl[rx + (p - rs)] : kInvalid;
         }
-        if (__ballot(mine && my_cnt == 0)) ok = false;
+#pragma unroll
+        for (uint32_t c = 0; c < kScanUnroll; ++c) mw[c] = w[c] != kInvalid ? member[it.mem_base + w[c]] : 0ull;
+        for (uint32_t i = 0; i < k; ++i) {
+            const uint32_t sh_i = (uint32_t)__builtin_amdgcn_readlane((int)my_shift, (int)i);
+            uint32_t cnt = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < kScanUnroll; ++c) cnt += __popcll(__ballot((mw[c] >> sh_i) & 1ull));
+            if (lane == i) my_cnt = min(my_cnt + cnt, my_sat);
+        }
+        const uint32_t nxt = o + kScanUnroll * kWave;
+        scanned += min(kScanUnroll * kWave, total - o);
+        // a left vertex whose run has been passed without a right neighbour
+        if (__ballot(lane < k && my_cnt == 0 && my_end <= nxt)) {
+            ok = false;
+            break;
+        }
+        // labels that still have an unsaturated left vertex
+        const uint32_t unsat = wave_or_u32(lane < k && my_cnt < my_sat ? 1u << my_li : 0u);
+        if (!unsat) break;
+        o = next_pos(start, run_y, unsat, nxt);
     }
-    const uint32_t T = ok ? (uint32_t)__ballot(lane < k && my_cnt < k) : 0u;
+    // every left vertex needs a right neighbour (the scan above leaves no
+    // run of an unsaturated left vertex unread)
+    if (ok && __ballot(lane < k && my_cnt == 0)) ok = false;
+    const uint32_t T = ok ? (uint32_t)__ballot(lane < k && my_cnt < my_sat) : 0u;
     const bool pass2 = ok && T;
     if (pass2) {
         // pass 2: right vertices adjacent to T (only the runs of T's labels),
         // compacted into LDS
+        const uint32_t tlabels = wave_or_u32(((T >> lane) & 1u) ? 1u << my_li : 0u);
         uint32_t nr = 0;
-        for (uint32_t li = 0; li < it.nlab; ++li) {
-            if (__ballot(((T >> lane) & 1u) && my_lab == labs[li]) == 0) continue;
-            const uint2 run = make_uint2(__shfl(my_run.x, li), __shfl(my_run.y, li));
-            rescanned += run.y;
-            for (uint32_t o = 0; o < run.y; o += kWave) {
-                const uint32_t p = o + lane;
-                const uint32_t m = p < run.y ? (left_mask(member, it, lslots, k, g.adj_bl[run.x + p]) & T) : 0u;
-                const unsigned long long bal = __ballot(m != 0);
-                if (m) {
-                    const uint32_t r = nr + __popcll(bal & lanemask_lt());
-                    if (r < kRightCap) sh.rmask[wl][r] = m;
-                }
-                nr += __popcll(bal);
+        uint32_t o2 = next_pos(start, run_y, tlabels, 0u);
+        while (o2 < total) {
+            const uint32_t p = o2 + lane;
+            const uint32_t li = run_of(start, nlab, min(p, total - 1));
+            const uint32_t rx = __shfl(run_x, li), rs = __shfl(start, li);
+            const bool in = p < total && ((tlabels >> li) & 1u);
+            const unsigned long long mw = in ? member[it.mem_base + g.adj_bl[rx + (p - rs)]] : 0ull;
+            uint32_t m = 0;
+            for (uint32_t tt = T; tt; tt &= tt - 1) {
+                const uint32_t i = __ffs(tt) - 1;
+                m |= (uint32_t)((mw >> (uint32_t)__builtin_amdgcn_readlane((int)my_shift, (int)i)) & 1ull) << i;
             }
+            rescanned += (uint32_t)__popcll(__ballot(in));
+            const unsigned long long bal = __ballot(m != 0);
+            if (m) {
+                const uint32_t r = nr + __popcll(bal & lanemask_lt());
+                if (r < kRightCap) sh.rmask[wl][r] = m;
+            }
+            nr += __popcll(bal);
+            o2 = next_pos(start, run_y, tlabels, o2 + kWave);
         }
-        nr = min(nr, (uint32_t)kRightCap);  // bounded by 31*31 < kRightCap
+        nr = min(nr, (uint32_t)kRightCap);  // each of T's vertices has < 32 right neighbours: < 32*32
         for (uint32_t r = lane; r < nr; r += kWave) sh.match_r[wl][r] = 0xFF;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         // Kuhn: augment from each left vertex of T in turn
@@ -361,11 +463,11 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
             while (sp >= 0) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 const uint32_t l = sh.st_left[wl][sp];
-                const uint32_t start = sh.st_pos[wl][sp];
+                const uint32_t st = sh.st_pos[wl][sp];
                 uint32_t hit = kInvalid;
-                for (uint32_t c0 = start & ~(uint32_t)(kWave - 1); c0 < nr; c0 += kWave) {
+                for (uint32_t c0 = st & ~(uint32_t)(kWave - 1); c0 < nr; c0 += kWave) {
                     const uint32_t r = c0 + lane;
-                    const bool h = r >= start && r < nr && ((sh.rmask[wl][r] >> l) & 1u) &&
+                    const bool h = r >= st && r < nr && ((sh.rmask[wl][r] >> l) & 1u) &&
                                    !sh.seen[wl][r];
                     const unsigned long long bal = __ballot(h);
                     if (bal) { hit = c0 + __ffsll((long long)bal) - 1; break; }
@@ -397,7 +499,7 @@ __device__ __forceinline__ unsigned long long gql_refine_one(
     // algorithmic bytes: candidate, one run lookup per left label, the
     // scanned part of the label runs, one membership word per (neighbour,
     // left vertex) probe; pass 2 rescans the runs when it runs
-    return 4ull + 16ull * it.nlab * (pass2 ? 2 : 1) + 4ull * (scanned + rescanned) * (1 + k);
+    return 4ull + 16ull * nlab * (pass2 ? 2 : 1) + 4ull * (scanned + rescanned) * (1 + k);
 }
 
 // The wavefront test over the deferred (item, index) pairs of class 3; the
@@ -422,7 +524,8 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine(
         w = __shfl(w, 0);
         if (w >= n) break;
         const uint2 e = list[w];
-        by += gql_refine_one(g, items[e.x], e.y, left_slots, left_lab, left_labs, cand_pool, bitmaps, wps, member,
+        const GItem it = items[e.x];   // in registers for the whole test
+        by += gql_refine_one(g, it, e.y, left_slots, left_lab, left_labs, cand_pool, bitmaps, wps, member,
                              rd.removed, sh, wl);
     }
     block_add(bytes, lane_id() == 0 ? by : 0ull);
