@@ -248,22 +248,12 @@ struct GqlShared {
     uint32_t st_via[kGqlWavesPerBlock][kMaxGqlDegree + 1];
 };
 
-// Bit i set when data vertex w is a live candidate of left vertex i (the
-// i-th query neighbour of u, bitmap slot lslots[i] = first slot + its id):
-// one membership word per w instead of one bitmap word per left vertex.
-__device__ __forceinline__ uint32_t mask_of(unsigned long long mw, const GItem& it,
-                                            const uint32_t* __restrict__ lslots, uint32_t k) {
-    const uint32_t first = it.bm_self - it.u;
-    uint32_t m = 0;
-    for (uint32_t i = 0; i < k; ++i) m |= (uint32_t)((mw >> (lslots[i] - first)) & 1ull) << i;
-    return m;
-}
-
-__device__ __forceinline__ uint32_t left_mask(const unsigned long long* __restrict__ member,
-                                              const GItem& it, const uint32_t* __restrict__ lslots,
-                                              uint32_t k, uint32_t w) {
-    return mask_of(member[it.mem_base + w], it, lslots, k);
-}
+// Membership words: bit u of member[mem_base + w] is set when data vertex w
+// is a live candidate of query vertex u, so the left vertices a neighbour w
+// can serve come from one load; left vertex i of an item (the i-th query
+// neighbour of u, bitmap slot left_slots[left_begin + i]) owns bit
+// left_slots[left_begin + i] - (bm_self - u). The kernels keep those bit
+// positions in registers, one per lane.
 
 constexpr uint32_t kScanUnroll = 4;   // 64-neighbour chunks a wavefront loads before testing
 
@@ -568,6 +558,20 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long bal,
     else return (bal >> gbase) & ((1ull << G) - 1ull);
 }
 
+// The k-bit left mask of a membership word for a G-lane group: lane i < k of
+// the group holds left vertex i's bit position (my_shift), read by shuffles
+// within the group. Every lane of the wavefront calls it.
+template <uint32_t G>
+__device__ __forceinline__ uint32_t group_mask(unsigned long long mw, uint32_t my_shift, uint32_t k) {
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kGroup; ++i) {
+        const uint32_t sh = __shfl(my_shift, i, G);
+        if (i < k) m |= (uint32_t)((mw >> sh) & 1ull) << i;
+    }
+    return m;
+}
+
 // The neighbours of v that can be right vertices of item it are the label
 // runs of its left labels in adj_bl, taken one after the other (at most
 // kGroup labels here: callers use the positions only for k <= kGroup, and
@@ -702,21 +706,25 @@ __global__ __launch_bounds__(kBlock) void k_gql_hall(
     for (uint64_t first = (uint64_t)blockIdx.x * kPerIter; first < n; first += (uint64_t)gridDim.x * kPerIter) {
         const uint64_t gid = first + threadIdx.x / G;
         bool live = gid < n;
-        uint32_t j = 0, v = kInvalid, k = 0, deg = 0;
+        uint32_t j = 0, v = kInvalid, k = 0, deg = 0, my_shift = 0;
         GItem it{};
         if (live) {
             const uint2 e = list[gid];
             it = items[e.x];
             j = e.y;
             v = cand_pool[it.cand_begin + j];
+            // lane lig < k: left vertex lig's bit in a membership word
+            if (lig < min(it.k, kGroup)) my_shift = left_slots[it.left_begin + lig] - (it.bm_self - it.u);
             live = v != kInvalid;
         }
         uint32_t pos = 0;
         // relevant degree <= G by construction of the list
         deg = min(group_relevant<G>(g, it, left_labs, v, live, lig, pos), G);
         if (live) k = min(it.k, kGroup);
-        uint32_t m = 0;
-        if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj_bl[pos]);
+        const bool has = live && lig < deg;
+        const unsigned long long mw = has ? member[it.mem_base + g.adj_bl[pos]] : 0ull;
+        const uint32_t m0 = group_mask<G>(mw, my_shift, k);   // every lane shuffles
+        const uint32_t m = has ? m0 : 0u;
         const bool fail = hall_fails<G>(m, live, k, lig, gbase);
         if (live && lig == 0) {
             if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member, rd.removed);
@@ -792,8 +800,11 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
             j = (uint32_t)gid - it.wave_start;
             live = j < it.len && !(rd.skip && rd.skip[ii]);
         }
+        uint32_t my_shift = 0;
         if (live) {
             v = cand_pool[it.cand_begin + j];
+            // lane lig < k: left vertex lig's bit in a membership word
+            if (lig < min(it.k, kGroup)) my_shift = left_slots[it.left_begin + lig] - (it.bm_self - it.u);
             live = v != kInvalid;
         }
         uint32_t pos = 0;
@@ -813,8 +824,10 @@ __global__ __launch_bounds__(kBlock) void k_gql_refine_small(
             if (dc == c) st.buf[c][base + __popcll(bal & lanemask_lt())] = make_uint2(ii, j);
         }
         if (live && (deg > kGroup || k > kGroup)) live = false;
-        uint32_t m = 0;
-        if (live && lig < deg) m = left_mask(member, it, left_slots + it.left_begin, k, g.adj_bl[pos]);
+        const bool has = live && lig < deg;
+        const unsigned long long mw = has ? member[it.mem_base + g.adj_bl[pos]] : 0ull;
+        const uint32_t m0 = group_mask<kGroup>(mw, my_shift, min(k, kGroup));
+        const uint32_t m = has ? m0 : 0u;
         const bool fail = hall_fails<kGroup>(m, live, k, lig, gbase);
         if (live && lig == 0) {
             if (fail) gql_drop(it, j, v, cand_pool, bitmaps, wps, member, rd.removed);
