@@ -106,8 +106,12 @@ def _pmc_rows(path: str, counter: str):
 
 
 def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, kernel, out_json):
-    """HBM traffic per launch of `kernel`: FETCH_SIZE and WRITE_SIZE (KiB per
-    dispatch) scaled by what the same counters read for the copy calibration."""
+    """HBM traffic per launch of the kernel class `kernel` (a name of KERNELS:
+    its dispatches are those whose names match KERNEL_MATCH[kernel]):
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the same
+    counters read for the copy calibration."""
+    import re
+    pattern = re.compile(KERNEL_MATCH.get(kernel, re.escape(kernel)))
     def avg(d, c, match):
         v = [x for n, x in _pmc_rows(d, c) if match(n)]
         return (sum(v) / len(v), len(v)) if v else (float("nan"), 0)
@@ -121,11 +125,11 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, kernel, ou
                     copy_bytes = json.loads(line)["copy_bytes"]
     cf, ncf = avg(copy_fetch_dir, "FETCH_SIZE", is_copy)
     cw, ncw = avg(copy_write_dir, "WRITE_SIZE", is_copy)
-    kf, nkf = avg(fetch_dir, "FETCH_SIZE", lambda n: kernel in n)
-    kw, nkw = avg(write_dir, "WRITE_SIZE", lambda n: kernel in n)
+    kf, nkf = avg(fetch_dir, "FETCH_SIZE", lambda n: pattern.search(n) is not None)
+    kw, nkw = avg(write_dir, "WRITE_SIZE", lambda n: pattern.search(n) is not None)
     sf = copy_bytes / (cf * 1024) if copy_bytes else float("nan")
     sw = copy_bytes / (cw * 1024) if copy_bytes else float("nan")
-    res = dict(kernel=kernel, fetch_kib=kf, write_kib=kw, dispatches=[nkf, nkw],
+    res = dict(kernel=kernel, match=pattern.pattern, fetch_kib=kf, write_kib=kw, dispatches=[nkf, nkw],
                copy_bytes=copy_bytes, copy_fetch_kib=cf, copy_write_kib=cw, copy_dispatches=[ncf, ncw],
                scale_fetch=sf, scale_write=sw,
                traffic_bytes=kf * 1024 * sf + kw * 1024 * sw)
@@ -175,8 +179,9 @@ def main():
     ap.add_argument("--pmc-summary", nargs=6,
                     metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR", "KERNEL", "OUT"),
                     help="summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs into OUT (json)")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_dfs_c2_batched_traffic.json"),
-                    help="HBM traffic summary of the dominant kernel (from --pmc-summary)")
+    ap.add_argument("--traffic-dir", default=os.path.join(ROOT, "profiles"),
+                    help="where the HBM traffic summaries (<class>_c2_batched_traffic.json, from "
+                         "--pmc-summary) of the dominant kernel class are read")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, 8)
@@ -304,9 +309,10 @@ def main():
     dom = max(ks.values(), key=lambda k: k["ms"])
     achieved = dom["bytes"] / dom["launches"] / (dom["ms"] / dom["launches"] / 1e3) / 1e9
     traffic = None
-    if os.path.exists(args.traffic):
-        tr = json.load(open(args.traffic))
-        if tr.get("kernel") == "k_" + dom["name"] and tr.get("traffic_bytes") == tr.get("traffic_bytes"):
+    tfile = os.path.join(args.traffic_dir, f"{dom['name']}_c2_batched_traffic.json")
+    if os.path.exists(tfile):
+        tr = json.load(open(tfile))
+        if tr.get("kernel") == dom["name"] and tr.get("traffic_bytes") == tr.get("traffic_bytes"):
             traffic = int(tr["traffic_bytes"])
     roofline = dict(kernel=dom["name"], bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
@@ -317,6 +323,11 @@ def main():
                                         "operand row + 4 B per survivor written below the root unless "
                                         "a leaf (kernel-counted; equals the oracle's sum when uncapped)")
         roofline["touched_bytes_per_launch"] = int(sum(s.probe_bytes for s in stats) / args.steps)
+    elif dom["name"] == "gql_refine":
+        roofline["bytes_definition"] = ("GQL refinement (all its kernels; launch = one kernel of a (round, query "
+                                        "vertex) group): per candidate tested 4 B + 16 B per left-label run "
+                                        "located + 4 B x (1 + k) per label-run entry scanned (the neighbour and "
+                                        "its membership word per left vertex), kernel-counted")
 
     cpu = cpu_n = gpu_sample = None
     if not args.no_cpu:
@@ -502,7 +513,7 @@ def roofline_of(ks, steps, traffic_file=None):
     traffic = None
     if traffic_file and os.path.exists(traffic_file):
         tr = json.load(open(traffic_file))
-        if tr.get("kernel") == "k_" + dom["name"] and tr.get("traffic_bytes") == tr.get("traffic_bytes"):
+        if tr.get("kernel") == dom["name"] and tr.get("traffic_bytes") == tr.get("traffic_bytes"):
             traffic = int(tr["traffic_bytes"])
     return dict(kernel=dom["name"], bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                 unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
@@ -786,6 +797,10 @@ def c4_batch_workload(args):
 
 
 KERNELS = ["gql_refine", "table_count", "table_fill", "dfs", "expand", "compact", "bitmap"]
+# the device kernels of each class, by name (rocprofv3 counter summaries)
+KERNEL_MATCH = {"gql_refine": r"k_gql_", "table_count": r"k_table_count", "table_fill": r"k_table_fill",
+                "dfs": r"k_dfs", "expand": r"k_expand", "compact": r"k_compact_",
+                "bitmap": r"k_set_(bits|member)|k_slot_prefix"}
 
 
 def kernel_stats(*devs):
