@@ -105,12 +105,62 @@ def test_pmc_summary_scales_counters_by_the_copy_calibration(tmp_path):
         with open(os.path.join(d[k], "copy.json"), "w") as fh:
             fh.write(json.dumps({"copy_bytes": 1 << 20, "reps": 1}) + "\n")
     out = str(tmp_path / "t.json")
-    bench.pmc_summary(d["kf"], d["kw"], d["cf"], d["cw"], "k_dfs", out)
+    bench.pmc_summary(d["kf"], d["kw"], d["cf"], d["cw"], "dfs", out)
     r = json.load(open(out))
+    assert r["kernel"] == "dfs"
     assert r["dispatches"] == [2, 1]
     assert r["fetch_kib"] == 500.0 and r["write_kib"] == 50.0
     assert r["scale_fetch"] == 2.0 and r["scale_write"] == 1.0
     assert r["traffic_bytes"] == 500 * 1024 * 2 + 50 * 1024
+
+
+def test_pmc_summary_averages_a_kernel_class_over_its_kernels(tmp_path):
+    # the GQL refinement class spans several kernels: all of their dispatches
+    # count, other kernels do not
+    sys.path.insert(0, ROOT)
+    import bench
+    d = {k: str(tmp_path / k) for k in ("kf", "kw", "cf", "cw")}
+    _write_counters(d["kf"], [
+        dict(Dispatch_Id=1, Kernel_Name="psm::k_gql_refine_small(...)", Counter_Name="FETCH_SIZE", Counter_Value=100),
+        dict(Dispatch_Id=2, Kernel_Name="void psm::k_gql_hall<16u>(...)", Counter_Name="FETCH_SIZE", Counter_Value=300),
+        dict(Dispatch_Id=3, Kernel_Name="psm::k_table_fill(...)", Counter_Name="FETCH_SIZE", Counter_Value=9999),
+    ])
+    _write_counters(d["kw"], [
+        dict(Dispatch_Id=1, Kernel_Name="psm::k_gql_refine(...)", Counter_Name="WRITE_SIZE", Counter_Value=8),
+        dict(Dispatch_Id=2, Kernel_Name="psm::k_gql_skip(...)", Counter_Name="WRITE_SIZE", Counter_Value=2),
+    ])
+    _write_counters(d["cf"], [dict(Dispatch_Id=1, Kernel_Name="elementwise_copy", Counter_Name="FETCH_SIZE",
+                                   Counter_Value=1024)])
+    _write_counters(d["cw"], [dict(Dispatch_Id=1, Kernel_Name="elementwise_copy", Counter_Name="WRITE_SIZE",
+                                   Counter_Value=1024)])
+    for k in ("cf", "cw"):
+        with open(os.path.join(d[k], "copy.json"), "w") as fh:
+            fh.write(json.dumps({"copy_bytes": 1 << 20, "reps": 1}) + "\n")
+    out = str(tmp_path / "g.json")
+    bench.pmc_summary(d["kf"], d["kw"], d["cf"], d["cw"], "gql_refine", out)
+    r = json.load(open(out))
+    assert r["kernel"] == "gql_refine" and r["dispatches"] == [2, 2]
+    assert r["fetch_kib"] == 200.0 and r["write_kib"] == 5.0
+    assert r["traffic_bytes"] == 205 * 1024
+
+
+def test_c2_queries_follow_the_dense_sparse_rule():
+    # even-indexed queries are dense (average degree above 3; K4 at 4
+    # vertices, the densest 4-vertex graph), odd-indexed ones are the trees
+    # of random walks (average degree 2(n-1)/n <= 3); sizes cycle through
+    # 4/8/16/24/32; every query is a connected subgraph of the data graph
+    g = W.data_graph("C2")
+    qs, meta = W.c2_queries(g, count=40)
+    for i, (q, m) in enumerate(zip(qs, meta)):
+        n = q.num_vertices
+        assert n == [4, 8, 16, 24, 32][i % 5] == m["n"]
+        assert _simple_connected(q)
+        avg = 2.0 * q.num_edges / n
+        if i % 2 == 0:
+            assert m["dense"] and W.is_dense(q)
+            assert (q.num_edges == 6) if n == 4 else avg > 3.0
+        else:
+            assert not m["dense"] and q.num_edges == n - 1 and avg <= 3.0
 
 
 def test_autotune_trial_order():
