@@ -1224,11 +1224,14 @@ __device__ __forceinline__ void add_by_query(unsigned long long* counter, size_t
 
 // One thread per task of the queries being expanded (the caller keeps the
 // tasks of the other queries apart): enumerate the valid extensions at the
-// task's depth. A level with exactly one valid extension is assigned at once
-// and the thread goes on to the next level (a chain of single candidates is
-// crossed in one round), until a level has none or several, or the depth
-// limit min(n - 1, kExpandMax) is reached (the leaf level is never assigned
-// here). Count pass (write = 0): out_count[t] = the tasks that replace task
+// task's depth. For a query with expand_q[q] == 2 (the caller sets it while
+// the query has a single task), a level with exactly one valid extension is
+// assigned at once and the thread goes on to the next level, so a chain of
+// single candidates at the top of a search is crossed in one round; it stops
+// at a level with none or several, or at the depth limit min(n - 1,
+// kExpandMax) (the leaf level is never assigned here). Otherwise every task
+// goes one level deeper, so a query's tasks stay at one depth and of
+// comparable size. Count pass (write = 0): out_count[t] = the tasks that replace task
 // t (its children, or t itself when it is at the limit), qchild[q] += them,
 // qchild[nq + q] += 1 if t was expanded. Write pass: those tasks at
 // out_pos[t]; calls += one per assignment made, once the expansion is
@@ -1245,7 +1248,8 @@ __global__ __launch_bounds__(kBlock) void k_expand(
     // every lane reaches the per-query sums at the end
     uint32_t q = kInvalid, out = 0, rej = 0, calls = 0, expanded = 0;
     unsigned long long bytes = 0, algo = 0;
-    if (t < n_tasks && expand_q[tasks[t].q]) {
+    const uint32_t mode = t < n_tasks ? expand_q[tasks[t].q] : 0u;
+    if (mode) {
         const DTask& task = tasks[t];
         q = task.q;
         const DPlan& P = plans[q];
@@ -1313,7 +1317,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(
                 // call); those below the root level are written survivors
                 calls += c;
                 algo += 4ull * rows + (d >= 1 ? 4ull * c : 0ull);
-                if (c == 1 && d + 1 < lim) {
+                if (c == 1 && d + 1 < lim && mode == 2) {
                     // a single extension: assign it and go on
                     I[d] = first;
                     M[d] = cand_pool[P.cand_base[d] + first];
@@ -1736,6 +1740,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     bool ring_short = true;               // wavefront view: the ring holds fewer than lanes / 8 tasks
     bool few_left = false;                // the lane's query had fewer than kFewTasks tasks left
     bool stop = false;                    // wavefront view: the time limit has passed
+    bool array_left = true;               // wavefront view: the pre-expanded array still has tasks
     bool check = false;                   // this iteration checks the lanes' caps and the limit
     uint32_t witer = 0;
     uint32_t steps = 0;
@@ -1769,12 +1774,13 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     late = true;
                 }
                 h = (len + left0 < lanes_total - min(load_relaxed(&Q->busy), lanes_total) ? 1u : 0u) |
-                    (len < (lanes_total >> 3) ? 2u : 0u) | (late ? 4u : 0u);
+                    (len < (lanes_total >> 3) ? 2u : 0u) | (late ? 4u : 0u) | (left0 ? 8u : 0u);
             }
             h = __shfl(h, 0);
             hungry = (h & 1u) != 0;
             ring_short = (h & 2u) != 0;
             stop = (h & 4u) != 0;
+            array_left = (h & 8u) != 0;
             few_left = false;
             if (!hungry && ring_short && active) few_left = load_relaxed(pending + q) < kFewTasks;
             check = stop || (witer & 255u) == 1u || __ballot(active && near_cap) != 0;
@@ -1793,12 +1799,14 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         const unsigned long long want = __ballot(!active && !waiting && !stop);
         // lanes whose previous task was short (small tasks, many claims)
         const unsigned long long quick = __ballot(!active && !waiting && last_steps < kShortTask);
-        // a wavefront with running lanes claims for its idle ones on every
-        // 16th iteration only: every claim is a round trip to the queue's
-        // words, which the running lanes of the wavefront wait for, and
+        // While the pre-expanded array holds tasks, claims are fetch-adds
+        // that never fail and idle lanes claim at once. Once only split-off
+        // work remains, a wavefront with running lanes claims for its idle
+        // ones on every 16th iteration: each claim is a compare-and-swap
+        // round trip that the wavefront's running lanes wait for, and
         // thousands of wavefronts racing for each donated task slow the
-        // donors down; a wavefront with no running lane claims at once
-        const bool claim_now = (witer & 15) == 1 || __ballot(active) == 0;
+        // donors down; a wavefront with no running lane claims at once.
+        const bool claim_now = array_left || (witer & 15) == 1 || __ballot(active) == 0;
         if (want && backoff == 0 && claim_now) {
             const uint32_t leader = __ffsll((long long)want) - 1;
             uint32_t base = 0, got = 0, per = 1, from_ring = 0, empty = 0;

@@ -1406,7 +1406,9 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         for (uint32_t i = 0; i < nq; ++i) {
             if (!ntask[i]) continue;
             if (round < kExpandMax && ntask[i] < per_q_target && progress[i]) {
-                ex[i] = 1;
+                // a query with one task may cross a chain of single
+                // candidates in this round (k_expand's mode 2)
+                ex[i] = ntask[i] == 1 ? 2 : 1;
                 any = true;
             } else {
                 retire[i] = 1;
