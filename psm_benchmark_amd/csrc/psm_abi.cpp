@@ -1363,6 +1363,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
     uint64_t n_tasks = tasks.size();   // tasks in d_tasks
     uint64_t n_final = 0;              // tasks in d_final
     std::vector<uint64_t> final_cnt(nq, 0);   // tasks per query in d_final
+    std::vector<uint64_t> final_at(nq, 0);    // where query i's segment starts in d_final
     HIP_TRY(ctx, d_final.alloc(std::max<uint64_t>(n_tasks, 1) * sizeof(DTask)));
     // moves the segments of the queries with retire[i] from d_tasks to the
     // end of d_final (segments in query order, as in d_tasks)
@@ -1374,6 +1375,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
             if (retire[i]) {
                 src.push_back(off);
                 dst.push_back(add);
+                final_at[i] = n_final + add;
                 add += ntask[i];
                 final_cnt[i] += ntask[i];
             }
@@ -1465,8 +1467,31 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         }
         n_tasks = total;
     }
-    // the final task array, in the order the queries' expansions ended
-    d_tasks.swap(d_final);
+    // the final task array in query order (segments were appended in the
+    // order the queries' expansions ended): the queue then hands the
+    // queries out in the order they were given
+    {
+        std::vector<uint64_t> src, dst;
+        uint64_t at = 0;
+        for (uint32_t i = 0; i < nq; ++i) {
+            if (!final_cnt[i]) continue;
+            src.push_back(final_at[i]);
+            dst.push_back(at);
+            at += final_cnt[i];
+        }
+        dst.push_back(at);
+        const uint32_t nseg = (uint32_t)src.size();
+        if (nseg > 1) {
+            std::vector<uint64_t> seg(src);
+            seg.insert(seg.end(), dst.begin(), dst.end());
+            HIP_TRY(ctx, upload(d_seg, seg, s()));
+            HIP_TRY(ctx, d_tasks.alloc(std::max<uint64_t>(at, 1) * sizeof(DTask)));
+            HIP_TRY(ctx, launch_gather_tasks(d_final.as<DTask>(), d_seg.as<uint64_t>(),
+                                             d_seg.as<uint64_t>() + nseg, nseg, at, d_tasks.as<DTask>(), s()));
+        } else {
+            d_tasks.swap(d_final);
+        }
+    }
     n_tasks = n_final;
     ntask = final_cnt;
     HIP_TRY(ctx, timers_flush(ctx));
