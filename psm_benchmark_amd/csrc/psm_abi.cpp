@@ -1467,17 +1467,28 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         }
         n_tasks = total;
     }
-    // the final task array in query order (segments were appended in the
-    // order the queries' expansions ended): the queue then hands the
-    // queries out in the order they were given
+    // The final task array interleaves the queries in chunks of
+    // kInterleave tasks, round after round in query order (segments were
+    // appended in the order the queries' expansions ended). Lanes claim the
+    // array from the front, so every query's search starts at once: in a
+    // query-ordered array the last queries of a batch started only after
+    // every earlier query's tasks had been claimed, and a capped query that
+    // needs a few milliseconds alone finished last.
     {
+        constexpr uint64_t kInterleave = 256;
         std::vector<uint64_t> src, dst;
         uint64_t at = 0;
-        for (uint32_t i = 0; i < nq; ++i) {
-            if (!final_cnt[i]) continue;
-            src.push_back(final_at[i]);
-            dst.push_back(at);
-            at += final_cnt[i];
+        for (uint64_t r = 0;; ++r) {
+            bool any = false;
+            for (uint32_t i = 0; i < nq; ++i) {
+                const uint64_t from = r * kInterleave;
+                if (final_cnt[i] <= from) continue;
+                any = true;
+                src.push_back(final_at[i] + from);
+                dst.push_back(at);
+                at += std::min(kInterleave, final_cnt[i] - from);
+            }
+            if (!any) break;
         }
         dst.push_back(at);
         const uint32_t nseg = (uint32_t)src.size();
