@@ -127,6 +127,20 @@ int psm_set_time_limit(psm_ctx* ctx, double seconds);
  *                                   halved while the block's LDS state
  *                                   would not fit. Blocks of one wavefront
  *                                   spread a small grid over every CU
+ *   PSM_TUNE_EXPAND_DEPTH           deepest level the BFS pre-expansion
+ *                                   assigns before the DFS takes over (the
+ *                                   DFS stack's starting depth), 1..31
+ *                                   (default 31; the leaf level is never
+ *                                   assigned there)
+ *   PSM_TUNE_INTERSECT              set intersection of the DFS kernels:
+ *                                   PSM_INTERSECT_ADAPTIVE (default: the
+ *                                   lane kernel advances galloping cursors
+ *                                   over up to four operand rows, the
+ *                                   wavefront kernel tests rows of 1024+
+ *                                   entries cooperatively through an
+ *                                   LDS-staged window) or
+ *                                   PSM_INTERSECT_BSEARCH (every operand row
+ *                                   by binary search, per lane)
  * No counterpart in the reference (its only knob is -threadnums). */
 #define PSM_TUNE_DFS_WAVES_PER_CU 0
 #define PSM_TUNE_EXPAND_TASKS_PER_LANE 1
@@ -134,8 +148,12 @@ int psm_set_time_limit(psm_ctx* ctx, double seconds);
 #define PSM_TUNE_QUEUE_SLOTS 3
 #define PSM_TUNE_CLAIM_DIVISOR 4
 #define PSM_TUNE_DFS_BLOCK_LANES 5
+#define PSM_TUNE_EXPAND_DEPTH 6
+#define PSM_TUNE_INTERSECT 7
 #define PSM_DFS_LANE 0
 #define PSM_DFS_WAVE 1
+#define PSM_INTERSECT_ADAPTIVE 0
+#define PSM_INTERSECT_BSEARCH 1
 int psm_set_tuning(psm_ctx* ctx, int key, double value);
 int psm_get_tuning(psm_ctx* ctx, int key, double* value);
 

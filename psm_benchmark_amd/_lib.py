@@ -100,6 +100,10 @@ PSM_TUNE_DFS_STRATEGY = 2
 PSM_TUNE_QUEUE_SLOTS = 3
 PSM_TUNE_CLAIM_DIVISOR = 4
 PSM_TUNE_DFS_BLOCK_LANES = 5
+PSM_TUNE_EXPAND_DEPTH = 6
+PSM_TUNE_INTERSECT = 7
+PSM_INTERSECT_ADAPTIVE = 0
+PSM_INTERSECT_BSEARCH = 1
 PSM_TREE_FILTER = 0
 PSM_TREE_PLAN = 1
 PSM_DFS_LANE = 0

@@ -1243,7 +1243,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(
     const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ tasks, uint32_t n_tasks,
     const uint8_t* __restrict__ expand_q, uint32_t* __restrict__ out_count,
     const uint64_t* __restrict__ out_pos, DTask* __restrict__ out_tasks, QCounters* qc,
-    unsigned long long* __restrict__ qchild, uint32_t nq, int write) {
+    unsigned long long* __restrict__ qchild, uint32_t nq, uint32_t max_depth, int write) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     // every lane reaches the per-query sums at the end
     uint32_t q = kInvalid, out = 0, rej = 0, calls = 0, expanded = 0;
@@ -1253,7 +1253,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         const DTask& task = tasks[t];
         q = task.q;
         const DPlan& P = plans[q];
-        const uint32_t lim = min(P.n - 1, kExpandMax);
+        const uint32_t lim = min(P.n - 1, min(max_depth, kExpandMax));
         uint32_t I[kExpandMax + 1], M[kExpandMax + 1];
         uint32_t d = task.depth;
         for (uint32_t e = 0; e < d; ++e) {
@@ -1410,7 +1410,8 @@ constexpr uint32_t kAutoSel = 0xFFFFFFFEu;
 __device__ __forceinline__ void enter_level(const DPlan& P, const DOp* __restrict__ ops,
                                             const uint64_t* __restrict__ row_off, uint32_t d,
                                             const LaneState& st, uint32_t forced_sel, Level& L,
-                                            unsigned long long& bytes, unsigned long long& algo) {
+                                            unsigned long long& bytes, unsigned long long& algo,
+                                            uint32_t max_cursors) {
     const uint32_t nops = P.op_count[d];
     L.nb = L.nx = 0;
     if (nops == 0) {
@@ -1442,7 +1443,7 @@ __device__ __forceinline__ void enter_level(const DPlan& P, const DOp* __restric
     }
     for (uint32_t j = 0; j < nops; ++j) {
         if (j == sel) continue;
-        if (L.nb < kCursors) {
+        if (L.nb < max_cursors) {
             const DOp op = ops[P.op_begin[d] + j];
             const uint64_t r = op.row_base + st.I(op.src_depth);
             push_cursor(L, row_off[r], row_off[r + 1]);
@@ -1507,7 +1508,7 @@ __device__ __forceinline__ int test_merge(const DPlan& P, const DOp* __restrict_
         uint32_t pos = 0;
         for (uint32_t j = 0; j < nops; ++j) {
             if (j == L.sel) continue;
-            if (pos++ < kCursors) continue;
+            if (pos++ < L.nb) continue;
             const DOp op = ops[P.op_begin[d] + j];
             const uint64_t rr = op.row_base + st.I(op.src_depth);
             const uint64_t hi0 = row_off[rr + 1];
@@ -1723,6 +1724,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     const uint32_t qmask = Q->mask;
     const uint32_t n0 = Q->n0;
     const uint32_t claim_div = max(1u, Q->claim_div);
+    // intersection strategy: merge cursors over up to kCursors operand rows
+    // (galloping), or binary search in every row
+    const uint32_t max_cursors = Q->intersect == PSM_INTERSECT_BSEARCH ? 0u : kCursors;
     unsigned long long* const notify = Q->notify;
 
     bool active = false, waiting = false, wring = false;   // wring: the claimed positions are ring positions
@@ -1912,9 +1916,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 if (wring) seq_release(seq, qmask, wslot);
                 for (uint32_t e = 0; e < tdepth; ++e) st.M_[st.at(e)] = cand_pool[P->cand_base[e] + st.I_[st.at(e)]];
                 if (thi == kAutoHi) {
-                    enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo);
+                    enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo, max_cursors);
                 } else {
-                    enter_level(*P, ops, row_off, d, st, tsel, L, lbytes, lalgo);
+                    enter_level(*P, ops, row_off, d, st, tsel, L, lbytes, lalgo, max_cursors);
                     L.cur = tlo;
                     L.end = thi;
                 }
@@ -2001,7 +2005,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 if (d >= 1) lalgo += 4;   // a survivor written below the root level
                 ++d;
                 if (d + 1 == n) Lp = L;   // restored on the way back up
-                enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo);
+                enter_level(*P, ops, row_off, d, st, kAutoSel, L, lbytes, lalgo, max_cursors);
             }
         } else if (d == tdepth) {
             finished = true;
@@ -2016,7 +2020,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 L.end = L.base0 + st.end_[a];
             } else {
                 const uint32_t sel = st.sel_[a] == kSelIdentity ? kIdentity : st.sel_[a];
-                enter_level(*P, ops, row_off, d, st, sel, L, lbytes, lalgo);
+                enter_level(*P, ops, row_off, d, st, sel, L, lbytes, lalgo, max_cursors);
                 L.cur = L.base0 + st.cur_[a];
                 L.end = L.base0 + st.end_[a];
             }
@@ -2354,6 +2358,9 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
     const uint32_t n0 = Q->n0;
     unsigned long long* const notify = Q->notify;
     const uint32_t hungry_below = gridDim.x / 2;   // queue shorter than half the wavefronts
+    // intersection strategy: rows of kCoopRow+ entries tested cooperatively
+    // through an LDS-staged window, or every row by per-lane binary search
+    const uint64_t coop_row = Q->intersect == PSM_INTERSECT_BSEARCH ? ~0ull : (uint64_t)kCoopRow;
     unsigned long long bytes = 0, algo = 0, busy_pos = 0, all_pos = 0;
 
     while (true) {
@@ -2532,7 +2539,7 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                     if (__ballot(ok) == 0) break;
                     const uint64_t rlo = S.m_olo[j][k0];
                     const uint64_t rlen = S.m_olen[j][k0];
-                    if (rlen < kCoopRow) {
+                    if (rlen < coop_row) {
                         // a short row: each lane's binary search is a few
                         // cached probes, cheaper than locating a window
                         if (ok) ok = row_has(idx, rlo, rlen, xx, bytes);
@@ -3247,11 +3254,12 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
                          const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
                          uint32_t n_tasks, const uint8_t* expand_q, uint32_t* out_count,
                          const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
-                         unsigned long long* qchild, uint32_t nq, int write, hipStream_t s) {
+                         unsigned long long* qchild, uint32_t nq, uint32_t max_depth, int write,
+                         hipStream_t s) {
     if (n_tasks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_expand, dim3((n_tasks + kBlock - 1) / kBlock), dim3(kBlock), 0, s, plans,
                        ops, row_off, idx, cand_pool, tasks, n_tasks, expand_q, out_count, out_pos,
-                       out_tasks, qc, qchild, nq, write);
+                       out_tasks, qc, qchild, nq, max_depth, write);
     return hipGetLastError();
 }
 

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "psm_abi.h"
+
 namespace psm {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;   // pruned candidate
@@ -149,6 +151,7 @@ struct DQueue {
     unsigned int head0;        // next task of the pre-expanded array
     unsigned int n0;           // pre-expanded tasks
     unsigned int claim_div;    // array tasks per lane above which a lane takes several per claim
+    unsigned int intersect;    // PSM_INTERSECT_ADAPTIVE or PSM_INTERSECT_BSEARCH (psm_abi.h)
     // Host-visible (coherent pinned) words, 4 per query, or null: once a
     // query's count is final the kernel writes its count, calls and visited
     // rejects to words 1..3 and then 1 to word 0, so a caller can return it
@@ -295,7 +298,8 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
                          const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
                          uint32_t n_tasks, const uint8_t* expand_q, uint32_t* out_count,
                          const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
-                         unsigned long long* qchild, uint32_t nq, int write, hipStream_t s);
+                         unsigned long long* qchild, uint32_t nq, uint32_t max_depth, int write,
+                         hipStream_t s);
 // Copies segments of task records (dst_off: exclusive prefix of the segment
 // lengths, n_seg + 1 entries; total = dst_off[n_seg]).
 hipError_t launch_gather_tasks(const DTask* src, const uint64_t* src_off, const uint64_t* dst_off,

@@ -64,6 +64,8 @@ struct psm_ctx {
     uint32_t queue_slots = 0;                 // ring slots for split-off work (0: automatic)
     uint32_t claim_div = kDefaultClaimDiv;    // array tasks per lane above which claims take several
     uint32_t block_lanes = kDefaultBlockLanes;  // lanes per workgroup of the lane DFS (0: 64)
+    uint32_t expand_depth = kExpandMax;         // deepest level pre-expansion assigns
+    uint32_t intersect = PSM_INTERSECT_ADAPTIVE;  // set intersection of the DFS kernels
     double wall_khz = 100000.0;               // device wall clock rate
     bool verbose = false;
 };
@@ -1402,12 +1404,13 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
     // of its tasks are above the depth limit (the kernel leaves the others
     // as they are); each round takes every task at least one level deeper
     std::vector<uint8_t> progress(nq, 1);
-    for (uint32_t round = 0; round <= kExpandMax && n_tasks; ++round) {
+    const uint32_t max_rounds = ctx->expand_depth;   // each round assigns at least one level
+    for (uint32_t round = 0; round <= max_rounds && n_tasks; ++round) {
         std::vector<uint8_t> ex(nq, 0), retire(nq, 0);
         bool any = false;
         for (uint32_t i = 0; i < nq; ++i) {
             if (!ntask[i]) continue;
-            if (round < kExpandMax && ntask[i] < per_q_target && progress[i]) {
+            if (round < max_rounds && ntask[i] < per_q_target && progress[i]) {
                 // a query with one task may cross a chain of single
                 // candidates in this round (k_expand's mode 2)
                 ex[i] = ntask[i] == 1 ? 2 : 1;
@@ -1431,7 +1434,8 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
               launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
                             cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
                             d_expand.as<uint8_t>(), d_cnt.as<uint32_t>(), nullptr, nullptr,
-                            qcnt.as<QCounters>(), d_qchild.as<unsigned long long>(), nq, 0, s()));
+                            qcnt.as<QCounters>(), d_qchild.as<unsigned long long>(), nq, ctx->expand_depth, 0,
+                            s()));
         HIP_TRY(ctx, launch_scan_u32(d_cnt.as<uint32_t>(), n_tasks, d_pos.as<uint64_t>(),
                                      d_tmp.as<unsigned long long>(), s()));
         // the tasks per query that replace this round's (one read-back per
@@ -1453,7 +1457,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
               launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
                             cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
                             d_expand.as<uint8_t>(), nullptr, d_pos.as<uint64_t>(), d_tasks2.as<DTask>(),
-                            qcnt.as<QCounters>(), nullptr, nq, 1, s()));
+                            qcnt.as<QCounters>(), nullptr, nq, ctx->expand_depth, 1, s()));
         d_tasks.swap(d_tasks2);
         verbose("expand round %u: %llu tasks -> %llu children, %.2f ms so far", round,
                 (unsigned long long)n_tasks, (unsigned long long)total, now_ms() - te0);
@@ -1530,6 +1534,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         hq.n0 = (unsigned int)n_tasks;
         hq.mask = (unsigned int)(ring - 1);
         hq.claim_div = ctx->claim_div;
+        hq.intersect = ctx->intersect;
         hq.notify = sink.rows ? nullptr : notify;
         if (enum_limit_s > 0) {
             const double left_s = enum_limit_s - (now_ms() - t_eval) / 1000.0;
@@ -1870,6 +1875,15 @@ int psm_set_tuning(psm_ctx* ctx, int key, double value) {
                 return set_err(ctx, PSM_EINVAL, "dfs block lanes must be 0, 64, 128 or 256");
             ctx->block_lanes = (uint32_t)value;
             return PSM_OK;
+        case PSM_TUNE_EXPAND_DEPTH:
+            if (!(value >= 1 && value <= kExpandMax)) return set_err(ctx, PSM_EINVAL, "expand depth outside [1, 31]");
+            ctx->expand_depth = (uint32_t)value;
+            return PSM_OK;
+        case PSM_TUNE_INTERSECT:
+            if (value != PSM_INTERSECT_ADAPTIVE && value != PSM_INTERSECT_BSEARCH)
+                return set_err(ctx, PSM_EINVAL, "intersect must be PSM_INTERSECT_ADAPTIVE or PSM_INTERSECT_BSEARCH");
+            ctx->intersect = (uint32_t)value;
+            return PSM_OK;
         case PSM_TUNE_QUEUE_SLOTS:
             if (!(value >= 0 && value <= (double)(1u << 28)))
                 return set_err(ctx, PSM_EINVAL, "queue slots outside [0, 2^28]");
@@ -1889,6 +1903,8 @@ int psm_get_tuning(psm_ctx* ctx, int key, double* value) {
         case PSM_TUNE_QUEUE_SLOTS: *value = ctx->queue_slots; return PSM_OK;
         case PSM_TUNE_CLAIM_DIVISOR: *value = ctx->claim_div; return PSM_OK;
         case PSM_TUNE_DFS_BLOCK_LANES: *value = ctx->block_lanes; return PSM_OK;
+        case PSM_TUNE_EXPAND_DEPTH: *value = ctx->expand_depth; return PSM_OK;
+        case PSM_TUNE_INTERSECT: *value = ctx->intersect; return PSM_OK;
         default: return set_err(ctx, PSM_EINVAL, "unknown tuning key");
     }
 }
@@ -2402,6 +2418,7 @@ int psm_evaluate_noalloc(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     hq.n0 = n_tasks;
     hq.mask = kScratchSlots - 1;
     hq.claim_div = ctx->claim_div;
+    hq.intersect = ctx->intersect;
     if (ctx->time_limit_s > 0) {
         double left_s = ctx->time_limit_s;
         if (b.t_begin >= 0) left_s -= (now_ms() - b.t_begin) / 1000.0;
@@ -2813,6 +2830,8 @@ int psm_server_create(psm_ctx* ctx, const psm_graph* g, uint32_t workers, uint32
         c->dfs_waves_per_cu = std::max<uint32_t>(1, ctx->dfs_waves_per_cu / workers);
         c->expand_tasks_per_lane = ctx->expand_tasks_per_lane;
         c->dfs_strategy = ctx->dfs_strategy;
+        c->expand_depth = ctx->expand_depth;
+        c->intersect = ctx->intersect;
         S->ctxs.push_back(c);
     }
     for (psm_ctx* c : S->ctxs) S->workers.emplace_back(serve_worker, S.get(), c);

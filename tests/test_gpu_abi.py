@@ -35,10 +35,15 @@ def test_tuning_round_trip_and_rejection(dg):
     autotune.set_config(dg.dev, {"claim_divisor": 3, "dfs_block_lanes": 256})
     got = autotune.get_config(dg.dev)
     assert got["claim_divisor"] == 3.0 and got["dfs_block_lanes"] == 256.0
+    autotune.set_config(dg.dev, {"intersect": "bsearch", "expand_depth": 5})
+    got = autotune.get_config(dg.dev)
+    assert got["intersect"] == "bsearch" and got["expand_depth"] == 5.0
     for key, bad in ((L.PSM_TUNE_DFS_WAVES_PER_CU, 0), (L.PSM_TUNE_DFS_WAVES_PER_CU, 33),
                      (L.PSM_TUNE_EXPAND_TASKS_PER_LANE, 0.0), (L.PSM_TUNE_DFS_STRATEGY, 7),
                      (L.PSM_TUNE_CLAIM_DIVISOR, 0), (L.PSM_TUNE_QUEUE_SLOTS, -1),
-                     (L.PSM_TUNE_DFS_BLOCK_LANES, 32), (L.PSM_TUNE_DFS_BLOCK_LANES, 512), (99, 1)):
+                     (L.PSM_TUNE_DFS_BLOCK_LANES, 32), (L.PSM_TUNE_DFS_BLOCK_LANES, 512),
+                     (L.PSM_TUNE_EXPAND_DEPTH, 0), (L.PSM_TUNE_EXPAND_DEPTH, 32),
+                     (L.PSM_TUNE_INTERSECT, 2), (99, 1)):
         assert L.lib().psm_set_tuning(h, key, float(bad)) == L.PSM_EINVAL
     v = ctypes.c_double()
     assert L.lib().psm_get_tuning(h, 99, ctypes.byref(v)) == L.PSM_EINVAL

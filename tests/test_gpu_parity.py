@@ -220,6 +220,26 @@ def test_block_widths_give_golden_counts(dev, hprd):
         autotune.set_config(dev, before)
 
 
+def test_intersection_and_depth_knobs_give_golden_counts(dev, hprd):
+    """Binary search in every operand row (no galloping cursors, no
+    LDS-staged windows) and shallow pre-expansion depths: batched counts and
+    call counts still equal the golden records, on both DFS kernels."""
+    from psm_benchmark_amd import autotune
+    from psm_benchmark_amd.matcher import batch_match
+    _, dg, _ = hprd
+    before = autotune.get_config(dev)
+    recs = golden(("GQL_GQL_GQL",))
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
+    try:
+        for inter, depth in (("bsearch", 31), ("bsearch", 1), ("adaptive", 2)):
+            autotune.set_config(dev, {"intersect": inter, "expand_depth": depth})
+            counts, calls, status, _ = batch_match(dg, qs, "GQL", "GQL", 0)
+            assert counts.tolist() == [r["embeddings"] for r in recs], (inter, depth)
+            assert calls.tolist() == [r["calls"] for r in recs], (inter, depth)
+    finally:
+        autotune.set_config(dev, before)
+
+
 def test_pipeline_batches_equal_golden(dev, hprd):
     """Batches through a two-context pipeline (two HIP streams, their
     launches overlapping) each give the golden counts and call counts."""
