@@ -1238,9 +1238,7 @@ __device__ __forceinline__ void add_by_query(unsigned long long* counter, size_t
 // committed. A task of a query that is not expanded (expand_q[q] == 0) is
 // dropped.
 __global__ __launch_bounds__(kBlock) void k_expand(
-    const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
-    const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
-    const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ tasks, uint32_t n_tasks,
+    const DPlan* __restrict__ plans, const DOp* __restrict__ ops, const DTask* __restrict__ tasks, uint32_t n_tasks,
     const uint8_t* __restrict__ expand_q, uint32_t* __restrict__ out_count,
     const uint64_t* __restrict__ out_pos, DTask* __restrict__ out_tasks, QCounters* qc,
     unsigned long long* __restrict__ qchild, uint32_t nq, uint32_t max_depth, int write) {
@@ -1253,6 +1251,9 @@ __global__ __launch_bounds__(kBlock) void k_expand(
         const DTask& task = tasks[t];
         q = task.q;
         const DPlan& P = plans[q];
+        const uint64_t* __restrict__ row_off = P.row_off;
+        const uint32_t* __restrict__ idx = P.idx;
+        const uint32_t* __restrict__ cand_pool = P.cand_pool;
         const uint32_t lim = min(P.n - 1, min(max_depth, kExpandMax));
         uint32_t I[kExpandMax + 1], M[kExpandMax + 1];
         uint32_t d = task.depth;
@@ -1696,10 +1697,8 @@ __device__ __forceinline__ void lane_flush(QCounters* qc, uint32_t q, uint64_t c
 // reaches its cap is marked done and its tasks are dropped (count =
 // min(cap, total)).
 __global__ __launch_bounds__(kBlock) void k_dfs(
-    const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
-    const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
-    const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ init, DTask* queue, uint32_t* seq,
-    DQueue* Q, QCounters* qc, uint32_t* pending, uint32_t max_n, EmitSink sink) {
+    const DPlan* __restrict__ plans, const DOp* __restrict__ ops, const DTask* __restrict__ init, DTask* queue,
+    uint32_t* seq, DQueue* Q, QCounters* qc, uint32_t* pending, uint32_t max_n, EmitSink sink) {
     extern __shared__ __align__(16) unsigned char lane_smem[];
     const uint32_t lane = lane_id();
     const uint32_t per_field = blockDim.x * max_n;
@@ -1750,6 +1749,10 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     uint32_t steps = 0;
     uint32_t last_steps = kShortTask;   // steps of the lane's previous task (none yet: counts as long)
     const DPlan* P = plans;
+    // the held query's candidate pool and tables
+    const uint64_t* __restrict__ row_off = P->row_off;
+    const uint32_t* __restrict__ idx = P->idx;
+    const uint32_t* __restrict__ cand_pool = P->cand_pool;
     const uint32_t lanes_total = gridDim.x * blockDim.x;
 
     while (true) {
@@ -1893,6 +1896,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             }
             q = tq;
             P = plans + q;
+            row_off = P->row_off;
+            idx = P->idx;
+            cand_pool = P->cand_pool;
             if (stop || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
                 if (wring) seq_release(seq, qmask, wslot);
                 ++busy_done;
@@ -2342,10 +2348,8 @@ __device__ __forceinline__ uint32_t run_length(const F& S, uint32_t depth, uint3
 
 template <uint32_t N>
 __global__ __launch_bounds__(kWave) void k_dfs_wave(
-    const DPlan* __restrict__ plans, const DOp* __restrict__ ops,
-    const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ idx,
-    const uint32_t* __restrict__ cand_pool, const DTask* __restrict__ init, DTask* queue, uint32_t* seq,
-    DQueue* Q, QCounters* qc, uint32_t* pending) {
+    const DPlan* __restrict__ plans, const DOp* __restrict__ ops, const DTask* __restrict__ init, DTask* queue,
+    uint32_t* seq, DQueue* Q, QCounters* qc, uint32_t* pending) {
     const uint32_t lane = threadIdx.x;
     __shared__ WaveFrame<N> S;
     if (lane == 0) {
@@ -2410,8 +2414,13 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
         }
         // read with coherent loads (see coherent_u32)
         const DTask* task = from_ring ? queue + (slot & qmask) : init + slot;
-        const uint32_t q = coherent_u32(&task->q);
+        // every lane reads the same record: q (and the plan's pointers) are
+        // wavefront-uniform
+        const uint32_t q = __builtin_amdgcn_readfirstlane(coherent_u32(&task->q));
         const DPlan& P = plans[q];
+        const uint64_t* __restrict__ row_off = P.row_off;
+        const uint32_t* __restrict__ idx = P.idx;
+        const uint32_t* __restrict__ cand_pool = P.cand_pool;
         uint32_t drop = 0;
         if (lane == 0) {
             drop = load_relaxed(&Q->timeout) || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED);
@@ -3250,15 +3259,13 @@ hipError_t launch_cfl_prune(const uint32_t* cnt, uint32_t* cand_pool, uint64_t c
     return hipGetLastError();
 }
 
-hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                         const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
-                         uint32_t n_tasks, const uint8_t* expand_q, uint32_t* out_count,
+hipError_t launch_expand(const DPlan* plans, const DOp* ops, const DTask* tasks, uint32_t n_tasks, const uint8_t* expand_q, uint32_t* out_count,
                          const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
                          unsigned long long* qchild, uint32_t nq, uint32_t max_depth, int write,
                          hipStream_t s) {
     if (n_tasks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_expand, dim3((n_tasks + kBlock - 1) / kBlock), dim3(kBlock), 0, s, plans,
-                       ops, row_off, idx, cand_pool, tasks, n_tasks, expand_q, out_count, out_pos,
+                       ops, tasks, n_tasks, expand_q, out_count, out_pos,
                        out_tasks, qc, qchild, nq, max_depth, write);
     return hipGetLastError();
 }
@@ -3293,8 +3300,7 @@ uint32_t dfs_lanes_per_block(uint32_t max_n, uint32_t want) {
 
 size_t dfs_lane_lds_bytes(uint32_t max_n) { return lane_lds_bytes(dfs_lanes_per_block(max_n, 0), max_n); }
 
-hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                      const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue, uint32_t* seq,
+hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const DTask* init, DTask* queue, uint32_t* seq,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                       uint32_t block_lanes, EmitSink sink, hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
@@ -3314,8 +3320,8 @@ hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_of
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 &&
         cus > 0)
         blocks = std::min<uint32_t>(blocks, (uint32_t)(per_cu * cus));
-    hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(lanes), lds, s, plans, ops, row_off, idx, cand_pool, init,
-                       queue, seq, dq, qc, pending, max_n, sink);
+    hipLaunchKernelGGL(k_dfs, dim3(blocks), dim3(lanes), lds, s, plans, ops, init, queue, seq, dq, qc, pending,
+                       max_n, sink);
     return hipGetLastError();
 }
 
@@ -3323,22 +3329,20 @@ size_t dfs_wave_lds_bytes(uint32_t max_n) {
     return max_n <= 16 ? sizeof(WaveFrame<16>) : max_n <= 32 ? sizeof(WaveFrame<32>) : sizeof(WaveFrame<64>);
 }
 
-hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                           const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue,
-                           uint32_t* seq,
+hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const DTask* init, DTask* queue, uint32_t* seq,
                            DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                            hipStream_t s) {
     if (grid_waves == 0) return hipSuccess;
     if (max_n == 0 || max_n > kMaxQ) return hipErrorInvalidValue;
     if (max_n <= 16)
-        hipLaunchKernelGGL(k_dfs_wave<16>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, init, queue, seq, dq, qc, pending);
+        hipLaunchKernelGGL(k_dfs_wave<16>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, init, queue, seq,
+                           dq, qc, pending);
     else if (max_n <= 32)
-        hipLaunchKernelGGL(k_dfs_wave<32>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, init, queue, seq, dq, qc, pending);
+        hipLaunchKernelGGL(k_dfs_wave<32>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, init, queue, seq,
+                           dq, qc, pending);
     else
-        hipLaunchKernelGGL(k_dfs_wave<64>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, row_off, idx,
-                           cand_pool, init, queue, seq, dq, qc, pending);
+        hipLaunchKernelGGL(k_dfs_wave<64>, dim3(grid_waves), dim3(kWave), 0, s, plans, ops, init, queue, seq,
+                           dq, qc, pending);
     return hipGetLastError();
 }
 

@@ -93,6 +93,13 @@ struct DPlan {
     uint32_t pad;
     uint64_t cap;
     uint64_t m_needed;            // depths whose data vertex a later depth compares
+    // the query's candidate pool and candidate-edge tables (device memory of
+    // the batch that prepared it): the kernels read every query's data
+    // through its own plan, so one launch can enumerate queries of several
+    // batches
+    const uint64_t* row_off;
+    const uint32_t* idx;
+    const uint32_t* cand_pool;
     uint64_t cand_base[kMaxQ];
     uint32_t cand_len[kMaxQ];
     uint32_t op_begin[kMaxQ];
@@ -294,8 +301,7 @@ hipError_t launch_cfl_support(const DevGraph& g, const uint32_t* cand_pool, uint
                               hipStream_t s);
 hipError_t launch_cfl_prune(const uint32_t* cnt, uint32_t* cand_pool, uint64_t cand_begin,
                             uint32_t len, uint32_t target, hipStream_t s);
-hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                         const uint32_t* idx, const uint32_t* cand_pool, const DTask* tasks,
+hipError_t launch_expand(const DPlan* plans, const DOp* ops, const DTask* tasks,
                          uint32_t n_tasks, const uint8_t* expand_q, uint32_t* out_count,
                          const uint64_t* out_pos, DTask* out_tasks, QCounters* qc,
                          unsigned long long* qchild, uint32_t nq, uint32_t max_depth, int write,
@@ -304,8 +310,7 @@ hipError_t launch_expand(const DPlan* plans, const DOp* ops, const uint64_t* row
 // lengths, n_seg + 1 entries; total = dst_off[n_seg]).
 hipError_t launch_gather_tasks(const DTask* src, const uint64_t* src_off, const uint64_t* dst_off,
                                uint32_t n_seg, uint64_t total, DTask* dst, hipStream_t s);
-hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                      const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue, uint32_t* seq,
+hipError_t launch_dfs(const DPlan* plans, const DOp* ops, const DTask* init, DTask* queue, uint32_t* seq,
                       DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                       uint32_t block_lanes, EmitSink sink, hipStream_t s);
 hipError_t launch_join_build(const uint32_t* rows, uint64_t n_rows, uint32_t width, JoinKeys keys,
@@ -330,9 +335,7 @@ hipError_t launch_root_tasks(DTask* tasks, uint32_t n, uint32_t q, uint64_t len,
 hipError_t launch_ring_init(uint32_t* seq, uint32_t slots, uint32_t n_tasks, hipStream_t s);
 size_t dfs_lane_lds_bytes(uint32_t max_n);
 size_t dfs_wave_lds_bytes(uint32_t max_n);
-hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const uint64_t* row_off,
-                           const uint32_t* idx, const uint32_t* cand_pool, const DTask* init, DTask* queue,
-                           uint32_t* seq,
+hipError_t launch_dfs_wave(const DPlan* plans, const DOp* ops, const DTask* init, DTask* queue, uint32_t* seq,
                            DQueue* dq, QCounters* qc, uint32_t* pending, uint32_t max_n, uint32_t grid_waves,
                            hipStream_t s);
 uint64_t scan_tmp_entries(uint64_t n);

@@ -451,8 +451,14 @@ struct Batch {
                          std::vector<std::vector<uint32_t>>& sets);
     int build_tables();
     int make_plans(int order_method);
-    int run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSink sink,
+    int run_plans(std::vector<DPlan> hp, std::vector<DOp> hops, EmitSink sink,
                   std::vector<QCounters>& qc, bool& timed_out);
+    // points a plan at this batch's candidate pool and tables
+    void bind_tables(DPlan& P) const {
+        P.row_off = row_off.as<uint64_t>();
+        P.idx = idx.as<uint32_t>();
+        P.cand_pool = cand_pool.as<uint32_t>();
+    }
     int evaluate(uint64_t cap, double enum_limit = -1);
     int join_count(QueryHost& q, uint64_t max_rows, JoinResult& jr);
     int split_join_count(QueryHost& q, int pattern, uint64_t max_rows, SplitJoinResult& jr);
@@ -1324,10 +1330,11 @@ void make_plan(const QueryHost& q, const std::vector<uint32_t>& order,
 // has enough tasks to spread over the chip, then one persistent DFS kernel
 // drains a shared work queue that running lanes refill by splitting their
 // ranges. `sink` optionally materialises the embeddings (one plan only).
-int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSink sink,
+int Batch::run_plans(std::vector<DPlan> hp, std::vector<DOp> hops, EmitSink sink,
                      std::vector<QCounters>& qc, bool& timed_out) {
     const uint32_t nq = (uint32_t)hp.size();
     if (hops.empty()) hops.push_back(DOp{});
+    for (DPlan& P : hp) bind_tables(P);
     HIP_TRY(ctx, upload(plans, hp, s()));
     HIP_TRY(ctx, upload(ops, hops, s()));
     HIP_TRY(ctx, qcnt.alloc(sizeof(QCounters) * nq));
@@ -1431,8 +1438,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         HIP_TRY(ctx, d_pos.alloc((n_tasks + 1) * 8));
         HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(n_tasks) * 8));
         TIMED(ctx, kKExpand,
-              launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
-                            cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
+              launch_expand(plans.as<DPlan>(), ops.as<DOp>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
                             d_expand.as<uint8_t>(), d_cnt.as<uint32_t>(), nullptr, nullptr,
                             qcnt.as<QCounters>(), d_qchild.as<unsigned long long>(), nq, ctx->expand_depth, 0,
                             s()));
@@ -1454,8 +1460,7 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         }
         HIP_TRY(ctx, d_tasks2.alloc(std::max<uint64_t>(total, 1) * sizeof(DTask)));
         TIMED(ctx, kKExpand,
-              launch_expand(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
-                            cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
+              launch_expand(plans.as<DPlan>(), ops.as<DOp>(), d_tasks.as<DTask>(), (uint32_t)n_tasks,
                             d_expand.as<uint8_t>(), nullptr, d_pos.as<uint64_t>(), d_tasks2.as<DTask>(),
                             qcnt.as<QCounters>(), nullptr, nq, ctx->expand_depth, 1, s()));
         d_tasks.swap(d_tasks2);
@@ -1555,15 +1560,13 @@ int Batch::run_plans(const std::vector<DPlan>& hp, std::vector<DOp> hops, EmitSi
         dfs_launch_ms = td0;
         if (wave_dfs) {
             TIMED(ctx, kKDfs,
-                  launch_dfs_wave(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
-                                  cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), d_queue.as<DTask>(),
+                  launch_dfs_wave(plans.as<DPlan>(), ops.as<DOp>(), d_tasks.as<DTask>(), d_queue.as<DTask>(),
                                   d_seq.as<uint32_t>(),
                                   d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
                                   s()));
         } else {
             TIMED(ctx, kKDfs,
-                  launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), row_off.as<uint64_t>(), idx.as<uint32_t>(),
-                             cand_pool.as<uint32_t>(), d_tasks.as<DTask>(), d_queue.as<DTask>(),
+                  launch_dfs(plans.as<DPlan>(), ops.as<DOp>(), d_tasks.as<DTask>(), d_queue.as<DTask>(),
                              d_seq.as<uint32_t>(),
                              d_q.as<DQueue>(), qcnt.as<QCounters>(), d_pending.as<uint32_t>(), max_n, gw,
                              ctx->block_lanes, sink, s()));
@@ -2389,6 +2392,7 @@ int psm_evaluate_noalloc(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     std::vector<DOp> hops;
     hops.reserve(kScratchOps);
     make_plan(h, h.plan.order, nullptr, cap, P, hops);
+    b.bind_tables(P);
     if (hops.size() > kScratchOps)
         return set_err(ctx, PSM_EINVAL, "query needs " + std::to_string(hops.size()) +
                                             " intersection operands; the device scratch holds " +
@@ -2434,13 +2438,10 @@ int psm_evaluate_noalloc(psm_query* q, uint64_t cap, psm_enum_stats* st) {
     const uint32_t gw = (uint32_t)ctx->num_cu * ctx->dfs_waves_per_cu / 4 * 4;
     const double tk = now_ms();
     if (ctx->dfs_strategy == PSM_DFS_WAVE) {
-        HIP_TRY(ctx, launch_dfs_wave(dplan, dops, b.row_off.as<uint64_t>(), b.idx.as<uint32_t>(),
-                                     b.cand_pool.as<uint32_t>(), dinit, dring, dseq, dq, dqc, dpend, P.n, gw,
-                                     ctx->stream));
+        HIP_TRY(ctx, launch_dfs_wave(dplan, dops, dinit, dring, dseq, dq, dqc, dpend, P.n, gw, ctx->stream));
     } else {
-        HIP_TRY(ctx, launch_dfs(dplan, dops, b.row_off.as<uint64_t>(), b.idx.as<uint32_t>(),
-                                b.cand_pool.as<uint32_t>(), dinit, dring, dseq, dq, dqc, dpend, P.n, gw,
-                                ctx->block_lanes, EmitSink{nullptr, nullptr, 0, nullptr}, ctx->stream));
+        HIP_TRY(ctx, launch_dfs(dplan, dops, dinit, dring, dseq, dq, dqc, dpend, P.n, gw, ctx->block_lanes,
+                                EmitSink{nullptr, nullptr, 0, nullptr}, ctx->stream));
     }
     QCounters qc{};
     uint32_t left = 0;
