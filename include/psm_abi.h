@@ -141,6 +141,10 @@ int psm_set_time_limit(psm_ctx* ctx, double seconds);
  *                                   LDS-staged window) or
  *                                   PSM_INTERSECT_BSEARCH (every operand row
  *                                   by binary search, per lane)
+ *   PSM_TUNE_SERVE_RESIDENT         servers created on this context admit
+ *                                   prepared batches into one resident
+ *                                   enumeration launch (1, default) or run
+ *                                   one launch per batch (0)
  * No counterpart in the reference (its only knob is -threadnums). */
 #define PSM_TUNE_DFS_WAVES_PER_CU 0
 #define PSM_TUNE_EXPAND_TASKS_PER_LANE 1
@@ -150,6 +154,7 @@ int psm_set_time_limit(psm_ctx* ctx, double seconds);
 #define PSM_TUNE_DFS_BLOCK_LANES 5
 #define PSM_TUNE_EXPAND_DEPTH 6
 #define PSM_TUNE_INTERSECT 7
+#define PSM_TUNE_SERVE_RESIDENT 8
 #define PSM_DFS_LANE 0
 #define PSM_DFS_WAVE 1
 #define PSM_INTERSECT_ADAPTIVE 0

@@ -102,6 +102,7 @@ PSM_TUNE_CLAIM_DIVISOR = 4
 PSM_TUNE_DFS_BLOCK_LANES = 5
 PSM_TUNE_EXPAND_DEPTH = 6
 PSM_TUNE_INTERSECT = 7
+PSM_TUNE_SERVE_RESIDENT = 8
 PSM_INTERSECT_ADAPTIVE = 0
 PSM_INTERSECT_BSEARCH = 1
 PSM_TREE_FILTER = 0

@@ -1584,13 +1584,54 @@ __device__ __forceinline__ void seq_release(uint32_t* seq, uint32_t mask, uint32
 // per query): the values first, then the flag, with system-scope stores.
 __device__ __forceinline__ void post_final(unsigned long long* notify, QCounters* qc, uint32_t q,
                                            unsigned long long count, unsigned long long calls,
-                                           unsigned long long rejects) {
+                                           unsigned long long rejects, unsigned long long flag = 1ull) {
     if (atomicCAS(&qc[q].notified, 0u, 1u) != 0u) return;
     unsigned long long* w = notify + 4ull * q;
     __hip_atomic_store(w + 1, count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(w + 2, calls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(w + 3, rejects, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(w, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(w, flag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Query q passed its time limit (resident launch): its remaining tasks are
+// dropped, and the count so far is posted as partial (flag 2).
+__device__ __forceinline__ void post_timeout(unsigned long long* notify, QCounters* qc, uint32_t q) {
+    atomicExch(&qc[q].timed_out, 1u);
+    if (notify) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        post_final(notify, qc, q, __hip_atomic_load(&qc[q].count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                   __hip_atomic_load(&qc[q].calls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                   __hip_atomic_load(&qc[q].rejects, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 2ull);
+    }
+}
+
+// Query q's deadline (resident launch): set by the first lane that starts
+// one of its tasks, lim ticks from then.
+__device__ __forceinline__ unsigned long long query_deadline(QCounters* qc, uint32_t q, unsigned long long lim) {
+    const unsigned long long d = __hip_atomic_load(&qc[q].deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (d) return d;
+    const unsigned long long want = (unsigned long long)wall_clock64() + lim;
+    const unsigned long long old = atomicCAS(&qc[q].deadline, 0ull, want);
+    return old ? old : want;
+}
+
+// Resident launch: the segments published so far (bit 31: closed). The
+// host's word is read over the bus at most once per kPollTicks for the whole
+// launch, by whichever wavefront asks first; the others read the copy.
+constexpr unsigned long long kPollTicks = 200;   // 2 us of the 100 MHz wall clock
+__device__ __forceinline__ uint32_t published(DQueue* Q) {
+    uint32_t v = __hip_atomic_load(&Q->pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long now = (unsigned long long)wall_clock64();
+    const unsigned long long last = __hip_atomic_load(&Q->poll_tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (now - last >= kPollTicks && atomicCAS(&Q->poll_tick, last, now) == last) {
+        const uint32_t h =
+            __hip_atomic_load(const_cast<unsigned int*>(Q->host_word), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (h > v) {
+            atomicMax(&Q->pub, h);
+            v = h;
+        }
+    }
+    return v;
 }
 
 // Query q's count reached its cap now (the first caller's time is kept).
@@ -1727,6 +1768,12 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     // (galloping), or binary search in every row
     const uint32_t max_cursors = Q->intersect == PSM_INTERSECT_BSEARCH ? 0u : kCursors;
     unsigned long long* const notify = Q->notify;
+    // resident launch: published segments instead of one array
+    const DSeg* const segs = Q->segs;
+    unsigned int* const seg_head = Q->seg_head;
+    uint32_t wseg = 0;                  // segment of the lane's claimed array positions
+    unsigned long long qdl = 0;         // deadline of the lane's query (0: none)
+    unsigned long long idle_since = 0;  // lane 0: since when the wavefront has seen the launch drained
 
     bool active = false, waiting = false, wring = false;   // wring: the claimed positions are ring positions
     uint32_t wslot = 0, wend = 0, backoff = 0;
@@ -1749,10 +1796,11 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     uint32_t steps = 0;
     uint32_t last_steps = kShortTask;   // steps of the lane's previous task (none yet: counts as long)
     const DPlan* P = plans;
-    // the held query's candidate pool and tables
-    const uint64_t* __restrict__ row_off = P->row_off;
-    const uint32_t* __restrict__ idx = P->idx;
-    const uint32_t* __restrict__ cand_pool = P->cand_pool;
+    // the held query's candidate pool and tables (set with its first task: in
+    // a resident launch no plan is read before it is published)
+    const uint64_t* __restrict__ row_off = nullptr;
+    const uint32_t* __restrict__ idx = nullptr;
+    const uint32_t* __restrict__ cand_pool = nullptr;
     const uint32_t lanes_total = gridDim.x * blockDim.x;
 
     while (true) {
@@ -1773,8 +1821,20 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             uint32_t h = 0;
             if (lane == 0) {
                 const uint32_t len = load_relaxed(&Q->tail) - load_relaxed(&Q->head);
-                const uint32_t h0 = load_relaxed(&Q->head0);
-                const uint32_t left0 = h0 < n0 ? n0 - h0 : 0u;
+                uint32_t left0;
+                if (segs) {
+                    // the current segment's tasks, and one for any later segment
+                    const uint32_t np = load_relaxed(&Q->pub) & 0x7FFFFFFFu;
+                    const uint32_t sc = load_relaxed(&Q->seg_cur);
+                    left0 = 0;
+                    if (sc < np) {
+                        const uint32_t sn = segs[sc].n, sh = load_relaxed(&seg_head[sc]);
+                        left0 = (sh < sn ? sn - sh : 0u) + (sc + 1 < np ? 1u : 0u);
+                    }
+                } else {
+                    const uint32_t h0 = load_relaxed(&Q->head0);
+                    left0 = h0 < n0 ? n0 - h0 : 0u;
+                }
                 bool late = load_relaxed(&Q->timeout) != 0;
                 if (!late && passed(deadline)) {
                     atomicExch(&Q->timeout, 1u);
@@ -1816,7 +1876,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         const bool claim_now = array_left || (witer & 15) == 1 || __ballot(active) == 0;
         if (want && backoff == 0 && claim_now) {
             const uint32_t leader = __ffsll((long long)want) - 1;
-            uint32_t base = 0, got = 0, per = 1, from_ring = 0, empty = 0;
+            uint32_t base = 0, got = 0, per = 1, from_ring = 0, empty = 0, cseg = 0;
             if (lane == leader) {
                 const uint32_t nw = (uint32_t)__popcll(want);
                 const uint32_t h = load_relaxed(&Q->head), t = load_relaxed(&Q->tail);
@@ -1836,6 +1896,31 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                         from_ring = 1;
                     } else {
                         atomicSub(&Q->busy, k);
+                    }
+                } else if (segs) {
+                    // resident launch: the oldest published segment with
+                    // tasks left, claimed as the array is (one fetch-add);
+                    // an exhausted one moves the shared cursor on
+                    const uint32_t np = published(Q) & 0x7FFFFFFFu;
+                    const uint32_t sc = load_relaxed(&Q->seg_cur);
+                    if (sc < np) {
+                        const uint32_t sn = segs[sc].n;
+                        const uint32_t sh = load_relaxed(&seg_head[sc]);
+                        if (sh < sn) {
+                            if (2u * (uint32_t)__popcll(quick & want) >= nw)
+                                per = min(kClaimMax, max(1u, (sn - sh) / (claim_div * lanes_total)));
+                            const uint32_t k = min(nw * per, sn - sh);
+                            atomicAdd(&Q->busy, k);
+                            const uint32_t old = atomicAdd(&seg_head[sc], k);
+                            got = old < sn ? min(k, sn - old) : 0u;
+                            base = old;
+                            cseg = sc;
+                            if (got < k) atomicSub(&Q->busy, k - got);
+                        } else {
+                            atomicCAS(&Q->seg_cur, sc, sc + 1);
+                        }
+                    } else {
+                        empty = 1;
                     }
                 } else {
                     const uint32_t h0 = load_relaxed(&Q->head0);
@@ -1866,11 +1951,13 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             per = __shfl(per, leader);
             from_ring = __shfl(from_ring, leader);
             empty = __shfl(empty, leader);
+            cseg = __shfl(cseg, leader);
             if (!active && !waiting) {
                 const uint32_t r = __popcll(want & lanemask_lt());
                 if (r * per < got) {
                     waiting = true;
                     wring = from_ring != 0;
+                    wseg = cseg;
                     wslot = base + r * per;
                     wend = min(wslot + per, base + got);
                 }
@@ -1886,8 +1973,9 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             if (wring) record_wait();
             // the record is read field by field with coherent loads (a local
             // copy of its prefix array would live in scratch memory)
-            const DTask* task = wring ? queue + (wslot & qmask) : init + wslot;
-            const uint32_t tq = coherent_u32(&task->q);
+            const DTask* task = wring ? queue + (wslot & qmask) : (segs ? segs[wseg].tasks : init) + wslot;
+            // array tasks of a segment hold the query's index in its batch
+            const uint32_t tq = coherent_u32(&task->q) + (wring || !segs ? 0u : segs[wseg].q_base);
             if (tq != q) {
                 // the lane's counters so far belong to the previous query
                 lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap, notify);
@@ -1899,7 +1987,10 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
             row_off = P->row_off;
             idx = P->idx;
             cand_pool = P->cand_pool;
-            if (stop || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
+            qdl = P->limit_ticks ? query_deadline(qc, q, P->limit_ticks) : 0ull;
+            const bool late = passed(qdl);
+            if (late) post_timeout(notify, qc, q);
+            if (stop || late || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
                 if (wring) seq_release(seq, qmask, wslot);
                 ++busy_done;
                 if (wslot + 1 < wend) {   // the lane's next claimed slot
@@ -1940,10 +2031,33 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         if (__ballot(active || waiting) == 0) {
             // the whole wavefront is idle: leave once no work can appear
             uint32_t leave = 0;
-            if (lane == 0)
+            if (lane == 0 && segs) {
+                // resident launch: drained = every published segment
+                // claimed, the ring empty and no task held; then leave once
+                // the host has closed the launch, or after idle_ticks with
+                // nothing new published (the host republishes a segment
+                // that arrives after the last wavefront left)
+                const uint32_t pv = published(Q);
+                const uint32_t np = pv & 0x7FFFFFFFu;
+                const uint32_t sc = load_relaxed(&Q->seg_cur);
+                const bool arrays = sc >= np || (sc + 1 == np && load_relaxed(&seg_head[sc]) >= segs[sc].n);
+                const bool drained = arrays && load_relaxed(&Q->head) == load_relaxed(&Q->tail) &&
+                                     load_relaxed(&Q->busy) == 0;
+                const unsigned long long now = (unsigned long long)wall_clock64();
+                if (!drained) {
+                    idle_since = 0;
+                } else if (pv >> 31) {
+                    leave = 1;
+                } else if (!idle_since) {
+                    idle_since = now;
+                } else if (Q->idle_ticks && now - idle_since > Q->idle_ticks) {
+                    leave = 1;
+                }
+            } else if (lane == 0) {
                 leave = load_relaxed(&Q->timeout) != 0 ||
                         (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->head0) >= n0 &&
                          load_relaxed(&Q->busy) == 0);
+            }
             if (__shfl(leave, 0)) break;
             // a short nap: idle wavefronts must take donated work at once
             // (longer waits between polls made lone searches slower)
@@ -2033,7 +2147,17 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         }
         ++steps;
         // the lanes' caps and the time limit (see the top of the loop)
-        if (check && active && (stop || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED))) active = false;
+        if (check && active) {
+            if (stop || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED)) {
+                active = false;
+            } else if (passed(qdl)) {
+                // the query's own time limit (resident launch): post what
+                // it has counted, the lane's share included
+                lane_flush(qc, q, P->cap, lcount, lcalls, lrej, lbytes, lalgo, near_cap, notify);
+                post_timeout(notify, qc, q);
+                active = false;
+            }
+        }
         // donation: split the shallowest level that still has >= 2 entries
         // into the ring when it holds fewer tasks than there are lanes
         // without one (work goes to lanes that wait for it), or, while it is

@@ -106,6 +106,9 @@ struct DPlan {
     uint32_t op_count[kMaxQ];
     uint32_t pad2[kMaxQ];
     uint64_t same_label[kMaxQ];   // earlier depths whose vertex has the same label
+    // resident launch: wall-clock ticks the query may run from the start of
+    // its first task (0: no limit)
+    uint64_t limit_ticks;
 };
 
 // One intersection operand of a depth: the table row E[b -> u_d] selected by
@@ -161,9 +164,34 @@ struct DQueue {
     unsigned int intersect;    // PSM_INTERSECT_ADAPTIVE or PSM_INTERSECT_BSEARCH (psm_abi.h)
     // Host-visible (coherent pinned) words, 4 per query, or null: once a
     // query's count is final the kernel writes its count, calls and visited
-    // rejects to words 1..3 and then 1 to word 0, so a caller can return it
-    // while the launch still runs.
+    // rejects to words 1..3 and then 1 to word 0 (2: the query passed its
+    // time limit and the count is partial), so a caller can return it while
+    // the launch still runs.
     unsigned long long* notify;
+    // Resident launch (segs != null, lane kernel only). The pre-expanded
+    // array is a list of segments that the host publishes while the launch
+    // runs: host_word (coherent pinned memory, written by the host) holds the
+    // number of segments published, and bit 31 once the host has closed the
+    // launch (no further segments). A segment's descriptor, plans, operands,
+    // counters and pending counts are written before it is published, into
+    // memory the launch has not read. The launch ends once it is closed and
+    // drained, or after idle_ticks drained with nothing new published.
+    const struct DSeg* segs;
+    unsigned int* seg_head;        // next task of each segment (zero at the launch)
+    const unsigned int* host_word;
+    unsigned int pub;              // the latest host_word a wavefront read
+    unsigned int seg_cur;          // segments before this one are known to be exhausted
+    unsigned long long poll_tick;  // wall clock of the latest read of host_word
+    unsigned long long idle_ticks; // 0: leave only once closed
+};
+
+// One published segment of a resident launch: a prepared batch's
+// pre-expanded tasks (query indices local to the batch) and the launch slot
+// of the batch's first query. One descriptor per 128-byte line.
+struct alignas(128) DSeg {
+    const DTask* tasks;
+    uint32_t n;
+    uint32_t q_base;
 };
 
 // Where the DFS writes complete embeddings (rows of n data vertex ids in
@@ -208,6 +236,9 @@ struct QCounters {
     unsigned long long end_tick;
     uint32_t notified;    // the final count has been posted to DQueue::notify
     uint32_t pad;
+    // resident launch: the query's deadline in wall-clock ticks, set by the
+    // first lane to start one of its tasks (DPlan::limit_ticks)
+    unsigned long long deadline;
 };
 
 // Kernel classes timed by the library (psm_kernel_stats).

@@ -66,6 +66,7 @@ struct psm_ctx {
     uint32_t block_lanes = kDefaultBlockLanes;  // lanes per workgroup of the lane DFS (0: 64)
     uint32_t expand_depth = kExpandMax;         // deepest level pre-expansion assigns
     uint32_t intersect = PSM_INTERSECT_ADAPTIVE;  // set intersection of the DFS kernels
+    uint32_t serve_resident = 1;                // servers admit batches into one resident launch
     double wall_khz = 100000.0;               // device wall clock rate
     bool verbose = false;
 };
@@ -453,6 +454,8 @@ struct Batch {
     int make_plans(int order_method);
     int run_plans(std::vector<DPlan> hp, std::vector<DOp> hops, EmitSink sink,
                   std::vector<QCounters>& qc, bool& timed_out);
+    int expand_tasks(const std::vector<DPlan>& hp, uint64_t target_total, DBuf& d_out, uint64_t& n_out,
+                     std::vector<uint64_t>& ntask);
     // points a plan at this batch's candidate pool and tables
     void bind_tables(DPlan& P) const {
         P.row_off = row_off.as<uint64_t>();
@@ -1326,25 +1329,16 @@ void make_plan(const QueryHost& q, const std::vector<uint32_t>& order,
 }
 
 // ---- enumeration ------------------------------------------------------------
-// Root tasks (one per plan) are pre-expanded level by level until each plan
-// has enough tasks to spread over the chip, then one persistent DFS kernel
-// drains a shared work queue that running lanes refill by splitting their
-// ranges. `sink` optionally materialises the embeddings (one plan only).
-int Batch::run_plans(std::vector<DPlan> hp, std::vector<DOp> hops, EmitSink sink,
-                     std::vector<QCounters>& qc, bool& timed_out) {
+// Pre-expansion of the root tasks of the uploaded plans (plans, ops and qcnt
+// hold hp.size() queries): root tasks are expanded level by level until
+// each query has its share of target_total tasks or reaches the depth limit.
+// The final task array goes to d_out (n_out tasks, ntask per query).
+int Batch::expand_tasks(const std::vector<DPlan>& hp, uint64_t target_total, DBuf& d_out, uint64_t& n_out,
+                        std::vector<uint64_t>& ntask) {
     const uint32_t nq = (uint32_t)hp.size();
-    if (hops.empty()) hops.push_back(DOp{});
-    for (DPlan& P : hp) bind_tables(P);
-    HIP_TRY(ctx, upload(plans, hp, s()));
-    HIP_TRY(ctx, upload(ops, hops, s()));
-    HIP_TRY(ctx, qcnt.alloc(sizeof(QCounters) * nq));
-    HIP_TRY(ctx, hipMemsetAsync(qcnt.p, 0, sizeof(QCounters) * nq, s()));
-
     std::vector<DTask> tasks;
-    std::vector<uint64_t> ntask(nq, 0);
-    uint32_t max_n = 1;
+    ntask.assign(nq, 0);
     for (uint32_t i = 0; i < nq; ++i) {
-        max_n = std::max(max_n, hp[i].n);
         if (hp[i].n == 0 || hp[i].cand_len[0] == 0) continue;
         DTask t{};
         t.q = i;
@@ -1352,12 +1346,6 @@ int Batch::run_plans(std::vector<DPlan> hp, std::vector<DOp> hops, EmitSink sink
         tasks.push_back(t);
         ntask[i] = 1;
     }
-    // persistent grid: dfs_waves_per_cu wavefronts per CU; the wavefront
-    // kernel takes one task per wavefront, the lane kernel one per lane
-    const bool wave_dfs = ctx->dfs_strategy == PSM_DFS_WAVE && sink.rows == nullptr;
-    const uint32_t grid_waves_full = (uint32_t)ctx->num_cu * ctx->dfs_waves_per_cu;
-    const uint64_t target_total = std::max<uint64_t>(
-        64, (uint64_t)((double)grid_waves_full * (wave_dfs ? 1 : 64) * ctx->expand_tasks_per_lane));
     const uint64_t budget_total = std::max<uint64_t>(target_total * 8, 1u << 22);
     uint32_t active_q = 0;
     for (uint32_t i = 0; i < nq; ++i) active_q += ntask[i] ? 1 : 0;
@@ -1512,11 +1500,42 @@ int Batch::run_plans(std::vector<DPlan> hp, std::vector<DOp> hops, EmitSink sink
             d_tasks.swap(d_final);
         }
     }
-    n_tasks = n_final;
+    n_out = n_final;
     ntask = final_cnt;
+    d_out.swap(d_tasks);
     HIP_TRY(ctx, timers_flush(ctx));
     t_expand += now_ms() - te0;
-    verbose("expand: %llu tasks in %.1f ms", (unsigned long long)n_tasks, now_ms() - te0);
+    verbose("expand: %llu tasks in %.1f ms", (unsigned long long)n_final, now_ms() - te0);
+    return PSM_OK;
+}
+
+
+// Root tasks (one per plan) are pre-expanded level by level until each plan
+// has enough tasks to spread over the chip, then one persistent DFS kernel
+// drains a shared work queue that running lanes refill by splitting their
+// ranges. `sink` optionally materialises the embeddings (one plan only).
+int Batch::run_plans(std::vector<DPlan> hp, std::vector<DOp> hops, EmitSink sink,
+                     std::vector<QCounters>& qc, bool& timed_out) {
+    const uint32_t nq = (uint32_t)hp.size();
+    if (hops.empty()) hops.push_back(DOp{});
+    for (DPlan& P : hp) bind_tables(P);
+    HIP_TRY(ctx, upload(plans, hp, s()));
+    HIP_TRY(ctx, upload(ops, hops, s()));
+    HIP_TRY(ctx, qcnt.alloc(sizeof(QCounters) * nq));
+    HIP_TRY(ctx, hipMemsetAsync(qcnt.p, 0, sizeof(QCounters) * nq, s()));
+
+    uint32_t max_n = 1;
+    for (uint32_t i = 0; i < nq; ++i) max_n = std::max(max_n, hp[i].n);
+    // persistent grid: dfs_waves_per_cu wavefronts per CU; the wavefront
+    // kernel takes one task per wavefront, the lane kernel one per lane
+    const bool wave_dfs = ctx->dfs_strategy == PSM_DFS_WAVE && sink.rows == nullptr;
+    const uint32_t grid_waves_full = (uint32_t)ctx->num_cu * ctx->dfs_waves_per_cu;
+    const uint64_t target_total = std::max<uint64_t>(
+        64, (uint64_t)((double)grid_waves_full * (wave_dfs ? 1 : 64) * ctx->expand_tasks_per_lane));
+    DBuf d_tasks;
+    uint64_t n_tasks = 0;
+    std::vector<uint64_t> ntask;
+    PSM_TRY(expand_tasks(hp, target_total, d_tasks, n_tasks, ntask));
 
     unsigned int donated = 0;
     timed_out = false;
@@ -1882,6 +1901,10 @@ int psm_set_tuning(psm_ctx* ctx, int key, double value) {
             if (!(value >= 1 && value <= kExpandMax)) return set_err(ctx, PSM_EINVAL, "expand depth outside [1, 31]");
             ctx->expand_depth = (uint32_t)value;
             return PSM_OK;
+        case PSM_TUNE_SERVE_RESIDENT:
+            if (value != 0 && value != 1) return set_err(ctx, PSM_EINVAL, "serve resident must be 0 or 1");
+            ctx->serve_resident = (uint32_t)value;
+            return PSM_OK;
         case PSM_TUNE_INTERSECT:
             if (value != PSM_INTERSECT_ADAPTIVE && value != PSM_INTERSECT_BSEARCH)
                 return set_err(ctx, PSM_EINVAL, "intersect must be PSM_INTERSECT_ADAPTIVE or PSM_INTERSECT_BSEARCH");
@@ -1908,6 +1931,7 @@ int psm_get_tuning(psm_ctx* ctx, int key, double* value) {
         case PSM_TUNE_DFS_BLOCK_LANES: *value = ctx->block_lanes; return PSM_OK;
         case PSM_TUNE_EXPAND_DEPTH: *value = ctx->expand_depth; return PSM_OK;
         case PSM_TUNE_INTERSECT: *value = ctx->intersect; return PSM_OK;
+        case PSM_TUNE_SERVE_RESIDENT: *value = ctx->serve_resident; return PSM_OK;
         default: return set_err(ctx, PSM_EINVAL, "unknown tuning key");
     }
 }
@@ -2624,10 +2648,19 @@ int psm_batch_match(psm_ctx* ctx, const psm_graph* g, uint32_t nq, const uint32_
 // ---------------------------------------------------------------------------
 // Serving: continuous batching over a resident data graph. Worker threads,
 // each with its own context and HIP stream, take the oldest pending requests
-// as one batch while slots and the memory budget allow; requests arriving
-// meanwhile are admitted by the next free worker. Batches of different
-// workers run concurrently on the device. No counterpart in the reference,
-// which runs one query per process.
+// as one batch while slots and the memory budget allow, and prepare it:
+// filter, tables, plans and the pre-expansion of its tasks. A prepared batch
+// is admitted into the resident enumeration launch, one persistent lane-DFS
+// launch that keeps running while batches are published into it (a segment
+// each): its search starts as soon as its preparation ends, beside the
+// searches of the batches before it, and each query's result is returned
+// once its own count is final. Device memory that the launch reads (the
+// batches' candidates, tables and tasks) is recycled when the launch ends:
+// the host closes it when it has nothing left to run or memory runs short,
+// and the next batch starts a new one. A batch whose queries exceed the
+// resident launch's size runs its own enumeration launch instead, as it does
+// with the resident launch switched off (PSM_TUNE_SERVE_RESIDENT 0).
+// No counterpart in the reference, which runs one query per process.
 // ---------------------------------------------------------------------------
 #include <condition_variable>
 #include <deque>
@@ -2639,6 +2672,72 @@ struct Request {
     std::unique_ptr<QueryHost> q;
     uint64_t est_bytes = 0;
     double submit_ms = 0;
+};
+
+constexpr uint32_t kEngSlots = 8192;       // query slots of one resident launch
+constexpr uint32_t kEngSegs = 1024;        // segments (admitted batches) of one launch
+constexpr uint32_t kEngOps = 1u << 20;     // intersection operands of one launch
+constexpr uint32_t kEngMaxN = 32;          // query vertices the resident launch takes
+// A batch's slots start on a 64-slot boundary and its operands on a
+// 16-operand one: its plans, counters, pending counts and operands then fill
+// 256-byte lines of their own, which the launch has not read before the
+// batch is published (the host writes them while the launch runs).
+constexpr uint32_t kSlotAlign = 64;
+constexpr uint32_t kOpAlign = 16;
+static_assert(sizeof(DPlan) * kSlotAlign % 256 == 0 && sizeof(QCounters) * kSlotAlign % 256 == 0 &&
+                  sizeof(uint32_t) * kSlotAlign % 256 == 0 && sizeof(DOp) * kOpAlign % 256 == 0,
+              "a batch's plans, counters, pending counts and operands start on 256-byte lines");
+static_assert(sizeof(DSeg) == 128, "one segment descriptor per 128-byte line");
+constexpr double kEngIdleExitMs = 500.0;   // a drained launch the host has not closed leaves after this
+constexpr double kEngIdleCloseMs = 2.0;    // the host closes a launch that has had nothing to run this long
+constexpr size_t kPrepLdsBytes = 48 * 1024;   // LDS per CU left to the preparation kernels
+
+// A batch prepared for the resident launch.
+struct Prepared {
+    std::vector<Request> reqs;
+    double start_ms = 0;      // server clock when its preparation began
+    double t_begin = 0;       // now_ms() then: the time limit counts from here
+    uint64_t est = 0;         // admission estimate charged while it was prepared
+    uint64_t held = 0;        // device bytes it holds until its launch ends
+    int rc = PSM_OK;          // preparation failed: every request gets rc
+    std::vector<DPlan> hp;    // bound to its tables, operand indices local
+    std::vector<DOp> hops;
+    std::vector<QCounters> qc0;   // counters of the pre-expansion (calls, bytes)
+    std::vector<uint64_t> ntask;
+    DBuf tasks, cand_pool, row_off, idx;
+    uint64_t n_tasks = 0;
+    uint32_t q_base = 0, seg = 0;   // placement in the running launch
+    std::vector<uint8_t> delivered;
+};
+
+// The resident launch and its host side.
+struct Engine {
+    psm_ctx* ctx = nullptr;         // launch stream and launch shape
+    hipStream_t us = nullptr;       // fills and uploads beside the running launch
+    uint32_t grid_waves = 0, ring = 0;
+    DBuf d_plans, d_ops, d_qc, d_pend, d_segs, d_heads, d_queue, d_seq, d_q;
+    unsigned int* hword = nullptr;  // coherent pinned: segments published | closed << 31
+    unsigned int* d_hword = nullptr;
+    unsigned long long* notify = nullptr;   // coherent pinned, 4 words per slot
+    unsigned long long* d_notify = nullptr;
+    hipEvent_t ev = nullptr;        // end of the running launch
+    bool running = false, closed = false;
+    uint32_t next_slot = 0, next_op = 0, n_segs = 0;
+    std::vector<std::unique_ptr<Prepared>> batches;   // published into the running launch
+    std::vector<std::pair<Prepared*, uint32_t>> owner;   // slot -> (batch, query)
+    std::vector<uint32_t> open;     // slots whose result is not delivered
+    double idle_since = -1;
+    uint64_t held = 0;              // device bytes the running launch's batches hold
+    uint64_t launches = 0;
+    ~Engine() {
+        d_plans.release(); d_ops.release(); d_qc.release(); d_pend.release(); d_segs.release();
+        d_heads.release(); d_queue.release(); d_seq.release(); d_q.release();
+        if (hword) (void)hipHostFree(hword);
+        if (notify) (void)hipHostFree(notify);
+        if (ev) (void)hipEventDestroy(ev);
+        if (us) (void)hipStreamDestroy(us);
+        if (ctx) psm_ctx_destroy(ctx);
+    }
 };
 
 }  // namespace
@@ -2654,12 +2753,17 @@ struct psm_server {
     std::vector<psm_ctx*> ctxs;
     std::vector<std::thread> workers;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_eng;
     std::deque<Request> pending;
     std::deque<psm_result> done;
     uint64_t mem_in_use = 0, peak_mem = 0, submitted = 0, completed = 0, batches = 0;
     bool stop = false;
     double t0 = 0;
+    // resident launch
+    std::unique_ptr<Engine> eng;          // null: every batch runs its own launch
+    std::thread eng_thread;
+    std::deque<std::unique_ptr<Prepared>> ready;   // prepared, not yet published
+    uint32_t preparing = 0;               // batches workers are preparing
 };
 
 namespace {
@@ -2679,6 +2783,57 @@ uint64_t footprint(const psm_graph* g, const QGraph& q) {
 }
 
 constexpr size_t kSmallBatch = 4;   // batches below this size use the default launch shape
+
+// Returns request i of a prepared batch (caller holds S->mu).
+void deliver_prepared(psm_server* S, Prepared& b, uint32_t i, uint64_t count, uint64_t calls, int status) {
+    if (b.delivered[i]) return;
+    b.delivered[i] = 1;
+    psm_result res{};
+    res.request_id = b.reqs[i].id;
+    res.count = S->cap ? std::min<uint64_t>(S->cap, count) : count;
+    res.calls = calls;
+    res.status = status;
+    res.batch_size = (uint32_t)b.reqs.size();
+    res.submit_ms = b.reqs[i].submit_ms;
+    res.start_ms = b.start_ms;
+    res.done_ms = now_ms() - S->t0;
+    S->done.push_back(res);
+    ++S->completed;
+}
+
+// Worker side of the resident launch: the batch's plans (bound to its
+// tables) and its pre-expanded tasks, with the device memory the launch
+// will read moved into p.
+int prepare_resident(psm_server* S, Batch& b, Prepared& p) {
+    psm_ctx* ctx = b.ctx;
+    tls_stream = b.s();
+    const uint32_t nq = (uint32_t)b.qs.size();
+    p.hp.assign(nq, DPlan{});
+    p.hops.clear();
+    for (uint32_t i = 0; i < nq; ++i) {
+        make_plan(*b.qs[i], b.qs[i]->plan.order, nullptr, S->cap, p.hp[i], p.hops);
+        b.bind_tables(p.hp[i]);
+    }
+    if (p.hops.empty()) p.hops.push_back(DOp{});
+    HIP_TRY(ctx, upload(b.plans, p.hp, b.s()));
+    HIP_TRY(ctx, upload(b.ops, p.hops, b.s()));
+    HIP_TRY(ctx, b.qcnt.alloc(sizeof(QCounters) * nq));
+    HIP_TRY(ctx, hipMemsetAsync(b.qcnt.p, 0, sizeof(QCounters) * nq, b.s()));
+    const uint64_t target_total = std::max<uint64_t>(
+        64, (uint64_t)((double)S->eng->grid_waves * 64 * ctx->expand_tasks_per_lane));
+    PSM_TRY(b.expand_tasks(p.hp, target_total, p.tasks, p.n_tasks, p.ntask));
+    HIP_TRY(ctx, download(p.qc0, b.qcnt.p, nq, b.s()));
+    p.cand_pool.swap(b.cand_pool);
+    p.row_off.swap(b.row_off);
+    p.idx.swap(b.idx);
+    // released by the engine once its launch has ended: nothing else is
+    // left to wait for then
+    for (DBuf* d : {&p.tasks, &p.cand_pool, &p.row_off, &p.idx}) {
+        d->st = S->eng->us;
+        p.held += d->bytes;
+    }
+    return PSM_OK;
+}
 
 void serve_worker(psm_server* S, psm_ctx* ctx) {
     (void)hipSetDevice(S->device);
@@ -2718,12 +2873,41 @@ void serve_worker(psm_server* S, psm_ctx* ctx) {
             S->mem_in_use += est;
             S->peak_mem = std::max(S->peak_mem, S->mem_in_use);
             ++S->batches;
+            ++S->preparing;
         }
         const double start = now_ms() - S->t0;
         Batch b;
         b.ctx = ctx;
         b.g = S->g;
-        for (Request& r : batch) b.qs.push_back(r.q.get());
+        uint32_t max_n = 0;
+        for (Request& r : batch) {
+            b.qs.push_back(r.q.get());
+            max_n = std::max(max_n, r.q->qg.n);
+        }
+        if (S->eng && max_n <= kEngMaxN) {
+            // resident launch: prepare, then hand the batch to the engine
+            std::unique_ptr<Prepared> p(new Prepared());
+            p->start_ms = start;
+            p->t_begin = now_ms();
+            p->est = est;
+            int rc = b.filter(S->filter);
+            if (rc == PSM_OK) rc = b.build_tables();
+            if (rc == PSM_OK) rc = b.make_plans(S->order);
+            if (rc == PSM_OK) rc = prepare_resident(S, b, *p);
+            p->rc = rc;
+            p->reqs = std::move(batch);
+            p->delivered.assign(p->reqs.size(), 0);
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                // the estimate gives way to what the launch will hold
+                S->mem_in_use = S->mem_in_use - est + p->held;
+                --S->preparing;
+                S->ready.push_back(std::move(p));
+            }
+            S->cv_eng.notify_one();
+            S->cv_done.notify_all();
+            continue;
+        }
         // A batch of a few requests runs with the library's default launch
         // shape (the plain path's): the served shape is tuned for full
         // batches, and its chip-wide pre-expansion of one query would cost
@@ -2785,7 +2969,316 @@ void serve_worker(psm_server* S, psm_ctx* ctx) {
         }
         // the batch's device buffers (candidates, tables, queue) are freed now
         S->mem_in_use -= est;
+        --S->preparing;
         S->cv_done.notify_all();
+        S->cv_eng.notify_one();
+    }
+}
+
+// ---- resident launch (engine thread) ----------------------------------------
+
+int engine_setup(psm_server* S, Engine& E, const psm_ctx* proto) {
+    PSM_TRY(psm_ctx_create(S->device, &E.ctx));
+    psm_ctx* c = E.ctx;
+    c->claim_div = proto->claim_div;
+    c->intersect = proto->intersect;
+    c->queue_slots = proto->queue_slots;
+    c->block_lanes = 0;
+    c->verbose = proto->verbose;
+    HIP_TRY(c, hipSetDevice(S->device));
+    HIP_TRY(c, hipStreamCreateWithFlags(&E.us, hipStreamNonBlocking));
+    HIP_TRY(c, hipEventCreateWithFlags(&E.ev, hipEventDisableTiming));
+    HIP_TRY(c, hipHostMalloc((void**)&E.hword, 256, hipHostMallocCoherent));
+    HIP_TRY(c, hipHostGetDevicePointer((void**)&E.d_hword, E.hword, 0));
+    HIP_TRY(c, hipHostMalloc((void**)&E.notify, (size_t)kEngSlots * 32, hipHostMallocCoherent));
+    HIP_TRY(c, hipHostGetDevicePointer((void**)&E.d_notify, E.notify, 0));
+    // wavefronts per CU: the served shape's, within what leaves the
+    // preparation kernels of later batches room in every CU's LDS
+    const size_t lds = dfs_lane_lds_bytes(kEngMaxN);
+    uint32_t waves = std::max<uint32_t>(1, proto->dfs_waves_per_cu);
+    while (waves > 1 && waves * lds + kPrepLdsBytes > 160 * 1024) --waves;
+    E.grid_waves = std::max<uint32_t>(4, (uint32_t)c->num_cu * waves / 4 * 4);
+    const uint64_t lanes_all = (uint64_t)E.grid_waves * 64;
+    const uint64_t room = c->queue_slots ? c->queue_slots : std::max<uint64_t>(1u << 18, 4 * lanes_all);
+    uint64_t ring = 64;
+    while (ring < room) ring <<= 1;
+    E.ring = (uint32_t)ring;
+    hipStream_t prev = tls_stream;
+    tls_stream = E.us;
+    hipError_t e = hipSuccess;
+    for (auto pr : {std::make_pair(&E.d_plans, sizeof(DPlan) * kEngSlots),
+                    std::make_pair(&E.d_ops, sizeof(DOp) * kEngOps),
+                    std::make_pair(&E.d_qc, sizeof(QCounters) * kEngSlots),
+                    std::make_pair(&E.d_pend, sizeof(uint32_t) * kEngSlots),
+                    std::make_pair(&E.d_segs, sizeof(DSeg) * kEngSegs),
+                    std::make_pair(&E.d_heads, sizeof(uint32_t) * kEngSegs),
+                    std::make_pair(&E.d_queue, sizeof(DTask) * ring),
+                    std::make_pair(&E.d_seq, sizeof(uint32_t) * ring),
+                    std::make_pair(&E.d_q, sizeof(DQueue))})
+        if (e == hipSuccess) e = pr.first->alloc(pr.second);
+    tls_stream = prev;
+    HIP_TRY(c, e);
+    E.owner.assign(kEngSlots, {nullptr, 0});
+    return PSM_OK;
+}
+
+// A new launch: counters, pending counts and segment heads cleared, the
+// ring reset, nothing published; then the persistent kernel.
+int engine_start(Engine& E) {
+    psm_ctx* c = E.ctx;
+    HIP_TRY(c, launch_fill_u32(E.d_qc.as<uint32_t>(), 0u, sizeof(QCounters) * kEngSlots / 4, E.us));
+    HIP_TRY(c, launch_fill_u32(E.d_pend.as<uint32_t>(), 0u, kEngSlots, E.us));
+    HIP_TRY(c, launch_fill_u32(E.d_heads.as<uint32_t>(), 0u, kEngSegs, E.us));
+    HIP_TRY(c, launch_ring_init(E.d_seq.as<uint32_t>(), E.ring, 0u, E.us));
+    DQueue hq{};
+    hq.mask = E.ring - 1;
+    hq.claim_div = c->claim_div;
+    hq.intersect = c->intersect;
+    hq.notify = E.d_notify;
+    hq.segs = E.d_segs.as<DSeg>();
+    hq.seg_head = E.d_heads.as<unsigned int>();
+    hq.host_word = E.d_hword;
+    hq.idle_ticks = (unsigned long long)(kEngIdleExitMs * c->wall_khz);
+    HIP_TRY(c, hipMemcpyAsync(E.d_q.p, &hq, sizeof(hq), hipMemcpyHostToDevice, E.us));
+    __atomic_store_n(E.hword, 0u, __ATOMIC_SEQ_CST);
+    memset(E.notify, 0, (size_t)kEngSlots * 32);
+    HIP_TRY(c, hipStreamSynchronize(E.us));
+    HIP_TRY(c, launch_dfs(E.d_plans.as<DPlan>(), E.d_ops.as<DOp>(), nullptr, E.d_queue.as<DTask>(),
+                          E.d_seq.as<uint32_t>(), E.d_q.as<DQueue>(), E.d_qc.as<QCounters>(),
+                          E.d_pend.as<uint32_t>(), kEngMaxN, E.grid_waves, 0, EmitSink{nullptr, nullptr, 0, nullptr},
+                          c->stream));
+    HIP_TRY(c, hipEventRecord(E.ev, c->stream));
+    E.running = true;
+    E.closed = false;
+    E.next_slot = E.next_op = E.n_segs = 0;
+    E.idle_since = -1;
+    ++E.launches;
+    return PSM_OK;
+}
+
+bool engine_fits(const Engine& E, const Prepared& b) {
+    const uint64_t q_base = (E.next_slot + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
+    const uint64_t op_base = (E.next_op + kOpAlign - 1) / kOpAlign * kOpAlign;
+    return E.n_segs < kEngSegs && q_base + b.reqs.size() <= kEngSlots && op_base + b.hops.size() <= kEngOps;
+}
+
+void engine_close(Engine& E) {
+    if (!E.running || E.closed) return;
+    __atomic_store_n(E.hword, E.n_segs | 0x80000000u, __ATOMIC_RELEASE);
+    E.closed = true;
+}
+
+// Writes a prepared batch's plans, operands, counters, pending counts and
+// segment descriptor into the running launch (memory it has not read), then
+// publishes the segment; the launch then owns the batch (bp is left empty).
+int engine_publish(psm_server* S, Engine& E, std::unique_ptr<Prepared>& bp) {
+    psm_ctx* c = E.ctx;
+    Prepared& b = *bp;
+    const uint32_t nq = (uint32_t)b.reqs.size();
+    const uint32_t q_base = (E.next_slot + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
+    const uint32_t op_base = (E.next_op + kOpAlign - 1) / kOpAlign * kOpAlign;
+    std::vector<DPlan> hp = b.hp;
+    uint64_t limit_ticks = 0;
+    if (S->time_limit_s > 0) {
+        const double left_s = S->time_limit_s - (now_ms() - b.t_begin) / 1000.0;
+        limit_ticks = (uint64_t)std::max(1.0, left_s * c->wall_khz * 1000.0);
+    }
+    for (DPlan& P : hp) {
+        for (uint32_t d = 0; d < P.n; ++d) P.op_begin[d] += op_base;
+        P.limit_ticks = limit_ticks;
+    }
+    std::vector<uint32_t> pend(nq);
+    for (uint32_t i = 0; i < nq; ++i) pend[i] = (uint32_t)b.ntask[i];
+    DSeg seg{};
+    seg.tasks = b.tasks.as<DTask>();
+    seg.n = (uint32_t)b.n_tasks;
+    seg.q_base = q_base;
+    HIP_TRY(c, hipMemcpyAsync(E.d_plans.as<DPlan>() + q_base, hp.data(), sizeof(DPlan) * nq,
+                              hipMemcpyHostToDevice, E.us));
+    HIP_TRY(c, hipMemcpyAsync(E.d_ops.as<DOp>() + op_base, b.hops.data(), sizeof(DOp) * b.hops.size(),
+                              hipMemcpyHostToDevice, E.us));
+    HIP_TRY(c, hipMemcpyAsync(E.d_qc.as<QCounters>() + q_base, b.qc0.data(), sizeof(QCounters) * nq,
+                              hipMemcpyHostToDevice, E.us));
+    HIP_TRY(c, hipMemcpyAsync(E.d_pend.as<uint32_t>() + q_base, pend.data(), sizeof(uint32_t) * nq,
+                              hipMemcpyHostToDevice, E.us));
+    HIP_TRY(c, hipMemcpyAsync(E.d_segs.as<DSeg>() + E.n_segs, &seg, sizeof(seg), hipMemcpyHostToDevice, E.us));
+    HIP_TRY(c, hipStreamSynchronize(E.us));
+    E.next_slot = q_base + nq;
+    E.next_op = op_base + (uint32_t)b.hops.size();
+    b.q_base = q_base;
+    b.seg = E.n_segs;
+    {
+        // a query without tasks (or without candidates) is final now
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (uint32_t i = 0; i < nq; ++i) {
+            E.owner[q_base + i] = {&b, i};
+            const QueryHost* q = b.reqs[i].q.get();
+            if (b.ntask[i] == 0 || q->status != PSM_OK)
+                deliver_prepared(S, b, i, 0, b.qc0[i].calls, q->status);
+            else
+                E.open.push_back(q_base + i);
+        }
+    }
+    S->cv_done.notify_all();
+    ++E.n_segs;
+    __atomic_store_n(E.hword, E.n_segs, __ATOMIC_RELEASE);
+    E.held += b.held;
+    E.batches.push_back(std::move(bp));
+    return PSM_OK;
+}
+
+// Results the launch has posted.
+void engine_poll(psm_server* S, Engine& E) {
+    bool any = false;
+    for (size_t k = 0; k < E.open.size();) {
+        const uint32_t slot = E.open[k];
+        const unsigned long long* w = E.notify + 4ull * slot;
+        const unsigned long long flag = __atomic_load_n(w, __ATOMIC_ACQUIRE);
+        if (!flag) {
+            ++k;
+            continue;
+        }
+        const uint64_t count = __atomic_load_n(w + 1, __ATOMIC_RELAXED);
+        const uint64_t calls = __atomic_load_n(w + 2, __ATOMIC_RELAXED);
+        if (!any) S->mu.lock();
+        any = true;
+        deliver_prepared(S, *E.owner[slot].first, E.owner[slot].second, count, calls,
+                         flag == 2 ? PSM_ETIMEOUT : PSM_OK);
+        E.open[k] = E.open.back();
+        E.open.pop_back();
+    }
+    if (any) {
+        S->mu.unlock();
+        S->cv_done.notify_all();
+    }
+}
+
+// The launch has ended: results it did not post come from its counters; a
+// batch published after its last wavefront left goes back to the queue for
+// the next launch; the memory its batches held is recycled.
+int engine_finish(psm_server* S, Engine& E, std::deque<std::unique_ptr<Prepared>>& again, int failed) {
+    psm_ctx* c = E.ctx;
+    std::vector<uint32_t> heads, pend;
+    std::vector<QCounters> qc;
+    int rc = failed;
+    if (rc == PSM_OK) {
+        hipError_t e = download(heads, E.d_heads.p, E.n_segs, E.us);
+        if (e == hipSuccess) e = download(pend, E.d_pend.p, E.next_slot, E.us);
+        if (e == hipSuccess) e = download(qc, E.d_qc.p, E.next_slot, E.us);
+        if (e != hipSuccess) rc = hip_fail(c, e, "resident launch read-back", __LINE__);
+    }
+    uint64_t freed = 0;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (auto& bp : E.batches) {
+            Prepared& b = *bp;
+            if (rc == PSM_OK && b.n_tasks && heads[b.seg] == 0) {
+                again.push_back(std::move(bp));   // never started: run it in the next launch
+                continue;
+            }
+            for (uint32_t i = 0; i < (uint32_t)b.reqs.size(); ++i) {
+                if (b.delivered[i]) continue;
+                if (rc != PSM_OK) {
+                    deliver_prepared(S, b, i, 0, 0, rc);
+                    continue;
+                }
+                const uint32_t slot = b.q_base + i;
+                const QCounters& k = qc[slot];
+                const bool partial = k.timed_out || (pend[slot] && !k.done) || heads[b.seg] < b.n_tasks;
+                deliver_prepared(S, b, i, k.count, k.calls, partial ? PSM_ETIMEOUT : PSM_OK);
+            }
+            freed += b.held;
+        }
+        S->mem_in_use -= freed;
+    }
+    // the blocks go back to the cache only now: the launch that read them has ended
+    E.batches.clear();
+    E.open.clear();
+    E.held = 0;
+    E.running = false;
+    S->cv_done.notify_all();
+    return rc;
+}
+
+void engine_loop(psm_server* S) {
+    Engine& E = *S->eng;
+    (void)hipSetDevice(S->device);
+    tls_stream = E.us;
+    std::deque<std::unique_ptr<Prepared>> again;   // batches for the next launch
+    while (true) {
+        std::deque<std::unique_ptr<Prepared>> got;
+        bool stopping = false, others_idle = false;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (!E.running && again.empty())
+                S->cv_eng.wait(lk, [&] {
+                    return !S->ready.empty() || (S->stop && S->pending.empty() && S->preparing == 0);
+                });
+            got.swap(S->ready);
+            stopping = S->stop && S->pending.empty() && S->preparing == 0;
+            others_idle = S->preparing == 0;
+        }
+        while (!got.empty()) {
+            again.push_back(std::move(got.front()));
+            got.pop_front();
+        }
+        if (again.empty() && !E.running && stopping) break;
+        // publish what fits into the running launch (a new one if none runs)
+        while (!again.empty()) {
+            std::unique_ptr<Prepared>& b = again.front();
+            if (b->rc != PSM_OK) {
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    for (uint32_t i = 0; i < (uint32_t)b->reqs.size(); ++i) deliver_prepared(S, *b, i, 0, 0, b->rc);
+                    S->mem_in_use -= b->held;
+                }
+                S->cv_done.notify_all();
+                again.pop_front();
+                continue;
+            }
+            if (E.running && (E.closed || !engine_fits(E, *b))) {
+                engine_close(E);   // the next launch takes it
+                break;
+            }
+            int rc = E.running ? PSM_OK : engine_start(E);
+            if (rc == PSM_OK) rc = engine_publish(S, E, b);
+            if (rc != PSM_OK) {
+                if (b) {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    for (uint32_t i = 0; i < (uint32_t)b->reqs.size(); ++i) deliver_prepared(S, *b, i, 0, 0, rc);
+                    S->mem_in_use -= b->held;
+                }
+                S->cv_done.notify_all();
+            }
+            again.pop_front();
+        }
+        if (!E.running) continue;
+        engine_poll(S, E);
+        // close the launch when it has had nothing to run for a while (its
+        // memory is then recycled), at once when stopping, or when the
+        // memory its batches hold reaches half the budget
+        const double now = now_ms();
+        if (!E.closed) {
+            const bool idle = E.open.empty() && again.empty() && others_idle;
+            if (!idle) {
+                E.idle_since = -1;
+            } else if (stopping) {
+                engine_close(E);
+            } else if (E.idle_since < 0) {
+                E.idle_since = now;
+            } else if (now - E.idle_since > kEngIdleCloseMs) {
+                engine_close(E);
+            }
+            if (E.held > S->mem_budget / 2) engine_close(E);
+        }
+        const hipError_t qe = hipEventQuery(E.ev);
+        if (qe == hipSuccess) {
+            engine_finish(S, E, again, PSM_OK);
+        } else if (qe != hipErrorNotReady) {
+            engine_finish(S, E, again, hip_fail(E.ctx, qe, "resident launch", __LINE__));
+        } else {
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
     }
 }
 
@@ -2815,6 +3308,14 @@ int psm_server_create(psm_ctx* ctx, const psm_graph* g, uint32_t workers, uint32
     }
     S->mem_budget = mem_budget;
     S->t0 = now_ms();
+    if (ctx->serve_resident) {
+        S->eng.reset(new Engine());
+        const int r = engine_setup(S.get(), *S->eng, ctx);
+        if (r != PSM_OK) {
+            set_err(ctx, r, "resident launch: " + (S->eng->ctx ? S->eng->ctx->err : std::string("no context")));
+            return r;
+        }
+    }
     for (uint32_t i = 0; i < workers; ++i) {
         psm_ctx* c = nullptr;
         const int r = psm_ctx_create(ctx->device, &c);
@@ -2836,6 +3337,7 @@ int psm_server_create(psm_ctx* ctx, const psm_graph* g, uint32_t workers, uint32
         S->ctxs.push_back(c);
     }
     for (psm_ctx* c : S->ctxs) S->workers.emplace_back(serve_worker, S.get(), c);
+    if (S->eng) S->eng_thread = std::thread(engine_loop, S.get());
     *out = S.release();
     return PSM_OK;
 }
@@ -2895,7 +3397,12 @@ int psm_server_destroy(psm_server* S) {
     }
     S->cv_work.notify_all();
     S->cv_done.notify_all();
+    S->cv_eng.notify_all();
     for (auto& t : S->workers) t.join();
+    // the workers have handed over every batch: the engine publishes them,
+    // closes its launch once everything has run, and returns
+    S->cv_eng.notify_all();
+    if (S->eng_thread.joinable()) S->eng_thread.join();
     for (psm_ctx* c : S->ctxs) psm_ctx_destroy(c);
     delete S;
     return PSM_OK;

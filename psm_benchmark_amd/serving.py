@@ -14,13 +14,28 @@ from .matcher import DeviceGraph, _check
 
 
 class Server:
+    """resident: admit prepared batches into one resident enumeration launch
+    (True, the library's default) or run one launch per batch (False); None
+    keeps the device context's setting (PSM_TUNE_SERVE_RESIDENT)."""
+
     def __init__(self, dg: DeviceGraph, workers: int = 2, max_batch: int = 64, mem_budget: int = 0,
-                 filter: str = "GQL", order: str = "GQL", cap: int = 0, time_limit: float = 60.0):
+                 filter: str = "GQL", order: str = "GQL", cap: int = 0, time_limit: float = 60.0,
+                 resident: bool | None = None):
         self.dg = dg
         h = ctypes.c_void_p()
-        _check(dg.dev.h, L.lib().psm_server_create(dg.dev.h, dg.h, workers, max_batch, mem_budget,
+        lib = L.lib()
+        prev = ctypes.c_double()
+        _check(dg.dev.h, lib.psm_get_tuning(dg.dev.h, L.PSM_TUNE_SERVE_RESIDENT, ctypes.byref(prev)),
+               "psm_get_tuning")
+        if resident is not None:
+            _check(dg.dev.h, lib.psm_set_tuning(dg.dev.h, L.PSM_TUNE_SERVE_RESIDENT, 1.0 if resident else 0.0),
+                   "psm_set_tuning")
+        try:
+            _check(dg.dev.h, lib.psm_server_create(dg.dev.h, dg.h, workers, max_batch, mem_budget,
                                                    L.FILTERS[filter], L.ORDERS[order], cap, time_limit,
                                                    ctypes.byref(h)), "psm_server_create")
+        finally:
+            lib.psm_set_tuning(dg.dev.h, L.PSM_TUNE_SERVE_RESIDENT, prev.value)
         self.h = h
         self._keep = {}
 

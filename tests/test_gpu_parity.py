@@ -6,6 +6,7 @@ import itertools
 import json
 import math
 import os
+import time
 
 import numpy as np
 import pytest
@@ -424,14 +425,17 @@ def test_time_limit_reports_partial_count(hprd):
     assert match(dg, q, "GQL_GQL_GQL").count == 1015922
 
 
-def test_served_results_equal_alone_and_in_the_mix(hprd):
+@pytest.mark.parametrize("resident", [True, False], ids=["resident", "per_batch"])
+def test_served_results_equal_alone_and_in_the_mix(hprd, resident):
     """Requests served through continuous batching (several workers, mixed
-    arrivals) give each query exactly its count when matched alone."""
+    arrivals; batches admitted into one resident launch, or one launch per
+    batch) give each query exactly its count and Call Count when matched
+    alone."""
     from psm_benchmark_amd.serving import Server, run_poisson
     g, dg, _ = hprd
     recs = golden(("GQL_GQL_GQL",))
     qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
-    srv = Server(dg, workers=3, max_batch=4, filter="GQL", order="GQL")
+    srv = Server(dg, workers=3, max_batch=4, filter="GQL", order="GQL", resident=resident)
     try:
         results, summary = run_poisson(srv, qs, rate_qps=500.0, n_requests=60, seed=7)
         print(summary, srv.stats(), flush=True)
@@ -442,6 +446,71 @@ def test_served_results_equal_alone_and_in_the_mix(hprd):
         assert r.count == recs[k % len(recs)]["embeddings"], k
         assert r.calls == recs[k % len(recs)]["calls"], k
     assert max(r.batch_size for r in results.values()) >= 1
+
+
+def test_resident_launch_reopens_and_takes_single_request_batches(hprd):
+    """One request per batch, in two waves with a pause between them: the
+    first launch closes once it has nothing to run, the second wave starts a
+    new one; every count and Call Count is the golden one."""
+    from psm_benchmark_amd.serving import Server
+    g, dg, _ = hprd
+    recs = golden(("GQL_GQL_GQL",))
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
+    srv = Server(dg, workers=4, max_batch=1, filter="GQL", order="GQL", resident=True)
+    results = {}
+    try:
+        for wave in range(2):
+            for k in range(len(qs)):
+                srv.submit(wave * len(qs) + k, qs[k])
+            t = time.time()
+            while len(results) < (wave + 1) * len(qs) and time.time() - t < 60:
+                for r in srv.poll(64, 20.0):
+                    results[int(r.request_id)] = r
+            time.sleep(0.05)
+    finally:
+        srv.close()
+    assert len(results) == 2 * len(qs)
+    for k, r in results.items():
+        assert r.status == 0 and r.batch_size == 1, k
+        assert r.count == recs[k % len(recs)]["embeddings"], k
+        assert r.calls == recs[k % len(recs)]["calls"], k
+
+
+def test_resident_launch_time_limit_gives_partial_counts(hprd):
+    """A time limit the heavy query cannot meet: the resident launch drops
+    each query's work at its own deadline and returns its partial count with
+    PSM_ETIMEOUT (the reference's Overtime: 1); with a generous limit the
+    same requests are exact."""
+    from psm_benchmark_amd import _lib as L
+    from psm_benchmark_amd.serving import Server
+    g, dg, _ = hprd
+    recs = golden(("GQL_GQL_GQL",))
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
+    full = [r["embeddings"] for r in recs]
+    qs.append(load_text(os.path.join(HPRD, "Q10-10.txt")))
+    full.append(1015922)
+    for limit in (1e-9, 60.0):
+        srv = Server(dg, workers=2, max_batch=4, filter="GQL", order="GQL", time_limit=limit, resident=True)
+        results = {}
+        try:
+            for k in range(len(qs)):
+                srv.submit(k, qs[k])
+            t = time.time()
+            while len(results) < len(qs) and time.time() - t < 60:
+                for r in srv.poll(64, 20.0):
+                    results[int(r.request_id)] = r
+        finally:
+            srv.close()
+        assert len(results) == len(qs)
+        for k, r in results.items():
+            if limit >= 1:
+                assert r.status == 0 and r.count == full[k], k
+            elif r.status == L.PSM_ETIMEOUT:
+                assert r.count < full[k], k
+            else:   # a light query can end within its first task
+                assert r.status == 0 and r.count == full[k], k
+        if limit < 1:   # the heavy one cannot
+            assert results[len(qs) - 1].status == L.PSM_ETIMEOUT
 
 
 def test_served_dpiso_results_equal_golden(hprd):
