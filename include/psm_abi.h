@@ -351,12 +351,22 @@ int psm_batch_destroy(psm_batch* b);
  * its own HIP stream, admit the oldest pending requests as one batch while the
  * batch holds fewer than max_batch requests and the estimated device
  * footprint of running batches stays within mem_budget (0: half the free
- * device memory); later arrivals join the next batch a worker forms, and
- * batches of different workers overlap on the device. Every batch runs
- * filter, tables, plan and the persistent enumeration with `cap` and the
- * time limit, with the creating context's launch shape (psm_set_tuning).
- * Times in psm_result are milliseconds since server creation.
- * No counterpart in the reference, which runs one query per process. */
+ * device memory), and prepare it (filter, tables, plan, task pre-expansion).
+ * A prepared batch is published into the resident enumeration launch: one
+ * persistent kernel that keeps running while batches join it, so each
+ * batch's search starts when its preparation ends, beside earlier batches'
+ * searches. Each request's result is returned once its own count is final
+ * (its cap reached, its last task ended, or its time limit passed: then
+ * PSM_ETIMEOUT with the partial count). The time limit counts from the start
+ * of the batch's preparation. The launch ends when it has had nothing to run
+ * for a short while or its batches hold half the memory budget; the next
+ * batch starts a new one. With PSM_TUNE_SERVE_RESIDENT 0 on the creating
+ * context (and for queries above 32 vertices) every batch runs its own
+ * enumeration launch instead, batches of different workers overlapping on
+ * the device. `cap` and the creating context's launch shape
+ * (psm_set_tuning) apply to both. Times in psm_result are milliseconds since
+ * server creation. No counterpart in the reference, which runs one query per
+ * process. */
 typedef struct psm_server psm_server;
 typedef struct psm_result {
     uint64_t request_id;

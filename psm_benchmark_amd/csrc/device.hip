@@ -1634,6 +1634,73 @@ __device__ __forceinline__ uint32_t published(DQueue* Q) {
     return v;
 }
 
+// Resident launch: tasks left in the current segment, plus one if a later
+// segment is published (a hint for the donation rules).
+__device__ __forceinline__ uint32_t segment_left(DQueue* Q, const DSeg* segs, unsigned int* seg_head) {
+    const uint32_t np = __hip_atomic_load(&Q->pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0x7FFFFFFFu;
+    const uint32_t sc = __hip_atomic_load(&Q->seg_cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (sc >= np) return 0;
+    const uint32_t sn = segs[sc].n;
+    const uint32_t sh = __hip_atomic_load(&seg_head[sc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (sh < sn ? sn - sh : 0u) + (sc + 1 < np ? 1u : 0u);
+}
+
+// Resident launch, one lane of an idle wavefront: is the launch drained
+// (every published segment claimed, the ring empty, no task held), and
+// should the wavefront leave? It leaves once the launch is drained and
+// closed by the host, or drained for idle_ticks since the wavefront last had
+// work, with nothing new published. The first to leave that way marks the
+// launch exiting: from then on no segment is claimed, and every wavefront
+// leaves once the ring is empty and no task is held (a claim that raced
+// with the mark runs to its end). What a segment published meanwhile has
+// left unclaimed is published again by the host in its next launch, so a
+// new segment never runs on the few wavefronts a leaving launch still has.
+__device__ __forceinline__ bool resident_idle(DQueue* Q, const DSeg* segs, unsigned int* seg_head,
+                                              unsigned long long& idle_since, bool& leave) {
+    if (__hip_atomic_load(&Q->exiting, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        leave = __hip_atomic_load(&Q->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                    __hip_atomic_load(&Q->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+                __hip_atomic_load(&Q->busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        return true;
+    }
+    const uint32_t pv = published(Q);
+    const uint32_t np = pv & 0x7FFFFFFFu;
+    const uint32_t sc = __hip_atomic_load(&Q->seg_cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool arrays =
+        sc >= np ||
+        (sc + 1 == np && __hip_atomic_load(&seg_head[sc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= segs[sc].n);
+    const bool drained = arrays &&
+                         __hip_atomic_load(&Q->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                             __hip_atomic_load(&Q->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+                         __hip_atomic_load(&Q->busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    const unsigned long long now = (unsigned long long)wall_clock64();
+    leave = false;
+    if (!drained) {
+        idle_since = 0;
+    } else if (pv >> 31) {
+        leave = true;
+    } else if (!idle_since) {
+        idle_since = now;
+    } else if (Q->idle_ticks && now - idle_since > Q->idle_ticks) {
+        atomicExch(&Q->exiting, 1u);
+        leave = true;
+    }
+    return drained;
+}
+
+// Resident launch: an idle wavefront's nap, growing with every poll that
+// found nothing (to 8 x 3.4 us once the launch is drained, 4 x 3.4 us while
+// other searches run and may split off work). A resident launch spends most
+// of its time with most wavefronts idle, beside the kernels that prepare the
+// next batches; each poll is a handful of device-scope loads and atomics on
+// the queue's two lines, the lines the running wavefronts claim and split
+// on. Polling every microsecond from every idle wavefront of a grid of 4 per
+// CU served C4 at a seventh of the rate of one per CU.
+__device__ __forceinline__ void resident_nap(bool drained, uint32_t& nap) {
+    nap = min(nap + 1u, drained ? 8u : 4u);
+    for (uint32_t i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(127);
+}
+
 // Query q's count reached its cap now (the first caller's time is kept).
 __device__ __forceinline__ void mark_final(QCounters* qc, uint32_t q, unsigned long long cap,
                                            unsigned long long* notify) {
@@ -1774,6 +1841,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
     uint32_t wseg = 0;                  // segment of the lane's claimed array positions
     unsigned long long qdl = 0;         // deadline of the lane's query (0: none)
     unsigned long long idle_since = 0;  // lane 0: since when the wavefront has seen the launch drained
+    uint32_t nap = 0;                   // resident launch: naps of an idle wavefront, growing
 
     bool active = false, waiting = false, wring = false;   // wring: the claimed positions are ring positions
     uint32_t wslot = 0, wend = 0, backoff = 0;
@@ -1823,14 +1891,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                 const uint32_t len = load_relaxed(&Q->tail) - load_relaxed(&Q->head);
                 uint32_t left0;
                 if (segs) {
-                    // the current segment's tasks, and one for any later segment
-                    const uint32_t np = load_relaxed(&Q->pub) & 0x7FFFFFFFu;
-                    const uint32_t sc = load_relaxed(&Q->seg_cur);
-                    left0 = 0;
-                    if (sc < np) {
-                        const uint32_t sn = segs[sc].n, sh = load_relaxed(&seg_head[sc]);
-                        left0 = (sh < sn ? sn - sh : 0u) + (sc + 1 < np ? 1u : 0u);
-                    }
+                    left0 = segment_left(Q, segs, seg_head);
                 } else {
                     const uint32_t h0 = load_relaxed(&Q->head0);
                     left0 = h0 < n0 ? n0 - h0 : 0u;
@@ -1901,7 +1962,7 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
                     // resident launch: the oldest published segment with
                     // tasks left, claimed as the array is (one fetch-add);
                     // an exhausted one moves the shared cursor on
-                    const uint32_t np = published(Q) & 0x7FFFFFFFu;
+                    const uint32_t np = load_relaxed(&Q->exiting) ? 0u : published(Q) & 0x7FFFFFFFu;
                     const uint32_t sc = load_relaxed(&Q->seg_cur);
                     if (sc < np) {
                         const uint32_t sn = segs[sc].n;
@@ -2030,40 +2091,28 @@ __global__ __launch_bounds__(kBlock) void k_dfs(
         }
         if (__ballot(active || waiting) == 0) {
             // the whole wavefront is idle: leave once no work can appear
-            uint32_t leave = 0;
+            uint32_t leave = 0, drained_w = 0;
             if (lane == 0 && segs) {
-                // resident launch: drained = every published segment
-                // claimed, the ring empty and no task held; then leave once
-                // the host has closed the launch, or after idle_ticks with
-                // nothing new published (the host republishes a segment
-                // that arrives after the last wavefront left)
-                const uint32_t pv = published(Q);
-                const uint32_t np = pv & 0x7FFFFFFFu;
-                const uint32_t sc = load_relaxed(&Q->seg_cur);
-                const bool arrays = sc >= np || (sc + 1 == np && load_relaxed(&seg_head[sc]) >= segs[sc].n);
-                const bool drained = arrays && load_relaxed(&Q->head) == load_relaxed(&Q->tail) &&
-                                     load_relaxed(&Q->busy) == 0;
-                const unsigned long long now = (unsigned long long)wall_clock64();
-                if (!drained) {
-                    idle_since = 0;
-                } else if (pv >> 31) {
-                    leave = 1;
-                } else if (!idle_since) {
-                    idle_since = now;
-                } else if (Q->idle_ticks && now - idle_since > Q->idle_ticks) {
-                    leave = 1;
-                }
+                bool lv = false;
+                drained_w = resident_idle(Q, segs, seg_head, idle_since, lv) ? 1u : 0u;
+                leave = lv ? 1u : 0u;
             } else if (lane == 0) {
                 leave = load_relaxed(&Q->timeout) != 0 ||
                         (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->head0) >= n0 &&
                          load_relaxed(&Q->busy) == 0);
             }
             if (__shfl(leave, 0)) break;
+            if (segs) {
+                resident_nap(__shfl(drained_w, 0) != 0, nap);
+                continue;
+            }
             // a short nap: idle wavefronts must take donated work at once
             // (longer waits between polls made lone searches slower)
             __builtin_amdgcn_s_sleep(4);
             continue;
         }
+        nap = 0;
+        idle_since = 0;
         ++all_iters;
         if (!active) continue;
         ++busy_iters;
@@ -2490,12 +2539,17 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
     // through an LDS-staged window, or every row by per-lane binary search
     const uint64_t coop_row = Q->intersect == PSM_INTERSECT_BSEARCH ? ~0ull : (uint64_t)kCoopRow;
     unsigned long long bytes = 0, algo = 0, busy_pos = 0, all_pos = 0;
+    // resident launch: published segments instead of one array
+    const DSeg* const segs = Q->segs;
+    unsigned int* const seg_head = Q->seg_head;
+    unsigned long long idle_since = 0;   // lane 0: since when the launch has looked drained
+    uint32_t nap = 0;
 
     while (true) {
         // ---- claim one task (none once the time limit has passed: queued
         // tasks stay counted in pending[] and their queries report a timeout)
         // split-off work from the ring first, then the pre-expanded array
-        uint32_t slot = kInvalid, late = 0, from_ring = 0;
+        uint32_t slot = kInvalid, late = 0, from_ring = 0, sseg = 0;
         if (lane == 0) late = load_relaxed(&Q->timeout);
         if (__shfl(late, 0)) break;
         if (lane == 0) {
@@ -2507,6 +2561,26 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                     from_ring = 1;
                 } else {
                     atomicSub(&Q->busy, 1u);
+                }
+            } else if (segs) {
+                // resident launch: the oldest published segment with tasks
+                // left (as in k_dfs)
+                const uint32_t np = load_relaxed(&Q->exiting) ? 0u : published(Q) & 0x7FFFFFFFu;
+                const uint32_t sc = load_relaxed(&Q->seg_cur);
+                if (sc < np) {
+                    const uint32_t sn = segs[sc].n;
+                    if (load_relaxed(&seg_head[sc]) < sn) {
+                        atomicAdd(&Q->busy, 1u);
+                        const uint32_t old = atomicAdd(&seg_head[sc], 1u);
+                        if (old < sn) {
+                            slot = old;
+                            sseg = sc;
+                        } else {
+                            atomicSub(&Q->busy, 1u);
+                        }
+                    } else {
+                        atomicCAS(&Q->seg_cur, sc, sc + 1);
+                    }
                 }
             } else {
                 const uint32_t h0 = load_relaxed(&Q->head0);
@@ -2521,35 +2595,55 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
         }
         slot = __shfl(slot, 0);
         from_ring = __shfl(from_ring, 0);
+        sseg = __shfl(sseg, 0);
         if (slot == kInvalid) {
             all_pos += 1;
-            uint32_t stop = 0;
-            if (lane == 0)
+            uint32_t stop = 0, drained_w = 0;
+            if (lane == 0 && segs) {
+                bool lv = false;
+                drained_w = resident_idle(Q, segs, seg_head, idle_since, lv) ? 1u : 0u;
+                stop = lv ? 1u : 0u;
+            } else if (lane == 0) {
                 stop = load_relaxed(&Q->timeout) ||
                        (load_relaxed(&Q->head) == load_relaxed(&Q->tail) && load_relaxed(&Q->head0) >= n0 &&
                         load_relaxed(&Q->busy) == 0);
+            }
             if (__shfl(stop, 0)) break;
+            if (segs) {
+                resident_nap(__shfl(drained_w, 0) != 0, nap);
+                continue;
+            }
             __builtin_amdgcn_s_sleep(8);
             continue;
         }
+        nap = 0;
+        idle_since = 0;
         if (from_ring) {
             while (seq_load(seq, qmask, slot) != slot + 1u) __builtin_amdgcn_s_sleep(1);
             record_wait();
         }
         // read with coherent loads (see coherent_u32)
-        const DTask* task = from_ring ? queue + (slot & qmask) : init + slot;
+        const DTask* task = from_ring ? queue + (slot & qmask) : (segs ? segs[sseg].tasks : init) + slot;
         // every lane reads the same record: q (and the plan's pointers) are
-        // wavefront-uniform
-        const uint32_t q = __builtin_amdgcn_readfirstlane(coherent_u32(&task->q));
+        // wavefront-uniform; array tasks of a segment hold the query's index
+        // in its batch
+        const uint32_t q = __builtin_amdgcn_readfirstlane(
+            coherent_u32(&task->q) + (from_ring || !segs ? 0u : segs[sseg].q_base));
         const DPlan& P = plans[q];
         const uint64_t* __restrict__ row_off = P.row_off;
         const uint32_t* __restrict__ idx = P.idx;
         const uint32_t* __restrict__ cand_pool = P.cand_pool;
         uint32_t drop = 0;
+        unsigned long long qdl = 0;   // lane 0: the query's own deadline (resident launch)
         if (lane == 0) {
             drop = load_relaxed(&Q->timeout) || __atomic_load_n(&qc[q].done, __ATOMIC_RELAXED);
             if (!drop && past_deadline(Q)) {
                 atomicExch(&Q->timeout, 1u);
+                drop = 1;
+            }
+            if (P.limit_ticks) qdl = query_deadline(qc, q, P.limit_ticks);
+            if (!drop && passed(qdl)) {
+                post_timeout(notify, qc, q);
                 drop = 1;
             }
             if (drop) {
@@ -2794,6 +2888,11 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                         atomicExch(&Q->timeout, 1u);
                         stop = 1;
                     }
+                    if (!stop && passed(qdl)) {
+                        // the query's own time limit: post what it has counted
+                        post_timeout(notify, qc, q);
+                        stop = 1;
+                    }
                 }
                 lcount = lcalls = lrej = 0;
                 if (__shfl(stop, 0)) break;
@@ -2803,8 +2902,8 @@ __global__ __launch_bounds__(kWave) void k_dfs_wave(
                 uint32_t hungry = 0;
                 if (lane == 0) {
                     const uint32_t h0 = load_relaxed(&Q->head0);
-                    hungry = load_relaxed(&Q->tail) - load_relaxed(&Q->head) + (h0 < n0 ? n0 - h0 : 0u) <
-                             hungry_below;
+                    const uint32_t left0 = segs ? segment_left(Q, segs, seg_head) : (h0 < n0 ? n0 - h0 : 0u);
+                    hungry = load_relaxed(&Q->tail) - load_relaxed(&Q->head) + left0 < hungry_below;
                 }
                 if (__shfl(hungry, 0)) {
                     const uint32_t e = t + lane;

@@ -175,7 +175,9 @@ struct DQueue {
     // launch (no further segments). A segment's descriptor, plans, operands,
     // counters and pending counts are written before it is published, into
     // memory the launch has not read. The launch ends once it is closed and
-    // drained, or after idle_ticks drained with nothing new published.
+    // drained, or after idle_ticks drained with nothing new published (then
+    // as a whole: a segment published meanwhile is left untouched, for the
+    // host to publish again in its next launch).
     const struct DSeg* segs;
     unsigned int* seg_head;        // next task of each segment (zero at the launch)
     const unsigned int* host_word;
@@ -183,6 +185,8 @@ struct DQueue {
     unsigned int seg_cur;          // segments before this one are known to be exhausted
     unsigned long long poll_tick;  // wall clock of the latest read of host_word
     unsigned long long idle_ticks; // 0: leave only once closed
+    unsigned int exiting;          // set by the first wavefront to leave after idle_ticks: no
+                                   // segment is claimed after it, every wavefront leaves
 };
 
 // One published segment of a resident launch: a prepared batch's

@@ -2690,7 +2690,12 @@ static_assert(sizeof(DPlan) * kSlotAlign % 256 == 0 && sizeof(QCounters) * kSlot
 static_assert(sizeof(DSeg) == 128, "one segment descriptor per 128-byte line");
 constexpr double kEngIdleExitMs = 500.0;   // a drained launch the host has not closed leaves after this
 constexpr double kEngIdleCloseMs = 2.0;    // the host closes a launch that has had nothing to run this long
-constexpr size_t kPrepLdsBytes = 48 * 1024;   // LDS per CU left to the preparation kernels
+constexpr double kResidentShare = 8.0;         // a batch is pre-expanded for 1/8 of the resident grid
+// The wavefront-per-search kernel has a resident mode (k_dfs_wave with
+// DQueue::segs) that the server does not use: a served test of it once left
+// the GPU faulted and the cause has not been found, so resident launches
+// always run the lane kernel.
+constexpr bool kResidentWave = false;
 
 // A batch prepared for the resident launch.
 struct Prepared {
@@ -2705,7 +2710,8 @@ struct Prepared {
     std::vector<QCounters> qc0;   // counters of the pre-expansion (calls, bytes)
     std::vector<uint64_t> ntask;
     DBuf tasks, cand_pool, row_off, idx;
-    uint64_t n_tasks = 0;
+    uint64_t task_off = 0;          // tasks before this position ran in an earlier launch
+    uint64_t n_tasks = 0;           // tasks from task_off on
     uint32_t q_base = 0, seg = 0;   // placement in the running launch
     std::vector<uint8_t> delivered;
 };
@@ -2715,6 +2721,7 @@ struct Engine {
     psm_ctx* ctx = nullptr;         // launch stream and launch shape
     hipStream_t us = nullptr;       // fills and uploads beside the running launch
     uint32_t grid_waves = 0, ring = 0;
+    bool wave = false;              // wavefront-per-search kernel (else lane-per-search)
     DBuf d_plans, d_ops, d_qc, d_pend, d_segs, d_heads, d_queue, d_seq, d_q;
     unsigned int* hword = nullptr;  // coherent pinned: segments published | closed << 31
     unsigned int* d_hword = nullptr;
@@ -2730,6 +2737,12 @@ struct Engine {
     uint64_t held = 0;              // device bytes the running launch's batches hold
     uint64_t launches = 0;
     ~Engine() {
+        // a launch still running reads the pinned words and the arrays below:
+        // close it and let it drain first
+        if (running && hword) {
+            __atomic_store_n(hword, n_segs | 0x80000000u, __ATOMIC_RELEASE);
+            if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        }
         d_plans.release(); d_ops.release(); d_qc.release(); d_pend.release(); d_segs.release();
         d_heads.release(); d_queue.release(); d_seq.release(); d_q.release();
         if (hword) (void)hipHostFree(hword);
@@ -2819,8 +2832,13 @@ int prepare_resident(psm_server* S, Batch& b, Prepared& p) {
     HIP_TRY(ctx, upload(b.ops, p.hops, b.s()));
     HIP_TRY(ctx, b.qcnt.alloc(sizeof(QCounters) * nq));
     HIP_TRY(ctx, hipMemsetAsync(b.qcnt.p, 0, sizeof(QCounters) * nq, b.s()));
+    // a batch shares the resident grid with the batches around it: the
+    // lane kernel's batches are expanded for a share of its lanes, the
+    // wavefront kernel's for its wavefronts
+    const bool wave = S->eng->wave;
+    const double workers = (double)S->eng->grid_waves * (wave ? 1 : 64);
     const uint64_t target_total = std::max<uint64_t>(
-        64, (uint64_t)((double)S->eng->grid_waves * 64 * ctx->expand_tasks_per_lane));
+        64, (uint64_t)(workers * ctx->expand_tasks_per_lane / (wave ? 1.0 : kResidentShare)));
     PSM_TRY(b.expand_tasks(p.hp, target_total, p.tasks, p.n_tasks, p.ntask));
     HIP_TRY(ctx, download(p.qc0, b.qcnt.p, nq, b.s()));
     p.cand_pool.swap(b.cand_pool);
@@ -2992,12 +3010,15 @@ int engine_setup(psm_server* S, Engine& E, const psm_ctx* proto) {
     HIP_TRY(c, hipHostGetDevicePointer((void**)&E.d_hword, E.hword, 0));
     HIP_TRY(c, hipHostMalloc((void**)&E.notify, (size_t)kEngSlots * 32, hipHostMallocCoherent));
     HIP_TRY(c, hipHostGetDevicePointer((void**)&E.d_notify, E.notify, 0));
-    // wavefronts per CU: the served shape's, within what leaves the
-    // preparation kernels of later batches room in every CU's LDS
-    const size_t lds = dfs_lane_lds_bytes(kEngMaxN);
-    uint32_t waves = std::max<uint32_t>(1, proto->dfs_waves_per_cu);
-    while (waves > 1 && waves * lds + kPrepLdsBytes > 160 * 1024) --waves;
-    E.grid_waves = std::max<uint32_t>(4, (uint32_t)c->num_cu * waves / 4 * 4);
+    // The lane kernel, one wavefront per CU. The resident launch shares
+    // every CU with the kernels that prepare the next batches, and they are
+    // most of a served request's device time: a further resident wavefront
+    // per CU holds LDS (35 KB per wavefront of 32-vertex searches) and
+    // registers they need for as long as the launch runs, and with two per
+    // CU C4 was served at half the rate of one per CU. (The wavefront
+    // kernel's resident mode is not used: see kResidentWave.)
+    E.wave = kResidentWave && proto->dfs_strategy == PSM_DFS_WAVE;
+    E.grid_waves = std::max<uint32_t>(4, (uint32_t)c->num_cu / 4 * 4);
     const uint64_t lanes_all = (uint64_t)E.grid_waves * 64;
     const uint64_t room = c->queue_slots ? c->queue_slots : std::max<uint64_t>(1u << 18, 4 * lanes_all);
     uint64_t ring = 64;
@@ -3022,6 +3043,12 @@ int engine_setup(psm_server* S, Engine& E, const psm_ctx* proto) {
     return PSM_OK;
 }
 
+void engine_close(Engine& E) {
+    if (!E.running || E.closed) return;
+    __atomic_store_n(E.hword, E.n_segs | 0x80000000u, __ATOMIC_RELEASE);
+    E.closed = true;
+}
+
 // A new launch: counters, pending counts and segment heads cleared, the
 // ring reset, nothing published; then the persistent kernel.
 int engine_start(Engine& E) {
@@ -3043,16 +3070,34 @@ int engine_start(Engine& E) {
     __atomic_store_n(E.hword, 0u, __ATOMIC_SEQ_CST);
     memset(E.notify, 0, (size_t)kEngSlots * 32);
     HIP_TRY(c, hipStreamSynchronize(E.us));
-    HIP_TRY(c, launch_dfs(E.d_plans.as<DPlan>(), E.d_ops.as<DOp>(), nullptr, E.d_queue.as<DTask>(),
-                          E.d_seq.as<uint32_t>(), E.d_q.as<DQueue>(), E.d_qc.as<QCounters>(),
-                          E.d_pend.as<uint32_t>(), kEngMaxN, E.grid_waves, 0, EmitSink{nullptr, nullptr, 0, nullptr},
-                          c->stream));
-    HIP_TRY(c, hipEventRecord(E.ev, c->stream));
+    // Whatever the launch call returns, the kernel may have started: it is
+    // tracked (its end event recorded) from here on, and a failed start is
+    // closed at once, so nothing it reads is released before it has left.
+    (void)hipGetLastError();   // the launch wrappers report hipGetLastError()
+    hipError_t le;
+    if (E.wave)
+        le = launch_dfs_wave(E.d_plans.as<DPlan>(), E.d_ops.as<DOp>(), nullptr, E.d_queue.as<DTask>(),
+                             E.d_seq.as<uint32_t>(), E.d_q.as<DQueue>(), E.d_qc.as<QCounters>(),
+                             E.d_pend.as<uint32_t>(), kEngMaxN, E.grid_waves, c->stream);
+    else
+        le = launch_dfs(E.d_plans.as<DPlan>(), E.d_ops.as<DOp>(), nullptr, E.d_queue.as<DTask>(),
+                        E.d_seq.as<uint32_t>(), E.d_q.as<DQueue>(), E.d_qc.as<QCounters>(),
+                        E.d_pend.as<uint32_t>(), kEngMaxN, E.grid_waves, 0, EmitSink{nullptr, nullptr, 0, nullptr},
+                        c->stream);
     E.running = true;
     E.closed = false;
     E.next_slot = E.next_op = E.n_segs = 0;
     E.idle_since = -1;
     ++E.launches;
+    const hipError_t re = hipEventRecord(E.ev, c->stream);
+    if (le != hipSuccess || re != hipSuccess) {
+        engine_close(E);
+        if (re != hipSuccess) {   // no end event to wait for: wait for the stream itself
+            (void)hipStreamSynchronize(c->stream);
+            E.running = false;
+        }
+        return hip_fail(c, le != hipSuccess ? le : re, "resident launch start", __LINE__);
+    }
     return PSM_OK;
 }
 
@@ -3060,12 +3105,6 @@ bool engine_fits(const Engine& E, const Prepared& b) {
     const uint64_t q_base = (E.next_slot + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
     const uint64_t op_base = (E.next_op + kOpAlign - 1) / kOpAlign * kOpAlign;
     return E.n_segs < kEngSegs && q_base + b.reqs.size() <= kEngSlots && op_base + b.hops.size() <= kEngOps;
-}
-
-void engine_close(Engine& E) {
-    if (!E.running || E.closed) return;
-    __atomic_store_n(E.hword, E.n_segs | 0x80000000u, __ATOMIC_RELEASE);
-    E.closed = true;
 }
 
 // Writes a prepared batch's plans, operands, counters, pending counts and
@@ -3090,7 +3129,7 @@ int engine_publish(psm_server* S, Engine& E, std::unique_ptr<Prepared>& bp) {
     std::vector<uint32_t> pend(nq);
     for (uint32_t i = 0; i < nq; ++i) pend[i] = (uint32_t)b.ntask[i];
     DSeg seg{};
-    seg.tasks = b.tasks.as<DTask>();
+    seg.tasks = b.tasks.as<DTask>() + b.task_off;
     seg.n = (uint32_t)b.n_tasks;
     seg.q_base = q_base;
     HIP_TRY(c, hipMemcpyAsync(E.d_plans.as<DPlan>() + q_base, hp.data(), sizeof(DPlan) * nq,
@@ -3108,13 +3147,15 @@ int engine_publish(psm_server* S, Engine& E, std::unique_ptr<Prepared>& bp) {
     b.q_base = q_base;
     b.seg = E.n_segs;
     {
-        // a query without tasks (or without candidates) is final now
+        // a query without tasks left (or without candidates) is final now
         std::lock_guard<std::mutex> lk(S->mu);
         for (uint32_t i = 0; i < nq; ++i) {
             E.owner[q_base + i] = {&b, i};
             const QueryHost* q = b.reqs[i].q.get();
+            if (b.delivered[i]) continue;
             if (b.ntask[i] == 0 || q->status != PSM_OK)
-                deliver_prepared(S, b, i, 0, b.qc0[i].calls, q->status);
+                deliver_prepared(S, b, i, b.qc0[i].count, b.qc0[i].calls,
+                                 q->status != PSM_OK ? q->status : (b.qc0[i].timed_out ? PSM_ETIMEOUT : PSM_OK));
             else
                 E.open.push_back(q_base + i);
         }
@@ -3153,11 +3194,14 @@ void engine_poll(psm_server* S, Engine& E) {
     }
 }
 
-// The launch has ended: results it did not post come from its counters; a
-// batch published after its last wavefront left goes back to the queue for
-// the next launch; the memory its batches held is recycled.
+// The launch has ended: results it did not post come from its counters. A
+// batch whose tasks it did not all claim (published after its wavefronts
+// began to leave) goes back to the queue with what is left, its queries'
+// counters and pending counts carried over, for the next launch. The memory
+// of the other batches is recycled.
 int engine_finish(psm_server* S, Engine& E, std::deque<std::unique_ptr<Prepared>>& again, int failed) {
     psm_ctx* c = E.ctx;
+    engine_poll(S, E);   // results posted before the end (their words are cleared at the next start)
     std::vector<uint32_t> heads, pend;
     std::vector<QCounters> qc;
     int rc = failed;
@@ -3172,8 +3216,15 @@ int engine_finish(psm_server* S, Engine& E, std::deque<std::unique_ptr<Prepared>
         std::lock_guard<std::mutex> lk(S->mu);
         for (auto& bp : E.batches) {
             Prepared& b = *bp;
-            if (rc == PSM_OK && b.n_tasks && heads[b.seg] == 0) {
-                again.push_back(std::move(bp));   // never started: run it in the next launch
+            if (rc == PSM_OK && b.n_tasks && heads[b.seg] < b.n_tasks) {
+                const uint32_t nq = (uint32_t)b.reqs.size();
+                b.task_off += heads[b.seg];
+                b.n_tasks -= heads[b.seg];
+                for (uint32_t i = 0; i < nq; ++i) {
+                    b.qc0[i] = qc[b.q_base + i];
+                    b.ntask[i] = pend[b.q_base + i];
+                }
+                again.push_back(std::move(bp));
                 continue;
             }
             for (uint32_t i = 0; i < (uint32_t)b.reqs.size(); ++i) {
@@ -3184,7 +3235,7 @@ int engine_finish(psm_server* S, Engine& E, std::deque<std::unique_ptr<Prepared>
                 }
                 const uint32_t slot = b.q_base + i;
                 const QCounters& k = qc[slot];
-                const bool partial = k.timed_out || (pend[slot] && !k.done) || heads[b.seg] < b.n_tasks;
+                const bool partial = k.timed_out || (pend[slot] && !k.done);
                 deliver_prepared(S, b, i, k.count, k.calls, partial ? PSM_ETIMEOUT : PSM_OK);
             }
             freed += b.held;
@@ -3275,7 +3326,12 @@ void engine_loop(psm_server* S) {
         if (qe == hipSuccess) {
             engine_finish(S, E, again, PSM_OK);
         } else if (qe != hipErrorNotReady) {
-            engine_finish(S, E, again, hip_fail(E.ctx, qe, "resident launch", __LINE__));
+            // the launch is over only once its kernel has left: close it and
+            // wait, so nothing it reads is released under it
+            const int rc = hip_fail(E.ctx, qe, "resident launch", __LINE__);
+            engine_close(E);
+            (void)hipStreamSynchronize(E.ctx->stream);
+            engine_finish(S, E, again, rc);
         } else {
             std::this_thread::sleep_for(std::chrono::microseconds(50));
         }

@@ -94,9 +94,16 @@ def run_poisson(server: Server, queries, rate_qps: float, n_requests: int, seed:
             results[int(r.request_id)] = r
     wall = time.perf_counter() - t0
     lat = np.array([results[k].done_ms - results[k].submit_ms for k in range(n_requests)])
+    fifth = max(1, n_requests // 5)
+    # offered_qps: the Poisson stream's own rate (requests / last arrival);
+    # a server that keeps up has the same latency in the last fifth of the
+    # stream as in the first, one that falls behind a growing one
     return results, dict(rate_qps=rate_qps, requests=n_requests, wall_s=round(wall, 3),
+                         offered_qps=round(n_requests / float(arrivals[-1]), 2),
                          sustained_qps=round(n_requests / wall, 2),
                          p50_ms=round(float(np.percentile(lat, 50)), 3),
                          p99_ms=round(float(np.percentile(lat, 99)), 3),
+                         p50_first_fifth_ms=round(float(np.percentile(lat[:fifth], 50)), 3),
+                         p50_last_fifth_ms=round(float(np.percentile(lat[-fifth:], 50)), 3),
                          mean_batch=round(float(np.mean([results[k].batch_size for k in range(n_requests)])), 2),
                          timeouts=int(sum(1 for k in range(n_requests) if results[k].status == L.PSM_ETIMEOUT)))

@@ -425,13 +425,27 @@ def test_time_limit_reports_partial_count(hprd):
     assert match(dg, q, "GQL_GQL_GQL").count == 1015922
 
 
+def _resident_lane_only(dev):
+    """Resident launches always run the lane kernel (the wavefront kernel's
+    resident mode is switched off: a served test of it once left the GPU
+    faulted, cause not found), so the wavefront parametrization would only
+    repeat the lane one."""
+    from psm_benchmark_amd import _lib as L
+    v = ctypes.c_double()
+    L.lib().psm_get_tuning(dev.h, L.PSM_TUNE_DFS_STRATEGY, ctypes.byref(v))
+    if int(v.value) == L.PSM_DFS_WAVE:
+        pytest.skip("resident launches run the lane kernel; covered by the lane parametrization")
+
+
 @pytest.mark.parametrize("resident", [True, False], ids=["resident", "per_batch"])
-def test_served_results_equal_alone_and_in_the_mix(hprd, resident):
+def test_served_results_equal_alone_and_in_the_mix(dev, hprd, resident):
     """Requests served through continuous batching (several workers, mixed
     arrivals; batches admitted into one resident launch, or one launch per
     batch) give each query exactly its count and Call Count when matched
     alone."""
     from psm_benchmark_amd.serving import Server, run_poisson
+    if resident:
+        _resident_lane_only(dev)
     g, dg, _ = hprd
     recs = golden(("GQL_GQL_GQL",))
     qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
@@ -448,11 +462,12 @@ def test_served_results_equal_alone_and_in_the_mix(hprd, resident):
     assert max(r.batch_size for r in results.values()) >= 1
 
 
-def test_resident_launch_reopens_and_takes_single_request_batches(hprd):
+def test_resident_launch_reopens_and_takes_single_request_batches(dev, hprd):
     """One request per batch, in two waves with a pause between them: the
     first launch closes once it has nothing to run, the second wave starts a
     new one; every count and Call Count is the golden one."""
     from psm_benchmark_amd.serving import Server
+    _resident_lane_only(dev)
     g, dg, _ = hprd
     recs = golden(("GQL_GQL_GQL",))
     qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]
@@ -476,13 +491,14 @@ def test_resident_launch_reopens_and_takes_single_request_batches(hprd):
         assert r.calls == recs[k % len(recs)]["calls"], k
 
 
-def test_resident_launch_time_limit_gives_partial_counts(hprd):
+def test_resident_launch_time_limit_gives_partial_counts(dev, hprd):
     """A time limit the heavy query cannot meet: the resident launch drops
     each query's work at its own deadline and returns its partial count with
     PSM_ETIMEOUT (the reference's Overtime: 1); with a generous limit the
     same requests are exact."""
     from psm_benchmark_amd import _lib as L
     from psm_benchmark_amd.serving import Server
+    _resident_lane_only(dev)
     g, dg, _ = hprd
     recs = golden(("GQL_GQL_GQL",))
     qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs]

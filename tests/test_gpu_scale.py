@@ -197,15 +197,21 @@ def test_c4_served_equals_alone(dev, c3):
         r = match(dg, q, "GQL_GQL_GQL", cap=cap)
         assert not r.timed_out
         alone.append(r.count)
-    srv = Server(dg, workers=4, max_batch=32, filter="GQL", order="GQL", cap=cap, time_limit=SAFETY_S)
-    try:
-        res, summary = run_poisson(srv, qs, rate_qps=1000.0, n_requests=192, seed=4)
-    finally:
-        srv.close()
-    _say("C4", summary)
-    for k, r in res.items():
-        assert r.status == L.PSM_OK, k
-        assert r.count == alone[k % len(qs)], k
+    # served through the resident launch (lane kernel) and through one
+    # launch per batch
+    for strategy, resident in (("lane", True), ("wave", False)):
+        _set(dev, strategy)
+        srv = Server(dg, workers=4, max_batch=32, filter="GQL", order="GQL", cap=cap, time_limit=SAFETY_S,
+                     resident=resident)
+        try:
+            res, summary = run_poisson(srv, qs, rate_qps=1000.0, n_requests=192, seed=4)
+        finally:
+            srv.close()
+        _say("C4", strategy, "resident" if resident else "per batch", summary)
+        for k, r in res.items():
+            assert r.status == L.PSM_OK, (strategy, resident, k)
+            assert r.count == alone[k % len(qs)], (strategy, resident, k)
+    _set(dev, "lane")
 
 
 # ---- C5 ----------------------------------------------------------------------
