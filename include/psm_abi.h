@@ -349,9 +349,11 @@ int psm_batch_destroy(psm_batch* b);
 /* ---- serving form ----------------------------------------------------------
  * Continuous batching over a resident data graph: `workers` threads, each with
  * its own HIP stream, admit the oldest pending requests as one batch while the
- * batch holds fewer than max_batch requests and the estimated device
- * footprint of running batches stays within mem_budget (0: half the free
- * device memory), and prepare it (filter, tables, plan, task pre-expansion).
+ * batch holds fewer than max_batch requests and the device footprint of
+ * running batches stays within mem_budget (0: half the free device memory;
+ * a request counts a bound of its bitmaps, membership words and candidate
+ * sets until its tables are built, then their real size), and prepare it
+ * (filter, tables, plan, task pre-expansion).
  * A prepared batch is published into the resident enumeration launch: one
  * persistent kernel that keeps running while batches join it, so each
  * batch's search starts when its preparation ends, beside earlier batches'

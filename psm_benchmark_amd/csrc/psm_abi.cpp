@@ -2736,6 +2736,8 @@ struct Engine {
     double idle_since = -1;
     uint64_t held = 0;              // device bytes the running launch's batches hold
     uint64_t launches = 0;
+    double t_start = 0;             // host clock at the running launch's start
+    const char* close_reason = "";
     ~Engine() {
         // a launch still running reads the pinned words and the arrays below:
         // close it and let it drain first
@@ -2781,16 +2783,18 @@ struct psm_server {
 
 namespace {
 
-// Upper bound of a request's device footprint before it runs: candidate
-// sets (label buckets), table rows, bitmaps + ranks of its query vertices.
+// A request's device footprint up to its tables, bounded before it runs:
+// bitmaps + ranks of its query vertices, its GQL membership words, and its
+// candidate sets (label buckets, with their compaction scratch). Its tables
+// are charged at their real size once built: bounded by the buckets and
+// the degrees they are 10-25x larger than what C4's batches build.
 uint64_t footprint(const psm_graph* g, const QGraph& q) {
     const uint64_t wps = ((uint64_t)g->V + 31) / 32;
-    // bitmaps + ranks of its query vertices, and its GQL membership words
     uint64_t b = (uint64_t)q.n * wps * 8 + (uint64_t)g->V * 8;
     for (uint32_t u = 0; u < q.n; ++u) {
         const uint32_t l = q.lab[u];
         const uint64_t bucket = l < g->L ? g->h_ri_off[l + 1] - g->h_ri_off[l] : 0;
-        b += bucket * 4 * (2 + q.degree(u)) + bucket * 8 * q.degree(u);
+        b += bucket * 4 * 2;
     }
     return b + (64ull << 20);
 }
@@ -2919,6 +2923,7 @@ void serve_worker(psm_server* S, psm_ctx* ctx) {
                 std::lock_guard<std::mutex> lk(S->mu);
                 // the estimate gives way to what the launch will hold
                 S->mem_in_use = S->mem_in_use - est + p->held;
+                S->peak_mem = std::max(S->peak_mem, S->mem_in_use);
                 --S->preparing;
                 S->ready.push_back(std::move(p));
             }
@@ -2973,6 +2978,12 @@ void serve_worker(psm_server* S, psm_ctx* ctx) {
         }
         int rc = b.filter(S->filter);
         if (rc == PSM_OK) rc = b.build_tables();
+        const uint64_t tables = b.row_off.bytes + b.idx.bytes;   // charged at their real size
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->mem_in_use += tables;
+            S->peak_mem = std::max(S->peak_mem, S->mem_in_use);
+        }
         if (rc == PSM_OK) rc = b.make_plans(S->order);
         if (rc == PSM_OK) rc = b.evaluate(S->cap);
         b.poll = nullptr;
@@ -2986,7 +2997,7 @@ void serve_worker(psm_server* S, psm_ctx* ctx) {
             deliver(i, q->count, q->calls, rc != PSM_OK ? rc : (q->timed_out ? PSM_ETIMEOUT : q->status));
         }
         // the batch's device buffers (candidates, tables, queue) are freed now
-        S->mem_in_use -= est;
+        S->mem_in_use -= est + tables;
         --S->preparing;
         S->cv_done.notify_all();
         S->cv_eng.notify_one();
@@ -2999,7 +3010,10 @@ int engine_setup(psm_server* S, Engine& E, const psm_ctx* proto) {
     PSM_TRY(psm_ctx_create(S->device, &E.ctx));
     psm_ctx* c = E.ctx;
     c->claim_div = proto->claim_div;
-    c->intersect = proto->intersect;
+    // the lane kernel's own intersection (galloping cursors): a tuned
+    // PSM_INTERSECT_BSEARCH is chosen for the wavefront kernel, and served C4
+    // through the resident lane kernel at two thirds of the rate
+    c->intersect = PSM_INTERSECT_ADAPTIVE;
     c->queue_slots = proto->queue_slots;
     c->block_lanes = 0;
     c->verbose = proto->verbose;
@@ -3043,10 +3057,11 @@ int engine_setup(psm_server* S, Engine& E, const psm_ctx* proto) {
     return PSM_OK;
 }
 
-void engine_close(Engine& E) {
+void engine_close(Engine& E, const char* reason) {
     if (!E.running || E.closed) return;
     __atomic_store_n(E.hword, E.n_segs | 0x80000000u, __ATOMIC_RELEASE);
     E.closed = true;
+    E.close_reason = reason;
 }
 
 // A new launch: counters, pending counts and segment heads cleared, the
@@ -3090,8 +3105,9 @@ int engine_start(Engine& E) {
     E.idle_since = -1;
     ++E.launches;
     const hipError_t re = hipEventRecord(E.ev, c->stream);
+    E.t_start = now_ms();
     if (le != hipSuccess || re != hipSuccess) {
-        engine_close(E);
+        engine_close(E, "start failed");
         if (re != hipSuccess) {   // no end event to wait for: wait for the stream itself
             (void)hipStreamSynchronize(c->stream);
             E.running = false;
@@ -3212,6 +3228,10 @@ int engine_finish(psm_server* S, Engine& E, std::deque<std::unique_ptr<Prepared>
         if (e != hipSuccess) rc = hip_fail(c, e, "resident launch read-back", __LINE__);
     }
     uint64_t freed = 0;
+    if (c->verbose)
+        fprintf(stderr, "[psm] resident launch %llu: %.1f ms, %u segments, %u slots, %.1f MB held, closed: %s\n",
+                (unsigned long long)E.launches, now_ms() - E.t_start, E.n_segs, E.next_slot, E.held / 1e6,
+                E.closed ? E.close_reason : "no (left idle)");
     {
         std::lock_guard<std::mutex> lk(S->mu);
         for (auto& bp : E.batches) {
@@ -3258,7 +3278,7 @@ void engine_loop(psm_server* S) {
     std::deque<std::unique_ptr<Prepared>> again;   // batches for the next launch
     while (true) {
         std::deque<std::unique_ptr<Prepared>> got;
-        bool stopping = false, others_idle = false;
+        bool stopping = false;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             if (!E.running && again.empty())
@@ -3267,7 +3287,6 @@ void engine_loop(psm_server* S) {
                 });
             got.swap(S->ready);
             stopping = S->stop && S->pending.empty() && S->preparing == 0;
-            others_idle = S->preparing == 0;
         }
         while (!got.empty()) {
             again.push_back(std::move(got.front()));
@@ -3288,7 +3307,7 @@ void engine_loop(psm_server* S) {
                 continue;
             }
             if (E.running && (E.closed || !engine_fits(E, *b))) {
-                engine_close(E);   // the next launch takes it
+                engine_close(E, "slots");   // the next launch takes it
                 break;
             }
             int rc = E.running ? PSM_OK : engine_start(E);
@@ -3305,22 +3324,25 @@ void engine_loop(psm_server* S) {
         }
         if (!E.running) continue;
         engine_poll(S, E);
-        // close the launch when it has had nothing to run for a while (its
-        // memory is then recycled), at once when stopping, or when the
-        // memory its batches hold reaches half the budget
+        // close the launch when it has had nothing to run for 2 ms, even
+        // while workers prepare further batches (a long preparation would
+        // otherwise run beside an idle launch that holds LDS and registers
+        // its kernels need; the next batch starts a new launch, and the
+        // memory of this one is recycled), at once when stopping, or when
+        // the memory its batches hold reaches 60 % of the budget
         const double now = now_ms();
         if (!E.closed) {
-            const bool idle = E.open.empty() && again.empty() && others_idle;
+            const bool idle = E.open.empty() && again.empty();
             if (!idle) {
                 E.idle_since = -1;
             } else if (stopping) {
-                engine_close(E);
+                engine_close(E, "stopping");
             } else if (E.idle_since < 0) {
                 E.idle_since = now;
             } else if (now - E.idle_since > kEngIdleCloseMs) {
-                engine_close(E);
+                engine_close(E, "idle");
             }
-            if (E.held > S->mem_budget / 2) engine_close(E);
+            if ((double)E.held > 0.6 * (double)S->mem_budget) engine_close(E, "memory");
         }
         const hipError_t qe = hipEventQuery(E.ev);
         if (qe == hipSuccess) {
@@ -3329,7 +3351,7 @@ void engine_loop(psm_server* S) {
             // the launch is over only once its kernel has left: close it and
             // wait, so nothing it reads is released under it
             const int rc = hip_fail(E.ctx, qe, "resident launch", __LINE__);
-            engine_close(E);
+            engine_close(E, "launch error");
             (void)hipStreamSynchronize(E.ctx->stream);
             engine_finish(S, E, again, rc);
         } else {
