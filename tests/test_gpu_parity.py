@@ -529,6 +529,32 @@ def test_resident_launch_time_limit_gives_partial_counts(dev, hprd):
             assert results[len(qs) - 1].status == L.PSM_ETIMEOUT
 
 
+def test_resident_server_runs_large_queries_beside_the_resident_launch(dev, hprd):
+    """A batch holding a query above the resident launch's 32 vertices runs
+    its own enumeration launch beside the resident one: with small and
+    40-vertex queries mixed in one stream, every served count equals the
+    query's count matched alone (cap 1,000)."""
+    from psm_benchmark_amd import gen
+    from psm_benchmark_amd.matcher import match
+    from psm_benchmark_amd.serving import Server, run_poisson
+    _resident_lane_only(dev)
+    g, dg, _ = hprd
+    recs = golden(("GQL_GQL_GQL",))
+    qs = [load_text(os.path.join(HPRD, r["query"] + ".txt")) for r in recs[:4]]
+    qs += [gen.query(g, 40, False, seed) for seed in (3, 5)]
+    cap = 1000
+    alone = [match(dg, q, "GQL_GQL_GQL", cap=cap).count for q in qs]
+    srv = Server(dg, workers=3, max_batch=3, filter="GQL", order="GQL", cap=cap, resident=True)
+    try:
+        results, summary = run_poisson(srv, qs, rate_qps=300.0, n_requests=36, seed=5)
+    finally:
+        srv.close()
+    assert max(q.num_vertices for q in qs) > 32
+    for k, r in results.items():
+        assert r.status == 0, k
+        assert r.count == alone[k % len(qs)], k
+
+
 def test_served_dpiso_results_equal_golden(hprd):
     """The serving loop with the DPiso filter and order: every request's count
     equals the reference's DPiso_DPiso_LFTJ log, and its assignment count the
