@@ -174,7 +174,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="C2: leave out the C4 serving leg")
     ap.add_argument("--serving-plain", type=int, default=128, help="C2 line's C4 leg: plain-path queries")
     ap.add_argument("--serving-shape", default="lane,1,2", help="C2 line's C4 leg: launch shape of the server")
-    ap.add_argument("--workers", type=int, default=4, help="C4: server workers (HIP streams)")
+    ap.add_argument("--workers", type=int, default=8, help="C4: server workers (HIP streams)")
     ap.add_argument("--max-batch", type=int, default=128, help="C4: requests per batch at most")
     ap.add_argument("--low-rate", type=float, default=10.0, help="C4: extra rate below capacity (QPS)")
     ap.add_argument("--low-requests", type=int, default=100, help="C4: requests at the extra rate")
