@@ -83,15 +83,14 @@ __device__ __forceinline__ uint32_t find_u64(const uint64_t* starts, uint32_t n,
 }
 
 // FilterVertices.cpp:786-803: the data vertex's neighbour-label counts
-// dominate the query vertex's, both held as label-sorted runs.
-__device__ bool nlf_dominates(const DevGraph& g, uint32_t v, const uint2* qr, uint32_t qn) {
-    uint32_t i = g.nlf_off[v];
-    const uint32_t ie = g.nlf_off[v + 1];
+// (runs[i, ie)) dominate the query vertex's, both held as label-sorted runs.
+__device__ bool nlf_dominates(const uint2* __restrict__ runs, uint32_t i, uint32_t ie, const uint2* qr,
+                              uint32_t qn) {
     for (uint32_t j = 0; j < qn; ++j) {
         const uint2 r = qr[j];
         bool found = false;
         while (i < ie) {
-            const uint2 d = g.nlf[i];
+            const uint2 d = runs[i];
             if (d.x < r.x) { ++i; continue; }
             if (d.x == r.x && d.y >= r.y) { found = true; ++i; }
             break;
@@ -139,11 +138,13 @@ __global__ __launch_bounds__(kBlock) void k_compact_flag(
         if (it.mode == kModeValid) {
             keep = src_pool[it.src_begin + e] != kInvalid;
         } else {
-            const uint32_t v = g.ri[it.src_begin + e];
+            // the bucket's degrees and runs are read in reverse-index order
+            const uint32_t j = it.src_begin + e;
             const QVert q = qv[it.qv];
-            keep = g.off[v + 1] - g.off[v] >= q.degree;
-            if (keep && it.mode == kModeCflGen) keep = cnt_pool[it.cnt_base + v] == it.target;
-            if (keep && it.mode != kModeLDF) keep = nlf_dominates(g, v, qnlf + q.nlf_begin, q.nlf_count);
+            keep = g.deg_ri[j] >= q.degree;
+            if (keep && it.mode == kModeCflGen) keep = cnt_pool[it.cnt_base + g.ri[j]] == it.target;
+            if (keep && it.mode != kModeLDF)
+                keep = nlf_dominates(g.nlf_ri, g.nlf_ri_off[j], g.nlf_ri_off[j + 1], qnlf + q.nlf_begin, q.nlf_count);
         }
     }
     const unsigned long long m = __ballot(keep);

@@ -31,6 +31,12 @@ struct DevGraph {
     const uint2* nlf;        // (label, count), sorted by label
     const uint32_t* adj_bl;  // nnz: neighbours of each vertex ordered by (label, id)
     const uint32_t* nlf_beg; // per run: its first position in adj_bl
+    // the degree and neighbour-label runs again, in reverse-index order
+    // (position j describes vertex ri[j]): a scan of a label bucket reads
+    // them as streams instead of one random line per vertex and field
+    const uint32_t* deg_ri;     // V
+    const uint32_t* nlf_ri_off; // V+1
+    const uint2* nlf_ri;        // the runs of ri[0], ri[1], ...
 };
 
 // One query vertex as the candidate filters see it.

@@ -304,7 +304,7 @@ struct psm_graph {
     uint32_t V = 0, L = 0, max_deg = 0, max_freq = 0;
     uint64_t nnz = 0;
     std::vector<uint32_t> h_ri_off;   // host copy: bucket sizes drive work items
-    DBuf off, adj, lab, ri_off, ri, nlf_off, nlf, adj_bl, nlf_beg;
+    DBuf off, adj, lab, ri_off, ri, nlf_off, nlf, adj_bl, nlf_beg, deg_ri, nlf_ri_off, nlf_ri;
     DevGraph dg{};
 };
 
@@ -364,6 +364,32 @@ static void build_graph_indexes(uint32_t V, const uint32_t* off, const uint32_t*
     nlf_beg.resize(nlf_off[V]);
     adj_bl.resize(off[V]);
     for (unsigned t = 0; t < nt; ++t) th.emplace_back(pass, 1, t);
+    for (auto& x : th) x.join();
+}
+
+// The degrees and neighbour-label runs in reverse-index order: position j
+// describes vertex ri[j] (what a label-bucket scan reads, as streams).
+static void build_ri_order(uint32_t V, const uint32_t* off, const std::vector<uint32_t>& ri,
+                           const std::vector<uint32_t>& nlf_off, const std::vector<uint2>& nlf,
+                           std::vector<uint32_t>& deg_ri, std::vector<uint32_t>& nlf_ri_off,
+                           std::vector<uint2>& nlf_ri) {
+    deg_ri.resize(V);
+    nlf_ri_off.assign(V + 1, 0);
+    for (uint32_t j = 0; j < V; ++j) {
+        const uint32_t v = ri[j];
+        deg_ri[j] = off[v + 1] - off[v];
+        nlf_ri_off[j + 1] = nlf_ri_off[j] + (nlf_off[v + 1] - nlf_off[v]);
+    }
+    nlf_ri.resize(nlf.size());
+    const unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            for (uint32_t j = t; j < V; j += nt) {
+                const uint32_t v = ri[j];
+                std::copy(nlf.begin() + nlf_off[v], nlf.begin() + nlf_off[v + 1], nlf_ri.begin() + nlf_ri_off[j]);
+            }
+        });
     for (auto& x : th) x.join();
 }
 
@@ -1963,6 +1989,9 @@ int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32
     std::vector<uint32_t> ri, nlf_off, adj_bl, nlf_beg;
     std::vector<uint2> nlf;
     build_graph_indexes(V, off, adj, lab, g->L, g->h_ri_off, ri, nlf_off, nlf, adj_bl, nlf_beg);
+    std::vector<uint32_t> deg_ri, nlf_ri_off;
+    std::vector<uint2> nlf_ri;
+    build_ri_order(V, off, ri, nlf_off, nlf, deg_ri, nlf_ri_off, nlf_ri);
     for (uint32_t l = 0; l < g->L; ++l)
         g->max_freq = std::max(g->max_freq, g->h_ri_off[l + 1] - g->h_ri_off[l]);
     hipStream_t s = ctx->stream;
@@ -1980,6 +2009,9 @@ int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32
     HIP_TRY(ctx, up(g->nlf, nlf.data(), nlf.size() * sizeof(uint2)));
     HIP_TRY(ctx, up(g->adj_bl, adj_bl.data(), adj_bl.size() * 4));
     HIP_TRY(ctx, up(g->nlf_beg, nlf_beg.data(), nlf_beg.size() * 4));
+    HIP_TRY(ctx, up(g->deg_ri, deg_ri.data(), deg_ri.size() * 4));
+    HIP_TRY(ctx, up(g->nlf_ri_off, nlf_ri_off.data(), nlf_ri_off.size() * 4));
+    HIP_TRY(ctx, up(g->nlf_ri, nlf_ri.data(), nlf_ri.size() * sizeof(uint2)));
     HIP_TRY(ctx, hipStreamSynchronize(s));
     DevGraph& d = g->dg;
     d.V = V;
@@ -1994,6 +2026,9 @@ int psm_graph_upload(psm_ctx* ctx, uint32_t V, const uint32_t* off, const uint32
     d.nlf = g->nlf.as<uint2>();
     d.adj_bl = g->adj_bl.as<uint32_t>();
     d.nlf_beg = g->nlf_beg.as<uint32_t>();
+    d.deg_ri = g->deg_ri.as<uint32_t>();
+    d.nlf_ri_off = g->nlf_ri_off.as<uint32_t>();
+    d.nlf_ri = g->nlf_ri.as<uint2>();
     *out = g.release();
     return PSM_OK;
 }
