@@ -199,8 +199,8 @@ def main():
         pmc_summary(*args.pmc_summary)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (queries shard "
-              "independently across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (batched and served "
+              "queries shard independently across GPUs, one copy of the data graph per GPU)", file=sys.stderr)
         sys.exit(2)
     if args.time_limit is None:
         args.time_limit = 0.0 if args.workload in ("C1", "C2") else 20.0

@@ -89,8 +89,8 @@ const char* psm_last_error(const psm_ctx* ctx);
  * reset: out3 = {launches, total milliseconds (HIP events on the context's
  * stream), algorithmic bytes counted by the kernels}. Classes: 0 GQL
  * refinement, 1 table count, 2 table fill, 3 enumeration DFS, 4 task
- * pre-expansion, 5 compaction, 6 bitmaps. Instrumentation of this build; the
- * reference logs phase times only (Args.cpp:517-535). */
+ * pre-expansion, 5 compaction, 6 bitmaps. Instrumentation of this library;
+ * the reference logs phase times only (Args.cpp:517-535). */
 int psm_kernel_stats(psm_ctx* ctx, int kind, double* out3);
 int psm_kernel_stats_reset(psm_ctx* ctx);
 
