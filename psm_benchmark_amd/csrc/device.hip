@@ -181,6 +181,89 @@ __global__ __launch_bounds__(kBlock) void k_compact_scatter(
     out[pos] = it.mode == kModeValid ? src_pool[it.src_begin + e] : src_ri[it.src_begin + e];
 }
 
+// Label-shared bucket scan, pass 1 (FilterVertices.cpp:19-65 LDF / NLF for
+// every (query, u) of a batch): a wavefront takes chunk r of its group's
+// bucket (lane l: element r * 64 + l) and subset s of the group's items
+// (lane k holds item s * 64 + k), and tests the chunk against each item of
+// the subset in turn: the element's degree and run bounds stay in
+// registers, its runs come from the L1 after the first item, the item's
+// fields come from lane k by readlane. Lane k stores item k's mask and
+// count at the end.
+static_assert(kBucketChunk == kWave, "a bucket chunk is one element per lane");
+__global__ __launch_bounds__(kBlock) void k_bucket_flag(
+    DevGraph g, const CGroup* __restrict__ groups, uint32_t n_groups, uint32_t n_waves,
+    const CBucketItem* __restrict__ bitems, const uint2* __restrict__ qnlf,
+    unsigned long long* __restrict__ masks, uint32_t* __restrict__ chunk_counts) {
+    const uint32_t lane = lane_id();
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave));
+    if (wv >= n_waves) return;   // whole wavefronts: no barrier below
+    const CGroup gr = groups[find_item(groups, n_groups, wv, &CGroup::wave_start)];
+    const uint32_t local = wv - gr.wave_start;
+    const uint32_t r = local / gr.n_sub, sub = local % gr.n_sub;
+    const uint32_t first = sub * kWave;
+    const uint32_t n = min(gr.n_items - first, (uint32_t)kWave);
+    CBucketItem mine{};
+    if (lane < n) mine = bitems[gr.item_begin + first + lane];
+    const uint32_t e = r * kWave + lane;
+    const bool in = e < gr.len;
+    const uint64_t j = gr.src_begin + e;
+    uint32_t deg = 0, o0 = 0, o1 = 0;
+    if (in) {
+        deg = g.deg_ri[j];
+        o0 = g.nlf_ri_off[j];
+        o1 = g.nlf_ri_off[j + 1];
+    }
+    unsigned long long my_m = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t qd = (uint32_t)__builtin_amdgcn_readlane((int)mine.degree, (int)k);
+        const uint32_t qb = (uint32_t)__builtin_amdgcn_readlane((int)mine.nlf_begin, (int)k);
+        const uint32_t qc = (uint32_t)__builtin_amdgcn_readlane((int)mine.nlf_count, (int)k);
+        bool keep = in && deg >= qd;
+        if (keep && !(qc >> 31)) keep = nlf_dominates(g.nlf_ri, o0, o1, qnlf + qb, qc);
+        const unsigned long long m = __ballot(keep);
+        if (lane == k) my_m = m;
+    }
+    if (lane < n) {
+        masks[mine.chunk_start + r] = my_m;
+        chunk_counts[mine.chunk_start + r] = (uint32_t)__popcll(my_m);
+    }
+}
+
+// Pass 2: each item's survivors of chunk r at the chunk's exclusive offset,
+// as vertex ids (the bucket's reverse index); lane k loads item k's mask and
+// offset, and the loop reads them by readlane.
+__global__ __launch_bounds__(kBlock) void k_bucket_scatter(
+    DevGraph g, const CGroup* __restrict__ groups, uint32_t n_groups, uint32_t n_waves,
+    const CBucketItem* __restrict__ bitems, const unsigned long long* __restrict__ masks,
+    const uint64_t* __restrict__ chunk_prefix, uint32_t* __restrict__ out) {
+    const uint32_t lane = lane_id();
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave));
+    if (wv >= n_waves) return;
+    const CGroup gr = groups[find_item(groups, n_groups, wv, &CGroup::wave_start)];
+    const uint32_t local = wv - gr.wave_start;
+    const uint32_t r = local / gr.n_sub, sub = local % gr.n_sub;
+    const uint32_t first = sub * kWave;
+    const uint32_t n = min(gr.n_items - first, (uint32_t)kWave);
+    unsigned long long my_m = 0;
+    uint64_t my_p = 0;
+    if (lane < n) {
+        const uint32_t c = bitems[gr.item_begin + first + lane].chunk_start + r;
+        my_m = masks[c];
+        my_p = chunk_prefix[c];
+    }
+    const uint32_t e = r * kWave + lane;
+    const uint32_t vid = e < gr.len ? g.ri[gr.src_begin + e] : 0u;
+    const unsigned long long below = lanemask_lt();
+    for (uint32_t k = 0; k < n; ++k) {
+        const unsigned long long m = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(my_m >> 32), (int)k) << 32) |
+                                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)my_m, (int)k);
+        if (!m) continue;
+        const uint64_t p = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(my_p >> 32), (int)k) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)my_p, (int)k);
+        if ((m >> lane) & 1ull) out[p + __popcll(m & below)] = vid;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Candidate bitmaps: one bit per data vertex per (query, query vertex) slot.
 // ---------------------------------------------------------------------------
@@ -3316,6 +3399,26 @@ hipError_t launch_compact_scatter(const CItem* items, uint32_t n_items, uint32_t
     if (n_blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_compact_scatter, dim3(n_blocks), dim3(kBlock), 0, s, items, n_items,
                        src_ri, src_pool, masks, block_prefix, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_flag(const DevGraph& g, const CGroup* groups, uint32_t n_groups, uint32_t n_waves,
+                              const CBucketItem* bitems, const uint2* qnlf, unsigned long long* masks,
+                              uint32_t* chunk_counts, hipStream_t s) {
+    if (n_waves == 0 || n_groups == 0) return hipSuccess;
+    const uint32_t waves_per_block = kBlock / kWave;
+    hipLaunchKernelGGL(k_bucket_flag, dim3((n_waves + waves_per_block - 1) / waves_per_block), dim3(kBlock), 0, s,
+                       g, groups, n_groups, n_waves, bitems, qnlf, masks, chunk_counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_scatter(const DevGraph& g, const CGroup* groups, uint32_t n_groups, uint32_t n_waves,
+                                 const CBucketItem* bitems, const unsigned long long* masks,
+                                 const uint64_t* chunk_prefix, uint32_t* out, hipStream_t s) {
+    if (n_waves == 0 || n_groups == 0) return hipSuccess;
+    const uint32_t waves_per_block = kBlock / kWave;
+    hipLaunchKernelGGL(k_bucket_scatter, dim3((n_waves + waves_per_block - 1) / waves_per_block), dim3(kBlock), 0,
+                       s, g, groups, n_groups, n_waves, bitems, masks, chunk_prefix, out);
     return hipGetLastError();
 }
 

@@ -67,6 +67,32 @@ struct CItem {
     uint32_t pad;
 };
 
+// Label-shared bucket scan (LDF / NLF over every (query, u) of a batch): the
+// items of one label all scan that label's bucket, so a wavefront takes one
+// 64-element chunk of the bucket and tests up to 64 items of the label
+// against it (a subset; the chunk's degrees and neighbour-label runs are
+// read once per subset instead of once per item). Wavefront w of a group
+// takes chunk w / n_sub and subset w % n_sub (the subsets of one chunk are
+// adjacent, so they share its lines in the caches). An item's keep mask and
+// count for chunk r of its bucket sit at chunk_start + r.
+constexpr uint32_t kBucketChunk = 64;   // bucket elements per wavefront (one per lane)
+struct CGroup {
+    uint64_t src_begin;    // first element of the bucket (reverse-index order)
+    uint32_t len;          // bucket length
+    uint32_t wave_start;   // first wavefront of this group
+    uint32_t item_begin;   // into the CBucketItem array
+    uint32_t n_items;
+    uint32_t n_sub;        // item subsets of 64: ceil(n_items / 64)
+    uint32_t pad;
+};
+// One (query, u) of a label-shared scan, as the scan needs it.
+struct CBucketItem {
+    uint32_t chunk_start;
+    uint32_t degree;       // d(u)
+    uint32_t nlf_begin;    // into the query NLF run array
+    uint32_t nlf_count;    // runs; bit 31 set: LDF (no NLF test)
+};
+
 // GQL refinement work item: the candidates of one query vertex u.
 // The candidate sets of one query are also held as membership words:
 // member[mem_base + w] has bit u set when data vertex w is in C(u).
@@ -273,6 +299,15 @@ hipError_t launch_compact_scatter(const CItem* items, uint32_t n_items, uint32_t
                                   const uint32_t* src_ri, const uint32_t* src_pool,
                                   const unsigned long long* masks, const uint64_t* block_prefix,
                                   uint32_t* out, hipStream_t s);
+// Label-shared bucket scan (CGroup): masks / chunk_counts per item chunk,
+// then the survivors' vertex ids written at chunk_prefix (exclusive scan of
+// chunk_counts). n_waves: the wavefronts of all groups.
+hipError_t launch_bucket_flag(const DevGraph& g, const CGroup* groups, uint32_t n_groups, uint32_t n_waves,
+                              const CBucketItem* bitems, const uint2* qnlf, unsigned long long* masks,
+                              uint32_t* chunk_counts, hipStream_t s);
+hipError_t launch_bucket_scatter(const DevGraph& g, const CGroup* groups, uint32_t n_groups, uint32_t n_waves,
+                                 const CBucketItem* bitems, const unsigned long long* masks,
+                                 const uint64_t* chunk_prefix, uint32_t* out, hipStream_t s);
 hipError_t launch_set_bits(const uint32_t* pool, const uint64_t* seg_begin, const uint32_t* seg_len,
                            const uint32_t* seg_slot, uint32_t n_seg, uint32_t total_max,
                            uint32_t* bitmaps, uint64_t words_per_slot, hipStream_t s);

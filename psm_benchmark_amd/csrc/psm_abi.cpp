@@ -500,12 +500,14 @@ struct Batch {
 // ---- candidate filters ------------------------------------------------------
 
 // NLF / LDF over label buckets for every (query, u) with a live query: one
-// compaction over all buckets. Candidate pool = concatenation of all sets.
+// label-shared scan over the buckets of the labels in use (CGroup).
+// Candidate pool = concatenation of all sets, in (query, u) order.
 int Batch::bucket_filter(bool use_nlf, std::vector<QueryHost*>& active) {
     std::vector<QVert> qv;
     std::vector<uint2> qnlf;
     std::vector<CItem> items;
-    uint32_t blocks = 0;
+    uint32_t chunks = 0;
+    std::map<uint32_t, std::vector<uint32_t>> by_label;   // label -> its items
     for (QueryHost* q : qs) {
         q->cand_count.assign(q->qg.n, 0);
         q->cand_begin.assign(q->qg.n, 0);
@@ -520,44 +522,66 @@ int Batch::bucket_filter(bool use_nlf, std::vector<QueryHost*>& active) {
             const uint32_t l = v.label;
             it.src_begin = l < g->L ? g->h_ri_off[l] : 0;
             it.len = l < g->L ? g->h_ri_off[l + 1] - g->h_ri_off[l] : 0;
-            it.block_start = blocks;
+            it.block_start = chunks;   // first chunk of 64 elements
             it.mode = use_nlf ? kModeNLF : kModeLDF;
             it.qv = (uint32_t)qv.size();
+            if (it.len) by_label[l].push_back((uint32_t)items.size());
             qv.push_back(v);
             items.push_back(it);
-            blocks += (it.len + 255) / 256;
+            chunks += (it.len + kBucketChunk - 1) / kBucketChunk;
         }
     }
-    DBuf d_items, d_qv, d_qnlf, d_masks, d_bc, d_bp, d_tmp, d_starts, d_ends, d_cnt, d_beg;
-    HIP_TRY(ctx, upload(d_items, items, s()));
-    HIP_TRY(ctx, upload(d_qv, qv, s()));
+    std::vector<CGroup> groups;
+    std::vector<CBucketItem> bitems;
+    uint64_t waves64 = 0;
+    for (auto& [l, list] : by_label) {
+        CGroup gr{};
+        gr.src_begin = g->h_ri_off[l];
+        gr.len = g->h_ri_off[l + 1] - g->h_ri_off[l];
+        gr.wave_start = (uint32_t)waves64;
+        gr.item_begin = (uint32_t)bitems.size();
+        gr.n_items = (uint32_t)list.size();
+        gr.n_sub = (gr.n_items + kBucketChunk - 1) / kBucketChunk;
+        for (uint32_t i : list) {
+            const QVert& v = qv[items[i].qv];
+            bitems.push_back(CBucketItem{items[i].block_start, v.degree, v.nlf_begin,
+                                         v.nlf_count | (use_nlf ? 0u : 1u << 31)});
+        }
+        groups.push_back(gr);
+        waves64 += (uint64_t)((gr.len + kBucketChunk - 1) / kBucketChunk) * gr.n_sub;
+    }
+    if (waves64 > 0xFFFFFFFFull) return set_err(ctx, PSM_ENOMEM, "batch too large for one bucket scan");
+    const uint32_t waves = (uint32_t)waves64;
+    DBuf d_qnlf, d_groups, d_bitems, d_masks, d_cc, d_cp, d_tmp, d_starts, d_ends, d_cnt, d_beg;
     HIP_TRY(ctx, upload(d_qnlf, qnlf, s()));
-    HIP_TRY(ctx, d_masks.alloc((size_t)blocks * 4 * 8));
-    HIP_TRY(ctx, d_bc.alloc((size_t)blocks * 4));
-    HIP_TRY(ctx, d_bp.alloc((size_t)(blocks + 1) * 8));
-    HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(blocks) * 8));
-    HIP_TRY(ctx, launch_compact_flag(g->dg, d_items.as<CItem>(), (uint32_t)items.size(), blocks,
-                                     d_qv.as<QVert>(), d_qnlf.as<uint2>(), nullptr, nullptr,
-                                     d_masks.as<unsigned long long>(), d_bc.as<uint32_t>(), s()));
-    HIP_TRY(ctx, launch_scan_u32(d_bc.as<uint32_t>(), blocks, d_bp.as<uint64_t>(),
+    HIP_TRY(ctx, upload(d_groups, groups, s()));
+    HIP_TRY(ctx, upload(d_bitems, bitems, s()));
+    HIP_TRY(ctx, d_masks.alloc((size_t)chunks * 8));
+    HIP_TRY(ctx, d_cc.alloc((size_t)chunks * 4));
+    HIP_TRY(ctx, d_cp.alloc((size_t)(chunks + 1) * 8));
+    HIP_TRY(ctx, d_tmp.alloc(scan_tmp_entries(chunks) * 8));
+    HIP_TRY(ctx, launch_bucket_flag(g->dg, d_groups.as<CGroup>(), (uint32_t)groups.size(), waves,
+                                    d_bitems.as<CBucketItem>(), d_qnlf.as<uint2>(),
+                                    d_masks.as<unsigned long long>(), d_cc.as<uint32_t>(), s()));
+    HIP_TRY(ctx, launch_scan_u32(d_cc.as<uint32_t>(), chunks, d_cp.as<uint64_t>(),
                                  d_tmp.as<unsigned long long>(), s()));
-    // per item: begin = prefix[block_start], count = prefix[next start] - begin
+    // per item: begin = prefix[first chunk], count = prefix[next item's first chunk] - begin
     const uint32_t ni = (uint32_t)items.size();
     std::vector<uint64_t> starts(ni), ends(ni), zeros(ni, 0);
     for (uint32_t i = 0; i < ni; ++i) {
         starts[i] = items[i].block_start;
-        ends[i] = i + 1 < ni ? items[i + 1].block_start : blocks;
+        ends[i] = i + 1 < ni ? items[i + 1].block_start : chunks;
     }
     HIP_TRY(ctx, upload(d_starts, starts, s()));
     HIP_TRY(ctx, upload(d_ends, ends, s()));
     HIP_TRY(ctx, d_cnt.alloc((size_t)ni * 8));
     HIP_TRY(ctx, d_beg.alloc((size_t)ni * 8));
-    HIP_TRY(ctx, launch_span(d_bp.as<uint64_t>(), d_starts.as<uint64_t>(), d_ends.as<uint64_t>(), ni,
+    HIP_TRY(ctx, launch_span(d_cp.as<uint64_t>(), d_starts.as<uint64_t>(), d_ends.as<uint64_t>(), ni,
                              d_cnt.as<uint64_t>(), s()));
     {
         DBuf d_zero;
         HIP_TRY(ctx, upload(d_zero, zeros, s()));
-        HIP_TRY(ctx, launch_span(d_bp.as<uint64_t>(), d_zero.as<uint64_t>(), d_starts.as<uint64_t>(),
+        HIP_TRY(ctx, launch_span(d_cp.as<uint64_t>(), d_zero.as<uint64_t>(), d_starts.as<uint64_t>(),
                                  ni, d_beg.as<uint64_t>(), s()));
         std::vector<uint64_t> cnt, beg;
         HIP_TRY(ctx, download(cnt, d_cnt.p, ni, s()));
@@ -573,9 +597,9 @@ int Batch::bucket_filter(bool use_nlf, std::vector<QueryHost*>& active) {
         }
     }
     HIP_TRY(ctx, cand_pool.alloc(cand_total * 4));
-    HIP_TRY(ctx, launch_compact_scatter(d_items.as<CItem>(), ni, blocks, g->ri.as<uint32_t>(), nullptr,
-                                        d_masks.as<unsigned long long>(), d_bp.as<uint64_t>(),
-                                        cand_pool.as<uint32_t>(), s()));
+    HIP_TRY(ctx, launch_bucket_scatter(g->dg, d_groups.as<CGroup>(), (uint32_t)groups.size(), waves,
+                                       d_bitems.as<CBucketItem>(), d_masks.as<unsigned long long>(),
+                                       d_cp.as<uint64_t>(), cand_pool.as<uint32_t>(), s()));
     HIP_TRY(ctx, hipStreamSynchronize(s()));
     // FilterVertices.cpp:19-65: the reference stops at the first empty set
     active.clear();

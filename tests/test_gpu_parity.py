@@ -123,6 +123,48 @@ def test_candidates_and_tables_match_oracle(hprd, method, qname):
     assert o.tolist() == ref.order
 
 
+@pytest.mark.parametrize("n,labels,dense", [(12, 1, 1), (24, 1, 1), (36, 1, 1), (40, 1, 2), (40, 2, 2)])
+def test_gql_candidates_with_large_label_groups(dev, n, labels, dense):
+    """GQL refinement where many query vertices share a label: the
+    membership words are 2, 4 and 8 bytes wide (label groups above 8, 16
+    and 32 vertices); dense queries on a power-law graph make the
+    refinement drop about half of the candidates. Candidates are compared
+    with the oracle's."""
+    from psm_benchmark_amd import gen
+    from psm_benchmark_amd.matcher import DeviceGraph, Query
+    data = gen.chung_lu_graph(2000, 16000, labels, 1.0, seed=n + labels)
+    q = gen.query(data, n, dense, 3 * n + labels)
+    dg = DeviceGraph(dev, data)
+    ref = ORun(OGraph(data), OGraph(q), "GQL_GQL_GQL", cap=1, time_limit=0.5)
+    run = Query(dg, q)
+    run.filter("GQL")
+    assert run.candidate_counts().tolist() == ref.cand_counts
+    for u in range(n):
+        assert np.array_equal(run.candidates(u), ref.candidates(u)), u
+
+
+def test_gql_batch_with_wide_membership_words_keeps_small_query_counts(dev):
+    """Small queries batched with a 40-vertex query whose largest label
+    group holds 33 vertices (8-byte membership words for the whole batch)
+    count and call as the oracle does. The cap (1e5) stops the large
+    query's search early and lies above every small query's count."""
+    from psm_benchmark_amd import gen
+    from psm_benchmark_amd.matcher import Batch, DeviceGraph
+    data = gen.chung_lu_graph(2000, 16000, 2, 1.0, seed=42)
+    big = gen.query(data, 40, 2, 122)
+    small = [gen.query(data, 3 + i % 4, 1, 300 + i) for i in (1, 3, 4, 6)]
+    dg = DeviceGraph(dev, data)
+    b = Batch(dg, small + [big])
+    b.prepare("GQL", "GQL")
+    counts, calls, status, _ = b.evaluate(100000, 30.0)
+    b.close()
+    assert (int(status[4]), int(counts[4])) == (0, 100000)
+    od = OGraph(data)
+    for i, q in enumerate(small):
+        ref = ORun(od, OGraph(q), "GQL_GQL_GQL")
+        assert (int(status[i]), int(counts[i]), int(calls[i])) == (0, ref.count, ref.calls), i
+
+
 def test_c1_reference_vectors_on_device(dev):
     from psm_benchmark_amd import gen
     from psm_benchmark_amd.matcher import DeviceGraph, match
