@@ -401,7 +401,7 @@ def main():
         "batched_over_plain": round(nq * args.steps / total / plain_qps, 2),
         "stage_ms": {k: round(float(np.mean([getattr(s, k) for s in stats])), 2)
                      for k in ("filter_ms", "table_ms", "plan_ms", "enum_ms")},
-        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"] / args.steps, 3),
+        "kernels": {k: dict(launches=round(v["launches"] / args.steps, 2), ms=round(v["ms"] / args.steps, 3),
                             bytes=int(v["bytes"] / args.steps)) for k, v in ks.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -492,7 +492,7 @@ def plain_workload(args):
         "embeddings_per_s": round(r.count * args.steps / total, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "p99": round(float(np.percentile(lat_ms, 99)), 3)},
-        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"] / args.steps, 4),
+        "kernels": {k: dict(launches=round(v["launches"] / args.steps, 2), ms=round(v["ms"] / args.steps, 4),
                             bytes=int(v["bytes"] / args.steps)) for k, v in ks.items()},
         "roofline": roofline_of(ks, args.steps),
         "cpu_baseline": cpu,
@@ -646,7 +646,7 @@ def large_workload(args):
                   "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
                   "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
                   "queries_per_s": round(len(lat) / max(1e-9, sum(lat)), 3)},
-        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"] / args.steps, 3),
+        "kernels": {k: dict(launches=round(v["launches"] / args.steps, 2), ms=round(v["ms"] / args.steps, 3),
                             bytes=int(v["bytes"] / args.steps)) for k, v in ks.items()},
         "roofline": roofline_of(ks, args.steps),
         "cpu_baseline": None,
@@ -813,7 +813,7 @@ def c4_batch_workload(args):
         "timeouts": timeouts,
         "stage_ms": {k: round(float(np.mean([getattr(s, k) for s in stats])), 2)
                      for k in ("filter_ms", "table_ms", "plan_ms", "enum_ms")},
-        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"] / len(times), 3),
+        "kernels": {k: dict(launches=round(v["launches"] / len(times), 2), ms=round(v["ms"] / len(times), 3),
                             bytes=int(v["bytes"] / len(times))) for k, v in ks.items()},
         "roofline": None, "cpu_baseline": None}))
 
