@@ -751,47 +751,57 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
         HIP_TRY(ctx, hipStreamSynchronize(s()));
     }
     // work items of every query vertex position p (the candidates of vertex p
-    // of each query), uploaded once: both rounds use the same items
+    // of each query); built again for the second round after the pool is
+    // compacted (the first round's drops leave most slots kInvalid)
     std::vector<GItem> items;
     std::vector<uint32_t> left, left_lab, labs, item_begin(max_n + 1, 0), cands(max_n, 0);
     uint32_t max_cands = 0;
-    for (uint32_t p = 0; p < max_n; ++p) {
-        item_begin[p] = (uint32_t)items.size();
-        uint32_t waves = 0;
-        for (size_t a = 0; a < active.size(); ++a) {
-            QueryHost* q = active[a];
-            if (p >= q->qg.n || q->cand_count[p] == 0) continue;
-            GItem it{};
-            it.cand_begin = q->cand_begin[p];
-            it.len = q->cand_count[p];
-            it.wave_start = waves;
-            it.bm_self = q->slot_base + p;
-            it.u = p;
-            it.mem_base = a * (uint64_t)g->V;
-            it.k = q->qg.degree(p);
-            it.left_begin = (uint32_t)left.size();
-            it.lab_begin = (uint32_t)labs.size();
-            const size_t l0 = labs.size();
-            for (uint32_t i = q->qg.off[p]; i < q->qg.off[p + 1]; ++i) {
-                left.push_back(q->slot_base + q->qg.adj[i]);
-                left_lab.push_back(q->qg.lab[q->qg.adj[i]]);
-                labs.push_back(q->qg.lab[q->qg.adj[i]]);
-            }
-            std::sort(labs.begin() + l0, labs.end());
-            labs.erase(std::unique(labs.begin() + l0, labs.end()), labs.end());
-            it.nlab = (uint32_t)(labs.size() - l0);
-            items.push_back(it);
-            waves += it.len;
-        }
-        cands[p] = waves;
-        max_cands = std::max(max_cands, waves);
-    }
-    item_begin[max_n] = (uint32_t)items.size();
     DBuf d_items, d_left, d_left_lab, d_labs, d_lists, d_nlists;
-    HIP_TRY(ctx, upload(d_items, items, s()));
-    HIP_TRY(ctx, upload(d_left, left, s()));
-    HIP_TRY(ctx, upload(d_left_lab, left_lab, s()));
-    HIP_TRY(ctx, upload(d_labs, labs, s()));
+    auto build_items = [&]() -> int {
+        items.clear();
+        left.clear();
+        left_lab.clear();
+        labs.clear();
+        max_cands = 0;
+        for (uint32_t p = 0; p < max_n; ++p) {
+            item_begin[p] = (uint32_t)items.size();
+            uint32_t waves = 0;
+            for (size_t a = 0; a < active.size(); ++a) {
+                QueryHost* q = active[a];
+                if (p >= q->qg.n || q->cand_count[p] == 0) continue;
+                GItem it{};
+                it.cand_begin = q->cand_begin[p];
+                it.len = q->cand_count[p];
+                it.wave_start = waves;
+                it.bm_self = q->slot_base + p;
+                it.u = p;
+                it.mem_base = a * (uint64_t)g->V;
+                it.k = q->qg.degree(p);
+                it.left_begin = (uint32_t)left.size();
+                it.lab_begin = (uint32_t)labs.size();
+                const size_t l0 = labs.size();
+                for (uint32_t i = q->qg.off[p]; i < q->qg.off[p + 1]; ++i) {
+                    left.push_back(q->slot_base + q->qg.adj[i]);
+                    left_lab.push_back(q->qg.lab[q->qg.adj[i]]);
+                    labs.push_back(q->qg.lab[q->qg.adj[i]]);
+                }
+                std::sort(labs.begin() + l0, labs.end());
+                labs.erase(std::unique(labs.begin() + l0, labs.end()), labs.end());
+                it.nlab = (uint32_t)(labs.size() - l0);
+                items.push_back(it);
+                waves += it.len;
+            }
+            cands[p] = waves;
+            max_cands = std::max(max_cands, waves);
+        }
+        item_begin[max_n] = (uint32_t)items.size();
+        HIP_TRY(ctx, upload(d_items, items, s()));
+        HIP_TRY(ctx, upload(d_left, left, s()));
+        HIP_TRY(ctx, upload(d_left_lab, left_lab, s()));
+        HIP_TRY(ctx, upload(d_labs, labs, s()));
+        return PSM_OK;
+    };
+    PSM_TRY(build_items());
     // deferred pairs of one (round, p): class c at c * max_cands; reused by
     // every launch group (stream order)
     HIP_TRY(ctx, d_lists.alloc(4ull * max_cands * sizeof(uint2)));
@@ -811,7 +821,16 @@ int Batch::gql_refine(std::vector<QueryHost*>& active) {
     HIP_TRY(ctx, d_skip.alloc(std::max<size_t>(1, items.size()) * 4));
     // Launch groups follow one another on the stream without host syncs: the
     // deferred-class kernels read their counts from d_nlists.
+    const uint32_t lists_cap = max_cands;   // d_lists holds 4 classes of this many pairs
     for (int round = 0; round < 2; ++round) {
+        if (round == 1) {
+            // the second round tests only the first round's survivors: the
+            // pool is compacted (order kept) and the items rebuilt on it
+            HIP_TRY(ctx, timers_flush(ctx));
+            PSM_TRY(compact_pool());
+            PSM_TRY(build_items());
+            if (max_cands > lists_cap) return set_err(ctx, PSM_EINVAL, "candidate sets grew between rounds");
+        }
         uint32_t* removed = d_removed.as<uint32_t>() + (uint64_t)round * n_slots;
         for (uint32_t p = 0; p < max_n; ++p) {
             const uint32_t ni = item_begin[p + 1] - item_begin[p];
