@@ -506,7 +506,7 @@ int Batch::bucket_filter(bool use_nlf, std::vector<QueryHost*>& active) {
     std::vector<QVert> qv;
     std::vector<uint2> qnlf;
     std::vector<CItem> items;
-    uint32_t chunks = 0;
+    uint64_t chunks64 = 0;
     std::map<uint32_t, std::vector<uint32_t>> by_label;   // label -> its items
     for (QueryHost* q : qs) {
         q->cand_count.assign(q->qg.n, 0);
@@ -522,15 +522,17 @@ int Batch::bucket_filter(bool use_nlf, std::vector<QueryHost*>& active) {
             const uint32_t l = v.label;
             it.src_begin = l < g->L ? g->h_ri_off[l] : 0;
             it.len = l < g->L ? g->h_ri_off[l + 1] - g->h_ri_off[l] : 0;
-            it.block_start = chunks;   // first chunk of 64 elements
+            it.block_start = (uint32_t)chunks64;   // first chunk of 64 elements (checked below)
             it.mode = use_nlf ? kModeNLF : kModeLDF;
             it.qv = (uint32_t)qv.size();
             if (it.len) by_label[l].push_back((uint32_t)items.size());
             qv.push_back(v);
             items.push_back(it);
-            chunks += (it.len + kBucketChunk - 1) / kBucketChunk;
+            chunks64 += (it.len + kBucketChunk - 1) / kBucketChunk;
+            if (chunks64 > 0xFFFFFFFFull) return set_err(ctx, PSM_ENOMEM, "batch too large for one bucket scan");
         }
     }
+    const uint32_t chunks = (uint32_t)chunks64;
     std::vector<CGroup> groups;
     std::vector<CBucketItem> bitems;
     uint64_t waves64 = 0;
